@@ -1,0 +1,162 @@
+// Host-buffer entry points: score_batch (src/gpu.rs:166-267) and the
+// all-atom energy / force functions of src/energy.rs, each run as a one-call
+// device-resident antigen + population evaluation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "dnab_internal.h"
+
+namespace {
+
+struct AgGuard {
+  dnab_antigen* p = nullptr;
+  ~AgGuard() { dnab_antigen_destroy(p); }
+};
+struct PopGuard {
+  dnab_population* p = nullptr;
+  ~PopGuard() { dnab_population_destroy(p); }
+};
+
+dnab_antigen_desc antigen_desc(const dnab_atom_cloud* ag, float cutoff) {
+  dnab_antigen_desc d{};
+  d.atoms = *ag;
+  d.cell_size = cutoff;
+  d.cutoff = cutoff;
+  return d;
+}
+
+// One candidate holding every antibody atom as a single residue.
+int single_candidate(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                     const dnab_atom_cloud* antibody, float cutoff, AgGuard& ag, PopGuard& pop) {
+  if (!(cutoff > 0.0f) || !std::isfinite(cutoff))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "cutoff must be positive");
+  dnab_antigen_desc ad = antigen_desc(antigen, cutoff);
+  DNAB_RETURN_IF(dnab_antigen_create(ctx, &ad, &ag.p));
+  const uint32_t n = static_cast<uint32_t>(antibody->n);
+  uint32_t cand_ag = 0, cand_res_off[2] = {0, n ? 1u : 0u}, res_atom_off[2] = {0, n}, res_ca = 0;
+  dnab_population_desc pd{};
+  pd.n_cand = 1;
+  pd.n_res = n ? 1 : 0;
+  pd.cand_antigen = &cand_ag;
+  pd.cand_res_off = cand_res_off;
+  pd.res_atom_off = res_atom_off;
+  pd.res_ca_atom = &res_ca;
+  pd.atoms = *antibody;
+  dnab_antigen* ags[1] = {ag.p};
+  return dnab_population_create(ctx, ags, 1, &pd, &pop.p);
+}
+
+bool cloud_ok(const dnab_atom_cloud* c) {
+  return c && (c->n == 0 || (c->x && c->y && c->z && c->charge && c->r_min_half && c->epsilon &&
+                             c->hydrophobic));
+}
+
+}  // namespace
+
+extern "C" {
+
+int dnab_interaction_energy_atoms_breakdown(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                          const dnab_atom_cloud* antibody, float cutoff,
+                          dnab_energy_breakdown* out) {
+  if (!ctx || !cloud_ok(antigen) || !cloud_ok(antibody) || !out) return DNAB_ERR_INVALID;
+  AgGuard ag;
+  PopGuard pop;
+  DNAB_RETURN_IF(single_candidate(ctx, antigen, antibody, cutoff, ag, pop));
+  DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, DNAB_EVAL_BREAKDOWN));
+  double lj = 0, co = 0, hy = 0;
+  DNAB_RETURN_IF(dnab_population_read_breakdown(ctx, pop.p, &lj, &co, &hy));
+  out->lj = lj;
+  out->coulomb = co;
+  out->hydrophobic = hy;
+  out->hbond = 0.0;
+  out->solvation = 0.0;
+  return DNAB_OK;
+}
+
+int dnab_interaction_energy_atoms(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                                  const dnab_atom_cloud* antibody, float cutoff,
+                                  double* energy) {
+  if (!ctx || !cloud_ok(antigen) || !cloud_ok(antibody) || !energy) return DNAB_ERR_INVALID;
+  AgGuard ag;
+  PopGuard pop;
+  DNAB_RETURN_IF(single_candidate(ctx, antigen, antibody, cutoff, ag, pop));
+  DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, 0));
+  return dnab_population_read_energies(ctx, pop.p, energy, nullptr);
+}
+
+int dnab_atom_forces(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                     const dnab_atom_cloud* antibody, float cutoff, float* fx, float* fy,
+                     float* fz, double* energy) {
+  if (!ctx || !cloud_ok(antigen) || !cloud_ok(antibody)) return DNAB_ERR_INVALID;
+  if (antibody->n && (!fx || !fy || !fz)) return DNAB_ERR_INVALID;
+  AgGuard ag;
+  PopGuard pop;
+  DNAB_RETURN_IF(single_candidate(ctx, antigen, antibody, cutoff, ag, pop));
+  DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, DNAB_EVAL_FORCES));
+  if (antibody->n) DNAB_RETURN_IF(dnab_population_read_forces(ctx, pop.p, fx, fy, fz));
+  if (energy) DNAB_RETURN_IF(dnab_population_read_energies(ctx, pop.p, energy, nullptr));
+  return DNAB_OK;
+}
+
+int dnab_score_batch(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                     const dnab_gpu_atom* candidates, size_t n_atoms, size_t n_ab,
+                     float* energies_out, size_t capacity, size_t* n_cand_out) {
+  if (!ctx || !cloud_ok(antigen)) return DNAB_ERR_INVALID;
+  // src/gpu.rs:172-174: n_cand = len / n_ab, empty result for n_ab == 0.
+  const size_t n_cand = n_ab == 0 ? 0 : n_atoms / n_ab;
+  if (n_cand_out) *n_cand_out = n_cand;
+  if (n_cand == 0) return DNAB_OK;
+  if (!candidates) return DNAB_ERR_INVALID;
+  if (!energies_out || capacity < n_cand) return DNAB_ERR_CAPACITY;
+  const size_t na = n_cand * n_ab;
+  std::vector<float> x(na), y(na), z(na), q(na), rmh(na), eps(na);
+  std::vector<uint8_t> hyd(na);
+  for (size_t i = 0; i < na; ++i) {
+    const dnab_gpu_atom& a = candidates[i];
+    x[i] = a.x;
+    y[i] = a.y;
+    z[i] = a.z;
+    q[i] = a.q;
+    rmh[i] = a.r_min_half;
+    eps[i] = a.epsilon;
+    hyd[i] = a.hydrophobic > 0.5f ? 1 : 0;  // shaders/energy.wgsl:91
+  }
+  std::vector<uint32_t> cand_ag(n_cand, 0), cand_res_off(n_cand + 1), res_atom_off(n_cand + 1),
+      res_ca(n_cand);
+  for (size_t c = 0; c <= n_cand; ++c) {
+    cand_res_off[c] = static_cast<uint32_t>(c);
+    res_atom_off[c] = static_cast<uint32_t>(c * n_ab);
+  }
+  for (size_t c = 0; c < n_cand; ++c) res_ca[c] = static_cast<uint32_t>(c * n_ab);
+  AgGuard ag;
+  PopGuard pop;
+  dnab_antigen_desc ad = antigen_desc(antigen, 10.0f);  // shaders/energy.wgsl:16
+  DNAB_RETURN_IF(dnab_antigen_create(ctx, &ad, &ag.p));
+  dnab_population_desc pd{};
+  pd.n_cand = n_cand;
+  pd.n_res = n_cand;
+  pd.cand_antigen = cand_ag.data();
+  pd.cand_res_off = cand_res_off.data();
+  pd.res_atom_off = res_atom_off.data();
+  pd.res_ca_atom = res_ca.data();
+  pd.atoms.n = na;
+  pd.atoms.x = x.data();
+  pd.atoms.y = y.data();
+  pd.atoms.z = z.data();
+  pd.atoms.charge = q.data();
+  pd.atoms.r_min_half = rmh.data();
+  pd.atoms.epsilon = eps.data();
+  pd.atoms.hydrophobic = hyd.data();
+  dnab_antigen* ags[1] = {ag.p};
+  DNAB_RETURN_IF(dnab_population_create(ctx, ags, 1, &pd, &pop.p));
+  DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, 0));
+  std::vector<double> e(n_cand);
+  DNAB_RETURN_IF(dnab_population_read_energies(ctx, pop.p, e.data(), nullptr));
+  for (size_t c = 0; c < n_cand; ++c) energies_out[c] = static_cast<float>(e[c]);
+  return DNAB_OK;
+}
+
+}  // extern "C"

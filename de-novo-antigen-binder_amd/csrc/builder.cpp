@@ -1,0 +1,275 @@
+// Host-side construction of all-atom clouds and synthetic inputs.
+//
+// protein_from_ca_trace semantics: src/allatom.rs:494-541 (ideal backbone
+// around each Calpha, most probable rotamer, push_residue at
+// src/allatom.rs:201-240); NERF placement: src/rotamer.rs:198-232,
+// build_side_chain: src/rotamer.rs:439-462. The synthetic Calpha generators
+// (ring layout of src/diffusion.rs:264-277, globular random walks) use this
+// library's own SplitMix64 stream; the reference's rand::SmallRng streams are
+// not reproduced.
+#include <cmath>
+#include <cstring>
+
+#include "dnab_internal.h"
+#include "ff_tables.h"
+
+namespace dnab {
+
+namespace {
+
+struct V3 {
+  float x, y, z;
+};
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline V3 normalize(V3 v) {
+  float l = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+  if (l < 1e-8f) l = 1e-8f;
+  return {v.x / l, v.y / l, v.z / l};
+}
+inline float deg2rad(float d) { return d * (3.14159265358979323846f / 180.0f); }
+
+}  // namespace
+
+// NERF (Parsons et al. 2005, J. Comput. Chem. 26(10)): place D so that
+// |CD| = bond_len, angle BCD = angle_deg, dihedral ABCD = dihedral_deg.
+void place_atom(const float a[3], const float b[3], const float c[3], float bond_len,
+                float angle_deg, float dihedral_deg, float out[3]) {
+  V3 A{a[0], a[1], a[2]}, B{b[0], b[1], b[2]}, C{c[0], c[1], c[2]};
+  float ang = deg2rad(angle_deg), dih = deg2rad(dihedral_deg);
+  V3 bc = normalize(sub(B, C));
+  V3 n = normalize(cross(sub(B, A), bc));
+  V3 m = cross(bc, n);
+  float sa = std::sin(ang), ca = std::cos(ang), sd = std::sin(dih), cd = std::cos(dih);
+  out[0] = C.x + bond_len * (-ca * bc.x + sa * cd * m.x + sa * sd * n.x);
+  out[1] = C.y + bond_len * (-ca * bc.y + sa * cd * m.y + sa * sd * n.y);
+  out[2] = C.z + bond_len * (-ca * bc.z + sa * cd * m.z + sa * sd * n.z);
+}
+
+// Axis-aligned ideal backbone around a Calpha, src/allatom.rs:502-517.
+int ideal_backbone(const float ca[3], int aa, float bb[5][3]) {
+  const float kNCa = 1.46f, kCaC = 1.52f, kCO = 1.23f, kCaCb = 1.52f;
+  bb[0][0] = ca[0] - kNCa; bb[0][1] = ca[1]; bb[0][2] = ca[2];
+  bb[1][0] = ca[0];        bb[1][1] = ca[1]; bb[1][2] = ca[2];
+  bb[2][0] = ca[0] + kCaC; bb[2][1] = ca[1]; bb[2][2] = ca[2];
+  bb[3][0] = bb[2][0] + 0.5f; bb[3][1] = bb[2][1] + kCO * 0.9f; bb[3][2] = bb[2][2];
+  bb[4][0] = ca[0]; bb[4][1] = ca[1] - kCaCb; bb[4][2] = ca[2];
+  return aa == kGly ? 4 : 5;
+}
+
+// Append all heavy atoms of one residue; returns the number appended.
+int build_residue_atoms(int aa, const float ca[3], const float chi[4], float* xyz_out,
+                        int* n_backbone_out) {
+  float atoms[16][3];
+  int nb = ideal_backbone(ca, aa, reinterpret_cast<float(*)[3]>(atoms));
+  const ResTopology& topo = kResidueTopology[aa];
+  if (nb > topo.n_atoms) nb = topo.n_atoms;
+  int n = nb;
+  const BondDef* defs = kSidechainDefs[aa];
+  for (int k = 0; k < kSidechainCount[aa] && n < topo.n_atoms; ++k) {
+    const BondDef& d = defs[k];
+    float dih = d.chi_idx == 255 ? d.dihedral_fixed : chi[d.chi_idx];
+    place_atom(atoms[d.ref_a], atoms[d.ref_b], atoms[d.ref_c], d.bond_len, d.angle_deg, dih,
+               atoms[n]);
+    ++n;
+  }
+  std::memcpy(xyz_out, atoms, sizeof(float) * 3 * n);
+  if (n_backbone_out) *n_backbone_out = nb;
+  return n;
+}
+
+uint64_t SplitMix64::next() {
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+float SplitMix64::uniform(float lo, float hi) {
+  float u = static_cast<float>(next() >> 40) * (1.0f / 16777216.0f);
+  return lo + (hi - lo) * u;
+}
+uint32_t SplitMix64::below(uint32_t n) {
+  return static_cast<uint32_t>(((next() >> 32) * static_cast<uint64_t>(n)) >> 32);
+}
+
+}  // namespace dnab
+
+using namespace dnab;
+
+extern "C" {
+
+int dnab_build_protein(const float* ca_x, const float* ca_y, const float* ca_z,
+                       const uint8_t* amino_acid, const uint8_t* rotamer_idx, size_t n_res,
+                       dnab_protein_out* out, size_t* n_atoms_out) {
+  if (!out || (n_res && (!ca_x || !ca_y || !ca_z || !amino_acid))) return DNAB_ERR_INVALID;
+  size_t need = 0;
+  for (size_t r = 0; r < n_res; ++r) {
+    if (amino_acid[r] >= kNumAminoAcids) return DNAB_ERR_INVALID;
+    need += kResidueTopology[amino_acid[r]].n_atoms;
+  }
+  if (n_atoms_out) *n_atoms_out = need;
+  if (need > out->atom_capacity) return DNAB_ERR_CAPACITY;
+  size_t a = 0;
+  if (out->res_start) out->res_start[0] = 0;
+  for (size_t r = 0; r < n_res; ++r) {
+    int aa = amino_acid[r];
+    float chi[4] = {0, 0, 0, 0};
+    if (kRotamerCount[aa]) {
+      int k = rotamer_idx ? rotamer_idx[r] % kRotamerCount[aa] : 0;
+      std::memcpy(chi, kRotamerLib[aa][k].chi, sizeof(chi));
+    }
+    float ca[3] = {ca_x[r], ca_y[r], ca_z[r]};
+    float xyz[16][3];
+    int n = build_residue_atoms(aa, ca, chi, &xyz[0][0], nullptr);
+    const ResTopology& topo = kResidueTopology[aa];
+    if (out->ca_atom_idx) out->ca_atom_idx[r] = static_cast<uint32_t>(a + topo.ca_idx);
+    if (out->amino_acid) out->amino_acid[r] = static_cast<uint8_t>(aa);
+    for (int k = 0; k < n; ++k, ++a) {
+      const ResAtomDef& d = topo.atoms[k];
+      const NbParams& nb = kAmberNb[d.atom_type];
+      out->x[a] = xyz[k][0];
+      out->y[a] = xyz[k][1];
+      out->z[a] = xyz[k][2];
+      if (out->charge) out->charge[a] = d.charge;
+      if (out->r_min_half) out->r_min_half[a] = nb.r_min_half;
+      if (out->epsilon) out->epsilon[a] = nb.epsilon;
+      if (out->hydrophobic) out->hydrophobic[a] = d.hydrophobic;
+      if (out->atom_type) out->atom_type[a] = d.atom_type;
+      if (out->residue_idx) out->residue_idx[a] = static_cast<uint32_t>(r);
+    }
+    if (out->res_start) out->res_start[r + 1] = static_cast<uint32_t>(a);
+  }
+  return DNAB_OK;
+}
+
+int dnab_place_atom(const float a[3], const float b[3], const float c[3], float bond_len,
+                    float angle_deg, float dihedral_deg, float out[3]) {
+  if (!a || !b || !c || !out) return DNAB_ERR_INVALID;
+  place_atom(a, b, c, bond_len, angle_deg, dihedral_deg, out);
+  return DNAB_OK;
+}
+
+int dnab_residue_atom_count(uint8_t amino_acid) {
+  return amino_acid < kNumAminoAcids ? kResidueTopology[amino_acid].n_atoms : -1;
+}
+
+int dnab_residue_cg_params(uint8_t amino_acid, float* charge, float* epsilon, float* sigma,
+                           uint8_t* hydrophobic) {
+  if (amino_acid >= kNumAminoAcids) return DNAB_ERR_INVALID;
+  if (charge) *charge = aa_charge(amino_acid);
+  if (epsilon) *epsilon = aa_lj_epsilon(amino_acid);
+  if (sigma) *sigma = aa_lj_sigma(amino_acid);
+  if (hydrophobic) *hydrophobic = aa_is_hydrophobic(amino_acid);
+  return DNAB_OK;
+}
+
+int dnab_ff_table(uint8_t amino_acid, int slot, uint8_t* atom_type, float* charge,
+                  uint8_t* hydrophobic, float* r_min_half, float* epsilon) {
+  if (amino_acid >= kNumAminoAcids) return DNAB_ERR_INVALID;
+  const ResTopology& t = kResidueTopology[amino_acid];
+  if (slot < 0 || slot >= t.n_atoms) return DNAB_ERR_INVALID;
+  const ResAtomDef& d = t.atoms[slot];
+  if (atom_type) *atom_type = d.atom_type;
+  if (charge) *charge = d.charge;
+  if (hydrophobic) *hydrophobic = d.hydrophobic;
+  if (r_min_half) *r_min_half = kAmberNb[d.atom_type].r_min_half;
+  if (epsilon) *epsilon = kAmberNb[d.atom_type].epsilon;
+  return DNAB_OK;
+}
+
+// Calpha random walk with 3.8 A steps, uniform directions, steps leaving the
+// sphere of `radius` around the origin resampled; uniform residues and
+// uniform rotamer index.
+int dnab_synth_globule_trace(size_t n_res, float radius, uint64_t seed, float* x, float* y,
+                             float* z, uint8_t* amino_acid, uint8_t* rotamer_idx) {
+  if (n_res && (!x || !y || !z || !amino_acid)) return DNAB_ERR_INVALID;
+  if (!(radius > 0.0f)) return DNAB_ERR_INVALID;
+  SplitMix64 rng{seed};
+  float px = 0, py = 0, pz = 0;
+  for (size_t i = 0; i < n_res; ++i) {
+    if (i > 0) {
+      for (int tries = 0; tries < 1000; ++tries) {
+        // uniform direction on the sphere (Marsaglia 1972)
+        float u, v, s;
+        do {
+          u = rng.uniform(-1.0f, 1.0f);
+          v = rng.uniform(-1.0f, 1.0f);
+          s = u * u + v * v;
+        } while (s >= 1.0f || s == 0.0f);
+        float f = 2.0f * std::sqrt(1.0f - s);
+        float dx = u * f, dy = v * f, dz = 1.0f - 2.0f * s;
+        float nx = px + 3.8f * dx, ny = py + 3.8f * dy, nz = pz + 3.8f * dz;
+        if (nx * nx + ny * ny + nz * nz <= radius * radius || tries == 999) {
+          px = nx; py = ny; pz = nz;
+          break;
+        }
+      }
+    }
+    x[i] = px; y[i] = py; z[i] = pz;
+    amino_acid[i] = static_cast<uint8_t>(rng.below(kNumAminoAcids));
+    if (rotamer_idx) rotamer_idx[i] = static_cast<uint8_t>(rng.below(5));
+  }
+  return DNAB_OK;
+}
+
+// Ring layout of random_allatom_antibody (src/diffusion.rs:264-277): residue i
+// at golden angle i*2.399 rad on a ring of `ring_radius` in the x-z plane
+// around `center`, jittered U(-5,5) in x and z and U(-R/2,R/2) in y, with a
+// uniformly drawn residue.
+int dnab_synth_binder_ring(size_t n_res, const float center[3], float ring_radius,
+                           uint64_t seed, float* x, float* y, float* z, uint8_t* amino_acid) {
+  if (n_res && (!x || !y || !z || !amino_acid || !center)) return DNAB_ERR_INVALID;
+  SplitMix64 rng{seed};
+  for (size_t i = 0; i < n_res; ++i) {
+    float angle = static_cast<float>(i) * 2.399f;
+    x[i] = center[0] + ring_radius * std::cos(angle) + rng.uniform(-5.0f, 5.0f);
+    y[i] = center[1] + rng.uniform(-ring_radius * 0.5f, ring_radius * 0.5f);
+    z[i] = center[2] + ring_radius * std::sin(angle) + rng.uniform(-5.0f, 5.0f);
+    amino_acid[i] = static_cast<uint8_t>(rng.below(kNumAminoAcids));
+  }
+  return DNAB_OK;
+}
+
+// Compact binder next to a globule: a Calpha walk (3.8 A steps) that starts
+// at a uniform point on the sphere of `shell_radius` around `center` and is
+// kept outside `min_radius`; uniform residues.
+int dnab_synth_binder_shell(size_t n_res, const float center[3], float shell_radius,
+                            float min_radius, uint64_t seed, float* x, float* y, float* z,
+                            uint8_t* amino_acid) {
+  if (n_res && (!x || !y || !z || !amino_acid || !center)) return DNAB_ERR_INVALID;
+  SplitMix64 rng{seed};
+  auto unit = [&rng](float& dx, float& dy, float& dz) {
+    float u, v, s;
+    do {
+      u = rng.uniform(-1.0f, 1.0f);
+      v = rng.uniform(-1.0f, 1.0f);
+      s = u * u + v * v;
+    } while (s >= 1.0f || s == 0.0f);
+    float f = 2.0f * std::sqrt(1.0f - s);
+    dx = u * f; dy = v * f; dz = 1.0f - 2.0f * s;
+  };
+  float dx, dy, dz;
+  unit(dx, dy, dz);
+  float px = shell_radius * dx, py = shell_radius * dy, pz = shell_radius * dz;
+  for (size_t i = 0; i < n_res; ++i) {
+    if (i > 0) {
+      for (int tries = 0; tries < 1000; ++tries) {
+        unit(dx, dy, dz);
+        float nx = px + 3.8f * dx, ny = py + 3.8f * dy, nz = pz + 3.8f * dz;
+        if (nx * nx + ny * ny + nz * nz >= min_radius * min_radius || tries == 999) {
+          px = nx; py = ny; pz = nz;
+          break;
+        }
+      }
+    }
+    x[i] = center[0] + px;
+    y[i] = center[1] + py;
+    z[i] = center[2] + pz;
+    amino_acid[i] = static_cast<uint8_t>(rng.below(kNumAminoAcids));
+  }
+  return DNAB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+// Context, errors, transfers and kernel timing.
+//
+// dnab_ctx_create replaces GpuContext::try_init (src/gpu.rs:143-243): where
+// the reference returns None when no adapter exists (src/gpu.rs:153-159) and
+// the caller falls back to the CPU (src/main.rs:319-327), this returns
+// DNAB_ERR_NO_DEVICE and leaves the fallback decision to the caller.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstring>
+
+#include "dnab_internal.h"
+
+int dnab_set_error(dnab_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->last_error = buf;
+  }
+  return code;
+}
+
+namespace dnab {
+
+int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return DNAB_OK;
+  DNAB_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return DNAB_OK;
+}
+
+int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return DNAB_OK;
+  DNAB_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNAB_OK;
+}
+
+static hipEvent_t take_event(dnab_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+hipEvent_t timer_begin(dnab_ctx* ctx, const char* name, hipEvent_t* end_out) {
+  (void)name;
+  *end_out = nullptr;
+  if (!ctx->timing) return nullptr;
+  hipEvent_t s = take_event(ctx), e = take_event(ctx);
+  if (!s || !e) return nullptr;
+  if (hipEventRecord(s, ctx->stream) != hipSuccess) return nullptr;
+  *end_out = e;
+  return s;
+}
+
+void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end) {
+  if (!start || !end) return;
+  if (hipEventRecord(end, ctx->stream) != hipSuccess) return;
+  for (auto& t : ctx->timers) {
+    if (t.name == name) {
+      t.pending.emplace_back(start, end);
+      return;
+    }
+  }
+  KernelTimer t;
+  t.name = name;
+  t.pending.emplace_back(start, end);
+  ctx->timers.push_back(std::move(t));
+}
+
+}  // namespace dnab
+
+using namespace dnab;
+
+extern "C" {
+
+const char* dnab_status_string(int status) {
+  switch (status) {
+    case DNAB_OK: return "ok";
+    case DNAB_ERR_INVALID: return "invalid argument";
+    case DNAB_ERR_NO_DEVICE: return "no usable GPU";
+    case DNAB_ERR_HIP: return "HIP runtime error";
+    case DNAB_ERR_ALLOC: return "allocation failed";
+    case DNAB_ERR_CAPACITY: return "output buffer too small";
+    case DNAB_ERR_UNSUPPORTED: return "unsupported input";
+    default: return "unknown status";
+  }
+}
+
+int dnab_device_count(int* count) {
+  if (!count) return DNAB_ERR_INVALID;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return DNAB_OK;
+}
+
+int dnab_ctx_create(int device, dnab_ctx** out) {
+  if (!out) return DNAB_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DNAB_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return DNAB_ERR_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return DNAB_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return DNAB_ERR_NO_DEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DNAB_ERR_NO_DEVICE;
+  dnab_ctx* ctx = new (std::nothrow) dnab_ctx();
+  if (!ctx) return DNAB_ERR_ALLOC;
+  ctx->device = device;
+  ctx->cu_count = prop.multiProcessorCount;
+  ctx->wave_size = prop.warpSize;
+  std::strncpy(ctx->arch, prop.gcnArchName, sizeof(ctx->arch) - 1);
+  // Default-priority stream; stream priorities are left untouched.
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return DNAB_ERR_HIP;
+  }
+  *out = ctx;
+  return DNAB_OK;
+}
+
+void dnab_ctx_destroy(dnab_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& t : ctx->timers)
+    for (auto& pr : t.pending) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* dnab_ctx_last_error(const dnab_ctx* ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+int dnab_ctx_synchronize(dnab_ctx* ctx) {
+  if (!ctx) return DNAB_ERR_INVALID;
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNAB_OK;
+}
+
+int dnab_ctx_device_info(const dnab_ctx* ctx, int* cu_count, int* wave_size, char* arch,
+                         size_t arch_len) {
+  if (!ctx) return DNAB_ERR_INVALID;
+  if (cu_count) *cu_count = ctx->cu_count;
+  if (wave_size) *wave_size = ctx->wave_size;
+  if (arch && arch_len) {
+    std::strncpy(arch, ctx->arch, arch_len - 1);
+    arch[arch_len - 1] = 0;
+  }
+  return DNAB_OK;
+}
+
+int dnab_ctx_set_timing(dnab_ctx* ctx, int enable) {
+  if (!ctx) return DNAB_ERR_INVALID;
+  ctx->timing = enable != 0;
+  return DNAB_OK;
+}
+
+int dnab_ctx_kernel_time(dnab_ctx* ctx, const char* kernel, double* total_ms,
+                         uint64_t* launches) {
+  if (!ctx || !kernel) return DNAB_ERR_INVALID;
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  double tot = 0.0;
+  uint64_t cnt = 0;
+  for (auto& t : ctx->timers) {
+    for (auto& pr : t.pending) {
+      float ms = 0.0f;
+      DNAB_HIP_CHECK(ctx, hipEventElapsedTime(&ms, pr.first, pr.second));
+      t.total_ms += ms;
+      t.launches += 1;
+      ctx->event_pool.push_back(pr.first);
+      ctx->event_pool.push_back(pr.second);
+    }
+    t.pending.clear();
+    if (t.name == kernel) {
+      tot = t.total_ms;
+      cnt = t.launches;
+    }
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = cnt;
+  return DNAB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+// Internal host-side definitions of the dnab library objects.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/dnab.h"
+
+namespace dnab {
+
+// Host RNG for synthetic inputs: SplitMix64 (Steele, Lea, Flood 2014).
+struct SplitMix64 {
+  uint64_t state;
+  uint64_t next();
+  float uniform(float lo, float hi);  // [lo, hi)
+  uint32_t below(uint32_t n);         // [0, n)
+};
+
+void place_atom(const float a[3], const float b[3], const float c[3], float bond_len,
+                float angle_deg, float dihedral_deg, float out[3]);
+int ideal_backbone(const float ca[3], int aa, float bb[5][3]);
+
+// Owning device allocation.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  hipError_t ensure(size_t count) { return count <= n ? hipSuccess : alloc(count); }
+};
+
+// A SpatialHashGrid laid out on the device: points sorted cell-major with
+// cells in Morton order of (key - key_min); within a cell, ascending
+// original index (the reference's append order, src/spatial.rs:35-38).
+struct GridDev {
+  float cell = 0.0f;
+  float inv = 0.0f;  // fl(1/cell), src/spatial.rs:26
+  int32_t kmin[3] = {0, 0, 0};
+  int32_t kmax[3] = {-1, -1, -1};
+  int bits = 0;       // Morton bits per axis
+  uint64_t ncodes = 0;
+  size_t n = 0;
+  size_t n_nonempty = 0;
+  DevBuf<uint32_t> cell_start;  // ncodes + 1
+  DevBuf<uint32_t> sorted_idx;  // sorted slot -> original index
+  DevBuf<int32_t> keys;         // 3 * n, per original point
+  DevBuf<float> x, y, z;        // original order
+};
+
+// View of one grid for kernels (passed by value).
+struct GridView {
+  const uint32_t* cell_start;
+  const uint32_t* sorted_idx;
+  int32_t kmin[3];
+  int32_t kmax[3];
+  int bits;
+  float inv;
+  uint32_t n;
+};
+
+inline GridView make_view(const GridDev& g) {
+  GridView v;
+  v.cell_start = g.cell_start.p;
+  v.sorted_idx = g.sorted_idx.p;
+  for (int a = 0; a < 3; ++a) {
+    v.kmin[a] = g.kmin[a];
+    v.kmax[a] = g.kmax[a];
+  }
+  v.bits = g.bits;
+  v.inv = g.inv;
+  v.n = static_cast<uint32_t>(g.n);
+  return v;
+}
+
+// Antigen atom record for the pair kernel: 32 bytes, read by one scalar
+// load per atom. q332 = 332*q; sqe = sqrt(epsilon).
+struct AgAtom {
+  float x, y, z, q332;
+  float rmh, sqe, hyd, pad;
+};
+
+// Per-antigen constants read by the pair kernels.
+struct AgView {
+  const AgAtom* atoms;         // grid-sorted
+  const float2* raw;           // (q, epsilon), grid-sorted, for the FP64 kernel
+  const uint32_t* cell_start;  // Morton cell ranges
+  int32_t kmin[3];
+  int32_t kmax[3];
+  int bits;
+  float inv;
+  float rc;
+  float rc2;
+  // candidate-atom sort grid around this antigen
+  float sort_origin[3];
+  float sort_inv;
+  int32_t sort_dim[3];
+  int sort_bits;
+  uint32_t sort_code_base;
+  uint32_t n_atoms;
+  // residue-level model: antigen Calpha cloud and its grid
+  const float* ca_x;
+  const float* ca_y;
+  const float* ca_z;
+  const float* ca_q;
+  const float* ca_eps;
+  const float* ca_sig;
+  const uint8_t* ca_hyd;
+  GridView ca_grid;
+  float center[3];
+  float restraint_radius;  // 0 = use the step parameter
+};
+
+struct KernelTimer {
+  std::string name;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double total_ms = 0.0;
+  uint64_t launches = 0;
+};
+
+}  // namespace dnab
+
+struct dnab_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  int cu_count = 0;
+  int wave_size = 64;
+  char arch[64] = {0};
+  bool timing = false;
+  std::vector<dnab::KernelTimer> timers;
+  std::vector<hipEvent_t> event_pool;
+};
+
+struct dnab_grid {
+  dnab_ctx* ctx = nullptr;
+  dnab::GridDev g;
+};
+
+struct dnab_antigen {
+  dnab_ctx* ctx = nullptr;
+  dnab::GridDev grid;
+  float cutoff = 10.0f;
+  float restraint_radius = 0.0f;
+  float center[3] = {0, 0, 0};
+  float bbox_lo[3] = {0, 0, 0};
+  float bbox_hi[3] = {0, 0, 0};
+  dnab::DevBuf<dnab::AgAtom> atoms;
+  dnab::DevBuf<float2> raw;
+  size_t n_ca = 0;
+  dnab::DevBuf<float> ca_x, ca_y, ca_z, ca_q, ca_eps, ca_sig;
+  dnab::DevBuf<uint8_t> ca_hyd;
+  dnab::GridDev ca_grid;
+};
+
+struct dnab_population {
+  dnab_ctx* ctx = nullptr;
+  size_t n_cand = 0, n_res = 0, n_atoms = 0, n_ag = 0;
+  std::vector<dnab_antigen*> antigens;
+  std::vector<dnab::AgView> views_host;
+  dnab::DevBuf<dnab::AgView> views;
+  // candidate atoms: pos = (x, y, z, q), par = (r_min_half, epsilon, hydrophobic, 0)
+  dnab::DevBuf<float4> pos, par;
+  dnab::DevBuf<uint32_t> atom_ag;    // per atom
+  dnab::DevBuf<uint32_t> atom_cand;  // per atom
+  dnab::DevBuf<uint32_t> cand_atom_off, cand_res_off, cand_ag;
+  dnab::DevBuf<uint32_t> res_atom_off, res_ca, res_cand;
+  dnab::DevBuf<uint8_t> res_aa;
+  dnab::DevBuf<float4> res_cg;  // residue-level params (q, eps, sigma, hydrophobic)
+  // candidate-atom spatial sort
+  uint32_t sort_codes = 0;
+  size_t tile_cap = 0;
+  dnab::DevBuf<uint32_t> sort_count;     // sort_codes + 1
+  dnab::DevBuf<uint32_t> sort_rank;      // per atom
+  dnab::DevBuf<uint32_t> perm;           // tile_cap * 64 slots -> atom (or ~0u)
+  dnab::DevBuf<uint32_t> ag_tile_start;  // n_ag + 1
+  bool sorted = false;
+  // evaluation outputs
+  dnab::DevBuf<float4> force;     // per atom (fx, fy, fz, sum|e|)
+  dnab::DevBuf<double> e_atom;    // per atom energy (FP32 kernel, FP64 sum)
+  dnab::DevBuf<double> bd_atom;   // per atom (lj, coulomb, hydrophobic)
+  dnab::DevBuf<double> f64_atom;  // per atom FP64 oracle (fx, fy, fz, e)
+  dnab::DevBuf<double> energy;    // per candidate
+  dnab::DevBuf<float> err_bound;  // per candidate
+  dnab::DevBuf<double> breakdown; // per candidate (lj, coulomb, hydrophobic)
+  unsigned last_flags = 0;
+  bool evaluated = false;
+  // gradient-step state
+  dnab::DevBuf<float4> res_force;  // residue-level model forces
+  dnab::DevBuf<float> sigma_table;
+  dnab::DevBuf<uint32_t> step_counter;
+  dnab::DevBuf<unsigned long long> pair_stats;  // (in cutoff, tested)
+  dnab::DevBuf<uint32_t> topk_idx;
+  dnab::DevBuf<double> topk_e;
+};
+
+// Error helpers (host).
+int dnab_set_error(dnab_ctx* ctx, int code, const char* fmt, ...);
+
+#define DNAB_HIP_CHECK(ctx, expr)                                                     \
+  do {                                                                                \
+    hipError_t e__ = (expr);                                                          \
+    if (e__ != hipSuccess)                                                            \
+      return dnab_set_error((ctx), DNAB_ERR_HIP, "%s failed: %s (%s:%d)", #expr,     \
+                            hipGetErrorString(e__), __FILE__, __LINE__);              \
+  } while (0)
+
+#define DNAB_ALLOC_CHECK(ctx, expr)                                                   \
+  do {                                                                                \
+    hipError_t e__ = (expr);                                                          \
+    if (e__ != hipSuccess)                                                            \
+      return dnab_set_error((ctx), DNAB_ERR_ALLOC, "%s failed: %s (%s:%d)", #expr,   \
+                            hipGetErrorString(e__), __FILE__, __LINE__);              \
+  } while (0)
+
+#define DNAB_RETURN_IF(rc) \
+  do {                     \
+    int rc__ = (rc);       \
+    if (rc__ != DNAB_OK) return rc__; \
+  } while (0)
+
+namespace dnab {
+// internal evaluate flag: the counting variant of the E+F kernel
+constexpr unsigned kEvalCount = 1u << 8;
+// grid.hip
+int grid_build_device(dnab_ctx* ctx, GridDev& g, const float* d_x, const float* d_y,
+                      const float* d_z, size_t n);
+// ctx.hip
+hipEvent_t timer_begin(dnab_ctx* ctx, const char* name, hipEvent_t* end_out);
+void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end);
+int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
+int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
+// pair.hip
+int population_sort(dnab_ctx* ctx, dnab_population* pop);
+int population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
+int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
+}  // namespace dnab

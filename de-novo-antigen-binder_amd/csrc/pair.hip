@@ -1,0 +1,1131 @@
+// All-atom AMBER LJ + Coulomb + hydrophobic energies and forces between
+// candidate binder atoms and a fixed antigen.
+//
+// Reference semantics: interaction_energy_atoms_breakdown
+// (src/energy.rs:261-314) and the WGSL kernel (shaders/energy.wgsl:46-118):
+// for every (candidate atom j, antigen atom i) with 0.25 <= r^2 <= rc^2,
+// r^2 = (dx*dx + dy*dy) + dz*dz in f32 with dx = x_j - x_i,
+//   LJ   eps_ij ((R/r)^12 - 2 (R/r)^6), R = rmh_i + rmh_j, eps_ij = sqrt(eps_i eps_j)
+//   Coul 332 q_i q_j / r
+//   hydrophobic -0.5 when both atoms are flagged and r^2 < 36.
+//
+// Layout and mapping (gfx950, wave64):
+//  * antigen atoms are sorted cell-major over its SpatialHashGrid (Morton
+//    order of cells), one 32-byte AgAtom each; a wave loads a cell's atoms
+//    64 at a time (one per lane, coalesced) and the pair loop broadcasts
+//    atom k to all lanes with v_readlane into SGPRs, so the pair loop itself
+//    has no memory accesses;
+//  * candidate atoms of the whole population are counting-sorted by a 4 A
+//    Morton cell around their antigen; a tile is 64 consecutive sorted atoms,
+//    one per lane, so a tile's atoms are spatially compact whichever
+//    candidate they belong to;
+//  * a workgroup of 4 waves shares one tile: the waves walk the grid cells
+//    overlapping the tile box + cutoff in the same fixed order, test each
+//    cell's antigen atoms against the box (one atom per lane + ballot), and
+//    split the surviving atoms by (sorted index & 3); partial sums meet in
+//    LDS in wave order. Each atom's result therefore does not depend on the
+//    tile it landed in, and repeated runs are bit-identical.
+//  * energy per atom: FP32 pair terms, TwoSum-compensated (lj + coulomb),
+//    hydrophobic pairs counted as an integer; per-atom value stored in FP64
+//    and summed per candidate in FP64 with DPP wave reductions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_common.h"
+#include "dnab_internal.h"
+#include "grid_device.h"
+
+namespace dnab {
+
+constexpr float kMinRSq = 0.25f;   // src/energy.rs:17
+constexpr float kHydroSq = 36.0f;  // src/energy.rs:20
+constexpr float kSortCell = 4.0f;  // candidate-sort cell edge
+constexpr int kSortMaxBits = 7;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr int kTileWaves = 4;
+
+__global__ void k_fill_u32b(uint32_t* p, uint64_t n, uint32_t v) {
+  uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+static inline uint32_t nblocks(uint64_t n, uint32_t t) {
+  uint64_t b = (n + t - 1) / t;
+  return static_cast<uint32_t>(b ? b : 1);
+}
+
+// ---- antigen records -------------------------------------------------------------
+
+__global__ void k_ag_gather(const uint32_t* __restrict__ sorted, uint32_t n,
+                            const float* __restrict__ x, const float* __restrict__ y,
+                            const float* __restrict__ z, const float* __restrict__ q,
+                            const float* __restrict__ rmh, const float* __restrict__ eps,
+                            const uint8_t* __restrict__ hyd, AgAtom* __restrict__ out,
+                            float2* __restrict__ raw) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint32_t i = sorted[s];
+  AgAtom a;
+  a.x = x[i];
+  a.y = y[i];
+  a.z = z[i];
+  a.q332 = 332.0f * q[i];
+  a.rmh = rmh[i];
+  a.sqe = sqrtf(eps[i]);
+  a.hyd = hyd[i] ? 1.0f : 0.0f;
+  a.pad = 0.0f;
+  out[s] = a;
+  raw[s] = make_float2(q[i], eps[i]);
+}
+
+// ---- candidate-atom spatial sort ---------------------------------------------------
+
+__device__ inline uint32_t sort_code(const AgView& V, float x, float y, float z) {
+  float p[3] = {x, y, z};
+  uint32_t c[3];
+  for (int a = 0; a < 3; ++a) {
+    float f = floorf((p[a] - V.sort_origin[a]) * V.sort_inv);
+    int32_t k = f32_to_i32_sat(f);
+    k = k < 0 ? 0 : (k >= V.sort_dim[a] ? V.sort_dim[a] - 1 : k);
+    c[a] = static_cast<uint32_t>(k);
+  }
+  return V.sort_code_base + morton3(c[0], c[1], c[2]);
+}
+
+__global__ void k_sort_count(const AgView* __restrict__ views, const float4* __restrict__ pos,
+                             const uint32_t* __restrict__ atom_ag, uint32_t n,
+                             uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float4 p = pos[j];
+  uint32_t code = sort_code(views[atom_ag[j]], p.x, p.y, p.z);
+  rank[j] = atomicAdd(&count[code], 1u);
+}
+
+// Exclusive scan of the sort histogram, one antigen segment after another,
+// each segment starting on a 64-slot tile boundary. Writes the tile range of
+// every antigen to ag_tile_start (n_ag + 1 entries).
+__global__ void __launch_bounds__(1024) k_sort_scan(const AgView* __restrict__ views,
+                                                    uint32_t n_ag, uint32_t* __restrict__ count,
+                                                    uint32_t* __restrict__ ag_tile_start) {
+  __shared__ uint32_t sums[1024];
+  __shared__ uint32_t carry;
+  const uint32_t t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t a = 0; a < n_ag; ++a) {
+    const uint32_t base = views[a].sort_code_base;
+    const uint32_t ncode = 1u << (3 * views[a].sort_bits);
+    const uint32_t chunk = (ncode + 1023) / 1024;
+    const uint32_t b = min(ncode, t * chunk), e = min(ncode, b + chunk);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += count[base + i];
+    sums[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+      uint32_t v = t >= off ? sums[t - off] : 0u;
+      __syncthreads();
+      sums[t] += v;
+      __syncthreads();
+    }
+    const uint32_t start = carry;
+    uint32_t run = start + (t ? sums[t - 1] : 0u);
+    for (uint32_t i = b; i < e; ++i) {
+      uint32_t v = count[base + i];
+      count[base + i] = run;
+      run += v;
+    }
+    __syncthreads();
+    if (t == 0) {
+      ag_tile_start[a] = start / kWave;
+      uint32_t end = start + sums[1023];
+      carry = (end + kWave - 1) / kWave * kWave;
+    }
+    __syncthreads();
+  }
+  if (t == 0) ag_tile_start[n_ag] = carry / kWave;
+}
+
+__global__ void k_sort_scatter(const AgView* __restrict__ views, const float4* __restrict__ pos,
+                               const uint32_t* __restrict__ atom_ag, uint32_t n,
+                               const uint32_t* __restrict__ start,
+                               const uint32_t* __restrict__ rank, uint32_t* __restrict__ perm) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float4 p = pos[j];
+  uint32_t code = sort_code(views[atom_ag[j]], p.x, p.y, p.z);
+  perm[start[code] + rank[j]] = j;
+}
+
+// ---- the pair tile kernel ---------------------------------------------------------
+
+struct TileOut {
+  float4* force;      // per atom (fx, fy, fz, sum |e|)
+  double* e_atom;     // per atom energy
+  double* bd_atom;    // per atom (lj, coulomb, hydrophobic), BREAKDOWN only
+  unsigned long long* stats;  // (in cutoff, tested), COUNT only
+};
+
+__device__ inline uint32_t tile_antigen(const uint32_t* __restrict__ ag_tile_start, uint32_t n_ag,
+                                        uint32_t t) {
+  uint32_t lo = 0, hi = n_ag;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (ag_tile_start[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool FORCES, bool BREAKDOWN, bool COUNT>
+__global__ void __launch_bounds__(256)
+k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
+            const uint32_t* __restrict__ ag_tile_start, const uint32_t* __restrict__ perm,
+            const float4* __restrict__ pos, const float4* __restrict__ par, TileOut out) {
+  const uint32_t t = blockIdx.x;
+  const uint32_t n_tiles = ag_tile_start[n_ag];
+  if (t >= n_tiles) return;
+  const uint32_t a_id = tile_antigen(ag_tile_start, n_ag, t);
+  const AgAtom* __restrict__ atoms = views[a_id].atoms;
+  const uint32_t* __restrict__ cstart = views[a_id].cell_start;
+  const float inv = views[a_id].inv;
+  const float rc = views[a_id].rc, rc2 = views[a_id].rc2;
+  const int bits = views[a_id].bits;
+  const int32_t kmin0 = views[a_id].kmin[0], kmin1 = views[a_id].kmin[1], kmin2 = views[a_id].kmin[2];
+  const int32_t kmax0 = views[a_id].kmax[0], kmax1 = views[a_id].kmax[1], kmax2 = views[a_id].kmax[2];
+
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t j = perm[t * kWave + lane];
+  const bool valid = j != kInvalid;
+  float4 P = make_float4(0.f, 0.f, 0.f, 0.f), Q = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid) {
+    P = pos[j];
+    Q = par[j];
+  }
+  const float xb = P.x, yb = P.y, zb = P.z, qb = P.w;
+  const float rmhb = Q.x;
+  const float sqeb = sqrtf(Q.y);
+  const float sqe12b = 12.0f * sqeb;
+  const bool hydb = Q.z != 0.0f;
+
+  const float inf = __builtin_inff();
+  const float lox = wave_reduce_f32<OpMin>(valid ? xb : inf);
+  const float loy = wave_reduce_f32<OpMin>(valid ? yb : inf);
+  const float loz = wave_reduce_f32<OpMin>(valid ? zb : inf);
+  const float hix = wave_reduce_f32<OpMax>(valid ? xb : -inf);
+  const float hiy = wave_reduce_f32<OpMax>(valid ? yb : -inf);
+  const float hiz = wave_reduce_f32<OpMax>(valid ? zb : -inf);
+
+  // Cells that can hold an antigen atom within rc of the tile box. cell_key
+  // is monotone, so a slightly widened range is conservative.
+  const float margin = rc * 1.001f + 1e-3f;
+  const int32_t kx0 = max(cell_key(lox - margin, inv), kmin0);
+  const int32_t ky0 = max(cell_key(loy - margin, inv), kmin1);
+  const int32_t kz0 = max(cell_key(loz - margin, inv), kmin2);
+  const int32_t kx1 = min(cell_key(hix + margin, inv), kmax0);
+  const int32_t ky1 = min(cell_key(hiy + margin, inv), kmax1);
+  const int32_t kz1 = min(cell_key(hiz + margin, inv), kmax2);
+
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  float es = 0.f, ec = 0.f, eabs = 0.f;        // TwoSum (sum, compensation), sum |e|
+  float ljs = 0.f, ljc = 0.f, cos_ = 0.f, coc = 0.f;  // BREAKDOWN
+  uint32_t nh = 0;
+  uint32_t n_in = 0, n_tested = 0;
+  (void)bits;
+
+  // Cells of the box go to waves by (kx + ky + kz) & 3, a function of the
+  // cell alone; each wave visits its cells in (kz, ky, kx) order. Lane l
+  // loads the range of the box's l-th cell of the current batch of 64.
+  const int32_t nx = kx1 - kx0 + 1, ny = ky1 - ky0 + 1, nz = kz1 - kz0 + 1;
+  const int32_t ncell = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
+  for (int32_t cbase = 0; cbase < ncell; cbase += kWave) {
+    const int32_t c = cbase + static_cast<int32_t>(lane);
+    uint32_t my_b = 0, my_e = 0;
+    if (c < ncell) {
+      const int32_t ix = c % nx, iy = (c / nx) % ny, iz = c / (nx * ny);
+      const int32_t kx = kx0 + ix, ky = ky0 + iy, kz = kz0 + iz;
+      if (((kx + ky + kz) & 3) == static_cast<int32_t>(wave)) {
+        const uint32_t code = morton3(static_cast<uint32_t>(kx - kmin0),
+                                      static_cast<uint32_t>(ky - kmin1),
+                                      static_cast<uint32_t>(kz - kmin2));
+        my_b = cstart[code];
+        my_e = cstart[code + 1];
+      }
+    }
+    uint64_t cmask = ballot64(my_e > my_b);
+    while (cmask) {
+      const int cl = static_cast<int>(__builtin_ctzll(cmask));
+      cmask &= cmask - 1;
+      const uint32_t cb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(my_b), cl));
+      const uint32_t ce = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(my_e), cl));
+      {
+        for (uint32_t base = cb; base < ce; base += kWave) {
+          // One coalesced load of the chunk's atoms (one per lane); the pair
+          // loop below broadcasts atom k from lane k with v_readlane.
+          const uint32_t i = base + lane;
+          float4 A0 = make_float4(0.f, 0.f, 0.f, 0.f), A1 = make_float4(0.f, 0.f, 0.f, 0.f);
+          bool near = false;
+          if (i < ce) {
+            A0 = *reinterpret_cast<const float4*>(&atoms[i].x);
+            A1 = *reinterpret_cast<const float4*>(&atoms[i].rmh);
+            const float bx = fmaxf(fmaxf(lox - A0.x, A0.x - hix), 0.0f);
+            const float by = fmaxf(fmaxf(loy - A0.y, A0.y - hiy), 0.0f);
+            const float bz = fmaxf(fmaxf(loz - A0.z, A0.z - hiz), 0.0f);
+            const float d2 = (bx * bx + by * by) + bz * bz;
+            near = d2 <= rc2;
+          }
+          uint64_t mask = ballot64(near);
+          while (mask) {
+            const int k = static_cast<int>(__builtin_ctzll(mask));
+            mask &= mask - 1;
+            AgAtom A;
+            A.x = readlane_f32(A0.x, k);
+            A.y = readlane_f32(A0.y, k);
+            A.z = readlane_f32(A0.z, k);
+            A.q332 = readlane_f32(A0.w, k);
+            A.rmh = readlane_f32(A1.x, k);
+            A.sqe = readlane_f32(A1.y, k);
+            A.hyd = readlane_f32(A1.z, k);
+            // Pair predicate in f32, exactly the reference's (src/energy.rs:279-284).
+            const float dx = xb - A.x;
+            const float dy = yb - A.y;
+            const float dz = zb - A.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            const bool in = valid && r2 <= rc2 && r2 >= kMinRSq;
+            if (COUNT) {
+              n_tested += valid ? 1u : 0u;
+              n_in += in ? 1u : 0u;
+            }
+            float rinv = __builtin_amdgcn_rsqf(r2);
+            rinv = in ? rinv : 0.0f;
+            const float rinv2 = rinv * rinv;
+            const float R = A.rmh + rmhb;
+            const float s2 = (R * R) * rinv2;
+            const float s6 = (s2 * s2) * s2;
+            const float s12 = s6 * s6;
+            const float eps = A.sqe * sqeb;
+            const float elj = eps * __builtin_fmaf(-2.0f, s6, s12);
+            const float ecl = (A.q332 * qb) * rinv;
+            float f = 0.0f;
+            if (FORCES) {
+              const float flj = ((A.sqe * sqe12b) * (s12 - s6)) * rinv2;
+              f = __builtin_fmaf(ecl, rinv2, flj);
+            }
+            if (A.hyd != 0.0f) {
+              const bool h = in && hydb && r2 < kHydroSq;
+              nh += h ? 1u : 0u;
+              if (FORCES) f = h ? __builtin_fmaf(-0.1f, rinv, f) : f;
+            }
+            if (FORCES) {
+              fx = __builtin_fmaf(f, dx, fx);
+              fy = __builtin_fmaf(f, dy, fy);
+              fz = __builtin_fmaf(f, dz, fz);
+            }
+            if (BREAKDOWN) {
+              float s, e;
+              two_sum(ljs, elj, s, e);
+              ljs = s;
+              ljc += e;
+              two_sum(cos_, ecl, s, e);
+              cos_ = s;
+              coc += e;
+            } else {
+              const float ep = elj + ecl;
+              float s, e;
+              two_sum(es, ep, s, e);
+              es = s;
+              ec += e;
+              eabs += fabsf(ep);
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // Combine the four waves' partial sums in wave order.
+  __shared__ float sh_f[kTileWaves][3][kWave];
+  __shared__ float sh_e[kTileWaves][4][kWave];
+  __shared__ uint32_t sh_nh[kTileWaves][kWave];
+  if (FORCES) {
+    sh_f[wave][0][lane] = fx;
+    sh_f[wave][1][lane] = fy;
+    sh_f[wave][2][lane] = fz;
+  }
+  if (BREAKDOWN) {
+    sh_e[wave][0][lane] = ljs;
+    sh_e[wave][1][lane] = ljc;
+    sh_e[wave][2][lane] = cos_;
+    sh_e[wave][3][lane] = coc;
+  } else {
+    sh_e[wave][0][lane] = es;
+    sh_e[wave][1][lane] = ec;
+    sh_e[wave][2][lane] = eabs;
+  }
+  sh_nh[wave][lane] = nh;
+  if (COUNT) {
+    uint32_t a = wave_sum_u32(n_in), b = wave_sum_u32(n_tested);
+    if (lane == 0) {
+      atomicAdd(&out.stats[0], static_cast<unsigned long long>(a));
+      atomicAdd(&out.stats[1], static_cast<unsigned long long>(b));
+    }
+  }
+  __syncthreads();
+  if (wave != 0 || !valid) return;
+  uint32_t nh_t = 0;
+  for (int w = 0; w < kTileWaves; ++w) nh_t += sh_nh[w][lane];
+  const double ehyd = -0.5 * static_cast<double>(nh_t);
+  float4 F = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (FORCES) {
+    for (int w = 0; w < kTileWaves; ++w) {
+      F.x += sh_f[w][0][lane];
+      F.y += sh_f[w][1][lane];
+      F.z += sh_f[w][2][lane];
+    }
+  }
+  if (BREAKDOWN) {
+    double lj = 0.0, co = 0.0;
+    for (int w = 0; w < kTileWaves; ++w) {
+      lj += static_cast<double>(sh_e[w][0][lane]) + static_cast<double>(sh_e[w][1][lane]);
+      co += static_cast<double>(sh_e[w][2][lane]) + static_cast<double>(sh_e[w][3][lane]);
+    }
+    out.bd_atom[3 * static_cast<size_t>(j)] = lj;
+    out.bd_atom[3 * static_cast<size_t>(j) + 1] = co;
+    out.bd_atom[3 * static_cast<size_t>(j) + 2] = ehyd;
+    out.e_atom[j] = (lj + co) + ehyd;
+  } else {
+    float S = sh_e[0][0][lane], C = sh_e[0][1][lane], Aabs = sh_e[0][2][lane];
+    for (int w = 1; w < kTileWaves; ++w) {
+      float s, e;
+      two_sum(S, sh_e[w][0][lane], s, e);
+      S = s;
+      C = (C + e) + sh_e[w][1][lane];
+      Aabs += sh_e[w][2][lane];
+    }
+    out.e_atom[j] = (static_cast<double>(S) + static_cast<double>(C)) + ehyd;
+    F.w = Aabs;
+  }
+  out.force[j] = F;
+}
+
+// All-FP64 oracle kernel: the same pair set (decided by the f32 predicate)
+// with every term, mixing rule and sum in double from the raw per-atom
+// parameters. Writes per atom (fx, fy, fz) and (lj, coulomb, hydrophobic).
+__global__ void __launch_bounds__(256)
+k_pair_tile_f64(const AgView* __restrict__ views, uint32_t n_ag,
+                const uint32_t* __restrict__ ag_tile_start, const uint32_t* __restrict__ perm,
+                const float4* __restrict__ pos, const float4* __restrict__ par,
+                double* __restrict__ f64_atom, double* __restrict__ bd_atom,
+                double* __restrict__ e_atom) {
+  const uint32_t t = blockIdx.x;
+  const uint32_t n_tiles = ag_tile_start[n_ag];
+  if (t >= n_tiles) return;
+  const uint32_t a_id = tile_antigen(ag_tile_start, n_ag, t);
+  const AgView& V = views[a_id];
+  const AgAtom* __restrict__ atoms = V.atoms;
+  const float2* __restrict__ raw = V.raw;
+  const uint32_t* __restrict__ cstart = V.cell_start;
+  const float inv = V.inv, rc = V.rc, rc2 = V.rc2;
+  const int32_t kmin0 = V.kmin[0], kmin1 = V.kmin[1], kmin2 = V.kmin[2];
+  const int32_t kmax0 = V.kmax[0], kmax1 = V.kmax[1], kmax2 = V.kmax[2];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t j = perm[t * kWave + lane];
+  const bool valid = j != kInvalid;
+  float4 P = make_float4(0.f, 0.f, 0.f, 0.f), Q = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid) {
+    P = pos[j];
+    Q = par[j];
+  }
+  const float xb = P.x, yb = P.y, zb = P.z;
+  const double qb = P.w, rmhb = Q.x, epsb = Q.y;
+  const bool hydb = Q.z != 0.0f;
+  const float inf = __builtin_inff();
+  const float lox = wave_reduce_f32<OpMin>(valid ? xb : inf);
+  const float loy = wave_reduce_f32<OpMin>(valid ? yb : inf);
+  const float loz = wave_reduce_f32<OpMin>(valid ? zb : inf);
+  const float hix = wave_reduce_f32<OpMax>(valid ? xb : -inf);
+  const float hiy = wave_reduce_f32<OpMax>(valid ? yb : -inf);
+  const float hiz = wave_reduce_f32<OpMax>(valid ? zb : -inf);
+  const float margin = rc * 1.001f + 1e-3f;
+  const int32_t kx0 = max(cell_key(lox - margin, inv), kmin0);
+  const int32_t ky0 = max(cell_key(loy - margin, inv), kmin1);
+  const int32_t kz0 = max(cell_key(loz - margin, inv), kmin2);
+  const int32_t kx1 = min(cell_key(hix + margin, inv), kmax0);
+  const int32_t ky1 = min(cell_key(hiy + margin, inv), kmax1);
+  const int32_t kz1 = min(cell_key(hiz + margin, inv), kmax2);
+  double fx = 0, fy = 0, fz = 0, lj = 0, co = 0;
+  uint32_t nh = 0;
+  for (int32_t kz = kz0; kz <= kz1; ++kz)
+    for (int32_t ky = ky0; ky <= ky1; ++ky)
+      for (int32_t kx = kx0; kx <= kx1; ++kx) {
+        const uint32_t code = morton3(static_cast<uint32_t>(kx - kmin0), static_cast<uint32_t>(ky - kmin1),
+                                      static_cast<uint32_t>(kz - kmin2));
+        const uint32_t cb = cstart[code], ce = cstart[code + 1];
+        for (uint32_t base = cb; base < ce; base += kWave) {
+          const uint32_t i = base + lane;
+          bool near = false;
+          if (i < ce) {
+            const float4 A = *reinterpret_cast<const float4*>(&atoms[i]);
+            const float bx = fmaxf(fmaxf(lox - A.x, A.x - hix), 0.0f);
+            const float by = fmaxf(fmaxf(loy - A.y, A.y - hiy), 0.0f);
+            const float bz = fmaxf(fmaxf(loz - A.z, A.z - hiz), 0.0f);
+            near = (bx * bx + by * by) + bz * bz <= rc2 && ((i & 3u) == wave);
+          }
+          uint64_t mask = ballot64(near);
+          while (mask) {
+            const uint32_t k = static_cast<uint32_t>(__builtin_ctzll(mask));
+            mask &= mask - 1;
+            const AgAtom A = atoms[base + k];
+            const float dx = xb - A.x, dy = yb - A.y, dz = zb - A.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            if (!(valid && r2 <= rc2 && r2 >= kMinRSq)) continue;
+            const float2 rw = raw[base + k];
+            const double ddx = static_cast<double>(xb) - static_cast<double>(A.x);
+            const double ddy = static_cast<double>(yb) - static_cast<double>(A.y);
+            const double ddz = static_cast<double>(zb) - static_cast<double>(A.z);
+            const double r2d = ddx * ddx + ddy * ddy + ddz * ddz;
+            const double r = sqrt(r2d);
+            const double R = static_cast<double>(A.rmh) + rmhb;
+            const double eps = sqrt(static_cast<double>(rw.y) * epsb);
+            const double s2 = (R * R) / r2d;
+            const double s6 = s2 * s2 * s2;
+            const double s12 = s6 * s6;
+            lj += eps * (s12 - 2.0 * s6);
+            double f = 12.0 * eps * (s12 - s6) / r2d;
+            const double ecl = 332.0 * qb * static_cast<double>(rw.x) / r;
+            co += ecl;
+            f += ecl / r2d;
+            if (A.hyd != 0.0f && hydb && r2 < kHydroSq) {
+              ++nh;
+              f += -0.1 / r;
+            }
+            fx += f * ddx;
+            fy += f * ddy;
+            fz += f * ddz;
+          }
+        }
+      }
+  __shared__ double sh[kTileWaves][5][kWave];
+  __shared__ uint32_t sh_nh[kTileWaves][kWave];
+  sh[wave][0][lane] = fx;
+  sh[wave][1][lane] = fy;
+  sh[wave][2][lane] = fz;
+  sh[wave][3][lane] = lj;
+  sh[wave][4][lane] = co;
+  sh_nh[wave][lane] = nh;
+  __syncthreads();
+  if (wave != 0 || !valid) return;
+  double acc[5] = {0, 0, 0, 0, 0};
+  uint32_t nh_t = 0;
+  for (int w = 0; w < kTileWaves; ++w) {
+    for (int c = 0; c < 5; ++c) acc[c] += sh[w][c][lane];
+    nh_t += sh_nh[w][lane];
+  }
+  const size_t jj = j;
+  f64_atom[3 * jj] = acc[0];
+  f64_atom[3 * jj + 1] = acc[1];
+  f64_atom[3 * jj + 2] = acc[2];
+  bd_atom[3 * jj] = acc[3];
+  bd_atom[3 * jj + 1] = acc[4];
+  bd_atom[3 * jj + 2] = -0.5 * static_cast<double>(nh_t);
+  e_atom[jj] = (acc[3] + acc[4]) + (-0.5 * static_cast<double>(nh_t));
+}
+
+// ---- per-candidate reductions (FP64, DPP) ----------------------------------------
+
+// a-priori error estimate of an FP32 candidate energy: 32 ulp(1) per unit of
+// sum |e_pair| (rsqrt, products and the compensated sum each a few ulp).
+constexpr float kErrPerAbs = 32.0f / 16777216.0f;
+
+__global__ void __launch_bounds__(256)
+k_reduce_energy(const uint32_t* __restrict__ cand_atom_off, uint32_t n_cand,
+                const double* __restrict__ e_atom, const float4* __restrict__ force,
+                int with_abs, double* __restrict__ energy, float* __restrict__ err) {
+  const uint32_t c = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  if (c >= n_cand) return;  // whole wave exits together
+  const uint32_t b = cand_atom_off[c], e = cand_atom_off[c + 1];
+  double s = 0.0, a = 0.0;
+  for (uint32_t k = b + lane; k < e; k += kWave) {
+    s += e_atom[k];
+    if (with_abs) a += static_cast<double>(force[k].w);
+  }
+  s = wave_sum_f64(s);
+  a = wave_sum_f64(a);
+  if (lane == 0) {
+    energy[c] = s;
+    err[c] = with_abs ? static_cast<float>(a) * kErrPerAbs : 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_reduce_breakdown(const uint32_t* __restrict__ cand_atom_off, uint32_t n_cand,
+                   const double* __restrict__ bd_atom, double* __restrict__ bd) {
+  const uint32_t c = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  if (c >= n_cand) return;
+  const uint32_t b = cand_atom_off[c], e = cand_atom_off[c + 1];
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (uint32_t k = b + lane; k < e; k += kWave) {
+    s0 += bd_atom[3 * static_cast<size_t>(k)];
+    s1 += bd_atom[3 * static_cast<size_t>(k) + 1];
+    s2 += bd_atom[3 * static_cast<size_t>(k) + 2];
+  }
+  s0 = wave_sum_f64(s0);
+  s1 = wave_sum_f64(s1);
+  s2 = wave_sum_f64(s2);
+  if (lane == 0) {
+    bd[3 * static_cast<size_t>(c)] = s0;
+    bd[3 * static_cast<size_t>(c) + 1] = s1;
+    bd[3 * static_cast<size_t>(c) + 2] = s2;
+  }
+}
+
+// ---- host side ------------------------------------------------------------------
+
+int population_sort(dnab_ctx* ctx, dnab_population* pop) {
+  const uint32_t n = static_cast<uint32_t>(pop->n_atoms);
+  hipLaunchKernelGGL(k_fill_u32b, dim3(nblocks(pop->sort_codes, 256)), dim3(256), 0, ctx->stream,
+                     pop->sort_count.p, static_cast<uint64_t>(pop->sort_codes) + 1, 0u);
+  hipLaunchKernelGGL(k_fill_u32b, dim3(nblocks(pop->tile_cap * kWave, 256)), dim3(256), 0,
+                     ctx->stream, pop->perm.p, static_cast<uint64_t>(pop->tile_cap) * kWave,
+                     kInvalid);
+  if (n) {
+    hipLaunchKernelGGL(k_sort_count, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                       pop->views.p, pop->pos.p, pop->atom_ag.p, n, pop->sort_count.p,
+                       pop->sort_rank.p);
+  }
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, ctx->stream, pop->views.p,
+                     static_cast<uint32_t>(pop->n_ag), pop->sort_count.p, pop->ag_tile_start.p);
+  if (n) {
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                       pop->views.p, pop->pos.p, pop->atom_ag.p, n, pop->sort_count.p,
+                       pop->sort_rank.p, pop->perm.p);
+  }
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  pop->sorted = true;
+  return DNAB_OK;
+}
+
+int population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
+  if (!pop->sorted) DNAB_RETURN_IF(population_sort(ctx, pop));
+  const uint32_t n_ag = static_cast<uint32_t>(pop->n_ag);
+  const dim3 grid(static_cast<uint32_t>(pop->tile_cap ? pop->tile_cap : 1)), block(256);
+  TileOut o{pop->force.p, pop->e_atom.p, pop->bd_atom.p, pop->pair_stats.p};
+  if (flags & DNAB_EVAL_FP64) {
+    hipLaunchKernelGGL(k_pair_tile_f64, grid, block, 0, ctx->stream, pop->views.p, n_ag,
+                       pop->ag_tile_start.p, pop->perm.p, pop->pos.p, pop->par.p,
+                       pop->f64_atom.p, pop->bd_atom.p, pop->e_atom.p);
+  } else if (flags & DNAB_EVAL_BREAKDOWN) {
+    hipLaunchKernelGGL((k_pair_tile<false, true, false>), grid, block, 0, ctx->stream,
+                       pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
+                       pop->par.p, o);
+  } else if (flags & kEvalCount) {
+    hipLaunchKernelGGL((k_pair_tile<true, false, true>), grid, block, 0, ctx->stream,
+                       pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
+                       pop->par.p, o);
+  } else if (flags & DNAB_EVAL_FORCES) {
+    hipEvent_t ev_end;
+    hipEvent_t ev = timer_begin(ctx, "pair_ef", &ev_end);
+    hipLaunchKernelGGL((k_pair_tile<true, false, false>), grid, block, 0, ctx->stream,
+                       pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
+                       pop->par.p, o);
+    timer_end(ctx, "pair_ef", ev, ev_end);
+  } else {
+    hipEvent_t ev_end;
+    hipEvent_t ev = timer_begin(ctx, "pair_e", &ev_end);
+    hipLaunchKernelGGL((k_pair_tile<false, false, false>), grid, block, 0, ctx->stream,
+                       pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
+                       pop->par.p, o);
+    timer_end(ctx, "pair_e", ev, ev_end);
+  }
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  return population_reduce_energies(ctx, pop, flags);
+}
+
+int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
+  const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
+  if (nc) {
+    const int with_abs = (flags & (DNAB_EVAL_FP64 | DNAB_EVAL_BREAKDOWN)) ? 0 : 1;
+    hipLaunchKernelGGL(k_reduce_energy, dim3(nblocks(nc, 4)), dim3(256), 0, ctx->stream,
+                       pop->cand_atom_off.p, nc, pop->e_atom.p, pop->force.p, with_abs,
+                       pop->energy.p, pop->err_bound.p);
+    if (flags & (DNAB_EVAL_FP64 | DNAB_EVAL_BREAKDOWN)) {
+      hipLaunchKernelGGL(k_reduce_breakdown, dim3(nblocks(nc, 4)), dim3(256), 0, ctx->stream,
+                         pop->cand_atom_off.p, nc, pop->bd_atom.p, pop->breakdown.p);
+    }
+  }
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  pop->last_flags = (flags & kEvalCount) ? (flags | DNAB_EVAL_FORCES) & ~kEvalCount : flags;
+  pop->evaluated = true;
+  return DNAB_OK;
+}
+
+static int pair_count(dnab_ctx* ctx, dnab_population* pop, uint64_t* in_cutoff,
+                      uint64_t* tested) {
+  if (!pop->sorted) DNAB_RETURN_IF(population_sort(ctx, pop));
+  unsigned long long zero[2] = {0, 0};
+  DNAB_RETURN_IF(upload(ctx, pop->pair_stats.p, zero, sizeof(zero)));
+  TileOut o{pop->force.p, pop->e_atom.p, pop->bd_atom.p, pop->pair_stats.p};
+  hipLaunchKernelGGL((k_pair_tile<true, false, true>),
+                     dim3(static_cast<uint32_t>(pop->tile_cap ? pop->tile_cap : 1)), dim3(256), 0,
+                     ctx->stream, pop->views.p, static_cast<uint32_t>(pop->n_ag),
+                     pop->ag_tile_start.p, pop->perm.p, pop->pos.p, pop->par.p, o);
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  unsigned long long h[2];
+  DNAB_RETURN_IF(download(ctx, h, pop->pair_stats.p, sizeof(h)));
+  if (in_cutoff) *in_cutoff = h[0];
+  if (tested) *tested = h[1];
+  DNAB_RETURN_IF(population_reduce_energies(ctx, pop, DNAB_EVAL_FORCES));
+  return DNAB_OK;
+}
+
+}  // namespace dnab
+
+using namespace dnab;
+
+namespace {
+
+bool atom_cloud_ok(const dnab_atom_cloud* c) {
+  return c && (c->n == 0 || (c->x && c->y && c->z && c->charge && c->r_min_half && c->epsilon &&
+                             c->hydrophobic));
+}
+
+template <class T>
+int up(dnab_ctx* ctx, DevBuf<T>& b, const T* src, size_t n) {
+  DNAB_ALLOC_CHECK(ctx, b.alloc(n));
+  return upload(ctx, b.p, src, n * sizeof(T));
+}
+
+int build_ca_grid(dnab_ctx* ctx, dnab_antigen* ag, const dnab_residue_cloud* ca) {
+  ag->n_ca = ca->n;
+  size_t n = ca->n;
+  DNAB_RETURN_IF(up(ctx, ag->ca_x, ca->x, n));
+  DNAB_RETURN_IF(up(ctx, ag->ca_y, ca->y, n));
+  DNAB_RETURN_IF(up(ctx, ag->ca_z, ca->z, n));
+  DNAB_RETURN_IF(up(ctx, ag->ca_q, ca->charge, n));
+  DNAB_RETURN_IF(up(ctx, ag->ca_eps, ca->epsilon, n));
+  DNAB_RETURN_IF(up(ctx, ag->ca_sig, ca->sigma, n));
+  DNAB_RETURN_IF(up(ctx, ag->ca_hyd, ca->hydrophobic, n));
+  ag->ca_grid.cell = 10.0f;  // src/diffusion.rs:1003
+  ag->ca_grid.inv = 1.0f / 10.0f;
+  return grid_build_device(ctx, ag->ca_grid, ag->ca_x.p, ag->ca_y.p, ag->ca_z.p, n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* d, dnab_antigen** out) {
+  if (!ctx || !d || !out || !atom_cloud_ok(&d->atoms)) return DNAB_ERR_INVALID;
+  if (d->ca.n && !(d->ca.x && d->ca.y && d->ca.z && d->ca.charge && d->ca.epsilon &&
+                   d->ca.sigma && d->ca.hydrophobic))
+    return DNAB_ERR_INVALID;
+  if (!(d->cell_size > 0.0f) || !std::isfinite(d->cell_size) || !(d->cutoff > 0.0f) ||
+      !std::isfinite(d->cutoff))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "antigen: cell_size and cutoff must be > 0");
+  *out = nullptr;
+  (void)hipSetDevice(ctx->device);
+  dnab_antigen* ag = new (std::nothrow) dnab_antigen();
+  if (!ag) return DNAB_ERR_ALLOC;
+  ag->ctx = ctx;
+  ag->cutoff = d->cutoff;
+  ag->restraint_radius = d->restraint_radius;
+  for (int a = 0; a < 3; ++a) ag->center[a] = d->center[a];
+  const size_t n = d->atoms.n;
+  const dnab_atom_cloud& c = d->atoms;
+  auto fail = [&](int rc) {
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ag;
+    return rc;
+  };
+  // bounding box of the atoms (host)
+  if (n) {
+    for (int a = 0; a < 3; ++a) {
+      ag->bbox_lo[a] = INFINITY;
+      ag->bbox_hi[a] = -INFINITY;
+    }
+    const float* p[3] = {c.x, c.y, c.z};
+    for (size_t i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        float v = p[a][i];
+        if (!std::isfinite(v))
+          return fail(dnab_set_error(ctx, DNAB_ERR_INVALID, "antigen: non-finite coordinate"));
+        ag->bbox_lo[a] = std::min(ag->bbox_lo[a], v);
+        ag->bbox_hi[a] = std::max(ag->bbox_hi[a], v);
+      }
+  }
+  GridDev& g = ag->grid;
+  g.cell = d->cell_size;
+  g.inv = 1.0f / d->cell_size;
+  DevBuf<float> q, rmh, eps;
+  DevBuf<uint8_t> hyd;
+  int rc = DNAB_OK;
+  if ((rc = up(ctx, g.x, c.x, n)) || (rc = up(ctx, g.y, c.y, n)) || (rc = up(ctx, g.z, c.z, n)) ||
+      (rc = up(ctx, q, c.charge, n)) || (rc = up(ctx, rmh, c.r_min_half, n)) ||
+      (rc = up(ctx, eps, c.epsilon, n)) || (rc = up(ctx, hyd, c.hydrophobic, n)))
+    return fail(rc);
+  if ((rc = grid_build_device(ctx, g, g.x.p, g.y.p, g.z.p, n))) return fail(rc);
+  if (ag->atoms.alloc(n) != hipSuccess || ag->raw.alloc(n) != hipSuccess)
+    return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "antigen: allocation failed"));
+  if (n) {
+    hipLaunchKernelGGL(k_ag_gather, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                       g.sorted_idx.p, static_cast<uint32_t>(n), g.x.p, g.y.p, g.z.p, q.p, rmh.p,
+                       eps.p, hyd.p, ag->atoms.p, ag->raw.p);
+  }
+  if (hipGetLastError() != hipSuccess)
+    return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "antigen: gather launch failed"));
+  if ((rc = build_ca_grid(ctx, ag, &d->ca))) return fail(rc);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "antigen: synchronize failed"));
+  *out = ag;
+  return DNAB_OK;
+}
+
+int dnab_antigen_info(const dnab_antigen* ag, size_t* n_atoms, size_t* n_cells_nonempty,
+                      float* cutoff, float* cell_size) {
+  if (!ag) return DNAB_ERR_INVALID;
+  if (n_atoms) *n_atoms = ag->grid.n;
+  if (n_cells_nonempty) *n_cells_nonempty = ag->grid.n_nonempty;
+  if (cutoff) *cutoff = ag->cutoff;
+  if (cell_size) *cell_size = ag->grid.cell;
+  return DNAB_OK;
+}
+
+void dnab_antigen_destroy(dnab_antigen* ag) {
+  if (!ag) return;
+  if (ag->ctx) (void)hipStreamSynchronize(ag->ctx->stream);
+  delete ag;
+}
+
+int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t n_antigens,
+                           const dnab_population_desc* d, dnab_population** out) {
+  if (!ctx || !out || !d || (n_antigens && !antigens) || n_antigens == 0)
+    return DNAB_ERR_INVALID;
+  *out = nullptr;
+  const size_t nc = d->n_cand, nr = d->n_res;
+  if (nc && (!d->cand_antigen || !d->cand_res_off)) return DNAB_ERR_INVALID;
+  if (nr && (!d->res_atom_off || !d->res_ca_atom)) return DNAB_ERR_INVALID;
+  if (!atom_cloud_ok(&d->atoms)) return DNAB_ERR_INVALID;
+  for (size_t a = 0; a < n_antigens; ++a)
+    if (!antigens[a] || antigens[a]->ctx != ctx)
+      return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: antigen %zu invalid", a);
+  const size_t na = d->atoms.n;
+  if (na >= 0xFFFFFF00ull || nc >= 0xFFFFFF00ull || nr >= 0xFFFFFF00ull)
+    return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "population: too large");
+  // validate ranges
+  if (nc) {
+    if (d->cand_res_off[0] != 0 || d->cand_res_off[nc] != nr)
+      return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: cand_res_off must span 0..n_res");
+    for (size_t c = 0; c < nc; ++c) {
+      if (d->cand_res_off[c + 1] < d->cand_res_off[c])
+        return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: cand_res_off not monotone");
+      if (d->cand_antigen[c] >= n_antigens)
+        return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: cand_antigen out of range");
+    }
+  } else if (nr) {
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: residues without candidates");
+  }
+  if (nr) {
+    if (d->res_atom_off[0] != 0 || d->res_atom_off[nr] != na)
+      return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: res_atom_off must span 0..n_atoms");
+    for (size_t r = 0; r < nr; ++r) {
+      if (d->res_atom_off[r + 1] < d->res_atom_off[r])
+        return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: res_atom_off not monotone");
+      if (d->res_ca_atom[r] < d->res_atom_off[r] || d->res_ca_atom[r] >= d->res_atom_off[r + 1])
+        return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: residue %zu Calpha outside it", r);
+    }
+  } else if (na) {
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: atoms without residues");
+  }
+  for (size_t i = 0; i < na; ++i)
+    if (!std::isfinite(d->atoms.x[i]) || !std::isfinite(d->atoms.y[i]) ||
+        !std::isfinite(d->atoms.z[i]))
+      return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: non-finite coordinate");
+  (void)hipSetDevice(ctx->device);
+  dnab_population* pop = new (std::nothrow) dnab_population();
+  if (!pop) return DNAB_ERR_ALLOC;
+  pop->ctx = ctx;
+  pop->n_cand = nc;
+  pop->n_res = nr;
+  pop->n_atoms = na;
+  pop->n_ag = n_antigens;
+  pop->antigens.assign(antigens, antigens + n_antigens);
+  auto fail = [&](int rc) {
+    (void)hipStreamSynchronize(ctx->stream);
+    delete pop;
+    return rc;
+  };
+  // host staging
+  std::vector<float4> hpos(na), hpar(na);
+  std::vector<uint32_t> hatom_ag(na), hatom_cand(na), hcand_atom_off(nc + 1), hres_cand(nr);
+  std::vector<float4> hres_cg(nr);
+  for (size_t i = 0; i < na; ++i) {
+    hpos[i] = make_float4(d->atoms.x[i], d->atoms.y[i], d->atoms.z[i], d->atoms.charge[i]);
+    hpar[i] = make_float4(d->atoms.r_min_half[i], d->atoms.epsilon[i],
+                          d->atoms.hydrophobic[i] ? 1.0f : 0.0f, 0.0f);
+  }
+  for (size_t c = 0; c < nc; ++c) {
+    uint32_t r0 = d->cand_res_off[c], r1 = d->cand_res_off[c + 1];
+    hcand_atom_off[c] = nr ? d->res_atom_off[r0] : 0;
+    for (uint32_t r = r0; r < r1; ++r) {
+      hres_cand[r] = static_cast<uint32_t>(c);
+      for (uint32_t k = d->res_atom_off[r]; k < d->res_atom_off[r + 1]; ++k) {
+        hatom_ag[k] = d->cand_antigen[c];
+        hatom_cand[k] = static_cast<uint32_t>(c);
+      }
+    }
+  }
+  if (nc) hcand_atom_off[nc] = static_cast<uint32_t>(na);
+  else hcand_atom_off[0] = 0;
+  for (size_t r = 0; r < nr; ++r) {
+    float q = 0, e = 0, s = 0;
+    uint8_t h = 0;
+    if (d->res_amino_acid) {
+      if (dnab_residue_cg_params(d->res_amino_acid[r], &q, &e, &s, &h) != DNAB_OK)
+        return fail(dnab_set_error(ctx, DNAB_ERR_INVALID, "population: bad residue type"));
+    }
+    hres_cg[r] = make_float4(q, e, s, static_cast<float>(h));
+  }
+  // per-antigen views and the candidate sort grids
+  pop->views_host.resize(n_antigens);
+  uint64_t code_base = 0;
+  for (size_t a = 0; a < n_antigens; ++a) {
+    const dnab_antigen* ag = antigens[a];
+    AgView& v = pop->views_host[a];
+    std::memset(&v, 0, sizeof(v));
+    v.atoms = ag->atoms.p;
+    v.raw = ag->raw.p;
+    v.cell_start = ag->grid.cell_start.p;
+    for (int k = 0; k < 3; ++k) {
+      v.kmin[k] = ag->grid.kmin[k];
+      v.kmax[k] = ag->grid.kmax[k];
+      v.center[k] = ag->center[k];
+    }
+    v.bits = ag->grid.bits;
+    v.inv = ag->grid.inv;
+    v.rc = ag->cutoff;
+    v.rc2 = ag->cutoff * ag->cutoff;
+    v.restraint_radius = ag->restraint_radius;
+    v.n_atoms = static_cast<uint32_t>(ag->grid.n);
+    float margin = ag->cutoff + 16.0f;
+    float cell = kSortCell;
+    int dim_max = 1;
+    for (;;) {
+      dim_max = 1;
+      for (int k = 0; k < 3; ++k) {
+        float ext = (ag->grid.n ? ag->bbox_hi[k] - ag->bbox_lo[k] : 0.0f) + 2.0f * margin;
+        int dim = static_cast<int>(std::ceil(ext / cell)) + 1;
+        v.sort_dim[k] = dim;
+        dim_max = std::max(dim_max, dim);
+      }
+      if (dim_max <= (1 << kSortMaxBits)) break;
+      cell *= 2.0f;
+    }
+    int bits = 0;
+    while ((1 << bits) < dim_max) ++bits;
+    for (int k = 0; k < 3; ++k)
+      v.sort_origin[k] = (ag->grid.n ? ag->bbox_lo[k] : 0.0f) - margin;
+    v.sort_inv = 1.0f / cell;
+    v.sort_bits = bits;
+    v.sort_code_base = static_cast<uint32_t>(code_base);
+    code_base += uint64_t(1) << (3 * bits);
+    v.ca_x = ag->ca_x.p;
+    v.ca_y = ag->ca_y.p;
+    v.ca_z = ag->ca_z.p;
+    v.ca_q = ag->ca_q.p;
+    v.ca_eps = ag->ca_eps.p;
+    v.ca_sig = ag->ca_sig.p;
+    v.ca_hyd = ag->ca_hyd.p;
+    v.ca_grid = make_view(ag->ca_grid);
+  }
+  if (code_base >= 0xFFFFFF00ull)
+    return fail(dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "population: sort grid too large"));
+  pop->sort_codes = static_cast<uint32_t>(code_base);
+  pop->tile_cap = (na + kWave - 1) / kWave + n_antigens;
+  int rc = DNAB_OK;
+  if ((rc = up(ctx, pop->views, pop->views_host.data(), n_antigens)) ||
+      (rc = up(ctx, pop->pos, hpos.data(), na)) || (rc = up(ctx, pop->par, hpar.data(), na)) ||
+      (rc = up(ctx, pop->atom_ag, hatom_ag.data(), na)) ||
+      (rc = up(ctx, pop->atom_cand, hatom_cand.data(), na)) ||
+      (rc = up(ctx, pop->cand_atom_off, hcand_atom_off.data(), nc + 1)) ||
+      (rc = up(ctx, pop->res_cand, hres_cand.data(), nr)) ||
+      (rc = up(ctx, pop->res_cg, hres_cg.data(), nr)))
+    return fail(rc);
+  if (nc && ((rc = up(ctx, pop->cand_res_off, d->cand_res_off, nc + 1)) ||
+             (rc = up(ctx, pop->cand_ag, d->cand_antigen, nc))))
+    return fail(rc);
+  if (nr && ((rc = up(ctx, pop->res_atom_off, d->res_atom_off, nr + 1)) ||
+             (rc = up(ctx, pop->res_ca, d->res_ca_atom, nr))))
+    return fail(rc);
+  if (pop->sort_count.alloc(static_cast<size_t>(pop->sort_codes) + 1) != hipSuccess ||
+      pop->sort_rank.alloc(na) != hipSuccess || pop->perm.alloc(pop->tile_cap * kWave) != hipSuccess ||
+      pop->ag_tile_start.alloc(n_antigens + 1) != hipSuccess || pop->force.alloc(na) != hipSuccess ||
+      pop->e_atom.alloc(na) != hipSuccess || pop->bd_atom.alloc(3 * na) != hipSuccess ||
+      pop->f64_atom.alloc(3 * na) != hipSuccess || pop->energy.alloc(nc) != hipSuccess ||
+      pop->err_bound.alloc(nc) != hipSuccess || pop->breakdown.alloc(3 * nc) != hipSuccess ||
+      pop->res_force.alloc(nr) != hipSuccess || pop->step_counter.alloc(4) != hipSuccess ||
+      pop->pair_stats.alloc(2) != hipSuccess)
+    return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
+  if ((rc = population_sort(ctx, pop))) return fail(rc);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "population: synchronize failed"));
+  *out = pop;
+  return DNAB_OK;
+}
+
+void dnab_population_destroy(dnab_population* pop) {
+  if (!pop) return;
+  if (pop->ctx) (void)hipStreamSynchronize(pop->ctx->stream);
+  delete pop;
+}
+
+int dnab_population_info(const dnab_population* pop, size_t* n_cand, size_t* n_res,
+                         size_t* n_atoms) {
+  if (!pop) return DNAB_ERR_INVALID;
+  if (n_cand) *n_cand = pop->n_cand;
+  if (n_res) *n_res = pop->n_res;
+  if (n_atoms) *n_atoms = pop->n_atoms;
+  return DNAB_OK;
+}
+
+int dnab_population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
+  if (!ctx || !pop || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  if (flags & ~(DNAB_EVAL_FORCES | DNAB_EVAL_BREAKDOWN | DNAB_EVAL_FP64)) return DNAB_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  return population_evaluate(ctx, pop, flags);
+}
+
+int dnab_population_read_energies(dnab_ctx* ctx, const dnab_population* pop, double* energy,
+                                  float* err_bound) {
+  if (!ctx || !pop || !energy) return DNAB_ERR_INVALID;
+  if (!pop->evaluated) return dnab_set_error(ctx, DNAB_ERR_INVALID, "population not evaluated");
+  DNAB_RETURN_IF(download(ctx, energy, pop->energy.p, pop->n_cand * sizeof(double)));
+  if (err_bound) DNAB_RETURN_IF(download(ctx, err_bound, pop->err_bound.p, pop->n_cand * sizeof(float)));
+  return DNAB_OK;
+}
+
+int dnab_population_read_breakdown(dnab_ctx* ctx, const dnab_population* pop, double* lj,
+                                   double* coulomb, double* hydrophobic) {
+  if (!ctx || !pop) return DNAB_ERR_INVALID;
+  if (!pop->evaluated || !(pop->last_flags & (DNAB_EVAL_BREAKDOWN | DNAB_EVAL_FP64)))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "no breakdown evaluation");
+  std::vector<double> h(3 * pop->n_cand);
+  DNAB_RETURN_IF(download(ctx, h.data(), pop->breakdown.p, h.size() * sizeof(double)));
+  for (size_t c = 0; c < pop->n_cand; ++c) {
+    if (lj) lj[c] = h[3 * c];
+    if (coulomb) coulomb[c] = h[3 * c + 1];
+    if (hydrophobic) hydrophobic[c] = h[3 * c + 2];
+  }
+  return DNAB_OK;
+}
+
+int dnab_population_read_forces(dnab_ctx* ctx, const dnab_population* pop, float* fx,
+                                float* fy, float* fz) {
+  if (!ctx || !pop || (pop->n_atoms && (!fx || !fy || !fz))) return DNAB_ERR_INVALID;
+  if (!pop->evaluated) return dnab_set_error(ctx, DNAB_ERR_INVALID, "population not evaluated");
+  const size_t n = pop->n_atoms;
+  if (pop->last_flags & DNAB_EVAL_FP64) {
+    std::vector<double> h(3 * n);
+    DNAB_RETURN_IF(download(ctx, h.data(), pop->f64_atom.p, h.size() * sizeof(double)));
+    for (size_t i = 0; i < n; ++i) {
+      fx[i] = static_cast<float>(h[3 * i]);
+      fy[i] = static_cast<float>(h[3 * i + 1]);
+      fz[i] = static_cast<float>(h[3 * i + 2]);
+    }
+    return DNAB_OK;
+  }
+  if (!(pop->last_flags & DNAB_EVAL_FORCES))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "last evaluation had no forces");
+  std::vector<float4> h(n);
+  DNAB_RETURN_IF(download(ctx, h.data(), pop->force.p, n * sizeof(float4)));
+  for (size_t i = 0; i < n; ++i) {
+    fx[i] = h[i].x;
+    fy[i] = h[i].y;
+    fz[i] = h[i].z;
+  }
+  return DNAB_OK;
+}
+
+int dnab_population_read_forces_f64(dnab_ctx* ctx, const dnab_population* pop, double* fx,
+                                    double* fy, double* fz) {
+  if (!ctx || !pop || (pop->n_atoms && (!fx || !fy || !fz))) return DNAB_ERR_INVALID;
+  if (!pop->evaluated || !(pop->last_flags & DNAB_EVAL_FP64))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "no FP64 evaluation");
+  const size_t n = pop->n_atoms;
+  std::vector<double> h(3 * n);
+  DNAB_RETURN_IF(download(ctx, h.data(), pop->f64_atom.p, h.size() * sizeof(double)));
+  for (size_t i = 0; i < n; ++i) {
+    fx[i] = h[3 * i];
+    fy[i] = h[3 * i + 1];
+    fz[i] = h[3 * i + 2];
+  }
+  return DNAB_OK;
+}
+
+int dnab_population_read_coords(dnab_ctx* ctx, const dnab_population* pop, float* x, float* y,
+                                float* z) {
+  if (!ctx || !pop || (pop->n_atoms && (!x || !y || !z))) return DNAB_ERR_INVALID;
+  const size_t n = pop->n_atoms;
+  std::vector<float4> h(n);
+  DNAB_RETURN_IF(download(ctx, h.data(), pop->pos.p, n * sizeof(float4)));
+  for (size_t i = 0; i < n; ++i) {
+    x[i] = h[i].x;
+    y[i] = h[i].y;
+    z[i] = h[i].z;
+  }
+  return DNAB_OK;
+}
+
+int dnab_population_write_coords(dnab_ctx* ctx, dnab_population* pop, const float* x,
+                                 const float* y, const float* z) {
+  if (!ctx || !pop || (pop->n_atoms && (!x || !y || !z))) return DNAB_ERR_INVALID;
+  const size_t n = pop->n_atoms;
+  std::vector<float4> h(n);
+  DNAB_RETURN_IF(download(ctx, h.data(), pop->pos.p, n * sizeof(float4)));
+  for (size_t i = 0; i < n; ++i) {
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]))
+      return dnab_set_error(ctx, DNAB_ERR_INVALID, "write_coords: non-finite coordinate");
+    h[i].x = x[i];
+    h[i].y = y[i];
+    h[i].z = z[i];
+  }
+  DNAB_RETURN_IF(upload(ctx, pop->pos.p, h.data(), n * sizeof(float4)));
+  pop->evaluated = false;
+  DNAB_RETURN_IF(population_sort(ctx, pop));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNAB_OK;
+}
+
+int dnab_population_read_pair_stats(dnab_ctx* ctx, const dnab_population* pop,
+                                    uint64_t* in_cutoff, uint64_t* tested) {
+  if (!ctx || !pop) return DNAB_ERR_INVALID;
+  unsigned long long h[2];
+  DNAB_RETURN_IF(download(ctx, h, pop->pair_stats.p, sizeof(h)));
+  if (in_cutoff) *in_cutoff = h[0];
+  if (tested) *tested = h[1];
+  return DNAB_OK;
+}
+
+int dnab_population_reset_pair_stats(dnab_ctx* ctx, dnab_population* pop) {
+  if (!ctx || !pop) return DNAB_ERR_INVALID;
+  unsigned long long zero[2] = {0, 0};
+  DNAB_RETURN_IF(upload(ctx, pop->pair_stats.p, zero, sizeof(zero)));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNAB_OK;
+}
+
+int dnab_population_pair_count(dnab_ctx* ctx, dnab_population* pop, uint64_t* in_cutoff,
+                               uint64_t* tested) {
+  if (!ctx || !pop || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  return pair_count(ctx, pop, in_cutoff, tested);
+}
+
+}  // extern "C"

@@ -1,0 +1,398 @@
+// Gradient (Langevin) step of the population loop and device-side top-k.
+//
+// The per-residue update restates src/diffusion.rs:300-339 (forces, the
+// radial restraint, clamp, uniform noise, set_ca_pos as a rigid translation,
+// src/allatom.rs:279-293) in the reference's operation order. The loop
+// restates the GPU phase of run_allatom (src/diffusion.rs:1018-1057) with all
+// candidates kept moving and scoring every `score_every` steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_common.h"
+#include "dnab_internal.h"
+#include "grid_device.h"
+
+namespace dnab {
+
+constexpr int kTopkChunk = 4096;
+constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
+
+struct StepParams {
+  float step, max_disp, restraint_k, restraint_radius;
+  int with_noise, force_model;
+  uint32_t seed_lo, seed_hi;
+};
+
+__device__ inline float clamp_rust(float v, float lo, float hi) {
+  // f32::clamp: NaN passes through.
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
+  return v;
+}
+
+__global__ void __launch_bounds__(256)
+k_langevin_step(const AgView* __restrict__ views, const uint32_t* __restrict__ res_cand,
+                const uint32_t* __restrict__ cand_ag, const uint32_t* __restrict__ res_atom_off,
+                const uint32_t* __restrict__ res_ca, const float4* __restrict__ res_cg,
+                const float4* __restrict__ force, float4* __restrict__ pos, uint32_t n_res,
+                StepParams sp, const float* __restrict__ sigma_table,
+                const uint32_t* __restrict__ counter) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_res) return;
+  const uint32_t iter = counter[0];
+  const uint32_t c = res_cand[r];
+  const AgView& V = views[cand_ag[c]];
+  const uint32_t ca = res_ca[r];
+  const float4 p = pos[ca];
+  const uint32_t a0 = res_atom_off[r], a1 = res_atom_off[r + 1];
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (sp.force_model == DNAB_FORCE_CG) {
+    const float4 g = res_cg[r];
+    CgCloud ag{V.ca_x, V.ca_y, V.ca_z, V.ca_q, V.ca_eps, V.ca_sig, V.ca_hyd};
+    cg_force_point(V.ca_grid, ag, p.x, p.y, p.z, g.x, g.y, g.z, g.w != 0.0f ? 1 : 0, fx, fy, fz);
+  } else {
+    for (uint32_t k = a0; k < a1; ++k) {
+      const float4 f = force[k];
+      fx += f.x;
+      fy += f.y;
+      fz += f.z;
+    }
+  }
+  // src/diffusion.rs:318-327
+  const float dx_c = p.x - V.center[0];
+  const float dy_c = p.y - V.center[1];
+  const float dz_c = p.z - V.center[2];
+  const float d = fmaxf(sqrtf(dx_c * dx_c + dy_c * dy_c + dz_c * dz_c), 0.001f);
+  const float r0 = V.restraint_radius > 0.0f ? V.restraint_radius : sp.restraint_radius;
+  const float excess = d - r0;
+  const float f_r = -sp.restraint_k * excess / d;
+  fx += f_r * dx_c;
+  fy += f_r * dy_c;
+  fz += f_r * dz_c;
+  // src/diffusion.rs:329-337
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  if (sp.with_noise) {
+    const float sigma = sigma_table[iter];
+    const Philox4 u = philox4x32_10(iter, r, 0x6e6f6973u, 0u, sp.seed_lo, sp.seed_hi);
+    const float two = sigma + sigma;
+    nx = u01_from_u32(u.v[0]) * two - sigma;
+    ny = u01_from_u32(u.v[1]) * two - sigma;
+    nz = u01_from_u32(u.v[2]) * two - sigma;
+  }
+  const float new_x = p.x + clamp_rust(sp.step * fx, -sp.max_disp, sp.max_disp) + nx;
+  const float new_y = p.y + clamp_rust(sp.step * fy, -sp.max_disp, sp.max_disp) + ny;
+  const float new_z = p.z + clamp_rust(sp.step * fz, -sp.max_disp, sp.max_disp) + nz;
+  // set_ca_pos: delta from the current Calpha, added to every atom.
+  const float ddx = new_x - p.x, ddy = new_y - p.y, ddz = new_z - p.z;
+  for (uint32_t k = a0; k < a1; ++k) {
+    float4 q = pos[k];
+    q.x += ddx;
+    q.y += ddy;
+    q.z += ddz;
+    pos[k] = q;
+  }
+}
+
+__global__ void k_advance(uint32_t* counter) {
+  if (threadIdx.x == 0) counter[0] += 1u;
+}
+
+__global__ void k_set_counter(uint32_t* counter, uint32_t v) {
+  if (threadIdx.x == 0) counter[0] = v;
+}
+
+// Ordering used for ranking: ascending energy, NaN last, ties by index.
+__device__ inline bool rank_less(double ea, uint32_t ia, double eb, uint32_t ib) {
+  const bool na = ea != ea, nb = eb != eb;
+  if (na != nb) return nb;
+  if (!na && ea != eb) return ea < eb;
+  return ia < ib;
+}
+
+// Each block sorts one chunk of up to kTopkChunk (energy, index) entries with
+// a bitonic network in LDS and writes its first k to out[block * k ...].
+__global__ void __launch_bounds__(1024)
+k_topk_chunk(const double* __restrict__ in_e, const uint32_t* __restrict__ in_idx, uint32_t n,
+             uint32_t k, double* __restrict__ out_e, uint32_t* __restrict__ out_idx) {
+  __shared__ double se[kTopkChunk];
+  __shared__ uint32_t si[kTopkChunk];
+  const uint32_t base = blockIdx.x * kTopkChunk;
+  const uint32_t len = min(static_cast<uint32_t>(kTopkChunk), n - base);
+  uint32_t m = 1;
+  while (m < len) m <<= 1;
+  for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+    if (t < len) {
+      se[t] = in_e[base + t];
+      si[t] = in_idx ? in_idx[base + t] : base + t;
+    } else {
+      se[t] = __builtin_huge_val();
+      si[t] = kNoIdx;
+    }
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= m; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = threadIdx.x; t < m / 2; t += blockDim.x) {
+        const uint32_t i = 2 * t - (t & (stride - 1));
+        const uint32_t j = i + stride;
+        const bool up = (i & size) == 0;
+        const double ea = se[i], eb = se[j];
+        const uint32_t ia = si[i], ib = si[j];
+        const bool swap = up ? rank_less(eb, ib, ea, ia) : rank_less(ea, ia, eb, ib);
+        if (swap) {
+          se[i] = eb;
+          se[j] = ea;
+          si[i] = ib;
+          si[j] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const uint32_t kk = min(k, len);
+  for (uint32_t t = threadIdx.x; t < kk; t += blockDim.x) {
+    out_e[blockIdx.x * k + t] = se[t];
+    out_idx[blockIdx.x * k + t] = si[t];
+  }
+  // chunks shorter than k pad their slots so later passes ignore them
+  for (uint32_t t = kk + threadIdx.x; t < k; t += blockDim.x) {
+    out_e[blockIdx.x * k + t] = __builtin_huge_val();
+    out_idx[blockIdx.x * k + t] = kNoIdx;
+  }
+}
+
+static inline uint32_t nblk(uint64_t n, uint32_t t) {
+  uint64_t b = (n + t - 1) / t;
+  return static_cast<uint32_t>(b ? b : 1);
+}
+
+struct TopkPlan {
+  uint32_t k = 0;
+  std::vector<uint32_t> lens;  // input length of each pass
+  DevBuf<double> e[2];
+  DevBuf<uint32_t> idx[2];
+};
+
+// Multi-pass top-k: chunks of 4096 reduce n to ceil(n/4096)*k entries per pass.
+static int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k) {
+  pl.k = k;
+  pl.lens.clear();
+  uint32_t len = n;
+  size_t cap = 1;
+  for (;;) {
+    pl.lens.push_back(len);
+    uint32_t chunks = nblk(len, kTopkChunk);
+    size_t out = static_cast<size_t>(chunks) * k;
+    cap = std::max(cap, out);
+    if (chunks == 1) break;
+    if (out >= len) return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "top_k too large for chunking");
+    len = static_cast<uint32_t>(out);
+  }
+  DNAB_ALLOC_CHECK(ctx, pl.e[0].alloc(cap));
+  DNAB_ALLOC_CHECK(ctx, pl.e[1].alloc(cap));
+  DNAB_ALLOC_CHECK(ctx, pl.idx[0].alloc(cap));
+  DNAB_ALLOC_CHECK(ctx, pl.idx[1].alloc(cap));
+  return DNAB_OK;
+}
+
+static void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
+                        uint32_t* final_idx) {
+  const double* in_e = energy;
+  const uint32_t* in_i = nullptr;
+  for (size_t p = 0; p < pl.lens.size(); ++p) {
+    const uint32_t len = pl.lens[p];
+    const bool last = p + 1 == pl.lens.size();
+    double* oe = last ? final_e : pl.e[p & 1].p;
+    uint32_t* oi = last ? final_idx : pl.idx[p & 1].p;
+    hipLaunchKernelGGL(k_topk_chunk, dim3(nblk(len, kTopkChunk)), dim3(1024), 0, ctx->stream, in_e,
+                       in_i, len, pl.k, oe, oi);
+    in_e = oe;
+    in_i = oi;
+  }
+}
+
+static StepParams make_step_params(const dnab_langevin_params* p) {
+  StepParams sp;
+  sp.step = p->step;
+  sp.max_disp = p->max_disp;
+  sp.restraint_k = p->restraint_k;
+  sp.restraint_radius = p->restraint_radius;
+  sp.with_noise = p->with_noise ? 1 : 0;
+  sp.force_model = p->force_model;
+  sp.seed_lo = static_cast<uint32_t>(p->seed);
+  sp.seed_hi = static_cast<uint32_t>(p->seed >> 32);
+  return sp;
+}
+
+static int validate_params(dnab_ctx* ctx, const dnab_langevin_params* p) {
+  if (p->force_model != DNAB_FORCE_ALLATOM && p->force_model != DNAB_FORCE_CG)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: unknown force model");
+  if (p->resort_every == 0) return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: resort_every must be >= 1");
+  if (!(p->schedule_total > 0)) return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: schedule_total must be > 0");
+  if (!std::isfinite(p->step) || !std::isfinite(p->max_disp) || !std::isfinite(p->restraint_k) ||
+      !std::isfinite(p->restraint_radius) || !(p->t_hot > 0) || !(p->t_cold > 0))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: non-finite parameter");
+  return DNAB_OK;
+}
+
+// sigma(iter) = noise_base * sqrt(T(iter)), T = t_hot (t_cold/t_hot)^(i/total)
+// in f32 as src/diffusion.rs:102-104 and 315, computed on the host.
+static int upload_sigma(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
+                        uint32_t n_iters) {
+  std::vector<float> s(n_iters + 1);
+  for (uint32_t i = 0; i <= n_iters; ++i) {
+    float it = static_cast<float>(i + p->schedule_offset);
+    float temp = p->t_hot * std::pow(p->t_cold / p->t_hot, it / static_cast<float>(p->schedule_total));
+    s[i] = p->noise_base * std::sqrt(temp);
+  }
+  DNAB_ALLOC_CHECK(ctx, pop->sigma_table.alloc(s.size()));
+  return upload(ctx, pop->sigma_table.p, s.data(), s.size() * sizeof(float));
+}
+
+static int launch_step(dnab_ctx* ctx, dnab_population* pop, const StepParams& sp) {
+  if (pop->n_res) {
+    hipLaunchKernelGGL(k_langevin_step, dim3(nblk(pop->n_res, 256)), dim3(256), 0, ctx->stream,
+                       pop->views.p, pop->res_cand.p, pop->cand_ag.p, pop->res_atom_off.p,
+                       pop->res_ca.p, pop->res_cg.p, pop->force.p, pop->pos.p,
+                       static_cast<uint32_t>(pop->n_res), sp, pop->sigma_table.p,
+                       pop->step_counter.p);
+  }
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p);
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  return DNAB_OK;
+}
+
+// One chunk of the loop: L steps starting at a multiple of L, scoring first.
+static int enqueue_chunk(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
+                         const StepParams& sp, uint32_t L, TopkPlan* tk, bool score) {
+  for (uint32_t l = 0; l < L; ++l) {
+    if (l % p->resort_every == 0) DNAB_RETURN_IF(population_sort(ctx, pop));
+    // The residue-level model needs atom energies only when scoring.
+    if (p->force_model == DNAB_FORCE_ALLATOM || (score && l == 0)) {
+      const unsigned flags = p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES;
+      DNAB_RETURN_IF(population_evaluate(ctx, pop, flags));
+    }
+    if (score && l == 0 && tk) topk_launch(ctx, *tk, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
+    DNAB_RETURN_IF(launch_step(ctx, pop, sp));
+  }
+  return DNAB_OK;
+}
+
+}  // namespace dnab
+
+using namespace dnab;
+
+extern "C" {
+
+int dnab_langevin_default_params(dnab_langevin_params* p) {
+  if (!p) return DNAB_ERR_INVALID;
+  std::memset(p, 0, sizeof(*p));
+  p->n_steps = 200;      // GPU_STEPS, src/diffusion.rs:36
+  p->score_every = 10;
+  p->top_k = 64;         // TOP_K, src/diffusion.rs:40
+  p->resort_every = 10;
+  p->seed = 0;
+  p->with_noise = 0;     // GPU phase, src/diffusion.rs:1023
+  p->force_model = DNAB_FORCE_ALLATOM;
+  p->use_graph = 0;
+  p->step = 0.08f;       // src/diffusion.rs:66
+  p->max_disp = 2.0f;    // src/diffusion.rs:82
+  p->noise_base = 0.25f; // src/diffusion.rs:67
+  p->t_hot = 3.0f;       // src/diffusion.rs:68
+  p->t_cold = 0.02f;     // src/diffusion.rs:69
+  p->schedule_offset = 0;
+  p->schedule_total = 200 + 600;  // GPU_STEPS + CPU_STEPS, src/diffusion.rs:1019
+  p->restraint_k = 0.02f;         // src/diffusion.rs:83
+  p->restraint_radius = 20.0f;    // src/diffusion.rs:81
+  return DNAB_OK;
+}
+
+int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
+                       uint32_t iter) {
+  if (!ctx || !pop || !p || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(validate_params(ctx, p));
+  (void)hipSetDevice(ctx->device);
+  DNAB_RETURN_IF(upload_sigma(ctx, pop, p, iter));
+  hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p, iter);
+  DNAB_RETURN_IF(population_sort(ctx, pop));
+  if (p->force_model == DNAB_FORCE_ALLATOM)
+    DNAB_RETURN_IF(population_evaluate(ctx, pop, DNAB_EVAL_FORCES));
+  DNAB_RETURN_IF(launch_step(ctx, pop, make_step_params(p)));
+  pop->evaluated = pop->evaluated && p->force_model == DNAB_FORCE_ALLATOM;
+  DNAB_RETURN_IF(population_sort(ctx, pop));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNAB_OK;
+}
+
+int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
+                      uint32_t* topk_idx, double* topk_energy, uint32_t* n_topk) {
+  if (!ctx || !pop || !p || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(validate_params(ctx, p));
+  if (p->top_k == 0 || p->top_k > 4096)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: top_k must be 1..4096");
+  (void)hipSetDevice(ctx->device);
+  const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
+  const uint32_t k = std::min<uint32_t>(p->top_k, nc);
+  if (n_topk) *n_topk = k;
+  if (nc == 0) return DNAB_OK;
+  DNAB_RETURN_IF(upload_sigma(ctx, pop, p, p->n_steps));
+  TopkPlan tk;
+  DNAB_RETURN_IF(topk_plan(ctx, tk, nc, k));
+  DNAB_ALLOC_CHECK(ctx, pop->topk_e.ensure(k));
+  DNAB_ALLOC_CHECK(ctx, pop->topk_idx.ensure(k));
+  hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p, 0u);
+  const StepParams sp = make_step_params(p);
+  const uint32_t L = p->score_every ? p->score_every : std::max<uint32_t>(p->n_steps, 1);
+  const bool graph = p->use_graph && p->n_steps >= L && L % p->resort_every == 0;
+  const bool saved_timing = ctx->timing;
+  uint32_t done = 0;
+  if (graph) {
+    ctx->timing = false;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    DNAB_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_chunk(ctx, pop, p, sp, L, &tk, p->score_every != 0);
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+    ctx->timing = saved_timing;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ce != hipSuccess) return dnab_set_error(ctx, DNAB_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return dnab_set_error(ctx, DNAB_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+    }
+    for (; done + L <= p->n_steps; done += L) {
+      hipError_t le = hipGraphLaunch(ge, ctx->stream);
+      if (le != hipSuccess) {
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        return dnab_set_error(ctx, DNAB_ERR_HIP, "graph launch failed: %s", hipGetErrorString(le));
+      }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+  }
+  for (; done < p->n_steps; done += L) {
+    const uint32_t len = std::min(L, p->n_steps - done);
+    DNAB_RETURN_IF(enqueue_chunk(ctx, pop, p, sp, len, &tk, p->score_every != 0));
+  }
+  // final scoring after the last update
+  DNAB_RETURN_IF(population_sort(ctx, pop));
+  DNAB_RETURN_IF(population_evaluate(ctx, pop, p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES));
+  topk_launch(ctx, tk, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  if (topk_idx) DNAB_RETURN_IF(download(ctx, topk_idx, pop->topk_idx.p, k * sizeof(uint32_t)));
+  if (topk_energy) DNAB_RETURN_IF(download(ctx, topk_energy, pop->topk_e.p, k * sizeof(double)));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNAB_OK;
+}
+
+}  // extern "C"

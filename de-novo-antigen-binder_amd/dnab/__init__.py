@@ -1,0 +1,20 @@
+"""dnab: MI355X-native batched all-atom binder scoring (Python handles).
+
+The product is libdnab.so (HIP kernels for gfx950 behind the C ABI of
+include/dnab.h). This package binds it with ctypes for tests, benchmarks and
+scripting; it raises on import when the library is not built.
+"""
+from . import _native
+from ._native import DnabError, device_count, visible_gpu
+from .api import (AtomEnergyBreakdown, DeviceAntigen, DevicePopulation, GpuContext,
+                  SpatialHashGrid, langevin_params)
+from .structure import (AMINO_ONE_LETTER, AtomCloud, AtomProtein, PopulationHost, ResidueCloud,
+                        aa_from_sequence, parse_peptide, population_from_proteins,
+                        protein_from_ca_trace)
+
+__all__ = [
+    "AMINO_ONE_LETTER", "AtomCloud", "AtomEnergyBreakdown", "AtomProtein", "DeviceAntigen",
+    "DevicePopulation", "DnabError", "GpuContext", "PopulationHost", "ResidueCloud",
+    "SpatialHashGrid", "aa_from_sequence", "device_count", "langevin_params", "parse_peptide",
+    "population_from_proteins", "protein_from_ca_trace", "visible_gpu", "_native",
+]

@@ -1,0 +1,379 @@
+"""Python handles over the device-side C ABI.
+
+GpuContext mirrors the reference's GpuContext (src/gpu.rs:134-243):
+``GpuContext.try_init()`` returns None when no usable GPU exists
+(src/gpu.rs:143-159), and ``score_batch`` keeps the reference's signature
+(src/gpu.rs:166-171). The energy/force methods mirror src/energy.rs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .structure import AtomCloud, AtomProtein, PopulationHost, ResidueCloud
+
+vp = C.c_void_p
+
+
+@dataclass
+class AtomEnergyBreakdown:
+    """src/energy.rs:238-250 (hbond and solvation are not computed: 0)."""
+    lj: float
+    coulomb: float
+    hydrophobic: float
+    hbond: float = 0.0
+    solvation: float = 0.0
+
+    def total(self) -> float:
+        return self.lj + self.coulomb + self.hydrophobic + self.hbond + self.solvation
+
+
+class GpuContext:
+    def __init__(self, handle: int):
+        self._h = vp(handle)
+
+    @classmethod
+    def try_init(cls, device: int = 0) -> "GpuContext | None":
+        h = vp()
+        st = N.lib.dnab_ctx_create(device, C.byref(h))
+        if st == N.DNAB_ERR_NO_DEVICE:
+            return None
+        N.check(st)
+        return cls(h.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            N.lib.dnab_ctx_destroy(self._h)
+            self._h = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, status: int):
+        N.check(status, self._h)
+
+    def synchronize(self):
+        self.check(N.lib.dnab_ctx_synchronize(self._h))
+
+    def device_info(self):
+        cu, ws = C.c_int(), C.c_int()
+        arch = C.create_string_buffer(64)
+        self.check(N.lib.dnab_ctx_device_info(self._h, C.byref(cu), C.byref(ws), arch, 64))
+        return {"cu_count": cu.value, "wave_size": ws.value, "arch": arch.value.decode()}
+
+    def set_timing(self, enable: bool):
+        self.check(N.lib.dnab_ctx_set_timing(self._h, 1 if enable else 0))
+
+    def kernel_time(self, name: str) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_uint64()
+        self.check(N.lib.dnab_ctx_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # ---- src/gpu.rs:166-267 ------------------------------------------------
+    def score_batch(self, antigen: AtomCloud, candidates: np.ndarray, n_ab: int) -> np.ndarray:
+        """candidates: (n_atoms, 8) float32 GpuAtom rows, candidate-major."""
+        cand = np.ascontiguousarray(candidates, np.float32).reshape(-1, 8)
+        n_atoms = cand.shape[0]
+        n_cand = n_atoms // n_ab if n_ab else 0
+        out = np.zeros(max(n_cand, 1), np.float32)
+        got = C.c_size_t(0)
+        agc = antigen.c()
+        self.check(N.lib.dnab_score_batch(self._h, C.byref(agc),
+                                          cand.ctypes.data_as(C.POINTER(N.GpuAtom)), n_atoms,
+                                          n_ab, N.ptr(out, C.c_float), len(out), C.byref(got)))
+        return out[:got.value]
+
+    # ---- src/energy.rs ---------------------------------------------------------
+    def interaction_energy_atoms_breakdown(self, antigen: AtomCloud, antibody: AtomCloud,
+                                           cutoff: float = 10.0) -> AtomEnergyBreakdown:
+        b = N.EnergyBreakdownC()
+        a, c = antigen.c(), antibody.c()
+        self.check(N.lib.dnab_interaction_energy_atoms_breakdown(self._h, C.byref(a), C.byref(c),
+                                                                 cutoff, C.byref(b)))
+        return AtomEnergyBreakdown(b.lj, b.coulomb, b.hydrophobic, b.hbond, b.solvation)
+
+    def interaction_energy_atoms(self, antigen: AtomCloud, antibody: AtomCloud,
+                                 cutoff: float = 10.0) -> float:
+        e = C.c_double()
+        a, c = antigen.c(), antibody.c()
+        self.check(N.lib.dnab_interaction_energy_atoms(self._h, C.byref(a), C.byref(c), cutoff,
+                                                       C.byref(e)))
+        return e.value
+
+    def atom_forces(self, antigen: AtomCloud, antibody: AtomCloud, cutoff: float = 10.0):
+        n = len(antibody)
+        fx, fy, fz = (np.zeros(max(n, 1), np.float32) for _ in range(3))
+        e = C.c_double()
+        a, c = antigen.c(), antibody.c()
+        self.check(N.lib.dnab_atom_forces(self._h, C.byref(a), C.byref(c), cutoff,
+                                          N.ptr(fx, C.c_float), N.ptr(fy, C.c_float),
+                                          N.ptr(fz, C.c_float), C.byref(e)))
+        return fx[:n], fy[:n], fz[:n], e.value
+
+    def compute_forces_with_grid(self, antigen: ResidueCloud, grid: "SpatialHashGrid",
+                                 antibody: ResidueCloud):
+        n = len(antibody)
+        fx, fy, fz = (np.zeros(max(n, 1), np.float32) for _ in range(3))
+        a, b = antigen.c(), antibody.c()
+        self.check(N.lib.dnab_cg_forces_with_grid(self._h, C.byref(a), grid.handle, C.byref(b),
+                                                  N.ptr(fx, C.c_float), N.ptr(fy, C.c_float),
+                                                  N.ptr(fz, C.c_float)))
+        return fx[:n], fy[:n], fz[:n]
+
+    def compute_forces(self, antigen: ResidueCloud, antibody: ResidueCloud):
+        n = len(antibody)
+        fx, fy, fz = (np.zeros(max(n, 1), np.float32) for _ in range(3))
+        a, b = antigen.c(), antibody.c()
+        self.check(N.lib.dnab_cg_forces(self._h, C.byref(a), C.byref(b), N.ptr(fx, C.c_float),
+                                        N.ptr(fy, C.c_float), N.ptr(fz, C.c_float)))
+        return fx[:n], fy[:n], fz[:n]
+
+
+class SpatialHashGrid:
+    """src/spatial.rs:12-68 on the device."""
+
+    def __init__(self, ctx: GpuContext, cell_size: float):
+        self.ctx = ctx
+        h = vp()
+        ctx.check(N.lib.dnab_grid_create(ctx.handle, cell_size, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def build(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(v, np.float32) for v in (x, y, z))
+        self.ctx.check(N.lib.dnab_grid_build(self.ctx.handle, self._h, N.ptr(x, C.c_float),
+                                             N.ptr(y, C.c_float), N.ptr(z, C.c_float), len(x)))
+
+    def query_neighbors(self, px, py, pz):
+        """Batched query: returns (offsets, indices) in the reference's order."""
+        px, py, pz = (np.ascontiguousarray(np.atleast_1d(v), np.float32) for v in (px, py, pz))
+        n = len(px)
+        off = np.zeros(n + 1, np.uint32)
+        total = C.c_size_t(0)
+        st = N.lib.dnab_grid_query(self.ctx.handle, self._h, N.ptr(px, C.c_float),
+                                   N.ptr(py, C.c_float), N.ptr(pz, C.c_float), n,
+                                   N.ptr(off, C.c_uint32), N.ptr(None, C.c_uint32), 0,
+                                   C.byref(total))
+        if st not in (N.DNAB_OK, N.DNAB_ERR_CAPACITY):
+            self.ctx.check(st)
+        idx = np.zeros(max(total.value, 1), np.uint32)
+        self.ctx.check(N.lib.dnab_grid_query(self.ctx.handle, self._h, N.ptr(px, C.c_float),
+                                             N.ptr(py, C.c_float), N.ptr(pz, C.c_float), n,
+                                             N.ptr(off, C.c_uint32), N.ptr(idx, C.c_uint32),
+                                             len(idx), C.byref(total)))
+        return off, idx[:total.value]
+
+    def cell_of(self, x, y, z) -> np.ndarray:
+        x, y, z = (np.ascontiguousarray(np.atleast_1d(v), np.float32) for v in (x, y, z))
+        keys = np.zeros((max(len(x), 1), 3), np.int32)
+        self.ctx.check(N.lib.dnab_grid_cell_of(self.ctx.handle, self._h, N.ptr(x, C.c_float),
+                                               N.ptr(y, C.c_float), N.ptr(z, C.c_float), len(x),
+                                               N.ptr(keys, C.c_int32)))
+        return keys[:len(x)]
+
+    def info(self):
+        n, ne = C.c_size_t(), C.c_size_t()
+        kmin, kmax = (C.c_int32 * 3)(), (C.c_int32 * 3)()
+        self.ctx.check(N.lib.dnab_grid_info(self._h, C.byref(n), C.byref(ne), kmin, kmax))
+        return {"n_points": n.value, "n_cells_nonempty": ne.value, "key_min": list(kmin),
+                "key_max": list(kmax)}
+
+    def close(self):
+        if self._h:
+            N.lib.dnab_grid_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceAntigen:
+    """A device-resident antigen (Morton-ordered cell-major SoA in HBM)."""
+
+    def __init__(self, ctx: GpuContext, protein: AtomProtein | AtomCloud, cutoff: float = 10.0,
+                 cell_size: float | None = None, center=None, with_ca: bool = True,
+                 restraint_radius: float = 0.0):
+        self.ctx = ctx
+        if isinstance(protein, AtomProtein):
+            cloud = protein.atoms
+            ca = protein.ca_residue_cloud() if with_ca else None
+            ctr = protein.ca_center_of_mass() if center is None else center
+        else:
+            cloud, ca = protein, None
+            ctr = np.zeros(3, np.float32) if center is None else center
+        self.cloud, self.ca = cloud, ca
+        self.center = np.asarray(ctr, np.float32)
+        d = N.AntigenDescC()
+        d.atoms = cloud.c()
+        d.ca = ca.c() if ca is not None else N.ResidueCloudC()
+        for k in range(3):
+            d.center[k] = float(self.center[k])
+        d.cell_size = float(cutoff if cell_size is None else cell_size)
+        d.cutoff = float(cutoff)
+        d.restraint_radius = float(restraint_radius)
+        self.restraint_radius = float(restraint_radius)
+        self.cutoff = float(cutoff)
+        h = vp()
+        ctx.check(N.lib.dnab_antigen_create(ctx.handle, C.byref(d), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        n, ne = C.c_size_t(), C.c_size_t()
+        rc, cs = C.c_float(), C.c_float()
+        self.ctx.check(N.lib.dnab_antigen_info(self._h, C.byref(n), C.byref(ne), C.byref(rc),
+                                               C.byref(cs)))
+        return {"n_atoms": n.value, "n_cells_nonempty": ne.value, "cutoff": rc.value,
+                "cell_size": cs.value}
+
+    def close(self):
+        if self._h:
+            N.lib.dnab_antigen_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def langevin_params(**kw) -> N.LangevinParamsC:
+    p = N.LangevinParamsC()
+    N.check(N.lib.dnab_langevin_default_params(C.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"unknown langevin parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+class DevicePopulation:
+    """A ragged batch of candidates in HBM, each bound to one antigen."""
+
+    def __init__(self, ctx: GpuContext, antigens: list[DeviceAntigen], host: PopulationHost):
+        self.ctx = ctx
+        self.antigens = list(antigens)
+        self.host = host
+        arr = (vp * len(antigens))(*[a.handle.value for a in antigens])
+        desc = host.c()
+        h = vp()
+        ctx.check(N.lib.dnab_population_create(ctx.handle, arr, len(antigens), C.byref(desc),
+                                               C.byref(h)))
+        self._h = h
+        self.n_cand, self.n_res, self.n_atoms = host.n_cand, host.n_res, len(host.atoms)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def evaluate(self, forces: bool = True, breakdown: bool = False, fp64: bool = False):
+        flags = (N.DNAB_EVAL_FORCES if forces else 0) | (N.DNAB_EVAL_BREAKDOWN if breakdown else 0)
+        flags |= N.DNAB_EVAL_FP64 if fp64 else 0
+        self.ctx.check(N.lib.dnab_population_evaluate(self.ctx.handle, self._h, flags))
+
+    def energies(self):
+        e = np.zeros(max(self.n_cand, 1), np.float64)
+        err = np.zeros(max(self.n_cand, 1), np.float32)
+        self.ctx.check(N.lib.dnab_population_read_energies(self.ctx.handle, self._h,
+                                                           N.ptr(e, C.c_double),
+                                                           N.ptr(err, C.c_float)))
+        return e[:self.n_cand], err[:self.n_cand]
+
+    def breakdown(self):
+        out = [np.zeros(max(self.n_cand, 1), np.float64) for _ in range(3)]
+        self.ctx.check(N.lib.dnab_population_read_breakdown(
+            self.ctx.handle, self._h, *[N.ptr(o, C.c_double) for o in out]))
+        return [o[:self.n_cand] for o in out]
+
+    def forces(self):
+        out = [np.zeros(max(self.n_atoms, 1), np.float32) for _ in range(3)]
+        self.ctx.check(N.lib.dnab_population_read_forces(
+            self.ctx.handle, self._h, *[N.ptr(o, C.c_float) for o in out]))
+        return [o[:self.n_atoms] for o in out]
+
+    def forces_f64(self):
+        out = [np.zeros(max(self.n_atoms, 1), np.float64) for _ in range(3)]
+        self.ctx.check(N.lib.dnab_population_read_forces_f64(
+            self.ctx.handle, self._h, *[N.ptr(o, C.c_double) for o in out]))
+        return [o[:self.n_atoms] for o in out]
+
+    def coords(self):
+        out = [np.zeros(max(self.n_atoms, 1), np.float32) for _ in range(3)]
+        self.ctx.check(N.lib.dnab_population_read_coords(
+            self.ctx.handle, self._h, *[N.ptr(o, C.c_float) for o in out]))
+        return [o[:self.n_atoms] for o in out]
+
+    def set_coords(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(v, np.float32) for v in (x, y, z))
+        self.ctx.check(N.lib.dnab_population_write_coords(self.ctx.handle, self._h,
+                                                          N.ptr(x, C.c_float),
+                                                          N.ptr(y, C.c_float),
+                                                          N.ptr(z, C.c_float)))
+
+    def pair_count(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        self.ctx.check(N.lib.dnab_population_pair_count(self.ctx.handle, self._h, C.byref(a),
+                                                        C.byref(b)))
+        return a.value, b.value
+
+    def pair_stats(self, reset: bool = False):
+        a, b = C.c_uint64(), C.c_uint64()
+        self.ctx.check(N.lib.dnab_population_read_pair_stats(self.ctx.handle, self._h,
+                                                             C.byref(a), C.byref(b)))
+        if reset:
+            self.ctx.check(N.lib.dnab_population_reset_pair_stats(self.ctx.handle, self._h))
+        return a.value, b.value
+
+    def reset_pair_stats(self):
+        self.ctx.check(N.lib.dnab_population_reset_pair_stats(self.ctx.handle, self._h))
+
+    def step(self, params: N.LangevinParamsC, iteration: int):
+        self.ctx.check(N.lib.dnab_langevin_step(self.ctx.handle, self._h, C.byref(params),
+                                                iteration))
+
+    def run(self, params: N.LangevinParamsC):
+        k = max(1, params.top_k)
+        idx = np.zeros(k, np.uint32)
+        e = np.zeros(k, np.float64)
+        n = C.c_uint32()
+        self.ctx.check(N.lib.dnab_langevin_run(self.ctx.handle, self._h, C.byref(params),
+                                               N.ptr(idx, C.c_uint32), N.ptr(e, C.c_double),
+                                               C.byref(n)))
+        return idx[:n.value], e[:n.value]
+
+    def close(self):
+        if self._h:
+            N.lib.dnab_population_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,333 @@
+/* dnab — MI355X (gfx950) batched all-atom binder scoring: C ABI.
+ *
+ * One shared library (libdnab.so) exposes the hot path of the reference's
+ * all-atom engine as plain C entry points: plain pointers and sizes, status
+ * codes, every output buffer owned by the caller. Each entry point names the
+ * reference interface it replaces (reference file:line).
+ *
+ * Conventions
+ *  - Every function returns DNAB_OK (0) or a DNAB_ERR_* code; no exceptions or
+ *    aborts cross the boundary. dnab_ctx_last_error() gives the message.
+ *  - A context owns one HIP stream on one device. A context is not
+ *    thread-safe: use one context per host thread.
+ *  - Host-pointer entry points (score_batch, energy/force/grid calls) copy in,
+ *    run, and copy back before returning. Device-resident objects
+ *    (dnab_antigen, dnab_population) keep their data in HBM between calls.
+ *  - Units: Angstrom, kcal/mol, elementary charge.
+ */
+#ifndef DNAB_H_
+#define DNAB_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DNAB_OK 0
+#define DNAB_ERR_INVALID 1     /* bad argument */
+#define DNAB_ERR_NO_DEVICE 2   /* no usable GPU (reference: try_init -> None) */
+#define DNAB_ERR_HIP 3         /* a HIP runtime call failed */
+#define DNAB_ERR_ALLOC 4       /* device or host allocation failed */
+#define DNAB_ERR_CAPACITY 5    /* caller buffer too small; required size reported */
+#define DNAB_ERR_UNSUPPORTED 6 /* valid input outside what this build handles */
+
+typedef struct dnab_ctx dnab_ctx;
+typedef struct dnab_grid dnab_grid;
+typedef struct dnab_antigen dnab_antigen;
+typedef struct dnab_population dnab_population;
+
+/* ---- data layouts ------------------------------------------------------- */
+
+/* 32-byte AoS atom, field-for-field the reference GpuAtom
+ * (src/gpu.rs:18-29, shaders/energy.wgsl:21-30). */
+typedef struct dnab_gpu_atom {
+  float x, y, z, q, r_min_half, epsilon, hydrophobic, _pad;
+} dnab_gpu_atom;
+
+/* Read-only SoA view of an AtomCloud (src/allatom.rs:16-31). atom_type and
+ * residue_idx are not used by the LJ/Coulomb/hydrophobic terms and may be
+ * NULL. */
+typedef struct dnab_atom_cloud {
+  size_t n;
+  const float* x;
+  const float* y;
+  const float* z;
+  const float* charge;
+  const float* r_min_half;
+  const float* epsilon;
+  const uint8_t* hydrophobic; /* 0 or 1 */
+  const uint8_t* atom_type;
+  const uint32_t* residue_idx;
+} dnab_atom_cloud;
+
+/* Read-only SoA view of a ResidueCloud (src/atom.rs:185-195). */
+typedef struct dnab_residue_cloud {
+  size_t n;
+  const float* x;
+  const float* y;
+  const float* z;
+  const float* charge;
+  const float* epsilon;
+  const float* sigma;
+  const uint8_t* hydrophobic;
+} dnab_residue_cloud;
+
+/* AtomEnergyBreakdown (src/energy.rs:238-250). The GPU path computes the LJ,
+ * Coulomb and hydrophobic terms; hbond and solvation are returned as 0. */
+typedef struct dnab_energy_breakdown {
+  double lj, coulomb, hydrophobic, hbond, solvation;
+} dnab_energy_breakdown;
+
+/* Output buffers for dnab_build_protein (caller-allocated). Per-atom arrays
+ * hold atom_capacity entries; per-residue arrays n_res (res_start n_res+1).
+ * Any pointer except x/y/z may be NULL. */
+typedef struct dnab_protein_out {
+  size_t atom_capacity;
+  float *x, *y, *z, *charge, *r_min_half, *epsilon;
+  uint8_t *hydrophobic, *atom_type;
+  uint32_t* residue_idx;
+  uint32_t* res_start;
+  uint32_t* ca_atom_idx;
+  uint8_t* amino_acid;
+} dnab_protein_out;
+
+/* ---- context (replaces GpuContext, src/gpu.rs:134-243) -------------------
+ * dnab_ctx_create replaces GpuContext::try_init (src/gpu.rs:143-243): a
+ * non-zero status (DNAB_ERR_NO_DEVICE) takes the place of `None`. */
+int dnab_ctx_create(int device, dnab_ctx** out);
+void dnab_ctx_destroy(dnab_ctx* ctx);
+const char* dnab_ctx_last_error(const dnab_ctx* ctx);
+const char* dnab_status_string(int status);
+int dnab_ctx_synchronize(dnab_ctx* ctx);
+int dnab_device_count(int* count);
+/* Device properties of the context's GPU: CU count, wave size, arch name. */
+int dnab_ctx_device_info(const dnab_ctx* ctx, int* cu_count, int* wave_size, char* arch,
+                         size_t arch_len);
+/* Kernel timing with HIP events on the context stream (no reference
+ * counterpart; the reference times whole runs with Instant, src/main.rs:153).
+ * When enabled, every launch of the pair kernel is bracketed by events;
+ * dnab_ctx_kernel_time returns the summed milliseconds and launch count. */
+int dnab_ctx_set_timing(dnab_ctx* ctx, int enable);
+int dnab_ctx_kernel_time(dnab_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
+
+/* ---- structure building (host only, no GPU needed) ------------------------
+ * protein_from_ca_trace (src/allatom.rs:523-541) with an optional rotamer
+ * index per residue (NULL = most probable rotamer, as the reference). */
+int dnab_build_protein(const float* ca_x, const float* ca_y, const float* ca_z,
+                       const uint8_t* amino_acid, const uint8_t* rotamer_idx, size_t n_res,
+                       dnab_protein_out* out, size_t* n_atoms_out);
+/* place_atom (src/rotamer.rs:198-232), NERF. */
+int dnab_place_atom(const float a[3], const float b[3], const float c[3], float bond_len,
+                    float angle_deg, float dihedral_deg, float out[3]);
+/* Heavy-atom count of one residue type (src/amber.rs:357-378); -1 if invalid. */
+int dnab_residue_atom_count(uint8_t amino_acid);
+/* Residue-level parameters (src/atom.rs:55-104). */
+int dnab_residue_cg_params(uint8_t amino_acid, float* charge, float* epsilon, float* sigma,
+                           uint8_t* hydrophobic);
+/* One topology slot (src/amber.rs:126-351) with its AMBER_NB parameters
+ * (src/amber.rs:46-63). */
+int dnab_ff_table(uint8_t amino_acid, int slot, uint8_t* atom_type, float* charge,
+                  uint8_t* hydrophobic, float* r_min_half, float* epsilon);
+/* Synthetic Calpha traces (no reference counterpart except the ring layout,
+ * which follows random_allatom_antibody, src/diffusion.rs:264-277). */
+int dnab_synth_globule_trace(size_t n_res, float radius, uint64_t seed, float* x, float* y,
+                             float* z, uint8_t* amino_acid, uint8_t* rotamer_idx);
+int dnab_synth_binder_ring(size_t n_res, const float center[3], float ring_radius,
+                           uint64_t seed, float* x, float* y, float* z, uint8_t* amino_acid);
+int dnab_synth_binder_shell(size_t n_res, const float center[3], float shell_radius,
+                            float min_radius, uint64_t seed, float* x, float* y, float* z,
+                            uint8_t* amino_acid);
+
+/* ---- spatial hash grid (replaces SpatialHashGrid, src/spatial.rs:12-68) ---
+ * dnab_grid_create  = SpatialHashGrid::new   (src/spatial.rs:23-28)
+ * dnab_grid_build   = SpatialHashGrid::build (src/spatial.rs:31-39)
+ * dnab_grid_query   = query_neighbors        (src/spatial.rs:44-58), batched:
+ *   probe p's list is indices[offsets[p] .. offsets[p+1]) in the reference's
+ *   emission order (dx, dy, dz nested, ascending index within a cell).
+ * dnab_grid_cell_of = cell_of                (src/spatial.rs:61-67), keys as
+ *   (kx, ky, kz) triples.
+ * If `indices` is NULL or `capacity` is too small, the offsets are still
+ * written, *n_total is set and DNAB_ERR_CAPACITY is returned. */
+int dnab_grid_create(dnab_ctx* ctx, float cell_size, dnab_grid** out);
+int dnab_grid_build(dnab_ctx* ctx, dnab_grid* grid, const float* x, const float* y,
+                    const float* z, size_t n);
+int dnab_grid_query(dnab_ctx* ctx, const dnab_grid* grid, const float* px, const float* py,
+                    const float* pz, size_t n_probe, uint32_t* offsets, uint32_t* indices,
+                    size_t capacity, size_t* n_total);
+int dnab_grid_cell_of(dnab_ctx* ctx, const dnab_grid* grid, const float* x, const float* y,
+                      const float* z, size_t n, int32_t* keys);
+/* Grid introspection: points, non-empty cells, key bounding box. */
+int dnab_grid_info(const dnab_grid* grid, size_t* n_points, size_t* n_cells_nonempty,
+                   int32_t key_min[3], int32_t key_max[3]);
+void dnab_grid_destroy(dnab_grid* grid);
+
+/* ---- residue-level forces -------------------------------------------------
+ * compute_forces_with_grid (src/energy.rs:322-398): `grid` must have been
+ * built over antigen->x/y/z. Outputs are overwritten (the reference zeroes
+ * them first, src/energy.rs:332-334); length antibody->n. */
+int dnab_cg_forces_with_grid(dnab_ctx* ctx, const dnab_residue_cloud* antigen,
+                             const dnab_grid* grid, const dnab_residue_cloud* antibody,
+                             float* fx, float* fy, float* fz);
+/* compute_forces (src/energy.rs:154-234), brute force over all antigen points. */
+int dnab_cg_forces(dnab_ctx* ctx, const dnab_residue_cloud* antigen,
+                   const dnab_residue_cloud* antibody, float* fx, float* fy, float* fz);
+
+/* ---- all-atom energies and forces (host buffers) ---------------------------
+ * interaction_energy_atoms_breakdown (src/energy.rs:261-314), LJ + Coulomb +
+ * hydrophobic terms with the reference pair predicate 0.25 <= r^2 <= cutoff^2
+ * (cutoff 10 in the reference, src/energy.rs:16). */
+int dnab_interaction_energy_atoms_breakdown(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                          const dnab_atom_cloud* antibody, float cutoff,
+                          dnab_energy_breakdown* out);
+/* interaction_energy_atoms (src/energy.rs:256-258), same three terms. */
+int dnab_interaction_energy_atoms(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                                  const dnab_atom_cloud* antibody, float cutoff,
+                                  double* energy);
+/* All-atom AMBER forces on the antibody atoms (no reference counterpart: the
+ * reference has residue-level forces only, src/energy.rs:154-234). Force on
+ * atom j is sum_i [12 eps (s^12 - s^6)/r^2 + 332 q_i q_j / r^3 + f_h] (p_j -
+ * p_i), s = R_ij/r, the exact negative gradient of the LJ and Coulomb terms;
+ * f_h = -0.1/r for hydrophobic pairs with r^2 < 36, the reference's force
+ * convention for the hydrophobic term (src/energy.rs:382-386). */
+int dnab_atom_forces(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                     const dnab_atom_cloud* antibody, float cutoff, float* fx, float* fy,
+                     float* fz, double* energy);
+
+/* score_batch (src/gpu.rs:166-267): candidates are n_cand = n_atoms / n_ab
+ * uniform-length blocks of GpuAtom; energies (LJ + Coulomb + hydrophobic,
+ * cutoff 10 as shaders/energy.wgsl:16) are written to energies_out[0..n_cand).
+ * n_ab == 0 or no candidates gives *n_cand = 0 and DNAB_OK, as the
+ * reference's empty Vec (src/gpu.rs:172-174). */
+int dnab_score_batch(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                     const dnab_gpu_atom* candidates, size_t n_atoms, size_t n_ab,
+                     float* energies_out, size_t capacity, size_t* n_cand);
+
+/* ---- device-resident antigen ---------------------------------------------
+ * The antigen's atoms are sorted into a Morton-ordered, cell-major SoA over
+ * the SpatialHashGrid cells of `cell_size` (src/spatial.rs semantics) and
+ * kept in HBM. `center` is the restraint centre of the gradient step (the
+ * antigen's Calpha centre of mass, src/diffusion.rs:991). `ca` is the
+ * antigen Calpha ResidueCloud used by the residue-level force model
+ * (src/diffusion.rs:994-1004); it may be empty when that model is unused. */
+typedef struct dnab_antigen_desc {
+  dnab_atom_cloud atoms;
+  dnab_residue_cloud ca;
+  float center[3];
+  float cell_size; /* grid cell edge; the reference uses 10 (src/diffusion.rs:1003) */
+  float cutoff;    /* pair cutoff; the reference uses 10 (src/energy.rs:16) */
+  float restraint_radius; /* gradient-step restraint radius around `center`;
+                             0 = the step parameter restraint_radius */
+} dnab_antigen_desc;
+int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* desc, dnab_antigen** out);
+int dnab_antigen_info(const dnab_antigen* ag, size_t* n_atoms, size_t* n_cells_nonempty,
+                      float* cutoff, float* cell_size);
+void dnab_antigen_destroy(dnab_antigen* ag);
+
+/* ---- device-resident ragged population ------------------------------------
+ * No reference counterpart: score_batch needs one atom count for every
+ * candidate (src/gpu.rs:166-174). Candidates here have their own residue and
+ * atom ranges and each is bound to one antigen. Atoms of one candidate are
+ * contiguous; residues of one candidate are contiguous. */
+typedef struct dnab_population_desc {
+  size_t n_cand, n_res;
+  const uint32_t* cand_antigen; /* n_cand, index into the antigens array */
+  const uint32_t* cand_res_off; /* n_cand + 1, residue ranges */
+  const uint32_t* res_atom_off; /* n_res + 1, atom ranges (global atom index) */
+  const uint32_t* res_ca_atom;  /* n_res, global atom index of each Calpha */
+  const uint8_t* res_amino_acid;/* n_res, residue types (residue-level model) */
+  dnab_atom_cloud atoms;        /* all candidate atoms, n = res_atom_off[n_res] */
+} dnab_population_desc;
+int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t n_antigens,
+                           const dnab_population_desc* desc, dnab_population** out);
+void dnab_population_destroy(dnab_population* pop);
+
+#define DNAB_EVAL_FORCES 1u    /* also compute per-atom forces */
+#define DNAB_EVAL_BREAKDOWN 2u /* keep LJ / Coulomb / hydrophobic separately */
+#define DNAB_EVAL_FP64 4u      /* all-FP64 oracle kernel instead of FP32 */
+/* One evaluation of every candidate against its antigen: per-candidate
+ * energies (FP32 pair terms, TwoSum-compensated per atom, FP64 total) and,
+ * with DNAB_EVAL_FORCES, per-atom forces. Results stay on the device. */
+int dnab_population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
+/* Per-candidate energies of the last evaluation; err_bound (nullable) is the
+ * a-priori bound on |E - E_fp64| from the FP32 pair terms. */
+int dnab_population_read_energies(dnab_ctx* ctx, const dnab_population* pop, double* energy,
+                                  float* err_bound);
+/* Per-candidate LJ, Coulomb, hydrophobic of the last DNAB_EVAL_BREAKDOWN run. */
+int dnab_population_read_breakdown(dnab_ctx* ctx, const dnab_population* pop, double* lj,
+                                   double* coulomb, double* hydrophobic);
+/* Per-atom forces of the last evaluation (FP64 runs are rounded to float;
+ * use dnab_population_read_forces_f64 for the oracle values). */
+int dnab_population_read_forces(dnab_ctx* ctx, const dnab_population* pop, float* fx,
+                                float* fy, float* fz);
+int dnab_population_read_forces_f64(dnab_ctx* ctx, const dnab_population* pop, double* fx,
+                                    double* fy, double* fz);
+int dnab_population_read_coords(dnab_ctx* ctx, const dnab_population* pop, float* x,
+                                float* y, float* z);
+int dnab_population_write_coords(dnab_ctx* ctx, dnab_population* pop, const float* x,
+                                 const float* y, const float* z);
+/* Pair statistics at the current coordinates: pairs inside the FP32
+ * predicate 0.25 <= r^2 <= cutoff^2, and pairs the kernel tested. */
+int dnab_population_pair_count(dnab_ctx* ctx, dnab_population* pop, uint64_t* in_cutoff,
+                               uint64_t* tested);
+int dnab_population_info(const dnab_population* pop, size_t* n_cand, size_t* n_res,
+                         size_t* n_atoms);
+/* Pair statistics accumulated by runs with count_pairs set (reset to zero by
+ * dnab_population_reset_pair_stats). */
+int dnab_population_read_pair_stats(dnab_ctx* ctx, const dnab_population* pop,
+                                    uint64_t* in_cutoff, uint64_t* tested);
+int dnab_population_reset_pair_stats(dnab_ctx* ctx, dnab_population* pop);
+
+/* ---- gradient step and population loop ------------------------------------
+ * The step restates allatom_diffusion_step lines src/diffusion.rs:300-339:
+ * per residue, force F (all-atom: sum of the residue's atom forces; or the
+ * residue-level grid force of src/energy.rs:322-398 on the Calpha), plus the
+ * restraint -k (d - R0)/d (ca - center) (src/diffusion.rs:317-327), then
+ * ca' = ca + clamp(step*F, +-max_disp) + U(-s, s) with s = noise_base*sqrt(T)
+ * when with_noise (src/diffusion.rs:315, 329-337), applied as a rigid
+ * translation of the residue's atoms (set_ca_pos, src/allatom.rs:289-293).
+ * T(iter) = t_hot (t_cold/t_hot)^((iter + schedule_offset)/schedule_total)
+ * (src/diffusion.rs:102-104). Monte-Carlo moves (src/diffusion.rs:341-418)
+ * are not part of this path. */
+#define DNAB_FORCE_ALLATOM 0 /* all-atom AMBER forces summed per residue */
+#define DNAB_FORCE_CG 1      /* residue-level Calpha grid forces (reference step) */
+typedef struct dnab_langevin_params {
+  uint32_t n_steps;         /* gradient steps (reference GPU phase: 200) */
+  uint32_t score_every;     /* score every k steps (0 = only at the end) */
+  uint32_t top_k;           /* survivors kept per scoring (reference: 64) */
+  uint32_t resort_every;    /* re-sort candidate atoms every k steps (>= 1) */
+  uint64_t seed;
+  int32_t with_noise;       /* reference GPU phase: 0, CPU phase: 1 */
+  int32_t force_model;      /* DNAB_FORCE_ALLATOM or DNAB_FORCE_CG */
+  int32_t use_graph;        /* replay the step loop from a captured hipGraph */
+  float step;               /* 0.08  (src/diffusion.rs:66) */
+  float max_disp;           /* 2.0   (src/diffusion.rs:82) */
+  float noise_base;         /* 0.25  (src/diffusion.rs:67) */
+  float t_hot, t_cold;      /* 3.0, 0.02 (src/diffusion.rs:68-69) */
+  uint32_t schedule_offset; /* iter offset into the temperature schedule */
+  uint32_t schedule_total;  /* schedule length (reference: GPU_STEPS + cpu_iterations) */
+  float restraint_k;        /* 0.02  (src/diffusion.rs:83) */
+  float restraint_radius;   /* 20.0  (INIT_RADIUS, src/diffusion.rs:81) */
+  int32_t count_pairs;      /* accumulate pair statistics over the run (same
+                               results, slower); read with
+                               dnab_population_read_pair_stats */
+} dnab_langevin_params;
+int dnab_langevin_default_params(dnab_langevin_params* p);
+/* One step at iteration `iter` (forces from a fresh evaluation). */
+int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
+                       uint32_t iter);
+/* The step loop of run_allatom's GPU phase (src/diffusion.rs:1018-1057), all
+ * candidates kept moving, with scoring every score_every steps after the
+ * update; the top_k (ascending energy, ties by index) of the final scoring
+ * are written to topk_idx / topk_energy (each top_k entries; fewer if the
+ * population is smaller, count in *n_topk). */
+int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
+                      uint32_t* topk_idx, double* topk_energy, uint32_t* n_topk);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* DNAB_H_ */

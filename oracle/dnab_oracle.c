@@ -1,0 +1,670 @@
+/* dnab oracle — CPU restatement of the reference's hot-path arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, as the checker and as the
+ * timed CPU baseline; the product (libdnab.so) never links or calls it.
+ *
+ * Build: oracle/Makefile (gcc -O3 -ffp-contract=off -fno-fast-math). The
+ * reference is Rust and compiles without contraction or reassociation;
+ * ffp-contract=off gives the same IEEE f32 evaluation here.
+ *
+ * Functions and the reference lines they restate:
+ *   or_cell_of                  SpatialHashGrid::cell_of      src/spatial.rs:61-67
+ *   or_grid_query               build + query_neighbors       src/spatial.rs:31-58
+ *   or_energy_f32               interaction_energy_atoms_breakdown, LJ/Coulomb/
+ *                               hydrophobic terms, f32 in the reference's order
+ *                                                             src/energy.rs:261-301
+ *   or_energy_f64               the same pair set (f32 predicate) with every term
+ *                               and sum in double, plus the analytic all-atom
+ *                               forces (negative gradient of LJ + Coulomb, the
+ *                               hydrophobic pull of src/energy.rs:382-386)
+ *   or_population_f64           or_energy_f64 over a ragged batch, threaded
+ *   or_cg_forces(_with_grid)    compute_forces(_with_grid)    src/energy.rs:154-234, 322-398
+ *   or_langevin_step            allatom_diffusion_step lines  src/diffusion.rs:300-339
+ *                               with set_ca_pos               src/allatom.rs:279-293
+ *   or_sigma                    temperature + noise scale     src/diffusion.rs:102-104, 315
+ *   or_philox4x32_10            counter RNG of the step's noise (Salmon et al. 2011)
+ *   or_baseline_energy_brute    timed CPU baseline: src/energy.rs:269-301 pattern,
+ *                               parallel over candidates like the Rayon loop of
+ *                               src/diffusion.rs:1083-1095
+ *   or_baseline_grid_ef         timed CPU baseline: grid E+F, src/energy.rs:322-398
+ *                               pattern with AMBER per-atom parameters
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define COULOMB_K 332.0f   /* src/energy.rs:13 */
+#define MIN_R_SQ 0.25f     /* src/energy.rs:17 */
+#define HYDRO_SQ 36.0f     /* src/energy.rs:20 */
+#define HYDRO_BONUS -0.5f  /* src/energy.rs:21 */
+
+typedef struct {
+  size_t n;
+  const float *x, *y, *z, *q, *rmh, *eps;
+  const uint8_t* hyd;
+} or_cloud;
+
+/* ---- SpatialHashGrid ---------------------------------------------------- */
+
+static int32_t f32_to_i32_sat(float f) {
+  if (f != f) return 0;
+  if (f >= 2147483648.0f) return INT32_MAX;
+  if (f <= -2147483648.0f) return INT32_MIN;
+  return (int32_t)f;
+}
+
+int32_t or_cell_of(float x, float inv_cell) {
+  volatile float p = x * inv_cell;
+  return f32_to_i32_sat(floorf(p));
+}
+
+float or_inv_cell(float cell) { return 1.0f / cell; }
+
+static int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+/* Neighbour lists by definition: for each probe, the 27 cells in dx, dy, dz
+ * nested order; within a cell every point whose key equals the cell, in
+ * ascending index (the order `build` appends them). O(27 n) per probe. */
+int or_grid_query(const float* x, const float* y, const float* z, size_t n, float cell,
+                  const float* px, const float* py, const float* pz, size_t n_probe,
+                  uint32_t* offsets, uint32_t* indices, size_t capacity) {
+  float inv = 1.0f / cell;
+  int32_t* keys = (int32_t*)malloc(3 * (n ? n : 1) * sizeof(int32_t));
+  if (!keys) return -1;
+  for (size_t i = 0; i < n; ++i) {
+    keys[3 * i] = or_cell_of(x[i], inv);
+    keys[3 * i + 1] = or_cell_of(y[i], inv);
+    keys[3 * i + 2] = or_cell_of(z[i], inv);
+  }
+  size_t o = 0;
+  for (size_t p = 0; p < n_probe; ++p) {
+    offsets[p] = (uint32_t)o;
+    int32_t cx = or_cell_of(px[p], inv), cy = or_cell_of(py[p], inv), cz = or_cell_of(pz[p], inv);
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          int32_t kx = wrap_add(cx, dx), ky = wrap_add(cy, dy), kz = wrap_add(cz, dz);
+          for (size_t i = 0; i < n; ++i)
+            if (keys[3 * i] == kx && keys[3 * i + 1] == ky && keys[3 * i + 2] == kz) {
+              if (indices && o < capacity) indices[o] = (uint32_t)i;
+              ++o;
+            }
+        }
+  }
+  offsets[n_probe] = (uint32_t)o;
+  free(keys);
+  return o <= capacity ? 0 : 1;
+}
+
+/* ---- all-atom energies --------------------------------------------------- */
+
+/* f32 in the reference's order (src/energy.rs:269-301): antibody outer,
+ * antigen inner, three sequential f32 accumulators. */
+void or_energy_f32(const or_cloud* ag, const or_cloud* ab, float cutoff, float out[3]) {
+  const float cut2 = cutoff * cutoff;
+  float lj = 0.0f, coul = 0.0f, hyd = 0.0f;
+  for (size_t j = 0; j < ab->n; ++j) {
+    const float bx = ab->x[j], by = ab->y[j], bz = ab->z[j], bq = ab->q[j];
+    const float brm = ab->rmh[j], be = ab->eps[j];
+    const uint8_t bh = ab->hyd[j];
+    for (size_t i = 0; i < ag->n; ++i) {
+      float dx = bx - ag->x[i];
+      float dy = by - ag->y[i];
+      float dz = bz - ag->z[i];
+      float r_sq = dx * dx + dy * dy + dz * dz;
+      if (r_sq > cut2 || r_sq < MIN_R_SQ) continue;
+      float r_ij = brm + ag->rmh[i];
+      float eps = sqrtf(be * ag->eps[i]);
+      float r2 = (r_ij * r_ij) / r_sq;
+      float r6 = r2 * r2 * r2;
+      float r12 = r6 * r6;
+      lj += eps * (r12 - 2.0f * r6);
+      float q2 = ag->q[i];
+      if (bq != 0.0f && q2 != 0.0f) coul += COULOMB_K * bq * q2 / sqrtf(r_sq);
+      if (bh == 1 && ag->hyd[i] == 1 && r_sq < HYDRO_SQ) hyd += HYDRO_BONUS;
+    }
+  }
+  out[0] = lj;
+  out[1] = coul;
+  out[2] = hyd;
+}
+
+/* FP64 oracle for one antibody atom against the whole antigen. The pair set
+ * is decided by the reference's f32 predicate; terms, mixing and sums are in
+ * double from the raw f32 parameters. */
+static void atom_f64(const or_cloud* ag, const or_cloud* ab, size_t j, float cut2,
+                     double acc[6], uint64_t* pairs) {
+  const float bx = ab->x[j], by = ab->y[j], bz = ab->z[j];
+  const double bq = ab->q[j], brm = ab->rmh[j], be = ab->eps[j];
+  const uint8_t bh = ab->hyd[j];
+  double lj = 0, co = 0, hy = 0, fx = 0, fy = 0, fz = 0;
+  uint64_t np = 0;
+  for (size_t i = 0; i < ag->n; ++i) {
+    float dx = bx - ag->x[i];
+    float dy = by - ag->y[i];
+    float dz = bz - ag->z[i];
+    float r_sq = dx * dx + dy * dy + dz * dz;
+    if (r_sq > cut2 || r_sq < MIN_R_SQ) continue;
+    ++np;
+    double ddx = (double)bx - (double)ag->x[i];
+    double ddy = (double)by - (double)ag->y[i];
+    double ddz = (double)bz - (double)ag->z[i];
+    double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    double r = sqrt(r2);
+    double R = (double)ag->rmh[i] + brm;
+    double eps = sqrt((double)ag->eps[i] * be);
+    double s2 = (R * R) / r2;
+    double s6 = s2 * s2 * s2;
+    double s12 = s6 * s6;
+    lj += eps * (s12 - 2.0 * s6);
+    double f = 12.0 * eps * (s12 - s6) / r2;
+    double ec = 332.0 * bq * (double)ag->q[i] / r;
+    co += ec;
+    f += ec / r2;
+    if (bh == 1 && ag->hyd[i] == 1 && r_sq < HYDRO_SQ) {
+      hy += -0.5;
+      f += -0.1 / r;
+    }
+    fx += f * ddx;
+    fy += f * ddy;
+    fz += f * ddz;
+  }
+  acc[0] = lj;
+  acc[1] = co;
+  acc[2] = hy;
+  acc[3] = fx;
+  acc[4] = fy;
+  acc[5] = fz;
+  if (pairs) *pairs += np;
+}
+
+void or_energy_f64(const or_cloud* ag, const or_cloud* ab, float cutoff, double out[3],
+                   double* fx, double* fy, double* fz, uint64_t* n_pairs) {
+  const float cut2 = cutoff * cutoff;
+  double lj = 0, co = 0, hy = 0;
+  uint64_t np = 0;
+  for (size_t j = 0; j < ab->n; ++j) {
+    double a[6];
+    atom_f64(ag, ab, j, cut2, a, &np);
+    lj += a[0];
+    co += a[1];
+    hy += a[2];
+    if (fx) fx[j] = a[3];
+    if (fy) fy[j] = a[4];
+    if (fz) fz[j] = a[5];
+  }
+  out[0] = lj;
+  out[1] = co;
+  out[2] = hy;
+  if (n_pairs) *n_pairs = np;
+}
+
+/* ---- threaded batch helpers ----------------------------------------------- */
+
+typedef struct {
+  const or_cloud* ag;
+  const or_cloud* atoms;   /* all candidate atoms */
+  const uint32_t* cand_off;
+  size_t n_cand;
+  float cutoff;
+  float cell;
+  double *e, *lj, *co, *hy, *fx, *fy, *fz;
+  float* e32;
+  float *f32x, *f32y, *f32z;
+  uint64_t* pairs;         /* per candidate */
+  size_t next;             /* work counter */
+  pthread_mutex_t mu;
+  int kind;
+  /* grid baseline */
+  const int32_t* ckey;     /* dense grid */
+  const uint32_t* cstart;
+  const uint32_t* cidx;
+  int32_t kmin[3], kdim[3];
+  float inv;
+} or_job;
+
+static size_t take(or_job* J, size_t chunk) {
+  pthread_mutex_lock(&J->mu);
+  size_t c = J->next;
+  J->next += chunk;
+  pthread_mutex_unlock(&J->mu);
+  return c;
+}
+
+static or_cloud sub_cloud(const or_cloud* a, size_t b, size_t e) {
+  or_cloud c;
+  c.n = e - b;
+  c.x = a->x + b;
+  c.y = a->y + b;
+  c.z = a->z + b;
+  c.q = a->q + b;
+  c.rmh = a->rmh + b;
+  c.eps = a->eps + b;
+  c.hyd = a->hyd + b;
+  return c;
+}
+
+static void* worker_f64(void* arg) {
+  or_job* J = (or_job*)arg;
+  const float cut2 = J->cutoff * J->cutoff;
+  for (;;) {
+    size_t c = take(J, 1);
+    if (c >= J->n_cand) break;
+    size_t b = J->cand_off[c], e = J->cand_off[c + 1];
+    double lj = 0, co = 0, hy = 0;
+    uint64_t np = 0;
+    for (size_t j = b; j < e; ++j) {
+      double a[6];
+      atom_f64(J->ag, J->atoms, j, cut2, a, &np);
+      lj += a[0];
+      co += a[1];
+      hy += a[2];
+      if (J->fx) {
+        J->fx[j] = a[3];
+        J->fy[j] = a[4];
+        J->fz[j] = a[5];
+      }
+    }
+    if (J->lj) {
+      J->lj[c] = lj;
+      J->co[c] = co;
+      J->hy[c] = hy;
+    }
+    J->e[c] = (lj + co) + hy;
+    if (J->pairs) J->pairs[c] = np;
+  }
+  return NULL;
+}
+
+static void run_threads(or_job* J, int nthreads, void* (*fn)(void*)) {
+  if (nthreads < 1) nthreads = 1;
+  pthread_mutex_init(&J->mu, NULL);
+  J->next = 0;
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  int started = 0;
+  for (int t = 1; t < nthreads; ++t)
+    if (pthread_create(&th[t], NULL, fn, J) == 0) ++started;
+    else break;
+  fn(J);
+  for (int t = 1; t <= started; ++t) pthread_join(th[t], NULL);
+  free(th);
+  pthread_mutex_destroy(&J->mu);
+}
+
+/* Per candidate: E = LJ + Coulomb + hydrophobic in FP64 (and the terms), per
+ * atom FP64 forces (nullable), per candidate in-cutoff pair counts. */
+void or_population_f64(const or_cloud* ag, const or_cloud* atoms, const uint32_t* cand_off,
+                       size_t n_cand, float cutoff, double* e, double* lj, double* co,
+                       double* hy, double* fx, double* fy, double* fz, uint64_t* pairs,
+                       int nthreads) {
+  or_job J;
+  memset(&J, 0, sizeof(J));
+  J.ag = ag;
+  J.atoms = atoms;
+  J.cand_off = cand_off;
+  J.n_cand = n_cand;
+  J.cutoff = cutoff;
+  J.e = e;
+  J.lj = lj;
+  J.co = co;
+  J.hy = hy;
+  J.fx = fx;
+  J.fy = fy;
+  J.fz = fz;
+  J.pairs = pairs;
+  run_threads(&J, nthreads, worker_f64);
+}
+
+/* ---- timed CPU baselines ---------------------------------------------------- */
+
+/* Reference pattern (src/energy.rs:269-301): per candidate a brute-force f32
+ * breakdown over all antigen atoms. */
+static void* worker_brute(void* arg) {
+  or_job* J = (or_job*)arg;
+  for (;;) {
+    size_t c = take(J, 1);
+    if (c >= J->n_cand) break;
+    or_cloud ab = sub_cloud(J->atoms, J->cand_off[c], J->cand_off[c + 1]);
+    float out[3];
+    or_energy_f32(J->ag, &ab, J->cutoff, out);
+    J->e32[c] = out[0] + out[1] + out[2];
+  }
+  return NULL;
+}
+
+void or_baseline_energy_brute(const or_cloud* ag, const or_cloud* atoms,
+                              const uint32_t* cand_off, size_t n_cand, float cutoff, float* e,
+                              int nthreads) {
+  or_job J;
+  memset(&J, 0, sizeof(J));
+  J.ag = ag;
+  J.atoms = atoms;
+  J.cand_off = cand_off;
+  J.n_cand = n_cand;
+  J.cutoff = cutoff;
+  J.e32 = e;
+  run_threads(&J, nthreads, worker_brute);
+}
+
+/* Grid E+F baseline: the antigen in a dense cell grid (cell = cutoff), each
+ * candidate atom probes the 27 neighbouring cells (src/spatial.rs:44-58) and
+ * accumulates AMBER energy and force in f32 (src/energy.rs:336-397 pattern). */
+static void* worker_grid(void* arg) {
+  or_job* J = (or_job*)arg;
+  const float cut2 = J->cutoff * J->cutoff;
+  const or_cloud* ag = J->ag;
+  for (;;) {
+    size_t c = take(J, 1);
+    if (c >= J->n_cand) break;
+    float etot = 0.0f;
+    for (size_t j = J->cand_off[c]; j < J->cand_off[c + 1]; ++j) {
+      const float bx = J->atoms->x[j], by = J->atoms->y[j], bz = J->atoms->z[j];
+      const float bq = J->atoms->q[j], brm = J->atoms->rmh[j], be = J->atoms->eps[j];
+      const uint8_t bh = J->atoms->hyd[j];
+      int32_t k[3] = {or_cell_of(bx, J->inv), or_cell_of(by, J->inv), or_cell_of(bz, J->inv)};
+      float ax = 0, ay = 0, az = 0, e = 0;
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dz = -1; dz <= 1; ++dz) {
+            int32_t gx = k[0] + dx - J->kmin[0], gy = k[1] + dy - J->kmin[1],
+                    gz = k[2] + dz - J->kmin[2];
+            if (gx < 0 || gy < 0 || gz < 0 || gx >= J->kdim[0] || gy >= J->kdim[1] ||
+                gz >= J->kdim[2])
+              continue;
+            size_t cell = ((size_t)gz * J->kdim[1] + gy) * J->kdim[0] + gx;
+            for (uint32_t s = J->cstart[cell]; s < J->cstart[cell + 1]; ++s) {
+              uint32_t i = J->cidx[s];
+              float ddx = bx - ag->x[i], ddy = by - ag->y[i], ddz = bz - ag->z[i];
+              float r_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+              if (r_sq > cut2 || r_sq < MIN_R_SQ) continue;
+              float r_ij = brm + ag->rmh[i];
+              float eps = sqrtf(be * ag->eps[i]);
+              float s2 = (r_ij * r_ij) / r_sq;
+              float s6 = s2 * s2 * s2;
+              float s12 = s6 * s6;
+              e += eps * (s12 - 2.0f * s6);
+              float f = (12.0f * eps / r_sq) * (s12 - s6);
+              float q2 = ag->q[i];
+              if (bq != 0.0f && q2 != 0.0f) {
+                float r = sqrtf(r_sq);
+                e += COULOMB_K * bq * q2 / r;
+                f += COULOMB_K * bq * q2 / (r_sq * r);
+              }
+              if (bh == 1 && ag->hyd[i] == 1 && r_sq < HYDRO_SQ) {
+                e += HYDRO_BONUS;
+                f += -0.1f / sqrtf(r_sq);
+              }
+              ax += f * ddx;
+              ay += f * ddy;
+              az += f * ddz;
+            }
+          }
+      if (J->f32x) {
+        J->f32x[j] = ax;
+        J->f32y[j] = ay;
+        J->f32z[j] = az;
+      }
+      etot += e;
+    }
+    J->e32[c] = etot;
+  }
+  return NULL;
+}
+
+int or_baseline_grid_ef(const or_cloud* ag, const or_cloud* atoms, const uint32_t* cand_off,
+                        size_t n_cand, float cutoff, float* e, float* fx, float* fy,
+                        float* fz, int nthreads) {
+  or_job J;
+  memset(&J, 0, sizeof(J));
+  J.ag = ag;
+  J.atoms = atoms;
+  J.cand_off = cand_off;
+  J.n_cand = n_cand;
+  J.cutoff = cutoff;
+  J.e32 = e;
+  J.f32x = fx;
+  J.f32y = fy;
+  J.f32z = fz;
+  J.inv = 1.0f / cutoff;
+  int32_t kmin[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, kmax[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  int32_t* keys = (int32_t*)malloc(3 * (ag->n ? ag->n : 1) * sizeof(int32_t));
+  if (!keys) return -1;
+  for (size_t i = 0; i < ag->n; ++i) {
+    const float p[3] = {ag->x[i], ag->y[i], ag->z[i]};
+    for (int a = 0; a < 3; ++a) {
+      keys[3 * i + a] = or_cell_of(p[a], J.inv);
+      if (keys[3 * i + a] < kmin[a]) kmin[a] = keys[3 * i + a];
+      if (keys[3 * i + a] > kmax[a]) kmax[a] = keys[3 * i + a];
+    }
+  }
+  size_t ncell = 1;
+  for (int a = 0; a < 3; ++a) {
+    J.kmin[a] = ag->n ? kmin[a] : 0;
+    J.kdim[a] = ag->n ? kmax[a] - kmin[a] + 1 : 0;
+    ncell *= (size_t)(J.kdim[a] > 0 ? J.kdim[a] : 0);
+  }
+  uint32_t* cstart = (uint32_t*)calloc(ncell + 1, sizeof(uint32_t));
+  uint32_t* cidx = (uint32_t*)malloc((ag->n ? ag->n : 1) * sizeof(uint32_t));
+  if (!cstart || !cidx) {
+    free(keys);
+    free(cstart);
+    free(cidx);
+    return -1;
+  }
+  for (size_t i = 0; i < ag->n; ++i) {
+    size_t cell = ((size_t)(keys[3 * i + 2] - J.kmin[2]) * J.kdim[1] + (keys[3 * i + 1] - J.kmin[1])) *
+                      J.kdim[0] + (keys[3 * i] - J.kmin[0]);
+    cstart[cell + 1]++;
+  }
+  for (size_t c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
+  uint32_t* fill = (uint32_t*)calloc(ncell + 1, sizeof(uint32_t));
+  for (size_t i = 0; i < ag->n; ++i) {
+    size_t cell = ((size_t)(keys[3 * i + 2] - J.kmin[2]) * J.kdim[1] + (keys[3 * i + 1] - J.kmin[1])) *
+                      J.kdim[0] + (keys[3 * i] - J.kmin[0]);
+    cidx[cstart[cell] + fill[cell]++] = (uint32_t)i;
+  }
+  J.cstart = cstart;
+  J.cidx = cidx;
+  run_threads(&J, nthreads, worker_grid);
+  free(fill);
+  free(keys);
+  free(cstart);
+  free(cidx);
+  return 0;
+}
+
+/* ---- residue-level forces ---------------------------------------------------- */
+
+typedef struct {
+  size_t n;
+  const float *x, *y, *z, *q, *eps, *sig;
+  const uint8_t* hyd;
+} or_rcloud;
+
+static void cg_pair(const or_rcloud* ag, size_t i, float bx, float by, float bz, float bq,
+                    float be, float bs, uint8_t bh, float* ax, float* ay, float* az) {
+  float dx = bx - ag->x[i];
+  float dy = by - ag->y[i];
+  float dz = bz - ag->z[i];
+  float r_sq = dx * dx + dy * dy + dz * dz;
+  if (r_sq > 100.0f || r_sq < MIN_R_SQ) return;  /* src/energy.rs:358 */
+  float eps_ij = sqrtf(be * ag->eps[i]);
+  float sig_ij = 0.5f * (bs + ag->sig[i]);
+  float sigma_sq_ij = sig_ij * sig_ij;
+  float s2 = sigma_sq_ij / r_sq;
+  float s6 = s2 * s2 * s2;
+  float s12 = s6 * s6;
+  float f_lj = (24.0f * eps_ij / r_sq) * (2.0f * s12 - s6);
+  float f_coul = 0.0f;
+  float q2 = ag->q[i];
+  if (bq != 0.0f && q2 != 0.0f) {
+    float r = sqrtf(r_sq);
+    f_coul = COULOMB_K * bq * q2 / (r_sq * r);
+  }
+  float f_hydro = 0.0f;
+  if (bh == 1 && ag->hyd[i] == 1 && r_sq < HYDRO_SQ) f_hydro = -0.1f / sqrtf(r_sq);
+  float f_total = f_lj + f_coul + f_hydro;
+  *ax += f_total * dx;
+  *ay += f_total * dy;
+  *az += f_total * dz;
+}
+
+/* compute_forces (src/energy.rs:154-234) */
+void or_cg_forces(const or_rcloud* ag, const or_rcloud* ab, float* fx, float* fy, float* fz) {
+  for (size_t j = 0; j < ab->n; ++j) {
+    float ax = 0, ay = 0, az = 0;
+    for (size_t i = 0; i < ag->n; ++i)
+      cg_pair(ag, i, ab->x[j], ab->y[j], ab->z[j], ab->q[j], ab->eps[j], ab->sig[j], ab->hyd[j],
+              &ax, &ay, &az);
+    fx[j] = ax;
+    fy[j] = ay;
+    fz[j] = az;
+  }
+}
+
+/* compute_forces_with_grid (src/energy.rs:322-398) with a grid of cell 10 and
+ * the query order of src/spatial.rs:44-58. */
+static void cg_point_grid(const or_rcloud* ag, const int32_t* keys, float inv, float bx,
+                          float by, float bz, float bq, float be, float bs, uint8_t bh,
+                          float* fx, float* fy, float* fz) {
+  float ax = 0, ay = 0, az = 0;
+  int32_t cx = or_cell_of(bx, inv), cy = or_cell_of(by, inv), cz = or_cell_of(bz, inv);
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        int32_t kx = wrap_add(cx, dx), ky = wrap_add(cy, dy), kz = wrap_add(cz, dz);
+        for (size_t i = 0; i < ag->n; ++i)
+          if (keys[3 * i] == kx && keys[3 * i + 1] == ky && keys[3 * i + 2] == kz)
+            cg_pair(ag, i, bx, by, bz, bq, be, bs, bh, &ax, &ay, &az);
+      }
+  *fx = ax;
+  *fy = ay;
+  *fz = az;
+}
+
+int or_cg_forces_with_grid(const or_rcloud* ag, float cell, const or_rcloud* ab, float* fx,
+                           float* fy, float* fz) {
+  float inv = 1.0f / cell;
+  int32_t* keys = (int32_t*)malloc(3 * (ag->n ? ag->n : 1) * sizeof(int32_t));
+  if (!keys) return -1;
+  for (size_t i = 0; i < ag->n; ++i) {
+    keys[3 * i] = or_cell_of(ag->x[i], inv);
+    keys[3 * i + 1] = or_cell_of(ag->y[i], inv);
+    keys[3 * i + 2] = or_cell_of(ag->z[i], inv);
+  }
+  for (size_t j = 0; j < ab->n; ++j)
+    cg_point_grid(ag, keys, inv, ab->x[j], ab->y[j], ab->z[j], ab->q[j], ab->eps[j], ab->sig[j],
+                  ab->hyd[j], &fx[j], &fy[j], &fz[j]);
+  free(keys);
+  return 0;
+}
+
+/* ---- the gradient step --------------------------------------------------------- */
+
+typedef struct { uint32_t v[4]; } or_philox4;
+
+static uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+
+/* Philox4x32-10, Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    uint32_t h0 = mulhi32(0xD2511F53u, c0), l0 = 0xD2511F53u * c0;
+    uint32_t h1 = mulhi32(0xCD9E8D57u, c2), l1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = h1 ^ c1 ^ k0, n1 = l1, n2 = h0 ^ c3 ^ k1, n3 = l0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* noise scale at iteration it: NOISE_BASE * sqrt(T_HOT (T_COLD/T_HOT)^(it/total)) */
+float or_sigma(uint32_t it, uint32_t total, float t_hot, float t_cold, float noise_base) {
+  float temp = t_hot * powf(t_cold / t_hot, (float)it / (float)total);
+  return noise_base * sqrtf(temp);
+}
+
+static float clampf_rust(float v, float lo, float hi) {
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
+  return v;
+}
+
+typedef struct {
+  float step, max_disp, restraint_k, restraint_radius, sigma;
+  int with_noise, force_model; /* 0 all-atom (forces given), 1 residue level */
+  uint32_t iter;
+  uint32_t seed_lo, seed_hi;
+  float center[3];
+} or_step_params;
+
+/* One step for a set of residues. x/y/z are all atoms (updated in place);
+ * res_off[n_res+1] atom ranges, res_ca[n_res] Calpha atom index; afx/afy/afz
+ * per-atom forces (all-atom model) or the residue-level inputs (cg model:
+ * ag_ca, res_q/eps/sig/hyd). res_id0 is the global index of residue 0 (the
+ * noise counter). */
+void or_langevin_step(float* x, float* y, float* z, size_t n_res, const uint32_t* res_off,
+                      const uint32_t* res_ca, const float* afx, const float* afy,
+                      const float* afz, const or_rcloud* ag_ca, const float* res_q,
+                      const float* res_eps, const float* res_sig, const uint8_t* res_hyd,
+                      uint32_t res_id0, const or_step_params* p) {
+  int32_t* keys = NULL;
+  float inv = 1.0f / 10.0f;
+  if (p->force_model == 1 && ag_ca && ag_ca->n) {
+    keys = (int32_t*)malloc(3 * ag_ca->n * sizeof(int32_t));
+    for (size_t i = 0; i < ag_ca->n; ++i) {
+      keys[3 * i] = or_cell_of(ag_ca->x[i], inv);
+      keys[3 * i + 1] = or_cell_of(ag_ca->y[i], inv);
+      keys[3 * i + 2] = or_cell_of(ag_ca->z[i], inv);
+    }
+  }
+  for (size_t r = 0; r < n_res; ++r) {
+    uint32_t ca = res_ca[r];
+    float px = x[ca], py = y[ca], pz = z[ca];
+    float fx = 0, fy = 0, fz = 0;
+    if (p->force_model == 1) {
+      if (keys)
+        cg_point_grid(ag_ca, keys, inv, px, py, pz, res_q[r], res_eps[r], res_sig[r], res_hyd[r],
+                      &fx, &fy, &fz);
+    } else {
+      for (uint32_t k = res_off[r]; k < res_off[r + 1]; ++k) {
+        fx += afx[k];
+        fy += afy[k];
+        fz += afz[k];
+      }
+    }
+    float dx_c = px - p->center[0], dy_c = py - p->center[1], dz_c = pz - p->center[2];
+    float d = sqrtf(dx_c * dx_c + dy_c * dy_c + dz_c * dz_c);
+    if (d < 0.001f) d = 0.001f;
+    float excess = d - p->restraint_radius;
+    float f_r = -p->restraint_k * excess / d;
+    fx += f_r * dx_c;
+    fy += f_r * dy_c;
+    fz += f_r * dz_c;
+    float nx = 0, ny = 0, nz = 0;
+    if (p->with_noise) {
+      uint32_t ctr[4] = {p->iter, res_id0 + (uint32_t)r, 0x6e6f6973u, 0u};
+      uint32_t key[2] = {p->seed_lo, p->seed_hi}, u[4];
+      or_philox4x32_10(ctr, key, u);
+      float two = p->sigma + p->sigma;
+      nx = (float)(u[0] >> 8) * (1.0f / 16777216.0f) * two - p->sigma;
+      ny = (float)(u[1] >> 8) * (1.0f / 16777216.0f) * two - p->sigma;
+      nz = (float)(u[2] >> 8) * (1.0f / 16777216.0f) * two - p->sigma;
+    }
+    float new_x = px + clampf_rust(p->step * fx, -p->max_disp, p->max_disp) + nx;
+    float new_y = py + clampf_rust(p->step * fy, -p->max_disp, p->max_disp) + ny;
+    float new_z = pz + clampf_rust(p->step * fz, -p->max_disp, p->max_disp) + nz;
+    float ddx = new_x - px, ddy = new_y - py, ddz = new_z - pz;
+    for (uint32_t k = res_off[r]; k < res_off[r + 1]; ++k) {
+      x[k] += ddx;
+      y[k] += ddy;
+      z[k] += ddz;
+    }
+  }
+  free(keys);
+}

@@ -1,0 +1,258 @@
+"""ctypes loader for the CPU oracle (liboracle.so, built from dnab_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product. Each function states the
+reference lines it restates (see dnab_oracle.c).
+
+Cloud arguments are any objects with numpy attributes x, y, z, charge,
+r_min_half, epsilon, hydrophobic (atom clouds) or x, y, z, charge, epsilon,
+sigma, hydrophobic (residue clouds).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+u8p = C.POINTER(C.c_uint8)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+
+
+class OrCloud(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("x", f32p), ("y", f32p), ("z", f32p), ("q", f32p),
+                ("rmh", f32p), ("eps", f32p), ("hyd", u8p)]
+
+
+class OrRCloud(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("x", f32p), ("y", f32p), ("z", f32p), ("q", f32p),
+                ("eps", f32p), ("sig", f32p), ("hyd", u8p)]
+
+
+class OrStepParams(C.Structure):
+    _fields_ = [("step", C.c_float), ("max_disp", C.c_float), ("restraint_k", C.c_float),
+                ("restraint_radius", C.c_float), ("sigma", C.c_float), ("with_noise", C.c_int),
+                ("force_model", C.c_int), ("iter", C.c_uint32), ("seed_lo", C.c_uint32),
+                ("seed_hi", C.c_uint32), ("center", C.c_float * 3)]
+
+
+def _load():
+    if not LIB.exists():
+        raise ImportError(f"{LIB} not built: run `make -C {HERE}`")
+    lib = C.CDLL(str(LIB))
+    protos = {
+        "or_cell_of": (C.c_int32, [C.c_float, C.c_float]),
+        "or_inv_cell": (C.c_float, [C.c_float]),
+        "or_grid_query": (C.c_int, [f32p, f32p, f32p, C.c_size_t, C.c_float, f32p, f32p, f32p,
+                                    C.c_size_t, u32p, u32p, C.c_size_t]),
+        "or_energy_f32": (None, [C.POINTER(OrCloud), C.POINTER(OrCloud), C.c_float, f32p]),
+        "or_energy_f64": (None, [C.POINTER(OrCloud), C.POINTER(OrCloud), C.c_float, f64p, f64p,
+                                 f64p, f64p, u64p]),
+        "or_population_f64": (None, [C.POINTER(OrCloud), C.POINTER(OrCloud), u32p, C.c_size_t,
+                                     C.c_float, f64p, f64p, f64p, f64p, f64p, f64p, f64p, u64p,
+                                     C.c_int]),
+        "or_baseline_energy_brute": (None, [C.POINTER(OrCloud), C.POINTER(OrCloud), u32p,
+                                            C.c_size_t, C.c_float, f32p, C.c_int]),
+        "or_baseline_grid_ef": (C.c_int, [C.POINTER(OrCloud), C.POINTER(OrCloud), u32p,
+                                          C.c_size_t, C.c_float, f32p, f32p, f32p, f32p,
+                                          C.c_int]),
+        "or_cg_forces": (None, [C.POINTER(OrRCloud), C.POINTER(OrRCloud), f32p, f32p, f32p]),
+        "or_cg_forces_with_grid": (C.c_int, [C.POINTER(OrRCloud), C.c_float,
+                                             C.POINTER(OrRCloud), f32p, f32p, f32p]),
+        "or_philox4x32_10": (None, [u32p, u32p, u32p]),
+        "or_sigma": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+        "or_langevin_step": (None, [f32p, f32p, f32p, C.c_size_t, u32p, u32p, f32p, f32p, f32p,
+                                    C.POINTER(OrRCloud), f32p, f32p, f32p, u8p, C.c_uint32,
+                                    C.POINTER(OrStepParams)]),
+    }
+    for name, (res, args) in protos.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _p(a, ct):
+    if a is None:
+        return C.cast(None, C.POINTER(ct))
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def cloud(c) -> tuple[OrCloud, list]:
+    keep = [_f32(c.x), _f32(c.y), _f32(c.z), _f32(c.charge), _f32(c.r_min_half),
+            _f32(c.epsilon), np.ascontiguousarray(c.hydrophobic, np.uint8)]
+    oc = OrCloud(len(keep[0]), _p(keep[0], C.c_float), _p(keep[1], C.c_float),
+                 _p(keep[2], C.c_float), _p(keep[3], C.c_float), _p(keep[4], C.c_float),
+                 _p(keep[5], C.c_float), _p(keep[6], C.c_uint8))
+    return oc, keep
+
+
+def rcloud(c) -> tuple[OrRCloud, list]:
+    keep = [_f32(c.x), _f32(c.y), _f32(c.z), _f32(c.charge), _f32(c.epsilon), _f32(c.sigma),
+            np.ascontiguousarray(c.hydrophobic, np.uint8)]
+    oc = OrRCloud(len(keep[0]), *[_p(k, C.c_float) for k in keep[:6]], _p(keep[6], C.c_uint8))
+    return oc, keep
+
+
+def cell_of(x: float, cell: float) -> int:
+    return int(lib.or_cell_of(x, lib.or_inv_cell(cell)))
+
+
+def grid_query(x, y, z, cell, px, py, pz):
+    """Neighbour lists by the definition of src/spatial.rs:31-58."""
+    x, y, z, px, py, pz = (_f32(np.atleast_1d(v)) for v in (x, y, z, px, py, pz))
+    off = np.zeros(len(px) + 1, np.uint32)
+    lib.or_grid_query(_p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), len(x), cell,
+                      _p(px, C.c_float), _p(py, C.c_float), _p(pz, C.c_float), len(px),
+                      _p(off, C.c_uint32), _p(None, C.c_uint32), 0)
+    idx = np.zeros(max(int(off[-1]), 1), np.uint32)
+    lib.or_grid_query(_p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), len(x), cell,
+                      _p(px, C.c_float), _p(py, C.c_float), _p(pz, C.c_float), len(px),
+                      _p(off, C.c_uint32), _p(idx, C.c_uint32), len(idx))
+    return off, idx[:int(off[-1])]
+
+
+def energy_f32(ag, ab, cutoff=10.0):
+    """(lj, coulomb, hydrophobic) in f32, reference order (src/energy.rs:269-301)."""
+    a, ka = cloud(ag)
+    b, kb = cloud(ab)
+    out = np.zeros(3, np.float32)
+    lib.or_energy_f32(C.byref(a), C.byref(b), cutoff, _p(out, C.c_float))
+    return out
+
+
+def energy_f64(ag, ab, cutoff=10.0):
+    """FP64 (lj, coulomb, hydrophobic), per-atom FP64 forces, pair count."""
+    a, ka = cloud(ag)
+    b, kb = cloud(ab)
+    n = len(kb[0])
+    out = np.zeros(3, np.float64)
+    fx, fy, fz = (np.zeros(max(n, 1), np.float64) for _ in range(3))
+    np_ = C.c_uint64()
+    lib.or_energy_f64(C.byref(a), C.byref(b), cutoff, _p(out, C.c_double), _p(fx, C.c_double),
+                      _p(fy, C.c_double), _p(fz, C.c_double), C.byref(np_))
+    return out, (fx[:n], fy[:n], fz[:n]), np_.value
+
+
+def population_f64(ag, atoms, cand_atom_off, cutoff=10.0, forces=True, nthreads=None):
+    """Per candidate FP64 energy + terms, per atom FP64 forces, per candidate pairs."""
+    nthreads = nthreads or os.cpu_count() or 1
+    a, ka = cloud(ag)
+    b, kb = cloud(atoms)
+    off = np.ascontiguousarray(cand_atom_off, np.uint32)
+    nc = len(off) - 1
+    na = len(kb[0])
+    e, lj, co, hy = (np.zeros(max(nc, 1), np.float64) for _ in range(4))
+    pairs = np.zeros(max(nc, 1), np.uint64)
+    f = [np.zeros(max(na, 1), np.float64) for _ in range(3)] if forces else [None] * 3
+    lib.or_population_f64(C.byref(a), C.byref(b), _p(off, C.c_uint32), nc, cutoff,
+                          _p(e, C.c_double), _p(lj, C.c_double), _p(co, C.c_double),
+                          _p(hy, C.c_double), *[_p(v, C.c_double) for v in f],
+                          _p(pairs, C.c_uint64), int(nthreads))
+    res = {"energy": e[:nc], "lj": lj[:nc], "coulomb": co[:nc], "hydrophobic": hy[:nc],
+           "pairs": pairs[:nc]}
+    if forces:
+        res["forces"] = tuple(v[:na] for v in f)
+    return res
+
+
+def baseline_energy_brute(ag, atoms, cand_atom_off, cutoff, nthreads):
+    a, ka = cloud(ag)
+    b, kb = cloud(atoms)
+    off = np.ascontiguousarray(cand_atom_off, np.uint32)
+    e = np.zeros(max(len(off) - 1, 1), np.float32)
+    lib.or_baseline_energy_brute(C.byref(a), C.byref(b), _p(off, C.c_uint32), len(off) - 1,
+                                 cutoff, _p(e, C.c_float), int(nthreads))
+    return e[:len(off) - 1]
+
+
+def baseline_grid_ef(ag, atoms, cand_atom_off, cutoff, nthreads, forces=False):
+    a, ka = cloud(ag)
+    b, kb = cloud(atoms)
+    off = np.ascontiguousarray(cand_atom_off, np.uint32)
+    nc = len(off) - 1
+    e = np.zeros(max(nc, 1), np.float32)
+    na = len(kb[0])
+    f = [np.zeros(max(na, 1), np.float32) for _ in range(3)] if forces else [None] * 3
+    rc = lib.or_baseline_grid_ef(C.byref(a), C.byref(b), _p(off, C.c_uint32), nc, cutoff,
+                                 _p(e, C.c_float), *[_p(v, C.c_float) for v in f], int(nthreads))
+    assert rc == 0
+    return (e[:nc], tuple(v[:na] for v in f)) if forces else e[:nc]
+
+
+def cg_forces(ag, ab):
+    a, ka = rcloud(ag)
+    b, kb = rcloud(ab)
+    n = len(kb[0])
+    f = [np.zeros(max(n, 1), np.float32) for _ in range(3)]
+    lib.or_cg_forces(C.byref(a), C.byref(b), *[_p(v, C.c_float) for v in f])
+    return tuple(v[:n] for v in f)
+
+
+def cg_forces_with_grid(ag, ab, cell=10.0):
+    a, ka = rcloud(ag)
+    b, kb = rcloud(ab)
+    n = len(kb[0])
+    f = [np.zeros(max(n, 1), np.float32) for _ in range(3)]
+    lib.or_cg_forces_with_grid(C.byref(a), cell, C.byref(b), *[_p(v, C.c_float) for v in f])
+    return tuple(v[:n] for v in f)
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib.or_philox4x32_10(_p(c, C.c_uint32), _p(k, C.c_uint32), _p(o, C.c_uint32))
+    return o
+
+
+def sigma(it, total, t_hot=3.0, t_cold=0.02, noise_base=0.25):
+    return float(lib.or_sigma(it, total, t_hot, t_cold, noise_base))
+
+
+def langevin_step(x, y, z, res_off, res_ca, params: dict, forces=None, ag_ca=None, res_cg=None,
+                  res_id0=0):
+    """In-place step on float32 arrays x, y, z (src/diffusion.rs:300-339)."""
+    p = OrStepParams()
+    for k, v in params.items():
+        if k == "center":
+            for i in range(3):
+                p.center[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    res_off = np.ascontiguousarray(res_off, np.uint32)
+    res_ca = np.ascontiguousarray(res_ca, np.uint32)
+    n_res = len(res_ca)
+    fx = fy = fz = None
+    if forces is not None:
+        fx, fy, fz = (_f32(v) for v in forces)
+    keep = []
+    agp = C.cast(None, C.POINTER(OrRCloud))
+    rq = re = rs = None
+    rh = None
+    if ag_ca is not None:
+        oc, k1 = rcloud(ag_ca)
+        keep.append((oc, k1))
+        agp = C.pointer(oc)
+        rq, re, rs = (_f32(v) for v in res_cg[:3])
+        rh = np.ascontiguousarray(res_cg[3], np.uint8)
+    lib.or_langevin_step(_p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), n_res,
+                         _p(res_off, C.c_uint32), _p(res_ca, C.c_uint32), _p(fx, C.c_float),
+                         _p(fy, C.c_float), _p(fz, C.c_float), agp, _p(rq, C.c_float),
+                         _p(re, C.c_float), _p(rs, C.c_float), _p(rh, C.c_uint8), res_id0,
+                         C.byref(p))

@@ -1,0 +1,186 @@
+"""GPU all-atom energies and forces vs the oracle.
+
+Tolerance (written here, applied per candidate energy and per atom-force
+component): |x - x64| <= 1e-3 + 1e-4 |x64| (kcal/mol, kcal/mol/A), x64 the
+oracle's FP64 value on the same f32 inputs and the same f32 pair predicate.
+The FP64 GPU kernel must agree with the CPU FP64 oracle to 1e-9 relative.
+Pair counts and hydrophobic terms are integer-exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from dnab import DeviceAntigen, DevicePopulation
+from dnab import workloads as W
+from dnab.structure import AtomCloud, population_from_proteins, protein_from_ca_trace, \
+    aa_from_sequence
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(x, x64):
+    return np.abs(np.asarray(x, np.float64) - x64) <= 1e-3 + 1e-4 * np.abs(x64)
+
+
+def cloud(xyz, q, rmh, eps, hyd):
+    xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+    n = len(xyz)
+    f = lambda v: np.broadcast_to(np.asarray(v, np.float32), (n,)).copy()
+    return AtomCloud(xyz[:, 0], xyz[:, 1], xyz[:, 2], f(q), f(rmh), f(eps),
+                     np.broadcast_to(np.asarray(hyd, np.uint8), (n,)).copy())
+
+
+def test_two_atom_closed_forms(ctx):
+    R = float(np.float32(1.908) + np.float32(1.824))
+    ag = cloud([0, 0, 0], 0.8, 1.908, 0.1094, 1)
+    ab = cloud([R, 0, 0], -0.5, 1.824, 0.17, 1)
+    b = ctx.interaction_energy_atoms_breakdown(ag, ab)
+    eps = np.sqrt(np.float64(np.float32(0.1094)) * np.float64(np.float32(0.17)))
+    assert abs(b.lj - (-eps)) < 1e-6
+    assert abs(b.coulomb - 332.0 * float(np.float32(0.8)) * float(np.float32(-0.5)) / R) < 1e-5
+    assert b.hydrophobic == -0.5 and b.hbond == 0.0 and b.solvation == 0.0
+    fx, fy, fz, e = ctx.atom_forces(ag, ab)
+    assert fx[0] < 0          # attraction dominates at the LJ minimum
+    # cutoff edge: r^2 == 100 is in, just beyond is out; r^2 < 0.25 is out
+    ab10 = cloud([10.0, 0, 0], -0.5, 1.824, 0.17, 0)
+    assert ctx.interaction_energy_atoms_breakdown(ag, ab10).coulomb != 0.0
+    ab_out = cloud([float(np.nextafter(np.float32(10), np.float32(11))), 0, 0], -0.5, 1.824, 0.17, 0)
+    assert ctx.interaction_energy_atoms_breakdown(ag, ab_out).coulomb == 0.0
+    ab_in = cloud([0.49, 0, 0], -0.5, 1.824, 0.17, 0)
+    assert ctx.interaction_energy_atoms(ag, ab_in) == 0.0
+
+
+def test_atom_breakdown_sums_to_total(ctx):
+    # src/energy.rs:496-507
+    ag = protein_from_ca_trace([0.0], [0.0], [0.0], aa_from_sequence("D"))
+    ab = protein_from_ca_trace([3.0], [0.0], [0.0], aa_from_sequence("K"))
+    b = ctx.interaction_energy_atoms_breakdown(ag.atoms, ab.atoms)
+    total = ctx.interaction_energy_atoms(ag.atoms, ab.atoms)
+    assert abs(b.total() - total) < 1e-4
+    e64 = O.energy_f64(ag.atoms, ab.atoms)[0]
+    assert tol(total, e64.sum())
+
+
+def eval_vs_oracle(ctx, w, n_check=None):
+    ags, pop = W.device_objects(ctx, w)
+    pop.evaluate(forces=True)
+    e, err = pop.energies()
+    fx, fy, fz = pop.forces()
+    off = w.pop.cand_atom_off()
+    assert len(w.antigens) == 1
+    r = O.population_f64(w.antigens[0].atoms, w.pop.atoms, off, w.cutoff)
+    return ags, pop, e, err, (fx, fy, fz), r
+
+
+@pytest.mark.parametrize("make", [lambda: W.config1(), lambda: W.config2(n_cand=96)],
+                         ids=["c1", "c2-96"])
+def test_energy_force_parity(ctx, make):
+    w = make()
+    ags, pop, e, err, f, r = eval_vs_oracle(ctx, w)
+    ok = tol(e, r["energy"])
+    assert ok.all(), (np.nonzero(~ok)[0][:5], e[~ok][:5], r["energy"][~ok][:5])
+    for a, b in zip(f, r["forces"]):
+        okf = tol(a, b)
+        assert okf.all(), (np.nonzero(~okf)[0][:5], a[~okf][:5], b[~okf][:5])
+    n_in, n_tested = pop.pair_count()
+    assert n_in == int(r["pairs"].sum())
+    assert n_tested >= n_in
+    print(f"pairs in cutoff {n_in}, tested {n_tested}, "
+          f"max |E-E64| {np.abs(e - r['energy']).max():.3g}")
+
+
+def test_breakdown_and_fp64_kernel(ctx):
+    w = W.config2(n_cand=64)
+    ags, pop = W.device_objects(ctx, w)
+    r = O.population_f64(w.antigens[0].atoms, w.pop.atoms, w.pop.cand_atom_off(), w.cutoff)
+    pop.evaluate(forces=False, breakdown=True)
+    lj, co, hy = pop.breakdown()
+    assert tol(lj, r["lj"]).all() and tol(co, r["coulomb"]).all()
+    assert np.array_equal(hy, r["hydrophobic"])
+    pop.evaluate(fp64=True)
+    e64, _ = pop.energies()
+    lj64, co64, hy64 = pop.breakdown()
+    scale = 1.0 + np.abs(r["lj"]) + np.abs(r["coulomb"])
+    assert (np.abs(e64 - r["energy"]) <= 1e-9 * scale).all()
+    assert (np.abs(lj64 - r["lj"]) <= 1e-9 * scale).all()
+    assert np.array_equal(hy64, r["hydrophobic"])
+    f64 = pop.forces_f64()
+    for a, b in zip(f64, r["forces"]):
+        assert (np.abs(a - b) <= 1e-9 * (1.0 + np.abs(b))).all()
+
+
+def test_deterministic_and_tile_independent(ctx):
+    w = W.config2(n_cand=128)
+    ags, pop = W.device_objects(ctx, w)
+    pop.evaluate(forces=True)
+    e1, _ = pop.energies()
+    f1 = pop.forces()
+    pop.evaluate(forces=True)
+    e2, _ = pop.energies()
+    assert np.array_equal(e1.view(np.uint64), e2.view(np.uint64))
+    # the same candidates in another order land in other tiles
+    perm = np.random.default_rng(0).permutation(w.pop.n_cand)
+    prots = []
+    off = w.pop.cand_atom_off()
+    for c in perm:
+        r0, r1 = w.pop.cand_res_off[c], w.pop.cand_res_off[c + 1]
+        cl = w.pop.candidate_cloud(int(c))
+        res_start = (w.pop.res_atom_off[r0:r1 + 1] - off[c]).astype(np.uint32)
+        from dnab.structure import AtomProtein
+        prots.append(AtomProtein(cl, res_start, w.pop.res_amino_acid[r0:r1].copy(),
+                                 (w.pop.res_ca_atom[r0:r1] - off[c]).astype(np.uint32)))
+    pop2 = DevicePopulation(ctx, ags, population_from_proteins(prots))
+    pop2.evaluate(forces=True)
+    e3, _ = pop2.energies()
+    assert np.array_equal(e3.view(np.uint64), e1[perm].view(np.uint64))
+    # single-candidate scoring of the same atoms gives the same bits (ragged == single)
+    for c in (0, 7, 31):
+        single = ctx.interaction_energy_atoms(w.antigens[0].atoms, w.pop.candidate_cloud(c),
+                                              cutoff=w.cutoff)
+        assert np.float64(single).view(np.uint64) == e1[c].view(np.uint64)
+
+
+def test_score_batch_mirrors_reference(ctx):
+    w = W.config2(n_cand=48)
+    ag = w.antigens[0].atoms
+    cand = w.pop.atoms.gpu_atoms()
+    n_ab = int(w.pop.cand_atom_off()[1])     # uniform-length view, as src/gpu.rs:166
+    n_cand = len(cand) // n_ab
+    e = ctx.score_batch(ag, cand[: n_cand * n_ab], n_ab)
+    assert e.dtype == np.float32 and len(e) == n_cand
+    off = np.arange(n_cand + 1, dtype=np.uint32) * n_ab
+    sub = w.pop.atoms.subset(slice(0, n_cand * n_ab))
+    r = O.population_f64(ag, sub, off, 10.0)
+    assert tol(e, r["energy"]).all()
+    assert len(ctx.score_batch(ag, cand, 0)) == 0          # src/gpu.rs:172-174
+    assert len(ctx.score_batch(ag, cand[:0], n_ab)) == 0
+
+
+def test_empty_inputs(ctx):
+    ab = cloud([[0, 0, 0], [1, 1, 1]], 0.5, 1.9, 0.1, 1)
+    empty = cloud(np.zeros((0, 3)), 0, 1.9, 0.1, 0)
+    assert ctx.interaction_energy_atoms(empty, ab) == 0.0
+    assert ctx.interaction_energy_atoms(ab, empty) == 0.0
+    fx, fy, fz, e = ctx.atom_forces(empty, ab)
+    assert (fx == 0).all() and e == 0.0
+    # a population with a candidate of zero residues
+    w = W.config2(n_cand=4)
+    p = w.pop
+    from dnab.structure import PopulationHost
+    host = PopulationHost(np.zeros(5, np.uint32),
+                          np.concatenate([p.cand_res_off[:3], p.cand_res_off[2:]]).astype(np.uint32),
+                          p.res_atom_off, p.res_ca_atom, p.res_amino_acid, p.atoms)
+    ags = [DeviceAntigen(ctx, w.antigens[0], cutoff=10.0)]
+    pop = DevicePopulation(ctx, ags, host)
+    pop.evaluate()
+    e, _ = pop.energies()
+    assert e[2] == 0.0
+
+
+def test_err_estimate_reported(ctx):
+    w = W.config2(n_cand=64)
+    ags, pop, e, err, f, r = eval_vs_oracle(ctx, w)
+    actual = np.abs(e - r["energy"])
+    assert (err >= 0).all()
+    print(f"max actual {actual.max():.3g}, max estimate {err.max():.3g}, "
+          f"frac actual<=estimate {(actual <= err + 1e-9).mean():.3f}")

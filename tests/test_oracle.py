@@ -1,0 +1,297 @@
+"""CPU tests that pin the oracle (oracle/dnab_oracle.c) independently of it:
+closed forms, the reference's own unit tests restated, brute-force set
+definitions in numpy, finite differences, published known-answer vectors."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from dnab.structure import AtomCloud, ResidueCloud, protein_from_ca_trace, aa_from_sequence
+
+A = aa_from_sequence
+
+
+def cloud(xyz, q, rmh, eps, hyd):
+    xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+    n = len(xyz)
+    f = lambda v: np.broadcast_to(np.asarray(v, np.float32), (n,)).copy()
+    return AtomCloud(xyz[:, 0], xyz[:, 1], xyz[:, 2], f(q), f(rmh), f(eps),
+                     np.broadcast_to(np.asarray(hyd, np.uint8), (n,)).copy())
+
+
+# ---- SpatialHashGrid keys and lists (src/spatial.rs) -------------------------
+
+def np_keys(x, cell):
+    inv = np.float32(1.0) / np.float32(cell)
+    with np.errstate(invalid="ignore", over="ignore"):
+        p = np.floor(np.asarray(x, np.float32) * inv)
+    out = np.where(np.isnan(p), 0, np.clip(p, -2.0 ** 31, 2.0 ** 31 - 1)).astype(np.int64)
+    return out.astype(np.int32)
+
+
+def test_cell_of_closed_forms():
+    assert O.cell_of(0.0, 10.0) == 0
+    assert O.cell_of(9.999, 10.0) == 0
+    assert O.cell_of(10.0, 10.0) == 1
+    assert O.cell_of(-0.001, 10.0) == -1
+    assert O.cell_of(-10.0, 10.0) == -1
+    assert O.cell_of(float("nan"), 10.0) == 0          # Rust `as i32`: NaN -> 0
+    assert O.cell_of(1e30, 10.0) == 2 ** 31 - 1        # saturating
+    assert O.cell_of(-1e30, 10.0) == -(2 ** 31)
+    assert O.cell_of(float("inf"), 10.0) == 2 ** 31 - 1
+
+
+def test_cell_of_uses_rounded_reciprocal():
+    # fl(1/12) * x differs from x / 12 for some x: the reference multiplies by
+    # the f32 reciprocal (src/spatial.rs:26, 63), so must the oracle.
+    inv = np.float32(1.0) / np.float32(12.0)
+    k = np.arange(1, 20000, dtype=np.float32) * np.float32(12.0)
+    xs = np.concatenate([np.nextafter(k, np.float32(0)), np.nextafter(-k, np.float32(0))])
+    mul = np.floor(xs * inv)
+    div = np.floor(xs / np.float32(12.0))
+    diff = np.nonzero(mul != div)[0]
+    assert len(diff) > 0, "no discriminating value found"
+    for i in diff[:20]:
+        assert O.cell_of(float(xs[i]), 12.0) == int(mul[i])
+    rng = np.random.default_rng(5)
+    xs = rng.uniform(-300, 300, 5000).astype(np.float32)
+    assert [O.cell_of(float(v), 12.0) for v in xs] == list(np_keys(xs, 12.0))
+
+
+def test_query_finds_nearby_atom():
+    # src/spatial.rs:75-87
+    off, idx = O.grid_query([0, 50, 100], [0, 50, 100], [0, 50, 100], 10.0, [1.0], [1.0], [1.0])
+    assert 0 in idx and 1 not in idx
+
+
+@pytest.mark.parametrize("cell", [8.0, 10.0, 12.0, 3.7])
+def test_query_matches_definition(cell):
+    rng = np.random.default_rng(int(cell * 10))
+    p = rng.uniform(-25, 25, (300, 3)).astype(np.float32)
+    q = rng.uniform(-35, 35, (40, 3)).astype(np.float32)
+    off, idx = O.grid_query(p[:, 0], p[:, 1], p[:, 2], cell, q[:, 0], q[:, 1], q[:, 2])
+    kp = np.stack([np_keys(p[:, a], cell) for a in range(3)], 1).astype(np.int64)
+    kq = np.stack([np_keys(q[:, a], cell) for a in range(3)], 1).astype(np.int64)
+    for j in range(len(q)):
+        expect = []
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    c = kq[j] + (dx, dy, dz)
+                    expect += list(np.nonzero((kp == c).all(1))[0])
+        assert list(idx[off[j]:off[j + 1]]) == expect
+        # as a set: every point within one cell in each axis
+        cheb = np.abs(kp - kq[j]).max(1)
+        assert sorted(expect) == list(np.nonzero(cheb <= 1)[0])
+
+
+# ---- all-atom energy closed forms (src/energy.rs:261-314) --------------------
+
+def two_atoms(r, q1=0.0, q2=0.0, h1=0, h2=0, rmh=(1.908, 1.824), eps=(0.1094, 0.17)):
+    ag = cloud([0, 0, 0], q1, rmh[0], eps[0], h1)
+    ab = cloud([r, 0, 0], q2, rmh[1], eps[1], h2)
+    return ag, ab
+
+
+def test_lj_minimum_closed_form():
+    R = np.float32(1.908) + np.float32(1.824)
+    ag, ab = two_atoms(float(R))
+    e32 = O.energy_f32(ag, ab)
+    e64, (fx, fy, fz), n = O.energy_f64(ag, ab)
+    eps = math.sqrt(float(np.float32(0.1094)) * float(np.float32(0.17)))
+    assert n == 1
+    assert e64[0] == pytest.approx(-eps, rel=1e-12)
+    assert abs(e32[0] - (-eps)) < 1e-6
+    # zero slope at R_ij (the f32 position differs from R_ij by < 1 ulp)
+    assert abs(fx[0]) < 1e-6
+    ag2, ab2 = two_atoms(float(R) * 1.1)
+    assert abs(O.energy_f64(ag2, ab2)[1][0][0]) > 1e-2
+    assert e64[1] == 0 and e64[2] == 0
+
+
+def test_coulomb_and_hydrophobic_closed_forms():
+    ag, ab = two_atoms(4.0, q1=0.8, q2=-0.5, h1=1, h2=1)
+    e64, (fx, _, _), _ = O.energy_f64(ag, ab)
+    q1, q2 = float(np.float32(0.8)), float(np.float32(-0.5))
+    assert e64[1] == pytest.approx(332.0 * q1 * q2 / 4.0, rel=1e-12)
+    assert e64[2] == -0.5
+    ag, ab = two_atoms(6.5, h1=1, h2=1)
+    assert O.energy_f64(ag, ab)[0][2] == 0.0      # beyond 6 A: no bonus
+    ag, ab = two_atoms(4.0, h1=1, h2=0)
+    assert O.energy_f64(ag, ab)[0][2] == 0.0      # both atoms must be flagged
+    ag, ab = two_atoms(4.0, q1=0.8, q2=-0.5)
+    assert fx[0] < 0                               # opposite charges attract (src/energy.rs:445)
+
+
+def test_cutoff_predicate_edges():
+    # r^2 == 100 is inside (reference skips r^2 > 100), r^2 < 0.25 is outside.
+    ag, ab = two_atoms(10.0, q1=1.0, q2=1.0)
+    assert O.energy_f64(ag, ab)[2] == 1
+    ag, ab = two_atoms(float(np.nextafter(np.float32(10.0), np.float32(11))), q1=1.0, q2=1.0)
+    assert O.energy_f64(ag, ab)[2] == 0
+    ag, ab = two_atoms(0.5, q1=1.0, q2=1.0)
+    assert O.energy_f64(ag, ab)[2] == 1
+    ag, ab = two_atoms(0.49, q1=1.0, q2=1.0)
+    assert O.energy_f64(ag, ab)[2] == 0
+    ag, ab = two_atoms(11.9, q1=1.0, q2=1.0)
+    assert O.energy_f64(ag, ab, cutoff=12.0)[2] == 1
+
+
+def test_atom_breakdown_sums_to_total():
+    # src/energy.rs:496-507: Asp at the origin, Lys 3 A away.
+    ag = protein_from_ca_trace([0.0], [0.0], [0.0], A("D"))
+    ab = protein_from_ca_trace([3.0], [0.0], [0.0], A("K"))
+    e32 = O.energy_f32(ag.atoms, ab.atoms)
+    e64 = O.energy_f64(ag.atoms, ab.atoms)[0]
+    tot = float(e32[0] + e32[1] + e32[2])
+    assert abs(tot - float(np.float32(e32[0]) + np.float32(e32[1]) + np.float32(e32[2]))) < 1e-4
+    assert abs(tot - e64.sum()) <= 1e-3 + 1e-4 * abs(e64.sum()) * 50  # f32 reference vs FP64
+
+
+def test_f32_mirror_close_to_f64_on_random_system():
+    rng = np.random.default_rng(7)
+    n_ag, n_ab = 300, 60
+    ag = cloud(rng.uniform(-12, 12, (n_ag, 3)), rng.uniform(-0.9, 0.9, n_ag),
+               rng.choice([1.908, 1.824, 1.6612], n_ag), rng.choice([0.086, 0.17, 0.21], n_ag),
+               rng.integers(0, 2, n_ag))
+    ab = cloud(rng.uniform(-12, 12, (n_ab, 3)) + [14, 0, 0], rng.uniform(-0.9, 0.9, n_ab),
+               rng.choice([1.908, 1.824], n_ab), rng.choice([0.086, 0.17], n_ab),
+               rng.integers(0, 2, n_ab))
+    e32 = O.energy_f32(ag, ab)
+    e64 = O.energy_f64(ag, ab)[0]
+    for a, b in zip(e32, e64):
+        assert abs(a - b) <= 1e-2 + 1e-3 * abs(b)
+
+
+def _energy_lj_coul(ag, ab_xyz, base):
+    ab = cloud(ab_xyz, base.charge, base.r_min_half, base.epsilon, 0)
+    e = O.energy_f64(ag, ab)[0]
+    return e[0] + e[1]
+
+
+def test_forces_are_negative_gradient_fd():
+    rng = np.random.default_rng(11)
+    ag = cloud(rng.uniform(-5, 5, (40, 3)), rng.uniform(-0.8, 0.8, 40), 1.908, 0.1094, 0)
+    base = cloud([[7.0, 0.5, -0.3]], 0.6, 1.824, 0.17, 0)
+    p = np.array([[7.0, 0.5, -0.3]])
+    _, (fx, fy, fz), _ = O.energy_f64(ag, base)
+    h = 1e-3
+    grad = []
+    for a in range(3):
+        pp, pm = p.copy(), p.copy()
+        pp[0, a] += h
+        pm[0, a] -= h
+        # coordinates are f32: use an exactly representable step
+        ep = _energy_lj_coul(ag, pp.astype(np.float32), base)
+        em = _energy_lj_coul(ag, pm.astype(np.float32), base)
+        hh = float(np.float32(pp[0, a]) - np.float32(pm[0, a]))
+        grad.append((ep - em) / hh)
+    for f, g in zip((fx[0], fy[0], fz[0]), grad):
+        assert f == pytest.approx(-g, rel=2e-5, abs=2e-6)
+
+
+def test_lj_repulsion_close_range():
+    ag, ab = two_atoms(1.0)
+    _, (fx, _, _), _ = O.energy_f64(ag, ab)
+    assert fx[0] > 0
+
+
+def test_population_f64_matches_single_calls():
+    rng = np.random.default_rng(3)
+    ag = cloud(rng.uniform(-8, 8, (120, 3)), rng.uniform(-0.8, 0.8, 120), 1.908, 0.1094,
+               rng.integers(0, 2, 120))
+    parts = [cloud(rng.uniform(-8, 8, (k, 3)) + [10, 0, 0], rng.uniform(-0.8, 0.8, k), 1.824,
+                   0.17, rng.integers(0, 2, k)) for k in (5, 17, 1, 30)]
+    atoms = AtomCloud.concat(parts)
+    off = np.cumsum([0] + [len(p) for p in parts]).astype(np.uint32)
+    r = O.population_f64(ag, atoms, off, 10.0, nthreads=3)
+    for c, p in enumerate(parts):
+        e = O.energy_f64(ag, p)[0]
+        assert r["energy"][c] == pytest.approx(e.sum(), rel=1e-13, abs=1e-12)
+
+
+# ---- residue-level forces (src/energy.rs:154-234, 322-398, 400-473) ---------
+
+def residue_cloud(points, aas):
+    p = np.asarray(points, np.float32).reshape(-1, 3)
+    return ResidueCloud.from_points(p[:, 0], p[:, 1], p[:, 2], A(aas))
+
+
+def test_cg_grid_forces_match_brute_force():
+    ag = residue_cloud([[0, 0, 0], [5, 0, 0]], "KL")
+    ab = residue_cloud([[3, 0, 0]], "D")
+    bf = O.cg_forces(ag, ab)
+    gr = O.cg_forces_with_grid(ag, ab, 10.0)
+    for a, b in zip(bf, gr):
+        assert abs(a[0] - b[0]) < 1e-4
+    assert bf[0][0] < 0                                # opposite_charges_attract
+    ag = residue_cloud([[0, 0, 0]], "G")
+    ab = residue_cloud([[1, 0, 0]], "G")
+    assert O.cg_forces(ag, ab)[0][0] > 0               # lj_repulsion_at_close_range
+
+
+def test_cg_grid_equals_brute_on_random_points():
+    rng = np.random.default_rng(9)
+    ag = residue_cloud(rng.uniform(-20, 20, (200, 3)), "".join(rng.choice(list("ARNDCQEGHILKMFPSTWYV"), 200)))
+    ab = residue_cloud(rng.uniform(-25, 25, (50, 3)), "".join(rng.choice(list("ARNDCQEGHILKMFPSTWYV"), 50)))
+    bf = O.cg_forces(ag, ab)
+    gr = O.cg_forces_with_grid(ag, ab, 10.0)
+    for a, b in zip(bf, gr):
+        np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-4)
+
+
+# ---- the step's helpers (src/diffusion.rs:102-109, 300-339) ------------------
+
+def test_philox_known_answers():
+    # Random123 philox4x32-10 known-answer vectors (Salmon et al. 2011).
+    assert list(O.philox([0, 0, 0, 0], [0, 0])) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert list(O.philox([0xffffffff] * 4, [0xffffffff] * 2)) == \
+        [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert list(O.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+                         [0xa4093822, 0x299f31d0])) == [0xd16cfe09, 0x94fdcceb, 0x5001e420,
+                                                        0x24126ea1]
+
+
+def test_temperature_schedule_endpoints():
+    assert O.sigma(0, 800) == pytest.approx(0.25 * math.sqrt(3.0), rel=1e-6)
+    assert O.sigma(800, 800) == pytest.approx(0.25 * math.sqrt(0.02), rel=1e-5)
+
+
+def _one_residue(ca=(30.0, 0.0, 0.0)):
+    x = np.array([ca[0], ca[0] + 1.0], np.float32)
+    y = np.array([ca[1], ca[1]], np.float32)
+    z = np.array([ca[2], ca[2]], np.float32)
+    return x, y, z, np.array([0, 2], np.uint32), np.array([0], np.uint32)
+
+
+def base_params(**kw):
+    p = dict(step=0.08, max_disp=2.0, restraint_k=0.02, restraint_radius=20.0, sigma=0.0,
+             with_noise=0, force_model=0, iter=0, seed_lo=0, seed_hi=0, center=(0, 0, 0))
+    p.update(kw)
+    return p
+
+
+def test_step_restraint_closed_form():
+    x, y, z, off, ca = _one_residue((30.0, 0.0, 0.0))
+    zero = np.zeros(2, np.float32)
+    O.langevin_step(x, y, z, off, ca, base_params(), forces=(zero, zero, zero))
+    # F = -0.02 (30 - 20)/30 * 30 = -0.2; dx = clamp(0.08 * -0.2) = -0.016
+    assert x[0] == np.float32(30.0) + np.float32(np.float32(0.08) * np.float32(-0.2))
+    assert x[1] - x[0] == pytest.approx(1.0, abs=1e-6)      # rigid translation
+
+
+def test_step_clamp_and_noise_bounds():
+    x, y, z, off, ca = _one_residue((20.0, 0.0, 0.0))
+    big = np.array([1e6, 0], np.float32)
+    zero = np.zeros(2, np.float32)
+    O.langevin_step(x, y, z, off, ca, base_params(), forces=(big, zero, zero))
+    assert x[0] == np.float32(22.0)                          # clamped at MAX_DISP
+    xs = []
+    for it in range(200):
+        x, y, z, off, ca = _one_residue((20.0, 0.0, 0.0))
+        O.langevin_step(x, y, z, off, ca, base_params(with_noise=1, sigma=0.5, iter=it, seed_lo=7),
+                        forces=(zero, zero, zero))
+        xs.append(float(x[0]) - 20.0)
+    xs = np.array(xs)
+    assert (np.abs(xs) <= 0.5).all() and xs.std() > 0.2      # U(-s, s)
