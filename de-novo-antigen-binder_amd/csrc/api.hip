@@ -79,12 +79,13 @@ int dnab_interaction_energy_atoms_breakdown(dnab_ctx* ctx, const dnab_atom_cloud
 int dnab_interaction_energy_atoms(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
                                   const dnab_atom_cloud* antibody, float cutoff,
                                   double* energy) {
-  if (!ctx || !cloud_ok(antigen) || !cloud_ok(antibody) || !energy) return DNAB_ERR_INVALID;
-  AgGuard ag;
-  PopGuard pop;
-  DNAB_RETURN_IF(single_candidate(ctx, antigen, antibody, cutoff, ag, pop));
-  DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, 0));
-  return dnab_population_read_energies(ctx, pop.p, energy, nullptr);
+  // src/energy.rs:256-258: the breakdown's total(), summed in the order of
+  // src/energy.rs:247-249.
+  if (!energy) return DNAB_ERR_INVALID;
+  dnab_energy_breakdown b{};
+  DNAB_RETURN_IF(dnab_interaction_energy_atoms_breakdown(ctx, antigen, antibody, cutoff, &b));
+  *energy = (((b.lj + b.coulomb) + b.hydrophobic) + b.hbond) + b.solvation;
+  return DNAB_OK;
 }
 
 int dnab_atom_forces(dnab_ctx* ctx, const dnab_atom_cloud* antigen,

@@ -109,12 +109,18 @@ struct AgView {
   float inv;
   float rc;
   float rc2;
-  // candidate-atom sort grid around this antigen
+  // candidate-atom sort grid around this antigen: 4 x 4 x 4-cell bricks in
+  // row-major order, Morton order inside a brick (64 codes per brick)
   float sort_origin[3];
   float sort_inv;
   int32_t sort_dim[3];
-  int sort_bits;
+  int32_t sort_nb[3];
+  uint32_t sort_ncode;
   uint32_t sort_code_base;
+  // atoms farther than rc from this box (the antigen atoms' bounding box)
+  // have no pair and are left out of the tiles
+  float excl_lo[3];
+  float excl_hi[3];
   uint32_t n_atoms;
   // residue-level model: antigen Calpha cloud and its grid
   const float* ca_x;
@@ -124,6 +130,7 @@ struct AgView {
   const float* ca_eps;
   const float* ca_sig;
   const uint8_t* ca_hyd;
+  const uint8_t* ca_aa;  // residue types (features); may be null
   GridView ca_grid;
   float center[3];
   float restraint_radius;  // 0 = use the step parameter
@@ -167,7 +174,8 @@ struct dnab_antigen {
   dnab::DevBuf<float2> raw;
   size_t n_ca = 0;
   dnab::DevBuf<float> ca_x, ca_y, ca_z, ca_q, ca_eps, ca_sig;
-  dnab::DevBuf<uint8_t> ca_hyd;
+  dnab::DevBuf<uint8_t> ca_hyd, ca_aa;
+  bool has_ca_aa = false;
   dnab::GridDev ca_grid;
 };
 
@@ -183,13 +191,17 @@ struct dnab_population {
   dnab::DevBuf<uint32_t> atom_cand;  // per atom
   dnab::DevBuf<uint32_t> cand_atom_off, cand_res_off, cand_ag;
   dnab::DevBuf<uint32_t> res_atom_off, res_ca, res_cand;
-  dnab::DevBuf<uint8_t> res_aa;
+  dnab::DevBuf<uint8_t> res_aa;         // residue types, when given
+  bool has_res_aa = false;
   dnab::DevBuf<float4> res_cg;  // residue-level params (q, eps, sigma, hydrophobic)
   // candidate-atom spatial sort
   uint32_t sort_codes = 0;
   size_t tile_cap = 0;
-  dnab::DevBuf<uint32_t> sort_count;     // sort_codes + 1
+  dnab::DevBuf<uint32_t> sort_count;     // sort_codes, zero between sorts
+  dnab::DevBuf<uint32_t> sort_start;     // sort_codes
   dnab::DevBuf<uint32_t> sort_rank;      // per atom
+  dnab::DevBuf<uint32_t> sort_code;      // per atom (kExcluded when left out)
+  bool counts_pending = false;           // counts filled by the step kernel
   dnab::DevBuf<uint32_t> perm;           // tile_cap * 64 slots -> atom (or ~0u)
   dnab::DevBuf<uint32_t> ag_tile_start;  // n_ag + 1
   bool sorted = false;
@@ -210,6 +222,15 @@ struct dnab_population {
   dnab::DevBuf<unsigned long long> pair_stats;  // (in cutoff, tested)
   dnab::DevBuf<uint32_t> topk_idx;
   dnab::DevBuf<double> topk_e;
+  // candidate mask: only masked-in candidates are sorted and scored
+  dnab::DevBuf<uint8_t> cand_mask;
+  bool masked = false;
+  // prescreen
+  dnab::DevBuf<uint16_t> feat;   // n_cand x 3360 bf16
+  dnab::DevBuf<float> scores;    // n_cand
+  dnab::DevBuf<double> scores64;
+  dnab::DevBuf<uint32_t> keep_idx;
+  dnab::DevBuf<double> keep_e;
 };
 
 // Error helpers (host).
@@ -240,6 +261,7 @@ int dnab_set_error(dnab_ctx* ctx, int code, const char* fmt, ...);
 namespace dnab {
 // internal evaluate flag: the counting variant of the E+F kernel
 constexpr unsigned kEvalCount = 1u << 8;
+constexpr uint32_t kExcluded = 0xFFFFFFFFu;
 // grid.hip
 int grid_build_device(dnab_ctx* ctx, GridDev& g, const float* d_x, const float* d_y,
                       const float* d_z, size_t n);
@@ -248,8 +270,21 @@ hipEvent_t timer_begin(dnab_ctx* ctx, const char* name, hipEvent_t* end_out);
 void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end);
 int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
+// step.hip: multi-pass top-k (ascending, NaN last, ties by index)
+struct TopkPlan {
+  uint32_t k = 0;
+  std::vector<uint32_t> lens;
+  DevBuf<double> e[2];
+  DevBuf<uint32_t> idx[2];
+};
+int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k);
+void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
+                 uint32_t* final_idx);
 // pair.hip
 int population_sort(dnab_ctx* ctx, dnab_population* pop);
 int population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
+int population_pair(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
+int discard_pending_counts(dnab_ctx* ctx, dnab_population* pop);
+int population_count(dnab_ctx* ctx, dnab_population* pop);
 int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
 }  // namespace dnab

@@ -38,14 +38,14 @@
 #include "device_common.h"
 #include "dnab_internal.h"
 #include "grid_device.h"
+#include "sort_device.h"
 
 namespace dnab {
 
 constexpr float kMinRSq = 0.25f;   // src/energy.rs:17
 constexpr float kHydroSq = 36.0f;  // src/energy.rs:20
 constexpr float kSortCell = 4.0f;  // candidate-sort cell edge
-constexpr int kSortMaxBits = 7;
-constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;  // empty tile slot
 constexpr int kTileWaves = 4;
 
 __global__ void k_fill_u32b(uint32_t* p, uint64_t n, uint32_t v) {
@@ -84,82 +84,96 @@ __global__ void k_ag_gather(const uint32_t* __restrict__ sorted, uint32_t n,
 }
 
 // ---- candidate-atom spatial sort ---------------------------------------------------
+// count (one wave per candidate, or the step kernel for the next step) ->
+// scan (one workgroup: exclusive starts per antigen segment, each segment
+// starting on a 64-slot tile; zeroes the counts; fills the padding slots) ->
+// scatter.
 
-__device__ inline uint32_t sort_code(const AgView& V, float x, float y, float z) {
-  float p[3] = {x, y, z};
-  uint32_t c[3];
-  for (int a = 0; a < 3; ++a) {
-    float f = floorf((p[a] - V.sort_origin[a]) * V.sort_inv);
-    int32_t k = f32_to_i32_sat(f);
-    k = k < 0 ? 0 : (k >= V.sort_dim[a] ? V.sort_dim[a] - 1 : k);
-    c[a] = static_cast<uint32_t>(k);
+__global__ void __launch_bounds__(256)
+k_sort_count(const AgView* __restrict__ views, const uint32_t* __restrict__ cand_ag,
+             const uint32_t* __restrict__ cand_atom_off, uint32_t n_cand,
+             const uint8_t* __restrict__ cand_mask, const float4* __restrict__ pos,
+             uint32_t* __restrict__ count, uint32_t* __restrict__ code_out,
+             uint32_t* __restrict__ rank_out, float4* __restrict__ force,
+             double* __restrict__ e_atom, double* __restrict__ bd_atom,
+             double* __restrict__ f64_atom) {
+  const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if (c >= n_cand) return;
+  const AgView& V = views[cand_ag[c]];
+  const bool out = cand_mask && !cand_mask[c];
+  for (uint32_t j = cand_atom_off[c] + lane; j < cand_atom_off[c + 1]; j += kWave) {
+    uint32_t code = kExcluded;
+    if (!out) {
+      const float4 p = pos[j];
+      code = sort_code_of(V, p.x, p.y, p.z);
+    }
+    code_out[j] = code;
+    if (code == kExcluded) zero_atom_outputs(j, force, e_atom, bd_atom, f64_atom);
+    else rank_out[j] = atomicAdd(&count[code], 1u);
   }
-  return V.sort_code_base + morton3(c[0], c[1], c[2]);
 }
 
-__global__ void k_sort_count(const AgView* __restrict__ views, const float4* __restrict__ pos,
-                             const uint32_t* __restrict__ atom_ag, uint32_t n,
-                             uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  float4 p = pos[j];
-  uint32_t code = sort_code(views[atom_ag[j]], p.x, p.y, p.z);
-  rank[j] = atomicAdd(&count[code], 1u);
-}
-
-// Exclusive scan of the sort histogram, one antigen segment after another,
-// each segment starting on a 64-slot tile boundary. Writes the tile range of
-// every antigen to ag_tile_start (n_ag + 1 entries).
-__global__ void __launch_bounds__(1024) k_sort_scan(const AgView* __restrict__ views,
-                                                    uint32_t n_ag, uint32_t* __restrict__ count,
-                                                    uint32_t* __restrict__ ag_tile_start) {
-  __shared__ uint32_t sums[1024];
-  __shared__ uint32_t carry;
-  const uint32_t t = threadIdx.x;
-  if (t == 0) carry = 0;
-  __syncthreads();
+__global__ void __launch_bounds__(1024)
+k_sort_scan(const AgView* __restrict__ views, uint32_t n_ag, uint32_t* __restrict__ count,
+            uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
+            uint32_t* __restrict__ ag_tile_start, uint32_t* __restrict__ counter) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t woff[16];
+  __shared__ uint32_t block_total;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  uint32_t carry = 0;
   for (uint32_t a = 0; a < n_ag; ++a) {
-    const uint32_t base = views[a].sort_code_base;
-    const uint32_t ncode = 1u << (3 * views[a].sort_bits);
-    const uint32_t chunk = (ncode + 1023) / 1024;
-    const uint32_t b = min(ncode, t * chunk), e = min(ncode, b + chunk);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += count[base + i];
-    sums[t] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-      uint32_t v = t >= off ? sums[t - off] : 0u;
+    const uint32_t base = views[a].sort_code_base, ncode = views[a].sort_ncode;
+    const uint32_t seg_start = carry;
+    for (uint32_t c0 = 0; c0 < ncode; c0 += 4096) {
+      const uint32_t idx = c0 + 4 * t;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (idx < ncode) v = *reinterpret_cast<const uint4*>(&count[base + idx]);
+      const uint32_t s0 = v.x, s1 = s0 + v.y, s2 = s1 + v.z, tsum = s2 + v.w;
+      uint32_t incl = tsum;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 64);
+        if (lane >= static_cast<uint32_t>(o)) incl += u;
+      }
+      if (lane == 63) wsum[w] = incl;
       __syncthreads();
-      sums[t] += v;
+      if (t == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < 16; ++k) {
+          woff[k] = run;
+          run += wsum[k];
+        }
+        block_total = run;
+      }
+      __syncthreads();
+      if (idx < ncode) {
+        const uint32_t off = carry + woff[w] + (incl - tsum);
+        *reinterpret_cast<uint4*>(&start[base + idx]) = make_uint4(off, off + s0, off + s1, off + s2);
+        *reinterpret_cast<uint4*>(&count[base + idx]) = make_uint4(0, 0, 0, 0);
+      }
+      carry += block_total;
       __syncthreads();
     }
-    const uint32_t start = carry;
-    uint32_t run = start + (t ? sums[t - 1] : 0u);
-    for (uint32_t i = b; i < e; ++i) {
-      uint32_t v = count[base + i];
-      count[base + i] = run;
-      run += v;
-    }
-    __syncthreads();
-    if (t == 0) {
-      ag_tile_start[a] = start / kWave;
-      uint32_t end = start + sums[1023];
-      carry = (end + kWave - 1) / kWave * kWave;
-    }
-    __syncthreads();
+    const uint32_t pad_end = (carry + kWave - 1) / kWave * kWave;
+    for (uint32_t q = carry + t; q < pad_end; q += blockDim.x) perm[q] = kInvalid;
+    if (t == 0) ag_tile_start[a] = seg_start / kWave;
+    carry = pad_end;
   }
-  if (t == 0) ag_tile_start[n_ag] = carry / kWave;
+  if (t == 0) {
+    ag_tile_start[n_ag] = carry / kWave;
+    const uint32_t next = counter[0];
+    counter[1] = next;
+    counter[0] = next + 1u;
+  }
 }
 
-__global__ void k_sort_scatter(const AgView* __restrict__ views, const float4* __restrict__ pos,
-                               const uint32_t* __restrict__ atom_ag, uint32_t n,
-                               const uint32_t* __restrict__ start,
-                               const uint32_t* __restrict__ rank, uint32_t* __restrict__ perm) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_sort_scatter(const uint32_t* __restrict__ code_in,
+                               const uint32_t* __restrict__ rank, uint32_t n,
+                               const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  float4 p = pos[j];
-  uint32_t code = sort_code(views[atom_ag[j]], p.x, p.y, p.z);
-  perm[start[code] + rank[j]] = j;
+  const uint32_t code = code_in[j];
+  if (code != kExcluded) perm[start[code] + rank[j]] = j;
 }
 
 // ---- the pair tile kernel ---------------------------------------------------------
@@ -546,10 +560,18 @@ constexpr float kErrPerAbs = 32.0f / 16777216.0f;
 __global__ void __launch_bounds__(256)
 k_reduce_energy(const uint32_t* __restrict__ cand_atom_off, uint32_t n_cand,
                 const double* __restrict__ e_atom, const float4* __restrict__ force,
-                int with_abs, double* __restrict__ energy, float* __restrict__ err) {
+                int with_abs, const uint8_t* __restrict__ cand_mask,
+                double* __restrict__ energy, float* __restrict__ err) {
   const uint32_t c = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63u;
   if (c >= n_cand) return;  // whole wave exits together
+  if (cand_mask && !cand_mask[c]) {  // not scored: ranks last
+    if (lane == 0) {
+      energy[c] = __builtin_huge_val();
+      err[c] = 0.0f;
+    }
+    return;
+  }
   const uint32_t b = cand_atom_off[c], e = cand_atom_off[c + 1];
   double s = 0.0, a = 0.0;
   for (uint32_t k = b + lane; k < e; k += kWave) {
@@ -589,32 +611,60 @@ k_reduce_breakdown(const uint32_t* __restrict__ cand_atom_off, uint32_t n_cand,
 
 // ---- host side ------------------------------------------------------------------
 
-int population_sort(dnab_ctx* ctx, dnab_population* pop) {
-  const uint32_t n = static_cast<uint32_t>(pop->n_atoms);
-  hipLaunchKernelGGL(k_fill_u32b, dim3(nblocks(pop->sort_codes, 256)), dim3(256), 0, ctx->stream,
-                     pop->sort_count.p, static_cast<uint64_t>(pop->sort_codes) + 1, 0u);
-  hipLaunchKernelGGL(k_fill_u32b, dim3(nblocks(pop->tile_cap * kWave, 256)), dim3(256), 0,
-                     ctx->stream, pop->perm.p, static_cast<uint64_t>(pop->tile_cap) * kWave,
-                     kInvalid);
+// Drops counts made for positions or a mask that no longer hold; the scan
+// would otherwise consume them. Never part of a captured step loop.
+int discard_pending_counts(dnab_ctx* ctx, dnab_population* pop) {
+  if (!pop->counts_pending) return DNAB_OK;
+  const uint64_t n = pop->sort_codes;
   if (n) {
-    hipLaunchKernelGGL(k_sort_count, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
-                       pop->views.p, pop->pos.p, pop->atom_ag.p, n, pop->sort_count.p,
-                       pop->sort_rank.p);
-  }
-  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, ctx->stream, pop->views.p,
-                     static_cast<uint32_t>(pop->n_ag), pop->sort_count.p, pop->ag_tile_start.p);
-  if (n) {
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
-                       pop->views.p, pop->pos.p, pop->atom_ag.p, n, pop->sort_count.p,
-                       pop->sort_rank.p, pop->perm.p);
+    hipLaunchKernelGGL(k_fill_u32b, dim3(nblocks(n, 256) > 1024 ? 1024 : nblocks(n, 256)),
+                       dim3(256), 0, ctx->stream, pop->sort_count.p, n, 0u);
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
+  pop->counts_pending = false;
+  return DNAB_OK;
+}
+
+int population_count(dnab_ctx* ctx, dnab_population* pop) {
+  if (pop->counts_pending) return DNAB_OK;
+  const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
+  if (nc) {
+    hipLaunchKernelGGL(k_sort_count, dim3(nblocks(nc, 4)), dim3(256), 0, ctx->stream,
+                       pop->views.p, pop->cand_ag.p, pop->cand_atom_off.p, nc,
+                       pop->masked ? pop->cand_mask.p : nullptr, pop->pos.p, pop->sort_count.p,
+                       pop->sort_code.p, pop->sort_rank.p, pop->force.p, pop->e_atom.p,
+                       pop->bd_atom.p, pop->f64_atom.p);
+  }
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  pop->counts_pending = true;
+  return DNAB_OK;
+}
+
+int population_sort(dnab_ctx* ctx, dnab_population* pop) {
+  const uint32_t n = static_cast<uint32_t>(pop->n_atoms);
+  // pending counts were made by the step kernel for all candidates
+  if (pop->masked) DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
+  DNAB_RETURN_IF(population_count(ctx, pop));
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, ctx->stream, pop->views.p,
+                     static_cast<uint32_t>(pop->n_ag), pop->sort_count.p, pop->sort_start.p,
+                     pop->perm.p, pop->ag_tile_start.p, pop->step_counter.p);
+  if (n) {
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                       pop->sort_code.p, pop->sort_rank.p, n, pop->sort_start.p, pop->perm.p);
+  }
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  pop->counts_pending = false;
   pop->sorted = true;
   return DNAB_OK;
 }
 
 int population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
   if (!pop->sorted) DNAB_RETURN_IF(population_sort(ctx, pop));
+  DNAB_RETURN_IF(population_pair(ctx, pop, flags));
+  return population_reduce_energies(ctx, pop, flags);
+}
+
+int population_pair(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
   const uint32_t n_ag = static_cast<uint32_t>(pop->n_ag);
   const dim3 grid(static_cast<uint32_t>(pop->tile_cap ? pop->tile_cap : 1)), block(256);
   TileOut o{pop->force.p, pop->e_atom.p, pop->bd_atom.p, pop->pair_stats.p};
@@ -646,7 +696,9 @@ int population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
     timer_end(ctx, "pair_e", ev, ev_end);
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
-  return population_reduce_energies(ctx, pop, flags);
+  pop->last_flags = (flags & kEvalCount) ? (flags | DNAB_EVAL_FORCES) & ~kEvalCount : flags;
+  pop->evaluated = true;
+  return DNAB_OK;
 }
 
 int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
@@ -655,7 +707,8 @@ int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned fla
     const int with_abs = (flags & (DNAB_EVAL_FP64 | DNAB_EVAL_BREAKDOWN)) ? 0 : 1;
     hipLaunchKernelGGL(k_reduce_energy, dim3(nblocks(nc, 4)), dim3(256), 0, ctx->stream,
                        pop->cand_atom_off.p, nc, pop->e_atom.p, pop->force.p, with_abs,
-                       pop->energy.p, pop->err_bound.p);
+                       pop->masked ? pop->cand_mask.p : nullptr, pop->energy.p,
+                       pop->err_bound.p);
     if (flags & (DNAB_EVAL_FP64 | DNAB_EVAL_BREAKDOWN)) {
       hipLaunchKernelGGL(k_reduce_breakdown, dim3(nblocks(nc, 4)), dim3(256), 0, ctx->stream,
                          pop->cand_atom_off.p, nc, pop->bd_atom.p, pop->breakdown.p);
@@ -703,7 +756,8 @@ int up(dnab_ctx* ctx, DevBuf<T>& b, const T* src, size_t n) {
   return upload(ctx, b.p, src, n * sizeof(T));
 }
 
-int build_ca_grid(dnab_ctx* ctx, dnab_antigen* ag, const dnab_residue_cloud* ca) {
+int build_ca_grid(dnab_ctx* ctx, dnab_antigen* ag, const dnab_residue_cloud* ca,
+                  const uint8_t* aa) {
   ag->n_ca = ca->n;
   size_t n = ca->n;
   DNAB_RETURN_IF(up(ctx, ag->ca_x, ca->x, n));
@@ -713,6 +767,10 @@ int build_ca_grid(dnab_ctx* ctx, dnab_antigen* ag, const dnab_residue_cloud* ca)
   DNAB_RETURN_IF(up(ctx, ag->ca_eps, ca->epsilon, n));
   DNAB_RETURN_IF(up(ctx, ag->ca_sig, ca->sigma, n));
   DNAB_RETURN_IF(up(ctx, ag->ca_hyd, ca->hydrophobic, n));
+  if (aa) {
+    DNAB_RETURN_IF(up(ctx, ag->ca_aa, aa, n));
+    ag->has_ca_aa = true;
+  }
   ag->ca_grid.cell = 10.0f;  // src/diffusion.rs:1003
   ag->ca_grid.inv = 1.0f / 10.0f;
   return grid_build_device(ctx, ag->ca_grid, ag->ca_x.p, ag->ca_y.p, ag->ca_z.p, n);
@@ -781,7 +839,11 @@ int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* d, dnab_antigen*
   }
   if (hipGetLastError() != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "antigen: gather launch failed"));
-  if ((rc = build_ca_grid(ctx, ag, &d->ca))) return fail(rc);
+  if (d->ca_amino_acid)
+    for (size_t i = 0; i < d->ca.n; ++i)
+      if (d->ca_amino_acid[i] >= 20)
+        return fail(dnab_set_error(ctx, DNAB_ERR_INVALID, "antigen: bad residue type"));
+  if ((rc = build_ca_grid(ctx, ag, &d->ca, d->ca_amino_acid))) return fail(rc);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "antigen: synchronize failed"));
   *out = ag;
@@ -914,28 +976,33 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
     v.rc2 = ag->cutoff * ag->cutoff;
     v.restraint_radius = ag->restraint_radius;
     v.n_atoms = static_cast<uint32_t>(ag->grid.n);
-    float margin = ag->cutoff + 16.0f;
+    // sort grid: the antigen box + cutoff (atoms beyond are excluded), 4 A
+    // cells (coarser when the code space would exceed 2^22 per antigen)
+    const float margin = ag->cutoff + 1.0f;
     float cell = kSortCell;
-    int dim_max = 1;
     for (;;) {
-      dim_max = 1;
+      uint64_t codes = 64;
       for (int k = 0; k < 3; ++k) {
         float ext = (ag->grid.n ? ag->bbox_hi[k] - ag->bbox_lo[k] : 0.0f) + 2.0f * margin;
         int dim = static_cast<int>(std::ceil(ext / cell)) + 1;
         v.sort_dim[k] = dim;
-        dim_max = std::max(dim_max, dim);
+        v.sort_nb[k] = (dim + 3) / 4;
+        codes *= static_cast<uint64_t>(v.sort_nb[k]);
       }
-      if (dim_max <= (1 << kSortMaxBits)) break;
+      if (codes <= (uint64_t(1) << 22)) {
+        v.sort_ncode = static_cast<uint32_t>(codes);
+        break;
+      }
       cell *= 2.0f;
     }
-    int bits = 0;
-    while ((1 << bits) < dim_max) ++bits;
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < 3; ++k) {
       v.sort_origin[k] = (ag->grid.n ? ag->bbox_lo[k] : 0.0f) - margin;
+      v.excl_lo[k] = ag->grid.n ? ag->bbox_lo[k] : INFINITY;
+      v.excl_hi[k] = ag->grid.n ? ag->bbox_hi[k] : -INFINITY;
+    }
     v.sort_inv = 1.0f / cell;
-    v.sort_bits = bits;
     v.sort_code_base = static_cast<uint32_t>(code_base);
-    code_base += uint64_t(1) << (3 * bits);
+    code_base += v.sort_ncode;
     v.ca_x = ag->ca_x.p;
     v.ca_y = ag->ca_y.p;
     v.ca_z = ag->ca_z.p;
@@ -943,6 +1010,7 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
     v.ca_eps = ag->ca_eps.p;
     v.ca_sig = ag->ca_sig.p;
     v.ca_hyd = ag->ca_hyd.p;
+    v.ca_aa = ag->has_ca_aa ? ag->ca_aa.p : nullptr;
     v.ca_grid = make_view(ag->ca_grid);
   }
   if (code_base >= 0xFFFFFF00ull)
@@ -964,15 +1032,28 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   if (nr && ((rc = up(ctx, pop->res_atom_off, d->res_atom_off, nr + 1)) ||
              (rc = up(ctx, pop->res_ca, d->res_ca_atom, nr))))
     return fail(rc);
-  if (pop->sort_count.alloc(static_cast<size_t>(pop->sort_codes) + 1) != hipSuccess ||
-      pop->sort_rank.alloc(na) != hipSuccess || pop->perm.alloc(pop->tile_cap * kWave) != hipSuccess ||
+  if (nr && d->res_amino_acid) {  // types checked above (< 20)
+    if ((rc = up(ctx, pop->res_aa, d->res_amino_acid, nr))) return fail(rc);
+    pop->has_res_aa = true;
+  }
+  if (pop->sort_count.alloc(static_cast<size_t>(pop->sort_codes) + 4) != hipSuccess ||
+      pop->sort_start.alloc(static_cast<size_t>(pop->sort_codes) + 4) != hipSuccess ||
+      pop->sort_code.alloc(na) != hipSuccess || pop->sort_rank.alloc(na) != hipSuccess || pop->perm.alloc(pop->tile_cap * kWave) != hipSuccess ||
       pop->ag_tile_start.alloc(n_antigens + 1) != hipSuccess || pop->force.alloc(na) != hipSuccess ||
       pop->e_atom.alloc(na) != hipSuccess || pop->bd_atom.alloc(3 * na) != hipSuccess ||
       pop->f64_atom.alloc(3 * na) != hipSuccess || pop->energy.alloc(nc) != hipSuccess ||
       pop->err_bound.alloc(nc) != hipSuccess || pop->breakdown.alloc(3 * nc) != hipSuccess ||
       pop->res_force.alloc(nr) != hipSuccess || pop->step_counter.alloc(4) != hipSuccess ||
-      pop->pair_stats.alloc(2) != hipSuccess)
+      pop->pair_stats.alloc(2) != hipSuccess || pop->cand_mask.alloc(nc) != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
+  {
+    std::vector<uint8_t> ones(nc ? nc : 1, 1);
+    if ((rc = upload(ctx, pop->cand_mask.p, ones.data(), nc))) return fail(rc);
+    std::vector<uint32_t> zeros(static_cast<size_t>(pop->sort_codes) + 4, 0u);
+    if ((rc = upload(ctx, pop->sort_count.p, zeros.data(), zeros.size() * 4))) return fail(rc);
+    uint32_t ctr[4] = {0, 0, 0, 0};
+    if ((rc = upload(ctx, pop->step_counter.p, ctr, sizeof(ctr)))) return fail(rc);
+  }
   if ((rc = population_sort(ctx, pop))) return fail(rc);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "population: synchronize failed"));
@@ -1096,6 +1177,7 @@ int dnab_population_write_coords(dnab_ctx* ctx, dnab_population* pop, const floa
     h[i].y = y[i];
     h[i].z = z[i];
   }
+  DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
   DNAB_RETURN_IF(upload(ctx, pop->pos.p, h.data(), n * sizeof(float4)));
   pop->evaluated = false;
   DNAB_RETURN_IF(population_sort(ctx, pop));

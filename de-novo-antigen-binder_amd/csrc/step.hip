@@ -15,6 +15,7 @@
 #include "device_common.h"
 #include "dnab_internal.h"
 #include "grid_device.h"
+#include "sort_device.h"
 
 namespace dnab {
 
@@ -34,71 +35,105 @@ __device__ inline float clamp_rust(float v, float lo, float hi) {
   return v;
 }
 
+// One wave per candidate: (1) the candidate's FP64 energy from its atoms'
+// energies (DPP wave sum; replaces the separate reduction in the loop), (2)
+// the gradient step of each residue (lanes = residues), (3) optionally the
+// sort counts of the moved atoms for the next step's tiles.
 __global__ void __launch_bounds__(256)
-k_langevin_step(const AgView* __restrict__ views, const uint32_t* __restrict__ res_cand,
-                const uint32_t* __restrict__ cand_ag, const uint32_t* __restrict__ res_atom_off,
-                const uint32_t* __restrict__ res_ca, const float4* __restrict__ res_cg,
-                const float4* __restrict__ force, float4* __restrict__ pos, uint32_t n_res,
-                StepParams sp, const float* __restrict__ sigma_table,
-                const uint32_t* __restrict__ counter) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_res) return;
-  const uint32_t iter = counter[0];
-  const uint32_t c = res_cand[r];
+k_candidate_update(const AgView* __restrict__ views, const uint32_t* __restrict__ cand_ag,
+                   const uint32_t* __restrict__ cand_res_off,
+                   const uint32_t* __restrict__ cand_atom_off,
+                   const uint32_t* __restrict__ res_atom_off, const uint32_t* __restrict__ res_ca,
+                   const float4* __restrict__ res_cg, uint32_t n_cand, float4* __restrict__ force,
+                   double* __restrict__ e_atom, double* __restrict__ bd_atom,
+                   double* __restrict__ f64_atom, float4* __restrict__ pos, StepParams sp,
+                   const float* __restrict__ sigma_table, const uint32_t* __restrict__ counter,
+                   int do_energy, double* __restrict__ energy, float* __restrict__ err,
+                   int do_count, uint32_t* __restrict__ count, uint32_t* __restrict__ code_out,
+                   uint32_t* __restrict__ rank_out) {
+  const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if (c >= n_cand) return;  // whole wave
   const AgView& V = views[cand_ag[c]];
-  const uint32_t ca = res_ca[r];
-  const float4 p = pos[ca];
-  const uint32_t a0 = res_atom_off[r], a1 = res_atom_off[r + 1];
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  if (sp.force_model == DNAB_FORCE_CG) {
-    const float4 g = res_cg[r];
-    CgCloud ag{V.ca_x, V.ca_y, V.ca_z, V.ca_q, V.ca_eps, V.ca_sig, V.ca_hyd};
-    cg_force_point(V.ca_grid, ag, p.x, p.y, p.z, g.x, g.y, g.z, g.w != 0.0f ? 1 : 0, fx, fy, fz);
-  } else {
-    for (uint32_t k = a0; k < a1; ++k) {
-      const float4 f = force[k];
-      fx += f.x;
-      fy += f.y;
-      fz += f.z;
+  if (do_energy) {
+    double s = 0.0, a = 0.0;
+    for (uint32_t k = cand_atom_off[c] + lane; k < cand_atom_off[c + 1]; k += kWave) {
+      s += e_atom[k];
+      a += static_cast<double>(force[k].w);
+    }
+    s = wave_sum_f64(s);
+    a = wave_sum_f64(a);
+    if (lane == 0) {
+      energy[c] = s;
+      err[c] = static_cast<float>(a) * (32.0f / 16777216.0f);
     }
   }
-  // src/diffusion.rs:318-327
-  const float dx_c = p.x - V.center[0];
-  const float dy_c = p.y - V.center[1];
-  const float dz_c = p.z - V.center[2];
-  const float d = fmaxf(sqrtf(dx_c * dx_c + dy_c * dy_c + dz_c * dz_c), 0.001f);
+  const uint32_t iter = counter[1];
   const float r0 = V.restraint_radius > 0.0f ? V.restraint_radius : sp.restraint_radius;
-  const float excess = d - r0;
-  const float f_r = -sp.restraint_k * excess / d;
-  fx += f_r * dx_c;
-  fy += f_r * dy_c;
-  fz += f_r * dz_c;
-  // src/diffusion.rs:329-337
-  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-  if (sp.with_noise) {
-    const float sigma = sigma_table[iter];
-    const Philox4 u = philox4x32_10(iter, r, 0x6e6f6973u, 0u, sp.seed_lo, sp.seed_hi);
-    const float two = sigma + sigma;
-    nx = u01_from_u32(u.v[0]) * two - sigma;
-    ny = u01_from_u32(u.v[1]) * two - sigma;
-    nz = u01_from_u32(u.v[2]) * two - sigma;
-  }
-  const float new_x = p.x + clamp_rust(sp.step * fx, -sp.max_disp, sp.max_disp) + nx;
-  const float new_y = p.y + clamp_rust(sp.step * fy, -sp.max_disp, sp.max_disp) + ny;
-  const float new_z = p.z + clamp_rust(sp.step * fz, -sp.max_disp, sp.max_disp) + nz;
-  // set_ca_pos: delta from the current Calpha, added to every atom.
-  const float ddx = new_x - p.x, ddy = new_y - p.y, ddz = new_z - p.z;
-  for (uint32_t k = a0; k < a1; ++k) {
-    float4 q = pos[k];
-    q.x += ddx;
-    q.y += ddy;
-    q.z += ddz;
-    pos[k] = q;
+  for (uint32_t r = cand_res_off[c] + lane; r < cand_res_off[c + 1]; r += kWave) {
+    const uint32_t ca = res_ca[r];
+    const float4 p = pos[ca];
+    const uint32_t a0 = res_atom_off[r], a1 = res_atom_off[r + 1];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (sp.force_model == DNAB_FORCE_CG) {
+      const float4 g = res_cg[r];
+      CgCloud ag{V.ca_x, V.ca_y, V.ca_z, V.ca_q, V.ca_eps, V.ca_sig, V.ca_hyd};
+      cg_force_point(V.ca_grid, ag, p.x, p.y, p.z, g.x, g.y, g.z, g.w != 0.0f ? 1 : 0, fx, fy, fz);
+    } else {
+      for (uint32_t k = a0; k < a1; ++k) {
+        const float4 f = force[k];
+        fx += f.x;
+        fy += f.y;
+        fz += f.z;
+      }
+    }
+    // src/diffusion.rs:318-327
+    const float dx_c = p.x - V.center[0];
+    const float dy_c = p.y - V.center[1];
+    const float dz_c = p.z - V.center[2];
+    const float d = fmaxf(sqrtf(dx_c * dx_c + dy_c * dy_c + dz_c * dz_c), 0.001f);
+    const float excess = d - r0;
+    const float f_r = -sp.restraint_k * excess / d;
+    fx += f_r * dx_c;
+    fy += f_r * dy_c;
+    fz += f_r * dz_c;
+    // src/diffusion.rs:329-337
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (sp.with_noise) {
+      const float sigma = sigma_table[iter];
+      const Philox4 u = philox4x32_10(iter, r, 0x6e6f6973u, 0u, sp.seed_lo, sp.seed_hi);
+      const float two = sigma + sigma;
+      nx = u01_from_u32(u.v[0]) * two - sigma;
+      ny = u01_from_u32(u.v[1]) * two - sigma;
+      nz = u01_from_u32(u.v[2]) * two - sigma;
+    }
+    const float new_x = p.x + clamp_rust(sp.step * fx, -sp.max_disp, sp.max_disp) + nx;
+    const float new_y = p.y + clamp_rust(sp.step * fy, -sp.max_disp, sp.max_disp) + ny;
+    const float new_z = p.z + clamp_rust(sp.step * fz, -sp.max_disp, sp.max_disp) + nz;
+    // set_ca_pos: delta from the current Calpha, added to every atom.
+    const float ddx = new_x - p.x, ddy = new_y - p.y, ddz = new_z - p.z;
+    for (uint32_t k = a0; k < a1; ++k) {
+      float4 q = pos[k];
+      q.x += ddx;
+      q.y += ddy;
+      q.z += ddz;
+      pos[k] = q;
+      if (do_count) {
+        const uint32_t code = sort_code_of(V, q.x, q.y, q.z);
+        code_out[k] = code;
+        if (code == kExcluded) zero_atom_outputs(k, force, e_atom, bd_atom, f64_atom);
+        else rank_out[k] = atomicAdd(&count[code], 1u);
+      }
+    }
   }
 }
 
+// counter[1] = step index in use, counter[0] = the next one
 __global__ void k_advance(uint32_t* counter) {
-  if (threadIdx.x == 0) counter[0] += 1u;
+  if (threadIdx.x == 0) {
+    const uint32_t next = counter[0];
+    counter[1] = next;
+    counter[0] = next + 1u;
+  }
 }
 
 __global__ void k_set_counter(uint32_t* counter, uint32_t v) {
@@ -170,15 +205,8 @@ static inline uint32_t nblk(uint64_t n, uint32_t t) {
   return static_cast<uint32_t>(b ? b : 1);
 }
 
-struct TopkPlan {
-  uint32_t k = 0;
-  std::vector<uint32_t> lens;  // input length of each pass
-  DevBuf<double> e[2];
-  DevBuf<uint32_t> idx[2];
-};
-
 // Multi-pass top-k: chunks of 4096 reduce n to ceil(n/4096)*k entries per pass.
-static int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k) {
+int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k) {
   pl.k = k;
   pl.lens.clear();
   uint32_t len = n;
@@ -199,8 +227,8 @@ static int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k) {
   return DNAB_OK;
 }
 
-static void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
-                        uint32_t* final_idx) {
+void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
+                 uint32_t* final_idx) {
   const double* in_e = energy;
   const uint32_t* in_i = nullptr;
   for (size_t p = 0; p < pl.lens.size(); ++p) {
@@ -231,7 +259,6 @@ static StepParams make_step_params(const dnab_langevin_params* p) {
 static int validate_params(dnab_ctx* ctx, const dnab_langevin_params* p) {
   if (p->force_model != DNAB_FORCE_ALLATOM && p->force_model != DNAB_FORCE_CG)
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: unknown force model");
-  if (p->resort_every == 0) return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: resort_every must be >= 1");
   if (!(p->schedule_total > 0)) return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: schedule_total must be > 0");
   if (!std::isfinite(p->step) || !std::isfinite(p->max_disp) || !std::isfinite(p->restraint_k) ||
       !std::isfinite(p->restraint_radius) || !(p->t_hot > 0) || !(p->t_cold > 0))
@@ -253,31 +280,43 @@ static int upload_sigma(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin
   return upload(ctx, pop->sigma_table.p, s.data(), s.size() * sizeof(float));
 }
 
-static int launch_step(dnab_ctx* ctx, dnab_population* pop, const StepParams& sp) {
-  if (pop->n_res) {
-    hipLaunchKernelGGL(k_langevin_step, dim3(nblk(pop->n_res, 256)), dim3(256), 0, ctx->stream,
-                       pop->views.p, pop->res_cand.p, pop->cand_ag.p, pop->res_atom_off.p,
-                       pop->res_ca.p, pop->res_cg.p, pop->force.p, pop->pos.p,
-                       static_cast<uint32_t>(pop->n_res), sp, pop->sigma_table.p,
-                       pop->step_counter.p);
+static int launch_update(dnab_ctx* ctx, dnab_population* pop, const StepParams& sp,
+                         bool do_energy, bool do_count) {
+  // counts of the positions before the move would go stale
+  if (!do_count) DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
+  const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
+  if (nc) {
+    hipLaunchKernelGGL(k_candidate_update, dim3(nblk(nc, 4)), dim3(256), 0, ctx->stream,
+                       pop->views.p, pop->cand_ag.p, pop->cand_res_off.p, pop->cand_atom_off.p,
+                       pop->res_atom_off.p, pop->res_ca.p, pop->res_cg.p, nc, pop->force.p,
+                       pop->e_atom.p, pop->bd_atom.p, pop->f64_atom.p, pop->pos.p, sp,
+                       pop->sigma_table.p, pop->step_counter.p, do_energy ? 1 : 0,
+                       pop->energy.p, pop->err_bound.p, do_count ? 1 : 0, pop->sort_count.p,
+                       pop->sort_code.p, pop->sort_rank.p);
   }
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
+  pop->counts_pending = do_count;
+  pop->sorted = false;
   return DNAB_OK;
 }
 
-// One chunk of the loop: L steps starting at a multiple of L, scoring first.
+// One chunk of the loop: L steps; when `score`, the first step's energies
+// (the state the chunk starts from) go through top-k.
 static int enqueue_chunk(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                          const StepParams& sp, uint32_t L, TopkPlan* tk, bool score) {
+  const bool allatom = p->force_model == DNAB_FORCE_ALLATOM;
   for (uint32_t l = 0; l < L; ++l) {
-    if (l % p->resort_every == 0) DNAB_RETURN_IF(population_sort(ctx, pop));
-    // The residue-level model needs atom energies only when scoring.
-    if (p->force_model == DNAB_FORCE_ALLATOM || (score && l == 0)) {
-      const unsigned flags = p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES;
-      DNAB_RETURN_IF(population_evaluate(ctx, pop, flags));
+    const bool scoring = score && l == 0;
+    const bool eval = allatom || scoring;
+    if (eval) {
+      DNAB_RETURN_IF(population_sort(ctx, pop));  // also advances the step counter
+      DNAB_RETURN_IF(population_pair(ctx, pop, p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES));
+    } else {
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p);
     }
-    if (score && l == 0 && tk) topk_launch(ctx, *tk, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
-    DNAB_RETURN_IF(launch_step(ctx, pop, sp));
+    // the next step evaluates (and so needs counts) in the all-atom model
+    DNAB_RETURN_IF(launch_update(ctx, pop, sp, eval, allatom));
+    if (scoring && tk) topk_launch(ctx, *tk, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
   }
   return DNAB_OK;
 }
@@ -294,7 +333,6 @@ int dnab_langevin_default_params(dnab_langevin_params* p) {
   p->n_steps = 200;      // GPU_STEPS, src/diffusion.rs:36
   p->score_every = 10;
   p->top_k = 64;         // TOP_K, src/diffusion.rs:40
-  p->resort_every = 10;
   p->seed = 0;
   p->with_noise = 0;     // GPU phase, src/diffusion.rs:1023
   p->force_model = DNAB_FORCE_ALLATOM;
@@ -317,13 +355,22 @@ int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_
   DNAB_RETURN_IF(validate_params(ctx, p));
   (void)hipSetDevice(ctx->device);
   DNAB_RETURN_IF(upload_sigma(ctx, pop, p, iter));
+  pop->masked = false;
   hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p, iter);
-  DNAB_RETURN_IF(population_sort(ctx, pop));
-  if (p->force_model == DNAB_FORCE_ALLATOM)
-    DNAB_RETURN_IF(population_evaluate(ctx, pop, DNAB_EVAL_FORCES));
-  DNAB_RETURN_IF(launch_step(ctx, pop, make_step_params(p)));
-  pop->evaluated = pop->evaluated && p->force_model == DNAB_FORCE_ALLATOM;
-  DNAB_RETURN_IF(population_sort(ctx, pop));
+  const bool allatom = p->force_model == DNAB_FORCE_ALLATOM;
+  if (allatom) {
+    DNAB_RETURN_IF(population_sort(ctx, pop));  // counter[1] = iter
+    DNAB_RETURN_IF(population_pair(ctx, pop, DNAB_EVAL_FORCES));
+  } else {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p);
+  }
+  DNAB_RETURN_IF(launch_update(ctx, pop, make_step_params(p), allatom, false));
+  if (allatom) {
+    pop->last_flags = DNAB_EVAL_FORCES;
+    pop->evaluated = true;  // energies and forces of the state before the step
+  } else {
+    pop->evaluated = false;
+  }
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DNAB_OK;
 }
@@ -344,10 +391,15 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
   DNAB_RETURN_IF(topk_plan(ctx, tk, nc, k));
   DNAB_ALLOC_CHECK(ctx, pop->topk_e.ensure(k));
   DNAB_ALLOC_CHECK(ctx, pop->topk_idx.ensure(k));
+  pop->masked = false;
   hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p, 0u);
   const StepParams sp = make_step_params(p);
   const uint32_t L = p->score_every ? p->score_every : std::max<uint32_t>(p->n_steps, 1);
-  const bool graph = p->use_graph && p->n_steps >= L && L % p->resort_every == 0;
+  const bool graph = p->use_graph && p->n_steps >= L;
+  // A chunk must start in the state it ends in: in the all-atom model every
+  // update counts the moved atoms for the next sort, so counts are pending.
+  if (p->force_model == DNAB_FORCE_ALLATOM) DNAB_RETURN_IF(population_count(ctx, pop));
+  else DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
   const bool saved_timing = ctx->timing;
   uint32_t done = 0;
   if (graph) {
@@ -386,7 +438,8 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
   }
   // final scoring after the last update
   DNAB_RETURN_IF(population_sort(ctx, pop));
-  DNAB_RETURN_IF(population_evaluate(ctx, pop, p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES));
+  DNAB_RETURN_IF(population_pair(ctx, pop, p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES));
+  DNAB_RETURN_IF(population_reduce_energies(ctx, pop, DNAB_EVAL_FORCES));
   topk_launch(ctx, tk, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   if (topk_idx) DNAB_RETURN_IF(download(ctx, topk_idx, pop->topk_idx.p, k * sizeof(uint32_t)));

@@ -75,7 +75,7 @@ class ProteinOutC(C.Structure):
 class AntigenDescC(C.Structure):
     _fields_ = [("atoms", AtomCloudC), ("ca", ResidueCloudC), ("center", C.c_float * 3),
                 ("cell_size", C.c_float), ("cutoff", C.c_float),
-                ("restraint_radius", C.c_float)]
+                ("restraint_radius", C.c_float), ("ca_amino_acid", u8p)]
 
 
 class PopulationDescC(C.Structure):
@@ -86,13 +86,22 @@ class PopulationDescC(C.Structure):
 
 class LangevinParamsC(C.Structure):
     _fields_ = [("n_steps", C.c_uint32), ("score_every", C.c_uint32), ("top_k", C.c_uint32),
-                ("resort_every", C.c_uint32), ("seed", C.c_uint64), ("with_noise", C.c_int32),
+                ("seed", C.c_uint64), ("with_noise", C.c_int32),
                 ("force_model", C.c_int32), ("use_graph", C.c_int32), ("step", C.c_float),
                 ("max_disp", C.c_float), ("noise_base", C.c_float), ("t_hot", C.c_float),
                 ("t_cold", C.c_float), ("schedule_offset", C.c_uint32),
                 ("schedule_total", C.c_uint32), ("restraint_k", C.c_float),
                 ("restraint_radius", C.c_float), ("count_pairs", C.c_int32)]
 
+
+class TrainParamsC(C.Structure):
+    _fields_ = [("n_steps", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("seed", C.c_uint64)]
+
+
+FEATURE_DIM = 3360
+MLP_HIDDEN = 256
 
 assert C.sizeof(GpuAtom) == 32
 
@@ -166,6 +175,23 @@ _PROTOS = {
     "dnab_population_info": (C.c_int, [vp, szp, szp, szp]),
     "dnab_population_read_pair_stats": (C.c_int, [vp, vp, u64p, u64p]),
     "dnab_population_reset_pair_stats": (C.c_int, [vp, vp]),
+    "dnab_population_features": (C.c_int, [vp, vp, C.POINTER(C.c_uint16)]),
+    "dnab_mlp_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
+    "dnab_mlp_destroy": (None, [vp]),
+    "dnab_mlp_param_count": (C.c_int, [szp]),
+    "dnab_mlp_get_params": (C.c_int, [vp, vp, f32p, C.c_size_t]),
+    "dnab_mlp_set_params": (C.c_int, [vp, vp, f32p, C.c_size_t]),
+    "dnab_mlp_loss_grad": (C.c_int, [vp, vp, C.POINTER(C.c_uint16), f32p, C.c_uint32, f32p,
+                                     f32p, f32p]),
+    "dnab_dataset_create": (C.c_int, [vp, C.c_size_t, C.c_float, C.c_float, C.POINTER(vp)]),
+    "dnab_dataset_destroy": (None, [vp]),
+    "dnab_dataset_add_population": (C.c_int, [vp, vp, vp]),
+    "dnab_dataset_size": (C.c_int, [vp, szp]),
+    "dnab_train_default_params": (C.c_int, [C.POINTER(TrainParamsC)]),
+    "dnab_mlp_train": (C.c_int, [vp, vp, vp, C.POINTER(TrainParamsC), f32p]),
+    "dnab_mlp_predict": (C.c_int, [vp, vp, vp, f32p]),
+    "dnab_prescreen": (C.c_int, [vp, vp, vp, C.c_uint32, C.c_uint32, u32p, f64p, u32p]),
+    "dnab_exact_topk": (C.c_int, [vp, vp, C.c_uint32, u32p, f64p, u32p]),
     "dnab_langevin_default_params": (C.c_int, [C.POINTER(LangevinParamsC)]),
     "dnab_langevin_step": (C.c_int, [vp, vp, C.POINTER(LangevinParamsC), C.c_uint32]),
     "dnab_langevin_run": (C.c_int, [vp, vp, C.POINTER(LangevinParamsC), u32p, f64p, u32p]),

@@ -8,6 +8,7 @@ GpuContext mirrors the reference's GpuContext (src/gpu.rs:134-243):
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -32,8 +33,16 @@ class AtomEnergyBreakdown:
 
 
 class GpuContext:
+    """Owns a libdnab context. Objects made on it (grids, antigens,
+    populations, networks, datasets) register here and are destroyed before
+    the context, since their C handles refer to its stream."""
+
     def __init__(self, handle: int):
         self._h = vp(handle)
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     @classmethod
     def try_init(cls, device: int = 0) -> "GpuContext | None":
@@ -50,6 +59,8 @@ class GpuContext:
 
     def close(self):
         if self._h:
+            for obj in list(self._children):
+                obj.close()
             N.lib.dnab_ctx_destroy(self._h)
             self._h = vp()
 
@@ -150,6 +161,7 @@ class SpatialHashGrid:
 
     def __init__(self, ctx: GpuContext, cell_size: float):
         self.ctx = ctx
+        ctx._adopt(self)
         h = vp()
         ctx.check(N.lib.dnab_grid_create(ctx.handle, cell_size, C.byref(h)))
         self._h = h
@@ -216,6 +228,7 @@ class DeviceAntigen:
                  cell_size: float | None = None, center=None, with_ca: bool = True,
                  restraint_radius: float = 0.0):
         self.ctx = ctx
+        ctx._adopt(self)
         if isinstance(protein, AtomProtein):
             cloud = protein.atoms
             ca = protein.ca_residue_cloud() if with_ca else None
@@ -230,6 +243,8 @@ class DeviceAntigen:
         d.ca = ca.c() if ca is not None else N.ResidueCloudC()
         for k in range(3):
             d.center[k] = float(self.center[k])
+        if ca is not None:
+            d.ca_amino_acid = N.ptr(ca.amino_acid, C.c_uint8)
         d.cell_size = float(cutoff if cell_size is None else cell_size)
         d.cutoff = float(cutoff)
         d.restraint_radius = float(restraint_radius)
@@ -278,6 +293,7 @@ class DevicePopulation:
 
     def __init__(self, ctx: GpuContext, antigens: list[DeviceAntigen], host: PopulationHost):
         self.ctx = ctx
+        ctx._adopt(self)
         self.antigens = list(antigens)
         self.host = host
         arr = (vp * len(antigens))(*[a.handle.value for a in antigens])
@@ -377,3 +393,135 @@ class DevicePopulation:
             self.close()
         except Exception:
             pass
+
+
+class Mlp:
+    """The prescreen MLP (3360 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights)."""
+
+    def __init__(self, ctx: GpuContext, seed: int = 0):
+        self.ctx = ctx
+        ctx._adopt(self)
+        h = vp()
+        ctx.check(N.lib.dnab_mlp_create(ctx.handle, seed, C.byref(h)))
+        self._h = h
+        n = C.c_size_t()
+        N.check(N.lib.dnab_mlp_param_count(C.byref(n)))
+        self.n_params = n.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        self.ctx.check(N.lib.dnab_mlp_get_params(self.ctx.handle, self._h, N.ptr(p, C.c_float),
+                                                 self.n_params))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self.ctx.check(N.lib.dnab_mlp_set_params(self.ctx.handle, self._h, N.ptr(p, C.c_float),
+                                                 self.n_params))
+
+    def loss_grad(self, x_bf16: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x_bf16, np.uint16)
+        t = np.ascontiguousarray(t, np.float32)
+        B = len(t)
+        g = np.zeros(self.n_params, np.float32)
+        pred = np.zeros(B, np.float32)
+        loss = C.c_float()
+        self.ctx.check(N.lib.dnab_mlp_loss_grad(self.ctx.handle, self._h, N.ptr(x, C.c_uint16),
+                                                N.ptr(t, C.c_float), B, N.ptr(g, C.c_float),
+                                                C.byref(loss), N.ptr(pred, C.c_float)))
+        return loss.value, g, pred
+
+    def train(self, ds: "Dataset", **kw):
+        p = N.TrainParamsC()
+        N.check(N.lib.dnab_train_default_params(C.byref(p)))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        hist = np.zeros(max(p.n_steps, 1), np.float32)
+        self.ctx.check(N.lib.dnab_mlp_train(self.ctx.handle, self._h, ds.handle, C.byref(p),
+                                            N.ptr(hist, C.c_float)))
+        return hist[:p.n_steps]
+
+    def predict(self, pop: DevicePopulation) -> np.ndarray:
+        out = np.zeros(max(pop.n_cand, 1), np.float32)
+        self.ctx.check(N.lib.dnab_mlp_predict(self.ctx.handle, self._h, pop.handle,
+                                              N.ptr(out, C.c_float)))
+        return out[:pop.n_cand]
+
+    def prescreen(self, pop: DevicePopulation, n_keep: int, top_k: int):
+        idx = np.zeros(top_k, np.uint32)
+        e = np.zeros(top_k, np.float64)
+        n = C.c_uint32()
+        self.ctx.check(N.lib.dnab_prescreen(self.ctx.handle, self._h, pop.handle, n_keep, top_k,
+                                            N.ptr(idx, C.c_uint32), N.ptr(e, C.c_double),
+                                            C.byref(n)))
+        return idx[:n.value], e[:n.value]
+
+    def close(self):
+        if self._h:
+            N.lib.dnab_mlp_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Dataset:
+    """Device-resident (features, target) rows for training the prescreen."""
+
+    def __init__(self, ctx: GpuContext, capacity: int, energy_scale: float = 10.0,
+                 energy_cap: float = 0.0):
+        self.ctx = ctx
+        ctx._adopt(self)
+        self.energy_scale = energy_scale
+        self.energy_cap = energy_cap
+        h = vp()
+        ctx.check(N.lib.dnab_dataset_create(ctx.handle, capacity, energy_scale, energy_cap,
+                                            C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def add(self, pop: DevicePopulation):
+        self.ctx.check(N.lib.dnab_dataset_add_population(self.ctx.handle, self._h, pop.handle))
+
+    def __len__(self):
+        n = C.c_size_t()
+        N.check(N.lib.dnab_dataset_size(self._h, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self._h:
+            N.lib.dnab_dataset_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def population_features(pop: DevicePopulation) -> np.ndarray:
+    out = np.zeros((max(pop.n_cand, 1), N.FEATURE_DIM), np.uint16)
+    pop.ctx.check(N.lib.dnab_population_features(pop.ctx.handle, pop.handle,
+                                                 N.ptr(out, C.c_uint16)))
+    return out[:pop.n_cand]
+
+
+def exact_topk(pop: DevicePopulation, top_k: int):
+    idx = np.zeros(top_k, np.uint32)
+    e = np.zeros(top_k, np.float64)
+    n = C.c_uint32()
+    pop.ctx.check(N.lib.dnab_exact_topk(pop.ctx.handle, pop.handle, top_k,
+                                        N.ptr(idx, C.c_uint32), N.ptr(e, C.c_double),
+                                        C.byref(n)))
+    return idx[:n.value], e[:n.value]

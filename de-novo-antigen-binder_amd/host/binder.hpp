@@ -1,0 +1,335 @@
+// binder.hpp — C++ host interface over libdnab's C ABI (include/dnab.h),
+// mirroring the reference crate's API for the all-atom hot path so that its
+// call sites read the same:
+//
+//   binder::AtomCloud                      src/allatom.rs:16-80
+//   binder::AtomProtein, protein_from_ca_trace
+//                                          src/allatom.rs:92-101, 523-541
+//   binder::ResidueCloud                   src/atom.rs:185-257
+//   binder::SpatialHashGrid                src/spatial.rs:12-68 (device-built)
+//   binder::GpuContext::try_init           src/gpu.rs:143-243 (std::nullopt = None)
+//   binder::GpuContext::score_batch        src/gpu.rs:166-267
+//   binder::interaction_energy_atoms[_breakdown], compute_forces[_with_grid]
+//                                          src/energy.rs:154-398
+//   binder::run_allatom_gpu_phase          src/diffusion.rs:1008-1062 (GPU phase,
+//                                          gradient steps + scoring, no MC moves)
+//
+// Errors: functions that the reference cannot fail return values as the
+// reference does; a failing C call throws binder::Error (the reference
+// panics and aborts, Cargo.toml:27). Header-only; link with -ldnab.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dnab.h"
+
+namespace binder {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+inline void check(int status, const dnab_ctx* ctx = nullptr) {
+  if (status == DNAB_OK) return;
+  std::string msg = dnab_status_string(status);
+  if (ctx) msg += std::string(": ") + dnab_ctx_last_error(ctx);
+  throw Error(status, msg);
+}
+
+enum class AminoAcid : uint8_t {
+  Ala = 0, Arg, Asn, Asp, Cys, Gln, Glu, Gly, His, Ile,
+  Leu, Lys, Met, Phe, Pro, Ser, Thr, Trp, Tyr, Val
+};
+
+struct AtomCloud {
+  std::vector<float> x, y, z, charge, r_min_half, epsilon;
+  std::vector<uint8_t> hydrophobic, atom_type;
+  std::vector<uint32_t> residue_idx;
+  size_t len() const { return x.size(); }
+  bool is_empty() const { return x.empty(); }
+  dnab_atom_cloud view() const {
+    dnab_atom_cloud c{};
+    c.n = len();
+    c.x = x.data(); c.y = y.data(); c.z = z.data();
+    c.charge = charge.data(); c.r_min_half = r_min_half.data(); c.epsilon = epsilon.data();
+    c.hydrophobic = hydrophobic.data();
+    c.atom_type = atom_type.empty() ? nullptr : atom_type.data();
+    c.residue_idx = residue_idx.empty() ? nullptr : residue_idx.data();
+    return c;
+  }
+};
+
+struct ResidueCloud {
+  std::vector<float> x, y, z, charge, epsilon, sigma;
+  std::vector<uint8_t> hydrophobic;
+  std::vector<AminoAcid> amino_acid;
+  size_t len() const { return x.size(); }
+  // ResidueCloud::push (src/atom.rs:226-235)
+  void push(float px, float py, float pz, AminoAcid aa) {
+    float q = 0, e = 0, s = 0;
+    uint8_t h = 0;
+    check(dnab_residue_cg_params(static_cast<uint8_t>(aa), &q, &e, &s, &h));
+    x.push_back(px); y.push_back(py); z.push_back(pz);
+    charge.push_back(q); epsilon.push_back(e); sigma.push_back(s);
+    hydrophobic.push_back(h); amino_acid.push_back(aa);
+  }
+  dnab_residue_cloud view() const {
+    dnab_residue_cloud c{};
+    c.n = len();
+    c.x = x.data(); c.y = y.data(); c.z = z.data();
+    c.charge = charge.data(); c.epsilon = epsilon.data(); c.sigma = sigma.data();
+    c.hydrophobic = hydrophobic.data();
+    return c;
+  }
+};
+
+struct AtomProtein {
+  AtomCloud atoms;
+  std::vector<uint32_t> res_start{0};
+  std::vector<AminoAcid> amino_acid;
+  std::vector<uint32_t> ca_atom_idx;
+  size_t n_residues() const { return amino_acid.size(); }
+  size_t n_atoms() const { return atoms.len(); }
+  // src/allatom.rs:183-195
+  std::array<float, 3> ca_center_of_mass() const {
+    std::array<float, 3> c{0.f, 0.f, 0.f};
+    if (n_residues() == 0) return c;
+    for (uint32_t i : ca_atom_idx) {
+      c[0] += atoms.x[i]; c[1] += atoms.y[i]; c[2] += atoms.z[i];
+    }
+    float n = static_cast<float>(n_residues());
+    return {c[0] / n, c[1] / n, c[2] / n};
+  }
+};
+
+// protein_from_ca_trace (src/allatom.rs:523-541)
+inline AtomProtein protein_from_ca_trace(const std::vector<float>& x, const std::vector<float>& y,
+                                         const std::vector<float>& z,
+                                         const std::vector<AminoAcid>& aa) {
+  const size_t n = aa.size();
+  AtomProtein p;
+  const size_t cap = 14 * n + 1;
+  auto& a = p.atoms;
+  a.x.resize(cap); a.y.resize(cap); a.z.resize(cap); a.charge.resize(cap);
+  a.r_min_half.resize(cap); a.epsilon.resize(cap); a.hydrophobic.resize(cap);
+  a.atom_type.resize(cap); a.residue_idx.resize(cap);
+  p.res_start.assign(n + 1, 0);
+  p.ca_atom_idx.assign(n, 0);
+  std::vector<uint8_t> aa8(n), aa_out(n + 1);
+  for (size_t i = 0; i < n; ++i) aa8[i] = static_cast<uint8_t>(aa[i]);
+  dnab_protein_out out{cap, a.x.data(), a.y.data(), a.z.data(), a.charge.data(),
+                       a.r_min_half.data(), a.epsilon.data(), a.hydrophobic.data(),
+                       a.atom_type.data(), a.residue_idx.data(), p.res_start.data(),
+                       p.ca_atom_idx.data(), aa_out.data()};
+  size_t na = 0;
+  check(dnab_build_protein(x.data(), y.data(), z.data(), aa8.data(), nullptr, n, &out, &na));
+  for (auto* v : {&a.x, &a.y, &a.z, &a.charge, &a.r_min_half, &a.epsilon}) v->resize(na);
+  a.hydrophobic.resize(na); a.atom_type.resize(na); a.residue_idx.resize(na);
+  p.amino_acid = aa;
+  return p;
+}
+
+struct AtomEnergyBreakdown {
+  double lj = 0, coulomb = 0, hydrophobic = 0, hbond = 0, solvation = 0;
+  // src/energy.rs:247-249
+  double total() const { return lj + coulomb + hydrophobic + hbond + solvation; }
+};
+
+class GpuContext;
+
+class SpatialHashGrid {
+ public:
+  // SpatialHashGrid::new (src/spatial.rs:23-28), on the context's device
+  SpatialHashGrid(const GpuContext& ctx, float cell_size);
+  ~SpatialHashGrid() { dnab_grid_destroy(g_); }
+  SpatialHashGrid(const SpatialHashGrid&) = delete;
+  SpatialHashGrid& operator=(const SpatialHashGrid&) = delete;
+  // build (src/spatial.rs:31-39)
+  void build(const std::vector<float>& x, const std::vector<float>& y,
+             const std::vector<float>& z) {
+    check(dnab_grid_build(ctx_, g_, x.data(), y.data(), z.data(), x.size()), ctx_);
+  }
+  // query_neighbors (src/spatial.rs:44-58): the callback sees each index in
+  // the reference's order.
+  template <class F>
+  void query_neighbors(float px, float py, float pz, F&& callback) const {
+    uint32_t off[2];
+    size_t total = 0;
+    int st = dnab_grid_query(ctx_, g_, &px, &py, &pz, 1, off, nullptr, 0, &total);
+    if (st != DNAB_OK && st != DNAB_ERR_CAPACITY) check(st, ctx_);
+    std::vector<uint32_t> idx(total ? total : 1);
+    check(dnab_grid_query(ctx_, g_, &px, &py, &pz, 1, off, idx.data(), idx.size(), &total), ctx_);
+    for (size_t k = 0; k < total; ++k) callback(idx[k]);
+  }
+  const dnab_grid* handle() const { return g_; }
+
+ private:
+  dnab_ctx* ctx_;
+  dnab_grid* g_ = nullptr;
+};
+
+class GpuContext {
+ public:
+  // GpuContext::try_init (src/gpu.rs:143-243): nullopt when no device.
+  static std::optional<GpuContext> try_init(int device = 0) {
+    dnab_ctx* c = nullptr;
+    int st = dnab_ctx_create(device, &c);
+    if (st == DNAB_ERR_NO_DEVICE) return std::nullopt;
+    check(st);
+    return GpuContext(c);
+  }
+  GpuContext(GpuContext&& o) noexcept : c_(o.c_) { o.c_ = nullptr; }
+  GpuContext& operator=(GpuContext&& o) noexcept {
+    std::swap(c_, o.c_);
+    return *this;
+  }
+  GpuContext(const GpuContext&) = delete;
+  ~GpuContext() { dnab_ctx_destroy(c_); }
+  dnab_ctx* raw() const { return c_; }
+
+  // score_batch (src/gpu.rs:166-267): n_cand = candidates.size() / n_ab.
+  std::vector<float> score_batch(const AtomCloud& antigen,
+                                 const std::vector<dnab_gpu_atom>& candidates,
+                                 size_t n_ab) const {
+    const size_t n_cand = n_ab ? candidates.size() / n_ab : 0;
+    std::vector<float> e(n_cand);
+    size_t got = 0;
+    dnab_atom_cloud ag = antigen.view();
+    check(dnab_score_batch(c_, &ag, candidates.data(), candidates.size(), n_ab, e.data(),
+                           e.size(), &got), c_);
+    e.resize(got);
+    return e;
+  }
+
+ private:
+  explicit GpuContext(dnab_ctx* c) : c_(c) {}
+  dnab_ctx* c_ = nullptr;
+};
+
+inline SpatialHashGrid::SpatialHashGrid(const GpuContext& ctx, float cell_size)
+    : ctx_(ctx.raw()) {
+  check(dnab_grid_create(ctx_, cell_size, &g_), ctx_);
+}
+
+// interaction_energy_atoms_breakdown (src/energy.rs:261-314): LJ, Coulomb and
+// hydrophobic terms (hbond, solvation = 0) at the reference cutoff of 10 A.
+inline AtomEnergyBreakdown interaction_energy_atoms_breakdown(const GpuContext& gpu,
+                                                              const AtomCloud& antigen,
+                                                              const AtomCloud& antibody,
+                                                              float cutoff = 10.0f) {
+  dnab_atom_cloud a = antigen.view(), b = antibody.view();
+  dnab_energy_breakdown o{};
+  check(dnab_interaction_energy_atoms_breakdown(gpu.raw(), &a, &b, cutoff, &o), gpu.raw());
+  return {o.lj, o.coulomb, o.hydrophobic, o.hbond, o.solvation};
+}
+
+// interaction_energy_atoms (src/energy.rs:256-258)
+inline double interaction_energy_atoms(const GpuContext& gpu, const AtomCloud& antigen,
+                                       const AtomCloud& antibody, float cutoff = 10.0f) {
+  dnab_atom_cloud a = antigen.view(), b = antibody.view();
+  double e = 0;
+  check(dnab_interaction_energy_atoms(gpu.raw(), &a, &b, cutoff, &e), gpu.raw());
+  return e;
+}
+
+// compute_forces_with_grid (src/energy.rs:322-398): outputs sized >= n_ab.
+inline void compute_forces_with_grid(const GpuContext& gpu, const ResidueCloud& antigen,
+                                     const SpatialHashGrid& grid, const ResidueCloud& antibody,
+                                     std::vector<float>& fx, std::vector<float>& fy,
+                                     std::vector<float>& fz) {
+  dnab_residue_cloud a = antigen.view(), b = antibody.view();
+  fx.resize(std::max(fx.size(), antibody.len()));
+  fy.resize(fx.size());
+  fz.resize(fx.size());
+  check(dnab_cg_forces_with_grid(gpu.raw(), &a, grid.handle(), &b, fx.data(), fy.data(),
+                                 fz.data()), gpu.raw());
+}
+
+// compute_forces (src/energy.rs:154-234)
+inline void compute_forces(const GpuContext& gpu, const ResidueCloud& antigen,
+                           const ResidueCloud& antibody, std::vector<float>& fx,
+                           std::vector<float>& fy, std::vector<float>& fz) {
+  dnab_residue_cloud a = antigen.view(), b = antibody.view();
+  fx.resize(std::max(fx.size(), antibody.len()));
+  fy.resize(fx.size());
+  fz.resize(fx.size());
+  check(dnab_cg_forces(gpu.raw(), &a, &b, fx.data(), fy.data(), fz.data()), gpu.raw());
+}
+
+struct AllAtomResult {
+  uint32_t candidate;  // index into the candidates passed in
+  double energy;       // LJ + Coulomb + hydrophobic at the final scoring
+};
+
+// The GPU phase of run_allatom (src/diffusion.rs:1008-1062) with every
+// candidate moving for params.n_steps gradient steps, scoring every
+// params.score_every steps and keeping params.top_k (default 64).
+inline std::vector<AllAtomResult> run_allatom_gpu_phase(
+    const GpuContext& gpu, const AtomProtein& antigen, const std::vector<AtomProtein>& cands,
+    const dnab_langevin_params& params, float cutoff = 10.0f, float cell_size = 10.0f) {
+  dnab_ctx* c = gpu.raw();
+  ResidueCloud ca;
+  for (size_t r = 0; r < antigen.n_residues(); ++r) {
+    uint32_t i = antigen.ca_atom_idx[r];
+    ca.push(antigen.atoms.x[i], antigen.atoms.y[i], antigen.atoms.z[i], antigen.amino_acid[r]);
+  }
+  dnab_antigen_desc ad{};
+  ad.atoms = antigen.atoms.view();
+  ad.ca = ca.view();
+  auto ctr = antigen.ca_center_of_mass();
+  std::memcpy(ad.center, ctr.data(), sizeof(ad.center));
+  ad.cell_size = cell_size;
+  ad.cutoff = cutoff;
+  dnab_antigen* ag = nullptr;
+  check(dnab_antigen_create(c, &ad, &ag), c);
+  std::unique_ptr<dnab_antigen, void (*)(dnab_antigen*)> ag_guard(ag, dnab_antigen_destroy);
+  // flatten the candidates
+  AtomCloud all;
+  std::vector<uint32_t> cand_ag(cands.size(), 0), cand_res_off{0}, res_atom_off{0}, res_ca;
+  std::vector<uint8_t> res_aa;
+  for (const auto& p : cands) {
+    const uint32_t base = static_cast<uint32_t>(all.len());
+    for (size_t r = 0; r < p.n_residues(); ++r) {
+      res_atom_off.push_back(base + p.res_start[r + 1]);
+      res_ca.push_back(base + p.ca_atom_idx[r]);
+      res_aa.push_back(static_cast<uint8_t>(p.amino_acid[r]));
+    }
+    cand_res_off.push_back(static_cast<uint32_t>(res_ca.size()));
+    auto app = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
+    app(all.x, p.atoms.x); app(all.y, p.atoms.y); app(all.z, p.atoms.z);
+    app(all.charge, p.atoms.charge); app(all.r_min_half, p.atoms.r_min_half);
+    app(all.epsilon, p.atoms.epsilon); app(all.hydrophobic, p.atoms.hydrophobic);
+  }
+  dnab_population_desc pd{};
+  pd.n_cand = cands.size();
+  pd.n_res = res_ca.size();
+  pd.cand_antigen = cand_ag.data();
+  pd.cand_res_off = cand_res_off.data();
+  pd.res_atom_off = res_atom_off.data();
+  pd.res_ca_atom = res_ca.data();
+  pd.res_amino_acid = res_aa.data();
+  pd.atoms = all.view();
+  dnab_population* pop = nullptr;
+  dnab_antigen* ags[1] = {ag};
+  check(dnab_population_create(c, ags, 1, &pd, &pop), c);
+  std::unique_ptr<dnab_population, void (*)(dnab_population*)> pop_guard(pop,
+                                                                         dnab_population_destroy);
+  std::vector<uint32_t> idx(params.top_k ? params.top_k : 1);
+  std::vector<double> e(idx.size());
+  uint32_t n = 0;
+  check(dnab_langevin_run(c, pop, &params, idx.data(), e.data(), &n), c);
+  std::vector<AllAtomResult> out(n);
+  for (uint32_t k = 0; k < n; ++k) out[k] = {idx[k], e[k]};
+  return out;
+}
+
+}  // namespace binder

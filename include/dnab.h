@@ -220,6 +220,9 @@ typedef struct dnab_antigen_desc {
   float cutoff;    /* pair cutoff; the reference uses 10 (src/energy.rs:16) */
   float restraint_radius; /* gradient-step restraint radius around `center`;
                              0 = the step parameter restraint_radius */
+  const uint8_t* ca_amino_acid; /* residue types of `ca` (AminoAcid order,
+                                   src/atom.rs:4-25); needed by the prescreen
+                                   features, may be NULL otherwise */
 } dnab_antigen_desc;
 int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* desc, dnab_antigen** out);
 int dnab_antigen_info(const dnab_antigen* ag, size_t* n_atoms, size_t* n_cells_nonempty,
@@ -297,7 +300,6 @@ typedef struct dnab_langevin_params {
   uint32_t n_steps;         /* gradient steps (reference GPU phase: 200) */
   uint32_t score_every;     /* score every k steps (0 = only at the end) */
   uint32_t top_k;           /* survivors kept per scoring (reference: 64) */
-  uint32_t resort_every;    /* re-sort candidate atoms every k steps (>= 1) */
   uint64_t seed;
   int32_t with_noise;       /* reference GPU phase: 0, CPU phase: 1 */
   int32_t force_model;      /* DNAB_FORCE_ALLATOM or DNAB_FORCE_CG */
@@ -325,6 +327,65 @@ int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_
  * population is smaller, count in *n_topk). */
 int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                       uint32_t* topk_idx, double* topk_energy, uint32_t* n_topk);
+
+/* ---- learned prescreen (no reference counterpart; the reference has no
+ * learned components, Cargo.toml:4) -------------------------------------------
+ * Features: for every (candidate residue i, antigen residue j) with Calpha
+ * distance d < 14 A, exp(-(d - mu_k)^2 / (2 * 0.8^2)), mu_k = 2 + 0.8 k,
+ * k = 0..15, summed into bin [pair(aa_i, aa_j)][k] over the 210 unordered
+ * residue-type pairs; a row is bf16(log1p(bins)), 3360 values.
+ * Model: 3360 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights and activations,
+ * FP32 accumulation (MFMA), fp32 master weights, Adam. Target: asinh(E / s)
+ * with E the exact LJ + Coulomb + hydrophobic energy, s = energy_scale. */
+#define DNAB_FEATURE_DIM 3360
+#define DNAB_MLP_HIDDEN 256
+typedef struct dnab_mlp dnab_mlp;
+typedef struct dnab_dataset dnab_dataset;
+
+/* Features of every candidate; copied to host_bf16 (n_cand * 3360) when not
+ * NULL. Requires the antigens' ca_amino_acid. */
+int dnab_population_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* host_bf16);
+int dnab_mlp_create(dnab_ctx* ctx, uint64_t seed, dnab_mlp** out);
+void dnab_mlp_destroy(dnab_mlp* mlp);
+/* Flat parameter vector: W1[3360][256], b1[256], W2[256][256], b2, W3[256][256],
+ * b3, w4[256], b4[1] (992,257 floats, fp32 master copy). */
+int dnab_mlp_param_count(size_t* n);
+int dnab_mlp_get_params(dnab_ctx* ctx, const dnab_mlp* mlp, float* out, size_t n);
+int dnab_mlp_set_params(dnab_ctx* ctx, dnab_mlp* mlp, const float* in, size_t n);
+/* Loss and gradient of mean((f(x_b) - t_b)^2) on one batch of host rows (the
+ * oracle check of the device forward and backward passes). */
+int dnab_mlp_loss_grad(dnab_ctx* ctx, dnab_mlp* mlp, const uint16_t* x_bf16, const float* t,
+                       uint32_t batch, float* grads, float* loss, float* pred);
+/* Training rows get the target t = asinh(min(E, energy_cap) / energy_scale):
+ * candidates at or above energy_cap (clashes, no contact) share one value, so
+ * the fit spends its capacity on the low-energy tail that screening ranks. */
+int dnab_dataset_create(dnab_ctx* ctx, size_t capacity, float energy_scale, float energy_cap,
+                        dnab_dataset** out);
+void dnab_dataset_destroy(dnab_dataset* ds);
+/* Appends one row per candidate: its features and the target computed from
+ * the exact FP32 kernel's energy of the candidate. */
+int dnab_dataset_add_population(dnab_ctx* ctx, dnab_dataset* ds, dnab_population* pop);
+int dnab_dataset_size(const dnab_dataset* ds, size_t* n);
+typedef struct dnab_train_params {
+  uint32_t n_steps, batch;   /* batch: multiple of 64 */
+  float lr, beta1, beta2, eps;
+  uint64_t seed;
+} dnab_train_params;
+int dnab_train_default_params(dnab_train_params* p);
+/* Adam training on random minibatches; loss_history (nullable) gets the mean
+ * squared error of every step. */
+int dnab_mlp_train(dnab_ctx* ctx, dnab_mlp* mlp, dnab_dataset* ds, const dnab_train_params* p,
+                   float* loss_history);
+/* Predicted target for every candidate (features + forward pass). */
+int dnab_mlp_predict(dnab_ctx* ctx, dnab_mlp* mlp, dnab_population* pop, float* scores);
+/* Screening: features + MLP for every candidate, the n_keep best predictions
+ * re-scored by the exact FP32 kernel, the top_k of those by exact energy
+ * (ascending, ties by index) written to idx / energy (*n_out entries). */
+int dnab_prescreen(dnab_ctx* ctx, dnab_mlp* mlp, dnab_population* pop, uint32_t n_keep,
+                   uint32_t top_k, uint32_t* idx, double* energy, uint32_t* n_out);
+/* Exact top-k of every candidate, for comparison: evaluate all + top-k. */
+int dnab_exact_topk(dnab_ctx* ctx, dnab_population* pop, uint32_t top_k, uint32_t* idx,
+                    double* energy, uint32_t* n_out);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -133,11 +133,23 @@ def test_deterministic_and_tile_independent(ctx):
     pop2.evaluate(forces=True)
     e3, _ = pop2.energies()
     assert np.array_equal(e3.view(np.uint64), e1[perm].view(np.uint64))
-    # single-candidate scoring of the same atoms gives the same bits (ragged == single)
+    # a one-candidate population of the same atoms gives the same bits (ragged == single)
+    from dnab.structure import PopulationHost
     for c in (0, 7, 31):
-        single = ctx.interaction_energy_atoms(w.antigens[0].atoms, w.pop.candidate_cloud(c),
-                                              cutoff=w.cutoff)
-        assert np.float64(single).view(np.uint64) == e1[c].view(np.uint64)
+        r0, r1 = w.pop.cand_res_off[c], w.pop.cand_res_off[c + 1]
+        cl = w.pop.candidate_cloud(c)
+        base = w.pop.cand_atom_off()[c]
+        host = PopulationHost(np.zeros(1, np.uint32), np.array([0, r1 - r0], np.uint32),
+                              (w.pop.res_atom_off[r0:r1 + 1] - base).astype(np.uint32),
+                              (w.pop.res_ca_atom[r0:r1] - base).astype(np.uint32),
+                              w.pop.res_amino_acid[r0:r1].copy(), cl)
+        single = DevicePopulation(ctx, ags, host)
+        single.evaluate(forces=True)
+        es, _ = single.energies()
+        assert es[0].view(np.uint64) == e1[c].view(np.uint64)
+        fs = single.forces()
+        for a, b in zip(fs, f1):
+            assert np.array_equal(a.view(np.uint32), b[base:base + len(cl)].view(np.uint32))
 
 
 def test_score_batch_mirrors_reference(ctx):

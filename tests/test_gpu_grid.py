@@ -14,7 +14,7 @@ AAS = "ARNDCQEGHILKMFPSTWYV"
 def np_keys(x, cell):
     inv = np.float32(1.0) / np.float32(cell)
     with np.errstate(invalid="ignore", over="ignore"):
-        p = np.floor(np.asarray(x, np.float32) * inv)
+        p = np.floor(np.asarray(x, np.float32) * inv).astype(np.float64)
     return np.where(np.isnan(p), 0, np.clip(p, -2.0 ** 31, 2.0 ** 31 - 1)).astype(np.int64)
 
 

@@ -25,7 +25,7 @@ def cloud(xyz, q, rmh, eps, hyd):
 def np_keys(x, cell):
     inv = np.float32(1.0) / np.float32(cell)
     with np.errstate(invalid="ignore", over="ignore"):
-        p = np.floor(np.asarray(x, np.float32) * inv)
+        p = np.floor(np.asarray(x, np.float32) * inv).astype(np.float64)
     out = np.where(np.isnan(p), 0, np.clip(p, -2.0 ** 31, 2.0 ** 31 - 1)).astype(np.int64)
     return out.astype(np.int32)
 
