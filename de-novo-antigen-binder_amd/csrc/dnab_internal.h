@@ -93,6 +93,20 @@ inline GridView make_view(const GridDev& g) {
 
 // Antigen atom record for the pair kernel: 32 bytes, read by one scalar
 // load per atom. q332 = 332*q; sqe = sqrt(epsilon).
+// Work a tile's own wave hands to a workgroup (tiles deep in the antigen):
+// the chunks it did not take, as (first antigen atom, atom count), and its
+// partial sums as 2^-20 fixed-point integers. Written and read with plain
+// stores and loads: the kernel boundary orders them.
+constexpr uint32_t kHandoverCap = 128;  // chunks per tile
+struct PairHandover {
+  uint32_t* rest;     // per tile: chunks handed over (0: none)
+  uint32_t* list;     // tiles with chunks handed over (*n_list of them)
+  uint32_t* n_list;   // cleared by the sort scan (or a reset) before each launch
+  uint2* chunks;      // per tile: kHandoverCap entries
+  long long* part;    // 5 x n_slots partial sums (energy, sum |e|, fx, fy, fz)
+  uint32_t n_slots;
+};
+
 struct AgAtom {
   float x, y, z, q332;
   float rmh, sqe, hyd, pad;
@@ -189,6 +203,7 @@ struct dnab_population {
   dnab::DevBuf<float4> pos, par;
   dnab::DevBuf<uint32_t> atom_ag;    // per atom
   dnab::DevBuf<uint32_t> atom_cand;  // per atom
+  dnab::DevBuf<uint32_t> atom_res;   // per atom: its residue
   dnab::DevBuf<uint32_t> cand_atom_off, cand_res_off, cand_ag;
   dnab::DevBuf<uint32_t> res_atom_off, res_ca, res_cand;
   dnab::DevBuf<uint8_t> res_aa;         // residue types, when given
@@ -203,6 +218,12 @@ struct dnab_population {
   dnab::DevBuf<uint32_t> sort_code;      // per atom (kExcluded when left out)
   bool counts_pending = false;           // counts filled by the step kernel
   dnab::DevBuf<uint32_t> perm;           // tile_cap * 64 slots -> atom (or ~0u)
+  dnab::DevBuf<float4> spos, spar;       // per slot: the atom's pos and par
+  dnab::DevBuf<uint32_t> ho_rest;         // handover: chunks left per tile
+  dnab::DevBuf<uint32_t> ho_list;         // handover: tiles with chunks left, and the count
+  bool ho_dirty = false;                  // count left set by the last pair launch
+  dnab::DevBuf<uint2> ho_chunks;          // handover: tile_cap * kHandoverCap
+  dnab::DevBuf<long long> ho_part;        // handover: 5 * tile_cap * 64
   dnab::DevBuf<uint32_t> ag_tile_start;  // n_ag + 1
   bool sorted = false;
   // evaluation outputs
@@ -216,7 +237,7 @@ struct dnab_population {
   unsigned last_flags = 0;
   bool evaluated = false;
   // gradient-step state
-  dnab::DevBuf<float4> res_force;  // residue-level model forces
+  dnab::DevBuf<float4> res_delta;  // per residue: this step's displacement
   dnab::DevBuf<float> sigma_table;
   dnab::DevBuf<uint32_t> step_counter;
   dnab::DevBuf<unsigned long long> pair_stats;  // (in cutoff, tested)

@@ -113,29 +113,40 @@ k_sort_count(const AgView* __restrict__ views, const uint32_t* __restrict__ cand
   }
 }
 
+// Exclusive starts of every sort code. Each 16 A brick (64 codes) starts on a
+// 64-slot tile boundary, so a tile never mixes atoms of two bricks (its box is
+// at most one brick wide); the slots a brick leaves empty hold kInvalid.
+// Thread t owns codes 4t..4t+3 of a 4096-code chunk: 16 threads per brick.
 __global__ void __launch_bounds__(1024)
 k_sort_scan(const AgView* __restrict__ views, uint32_t n_ag, uint32_t* __restrict__ count,
             uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
-            uint32_t* __restrict__ ag_tile_start, uint32_t* __restrict__ counter) {
+            uint32_t* __restrict__ ag_tile_start, uint32_t* __restrict__ counter,
+            uint32_t* __restrict__ n_list) {
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t woff[16];
   __shared__ uint32_t block_total;
-  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-  uint32_t carry = 0;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6, g = lane & 15u;
+  uint32_t carry = 0;  // a multiple of 64 between bricks
   for (uint32_t a = 0; a < n_ag; ++a) {
     const uint32_t base = views[a].sort_code_base, ncode = views[a].sort_ncode;
-    const uint32_t seg_start = carry;
+    if (t == 0) ag_tile_start[a] = carry / kWave;
     for (uint32_t c0 = 0; c0 < ncode; c0 += 4096) {
       const uint32_t idx = c0 + 4 * t;
       uint4 v = make_uint4(0, 0, 0, 0);
       if (idx < ncode) v = *reinterpret_cast<const uint4*>(&count[base + idx]);
       const uint32_t s0 = v.x, s1 = s0 + v.y, s2 = s1 + v.z, tsum = s2 + v.w;
-      uint32_t incl = tsum;
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(incl, o, 64);
-        if (lane >= static_cast<uint32_t>(o)) incl += u;
+      uint32_t incl = tsum;  // inclusive scan within the brick's 16 threads
+      for (int o = 1; o < 16; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 16);
+        if (g >= static_cast<uint32_t>(o)) incl += u;
       }
-      if (lane == 63) wsum[w] = incl;
+      const uint32_t brick_total = __shfl(incl, static_cast<int>((lane & ~15u) | 15u), 64);
+      const uint32_t padded = (brick_total + kWave - 1) & ~(kWave - 1);
+      const uint32_t p0 = __shfl(padded, 0, 64), p1 = __shfl(padded, 16, 64);
+      const uint32_t p2 = __shfl(padded, 32, 64), p3 = __shfl(padded, 48, 64);
+      const uint32_t gi = lane >> 4;
+      const uint32_t brick_off = (gi > 0 ? p0 : 0u) + (gi > 1 ? p1 : 0u) + (gi > 2 ? p2 : 0u);
+      if (lane == 0) wsum[w] = (p0 + p1) + (p2 + p3);
       __syncthreads();
       if (t == 0) {
         uint32_t run = 0;
@@ -146,34 +157,45 @@ k_sort_scan(const AgView* __restrict__ views, uint32_t n_ag, uint32_t* __restric
         block_total = run;
       }
       __syncthreads();
+      const uint32_t bstart = carry + woff[w] + brick_off;
       if (idx < ncode) {
-        const uint32_t off = carry + woff[w] + (incl - tsum);
+        const uint32_t off = bstart + (incl - tsum);
         *reinterpret_cast<uint4*>(&start[base + idx]) = make_uint4(off, off + s0, off + s1, off + s2);
         *reinterpret_cast<uint4*>(&count[base + idx]) = make_uint4(0, 0, 0, 0);
       }
+      // the brick's empty slots (< 64), 4 per thread
+      const uint32_t pad_end = bstart + padded;
+      for (uint32_t q = bstart + brick_total + 4 * g, e = 0; e < 4 && q < pad_end; ++e, ++q)
+        perm[q] = kInvalid;
       carry += block_total;
       __syncthreads();
     }
-    const uint32_t pad_end = (carry + kWave - 1) / kWave * kWave;
-    for (uint32_t q = carry + t; q < pad_end; q += blockDim.x) perm[q] = kInvalid;
-    if (t == 0) ag_tile_start[a] = seg_start / kWave;
-    carry = pad_end;
   }
   if (t == 0) {
     ag_tile_start[n_ag] = carry / kWave;
     const uint32_t next = counter[0];
     counter[1] = next;
     counter[0] = next + 1u;
+    *n_list = 0;  // the pair kernels' list of heavy tiles
   }
 }
 
+// perm[slot] = atom, plus sorted copies of the atom's position and parameters
+// so that the pair kernel reads its tile without going through perm.
 __global__ void k_sort_scatter(const uint32_t* __restrict__ code_in,
                                const uint32_t* __restrict__ rank, uint32_t n,
-                               const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
+                               const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
+                               const float4* __restrict__ pos, const float4* __restrict__ par,
+                               float4* __restrict__ spos, float4* __restrict__ spar) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t code = code_in[j];
-  if (code != kExcluded) perm[start[code] + rank[j]] = j;
+  if (code != kExcluded) {
+    const uint32_t slot = start[code] + rank[j];
+    perm[slot] = j;
+    spos[slot] = pos[j];
+    spar[slot] = par[j];
+  }
 }
 
 // ---- the pair tile kernel ---------------------------------------------------------
@@ -428,6 +450,452 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
   out.force[j] = F;
 }
 
+// ---- the hot pair kernels -----------------------------------------------------------
+//
+// Work unit. A tile is 64 sorted candidate atoms, one per lane. The antigen
+// atoms near it are cut into chunks: a chunk is up to 64 consecutive atoms of
+// one grid cell, a unit fixed by the antigen alone. For each chunk a lane
+// forms FP32 partial sums over the chunk's atoms in order (TwoSum-compensated
+// energy, forces, sum |e|, hydrophobic count), skipping atoms outside the tile
+// box (their terms would be exactly zero), and rounds them once to integers
+// in units of 2^-20 (kcal/mol, kcal/mol/A). Chunk results are added as 64-bit
+// integers, which is exact and order-free: an atom's result does not depend
+// on its tile, on which wave took which chunk, or on whether a chunk was
+// handled in place or through the queue, so repeated and differently batched
+// runs agree bit for bit.
+//
+// Balance. k_pair_chunks runs one wave per tile (four tiles per workgroup,
+// so every tile of a step is resident at once); it loads chunks four at a
+// time and sums them until it has taken kDirectChunks chunks or
+// kDirectSurvivors antigen atoms. A tile deep inside the antigen has ~10x the
+// median work: its wave hands the remaining chunks and its partial sums to
+// k_pair_rest, where a workgroup of kRestWaves waves shares them. The two
+// kernels meet through plain stores and loads in device memory (the kernel
+// boundary orders them), with no device-scope atomics or fences, which on
+// this GPU would reach across the XCDs' separate L2 caches.
+constexpr uint32_t kDirectSurvivors = 48;
+constexpr uint32_t kDirectChunks = 16;
+constexpr int kLoadAhead = 4;   // chunk loads in flight per round trip
+constexpr int kRestWaves = 8;
+constexpr double kFixScale = 1048576.0;          // 2^20
+constexpr double kFixUnit = 1.0 / 1048576.0;
+constexpr long long kFixLim = 1LL << 59;         // accumulator bound; sums of 4 stay below 2^61
+
+__device__ inline long long fix_round(double v) {
+  const double lim = static_cast<double>(kFixLim);
+  v = fmin(fmax(v * kFixScale, -lim), lim);   // NaN -> -lim (fmax drops NaN)
+  return __double2ll_rn(v);
+}
+__device__ inline long long fix_add(long long a, long long b) {
+  const long long s = a + b;  // |a|, |b| <= 2^59 + 2^25: no overflow
+  return s > kFixLim ? kFixLim : (s < -kFixLim ? -kFixLim : s);
+}
+
+// Per-lane integer accumulators of one tile slot.
+struct FixAcc {
+  long long e = 0, a = 0, x = 0, y = 0, z = 0;
+};
+
+// One pair term (the reference's predicate and formulas, src/energy.rs:261-314):
+// the f32 energy ep and, with FORCES, the scalar f with F = f (dx, dy, dz).
+template <bool FORCES>
+__device__ inline void pair_term(float4 B0, float4 B1, float xb, float yb, float zb, float qb,
+                                 float rmhb, float sqeb, float sqe12b, bool hydb, float rc2_lane,
+                                 float& ep, float& f, float& dx, float& dy, float& dz, bool& in,
+                                 uint32_t& h) {
+  // pair predicate in f32, exactly the reference's (src/energy.rs:279-284)
+  dx = xb - B0.x;
+  dy = yb - B0.y;
+  dz = zb - B0.z;
+  const float r2 = (dx * dx + dy * dy) + dz * dz;
+  in = r2 <= rc2_lane && r2 >= kMinRSq;
+  float rinv = __builtin_amdgcn_rsqf(r2);
+  rinv = in ? rinv : 0.0f;
+  const float rinv2 = rinv * rinv;
+  const float R = B1.x + rmhb;
+  const float s2 = (R * R) * rinv2;
+  const float s6 = (s2 * s2) * s2;
+  const float s12 = s6 * s6;
+  const float elj = (B1.y * sqeb) * __builtin_fmaf(-2.0f, s6, s12);
+  const float ecl = (B0.w * qb) * rinv;
+  f = 0.0f;
+  if (FORCES) {
+    const float flj = ((B1.y * sqe12b) * (s12 - s6)) * rinv2;
+    f = __builtin_fmaf(ecl, rinv2, flj);
+  }
+  h = 0;
+  if (B1.z != 0.0f) {  // uniform: the antigen atom is the same in every lane
+    h = (in && hydb && r2 < kHydroSq) ? 1u : 0u;
+    if (FORCES) f = h ? __builtin_fmaf(-0.1f, rinv, f) : f;
+  }
+  ep = elj + ecl;
+}
+
+// Pairs of the lane's atom with the `n_s` antigen atoms staged in `st` (in
+// chunk order); folds the chunk's sums into `acc`. Two pairs are formed at a
+// time and added in order, which keeps the sums those of a plain loop.
+template <bool FORCES, bool COUNT>
+__device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb, float yb,
+                                   float zb, float qb, float rmhb, float sqeb, float sqe12b,
+                                   bool hydb, float rc2_lane, bool valid, FixAcc& acc,
+                                   uint32_t& n_in, uint32_t& n_tested) {
+  float fx = 0.f, fy = 0.f, fz = 0.f, es = 0.f, ec = 0.f, eabs = 0.f;
+  uint32_t nh = 0;
+  auto add = [&](float ep, float f, float dx, float dy, float dz, bool in, uint32_t h) {
+    if (COUNT) {
+      n_tested += valid ? 1u : 0u;
+      n_in += in ? 1u : 0u;
+    }
+    nh += h;
+    if (FORCES) {
+      fx = __builtin_fmaf(f, dx, fx);
+      fy = __builtin_fmaf(f, dy, fy);
+      fz = __builtin_fmaf(f, dz, fz);
+    }
+    float sum, err;
+    two_sum(es, ep, sum, err);
+    es = sum;
+    ec += err;
+    eabs += fabsf(ep);
+  };
+  uint32_t q = 0;
+  for (; q + 2 <= n_s; q += 2) {
+    const float4 A0 = st[q][0], A1 = st[q][1], B0 = st[q + 1][0], B1 = st[q + 1][1];
+    float epa, fa, dxa, dya, dza, epb, fb, dxb, dyb, dzb;
+    bool ina, inb;
+    uint32_t ha, hb;
+    pair_term<FORCES>(A0, A1, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb, rc2_lane, epa, fa, dxa,
+                      dya, dza, ina, ha);
+    pair_term<FORCES>(B0, B1, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb, rc2_lane, epb, fb, dxb,
+                      dyb, dzb, inb, hb);
+    add(epa, fa, dxa, dya, dza, ina, ha);
+    add(epb, fb, dxb, dyb, dzb, inb, hb);
+  }
+  if (q < n_s) {
+    float ep, f, dx, dy, dz;
+    bool in;
+    uint32_t h;
+    pair_term<FORCES>(st[q][0], st[q][1], xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb, rc2_lane, ep,
+                      f, dx, dy, dz, in, h);
+    add(ep, f, dx, dy, dz, in, h);
+  }
+  // hydrophobic: -0.5 per pair, exactly -2^19 in fixed point
+  const long long eh = -static_cast<long long>(nh) << 19;
+  acc.e = fix_add(acc.e, fix_round(static_cast<double>(es) + static_cast<double>(ec)) + eh);
+  acc.a = fix_add(acc.a, fix_round(static_cast<double>(eabs)));
+  if (FORCES) {
+    acc.x = fix_add(acc.x, fix_round(static_cast<double>(fx)));
+    acc.y = fix_add(acc.y, fix_round(static_cast<double>(fy)));
+    acc.z = fix_add(acc.z, fix_round(static_cast<double>(fz)));
+  }
+}
+
+__device__ inline void write_fixed(const FixAcc& acc, uint32_t j, TileOut out) {
+  out.e_atom[j] = static_cast<double>(acc.e) * kFixUnit;
+  out.force[j] = make_float4(static_cast<float>(static_cast<double>(acc.x) * kFixUnit),
+                             static_cast<float>(static_cast<double>(acc.y) * kFixUnit),
+                             static_cast<float>(static_cast<double>(acc.z) * kFixUnit),
+                             static_cast<float>(static_cast<double>(acc.a) * kFixUnit));
+}
+
+// Stage the chunk atoms selected by `mask` (in chunk order) into `st`.
+__device__ inline void stage_chunk(float4 (*st)[2], uint64_t mask, uint32_t lane, float4 A0,
+                                   float4 A1) {
+  if ((mask >> lane) & 1ull) {
+    const uint32_t slot = __builtin_amdgcn_mbcnt_hi(
+        static_cast<uint32_t>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+    st[slot][0] = A0;
+    st[slot][1] = A1;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool FORCES, bool COUNT>
+__global__ void __launch_bounds__(256)
+k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
+              const uint32_t* __restrict__ ag_tile_start, const uint32_t* __restrict__ perm,
+              const float4* __restrict__ spos, const float4* __restrict__ spar, TileOut out,
+              PairHandover ho) {
+  __shared__ float4 s_atoms[kTileWaves][kWave][2];  // a wave's staged chunk atoms
+
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t t = blockIdx.x * kTileWaves + wave;  // one tile per wave
+  const uint32_t n_tiles = ag_tile_start[n_ag];
+  if (t >= n_tiles) return;  // whole wave; no workgroup barriers below
+  const uint32_t a_id = tile_antigen(ag_tile_start, n_ag, t);
+  const AgView& V = views[a_id];
+  const AgAtom* __restrict__ atoms = V.atoms;
+  const uint32_t* __restrict__ cstart = V.cell_start;
+  const float inv = V.inv, rc = V.rc, rc2 = V.rc2;
+  const int32_t kmin0 = V.kmin[0], kmin1 = V.kmin[1], kmin2 = V.kmin[2];
+  const int32_t kmax0 = V.kmax[0], kmax1 = V.kmax[1], kmax2 = V.kmax[2];
+
+  const uint32_t slot_id = t * kWave + lane;
+  const uint32_t j = perm[slot_id];
+  const float4 Ps = spos[slot_id], Qs = spar[slot_id];
+  const bool valid = j != kInvalid;
+  const float4 P = valid ? Ps : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 Q = valid ? Qs : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float xb = P.x, yb = P.y, zb = P.z, qb = P.w;
+  const float rmhb = Q.x;
+  const float sqeb = sqrtf(Q.y);
+  const float sqe12b = 12.0f * sqeb;
+  const bool hydb = Q.z != 0.0f;
+  const float rc2_lane = valid ? rc2 : -1.0f;  // an empty slot has no pair
+
+  const float inf = __builtin_inff();
+  const float lox = wave_reduce_f32<OpMin>(valid ? xb : inf);
+  const float loy = wave_reduce_f32<OpMin>(valid ? yb : inf);
+  const float loz = wave_reduce_f32<OpMin>(valid ? zb : inf);
+  const float hix = wave_reduce_f32<OpMax>(valid ? xb : -inf);
+  const float hiy = wave_reduce_f32<OpMax>(valid ? yb : -inf);
+  const float hiz = wave_reduce_f32<OpMax>(valid ? zb : -inf);
+  // cells that can hold an antigen atom within rc of the tile box (cell_key
+  // is monotone, so a slightly widened range is conservative)
+  const float margin = rc * 1.001f + 1e-3f;
+  const int32_t kx0 = max(cell_key(lox - margin, inv), kmin0);
+  const int32_t ky0 = max(cell_key(loy - margin, inv), kmin1);
+  const int32_t kz0 = max(cell_key(loz - margin, inv), kmin2);
+  const int32_t kx1 = min(cell_key(hix + margin, inv), kmax0);
+  const int32_t ky1 = min(cell_key(hiy + margin, inv), kmax1);
+  const int32_t kz1 = min(cell_key(hiz + margin, inv), kmax2);
+  const int32_t nx = kx1 - kx0 + 1, ny = ky1 - ky0 + 1, nz = kz1 - kz0 + 1;
+  const int32_t nxy = nx * ny;
+  const int32_t ncell = (nx > 0 && ny > 0 && nz > 0) ? nxy * nz : 0;
+
+  FixAcc acc;
+  uint32_t n_in = 0, n_tested = 0, taken_atoms = 0, taken_chunks = 0, handed = 0;
+  bool handing = false;
+
+  for (int32_t cb0 = 0; cb0 < ncell; cb0 += kWave) {
+    // lane = a cell of this batch, in (kz, ky, kx) order
+    const int32_t c = cb0 + static_cast<int32_t>(lane);
+    uint32_t b = 0, e = 0;
+    if (c < ncell) {
+      const int32_t iz = c / nxy, rem = c - iz * nxy;
+      const int32_t iy = rem / nx, ix = rem - iy * nx;
+      const uint32_t code = morton3(static_cast<uint32_t>(kx0 + ix - kmin0),
+                                    static_cast<uint32_t>(ky0 + iy - kmin1),
+                                    static_cast<uint32_t>(kz0 + iz - kmin2));
+      b = cstart[code];
+      e = cstart[code + 1];
+    }
+    const uint32_t nch = (e - b + kWave - 1) / kWave;
+    uint32_t incl = nch;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = __shfl_up(incl, o, 64);
+      if (lane >= static_cast<uint32_t>(o)) incl += u;
+    }
+    const uint32_t off = incl - nch;
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+    // chunks [k, total) of this batch go to the rest kernel when the tile
+    // has had its share and the handover table has room
+    uint32_t k = 0, k_hand = total;  // chunks [k_hand, total) are handed over
+    for (; k < total && k_hand == total; k += kLoadAhead) {
+      if (taken_atoms >= kDirectSurvivors || taken_chunks >= kDirectChunks) handing = true;
+      if (handing && handed + (total - k) <= kHandoverCap) {
+        k_hand = k;
+        break;
+      }
+      uint32_t cbv[kLoadAhead], cev[kLoadAhead];
+      float4 L0[kLoadAhead], L1[kLoadAhead];
+      for (int u = 0; u < kLoadAhead; ++u) {
+        const uint32_t kk = k + u;
+        cbv[u] = 0;
+        cev[u] = 0;
+        L0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        L1[u] = L0[u];
+        if (kk < total) {
+          const uint64_t m = ballot64(nch > 0 && off <= kk);
+          const int cl = 63 - static_cast<int>(__builtin_clzll(m));
+          cbv[u] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), cl)) +
+                   kWave * (kk - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off), cl)));
+          cev[u] = min(cbv[u] + kWave, static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(e), cl)));
+          if (cbv[u] + lane < cev[u]) {
+            L0[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].x);
+            L1[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].rmh);
+          }
+        }
+      }
+      for (int u = 0; u < kLoadAhead; ++u) {
+        if (cev[u] <= cbv[u]) continue;
+        // stop at this chunk once the tile has had its share (and the rest fits)
+        if ((taken_atoms >= kDirectSurvivors || taken_chunks >= kDirectChunks) &&
+            handed + (total - (k + u)) <= kHandoverCap) {
+          handing = true;
+          k_hand = k + u;
+          break;
+        }
+        ++taken_chunks;
+        const float bx = fmaxf(fmaxf(lox - L0[u].x, L0[u].x - hix), 0.0f);
+        const float by = fmaxf(fmaxf(loy - L0[u].y, L0[u].y - hiy), 0.0f);
+        const float bz = fmaxf(fmaxf(loz - L0[u].z, L0[u].z - hiz), 0.0f);
+        const bool near = cbv[u] + lane < cev[u] && ((bx * bx + by * by) + bz * bz) <= rc2;
+        const uint64_t mask = ballot64(near);
+        const uint32_t n_s = static_cast<uint32_t>(__builtin_popcountll(mask));
+        if (n_s == 0) continue;
+        taken_atoms += n_s;
+        stage_chunk(s_atoms[wave], mask, lane, L0[u], L1[u]);
+        chunk_pairs<FORCES, COUNT>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb,
+                                   rc2_lane, valid, acc, n_in, n_tested);
+      }
+    }
+    if (k_hand < total) {
+      // lane = cell writes its chunks with ordinal >= k_hand, in order
+      for (uint32_t m = 0; m < nch; ++m) {
+        const uint32_t o = off + m;
+        if (o >= k_hand) {
+          const uint32_t first = b + kWave * m;
+          ho.chunks[t * kHandoverCap + handed + (o - k_hand)] =
+              make_uint2(first, min(static_cast<uint32_t>(kWave), e - first));
+        }
+      }
+      handed += total - k_hand;
+    }
+  }
+
+  if (COUNT) {
+    const uint32_t ai = wave_sum_u32(n_in), at = wave_sum_u32(n_tested);
+    if (lane == 0 && (ai | at)) {
+      atomicAdd(&out.stats[0], static_cast<unsigned long long>(ai));
+      atomicAdd(&out.stats[1], static_cast<unsigned long long>(at));
+    }
+  }
+  if (lane == 0) {
+    ho.rest[t] = handed;
+    if (handed) {  // once per tile and launch; the bound guards a missed reset
+      const uint32_t idx = atomicAdd(ho.n_list, 1u);
+      if (idx < ho.n_slots / kWave) ho.list[idx] = t;
+    }
+  }
+  if (handed == 0) {
+    if (valid) write_fixed(acc, j, out);
+    return;
+  }
+  const size_t ns = ho.n_slots;
+  ho.part[0 * ns + slot_id] = acc.e;
+  ho.part[1 * ns + slot_id] = acc.a;
+  ho.part[2 * ns + slot_id] = acc.x;
+  ho.part[3 * ns + slot_id] = acc.y;
+  ho.part[4 * ns + slot_id] = acc.z;
+}
+
+// The chunks a tile's wave handed over: kRestWaves waves take them from an
+// LDS counter, sum per chunk into fixed-point integers, combine in LDS, add
+// the wave's partial sums and write the tile's outputs.
+template <bool FORCES, bool COUNT>
+__global__ void __launch_bounds__(kRestWaves * 64)
+k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
+            const uint32_t* __restrict__ ag_tile_start, const uint32_t* __restrict__ perm,
+            const float4* __restrict__ spos, const float4* __restrict__ spar, TileOut out,
+            PairHandover ho) {
+  __shared__ float4 s_atoms[kRestWaves][kWave][2];
+  __shared__ long long s_sum[5][kWave];
+  __shared__ uint32_t s_next;
+  const uint32_t n_heavy = min(*ho.n_list, ho.n_slots / kWave);
+  for (uint32_t h = blockIdx.x; h < n_heavy; h += gridDim.x) {  // workgroup-uniform
+  const uint32_t t = ho.list[h];
+  const uint32_t n = ho.rest[t];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const AgView& V = views[tile_antigen(ag_tile_start, n_ag, t)];
+  const AgAtom* __restrict__ atoms = V.atoms;
+  const float rc2 = V.rc2;
+  const uint32_t slot_id = t * kWave + lane;
+  const uint32_t j = perm[slot_id];
+  const float4 Ps = spos[slot_id], Qs = spar[slot_id];
+  const bool valid = j != kInvalid;
+  const float4 P = valid ? Ps : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 Q = valid ? Qs : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float xb = P.x, yb = P.y, zb = P.z, qb = P.w;
+  const float rmhb = Q.x;
+  const float sqeb = sqrtf(Q.y);
+  const float sqe12b = 12.0f * sqeb;
+  const bool hydb = Q.z != 0.0f;
+  const float rc2_lane = valid ? rc2 : -1.0f;
+  const float inf = __builtin_inff();
+  const float lox = wave_reduce_f32<OpMin>(valid ? xb : inf);
+  const float loy = wave_reduce_f32<OpMin>(valid ? yb : inf);
+  const float loz = wave_reduce_f32<OpMin>(valid ? zb : inf);
+  const float hix = wave_reduce_f32<OpMax>(valid ? xb : -inf);
+  const float hiy = wave_reduce_f32<OpMax>(valid ? yb : -inf);
+  const float hiz = wave_reduce_f32<OpMax>(valid ? zb : -inf);
+  __syncthreads();  // the previous tile's LDS is no longer read
+  if (tid < 5 * kWave) s_sum[tid >> 6][tid & 63u] = 0;
+  if (tid == 0) s_next = 0;
+  __syncthreads();
+
+  FixAcc acc;
+  uint32_t n_in = 0, n_tested = 0;
+  auto grab = [&]() -> uint32_t {
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(&s_next, 1u);
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(k)));
+  };
+  const uint2* table = ho.chunks + static_cast<size_t>(t) * kHandoverCap;
+  uint32_t k = grab();
+  uint2 ch = k < n ? table[k] : make_uint2(0, 0);
+  while (k < n) {
+    float4 A0 = make_float4(0.f, 0.f, 0.f, 0.f), A1 = A0;
+    if (lane < ch.y) {
+      A0 = *reinterpret_cast<const float4*>(&atoms[ch.x + lane].x);
+      A1 = *reinterpret_cast<const float4*>(&atoms[ch.x + lane].rmh);
+    }
+    const uint32_t k2 = grab();                     // next chunk's table entry,
+    const uint2 ch2 = k2 < n ? table[k2] : make_uint2(0, 0);  // in flight meanwhile
+    const float bx = fmaxf(fmaxf(lox - A0.x, A0.x - hix), 0.0f);
+    const float by = fmaxf(fmaxf(loy - A0.y, A0.y - hiy), 0.0f);
+    const float bz = fmaxf(fmaxf(loz - A0.z, A0.z - hiz), 0.0f);
+    const bool near = lane < ch.y && ((bx * bx + by * by) + bz * bz) <= rc2;
+    const uint64_t mask = ballot64(near);
+    const uint32_t n_s = static_cast<uint32_t>(__builtin_popcountll(mask));
+    if (n_s) {
+      stage_chunk(s_atoms[wave], mask, lane, A0, A1);
+      chunk_pairs<FORCES, COUNT>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb,
+                                 rc2_lane, valid, acc, n_in, n_tested);
+    }
+    k = k2;
+    ch = ch2;
+  }
+  // each wave's sums are at most 2^59 in size: eight of them and the partial stay below 2^63
+  atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[0][lane]), static_cast<unsigned long long>(acc.e));
+  atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[1][lane]), static_cast<unsigned long long>(acc.a));
+  if (FORCES) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[2][lane]), static_cast<unsigned long long>(acc.x));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[3][lane]), static_cast<unsigned long long>(acc.y));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[4][lane]), static_cast<unsigned long long>(acc.z));
+  }
+  if (COUNT) {
+    const uint32_t ai = wave_sum_u32(n_in), at = wave_sum_u32(n_tested);
+    if (lane == 0 && (ai | at)) {
+      atomicAdd(&out.stats[0], static_cast<unsigned long long>(ai));
+      atomicAdd(&out.stats[1], static_cast<unsigned long long>(at));
+    }
+  }
+  __syncthreads();
+  if (wave == 0 && valid) {
+    const size_t ns = ho.n_slots;
+    FixAcc tot;
+    tot.e = s_sum[0][lane] + ho.part[0 * ns + slot_id];
+    tot.a = s_sum[1][lane] + ho.part[1 * ns + slot_id];
+    tot.x = s_sum[2][lane] + ho.part[2 * ns + slot_id];
+    tot.y = s_sum[3][lane] + ho.part[3 * ns + slot_id];
+    tot.z = s_sum[4][lane] + ho.part[4 * ns + slot_id];
+    tot.e = min(max(tot.e, -kFixLim), kFixLim);
+    tot.a = min(max(tot.a, -kFixLim), kFixLim);
+    tot.x = min(max(tot.x, -kFixLim), kFixLim);
+    tot.y = min(max(tot.y, -kFixLim), kFixLim);
+    tot.z = min(max(tot.z, -kFixLim), kFixLim);
+    write_fixed(tot, j, out);
+  }
+  }  // next heavy tile
+}
+
+__global__ void k_pair_list_reset(uint32_t* n_list) {
+  if (threadIdx.x == 0) *n_list = 0;
+}
+
 // All-FP64 oracle kernel: the same pair set (decided by the f32 predicate)
 // with every term, mixing rule and sum in double from the raw per-atom
 // parameters. Writes per atom (fx, fy, fz) and (lj, coulomb, hydrophobic).
@@ -647,10 +1115,13 @@ int population_sort(dnab_ctx* ctx, dnab_population* pop) {
   DNAB_RETURN_IF(population_count(ctx, pop));
   hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, ctx->stream, pop->views.p,
                      static_cast<uint32_t>(pop->n_ag), pop->sort_count.p, pop->sort_start.p,
-                     pop->perm.p, pop->ag_tile_start.p, pop->step_counter.p);
+                     pop->perm.p, pop->ag_tile_start.p, pop->step_counter.p,
+                     pop->ho_list.p + pop->tile_cap);
+  pop->ho_dirty = false;
   if (n) {
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
-                       pop->sort_code.p, pop->sort_rank.p, n, pop->sort_start.p, pop->perm.p);
+                       pop->sort_code.p, pop->sort_rank.p, n, pop->sort_start.p, pop->perm.p,
+                       pop->pos.p, pop->par.p, pop->spos.p, pop->spar.p);
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   pop->counts_pending = false;
@@ -662,6 +1133,33 @@ int population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
   if (!pop->sorted) DNAB_RETURN_IF(population_sort(ctx, pop));
   DNAB_RETURN_IF(population_pair(ctx, pop, flags));
   return population_reduce_energies(ctx, pop, flags);
+}
+
+static PairHandover pair_handover(dnab_population* pop) {
+  PairHandover h;
+  h.rest = pop->ho_rest.p;
+  h.list = pop->ho_list.p;
+  h.n_list = pop->ho_list.p + pop->tile_cap;
+  h.chunks = pop->ho_chunks.p;
+  h.part = pop->ho_part.p;
+  h.n_slots = static_cast<uint32_t>(pop->tile_cap * kWave);
+  return h;
+}
+
+template <bool FORCES, bool COUNT>
+static void launch_pair(dnab_ctx* ctx, dnab_population* pop, const TileOut& o) {
+  const uint32_t n_ag = static_cast<uint32_t>(pop->n_ag);
+  const PairHandover h = pair_handover(pop);
+  const uint32_t cap = static_cast<uint32_t>(pop->tile_cap ? pop->tile_cap : 1);
+  if (pop->ho_dirty) hipLaunchKernelGGL(k_pair_list_reset, dim3(1), dim3(64), 0, ctx->stream, h.n_list);
+  pop->ho_dirty = true;  // cleared by the next sort
+  hipLaunchKernelGGL((k_pair_chunks<FORCES, COUNT>), dim3((cap + kTileWaves - 1) / kTileWaves),
+                     dim3(256), 0, ctx->stream, pop->views.p, n_ag, pop->ag_tile_start.p,
+                     pop->perm.p, pop->spos.p, pop->spar.p, o, h);
+  hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT>), dim3(static_cast<uint32_t>(ctx->cu_count * 2)),
+                     dim3(kRestWaves * 64), 0,
+                     ctx->stream, pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p,
+                     pop->spos.p, pop->spar.p, o, h);
 }
 
 int population_pair(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
@@ -677,22 +1175,16 @@ int population_pair(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
                        pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
                        pop->par.p, o);
   } else if (flags & kEvalCount) {
-    hipLaunchKernelGGL((k_pair_tile<true, false, true>), grid, block, 0, ctx->stream,
-                       pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
-                       pop->par.p, o);
+    launch_pair<true, true>(ctx, pop, o);
   } else if (flags & DNAB_EVAL_FORCES) {
     hipEvent_t ev_end;
     hipEvent_t ev = timer_begin(ctx, "pair_ef", &ev_end);
-    hipLaunchKernelGGL((k_pair_tile<true, false, false>), grid, block, 0, ctx->stream,
-                       pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
-                       pop->par.p, o);
+    launch_pair<true, false>(ctx, pop, o);
     timer_end(ctx, "pair_ef", ev, ev_end);
   } else {
     hipEvent_t ev_end;
     hipEvent_t ev = timer_begin(ctx, "pair_e", &ev_end);
-    hipLaunchKernelGGL((k_pair_tile<false, false, false>), grid, block, 0, ctx->stream,
-                       pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
-                       pop->par.p, o);
+    launch_pair<false, false>(ctx, pop, o);
     timer_end(ctx, "pair_e", ev, ev_end);
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
@@ -726,10 +1218,7 @@ static int pair_count(dnab_ctx* ctx, dnab_population* pop, uint64_t* in_cutoff,
   unsigned long long zero[2] = {0, 0};
   DNAB_RETURN_IF(upload(ctx, pop->pair_stats.p, zero, sizeof(zero)));
   TileOut o{pop->force.p, pop->e_atom.p, pop->bd_atom.p, pop->pair_stats.p};
-  hipLaunchKernelGGL((k_pair_tile<true, false, true>),
-                     dim3(static_cast<uint32_t>(pop->tile_cap ? pop->tile_cap : 1)), dim3(256), 0,
-                     ctx->stream, pop->views.p, static_cast<uint32_t>(pop->n_ag),
-                     pop->ag_tile_start.p, pop->perm.p, pop->pos.p, pop->par.p, o);
+  launch_pair<true, true>(ctx, pop, o);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   unsigned long long h[2];
   DNAB_RETURN_IF(download(ctx, h, pop->pair_stats.p, sizeof(h)));
@@ -926,7 +1415,8 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   };
   // host staging
   std::vector<float4> hpos(na), hpar(na);
-  std::vector<uint32_t> hatom_ag(na), hatom_cand(na), hcand_atom_off(nc + 1), hres_cand(nr);
+  std::vector<uint32_t> hatom_ag(na), hatom_cand(na), hatom_res(na), hcand_atom_off(nc + 1),
+      hres_cand(nr);
   std::vector<float4> hres_cg(nr);
   for (size_t i = 0; i < na; ++i) {
     hpos[i] = make_float4(d->atoms.x[i], d->atoms.y[i], d->atoms.z[i], d->atoms.charge[i]);
@@ -941,6 +1431,7 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
       for (uint32_t k = d->res_atom_off[r]; k < d->res_atom_off[r + 1]; ++k) {
         hatom_ag[k] = d->cand_antigen[c];
         hatom_cand[k] = static_cast<uint32_t>(c);
+        hatom_res[k] = r;
       }
     }
   }
@@ -1016,12 +1507,15 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   if (code_base >= 0xFFFFFF00ull)
     return fail(dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "population: sort grid too large"));
   pop->sort_codes = static_cast<uint32_t>(code_base);
-  pop->tile_cap = (na + kWave - 1) / kWave + n_antigens;
+  // every non-empty 16 A brick adds at most one partly filled tile
+  pop->tile_cap = (na + kWave - 1) / kWave +
+                  std::min<size_t>(na, static_cast<size_t>(pop->sort_codes / kWave)) + 1;
   int rc = DNAB_OK;
   if ((rc = up(ctx, pop->views, pop->views_host.data(), n_antigens)) ||
       (rc = up(ctx, pop->pos, hpos.data(), na)) || (rc = up(ctx, pop->par, hpar.data(), na)) ||
       (rc = up(ctx, pop->atom_ag, hatom_ag.data(), na)) ||
       (rc = up(ctx, pop->atom_cand, hatom_cand.data(), na)) ||
+      (rc = up(ctx, pop->atom_res, hatom_res.data(), na)) ||
       (rc = up(ctx, pop->cand_atom_off, hcand_atom_off.data(), nc + 1)) ||
       (rc = up(ctx, pop->res_cand, hres_cand.data(), nr)) ||
       (rc = up(ctx, pop->res_cg, hres_cg.data(), nr)))
@@ -1038,12 +1532,19 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   }
   if (pop->sort_count.alloc(static_cast<size_t>(pop->sort_codes) + 4) != hipSuccess ||
       pop->sort_start.alloc(static_cast<size_t>(pop->sort_codes) + 4) != hipSuccess ||
-      pop->sort_code.alloc(na) != hipSuccess || pop->sort_rank.alloc(na) != hipSuccess || pop->perm.alloc(pop->tile_cap * kWave) != hipSuccess ||
+      pop->sort_code.alloc(na) != hipSuccess || pop->sort_rank.alloc(na) != hipSuccess ||
+      pop->perm.alloc(pop->tile_cap * kWave) != hipSuccess ||
+      pop->spos.alloc(pop->tile_cap * kWave) != hipSuccess ||
+      pop->spar.alloc(pop->tile_cap * kWave) != hipSuccess ||
+      pop->ho_rest.alloc(pop->tile_cap) != hipSuccess ||
+      pop->ho_list.alloc(pop->tile_cap + 4) != hipSuccess ||
+      pop->ho_chunks.alloc(pop->tile_cap * kHandoverCap) != hipSuccess ||
+      pop->ho_part.alloc(5 * pop->tile_cap * kWave) != hipSuccess ||
       pop->ag_tile_start.alloc(n_antigens + 1) != hipSuccess || pop->force.alloc(na) != hipSuccess ||
       pop->e_atom.alloc(na) != hipSuccess || pop->bd_atom.alloc(3 * na) != hipSuccess ||
       pop->f64_atom.alloc(3 * na) != hipSuccess || pop->energy.alloc(nc) != hipSuccess ||
       pop->err_bound.alloc(nc) != hipSuccess || pop->breakdown.alloc(3 * nc) != hipSuccess ||
-      pop->res_force.alloc(nr) != hipSuccess || pop->step_counter.alloc(4) != hipSuccess ||
+      pop->res_delta.alloc(nr) != hipSuccess || pop->step_counter.alloc(4) != hipSuccess ||
       pop->pair_stats.alloc(2) != hipSuccess || pop->cand_mask.alloc(nc) != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
   {
@@ -1053,6 +1554,9 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
     if ((rc = upload(ctx, pop->sort_count.p, zeros.data(), zeros.size() * 4))) return fail(rc);
     uint32_t ctr[4] = {0, 0, 0, 0};
     if ((rc = upload(ctx, pop->step_counter.p, ctr, sizeof(ctr)))) return fail(rc);
+    if (hipMemsetAsync(pop->ho_rest.p, 0, pop->tile_cap * sizeof(uint32_t), ctx->stream) != hipSuccess ||
+        hipMemsetAsync(pop->ho_list.p, 0, (pop->tile_cap + 4) * sizeof(uint32_t), ctx->stream) != hipSuccess)
+      return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "population: clearing the handover table failed"));
   }
   if ((rc = population_sort(ctx, pop))) return fail(rc);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess)

@@ -36,21 +36,20 @@ __device__ inline float clamp_rust(float v, float lo, float hi) {
 }
 
 // One wave per candidate: (1) the candidate's FP64 energy from its atoms'
-// energies (DPP wave sum; replaces the separate reduction in the loop), (2)
-// the gradient step of each residue (lanes = residues), (3) optionally the
-// sort counts of the moved atoms for the next step's tiles.
+// energies (DPP wave sum; the loop needs no separate reduction), (2) the
+// gradient step of each residue (lanes = residues), whose displacement goes
+// to res_delta for k_atom_move.
 __global__ void __launch_bounds__(256)
 k_candidate_update(const AgView* __restrict__ views, const uint32_t* __restrict__ cand_ag,
                    const uint32_t* __restrict__ cand_res_off,
                    const uint32_t* __restrict__ cand_atom_off,
                    const uint32_t* __restrict__ res_atom_off, const uint32_t* __restrict__ res_ca,
-                   const float4* __restrict__ res_cg, uint32_t n_cand, float4* __restrict__ force,
-                   double* __restrict__ e_atom, double* __restrict__ bd_atom,
-                   double* __restrict__ f64_atom, float4* __restrict__ pos, StepParams sp,
+                   const float4* __restrict__ res_cg, uint32_t n_cand,
+                   const float4* __restrict__ force, const double* __restrict__ e_atom,
+                   const float4* __restrict__ pos, StepParams sp,
                    const float* __restrict__ sigma_table, const uint32_t* __restrict__ counter,
                    int do_energy, double* __restrict__ energy, float* __restrict__ err,
-                   int do_count, uint32_t* __restrict__ count, uint32_t* __restrict__ code_out,
-                   uint32_t* __restrict__ rank_out) {
+                   float4* __restrict__ res_delta) {
   const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
   if (c >= n_cand) return;  // whole wave
   const AgView& V = views[cand_ag[c]];
@@ -79,7 +78,16 @@ k_candidate_update(const AgView* __restrict__ views, const uint32_t* __restrict_
       CgCloud ag{V.ca_x, V.ca_y, V.ca_z, V.ca_q, V.ca_eps, V.ca_sig, V.ca_hyd};
       cg_force_point(V.ca_grid, ag, p.x, p.y, p.z, g.x, g.y, g.z, g.w != 0.0f ? 1 : 0, fx, fy, fz);
     } else {
-      for (uint32_t k = a0; k < a1; ++k) {
+      // sum in atom order; four loads in flight at a time
+      uint32_t k = a0;
+      for (; k + 4 <= a1; k += 4) {
+        const float4 f0 = force[k], f1 = force[k + 1], f2 = force[k + 2], f3 = force[k + 3];
+        fx += f0.x; fy += f0.y; fz += f0.z;
+        fx += f1.x; fy += f1.y; fz += f1.z;
+        fx += f2.x; fy += f2.y; fz += f2.z;
+        fx += f3.x; fy += f3.y; fz += f3.z;
+      }
+      for (; k < a1; ++k) {
         const float4 f = force[k];
         fx += f.x;
         fy += f.y;
@@ -109,22 +117,61 @@ k_candidate_update(const AgView* __restrict__ views, const uint32_t* __restrict_
     const float new_x = p.x + clamp_rust(sp.step * fx, -sp.max_disp, sp.max_disp) + nx;
     const float new_y = p.y + clamp_rust(sp.step * fy, -sp.max_disp, sp.max_disp) + ny;
     const float new_z = p.z + clamp_rust(sp.step * fz, -sp.max_disp, sp.max_disp) + nz;
-    // set_ca_pos: delta from the current Calpha, added to every atom.
-    const float ddx = new_x - p.x, ddy = new_y - p.y, ddz = new_z - p.z;
-    for (uint32_t k = a0; k < a1; ++k) {
-      float4 q = pos[k];
-      q.x += ddx;
-      q.y += ddy;
-      q.z += ddz;
-      pos[k] = q;
-      if (do_count) {
-        const uint32_t code = sort_code_of(V, q.x, q.y, q.z);
-        code_out[k] = code;
-        if (code == kExcluded) zero_atom_outputs(k, force, e_atom, bd_atom, f64_atom);
-        else rank_out[k] = atomicAdd(&count[code], 1u);
-      }
+    // set_ca_pos: the Calpha's displacement, added to every atom of the residue
+    res_delta[r] = make_float4(new_x - p.x, new_y - p.y, new_z - p.z, 0.0f);
+  }
+}
+
+// One thread per atom: move by its residue's displacement; optionally count
+// the atom for the next step's sort (or zero the outputs of an atom that left
+// the cutoff box of its antigen). Lanes of a wave with the same sort code
+// share one counter atomic (neighbouring atoms of a candidate often fall in
+// the same 4 A cell), which cuts the traffic on contended counters.
+__global__ void __launch_bounds__(256)
+k_atom_move(const AgView* __restrict__ views, const uint32_t* __restrict__ atom_ag,
+            const uint32_t* __restrict__ atom_res, const float4* __restrict__ res_delta,
+            uint32_t n, float4* __restrict__ pos, int do_count, uint32_t* __restrict__ count,
+            uint32_t* __restrict__ code_out, uint32_t* __restrict__ rank_out,
+            float4* __restrict__ force, double* __restrict__ e_atom, double* __restrict__ bd_atom,
+            double* __restrict__ f64_atom) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = k < n;
+  uint32_t code = kExcluded;
+  if (live) {
+    const float4 d = res_delta[atom_res[k]];
+    float4 q = pos[k];
+    q.x += d.x;
+    q.y += d.y;
+    q.z += d.z;
+    pos[k] = q;
+    if (do_count) {
+      code = sort_code_of(views[atom_ag[k]], q.x, q.y, q.z);
+      code_out[k] = code;
+      if (code == kExcluded) zero_atom_outputs(k, force, e_atom, bd_atom, f64_atom);
     }
   }
+  if (!do_count) return;  // uniform
+  const bool need = live && code != kExcluded;
+  // group the wave's lanes by code: a leader per distinct code
+  uint64_t rem = ballot64(need);
+  uint32_t leader = 0, below = 0, size = 0;
+  while (rem) {
+    const int l = static_cast<int>(__builtin_ctzll(rem));
+    const uint32_t lc = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(code), l));
+    const uint64_t grp = ballot64(need && code == lc);
+    if (need && code == lc) {
+      leader = static_cast<uint32_t>(l);
+      below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(grp >> 32),
+                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(grp), 0u));
+      size = static_cast<uint32_t>(__builtin_popcountll(grp));
+    }
+    rem &= ~grp;
+  }
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t base = 0;
+  if (need && lane == leader) base = atomicAdd(&count[code], size);
+  base = static_cast<uint32_t>(__shfl(static_cast<int>(base), static_cast<int>(leader), 64));
+  if (need) rank_out[k] = base + below;
 }
 
 // counter[1] = step index in use, counter[0] = the next one
@@ -285,14 +332,20 @@ static int launch_update(dnab_ctx* ctx, dnab_population* pop, const StepParams& 
   // counts of the positions before the move would go stale
   if (!do_count) DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
   const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
+  const uint32_t na = static_cast<uint32_t>(pop->n_atoms);
   if (nc) {
     hipLaunchKernelGGL(k_candidate_update, dim3(nblk(nc, 4)), dim3(256), 0, ctx->stream,
                        pop->views.p, pop->cand_ag.p, pop->cand_res_off.p, pop->cand_atom_off.p,
                        pop->res_atom_off.p, pop->res_ca.p, pop->res_cg.p, nc, pop->force.p,
-                       pop->e_atom.p, pop->bd_atom.p, pop->f64_atom.p, pop->pos.p, sp,
-                       pop->sigma_table.p, pop->step_counter.p, do_energy ? 1 : 0,
-                       pop->energy.p, pop->err_bound.p, do_count ? 1 : 0, pop->sort_count.p,
-                       pop->sort_code.p, pop->sort_rank.p);
+                       pop->e_atom.p, pop->pos.p, sp, pop->sigma_table.p, pop->step_counter.p,
+                       do_energy ? 1 : 0, pop->energy.p, pop->err_bound.p, pop->res_delta.p);
+  }
+  if (na) {
+    hipLaunchKernelGGL(k_atom_move, dim3(nblk(na, 256)), dim3(256), 0, ctx->stream,
+                       pop->views.p, pop->atom_ag.p, pop->atom_res.p, pop->res_delta.p, na,
+                       pop->pos.p, do_count ? 1 : 0, pop->sort_count.p, pop->sort_code.p,
+                       pop->sort_rank.p, pop->force.p, pop->e_atom.p, pop->bd_atom.p,
+                       pop->f64_atom.p);
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   pop->counts_pending = do_count;
