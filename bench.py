@@ -70,6 +70,67 @@ def cpu_baseline(w, budget_s=12.0):
                        f"brute-force energy pattern: {pairs * reps_b / dtb:.3g} pairs/s")}
 
 
+def prescreen_bench(ctx, args):
+    """Learned prescreen at the config-5 shape: train on exact energies of
+    shell binders around antigens of 100..1,000 residues, then screen a pool
+    of 65,536 candidates against a fresh 300-residue antigen; compare the
+    pipeline (features + MLP + exact re-score of the kept set) with exact
+    scoring of the whole pool."""
+    import numpy as np
+    import dnab
+    from dnab import workloads as W
+    t0 = time.perf_counter()
+    per = max(1024, args.prescreen_train // 10)
+    ds = dnab.Dataset(ctx, capacity=10 * per)
+    for k, n_res in enumerate(range(100, 1001, 100)):
+        r = W.globule_radius(n_res)
+        ag = W.globule_antigen(n_res, r, 5000 + k)
+        pop_h = W.shell_population(per, 20, ag.ca_center_of_mass(), r + W.SHELL_GAP, r,
+                                   (5 << 32) | (k << 24))
+        dag = dnab.DeviceAntigen(ctx, ag, cutoff=10.0, cell_size=10.0, restraint_radius=r + W.SHELL_GAP)
+        dpop = dnab.DevicePopulation(ctx, [dag], pop_h)
+        ds.add(dpop)
+        dpop.close()
+        dag.close()
+        log(f"[bench] prescreen training data: antigen {n_res} res, {len(ds)} rows")
+    t_data = time.perf_counter() - t0
+    mlp = dnab.Mlp(ctx, seed=1)
+    t0 = time.perf_counter()
+    steps = max(200, 2 * len(ds) // 512)
+    hist = mlp.train(ds, n_steps=steps, batch=512, lr=1e-3)
+    ctx.synchronize()
+    t_train = time.perf_counter() - t0
+    ds.close()
+    # screening pool
+    w = W.config2(n_cand=args.prescreen_pool, seed=55)
+    ags, pop = W.device_objects(ctx, w)
+    dnab.exact_topk(pop, 64)            # warm-up
+    t0 = time.perf_counter()
+    idx_e, e_e = dnab.exact_topk(pop, 64)
+    t_exact = time.perf_counter() - t0
+    out = {"pool": w.pop.n_cand, "train_rows": 10 * per, "train_steps": steps,
+           "train_s": round(t_train, 2), "data_s": round(t_data, 2),
+           "loss_first": float(np.mean(hist[:50])), "loss_last": float(np.mean(hist[-50:])),
+           "exact_cand_per_s": w.pop.n_cand / t_exact, "recall_at_64": {}, "pipeline_cand_per_s": {}}
+    best = None
+    for keep in (512, 1024, 2048):
+        mlp.prescreen(pop, keep, 64)    # warm-up
+        t0 = time.perf_counter()
+        idx_p, _ = mlp.prescreen(pop, keep, 64)
+        t_p = time.perf_counter() - t0
+        rec = len(set(idx_p.tolist()) & set(idx_e.tolist())) / 64
+        out["recall_at_64"][str(keep)] = rec
+        out["pipeline_cand_per_s"][str(keep)] = w.pop.n_cand / t_p
+        if best is None and rec >= 0.95:
+            best = w.pop.n_cand / t_p
+    out["screened_cand_per_s_at_95pct_recall"] = best
+    pop.close()
+    for a in ags:
+        a.close()
+    mlp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +140,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--n-cand", type=int, default=1024)
+    ap.add_argument("--no-prescreen", action="store_true")
+    ap.add_argument("--prescreen-train", type=int, default=1_000_000,
+                    help="training rows (candidate, exact energy) for the prescreen")
+    ap.add_argument("--prescreen-pool", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -189,6 +254,9 @@ def main():
             "flop_per_pair": FLOP_PER_PAIR_EF,
         },
     }
+    if not args.no_prescreen:
+        log("[bench] learned prescreen")
+        out["prescreen"] = prescreen_bench(ctx, args)
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline")
         cb = cpu_baseline(w, args.cpu_budget)

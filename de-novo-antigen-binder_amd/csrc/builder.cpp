@@ -8,7 +8,11 @@
 // library's own SplitMix64 stream; the reference's rand::SmallRng streams are
 // not reproduced.
 #include <cmath>
+#include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "dnab_internal.h"
 #include "ff_tables.h"
@@ -270,6 +274,83 @@ int dnab_synth_binder_shell(size_t n_res, const float center[3], float shell_rad
     amino_acid[i] = static_cast<uint8_t>(rng.below(kNumAminoAcids));
   }
   return DNAB_OK;
+}
+
+// A population of shell binders: candidate k is dnab_synth_binder_shell with
+// seed seed_base + k, built by dnab_build_protein, all candidates n_res long.
+// Traces are drawn first to size the atom arrays, then the candidates are
+// built in parallel into their slices (same result for any thread count).
+int dnab_synth_shell_population(size_t n_cand, size_t n_res, const float center[3],
+                                float shell_radius, float min_radius, uint64_t seed_base,
+                                int nthreads, dnab_protein_out* out, uint32_t* cand_res_off,
+                                size_t* n_atoms_out) {
+  if (!center || (out && !cand_res_off)) return DNAB_ERR_INVALID;
+  const size_t nr = n_cand * n_res;
+  std::vector<float> tx(nr), ty(nr), tz(nr);
+  std::vector<uint8_t> taa(nr);
+  std::vector<size_t> atom_off(n_cand + 1, 0);
+  const int nt = std::max(1, std::min(nthreads > 0 ? nthreads : 1, 64));
+  auto parallel = [&](auto fn) {
+    std::vector<std::thread> th;
+    const size_t per = (n_cand + nt - 1) / nt;
+    for (int k = 0; k < nt; ++k) {
+      const size_t c0 = std::min(n_cand, k * per), c1 = std::min(n_cand, c0 + per);
+      if (c0 < c1) th.emplace_back(fn, c0, c1);
+    }
+    for (auto& t : th) t.join();
+  };
+  parallel([&](size_t c0, size_t c1) {
+    for (size_t c = c0; c < c1; ++c)
+      dnab_synth_binder_shell(n_res, center, shell_radius, min_radius, seed_base + c,
+                              &tx[c * n_res], &ty[c * n_res], &tz[c * n_res], &taa[c * n_res]);
+  });
+  for (size_t c = 0; c < n_cand; ++c) {
+    size_t n = 0;
+    for (size_t r = 0; r < n_res; ++r) n += kResidueTopology[taa[c * n_res + r]].n_atoms;
+    atom_off[c + 1] = atom_off[c] + n;
+  }
+  if (n_atoms_out) *n_atoms_out = atom_off[n_cand];
+  if (!out) return DNAB_OK;
+  if (!out->x || !out->y || !out->z) return DNAB_ERR_INVALID;
+  if (atom_off[n_cand] > out->atom_capacity) return DNAB_ERR_CAPACITY;
+  for (size_t c = 0; c <= n_cand; ++c) cand_res_off[c] = static_cast<uint32_t>(c * n_res);
+  std::atomic<int> status{DNAB_OK};
+  parallel([&](size_t c0, size_t c1) {
+    std::vector<uint32_t> rs(n_res + 1), ca(n_res);
+    for (size_t c = c0; c < c1; ++c) {
+      const size_t a0 = atom_off[c], r0 = c * n_res;
+      dnab_protein_out o{};
+      o.atom_capacity = atom_off[c + 1] - a0;
+      o.x = out->x + a0;
+      o.y = out->y + a0;
+      o.z = out->z + a0;
+      o.charge = out->charge ? out->charge + a0 : nullptr;
+      o.r_min_half = out->r_min_half ? out->r_min_half + a0 : nullptr;
+      o.epsilon = out->epsilon ? out->epsilon + a0 : nullptr;
+      o.hydrophobic = out->hydrophobic ? out->hydrophobic + a0 : nullptr;
+      o.atom_type = out->atom_type ? out->atom_type + a0 : nullptr;
+      o.residue_idx = nullptr;
+      o.res_start = rs.data();
+      o.ca_atom_idx = ca.data();
+      o.amino_acid = out->amino_acid ? out->amino_acid + r0 : nullptr;
+      size_t got = 0;
+      const int st = dnab_build_protein(&tx[r0], &ty[r0], &tz[r0], &taa[r0], nullptr, n_res, &o,
+                                        &got);
+      if (st != DNAB_OK) {
+        status = st;
+        return;
+      }
+      for (size_t r = 0; r < n_res; ++r) {
+        if (out->res_start) out->res_start[r0 + r] = static_cast<uint32_t>(a0 + rs[r]);
+        if (out->ca_atom_idx) out->ca_atom_idx[r0 + r] = static_cast<uint32_t>(a0 + ca[r]);
+        if (out->residue_idx)
+          for (uint32_t k = rs[r]; k < rs[r + 1]; ++k)
+            out->residue_idx[a0 + k] = static_cast<uint32_t>(r0 + r);
+      }
+    }
+  });
+  if (out->res_start) out->res_start[nr] = static_cast<uint32_t>(atom_off[n_cand]);
+  return status.load();
 }
 
 }  // extern "C"

@@ -698,9 +698,10 @@ int dnab_mlp_predict(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop, float* sc
 
 int dnab_prescreen(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop, uint32_t n_keep,
                    uint32_t top_k, uint32_t* idx, double* energy, uint32_t* n_out) {
-  if (!ctx || !n || !pop || pop->ctx != ctx || n_keep == 0 || top_k == 0 || top_k > 4096 ||
-      n_keep > 4096)
-    return DNAB_ERR_INVALID;
+  if (!ctx || !n || !pop || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  if (n_keep == 0 || top_k == 0 || n_keep > 2048 || top_k > n_keep)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID,
+                          "prescreen: need 1 <= top_k <= n_keep <= 2048 (got %u, %u)", top_k, n_keep);
   (void)hipSetDevice(ctx->device);
   const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
   const uint32_t keep = std::min(n_keep, nc);

@@ -264,7 +264,10 @@ int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k) {
     size_t out = static_cast<size_t>(chunks) * k;
     cap = std::max(cap, out);
     if (chunks == 1) break;
-    if (out >= len) return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "top_k too large for chunking");
+    if (out >= len)
+      return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED,
+                            "top-%u of %u entries: k must stay below half of a %d-entry chunk", k,
+                            n, kTopkChunk);
     len = static_cast<uint32_t>(out);
   }
   DNAB_ALLOC_CHECK(ctx, pl.e[0].alloc(cap));

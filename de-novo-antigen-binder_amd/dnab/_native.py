@@ -138,6 +138,9 @@ _PROTOS = {
                                          f32p, u8p]),
     "dnab_synth_binder_shell": (C.c_int, [C.c_size_t, f32p, C.c_float, C.c_float, C.c_uint64,
                                           f32p, f32p, f32p, u8p]),
+    "dnab_synth_shell_population": (C.c_int, [C.c_size_t, C.c_size_t, f32p, C.c_float, C.c_float,
+                                               C.c_uint64, C.c_int, C.POINTER(ProteinOutC), u32p,
+                                               szp]),
     "dnab_grid_create": (C.c_int, [vp, C.c_float, C.POINTER(vp)]),
     "dnab_grid_build": (C.c_int, [vp, vp, f32p, f32p, f32p, C.c_size_t]),
     "dnab_grid_query": (C.c_int, [vp, vp, f32p, f32p, f32p, C.c_size_t, u32p, u32p, C.c_size_t,

@@ -23,8 +23,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
-from .structure import (AtomProtein, PopulationHost, parse_peptide, population_from_proteins,
-                        protein_from_ca_trace)
+from .structure import (AtomCloud, AtomProtein, PopulationHost, parse_peptide,
+                        population_from_proteins, protein_from_ca_trace)
 
 
 def globule_antigen(n_res: int, radius: float, seed: int) -> AtomProtein:
@@ -69,6 +69,39 @@ def shell_binder(n_res: int, center, shell_radius: float, min_radius: float,
     return protein_from_ca_trace(x, y, z, aa)
 
 
+def shell_population(n_cand: int, n_res: int, center, shell_radius: float, min_radius: float,
+                     seed_base: int, antigen_id: int = 0, nthreads: int = 16) -> PopulationHost:
+    """n_cand shell binders at once (dnab_synth_shell_population): the same
+    candidates as shell_binder(n_res, center, shell_radius, min_radius,
+    seed_base + k) for k < n_cand, built by host threads."""
+    ctr = np.ascontiguousarray(center, np.float32)
+    na = C.c_size_t(0)
+    N.check(N.lib.dnab_synth_shell_population(n_cand, n_res, N.ptr(ctr, C.c_float), shell_radius,
+                                              min_radius, seed_base, nthreads, None, None,
+                                              C.byref(na)))
+    m = na.value
+    nr = n_cand * n_res
+    f = {k: np.zeros(max(m, 1), np.float32) for k in ("x", "y", "z", "charge", "rmh", "eps")}
+    hyd = np.zeros(max(m, 1), np.uint8)
+    at = np.zeros(max(m, 1), np.uint8)
+    ri = np.zeros(max(m, 1), np.uint32)
+    rs = np.zeros(nr + 1, np.uint32)
+    ca = np.zeros(max(nr, 1), np.uint32)
+    aa = np.zeros(max(nr, 1), np.uint8)
+    off = np.zeros(n_cand + 1, np.uint32)
+    out = N.ProteinOutC(m, N.ptr(f["x"], C.c_float), N.ptr(f["y"], C.c_float),
+                        N.ptr(f["z"], C.c_float), N.ptr(f["charge"], C.c_float),
+                        N.ptr(f["rmh"], C.c_float), N.ptr(f["eps"], C.c_float),
+                        N.ptr(hyd, C.c_uint8), N.ptr(at, C.c_uint8), N.ptr(ri, C.c_uint32),
+                        N.ptr(rs, C.c_uint32), N.ptr(ca, C.c_uint32), N.ptr(aa, C.c_uint8))
+    N.check(N.lib.dnab_synth_shell_population(n_cand, n_res, N.ptr(ctr, C.c_float), shell_radius,
+                                              min_radius, seed_base, nthreads, C.byref(out),
+                                              N.ptr(off, C.c_uint32), C.byref(na)))
+    cloud = AtomCloud(f["x"][:m], f["y"][:m], f["z"][:m], f["charge"][:m], f["rmh"][:m],
+                      f["eps"][:m], hyd[:m], at[:m], ri[:m])
+    return PopulationHost(np.full(n_cand, antigen_id, np.uint32), off, rs, ca[:nr], aa[:nr], cloud)
+
+
 # Binders start this far outside the globule radius: at the antigen surface,
 # where their atoms have in-cutoff antigen neighbours (at +8 A only ~7% of
 # 20-residue binders touch a 300-residue antigen).
@@ -107,9 +140,8 @@ def config2(n_cand: int = 1024, n_res: int = 20, seed: int = 2) -> Workload:
     r = 25.0
     ag = globule_antigen(300, r, 2000 + seed)
     ctr = ag.ca_center_of_mass()
-    binders = [shell_binder(n_res, ctr, r + SHELL_GAP, r, (2 << 32) | (seed << 20) | k)
-               for k in range(n_cand)]
-    return Workload("c2", [ag], population_from_proteins(binders), 10.0, 10.0, [r + SHELL_GAP],
+    pop = shell_population(n_cand, n_res, ctr, r + SHELL_GAP, r, (2 << 32) | (seed << 20))
+    return Workload("c2", [ag], pop, 10.0, 10.0, [r + SHELL_GAP],
                     200, 10,
                     64, 0, f"300-res globule vs {n_cand} x {n_res}-res shell binders")
 

@@ -140,6 +140,15 @@ int dnab_synth_binder_ring(size_t n_res, const float center[3], float ring_radiu
 int dnab_synth_binder_shell(size_t n_res, const float center[3], float shell_radius,
                             float min_radius, uint64_t seed, float* x, float* y, float* z,
                             uint8_t* amino_acid);
+/* n_cand shell binders of n_res residues (candidate k: seed seed_base + k),
+ * built into one population's arrays with nthreads host threads: per-atom
+ * arrays (atom_capacity), amino_acid / ca_atom_idx / res_start (global
+ * indices, n_cand * n_res (+1) entries), cand_res_off (n_cand + 1). With out
+ * NULL only *n_atoms_out is set (sizing). */
+int dnab_synth_shell_population(size_t n_cand, size_t n_res, const float center[3],
+                                float shell_radius, float min_radius, uint64_t seed_base,
+                                int nthreads, dnab_protein_out* out, uint32_t* cand_res_off,
+                                size_t* n_atoms_out);
 
 /* ---- spatial hash grid (replaces SpatialHashGrid, src/spatial.rs:12-68) ---
  * dnab_grid_create  = SpatialHashGrid::new   (src/spatial.rs:23-28)
@@ -380,7 +389,8 @@ int dnab_mlp_train(dnab_ctx* ctx, dnab_mlp* mlp, dnab_dataset* ds, const dnab_tr
 int dnab_mlp_predict(dnab_ctx* ctx, dnab_mlp* mlp, dnab_population* pop, float* scores);
 /* Screening: features + MLP for every candidate, the n_keep best predictions
  * re-scored by the exact FP32 kernel, the top_k of those by exact energy
- * (ascending, ties by index) written to idx / energy (*n_out entries). */
+ * (ascending, ties by index) written to idx / energy (*n_out entries).
+ * 1 <= top_k <= n_keep <= 2048. */
 int dnab_prescreen(dnab_ctx* ctx, dnab_mlp* mlp, dnab_population* pop, uint32_t n_keep,
                    uint32_t top_k, uint32_t* idx, double* energy, uint32_t* n_out);
 /* Exact top-k of every candidate, for comparison: evaluate all + top-k. */

@@ -129,3 +129,18 @@ def test_population_desc_layout():
     assert off[0] == 0 and off[-1] == len(w.pop.atoms)
     assert (w.pop.res_ca_atom >= w.pop.res_atom_off[:-1]).all()
     assert (w.pop.res_ca_atom < w.pop.res_atom_off[1:]).all()
+
+
+def test_shell_population_matches_per_binder_path():
+    # the threaded batch generator builds the same candidates as one call per binder
+    import numpy as np
+    from dnab import workloads as W
+    from dnab.structure import population_from_proteins
+    ctr = np.array([1.0, -2.0, 0.5], np.float32)
+    seed = (2 << 32) | (7 << 20)
+    batch = W.shell_population(37, 9, ctr, 28.0, 25.0, seed, nthreads=5)
+    ref = population_from_proteins([W.shell_binder(9, ctr, 28.0, 25.0, seed + k) for k in range(37)])
+    for name in ("cand_res_off", "res_atom_off", "res_ca_atom", "res_amino_acid"):
+        assert np.array_equal(getattr(batch, name), getattr(ref, name)), name
+    for name in ("x", "y", "z", "charge", "r_min_half", "epsilon", "hydrophobic"):
+        assert np.array_equal(getattr(batch.atoms, name), getattr(ref.atoms, name)), name
