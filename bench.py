@@ -80,6 +80,7 @@ def saturated_pair_rate(ctx, n_cand, reps=20):
     w = W.config2(n_cand=n_cand, seed=77)
     ags, pop = W.device_objects(ctx, w)
     n_in, n_tested = pop.pair_count()
+    tuning, _ = pop.autotune(reps=3)
     pop.evaluate(forces=True)                      # warm-up
     ctx.synchronize()
     ctx.set_timing(True)
@@ -97,7 +98,7 @@ def saturated_pair_rate(ctx, n_cand, reps=20):
     return {"candidates": n_cand, "pairs_in_cutoff_per_launch": int(n_in),
             "pairs_tested_per_launch": int(n_tested), "kernel_avg_ms": avg_ms,
             "achieved": achieved, "frac": achieved / PEAK_FP32_TFLOPS,
-            "pairs_per_s": n_in / (avg_ms * 1e-3)}
+            "pairs_per_s": n_in / (avg_ms * 1e-3), "pair_tuning": tuning}
 
 
 def prescreen_bench(ctx, args):
@@ -203,9 +204,13 @@ def main():
                                     restraint_radius=w.restraint_radius[0],
                                     use_graph=args.graph, count_pairs=1 if count else 0)
 
-    # warm-up (untimed), then back to the initial state
+    # warm-up (untimed); the autotuner picks the pair kernels' work split on
+    # the state the warm-up leaves (results do not depend on it); then back to
+    # the initial state
     if args.warmup:
         pop.run(params(args.warmup))
+    tuning, tuning_ms = pop.autotune(reps=3)
+    log(f"[bench] autotuned pair work split {tuning} ({tuning_ms * 1e3:.1f} us per evaluation)")
     pop.set_coords(x0, y0, z0)
     ctx.synchronize()
     ctx.set_timing(not args.graph)
@@ -270,6 +275,7 @@ def main():
             "pairs_tested": int(n_tested),
             "evaluations": n_evals,
             "replay_identical": replay_ok,
+            "pair_tuning": tuning,
         },
         "roofline": {
             "bound": "mfma",

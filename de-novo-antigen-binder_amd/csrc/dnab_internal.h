@@ -107,6 +107,14 @@ struct PairHandover {
   uint32_t n_slots;
 };
 
+// Work split of the pair kernels (dnab_pair_tuning); results do not depend
+// on it, only the time does.
+struct PairTuning {
+  uint32_t direct_survivors;  // a tile's wave sums this many antigen atoms itself
+  uint32_t direct_chunks;     // ... or this many chunks, then hands the rest on
+  uint32_t rest_waves;        // waves per workgroup for handed tiles (4, 8, 16)
+};
+
 struct AgAtom {
   float x, y, z, q332;
   float rmh, sqe, hyd, pad;
@@ -219,6 +227,7 @@ struct dnab_population {
   bool counts_pending = false;           // counts filled by the step kernel
   dnab::DevBuf<uint32_t> perm;           // tile_cap * 64 slots -> atom (or ~0u)
   dnab::DevBuf<float4> spos, spar;       // per slot: the atom's pos and par
+  dnab::PairTuning tune{48, 16, 8};
   dnab::DevBuf<uint32_t> ho_rest;         // handover: chunks left per tile
   dnab::DevBuf<uint32_t> ho_list;         // handover: tiles with chunks left, and the count
   bool ho_dirty = false;                  // count left set by the last pair launch

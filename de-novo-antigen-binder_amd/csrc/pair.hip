@@ -466,17 +466,14 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
 //
 // Balance. k_pair_chunks runs one wave per tile (four tiles per workgroup,
 // so every tile of a step is resident at once); it loads chunks four at a
-// time and sums them until it has taken kDirectChunks chunks or
-// kDirectSurvivors antigen atoms. A tile deep inside the antigen has ~10x the
+// time and sums them until it has taken its share (PairTuning: a number of
+// chunks or of surviving antigen atoms). A tile deep inside the antigen has ~10x the
 // median work: its wave hands the remaining chunks and its partial sums to
-// k_pair_rest, where a workgroup of kRestWaves waves shares them. The two
+// k_pair_rest, where a workgroup of WAVES waves shares them. The two
 // kernels meet through plain stores and loads in device memory (the kernel
 // boundary orders them), with no device-scope atomics or fences, which on
 // this GPU would reach across the XCDs' separate L2 caches.
-constexpr uint32_t kDirectSurvivors = 48;
-constexpr uint32_t kDirectChunks = 16;
 constexpr int kLoadAhead = 4;   // chunk loads in flight per round trip
-constexpr int kRestWaves = 8;
 constexpr double kFixScale = 1048576.0;          // 2^20
 constexpr double kFixUnit = 1.0 / 1048576.0;
 constexpr long long kFixLim = 1LL << 59;         // accumulator bound; sums of 4 stay below 2^61
@@ -617,7 +614,7 @@ __global__ void __launch_bounds__(256)
 k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
               const uint32_t* __restrict__ ag_tile_start, const uint32_t* __restrict__ perm,
               const float4* __restrict__ spos, const float4* __restrict__ spar, TileOut out,
-              PairHandover ho) {
+              PairHandover ho, PairTuning tune) {
   __shared__ float4 s_atoms[kTileWaves][kWave][2];  // a wave's staged chunk atoms
 
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -694,7 +691,7 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
     // has had its share and the handover table has room
     uint32_t k = 0, k_hand = total;  // chunks [k_hand, total) are handed over
     for (; k < total && k_hand == total; k += kLoadAhead) {
-      if (taken_atoms >= kDirectSurvivors || taken_chunks >= kDirectChunks) handing = true;
+      if (taken_atoms >= tune.direct_survivors || taken_chunks >= tune.direct_chunks) handing = true;
       if (handing && handed + (total - k) <= kHandoverCap) {
         k_hand = k;
         break;
@@ -722,7 +719,7 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
       for (int u = 0; u < kLoadAhead; ++u) {
         if (cev[u] <= cbv[u]) continue;
         // stop at this chunk once the tile has had its share (and the rest fits)
-        if ((taken_atoms >= kDirectSurvivors || taken_chunks >= kDirectChunks) &&
+        if ((taken_atoms >= tune.direct_survivors || taken_chunks >= tune.direct_chunks) &&
             handed + (total - (k + u)) <= kHandoverCap) {
           handing = true;
           k_hand = k + u;
@@ -782,16 +779,16 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
   ho.part[4 * ns + slot_id] = acc.z;
 }
 
-// The chunks a tile's wave handed over: kRestWaves waves take them from an
+// The chunks a tile's wave handed over: WAVES waves take them from an
 // LDS counter, sum per chunk into fixed-point integers, combine in LDS, add
 // the wave's partial sums and write the tile's outputs.
-template <bool FORCES, bool COUNT>
-__global__ void __launch_bounds__(kRestWaves * 64)
+template <bool FORCES, bool COUNT, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64)
 k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
             const uint32_t* __restrict__ ag_tile_start, const uint32_t* __restrict__ perm,
             const float4* __restrict__ spos, const float4* __restrict__ spar, TileOut out,
             PairHandover ho) {
-  __shared__ float4 s_atoms[kRestWaves][kWave][2];
+  __shared__ float4 s_atoms[WAVES][kWave][2];
   __shared__ long long s_sum[5][kWave];
   __shared__ uint32_t s_next;
   const uint32_t n_heavy = min(*ho.n_list, ho.n_slots / kWave);
@@ -822,7 +819,7 @@ k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
   const float hiy = wave_reduce_f32<OpMax>(valid ? yb : -inf);
   const float hiz = wave_reduce_f32<OpMax>(valid ? zb : -inf);
   __syncthreads();  // the previous tile's LDS is no longer read
-  if (tid < 5 * kWave) s_sum[tid >> 6][tid & 63u] = 0;
+  for (uint32_t i = tid; i < 5 * kWave; i += WAVES * kWave) s_sum[i >> 6][i & 63u] = 0;
   if (tid == 0) s_next = 0;
   __syncthreads();
 
@@ -1155,11 +1152,24 @@ static void launch_pair(dnab_ctx* ctx, dnab_population* pop, const TileOut& o) {
   pop->ho_dirty = true;  // cleared by the next sort
   hipLaunchKernelGGL((k_pair_chunks<FORCES, COUNT>), dim3((cap + kTileWaves - 1) / kTileWaves),
                      dim3(256), 0, ctx->stream, pop->views.p, n_ag, pop->ag_tile_start.p,
-                     pop->perm.p, pop->spos.p, pop->spar.p, o, h);
-  hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT>), dim3(static_cast<uint32_t>(ctx->cu_count * 2)),
-                     dim3(kRestWaves * 64), 0,
-                     ctx->stream, pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p,
-                     pop->spos.p, pop->spar.p, o, h);
+                     pop->perm.p, pop->spos.p, pop->spar.p, o, h, pop->tune);
+  const dim3 rest_grid(static_cast<uint32_t>(ctx->cu_count * 2));
+  switch (pop->tune.rest_waves) {
+    case 4:
+      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, 4>), rest_grid, dim3(4 * 64), 0, ctx->stream,
+                         pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->spos.p,
+                         pop->spar.p, o, h);
+      break;
+    case 16:
+      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, 16>), rest_grid, dim3(16 * 64), 0, ctx->stream,
+                         pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->spos.p,
+                         pop->spar.p, o, h);
+      break;
+    default:
+      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, 8>), rest_grid, dim3(8 * 64), 0, ctx->stream,
+                         pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->spos.p,
+                         pop->spar.p, o, h);
+  }
 }
 
 int population_pair(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
@@ -1686,6 +1696,78 @@ int dnab_population_write_coords(dnab_ctx* ctx, dnab_population* pop, const floa
   pop->evaluated = false;
   DNAB_RETURN_IF(population_sort(ctx, pop));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNAB_OK;
+}
+
+int dnab_population_get_tuning(const dnab_population* pop, dnab_pair_tuning* out) {
+  if (!pop || !out) return DNAB_ERR_INVALID;
+  out->direct_survivors = pop->tune.direct_survivors;
+  out->direct_chunks = pop->tune.direct_chunks;
+  out->rest_waves = pop->tune.rest_waves;
+  return DNAB_OK;
+}
+
+int dnab_population_set_tuning(dnab_ctx* ctx, dnab_population* pop, const dnab_pair_tuning* t) {
+  if (!ctx || !pop || !t || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  if (t->direct_survivors == 0 || t->direct_chunks == 0 ||
+      (t->rest_waves != 4 && t->rest_waves != 8 && t->rest_waves != 16))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID,
+                          "tuning: survivors and chunks >= 1, rest_waves 4, 8 or 16");
+  pop->tune = PairTuning{t->direct_survivors, t->direct_chunks, t->rest_waves};
+  return DNAB_OK;
+}
+
+int dnab_population_autotune(dnab_ctx* ctx, dnab_population* pop, uint32_t reps,
+                             dnab_pair_tuning* best, double* best_ms) {
+  if (!ctx || !pop || pop->ctx != ctx || reps == 0) return DNAB_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  pop->masked = false;
+  if (!pop->sorted) DNAB_RETURN_IF(population_sort(ctx, pop));
+  TileOut o{pop->force.p, pop->e_atom.p, pop->bd_atom.p, pop->pair_stats.p};
+  hipEvent_t e0, e1;
+  DNAB_HIP_CHECK(ctx, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return dnab_set_error(ctx, DNAB_ERR_HIP, "autotune: event creation failed");
+  }
+  const uint32_t surv[] = {24, 48, 96, 192}, chunks[] = {8, 16, 32}, waves[] = {4, 8, 16};
+  const PairTuning saved = pop->tune;
+  PairTuning win = saved;
+  double win_ms = 1e30;
+  int rc = DNAB_OK;
+  launch_pair<true, false>(ctx, pop, o);  // warm-up
+  for (uint32_t w : waves)
+    for (uint32_t c : chunks)
+      for (uint32_t sv : surv) {
+        pop->tune = PairTuning{sv, c, w};
+        (void)hipEventRecord(e0, ctx->stream);
+        for (uint32_t r = 0; r < reps; ++r) launch_pair<true, false>(ctx, pop, o);
+        (void)hipEventRecord(e1, ctx->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          rc = dnab_set_error(ctx, DNAB_ERR_HIP, "autotune: launch failed");
+          break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double per = static_cast<double>(ms) / reps;
+        if (per < win_ms) {
+          win_ms = per;
+          win = pop->tune;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  pop->tune = rc ? saved : win;
+  if (rc) return rc;
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  DNAB_RETURN_IF(population_reduce_energies(ctx, pop, DNAB_EVAL_FORCES));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (best) {
+    best->direct_survivors = win.direct_survivors;
+    best->direct_chunks = win.direct_chunks;
+    best->rest_waves = win.rest_waves;
+  }
+  if (best_ms) *best_ms = win_ms;
   return DNAB_OK;
 }
 

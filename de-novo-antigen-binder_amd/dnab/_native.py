@@ -65,6 +65,11 @@ class EnergyBreakdownC(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("lj", "coulomb", "hydrophobic", "hbond", "solvation")]
 
 
+class PairTuningC(C.Structure):
+    _fields_ = [("direct_survivors", C.c_uint32), ("direct_chunks", C.c_uint32),
+                ("rest_waves", C.c_uint32)]
+
+
 class ProteinOutC(C.Structure):
     _fields_ = [("atom_capacity", C.c_size_t), ("x", f32p), ("y", f32p), ("z", f32p),
                 ("charge", f32p), ("r_min_half", f32p), ("epsilon", f32p),
@@ -138,6 +143,10 @@ _PROTOS = {
                                          f32p, u8p]),
     "dnab_synth_binder_shell": (C.c_int, [C.c_size_t, f32p, C.c_float, C.c_float, C.c_uint64,
                                           f32p, f32p, f32p, u8p]),
+    "dnab_population_get_tuning": (C.c_int, [vp, C.POINTER(PairTuningC)]),
+    "dnab_population_set_tuning": (C.c_int, [vp, vp, C.POINTER(PairTuningC)]),
+    "dnab_population_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(PairTuningC),
+                                           C.POINTER(C.c_double)]),
     "dnab_synth_shell_population": (C.c_int, [C.c_size_t, C.c_size_t, f32p, C.c_float, C.c_float,
                                                C.c_uint64, C.c_int, C.POINTER(ProteinOutC), u32p,
                                                szp]),

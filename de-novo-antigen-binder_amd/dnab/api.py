@@ -352,6 +352,25 @@ class DevicePopulation:
                                                           N.ptr(y, C.c_float),
                                                           N.ptr(z, C.c_float)))
 
+    def tuning(self) -> dict:
+        t = N.PairTuningC()
+        N.check(N.lib.dnab_population_get_tuning(self._h, C.byref(t)))
+        return {"direct_survivors": t.direct_survivors, "direct_chunks": t.direct_chunks,
+                "rest_waves": t.rest_waves}
+
+    def set_tuning(self, direct_survivors: int, direct_chunks: int, rest_waves: int):
+        t = N.PairTuningC(direct_survivors, direct_chunks, rest_waves)
+        self.ctx.check(N.lib.dnab_population_set_tuning(self.ctx.handle, self._h, C.byref(t)))
+
+    def autotune(self, reps: int = 3) -> tuple[dict, float]:
+        """Pick the fastest pair-kernel work split for the current state."""
+        t = N.PairTuningC()
+        ms = C.c_double()
+        self.ctx.check(N.lib.dnab_population_autotune(self.ctx.handle, self._h, reps, C.byref(t),
+                                                      C.byref(ms)))
+        return ({"direct_survivors": t.direct_survivors, "direct_chunks": t.direct_chunks,
+                 "rest_waves": t.rest_waves}, ms.value)
+
     def pair_count(self):
         a, b = C.c_uint64(), C.c_uint64()
         self.ctx.check(N.lib.dnab_population_pair_count(self.ctx.handle, self._h, C.byref(a),

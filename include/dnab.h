@@ -282,6 +282,23 @@ int dnab_population_write_coords(dnab_ctx* ctx, dnab_population* pop, const floa
                                  const float* y, const float* z);
 /* Pair statistics at the current coordinates: pairs inside the FP32
  * predicate 0.25 <= r^2 <= cutoff^2, and pairs the kernel tested. */
+/* Work split of the pair kernels and its autotuner (no reference
+ * counterpart; north_star (b)). Per-chunk sums are added as exact fixed-point
+ * integers, so every setting gives bit-identical results: only the time
+ * changes. */
+typedef struct dnab_pair_tuning {
+  uint32_t direct_survivors; /* antigen atoms a tile's own wave sums (>= 1) */
+  uint32_t direct_chunks;    /* ... or chunks (>= 1); the rest goes to a workgroup */
+  uint32_t rest_waves;       /* waves per workgroup for handed-on tiles: 4, 8 or 16 */
+} dnab_pair_tuning;
+int dnab_population_get_tuning(const dnab_population* pop, dnab_pair_tuning* out);
+int dnab_population_set_tuning(dnab_ctx* ctx, dnab_population* pop, const dnab_pair_tuning* t);
+/* Times `reps` energy+force evaluations of the population's current state
+ * (HIP events on the context stream) for each setting of a fixed grid
+ * (survivors 24/48/96/192 x chunks 8/16/32 x waves 4/8/16), keeps the
+ * fastest and returns it with its mean time per evaluation. */
+int dnab_population_autotune(dnab_ctx* ctx, dnab_population* pop, uint32_t reps,
+                             dnab_pair_tuning* best, double* best_ms);
 int dnab_population_pair_count(dnab_ctx* ctx, dnab_population* pop, uint64_t* in_cutoff,
                                uint64_t* tested);
 int dnab_population_info(const dnab_population* pop, size_t* n_cand, size_t* n_res,

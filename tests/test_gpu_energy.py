@@ -196,3 +196,34 @@ def test_err_estimate_reported(ctx):
     assert (err >= 0).all()
     print(f"max actual {actual.max():.3g}, max estimate {err.max():.3g}, "
           f"frac actual<=estimate {(actual <= err + 1e-9).mean():.3f}")
+
+
+def test_tuning_changes_speed_not_results(ctx):
+    # the pair kernels add per-chunk sums as exact integers: any work split
+    # gives the same bits; the autotuner only picks the fastest
+    w = W.config2(n_cand=256)
+    ags, pop = W.device_objects(ctx, w)
+    ref = None
+    for sv, ch, wv in ((1, 1, 4), (48, 16, 8), (10000, 10000, 16), (24, 8, 16)):
+        pop.set_tuning(sv, ch, wv)
+        pop.evaluate(forces=True)
+        e, err = pop.energies()
+        f = pop.forces()
+        bits = (e.view(np.uint64).copy(), err.view(np.uint32).copy(),
+                [a.view(np.uint32).copy() for a in f])
+        if ref is None:
+            ref = bits
+        else:
+            assert np.array_equal(bits[0], ref[0]) and np.array_equal(bits[1], ref[1])
+            for a, b in zip(bits[2], ref[2]):
+                assert np.array_equal(a, b)
+    best, ms = pop.autotune(reps=2)
+    assert best == pop.tuning() and ms > 0
+    assert best["rest_waves"] in (4, 8, 16)
+    pop.evaluate(forces=True)
+    e2, _ = pop.energies()
+    assert np.array_equal(e2.view(np.uint64), ref[0])
+    with pytest.raises(Exception):
+        pop.set_tuning(0, 16, 8)
+    with pytest.raises(Exception):
+        pop.set_tuning(48, 16, 5)
