@@ -172,6 +172,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--n-cand", type=int, default=1024)
     ap.add_argument("--no-prescreen", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--prescreen-train", type=int, default=1_000_000,
                     help="training rows (candidate, exact energy) for the prescreen")
     ap.add_argument("--prescreen-pool", type=int, default=65536)
@@ -292,6 +293,15 @@ def main():
     }
     log("[bench] pair kernels with enough work (16,384 binders in contact)")
     out["roofline"]["saturated"] = saturated_pair_rate(ctx, 16384)
+    if not args.no_precision:
+        log("[bench] precision study at the config-4 shape")
+        wp = W.config4(n_cand=1024)
+        ags4, pop4 = W.device_objects(ctx, wp)
+        out["precision"] = dict(workload=wp.description, antigen_atoms=int(wp.antigens[0].n_atoms()),
+                                **dnab.precision_study(pop4))
+        pop4.close()
+        for a in ags4:
+            a.close()
     if not args.no_prescreen:
         log("[bench] learned prescreen")
         out["prescreen"] = prescreen_bench(ctx, args)

@@ -7,14 +7,15 @@ scripting; it raises on import when the library is not built.
 from . import _native
 from ._native import DnabError, device_count, visible_gpu
 from .api import (AtomEnergyBreakdown, Dataset, DeviceAntigen, DevicePopulation, GpuContext,
-                  Mlp, SpatialHashGrid, exact_topk, langevin_params, population_features)
+                  Mlp, SpatialHashGrid, exact_topk, langevin_params, population_features,
+                  precision_study)
 from .structure import (AMINO_ONE_LETTER, AtomCloud, AtomProtein, PopulationHost, ResidueCloud,
                         aa_from_sequence, parse_peptide, population_from_proteins,
                         protein_from_ca_trace)
 
 __all__ = [
     "AMINO_ONE_LETTER", "AtomCloud", "AtomEnergyBreakdown", "AtomProtein", "Dataset",
-    "DeviceAntigen", "Mlp", "exact_topk", "population_features",
+    "DeviceAntigen", "Mlp", "exact_topk", "population_features", "precision_study",
     "DevicePopulation", "DnabError", "GpuContext", "PopulationHost", "ResidueCloud",
     "SpatialHashGrid", "aa_from_sequence", "device_count", "langevin_params", "parse_peptide",
     "population_from_proteins", "protein_from_ca_trace", "visible_gpu", "_native",

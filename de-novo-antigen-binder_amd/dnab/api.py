@@ -529,6 +529,34 @@ class Dataset:
             pass
 
 
+def precision_study(pop: "DevicePopulation", n_atoms_per_cand=None) -> dict:
+    """FP32 path (per-chunk TwoSum, exact fixed-point across chunks, FP64
+    totals) against the all-FP64 kernel on the same pair set: per-candidate
+    energy error, per-atom force RMS error, and how often the a-priori error
+    estimate covers the measured error."""
+    pop.evaluate(forces=True)
+    e32, est = pop.energies()
+    f32 = np.stack(pop.forces(), 1).astype(np.float64)
+    pop.evaluate(fp64=True)
+    e64, _ = pop.energies()
+    f64 = np.stack(pop.forces_f64(), 1)
+    de = np.abs(e32 - e64)
+    rel = de / np.maximum(np.abs(e64), 1e-30)
+    df = f32 - f64
+    tol_e = 1e-3 + 1e-4 * np.abs(e64)
+    tol_f = 1e-3 + 1e-4 * np.abs(f64)
+    return {"candidates": int(len(e32)), "atoms": int(len(f32)),
+            "energy_abs_err_max": float(de.max()), "energy_abs_err_median": float(np.median(de)),
+            "energy_rel_err_max": float(rel.max()),
+            "force_rms_err": float(np.sqrt(np.mean(df ** 2))),
+            "force_rms": float(np.sqrt(np.mean(f64 ** 2))),
+            "force_abs_err_max": float(np.abs(df).max()),
+            "energy_within_tol": float(np.mean(de <= tol_e)),
+            "force_within_tol": float(np.mean(np.abs(df) <= tol_f)),
+            "estimate_covers": float(np.mean(de <= est + 1e-12)),
+            "energy_range": [float(e64.min()), float(np.median(e64)), float(e64.max())]}
+
+
 def population_features(pop: DevicePopulation) -> np.ndarray:
     out = np.zeros((max(pop.n_cand, 1), N.FEATURE_DIM), np.uint16)
     pop.ctx.check(N.lib.dnab_population_features(pop.ctx.handle, pop.handle,

@@ -167,9 +167,8 @@ def config4(n_cand: int = 1024, n_res: int = 30, seed: int = 4) -> Workload:
     r = globule_radius(5000)
     ag = globule_antigen(5000, r, 4000 + seed)
     ctr = ag.ca_center_of_mass()
-    binders = [shell_binder(n_res, ctr, r + SHELL_GAP, r, (4 << 32) | (seed << 20) | k)
-               for k in range(n_cand)]
-    return Workload("c4", [ag], population_from_proteins(binders), 12.0, 12.0, [r + SHELL_GAP],
+    pop = shell_population(n_cand, n_res, ctr, r + SHELL_GAP, r, (4 << 32) | (seed << 20))
+    return Workload("c4", [ag], pop, 12.0, 12.0, [r + SHELL_GAP],
                     200, 10,
                     64, 0, f"5000-res globule vs {n_cand} x {n_res}-res binders")
 
