@@ -70,6 +70,36 @@ def cpu_baseline(w, budget_s=12.0):
                        f"brute-force energy pattern: {pairs * reps_b / dtb:.3g} pairs/s")}
 
 
+def copy_calibration(mib):
+    """A device-to-device copy of `mib` MiB through the HIP runtime (its copy
+    kernel reads and writes exactly that many bytes): the reference point
+    that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def measured_traffic():
+    """HBM bytes per pair launch (both kernels) of the c2 loop, from the
+    committed FETCH_SIZE / WRITE_SIZE profile, or None."""
+    f = ROOT / "profiles" / "pair_traffic_c2.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split()[1])
+    return None
+
+
 def saturated_pair_rate(ctx, n_cand, reps=20):
     """The pair kernels fed enough work: the c2 antigen with n_cand binders at
     their starting positions (surface contact), energies + forces evaluated
@@ -173,6 +203,11 @@ def main():
     ap.add_argument("--n-cand", type=int, default=1024)
     ap.add_argument("--no-prescreen", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-saturated", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help="fixed pair work split 'survivors,chunks,waves' instead of autotuning")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="MiB copied device-to-device first (reference point for traffic counters)")
     ap.add_argument("--prescreen-train", type=int, default=1_000_000,
                     help="training rows (candidate, exact energy) for the prescreen")
     ap.add_argument("--prescreen-pool", type=int, default=65536)
@@ -187,6 +222,8 @@ def main():
     import dnab
     from dnab import workloads as W
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     ctx = dnab.GpuContext.try_init(0)
     if ctx is None:
         print("no gfx950 GPU visible", file=sys.stderr)
@@ -210,8 +247,13 @@ def main():
     # the initial state
     if args.warmup:
         pop.run(params(args.warmup))
-    tuning, tuning_ms = pop.autotune(reps=3)
-    log(f"[bench] autotuned pair work split {tuning} ({tuning_ms * 1e3:.1f} us per evaluation)")
+    if args.tuning:
+        pop.set_tuning(*(int(v) for v in args.tuning.split(",")))
+        tuning = pop.tuning()
+        log(f"[bench] fixed pair work split {tuning}")
+    else:
+        tuning, tuning_ms = pop.autotune(reps=3)
+        log(f"[bench] autotuned pair work split {tuning} ({tuning_ms * 1e3:.1f} us per evaluation)")
     pop.set_coords(x0, y0, z0)
     ctx.synchronize()
     ctx.set_timing(not args.graph)
@@ -285,14 +327,15 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "k_pair_chunks + k_pair_rest (energy + forces)",
             "kernel_avg_ms": avg_ms,
             "flop_per_pair": FLOP_PER_PAIR_EF,
         },
     }
-    log("[bench] pair kernels with enough work (16,384 binders in contact)")
-    out["roofline"]["saturated"] = saturated_pair_rate(ctx, 16384)
+    if not args.no_saturated:
+        log("[bench] pair kernels with enough work (16,384 binders in contact)")
+        out["roofline"]["saturated"] = saturated_pair_rate(ctx, 16384)
     if not args.no_precision:
         log("[bench] precision study at the config-4 shape")
         wp = W.config4(n_cand=1024)
