@@ -88,10 +88,11 @@ def copy_calibration(mib):
     hip.hipFree(b)
 
 
-def measured_traffic():
-    """HBM bytes per pair launch (both kernels) of the c2 loop, from the
-    committed FETCH_SIZE / WRITE_SIZE profile, or None."""
-    f = ROOT / "profiles" / "pair_traffic_c2.txt"
+def measured_traffic(step_kernels):
+    """HBM bytes per launch of the dominant kernel of the c2 loop (the fused
+    trajectory kernel, or the two pair kernels of a step), from the committed
+    FETCH_SIZE / WRITE_SIZE profile, or None."""
+    f = ROOT / "profiles" / ("pair_traffic_c2.txt" if step_kernels else "trajectory_traffic_c2.txt")
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
@@ -197,7 +198,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--graph", type=int, default=1, help="replay the step loop as a hipGraph")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="per-step kernel path: replay the step loop as a hipGraph")
+    ap.add_argument("--step-kernels", type=int, default=0,
+                    help="time the per-step kernel path (1) instead of the fused trajectory kernel (0)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--n-cand", type=int, default=1024)
@@ -237,16 +241,47 @@ def main():
     ags, pop = W.device_objects(ctx, w)
     x0, y0, z0 = pop.coords()
 
-    def params(n, count=False):
+    def params(n, count=False, step_kernels=args.step_kernels):
         return dnab.langevin_params(n_steps=n, score_every=w.score_every, top_k=w.top_k,
                                     restraint_radius=w.restraint_radius[0],
-                                    use_graph=args.graph, count_pairs=1 if count else 0)
+                                    use_graph=args.graph, count_pairs=1 if count else 0,
+                                    step_kernels=step_kernels)
 
-    # warm-up (untimed); the autotuner picks the pair kernels' work split on
-    # the state the warm-up leaves (results do not depend on it); then back to
-    # the initial state
+    def timed_run(step_kernels):
+        """Wall time of the loop from the initial state, and the event-timed
+        duration of its dominant kernel (per launch)."""
+        pop.set_coords(x0, y0, z0)
+        ctx.synchronize()
+        name = "pair_ef" if step_kernels else "trajectory"
+        # the fused loop is one launch: its events cost nothing; graph
+        # replays carry no per-kernel events
+        ctx.set_timing(not step_kernels or not args.graph)
+        k0 = ctx.kernel_time(name)
+        t0 = time.perf_counter()
+        idx, _ = pop.run(params(args.steps, step_kernels=step_kernels))
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        k1 = ctx.kernel_time(name)
+        ctx.set_timing(False)
+        if step_kernels and args.graph:  # per-kernel time from an eager pass
+            pop.set_coords(x0, y0, z0)
+            ctx.set_timing(True)
+            k0 = ctx.kernel_time(name)
+            pop.run(dnab.langevin_params(n_steps=args.steps, score_every=w.score_every,
+                                         top_k=w.top_k, restraint_radius=w.restraint_radius[0],
+                                         use_graph=0, step_kernels=1))
+            k1 = ctx.kernel_time(name)
+            ctx.set_timing(False)
+        launches = max(1, k1[1] - k0[1])
+        return dt, (k1[0] - k0[0]) / launches, idx
+
+    # warm-up (untimed) of both paths; the autotuner picks the per-step pair
+    # kernels' work split on the state the warm-up leaves (results do not
+    # depend on it)
     if args.warmup:
-        pop.run(params(args.warmup))
+        pop.run(params(args.warmup, step_kernels=1))
+        pop.set_coords(x0, y0, z0)
+        pop.run(params(args.warmup, step_kernels=0))
     if args.tuning:
         pop.set_tuning(*(int(v) for v in args.tuning.split(",")))
         tuning = pop.tuning()
@@ -254,17 +289,14 @@ def main():
     else:
         tuning, tuning_ms = pop.autotune(reps=3)
         log(f"[bench] autotuned pair work split {tuning} ({tuning_ms * 1e3:.1f} us per evaluation)")
-    pop.set_coords(x0, y0, z0)
-    ctx.synchronize()
-    ctx.set_timing(not args.graph)
-    k0_ms, k0_n = ctx.kernel_time("pair_ef")
-    t0 = time.perf_counter()
-    idx, etop = pop.run(params(args.steps))
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    k1_ms, k1_n = ctx.kernel_time("pair_ef")
-    ctx.set_timing(False)
+    other = 1 - args.step_kernels
+    dt, avg_ms, idx = timed_run(args.step_kernels)
     x1, y1, z1 = pop.coords()
+    dt_other, avg_other, idx_other = timed_run(other)
+    xo, yo, zo = pop.coords()
+    paths_identical = bool(np.array_equal(xo, x1) and np.array_equal(idx_other, idx))
+    log(f"[bench] {'step kernels' if args.step_kernels else 'fused'}: {dt * 1e3:.2f} ms; "
+        f"{'step kernels' if other else 'fused'}: {dt_other * 1e3:.2f} ms; identical {paths_identical}")
 
     # exact pair count: replay the same deterministic trajectory, counting
     pop.set_coords(x0, y0, z0)
@@ -275,21 +307,14 @@ def main():
     replay_ok = bool(np.array_equal(xc, x1) and np.array_equal(idx2, idx))
     n_evals = args.steps + 1            # K steps + the final scoring evaluation
     pairs_per_eval = n_in / n_evals
-
-    # pair-kernel duration: event-timed eager launches (graph replays carry no
-    # per-kernel events, so time an eager pass of the same loop)
-    if args.graph:
-        pop.set_coords(x0, y0, z0)
-        ctx.set_timing(True)
-        k0_ms, k0_n = ctx.kernel_time("pair_ef")
-        pop.run(dnab.langevin_params(n_steps=args.steps, score_every=w.score_every,
-                                     top_k=w.top_k, restraint_radius=w.restraint_radius[0],
-                                     use_graph=0))
-        k1_ms, k1_n = ctx.kernel_time("pair_ef")
-        ctx.set_timing(False)
-    launches = max(1, k1_n - k0_n)
-    avg_ms = (k1_ms - k0_ms) / launches
-    achieved = pairs_per_eval * FLOP_PER_PAIR_EF / (avg_ms * 1e-3) / 1e12
+    if args.step_kernels:
+        achieved = pairs_per_eval * FLOP_PER_PAIR_EF / (avg_ms * 1e-3) / 1e12
+        kernel = "k_pair_chunks + k_pair_rest (energy + forces), per step"
+    else:
+        achieved = n_in * FLOP_PER_PAIR_EF / (avg_ms * 1e-3) / 1e12
+        kernel = "k_trajectory (every step and the final evaluation, one launch)"
+    step_ms = {"fused": (dt_other if args.step_kernels else dt) * 1e3 / args.steps,
+               "step_kernels": (dt if args.step_kernels else dt_other) * 1e3 / args.steps}
 
     value = n_in / dt
     out = {
@@ -313,7 +338,10 @@ def main():
             "candidates": int(w.pop.n_cand),
             "candidate_atoms": int(len(w.pop.atoms)),
             "cutoff_A": w.cutoff,
-            "graph": bool(args.graph),
+            "path": "step kernels" if args.step_kernels else "fused trajectory kernel",
+            "graph": bool(args.graph) if args.step_kernels else None,
+            "ms_per_step_by_path": step_ms,
+            "paths_bit_identical": paths_identical,
             "pairs_in_cutoff": int(n_in),
             "pairs_tested": int(n_tested),
             "evaluations": n_evals,
@@ -327,8 +355,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": measured_traffic(),
-            "kernel": "k_pair_chunks + k_pair_rest (energy + forces)",
+            "traffic": measured_traffic(args.step_kernels),
+            "kernel": kernel,
             "kernel_avg_ms": avg_ms,
             "flop_per_pair": FLOP_PER_PAIR_EF,
         },

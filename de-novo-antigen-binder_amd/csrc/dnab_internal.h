@@ -214,6 +214,12 @@ struct dnab_population {
   dnab::DevBuf<uint32_t> atom_res;   // per atom: its residue
   dnab::DevBuf<uint32_t> cand_atom_off, cand_res_off, cand_ag;
   dnab::DevBuf<uint32_t> res_atom_off, res_ca, res_cand;
+  uint32_t max_cand_atoms = 0, max_cand_res = 0;  // largest candidate
+  std::vector<uint32_t> h_cand_atom_off, h_cand_res_off;  // host copies
+  // fused trajectory kernel: workgroup shape, grouping by measured work
+  uint32_t traj_waves = 16, traj_cpw = 6, traj_cells = 2;
+  bool traj_balance = true;
+  dnab::DevBuf<uint32_t> traj_work, traj_group_off, traj_group_cand;
   dnab::DevBuf<uint8_t> res_aa;         // residue types, when given
   bool has_res_aa = false;
   dnab::DevBuf<float4> res_cg;  // residue-level params (q, eps, sigma, hydrophobic)
@@ -252,6 +258,11 @@ struct dnab_population {
   dnab::DevBuf<unsigned long long> pair_stats;  // (in cutoff, tested)
   dnab::DevBuf<uint32_t> topk_idx;
   dnab::DevBuf<double> topk_e;
+  // score history of the last loop run: hist_rows x hist_k (top k per scoring)
+  dnab::DevBuf<uint32_t> hist_idx;
+  dnab::DevBuf<double> hist_e;
+  uint32_t hist_rows = 0, hist_k = 0;
+  dnab::DevBuf<double> score_e;  // fused loop: candidate energies per scoring
   // candidate mask: only masked-in candidates are sorted and scored
   dnab::DevBuf<uint8_t> cand_mask;
   bool masked = false;
@@ -303,13 +314,15 @@ int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 // step.hip: multi-pass top-k (ascending, NaN last, ties by index)
 struct TopkPlan {
   uint32_t k = 0;
+  uint32_t rows = 1;
   std::vector<uint32_t> lens;
   DevBuf<double> e[2];
   DevBuf<uint32_t> idx[2];
 };
-int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k);
+int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k, uint32_t rows = 1);
 void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
-                 uint32_t* final_idx);
+                 uint32_t* final_idx, const uint32_t* counter = nullptr, uint32_t row_div = 1,
+                 uint32_t row_fixed = 0);
 // pair.hip
 int population_sort(dnab_ctx* ctx, dnab_population* pop);
 int population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags);

@@ -11,23 +11,23 @@
 //
 // Layout and mapping (gfx950, wave64):
 //  * antigen atoms are sorted cell-major over its SpatialHashGrid (Morton
-//    order of cells), one 32-byte AgAtom each; a wave loads a cell's atoms
-//    64 at a time (one per lane, coalesced) and the pair loop broadcasts
-//    atom k to all lanes with v_readlane into SGPRs, so the pair loop itself
-//    has no memory accesses;
-//  * candidate atoms of the whole population are counting-sorted by a 4 A
-//    Morton cell around their antigen; a tile is 64 consecutive sorted atoms,
-//    one per lane, so a tile's atoms are spatially compact whichever
-//    candidate they belong to;
-//  * a workgroup of 4 waves shares one tile: the waves walk the grid cells
-//    overlapping the tile box + cutoff in the same fixed order, test each
-//    cell's antigen atoms against the box (one atom per lane + ballot), and
-//    split the surviving atoms by (sorted index & 3); partial sums meet in
-//    LDS in wave order. Each atom's result therefore does not depend on the
-//    tile it landed in, and repeated runs are bit-identical.
-//  * energy per atom: FP32 pair terms, TwoSum-compensated (lj + coulomb),
-//    hydrophobic pairs counted as an integer; per-atom value stored in FP64
-//    and summed per candidate in FP64 with DPP wave reductions.
+//    order of cells), one 32-byte AgAtom each; a wave loads a chunk (up to
+//    64 atoms of one cell) one atom per lane, keeps the atoms near its tile
+//    (ballot) and stages them in LDS, from where the pair loop broadcasts
+//    them to all lanes;
+//  * candidate atoms of the whole population are counting-sorted every
+//    evaluation into 4 A Morton cells inside 16 A bricks around their
+//    antigen; a tile is 64 consecutive sorted slots, one atom per lane, and
+//    every brick starts on a tile boundary, so a tile's atoms are spatially
+//    compact whichever candidates they belong to;
+//  * k_pair_chunks runs one wave per tile and hands the chunks beyond its
+//    share to k_pair_rest, where the waves of a workgroup share them;
+//  * every (atom, chunk) sum is rounded once to a 64-bit integer in units of
+//    2^-20 (pair_device.h), so each atom's result does not depend on its
+//    tile or on which wave took which chunk, and repeated runs are
+//    bit-identical;
+//  * the per-atom energy is stored in FP64 and summed per candidate in FP64
+//    with DPP wave reductions.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,12 +38,11 @@
 #include "device_common.h"
 #include "dnab_internal.h"
 #include "grid_device.h"
+#include "pair_device.h"
 #include "sort_device.h"
 
 namespace dnab {
 
-constexpr float kMinRSq = 0.25f;   // src/energy.rs:17
-constexpr float kHydroSq = 36.0f;  // src/energy.rs:20
 constexpr float kSortCell = 4.0f;  // candidate-sort cell edge
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;  // empty tile slot
 constexpr int kTileWaves = 4;
@@ -473,140 +472,12 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
 // kernels meet through plain stores and loads in device memory (the kernel
 // boundary orders them), with no device-scope atomics or fences, which on
 // this GPU would reach across the XCDs' separate L2 caches.
-constexpr int kLoadAhead = 4;   // chunk loads in flight per round trip
-constexpr double kFixScale = 1048576.0;          // 2^20
-constexpr double kFixUnit = 1.0 / 1048576.0;
-constexpr long long kFixLim = 1LL << 59;         // accumulator bound; sums of 4 stay below 2^61
-
-__device__ inline long long fix_round(double v) {
-  const double lim = static_cast<double>(kFixLim);
-  v = fmin(fmax(v * kFixScale, -lim), lim);   // NaN -> -lim (fmax drops NaN)
-  return __double2ll_rn(v);
-}
-__device__ inline long long fix_add(long long a, long long b) {
-  const long long s = a + b;  // |a|, |b| <= 2^59 + 2^25: no overflow
-  return s > kFixLim ? kFixLim : (s < -kFixLim ? -kFixLim : s);
-}
-
-// Per-lane integer accumulators of one tile slot.
-struct FixAcc {
-  long long e = 0, a = 0, x = 0, y = 0, z = 0;
-};
-
-// One pair term (the reference's predicate and formulas, src/energy.rs:261-314):
-// the f32 energy ep and, with FORCES, the scalar f with F = f (dx, dy, dz).
-template <bool FORCES>
-__device__ inline void pair_term(float4 B0, float4 B1, float xb, float yb, float zb, float qb,
-                                 float rmhb, float sqeb, float sqe12b, bool hydb, float rc2_lane,
-                                 float& ep, float& f, float& dx, float& dy, float& dz, bool& in,
-                                 uint32_t& h) {
-  // pair predicate in f32, exactly the reference's (src/energy.rs:279-284)
-  dx = xb - B0.x;
-  dy = yb - B0.y;
-  dz = zb - B0.z;
-  const float r2 = (dx * dx + dy * dy) + dz * dz;
-  in = r2 <= rc2_lane && r2 >= kMinRSq;
-  float rinv = __builtin_amdgcn_rsqf(r2);
-  rinv = in ? rinv : 0.0f;
-  const float rinv2 = rinv * rinv;
-  const float R = B1.x + rmhb;
-  const float s2 = (R * R) * rinv2;
-  const float s6 = (s2 * s2) * s2;
-  const float s12 = s6 * s6;
-  const float elj = (B1.y * sqeb) * __builtin_fmaf(-2.0f, s6, s12);
-  const float ecl = (B0.w * qb) * rinv;
-  f = 0.0f;
-  if (FORCES) {
-    const float flj = ((B1.y * sqe12b) * (s12 - s6)) * rinv2;
-    f = __builtin_fmaf(ecl, rinv2, flj);
-  }
-  h = 0;
-  if (B1.z != 0.0f) {  // uniform: the antigen atom is the same in every lane
-    h = (in && hydb && r2 < kHydroSq) ? 1u : 0u;
-    if (FORCES) f = h ? __builtin_fmaf(-0.1f, rinv, f) : f;
-  }
-  ep = elj + ecl;
-}
-
-// Pairs of the lane's atom with the `n_s` antigen atoms staged in `st` (in
-// chunk order); folds the chunk's sums into `acc`. Two pairs are formed at a
-// time and added in order, which keeps the sums those of a plain loop.
-template <bool FORCES, bool COUNT>
-__device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb, float yb,
-                                   float zb, float qb, float rmhb, float sqeb, float sqe12b,
-                                   bool hydb, float rc2_lane, bool valid, FixAcc& acc,
-                                   uint32_t& n_in, uint32_t& n_tested) {
-  float fx = 0.f, fy = 0.f, fz = 0.f, es = 0.f, ec = 0.f, eabs = 0.f;
-  uint32_t nh = 0;
-  auto add = [&](float ep, float f, float dx, float dy, float dz, bool in, uint32_t h) {
-    if (COUNT) {
-      n_tested += valid ? 1u : 0u;
-      n_in += in ? 1u : 0u;
-    }
-    nh += h;
-    if (FORCES) {
-      fx = __builtin_fmaf(f, dx, fx);
-      fy = __builtin_fmaf(f, dy, fy);
-      fz = __builtin_fmaf(f, dz, fz);
-    }
-    float sum, err;
-    two_sum(es, ep, sum, err);
-    es = sum;
-    ec += err;
-    eabs += fabsf(ep);
-  };
-  uint32_t q = 0;
-  for (; q + 2 <= n_s; q += 2) {
-    const float4 A0 = st[q][0], A1 = st[q][1], B0 = st[q + 1][0], B1 = st[q + 1][1];
-    float epa, fa, dxa, dya, dza, epb, fb, dxb, dyb, dzb;
-    bool ina, inb;
-    uint32_t ha, hb;
-    pair_term<FORCES>(A0, A1, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb, rc2_lane, epa, fa, dxa,
-                      dya, dza, ina, ha);
-    pair_term<FORCES>(B0, B1, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb, rc2_lane, epb, fb, dxb,
-                      dyb, dzb, inb, hb);
-    add(epa, fa, dxa, dya, dza, ina, ha);
-    add(epb, fb, dxb, dyb, dzb, inb, hb);
-  }
-  if (q < n_s) {
-    float ep, f, dx, dy, dz;
-    bool in;
-    uint32_t h;
-    pair_term<FORCES>(st[q][0], st[q][1], xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb, rc2_lane, ep,
-                      f, dx, dy, dz, in, h);
-    add(ep, f, dx, dy, dz, in, h);
-  }
-  // hydrophobic: -0.5 per pair, exactly -2^19 in fixed point
-  const long long eh = -static_cast<long long>(nh) << 19;
-  acc.e = fix_add(acc.e, fix_round(static_cast<double>(es) + static_cast<double>(ec)) + eh);
-  acc.a = fix_add(acc.a, fix_round(static_cast<double>(eabs)));
-  if (FORCES) {
-    acc.x = fix_add(acc.x, fix_round(static_cast<double>(fx)));
-    acc.y = fix_add(acc.y, fix_round(static_cast<double>(fy)));
-    acc.z = fix_add(acc.z, fix_round(static_cast<double>(fz)));
-  }
-}
-
 __device__ inline void write_fixed(const FixAcc& acc, uint32_t j, TileOut out) {
   out.e_atom[j] = static_cast<double>(acc.e) * kFixUnit;
   out.force[j] = make_float4(static_cast<float>(static_cast<double>(acc.x) * kFixUnit),
                              static_cast<float>(static_cast<double>(acc.y) * kFixUnit),
                              static_cast<float>(static_cast<double>(acc.z) * kFixUnit),
                              static_cast<float>(static_cast<double>(acc.a) * kFixUnit));
-}
-
-// Stage the chunk atoms selected by `mask` (in chunk order) into `st`.
-__device__ inline void stage_chunk(float4 (*st)[2], uint64_t mask, uint32_t lane, float4 A0,
-                                   float4 A1) {
-  if ((mask >> lane) & 1ull) {
-    const uint32_t slot = __builtin_amdgcn_mbcnt_hi(
-        static_cast<uint32_t>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-    st[slot][0] = A0;
-    st[slot][1] = A1;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <bool FORCES, bool COUNT>
@@ -1447,6 +1318,12 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   }
   if (nc) hcand_atom_off[nc] = static_cast<uint32_t>(na);
   else hcand_atom_off[0] = 0;
+  for (size_t c = 0; c < nc; ++c) {
+    pop->max_cand_atoms = std::max(pop->max_cand_atoms, hcand_atom_off[c + 1] - hcand_atom_off[c]);
+    pop->max_cand_res = std::max(pop->max_cand_res, d->cand_res_off[c + 1] - d->cand_res_off[c]);
+  }
+  pop->h_cand_atom_off = hcand_atom_off;
+  pop->h_cand_res_off.assign(d->cand_res_off, d->cand_res_off + nc + 1);
   for (size_t r = 0; r < nr; ++r) {
     float q = 0, e = 0, s = 0;
     uint8_t h = 0;
@@ -1564,6 +1441,8 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
     if ((rc = upload(ctx, pop->sort_count.p, zeros.data(), zeros.size() * 4))) return fail(rc);
     uint32_t ctr[4] = {0, 0, 0, 0};
     if ((rc = upload(ctx, pop->step_counter.p, ctr, sizeof(ctr)))) return fail(rc);
+    const unsigned long long stats0[2] = {0, 0};
+    if ((rc = upload(ctx, pop->pair_stats.p, stats0, sizeof(stats0)))) return fail(rc);
     if (hipMemsetAsync(pop->ho_rest.p, 0, pop->tile_cap * sizeof(uint32_t), ctx->stream) != hipSuccess ||
         hipMemsetAsync(pop->ho_list.p, 0, (pop->tile_cap + 4) * sizeof(uint32_t), ctx->stream) != hipSuccess)
       return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "population: clearing the handover table failed"));

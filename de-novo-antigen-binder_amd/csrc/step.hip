@@ -16,24 +16,12 @@
 #include "dnab_internal.h"
 #include "grid_device.h"
 #include "sort_device.h"
+#include "step_device.h"
 
 namespace dnab {
 
 constexpr int kTopkChunk = 4096;
 constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
-
-struct StepParams {
-  float step, max_disp, restraint_k, restraint_radius;
-  int with_noise, force_model;
-  uint32_t seed_lo, seed_hi;
-};
-
-__device__ inline float clamp_rust(float v, float lo, float hi) {
-  // f32::clamp: NaN passes through.
-  if (v < lo) v = lo;
-  if (v > hi) v = hi;
-  return v;
-}
 
 // One wave per candidate: (1) the candidate's FP64 energy from its atoms'
 // energies (DPP wave sum; the loop needs no separate reduction), (2) the
@@ -94,31 +82,8 @@ k_candidate_update(const AgView* __restrict__ views, const uint32_t* __restrict_
         fz += f.z;
       }
     }
-    // src/diffusion.rs:318-327
-    const float dx_c = p.x - V.center[0];
-    const float dy_c = p.y - V.center[1];
-    const float dz_c = p.z - V.center[2];
-    const float d = fmaxf(sqrtf(dx_c * dx_c + dy_c * dy_c + dz_c * dz_c), 0.001f);
-    const float excess = d - r0;
-    const float f_r = -sp.restraint_k * excess / d;
-    fx += f_r * dx_c;
-    fy += f_r * dy_c;
-    fz += f_r * dz_c;
-    // src/diffusion.rs:329-337
-    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    if (sp.with_noise) {
-      const float sigma = sigma_table[iter];
-      const Philox4 u = philox4x32_10(iter, r, 0x6e6f6973u, 0u, sp.seed_lo, sp.seed_hi);
-      const float two = sigma + sigma;
-      nx = u01_from_u32(u.v[0]) * two - sigma;
-      ny = u01_from_u32(u.v[1]) * two - sigma;
-      nz = u01_from_u32(u.v[2]) * two - sigma;
-    }
-    const float new_x = p.x + clamp_rust(sp.step * fx, -sp.max_disp, sp.max_disp) + nx;
-    const float new_y = p.y + clamp_rust(sp.step * fy, -sp.max_disp, sp.max_disp) + ny;
-    const float new_z = p.z + clamp_rust(sp.step * fz, -sp.max_disp, sp.max_disp) + nz;
-    // set_ca_pos: the Calpha's displacement, added to every atom of the residue
-    res_delta[r] = make_float4(new_x - p.x, new_y - p.y, new_z - p.z, 0.0f);
+    const float sigma = sp.with_noise ? sigma_table[iter] : 0.0f;
+    res_delta[r] = residue_delta(V, sp, r0, p, fx, fy, fz, sigma, iter, r);
   }
 }
 
@@ -197,11 +162,23 @@ __device__ inline bool rank_less(double ea, uint32_t ia, double eb, uint32_t ib)
 
 // Each block sorts one chunk of up to kTopkChunk (energy, index) entries with
 // a bitonic network in LDS and writes its first k to out[block * k ...].
+// blockIdx.y selects a row (an independent list; rows are in_stride apart in
+// the input and out_stride apart in the output). The output starts
+// out_row * out_stride further on, where out_row = counter[1] / row_div when
+// `counter` is given (the step index in use: the scoring's row), else
+// row_fixed.
 __global__ void __launch_bounds__(1024)
 k_topk_chunk(const double* __restrict__ in_e, const uint32_t* __restrict__ in_idx, uint32_t n,
-             uint32_t k, double* __restrict__ out_e, uint32_t* __restrict__ out_idx) {
+             uint32_t k, double* __restrict__ out_e, uint32_t* __restrict__ out_idx,
+             size_t in_stride, size_t out_stride, const uint32_t* __restrict__ counter,
+             uint32_t row_div, uint32_t row_fixed) {
   __shared__ double se[kTopkChunk];
   __shared__ uint32_t si[kTopkChunk];
+  const size_t out_row = (counter ? counter[1] / row_div : row_fixed) + blockIdx.y;
+  in_e += blockIdx.y * in_stride;
+  if (in_idx) in_idx += blockIdx.y * in_stride;
+  out_e += out_row * out_stride;
+  out_idx += out_row * out_stride;
   const uint32_t base = blockIdx.x * kTopkChunk;
   const uint32_t len = min(static_cast<uint32_t>(kTopkChunk), n - base);
   uint32_t m = 1;
@@ -253,8 +230,9 @@ static inline uint32_t nblk(uint64_t n, uint32_t t) {
 }
 
 // Multi-pass top-k: chunks of 4096 reduce n to ceil(n/4096)*k entries per pass.
-int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k) {
+int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k, uint32_t rows) {
   pl.k = k;
+  pl.rows = rows;
   pl.lens.clear();
   uint32_t len = n;
   size_t cap = 1;
@@ -270,6 +248,7 @@ int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k) {
                             n, kTopkChunk);
     len = static_cast<uint32_t>(out);
   }
+  cap *= std::max<uint32_t>(rows, 1);
   DNAB_ALLOC_CHECK(ctx, pl.e[0].alloc(cap));
   DNAB_ALLOC_CHECK(ctx, pl.e[1].alloc(cap));
   DNAB_ALLOC_CHECK(ctx, pl.idx[0].alloc(cap));
@@ -277,17 +256,23 @@ int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k) {
   return DNAB_OK;
 }
 
+// Rows of pl.rows lists of n entries, n apart; row r's top k go to
+// final_e / final_idx + (out_row + r) * k, with out_row as in k_topk_chunk.
 void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
-                 uint32_t* final_idx) {
+                 uint32_t* final_idx, const uint32_t* counter, uint32_t row_div,
+                 uint32_t row_fixed) {
   const double* in_e = energy;
   const uint32_t* in_i = nullptr;
   for (size_t p = 0; p < pl.lens.size(); ++p) {
     const uint32_t len = pl.lens[p];
+    const uint32_t chunks = nblk(len, kTopkChunk);
     const bool last = p + 1 == pl.lens.size();
     double* oe = last ? final_e : pl.e[p & 1].p;
     uint32_t* oi = last ? final_idx : pl.idx[p & 1].p;
-    hipLaunchKernelGGL(k_topk_chunk, dim3(nblk(len, kTopkChunk)), dim3(1024), 0, ctx->stream, in_e,
-                       in_i, len, pl.k, oe, oi);
+    hipLaunchKernelGGL(k_topk_chunk, dim3(chunks, std::max<uint32_t>(pl.rows, 1)), dim3(1024), 0,
+                       ctx->stream, in_e, in_i, len, pl.k, oe, oi, static_cast<size_t>(len),
+                       static_cast<size_t>(chunks) * pl.k, last ? counter : nullptr,
+                       std::max<uint32_t>(row_div, 1), last ? row_fixed : 0u);
     in_e = oe;
     in_i = oi;
   }
@@ -357,7 +342,9 @@ static int launch_update(dnab_ctx* ctx, dnab_population* pop, const StepParams& 
 }
 
 // One chunk of the loop: L steps; when `score`, the first step's energies
-// (the state the chunk starts from) go through top-k.
+// (the state the chunk starts from) go through top-k into the scoring's row
+// of the score history (row = step index / score_every, read on the device,
+// so a captured chunk replays into successive rows).
 static int enqueue_chunk(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                          const StepParams& sp, uint32_t L, TopkPlan* tk, bool score) {
   const bool allatom = p->force_model == DNAB_FORCE_ALLATOM;
@@ -372,7 +359,9 @@ static int enqueue_chunk(dnab_ctx* ctx, dnab_population* pop, const dnab_langevi
     }
     // the next step evaluates (and so needs counts) in the all-atom model
     DNAB_RETURN_IF(launch_update(ctx, pop, sp, eval, allatom));
-    if (scoring && tk) topk_launch(ctx, *tk, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
+    if (scoring && tk)
+      topk_launch(ctx, *tk, pop->energy.p, pop->hist_e.p, pop->hist_idx.p, pop->step_counter.p,
+                  p->score_every);
   }
   return DNAB_OK;
 }
@@ -443,13 +432,62 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
   if (n_topk) *n_topk = k;
   if (nc == 0) return DNAB_OK;
   DNAB_RETURN_IF(upload_sigma(ctx, pop, p, p->n_steps));
-  TopkPlan tk;
-  DNAB_RETURN_IF(topk_plan(ctx, tk, nc, k));
-  DNAB_ALLOC_CHECK(ctx, pop->topk_e.ensure(k));
-  DNAB_ALLOC_CHECK(ctx, pop->topk_idx.ensure(k));
+  // score history: one row per scoring (steps 0, L, 2L, ... before the
+  // last, then the final state), each the top k of that scoring
+  const uint32_t n_mid = p->score_every ? (p->n_steps + p->score_every - 1) / p->score_every : 0;
+  const uint32_t rows = n_mid + 1;
+  DNAB_ALLOC_CHECK(ctx, pop->hist_e.ensure(static_cast<size_t>(rows) * k));
+  DNAB_ALLOC_CHECK(ctx, pop->hist_idx.ensure(static_cast<size_t>(rows) * k));
+  pop->hist_rows = 0;  // until this run completes
+  pop->hist_k = k;
   pop->masked = false;
   hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p, 0u);
   const StepParams sp = make_step_params(p);
+  if (!p->step_kernels && trajectory_supported(pop, p)) {
+    // every step in one launch, which leaves each scoring's candidate
+    // energies in a row of score_e; then the top k of every row at once
+    DNAB_ALLOC_CHECK(ctx, pop->score_e.ensure(static_cast<size_t>(rows) * nc));
+    TopkPlan tr;
+    DNAB_RETURN_IF(topk_plan(ctx, tr, nc, k, rows));
+    DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
+    DNAB_RETURN_IF(trajectory_run(ctx, pop, p, sp));
+    pop->sorted = false;
+    pop->last_flags = DNAB_EVAL_FORCES;
+    pop->evaluated = true;
+    topk_launch(ctx, tr, pop->score_e.p, pop->hist_e.p, pop->hist_idx.p);
+  } else {
+    DNAB_RETURN_IF(step_kernel_loop(ctx, pop, p, sp, nc, k, n_mid));
+  }
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  const size_t last = static_cast<size_t>(n_mid) * k;
+  if (topk_idx) DNAB_RETURN_IF(download(ctx, topk_idx, pop->hist_idx.p + last, k * sizeof(uint32_t)));
+  if (topk_energy) DNAB_RETURN_IF(download(ctx, topk_energy, pop->hist_e.p + last, k * sizeof(double)));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  pop->hist_rows = rows;
+  return DNAB_OK;
+}
+
+int dnab_population_read_scores(dnab_ctx* ctx, const dnab_population* pop, uint32_t* n_scores,
+                                uint32_t* k, uint32_t* topk_idx, double* topk_energy) {
+  if (!ctx || !pop || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  if (n_scores) *n_scores = pop->hist_rows;
+  if (k) *k = pop->hist_k;
+  const size_t n = static_cast<size_t>(pop->hist_rows) * pop->hist_k;
+  if (n == 0) return DNAB_OK;
+  if (topk_idx) DNAB_RETURN_IF(download(ctx, topk_idx, pop->hist_idx.p, n * sizeof(uint32_t)));
+  if (topk_energy) DNAB_RETURN_IF(download(ctx, topk_energy, pop->hist_e.p, n * sizeof(double)));
+  return DNAB_OK;
+}
+
+}  // extern "C"
+
+namespace dnab {
+
+// The loop as per-step population kernels, eager or replayed from a graph.
+int step_kernel_loop(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
+                     const StepParams& sp, uint32_t nc, uint32_t k, uint32_t n_mid) {
+  TopkPlan tk;
+  DNAB_RETURN_IF(topk_plan(ctx, tk, nc, k));
   const uint32_t L = p->score_every ? p->score_every : std::max<uint32_t>(p->n_steps, 1);
   const bool graph = p->use_graph && p->n_steps >= L;
   // A chunk must start in the state it ends in: in the all-atom model every
@@ -496,12 +534,8 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
   DNAB_RETURN_IF(population_sort(ctx, pop));
   DNAB_RETURN_IF(population_pair(ctx, pop, p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES));
   DNAB_RETURN_IF(population_reduce_energies(ctx, pop, DNAB_EVAL_FORCES));
-  topk_launch(ctx, tk, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
-  DNAB_HIP_CHECK(ctx, hipGetLastError());
-  if (topk_idx) DNAB_RETURN_IF(download(ctx, topk_idx, pop->topk_idx.p, k * sizeof(uint32_t)));
-  if (topk_energy) DNAB_RETURN_IF(download(ctx, topk_energy, pop->topk_e.p, k * sizeof(double)));
-  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  topk_launch(ctx, tk, pop->energy.p, pop->hist_e.p, pop->hist_idx.p, nullptr, 1, n_mid);
   return DNAB_OK;
 }
 
-}  // extern "C"
+}  // namespace dnab

@@ -70,6 +70,11 @@ class PairTuningC(C.Structure):
                 ("rest_waves", C.c_uint32)]
 
 
+class TrajTuningC(C.Structure):
+    _fields_ = [("waves", C.c_uint32), ("cand_per_group", C.c_uint32), ("balance", C.c_uint32),
+                ("cells_per_item", C.c_uint32)]
+
+
 class ProteinOutC(C.Structure):
     _fields_ = [("atom_capacity", C.c_size_t), ("x", f32p), ("y", f32p), ("z", f32p),
                 ("charge", f32p), ("r_min_half", f32p), ("epsilon", f32p),
@@ -96,7 +101,8 @@ class LangevinParamsC(C.Structure):
                 ("max_disp", C.c_float), ("noise_base", C.c_float), ("t_hot", C.c_float),
                 ("t_cold", C.c_float), ("schedule_offset", C.c_uint32),
                 ("schedule_total", C.c_uint32), ("restraint_k", C.c_float),
-                ("restraint_radius", C.c_float), ("count_pairs", C.c_int32)]
+                ("restraint_radius", C.c_float), ("count_pairs", C.c_int32),
+                ("step_kernels", C.c_int32)]
 
 
 class TrainParamsC(C.Structure):
@@ -145,6 +151,8 @@ _PROTOS = {
                                           f32p, f32p, f32p, u8p]),
     "dnab_population_get_tuning": (C.c_int, [vp, C.POINTER(PairTuningC)]),
     "dnab_population_set_tuning": (C.c_int, [vp, vp, C.POINTER(PairTuningC)]),
+    "dnab_population_get_trajectory_tuning": (C.c_int, [vp, C.POINTER(TrajTuningC)]),
+    "dnab_population_set_trajectory_tuning": (C.c_int, [vp, vp, C.POINTER(TrajTuningC)]),
     "dnab_population_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(PairTuningC),
                                            C.POINTER(C.c_double)]),
     "dnab_synth_shell_population": (C.c_int, [C.c_size_t, C.c_size_t, f32p, C.c_float, C.c_float,
@@ -207,6 +215,7 @@ _PROTOS = {
     "dnab_langevin_default_params": (C.c_int, [C.POINTER(LangevinParamsC)]),
     "dnab_langevin_step": (C.c_int, [vp, vp, C.POINTER(LangevinParamsC), C.c_uint32]),
     "dnab_langevin_run": (C.c_int, [vp, vp, C.POINTER(LangevinParamsC), u32p, f64p, u32p]),
+    "dnab_population_read_scores": (C.c_int, [vp, vp, u32p, u32p, u32p, f64p]),
 }
 for _name, (_res, _args) in _PROTOS.items():
     _fn = getattr(lib, _name)

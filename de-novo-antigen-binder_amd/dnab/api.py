@@ -362,6 +362,18 @@ class DevicePopulation:
         t = N.PairTuningC(direct_survivors, direct_chunks, rest_waves)
         self.ctx.check(N.lib.dnab_population_set_tuning(self.ctx.handle, self._h, C.byref(t)))
 
+    def trajectory_tuning(self) -> dict:
+        t = N.TrajTuningC()
+        N.check(N.lib.dnab_population_get_trajectory_tuning(self._h, C.byref(t)))
+        return {"waves": t.waves, "cand_per_group": t.cand_per_group, "balance": t.balance,
+                "cells_per_item": t.cells_per_item}
+
+    def set_trajectory_tuning(self, waves: int, cand_per_group: int, balance: int = 1,
+                              cells_per_item: int = 2):
+        t = N.TrajTuningC(waves, cand_per_group, balance, cells_per_item)
+        self.ctx.check(N.lib.dnab_population_set_trajectory_tuning(self.ctx.handle, self._h,
+                                                                   C.byref(t)))
+
     def autotune(self, reps: int = 3) -> tuple[dict, float]:
         """Pick the fastest pair-kernel work split for the current state."""
         t = N.PairTuningC()
@@ -401,6 +413,19 @@ class DevicePopulation:
                                                N.ptr(idx, C.c_uint32), N.ptr(e, C.c_double),
                                                C.byref(n)))
         return idx[:n.value], e[:n.value]
+
+    def scores(self):
+        """Score history of the last run: (idx, energy), each (n_scores, k);
+        row j is the top k of the j-th scoring, the last row the final one."""
+        n, k = C.c_uint32(), C.c_uint32()
+        self.ctx.check(N.lib.dnab_population_read_scores(self.ctx.handle, self._h, C.byref(n),
+                                                         C.byref(k), None, None))
+        idx = np.zeros((max(n.value, 1), max(k.value, 1)), np.uint32)
+        e = np.zeros_like(idx, dtype=np.float64)
+        self.ctx.check(N.lib.dnab_population_read_scores(self.ctx.handle, self._h, None, None,
+                                                         N.ptr(idx, C.c_uint32),
+                                                         N.ptr(e, C.c_double)))
+        return idx[:n.value, :k.value], e[:n.value, :k.value]
 
     def close(self):
         if self._h:

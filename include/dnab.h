@@ -303,8 +303,8 @@ int dnab_population_pair_count(dnab_ctx* ctx, dnab_population* pop, uint64_t* in
                                uint64_t* tested);
 int dnab_population_info(const dnab_population* pop, size_t* n_cand, size_t* n_res,
                          size_t* n_atoms);
-/* Pair statistics accumulated by runs with count_pairs set (reset to zero by
- * dnab_population_reset_pair_stats). */
+/* Pair statistics accumulated by runs with count_pairs set (zero when the
+ * population is created; reset by dnab_population_reset_pair_stats). */
 int dnab_population_read_pair_stats(dnab_ctx* ctx, const dnab_population* pop,
                                     uint64_t* in_cutoff, uint64_t* tested);
 int dnab_population_reset_pair_stats(dnab_ctx* ctx, dnab_population* pop);
@@ -341,6 +341,14 @@ typedef struct dnab_langevin_params {
   int32_t count_pairs;      /* accumulate pair statistics over the run (same
                                results, slower); read with
                                dnab_population_read_pair_stats */
+  int32_t step_kernels;     /* 0: run the all-atom loop as one fused kernel
+                               (a workgroup per group of candidates, every
+                               step in LDS) when a candidate fits one
+                               workgroup (up to ~1,500 atoms);
+                               1: one set of population kernels per step
+                               (hipGraph replay with use_graph). Results are
+                               bit-identical; pair statistics agree on pairs
+                               in the cutoff, not on pairs tested. */
 } dnab_langevin_params;
 int dnab_langevin_default_params(dnab_langevin_params* p);
 /* One step at iteration `iter` (forces from a fresh evaluation). */
@@ -353,6 +361,33 @@ int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_
  * population is smaller, count in *n_topk). */
 int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                       uint32_t* topk_idx, double* topk_energy, uint32_t* n_topk);
+/* Workgroup shape of the fused trajectory kernel (no reference counterpart):
+ * `waves` (4, 8 or 16) waves run the trajectories of up to `cand_per_group`
+ * (1..16) candidates together, sharing their pair work. With `balance` set,
+ * a first evaluation measures each candidate's pair work and candidates are
+ * grouped so that groups carry similar work (a candidate in deep contact runs
+ * alone), heaviest group first; otherwise groups are consecutive candidates.
+ * Groups that do not fit one workgroup's LDS hold fewer candidates.
+ * A work item is a run of `cells_per_item` grid cells of one tile.
+ * Default 16 waves, 6 candidates, balance on, 2 cells. Results do not
+ * depend on it. */
+typedef struct dnab_traj_tuning {
+  uint32_t waves;
+  uint32_t cand_per_group;
+  uint32_t balance;
+  uint32_t cells_per_item;  /* grid cells per work item (1..64) */
+} dnab_traj_tuning;
+int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_tuning* t);
+int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
+                                          const dnab_traj_tuning* t);
+/* Score history of the last dnab_langevin_run: one row per scoring (the
+ * states at steps 0, score_every, 2 score_every, ... before n_steps, then the
+ * final state), each its k best (ascending energy, ties by index). The last
+ * row is what the run returned. Arrays of n_scores * k entries; NULL arrays
+ * only report n_scores and k. (The reference scores once, at the end of the
+ * GPU phase, src/diffusion.rs:1026; this keeps every scoring of the loop.) */
+int dnab_population_read_scores(dnab_ctx* ctx, const dnab_population* pop, uint32_t* n_scores,
+                                uint32_t* k, uint32_t* topk_idx, double* topk_energy);
 
 /* ---- learned prescreen (no reference counterpart; the reference has no
  * learned components, Cargo.toml:4) -------------------------------------------

@@ -68,21 +68,92 @@ def test_run_topk_consistent(ctx):
     pop2.evaluate()
     e0, _ = pop2.energies()
     assert np.median(ef) < np.median(e0)
+    # the score history: scorings at steps 0 and 10, then the final state;
+    # row 0 is the top 16 of the initial state (ascending energy, ties by index)
+    hi, he = pop.scores()
+    assert hi.shape == (3, 16)
+    order = np.lexsort((np.arange(len(e0)), e0))[:16]
+    assert np.array_equal(hi[0], order)
+    assert np.array_equal(he[0].view(np.uint64), e0[order].view(np.uint64))
+    assert np.array_equal(hi[-1], idx)
+
+
+def _run_state(ctx, w, **kw):
+    ags, pop = W.device_objects(ctx, w)
+    p = langevin_params(restraint_radius=w.restraint_radius[0], **kw)
+    idx, e = pop.run(p)
+    ef, err = pop.energies()
+    hi, he = pop.scores()
+    return dict(idx=idx, e=e, coords=pop.coords(), forces=pop.forces(), energy=ef, err=err,
+                hist_idx=hi, hist_e=he,
+                stats=pop.pair_stats() if kw.get("count_pairs") else None)
+
+
+def _assert_same(a, b):
+    assert np.array_equal(a["idx"], b["idx"])
+    assert np.array_equal(a["hist_idx"], b["hist_idx"])
+    assert np.array_equal(a["hist_e"].view(np.uint64), b["hist_e"].view(np.uint64))
+    assert np.array_equal(a["e"].view(np.uint64), b["e"].view(np.uint64))
+    assert np.array_equal(a["energy"].view(np.uint64), b["energy"].view(np.uint64))
+    assert np.array_equal(a["err"].view(np.uint32), b["err"].view(np.uint32))
+    for u, v in zip(a["coords"], b["coords"]):
+        assert np.array_equal(u.view(np.uint32), v.view(np.uint32))
+    for u, v in zip(a["forces"], b["forces"]):
+        assert np.array_equal(np.asarray(u).view(np.uint32), np.asarray(v).view(np.uint32))
 
 
 def test_graph_replay_equals_eager(ctx):
+    # per-step kernels eager, per-step kernels replayed from a graph, and the
+    # fused trajectory kernel: the same trajectory bit for bit
     w = W.config2(n_cand=128)
-    res = []
-    for graph in (0, 1):
+    kw = dict(n_steps=30, score_every=10, top_k=8, with_noise=1, seed=5)
+    eager = _run_state(ctx, w, use_graph=0, step_kernels=1, **kw)
+    graph = _run_state(ctx, w, use_graph=1, step_kernels=1, **kw)
+    fused = _run_state(ctx, w, step_kernels=0, **kw)
+    _assert_same(eager, graph)
+    _assert_same(eager, fused)
+    # scorings at steps 0, 10, 20 and the final state; the last row is the result
+    assert fused["hist_idx"].shape == (4, 8)
+    assert np.array_equal(fused["hist_idx"][-1], fused["idx"])
+    assert (np.diff(fused["hist_e"], axis=1) >= 0).all()
+
+
+def test_trajectory_tuning_changes_speed_not_results(ctx):
+    # ragged multi-antigen population: groups mix antigens and binder lengths
+    w = W.config3(n_cand=80)
+    kw = dict(n_steps=12, score_every=4, top_k=8, with_noise=1, seed=9)
+    ref = None
+    for waves, cpw, bal, cells in ((4, 1, 0, 16), (8, 3, 0, 5), (16, 4, 0, 64), (16, 16, 0, 1),
+                                   (16, 6, 1, 16), (4, 2, 1, 3)):
         ags, pop = W.device_objects(ctx, w)
-        p = langevin_params(n_steps=30, score_every=10, top_k=8, use_graph=graph,
-                            restraint_radius=w.restraint_radius[0], with_noise=1, seed=5)
+        pop.set_trajectory_tuning(waves, cpw, bal, cells)
+        assert pop.trajectory_tuning() == {"waves": waves, "cand_per_group": cpw, "balance": bal,
+                                           "cells_per_item": cells}
+        p = langevin_params(restraint_radius=w.restraint_radius[0], **kw)
         idx, e = pop.run(p)
-        res.append((idx, e, pop.coords()))
-    assert np.array_equal(res[0][0], res[1][0])
-    assert np.array_equal(res[0][1].view(np.uint64), res[1][1].view(np.uint64))
-    for a, b in zip(res[0][2], res[1][2]):
-        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        hi, he = pop.scores()
+        st = dict(idx=idx, e=e, coords=pop.coords(), forces=pop.forces(), energy=pop.energies()[0],
+                  err=pop.energies()[1], hist_idx=hi, hist_e=he)
+        if ref is None:
+            ref = st
+        else:
+            _assert_same(ref, st)
+    with pytest.raises(RuntimeError):
+        pop.set_trajectory_tuning(12, 4)
+
+
+@pytest.mark.parametrize("config", ["config2", "config3"])
+def test_fused_trajectory_equals_step_kernels(ctx, config):
+    # no noise (the reference's GPU phase), pair counting on: the in-cutoff
+    # pair count is a property of the trajectory, the tested count is not
+    w = getattr(W, config)(n_cand=96)
+    kw = dict(n_steps=40, score_every=0, top_k=8, count_pairs=1)
+    step = _run_state(ctx, w, step_kernels=1, **kw)
+    fused = _run_state(ctx, w, step_kernels=0, **kw)
+    _assert_same(step, fused)
+    assert step["stats"][0] == fused["stats"][0] > 0
+    print(f"{config}: {fused['stats'][0]} pairs in cutoff over 41 evaluations; tested "
+          f"{step['stats'][1]} (step kernels) vs {fused['stats'][1]} (fused)")
 
 
 def test_multi_antigen_ragged_run(ctx):
