@@ -39,6 +39,9 @@ namespace dnab {
 constexpr uint32_t kTrajMaxTiles = 64;    // tiles per workgroup (one ballot finds a tile)
 constexpr uint32_t kTrajMaxCpw = 16;      // candidates per workgroup
 constexpr uint16_t kTrajPad = 0xFFFFu;    // residue slot of a padding atom
+constexpr uint32_t kTileChunks = 192;     // chunk list entries per tile
+constexpr uint32_t kTileOverflow = 0xFFFFFFFFu;
+constexpr uint32_t kTrajMaxAgAtoms = 1u << 26;  // a chunk entry holds (first atom << 6 | len - 1)
 constexpr size_t kTrajLdsLimit = 160 * 1024;
 constexpr float kTrajErrPerAbs = 32.0f / 16777216.0f;  // 32 ulp of f32 per unit of sum |e|
 
@@ -46,7 +49,8 @@ struct TrajTile {
   float lo[3], hi[3];
   int32_t k0[3];
   int32_t nx, nxy, ncell;
-  uint32_t cand;  // candidate slot in the workgroup
+  uint32_t nchunk;  // listed chunks, or kTileOverflow
+  uint32_t cand;    // candidate slot in the workgroup
 };
 
 struct TrajCand {
@@ -82,7 +86,8 @@ struct TrajArgs {
   uint32_t* work;   // optional: per candidate, antigen atoms taken into pair loops (all steps)
   uint32_t n_cand, n_steps, score_every;
   uint32_t n_groups;        // with a table
-  int32_t cells;            // grid cells per work item
+  int32_t cells;            // grid cells per work item (tiles whose chunk list overflowed)
+  uint32_t chunks;          // listed chunks per work item (1..kLoadAhead)
   uint32_t cpw;             // candidates per workgroup (the most, with a table)
   uint32_t a_cap, r_cap;    // LDS capacity: atom slots (multiple of 64), residues
   StepParams sp;
@@ -90,9 +95,12 @@ struct TrajArgs {
 
 // Bytes of dynamic LDS for a capacity of a_cap atom slots and r_cap residues:
 // 5 integer sums, energy, position, parameters, force per atom slot; the
-// residue slot per atom slot; the displacement per residue.
+// residue slot per atom slot; the displacement per residue; the chunk list
+// of each tile (a_cap / 64 tiles).
 __host__ __device__ inline size_t traj_lds_bytes(uint32_t a_cap, uint32_t r_cap) {
-  return static_cast<size_t>(a_cap) * (5 * 8 + 8 + 3 * 16 + 2) + static_cast<size_t>(r_cap) * 16;
+  const size_t res = (static_cast<size_t>(a_cap) * 2 + 3) & ~static_cast<size_t>(3);
+  return static_cast<size_t>(a_cap) * (5 * 8 + 8 + 3 * 16) + static_cast<size_t>(r_cap) * 16 + res +
+         static_cast<size_t>(a_cap / kWave) * kTileChunks * 4;
 }
 // Static LDS of a workgroup of `waves` waves (staged chunks, tiles, candidates, counter).
 __host__ __device__ inline size_t traj_static_lds(int waves) {
@@ -130,6 +138,8 @@ k_trajectory(TrajArgs g) {
   float4* s_force = s_par + AC;                             // [a_cap]
   float4* s_delta = s_force + AC;                           // [r_cap]
   uint16_t* s_res = reinterpret_cast<uint16_t*>(s_delta + g.r_cap);  // [a_cap]
+  uint32_t* s_desc = reinterpret_cast<uint32_t*>(s_dyn + ((reinterpret_cast<uintptr_t>(s_res + AC) -
+                                                             reinterpret_cast<uintptr_t>(s_dyn) + 3) & ~uintptr_t(3)));
 
   if (tid == 0) {
     uint32_t first, nc;
@@ -197,7 +207,8 @@ k_trajectory(TrajArgs g) {
 
   uint32_t n_in = 0, n_tested = 0;
   for (uint32_t s = 0;; ++s) {
-    // 1. tile boxes; clear the accumulators
+    // 1. tile boxes, the chunks of the grid cells within the cutoff of
+    //    each box (listed in LDS), and cleared accumulators
     for (uint32_t t = wave; t < T; t += WAVES) {
       const uint32_t k = t * kWave + lane;
       const AgView& V = g.views[s_cand[s_tile[t].cand].ag];
@@ -211,36 +222,79 @@ k_trajectory(TrajArgs g) {
       const float hiy = wave_reduce_f32<OpMax>(valid ? P.y : -inf);
       const float hiz = wave_reduce_f32<OpMax>(valid ? P.z : -inf);
       const bool any = ballot64(valid) != 0;
+      // cells that can hold an antigen atom within rc of the box (cell_key
+      // is monotone, so a slightly widened range is conservative)
+      const float margin = V.rc * 1.001f + 1e-3f;
+      const int32_t kx0 = max(cell_key(lox - margin, V.inv), V.kmin[0]);
+      const int32_t ky0 = max(cell_key(loy - margin, V.inv), V.kmin[1]);
+      const int32_t kz0 = max(cell_key(loz - margin, V.inv), V.kmin[2]);
+      const int32_t kx1 = min(cell_key(hix + margin, V.inv), V.kmax[0]);
+      const int32_t ky1 = min(cell_key(hiy + margin, V.inv), V.kmax[1]);
+      const int32_t kz1 = min(cell_key(hiz + margin, V.inv), V.kmax[2]);
+      // (a tile without a valid atom has an empty box and no cells)
+      const int32_t nx = any ? kx1 - kx0 + 1 : 0;
+      const int32_t ny = any ? ky1 - ky0 + 1 : 0;
+      const int32_t nz = any ? kz1 - kz0 + 1 : 0;
+      const int32_t nxy = nx * ny;
+      const int32_t ncell = (nx > 0 && ny > 0 && nz > 0) ? nxy * nz : 0;
+      uint32_t* desc = s_desc + t * kTileChunks;
+      uint32_t nt = 0;
+      bool over = false;
+      for (int32_t cb0 = 0; cb0 < ncell; cb0 += kWave) {
+        // lane = a cell, in (kz, ky, kx) order
+        const int32_t cl = cb0 + static_cast<int32_t>(lane);
+        uint32_t cb = 0, ce = 0;
+        if (cl < ncell) {
+          const int32_t iz = cl / nxy, rem = cl - iz * nxy;
+          const int32_t iy = rem / nx, ix = rem - iy * nx;
+          const uint32_t code = morton3(static_cast<uint32_t>(kx0 + ix - V.kmin[0]),
+                                        static_cast<uint32_t>(ky0 + iy - V.kmin[1]),
+                                        static_cast<uint32_t>(kz0 + iz - V.kmin[2]));
+          cb = V.cell_start[code];
+          ce = V.cell_start[code + 1];
+        }
+        const uint32_t nch = (ce - cb + kWave - 1) / kWave;
+        uint32_t incl = nch;
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t u = __shfl_up(incl, o, 64);
+          if (lane >= static_cast<uint32_t>(o)) incl += u;
+        }
+        const uint32_t tot = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+        if (!over && nt + tot <= kTileChunks) {
+          const uint32_t off = nt + incl - nch;
+          for (uint32_t m = 0; m < nch; ++m) {
+            const uint32_t f = cb + kWave * m;
+            desc[off + m] = (f << 6) | (min(static_cast<uint32_t>(kWave), ce - f) - 1u);
+          }
+        } else {
+          over = true;  // the tile's chunks go through cell items
+        }
+        nt += tot;
+      }
       if (lane == 0) {
         TrajTile& tb = s_tile[t];
-        tb.ncell = 0;
-        if (any) {
-          // cells that can hold an antigen atom within rc of the box (cell_key
-          // is monotone, so a slightly widened range is conservative)
-          const float margin = V.rc * 1.001f + 1e-3f;
-          const int32_t kx0 = max(cell_key(lox - margin, V.inv), V.kmin[0]);
-          const int32_t ky0 = max(cell_key(loy - margin, V.inv), V.kmin[1]);
-          const int32_t kz0 = max(cell_key(loz - margin, V.inv), V.kmin[2]);
-          const int32_t kx1 = min(cell_key(hix + margin, V.inv), V.kmax[0]);
-          const int32_t ky1 = min(cell_key(hiy + margin, V.inv), V.kmax[1]);
-          const int32_t kz1 = min(cell_key(hiz + margin, V.inv), V.kmax[2]);
-          const int32_t nx = kx1 - kx0 + 1, ny = ky1 - ky0 + 1, nz = kz1 - kz0 + 1;
-          tb.lo[0] = lox; tb.lo[1] = loy; tb.lo[2] = loz;
-          tb.hi[0] = hix; tb.hi[1] = hiy; tb.hi[2] = hiz;
-          tb.k0[0] = kx0; tb.k0[1] = ky0; tb.k0[2] = kz0;
-          tb.nx = nx;
-          tb.nxy = nx * ny;
-          tb.ncell = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
-        }
+        tb.lo[0] = lox; tb.lo[1] = loy; tb.lo[2] = loz;
+        tb.hi[0] = hix; tb.hi[1] = hiy; tb.hi[2] = hiz;
+        tb.k0[0] = kx0; tb.k0[1] = ky0; tb.k0[2] = kz0;
+        tb.nx = nx;
+        tb.nxy = nxy;
+        tb.ncell = ncell;
+        tb.nchunk = over ? kTileOverflow : nt;
       }
       for (int q = 0; q < 5; ++q) s_acc[q * AC + k] = 0;
     }
     if (tid == 0) s_next = 0;
     __syncthreads();
 
-    // 2. pairs: work items = (tile, run of g.cells cells), tile-major
+    // 2. pairs: work items, tile-major: runs of g.chunks listed chunks of a
+    //    tile, or runs of g.cells cells of a tile whose list overflowed
     {
-      const uint32_t nb = lane < T ? static_cast<uint32_t>((s_tile[lane].ncell + g.cells - 1) / g.cells) : 0u;
+      uint32_t nb = 0;
+      if (lane < T) {
+        const TrajTile& tl = s_tile[lane];
+        nb = tl.nchunk != kTileOverflow ? (tl.nchunk + g.chunks - 1) / g.chunks
+                                       : static_cast<uint32_t>((tl.ncell + g.cells - 1) / g.cells);
+      }
       uint32_t incl = nb;
       for (int o = 1; o < 64; o <<= 1) {
         const uint32_t u = __shfl_up(incl, o, 64);
@@ -294,53 +348,19 @@ k_trajectory(TrajArgs g) {
           rc2_lane = valid ? V.rc2 : -1.0f;  // an atom without pairs takes none
         }
         const AgAtom* __restrict__ atoms = V.atoms;
-        const uint32_t* __restrict__ cstart = V.cell_start;
         const float rc2 = V.rc2;
         const float lox = tb.lo[0], loy = tb.lo[1], loz = tb.lo[2];
         const float hix = tb.hi[0], hiy = tb.hi[1], hiz = tb.hi[2];
-        const int32_t nx = tb.nx, nxy = tb.nxy;
-        const int32_t cbeg = static_cast<int32_t>(b) * g.cells;
-        const int32_t cend = min(cbeg + g.cells, tb.ncell);
-        // lane = a cell of this item, in (kz, ky, kx) order
-        const int32_t cl = cbeg + static_cast<int32_t>(lane);
-        uint32_t cb = 0, ce = 0;
-        if (cl < cend) {
-          const int32_t iz = cl / nxy, rem = cl - iz * nxy;
-          const int32_t iy = rem / nx, ix = rem - iy * nx;
-          const uint32_t code = morton3(static_cast<uint32_t>(tb.k0[0] + ix - V.kmin[0]),
-                                        static_cast<uint32_t>(tb.k0[1] + iy - V.kmin[1]),
-                                        static_cast<uint32_t>(tb.k0[2] + iz - V.kmin[2]));
-          cb = cstart[code];
-          ce = cstart[code + 1];
-        }
-        const uint32_t nch = (ce - cb + kWave - 1) / kWave;
-        uint32_t cincl = nch;
-        for (int o = 1; o < 64; o <<= 1) {
-          const uint32_t u = __shfl_up(cincl, o, 64);
-          if (lane >= static_cast<uint32_t>(o)) cincl += u;
-        }
-        const uint32_t off = cincl - nch;
-        const uint32_t nchunk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cincl), 63));
-        for (uint32_t kc = 0; kc < nchunk; kc += kLoadAhead) {
-          uint32_t cbv[kLoadAhead], cev[kLoadAhead];
+        // up to kLoadAhead chunks [cbv, cev): loads in flight together, then
+        // the atoms near the tile box are staged and paired in chunk order
+        auto run_chunks = [&](const uint32_t (&cbv)[kLoadAhead], const uint32_t (&cev)[kLoadAhead]) {
           float4 L0[kLoadAhead], L1[kLoadAhead];
           for (int u = 0; u < kLoadAhead; ++u) {
-            const uint32_t kk = kc + u;
-            cbv[u] = 0;
-            cev[u] = 0;
             L0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             L1[u] = L0[u];
-            if (kk < nchunk) {
-              // the cell holding chunk kk, and the chunk's atom range
-              const uint64_t mc = ballot64(nch > 0 && off <= kk);
-              const int ln = 63 - static_cast<int>(__builtin_clzll(mc));
-              cbv[u] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cb), ln)) +
-                       kWave * (kk - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off), ln)));
-              cev[u] = min(cbv[u] + kWave, static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ce), ln)));
-              if (cbv[u] + lane < cev[u]) {
-                L0[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].x);
-                L1[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].rmh);
-              }
+            if (cbv[u] + lane < cev[u]) {
+              L0[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].x);
+              L1[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].rmh);
             }
           }
           for (int u = 0; u < kLoadAhead; ++u) {
@@ -356,6 +376,63 @@ k_trajectory(TrajArgs g) {
             stage_chunk(s_stage[wave], mask, lane, L0[u], L1[u]);
             chunk_pairs<true, COUNT>(s_stage[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb,
                                      rc2_lane, valid, acc, n_in, n_tested);
+          }
+        };
+        uint32_t cbv[kLoadAhead], cev[kLoadAhead];
+        if (tb.nchunk != kTileOverflow) {
+          // listed chunks b * g.chunks ...
+          const uint32_t base = b * g.chunks;
+          const uint32_t* desc = s_desc + t * kTileChunks;
+          for (int u = 0; u < kLoadAhead; ++u) {
+            cbv[u] = 0;
+            cev[u] = 0;
+            if (static_cast<uint32_t>(u) < g.chunks && base + u < tb.nchunk) {
+              const uint32_t d = desc[base + u];
+              cbv[u] = d >> 6;
+              cev[u] = cbv[u] + (d & 63u) + 1u;
+            }
+          }
+          run_chunks(cbv, cev);
+        } else {
+          // cells b * g.cells ...: their chunks, kLoadAhead at a time
+          const uint32_t* __restrict__ cstart = V.cell_start;
+          const int32_t nx = tb.nx, nxy = tb.nxy;
+          const int32_t cbeg = static_cast<int32_t>(b) * g.cells;
+          const int32_t cend = min(cbeg + g.cells, tb.ncell);
+          const int32_t cl = cbeg + static_cast<int32_t>(lane);
+          uint32_t cb = 0, ce = 0;
+          if (cl < cend) {
+            const int32_t iz = cl / nxy, rem = cl - iz * nxy;
+            const int32_t iy = rem / nx, ix = rem - iy * nx;
+            const uint32_t code = morton3(static_cast<uint32_t>(tb.k0[0] + ix - V.kmin[0]),
+                                          static_cast<uint32_t>(tb.k0[1] + iy - V.kmin[1]),
+                                          static_cast<uint32_t>(tb.k0[2] + iz - V.kmin[2]));
+            cb = cstart[code];
+            ce = cstart[code + 1];
+          }
+          const uint32_t nch = (ce - cb + kWave - 1) / kWave;
+          uint32_t cincl = nch;
+          for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(cincl, o, 64);
+            if (lane >= static_cast<uint32_t>(o)) cincl += u;
+          }
+          const uint32_t off = cincl - nch;
+          const uint32_t nchunk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cincl), 63));
+          for (uint32_t kc = 0; kc < nchunk; kc += kLoadAhead) {
+            for (int u = 0; u < kLoadAhead; ++u) {
+              const uint32_t kk = kc + u;
+              cbv[u] = 0;
+              cev[u] = 0;
+              if (kk < nchunk) {
+                // the cell holding chunk kk, and the chunk's atom range
+                const uint64_t mc = ballot64(nch > 0 && off <= kk);
+                const int ln = 63 - static_cast<int>(__builtin_clzll(mc));
+                cbv[u] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cb), ln)) +
+                         kWave * (kk - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off), ln)));
+                cev[u] = min(cbv[u] + kWave, static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ce), ln)));
+              }
+            }
+            run_chunks(cbv, cev);
           }
         }
       }
@@ -507,6 +584,8 @@ static bool traj_choose(const dnab_population* pop, TrajPlan& pl) {
 
 bool trajectory_supported(const dnab_population* pop, const dnab_langevin_params* p) {
   if (p->force_model != DNAB_FORCE_ALLATOM || pop->n_cand == 0) return false;
+  for (const AgView& v : pop->views_host)
+    if (v.n_atoms >= kTrajMaxAgAtoms) return false;
   TrajPlan pl;
   return traj_choose(pop, pl);
 }
@@ -624,6 +703,7 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
   a.work = nullptr;
   a.n_groups = 0;
   a.cells = static_cast<int32_t>(pop->traj_cells);
+  a.chunks = pop->traj_chunks;
   a.n_cand = static_cast<uint32_t>(pop->n_cand);
   a.n_steps = p->n_steps;
   a.score_every = p->score_every;
@@ -683,6 +763,7 @@ int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_
   t->cand_per_group = pop->traj_cpw;
   t->balance = pop->traj_balance ? 1u : 0u;
   t->cells_per_item = pop->traj_cells;
+  t->chunks_per_item = pop->traj_chunks;
   return DNAB_OK;
 }
 
@@ -696,6 +777,10 @@ int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
                           kTrajMaxCpw);
   if (t->cells_per_item < 1 || t->cells_per_item > 64)
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "trajectory tuning: cells_per_item must be 1..64");
+  if (t->chunks_per_item < 1 || t->chunks_per_item > static_cast<uint32_t>(kLoadAhead))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "trajectory tuning: chunks_per_item must be 1..%d",
+                          kLoadAhead);
+  pop->traj_chunks = t->chunks_per_item;
   pop->traj_waves = t->waves;
   pop->traj_cells = t->cells_per_item;
   pop->traj_cpw = t->cand_per_group;

@@ -368,14 +368,18 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
  * grouped so that groups carry similar work (a candidate in deep contact runs
  * alone), heaviest group first; otherwise groups are consecutive candidates.
  * Groups that do not fit one workgroup's LDS hold fewer candidates.
- * A work item is a run of `cells_per_item` grid cells of one tile.
- * Default 16 waves, 6 candidates, balance on, 2 cells. Results do not
- * depend on it. */
+ * Each step lists, per tile of 64 atoms, the antigen chunks (up to 64 atoms
+ * of one grid cell) of the cells within the cutoff of the tile's box; a work
+ * item is a run of `chunks_per_item` of them (or of `cells_per_item` cells
+ * when the list overflows). Default 16 waves, 6 candidates, balance on,
+ * 2 cells, 2 chunks. Results do not depend on it. */
 typedef struct dnab_traj_tuning {
   uint32_t waves;
   uint32_t cand_per_group;
   uint32_t balance;
-  uint32_t cells_per_item;  /* grid cells per work item (1..64) */
+  uint32_t cells_per_item;  /* grid cells per work item, for a tile whose
+                               chunk list overflowed (1..64) */
+  uint32_t chunks_per_item; /* listed chunks per work item (1..4) */
 } dnab_traj_tuning;
 int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_tuning* t);
 int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,

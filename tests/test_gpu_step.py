@@ -123,12 +123,12 @@ def test_trajectory_tuning_changes_speed_not_results(ctx):
     w = W.config3(n_cand=80)
     kw = dict(n_steps=12, score_every=4, top_k=8, with_noise=1, seed=9)
     ref = None
-    for waves, cpw, bal, cells in ((4, 1, 0, 16), (8, 3, 0, 5), (16, 4, 0, 64), (16, 16, 0, 1),
-                                   (16, 6, 1, 16), (4, 2, 1, 3)):
+    for waves, cpw, bal, cells, chunks in ((4, 1, 0, 16, 4), (8, 3, 0, 5, 1), (16, 4, 0, 64, 3),
+                                           (16, 16, 0, 1, 2), (16, 6, 1, 16, 2), (4, 2, 1, 3, 1)):
         ags, pop = W.device_objects(ctx, w)
-        pop.set_trajectory_tuning(waves, cpw, bal, cells)
+        pop.set_trajectory_tuning(waves, cpw, bal, cells, chunks)
         assert pop.trajectory_tuning() == {"waves": waves, "cand_per_group": cpw, "balance": bal,
-                                           "cells_per_item": cells}
+                                           "cells_per_item": cells, "chunks_per_item": chunks}
         p = langevin_params(restraint_radius=w.restraint_radius[0], **kw)
         idx, e = pop.run(p)
         hi, he = pop.scores()
@@ -142,11 +142,17 @@ def test_trajectory_tuning_changes_speed_not_results(ctx):
         pop.set_trajectory_tuning(12, 4)
 
 
-@pytest.mark.parametrize("config", ["config2", "config3"])
+@pytest.mark.parametrize("config", ["config2", "config3", "config2_fine_cells"])
 def test_fused_trajectory_equals_step_kernels(ctx, config):
     # no noise (the reference's GPU phase), pair counting on: the in-cutoff
-    # pair count is a property of the trajectory, the tested count is not
-    w = getattr(W, config)(n_cand=96)
+    # pair count is a property of the trajectory, the tested count is not.
+    # With 4 A antigen cells a tile's box spans hundreds of cells, more chunks
+    # than a tile's chunk list holds: those tiles go through cell items.
+    if config == "config2_fine_cells":
+        import dataclasses
+        w = dataclasses.replace(W.config2(n_cand=96), cell_size=4.0)
+    else:
+        w = getattr(W, config)(n_cand=96)
     kw = dict(n_steps=40, score_every=0, top_k=8, count_pairs=1)
     step = _run_state(ctx, w, step_kernels=1, **kw)
     fused = _run_state(ctx, w, step_kernels=0, **kw)
