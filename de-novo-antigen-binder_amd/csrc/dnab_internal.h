@@ -217,7 +217,7 @@ struct dnab_population {
   uint32_t max_cand_atoms = 0, max_cand_res = 0;  // largest candidate
   std::vector<uint32_t> h_cand_atom_off, h_cand_res_off;  // host copies
   // fused trajectory kernel: workgroup shape, grouping by measured work
-  uint32_t traj_waves = 16, traj_cpw = 6, traj_cells = 2, traj_chunks = 2;
+  uint32_t traj_waves = 16, traj_cpw = 4, traj_cells = 2, traj_chunks = 2;
   bool traj_balance = true;
   dnab::DevBuf<uint32_t> traj_work, traj_group_off, traj_group_cand;
   dnab::DevBuf<uint8_t> res_aa;         // residue types, when given

@@ -371,7 +371,7 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
  * Each step lists, per tile of 64 atoms, the antigen chunks (up to 64 atoms
  * of one grid cell) of the cells within the cutoff of the tile's box; a work
  * item is a run of `chunks_per_item` of them (or of `cells_per_item` cells
- * when the list overflows). Default 16 waves, 6 candidates, balance on,
+ * when the list overflows). Default 16 waves, 4 candidates, balance on,
  * 2 cells, 2 chunks. Results do not depend on it. */
 typedef struct dnab_traj_tuning {
   uint32_t waves;
