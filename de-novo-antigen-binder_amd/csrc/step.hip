@@ -83,7 +83,8 @@ k_candidate_update(const AgView* __restrict__ views, const uint32_t* __restrict_
       }
     }
     const float sigma = sp.with_noise ? sigma_table[iter] : 0.0f;
-    res_delta[r] = residue_delta(V, sp, r0, p, fx, fy, fz, sigma, iter, r);
+    res_delta[r] = residue_delta(V.center[0], V.center[1], V.center[2], sp, r0, p, fx, fy, fz,
+                                 sigma, iter, r);
   }
 }
 

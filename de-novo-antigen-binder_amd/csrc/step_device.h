@@ -25,14 +25,15 @@ __device__ inline float clamp_rust(float v, float lo, float hi) {
 }
 
 // Displacement of residue `r` (global index) whose Calpha sits at `p`, given
-// the summed force (fx, fy, fz) on its atoms; r0 is the restraint radius.
-__device__ inline float4 residue_delta(const AgView& V, const StepParams& sp, float r0, float4 p,
-                                       float fx, float fy, float fz, float sigma, uint32_t iter,
-                                       uint32_t r) {
+// the summed force (fx, fy, fz) on its atoms; (cx, cy, cz) is the antigen
+// centre and r0 the restraint radius.
+__device__ inline float4 residue_delta(float cx, float cy, float cz, const StepParams& sp, float r0,
+                                       float4 p, float fx, float fy, float fz, float sigma,
+                                       uint32_t iter, uint32_t r) {
   // src/diffusion.rs:318-327
-  const float dx_c = p.x - V.center[0];
-  const float dy_c = p.y - V.center[1];
-  const float dz_c = p.z - V.center[2];
+  const float dx_c = p.x - cx;
+  const float dy_c = p.y - cy;
+  const float dz_c = p.z - cz;
   const float d = fmaxf(sqrtf(dx_c * dx_c + dy_c * dy_c + dz_c * dz_c), 0.001f);
   const float excess = d - r0;
   const float f_r = -sp.restraint_k * excess / d;

@@ -48,12 +48,14 @@ constexpr float kTrajErrPerAbs = 32.0f / 16777216.0f;  // 32 ulp of f32 per unit
 struct TrajTile {
   float lo[3], hi[3];
   int32_t k0[3];
-  int32_t nx, nxy, ncell;
+  int32_t nx, ny, nz, nxy, ncell;
   uint32_t nchunk;  // listed chunks, or kTileOverflow
   uint32_t cand;    // candidate slot in the workgroup
 };
 
 struct TrajCand {
+  float center[3];     // antigen centre
+  float r_restr;       // restraint radius
   uint32_t id;         // candidate index
   uint32_t a0, n;      // global atom range
   uint32_t r0, nres;   // global residue range
@@ -95,11 +97,12 @@ struct TrajArgs {
 
 // Bytes of dynamic LDS for a capacity of a_cap atom slots and r_cap residues:
 // 5 integer sums, energy, position, parameters, force per atom slot; the
-// residue slot per atom slot; the displacement per residue; the chunk list
-// of each tile (a_cap / 64 tiles).
+// displacement and the (atom range, Calpha, index, candidate) record per
+// residue; the residue slot per atom slot; the chunk list of each tile
+// (a_cap / 64 tiles).
 __host__ __device__ inline size_t traj_lds_bytes(uint32_t a_cap, uint32_t r_cap) {
   const size_t res = (static_cast<size_t>(a_cap) * 2 + 3) & ~static_cast<size_t>(3);
-  return static_cast<size_t>(a_cap) * (5 * 8 + 8 + 3 * 16) + static_cast<size_t>(r_cap) * 16 + res +
+  return static_cast<size_t>(a_cap) * (5 * 8 + 8 + 3 * 16) + static_cast<size_t>(r_cap) * 32 + res +
          static_cast<size_t>(a_cap / kWave) * kTileChunks * 4;
 }
 // Static LDS of a workgroup of `waves` waves (staged chunks, tiles, candidates, counter).
@@ -137,7 +140,8 @@ k_trajectory(TrajArgs g) {
   float4* s_par = s_pos + AC;                               // [a_cap]
   float4* s_force = s_par + AC;                             // [a_cap]
   float4* s_delta = s_force + AC;                           // [r_cap]
-  uint16_t* s_res = reinterpret_cast<uint16_t*>(s_delta + g.r_cap);  // [a_cap]
+  uint4* s_rinfo = reinterpret_cast<uint4*>(s_delta + g.r_cap);       // [r_cap]
+  uint16_t* s_res = reinterpret_cast<uint16_t*>(s_rinfo + g.r_cap);  // [a_cap]
   uint32_t* s_desc = reinterpret_cast<uint32_t*>(s_dyn + ((reinterpret_cast<uintptr_t>(s_res + AC) -
                                                              reinterpret_cast<uintptr_t>(s_dyn) + 3) & ~uintptr_t(3)));
 
@@ -162,10 +166,19 @@ k_trajectory(TrajArgs g) {
       C.a_off = a_off;
       C.r_off = r_off;
       C.ag = g.cand_ag[c];
+      const AgView& V = g.views[C.ag];
+      C.center[0] = V.center[0];
+      C.center[1] = V.center[1];
+      C.center[2] = V.center[2];
+      C.r_restr = V.restraint_radius > 0.0f ? V.restraint_radius : g.sp.restraint_radius;
       s_cand[j] = C;
       s_work[j] = 0;
       const uint32_t slots = (C.n + kWave - 1) & ~(kWave - 1);
-      for (uint32_t t = a_off / kWave; t < (a_off + slots) / kWave; ++t) s_tile[t].cand = j;
+      for (uint32_t t = a_off / kWave; t < (a_off + slots) / kWave; ++t) {
+        s_tile[t].cand = j;
+        s_tile[t].nx = INT32_MIN;  // no chunk list yet (no cell range has this width)
+        s_tile[t].nchunk = 0;
+      }
       a_off += slots;
       r_off += C.nres;
     }
@@ -189,6 +202,16 @@ k_trajectory(TrajArgs g) {
       s_res[k] = kTrajPad;
     }
   }
+  // per residue: LDS atom range and Calpha slot, global index, candidate slot
+  for (uint32_t r = tid; r < R; r += WAVES * kWave) {
+    uint32_t j = 0;
+    while (j + 1 < NC && s_cand[j + 1].r_off <= r) ++j;
+    const TrajCand C = s_cand[j];
+    const uint32_t rg = C.r0 + (r - C.r_off);
+    const uint32_t b0 = C.a_off + (g.res_atom_off[rg] - C.a0);
+    const uint32_t b1 = C.a_off + (g.res_atom_off[rg + 1] - C.a0);
+    s_rinfo[r] = make_uint4(b0 | (b1 << 16), C.a_off + (g.res_ca[rg] - C.a0), rg, j);
+  }
   __syncthreads();
 
   // candidate j's energy and sum |e| (one wave), in the order of the
@@ -207,6 +230,8 @@ k_trajectory(TrajArgs g) {
 
   uint32_t n_in = 0, n_tested = 0;
   for (uint32_t s = 0;; ++s) {
+    // this step's noise amplitude, loaded early (used in phase 4)
+    const float sigma = (g.sp.with_noise && s < g.n_steps) ? g.sigma_table[s] : 0.0f;
     // 1. tile boxes, the chunks of the grid cells within the cutoff of
     //    each box (listed in LDS), and cleared accumulators
     for (uint32_t t = wave; t < T; t += WAVES) {
@@ -240,7 +265,15 @@ k_trajectory(TrajArgs g) {
       uint32_t* desc = s_desc + t * kTileChunks;
       uint32_t nt = 0;
       bool over = false;
-      for (int32_t cb0 = 0; cb0 < ncell; cb0 += kWave) {
+      // the antigen does not move: the list stands while the cell range does
+      const TrajTile& tc = s_tile[t];
+      const bool same = tc.nx == nx && tc.ny == ny && tc.nz == nz && tc.k0[0] == kx0 &&
+                        tc.k0[1] == ky0 && tc.k0[2] == kz0;
+      if (same) {
+        nt = tc.nchunk;
+        over = nt == kTileOverflow;
+      }
+      for (int32_t cb0 = 0; !same && cb0 < ncell; cb0 += kWave) {
         // lane = a cell, in (kz, ky, kx) order
         const int32_t cl = cb0 + static_cast<int32_t>(lane);
         uint32_t cb = 0, ce = 0;
@@ -277,6 +310,8 @@ k_trajectory(TrajArgs g) {
         tb.hi[0] = hix; tb.hi[1] = hiy; tb.hi[2] = hiz;
         tb.k0[0] = kx0; tb.k0[1] = ky0; tb.k0[2] = kz0;
         tb.nx = nx;
+        tb.ny = ny;
+        tb.nz = nz;
         tb.nxy = nxy;
         tb.ncell = ncell;
         tb.nchunk = over ? kTileOverflow : nt;
@@ -462,16 +497,10 @@ k_trajectory(TrajArgs g) {
     }
 
     // 4. residue update (src/diffusion.rs:300-339)
-    const float sigma = g.sp.with_noise ? g.sigma_table[s] : 0.0f;
     for (uint32_t r = tid; r < R; r += WAVES * kWave) {
-      uint32_t j = 0;
-      while (j + 1 < NC && s_cand[j + 1].r_off <= r) ++j;
-      const TrajCand C = s_cand[j];
-      const AgView& V = g.views[C.ag];
-      const float r_restr = V.restraint_radius > 0.0f ? V.restraint_radius : g.sp.restraint_radius;
-      const uint32_t rg = C.r0 + (r - C.r_off);
-      const uint32_t b0 = C.a_off + (g.res_atom_off[rg] - C.a0);
-      const uint32_t b1 = C.a_off + (g.res_atom_off[rg + 1] - C.a0);
+      const uint4 ri = s_rinfo[r];
+      const TrajCand& C = s_cand[ri.w];
+      const uint32_t b0 = ri.x & 0xFFFFu, b1 = ri.x >> 16;
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
       for (uint32_t k = b0; k < b1; ++k) {  // in atom order
         const float4 f = s_force[k];
@@ -479,8 +508,8 @@ k_trajectory(TrajArgs g) {
         fy += f.y;
         fz += f.z;
       }
-      const float4 p = s_pos[C.a_off + (g.res_ca[rg] - C.a0)];
-      s_delta[r] = residue_delta(V, g.sp, r_restr, p, fx, fy, fz, sigma, s, rg);
+      s_delta[r] = residue_delta(C.center[0], C.center[1], C.center[2], g.sp, C.r_restr,
+                                 s_pos[ri.y], fx, fy, fz, sigma, s, ri.z);
     }
     __syncthreads();
 
