@@ -48,6 +48,16 @@ struct DevBuf {
   hipError_t ensure(size_t count) { return count <= n ? hipSuccess : alloc(count); }
 };
 
+// Scratch of a multi-pass top-k (step.hip): chunk lengths per pass and two
+// ping-pong buffers; kept across calls, it keeps its buffers.
+struct TopkPlan {
+  uint32_t k = 0;
+  uint32_t rows = 1;
+  std::vector<uint32_t> lens;
+  DevBuf<double> e[2];
+  DevBuf<uint32_t> idx[2];
+};
+
 // A SpatialHashGrid laid out on the device: points sorted cell-major with
 // cells in Morton order of (key - key_min); within a cell, ascending
 // original index (the reference's append order, src/spatial.rs:35-38).
@@ -220,6 +230,8 @@ struct dnab_population {
   uint32_t traj_waves = 16, traj_cpw = 4, traj_cells = 2, traj_chunks = 2;
   bool traj_balance = true;
   dnab::DevBuf<uint32_t> traj_work, traj_group_off, traj_group_cand;
+  std::vector<uint32_t> h_group_off, h_group_cand;  // sources of the group-table copies
+  std::vector<float> h_sigma;                       // source of the noise-table copy
   dnab::DevBuf<uint8_t> res_aa;         // residue types, when given
   bool has_res_aa = false;
   dnab::DevBuf<float4> res_cg;  // residue-level params (q, eps, sigma, hydrophobic)
@@ -263,6 +275,7 @@ struct dnab_population {
   dnab::DevBuf<double> hist_e;
   uint32_t hist_rows = 0, hist_k = 0;
   dnab::DevBuf<double> score_e;  // fused loop: candidate energies per scoring
+  dnab::TopkPlan run_topk;       // the loop's top-k scratch
   // candidate mask: only masked-in candidates are sorted and scored
   dnab::DevBuf<uint8_t> cand_mask;
   bool masked = false;
@@ -312,13 +325,6 @@ void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end
 int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 // step.hip: multi-pass top-k (ascending, NaN last, ties by index)
-struct TopkPlan {
-  uint32_t k = 0;
-  uint32_t rows = 1;
-  std::vector<uint32_t> lens;
-  DevBuf<double> e[2];
-  DevBuf<uint32_t> idx[2];
-};
 int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k, uint32_t rows = 1);
 void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
                  uint32_t* final_idx, const uint32_t* counter = nullptr, uint32_t row_div = 1,

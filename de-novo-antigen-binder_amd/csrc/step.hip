@@ -250,10 +250,10 @@ int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k, uint32_t rows
     len = static_cast<uint32_t>(out);
   }
   cap *= std::max<uint32_t>(rows, 1);
-  DNAB_ALLOC_CHECK(ctx, pl.e[0].alloc(cap));
-  DNAB_ALLOC_CHECK(ctx, pl.e[1].alloc(cap));
-  DNAB_ALLOC_CHECK(ctx, pl.idx[0].alloc(cap));
-  DNAB_ALLOC_CHECK(ctx, pl.idx[1].alloc(cap));
+  DNAB_ALLOC_CHECK(ctx, pl.e[0].ensure(cap));  // a plan kept across calls keeps its buffers
+  DNAB_ALLOC_CHECK(ctx, pl.e[1].ensure(cap));
+  DNAB_ALLOC_CHECK(ctx, pl.idx[0].ensure(cap));
+  DNAB_ALLOC_CHECK(ctx, pl.idx[1].ensure(cap));
   return DNAB_OK;
 }
 
@@ -306,13 +306,15 @@ static int validate_params(dnab_ctx* ctx, const dnab_langevin_params* p) {
 // in f32 as src/diffusion.rs:102-104 and 315, computed on the host.
 static int upload_sigma(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                         uint32_t n_iters) {
-  std::vector<float> s(n_iters + 1);
+  // kept in the population: the asynchronous copy reads it after this returns
+  std::vector<float>& s = pop->h_sigma;
+  s.resize(n_iters + 1);
   for (uint32_t i = 0; i <= n_iters; ++i) {
     float it = static_cast<float>(i + p->schedule_offset);
     float temp = p->t_hot * std::pow(p->t_cold / p->t_hot, it / static_cast<float>(p->schedule_total));
     s[i] = p->noise_base * std::sqrt(temp);
   }
-  DNAB_ALLOC_CHECK(ctx, pop->sigma_table.alloc(s.size()));
+  DNAB_ALLOC_CHECK(ctx, pop->sigma_table.ensure(s.size()));
   return upload(ctx, pop->sigma_table.p, s.data(), s.size() * sizeof(float));
 }
 
@@ -448,7 +450,7 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
     // every step in one launch, which leaves each scoring's candidate
     // energies in a row of score_e; then the top k of every row at once
     DNAB_ALLOC_CHECK(ctx, pop->score_e.ensure(static_cast<size_t>(rows) * nc));
-    TopkPlan tr;
+    TopkPlan& tr = pop->run_topk;
     DNAB_RETURN_IF(topk_plan(ctx, tr, nc, k, rows));
     DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
     DNAB_RETURN_IF(trajectory_run(ctx, pop, p, sp));
@@ -487,7 +489,7 @@ namespace dnab {
 // The loop as per-step population kernels, eager or replayed from a graph.
 int step_kernel_loop(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                      const StepParams& sp, uint32_t nc, uint32_t k, uint32_t n_mid) {
-  TopkPlan tk;
+  TopkPlan& tk = pop->run_topk;
   DNAB_RETURN_IF(topk_plan(ctx, tk, nc, k));
   const uint32_t L = p->score_every ? p->score_every : std::max<uint32_t>(p->n_steps, 1);
   const bool graph = p->use_graph && p->n_steps >= L;

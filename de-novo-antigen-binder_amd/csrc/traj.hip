@@ -26,7 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <set>
+#include <functional>
 #include <vector>
 
 #include "device_common.h"
@@ -62,6 +62,15 @@ struct TrajCand {
   uint32_t a_off;      // first LDS atom slot (a multiple of 64)
   uint32_t r_off;      // first LDS residue slot
   uint32_t ag;         // antigen view
+};
+
+// The antigen fields a step reads, copied into LDS once per candidate slot.
+struct TrajAg {
+  const AgAtom* atoms;
+  const uint32_t* cell_start;
+  float excl_lo[3], excl_hi[3];
+  float rc, rc2, inv;
+  int32_t kmin[3], kmax[3];
 };
 
 struct TrajArgs {
@@ -105,13 +114,14 @@ __host__ __device__ inline size_t traj_lds_bytes(uint32_t a_cap, uint32_t r_cap)
   return static_cast<size_t>(a_cap) * (5 * 8 + 8 + 3 * 16) + static_cast<size_t>(r_cap) * 32 + res +
          static_cast<size_t>(a_cap / kWave) * kTileChunks * 4;
 }
-// Static LDS of a workgroup of `waves` waves (staged chunks, tiles, candidates, counter).
+// Static LDS of a workgroup of `waves` waves (staged chunks, tiles,
+// candidates, antigen fields, counters).
 __host__ __device__ inline size_t traj_static_lds(int waves) {
   return static_cast<size_t>(waves) * kWave * 32 + kTrajMaxTiles * sizeof(TrajTile) +
-         kTrajMaxCpw * sizeof(TrajCand) + 64;
+         kTrajMaxCpw * (sizeof(TrajCand) + sizeof(TrajAg) + 4) + 256;
 }
 
-__device__ inline bool traj_can_pair(const AgView& V, float4 p) {
+__device__ inline bool traj_can_pair(const TrajAg& V, float4 p) {
   const float bx = fmaxf(fmaxf(V.excl_lo[0] - p.x, p.x - V.excl_hi[0]), 0.0f);
   const float by = fmaxf(fmaxf(V.excl_lo[1] - p.y, p.y - V.excl_hi[1]), 0.0f);
   const float bz = fmaxf(fmaxf(V.excl_lo[2] - p.z, p.z - V.excl_hi[2]), 0.0f);
@@ -129,6 +139,7 @@ k_trajectory(TrajArgs g) {
   __shared__ float4 s_stage[WAVES][kWave][2];
   __shared__ TrajTile s_tile[kTrajMaxTiles];
   __shared__ TrajCand s_cand[kTrajMaxCpw];
+  __shared__ TrajAg s_ag[kTrajMaxCpw];
   __shared__ uint32_t s_next, s_ncand, s_nslot, s_nres;
   __shared__ uint32_t s_work[kTrajMaxCpw];
 
@@ -172,6 +183,18 @@ k_trajectory(TrajArgs g) {
       C.center[2] = V.center[2];
       C.r_restr = V.restraint_radius > 0.0f ? V.restraint_radius : g.sp.restraint_radius;
       s_cand[j] = C;
+      TrajAg& G = s_ag[j];
+      G.atoms = V.atoms;
+      G.cell_start = V.cell_start;
+      for (int d = 0; d < 3; ++d) {
+        G.excl_lo[d] = V.excl_lo[d];
+        G.excl_hi[d] = V.excl_hi[d];
+        G.kmin[d] = V.kmin[d];
+        G.kmax[d] = V.kmax[d];
+      }
+      G.rc = V.rc;
+      G.rc2 = V.rc2;
+      G.inv = V.inv;
       s_work[j] = 0;
       const uint32_t slots = (C.n + kWave - 1) & ~(kWave - 1);
       for (uint32_t t = a_off / kWave; t < (a_off + slots) / kWave; ++t) {
@@ -236,7 +259,7 @@ k_trajectory(TrajArgs g) {
     //    each box (listed in LDS), and cleared accumulators
     for (uint32_t t = wave; t < T; t += WAVES) {
       const uint32_t k = t * kWave + lane;
-      const AgView& V = g.views[s_cand[s_tile[t].cand].ag];
+      const TrajAg& V = s_ag[s_tile[t].cand];
       const float4 P = s_pos[k];
       const bool valid = s_res[k] != kTrajPad && traj_can_pair(V, P);
       const float inf = __builtin_inff();
@@ -367,7 +390,7 @@ k_trajectory(TrajArgs g) {
         const int32_t t = 63 - static_cast<int32_t>(__builtin_clzll(m));
         const uint32_t b = i - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(first), t));
         const TrajTile& tb = s_tile[t];
-        const AgView& V = g.views[s_cand[tb.cand].ag];
+        const TrajAg& V = s_ag[tb.cand];
         if (t != cur) {
           if (cur >= 0) flush();
           cur = t;
@@ -637,55 +660,74 @@ static bool traj_balance(const dnab_population* pop, uint32_t waves, uint32_t cp
                    [&](uint32_t x, uint32_t y) { return work[x] > work[y]; });
   struct Bin {
     uint64_t w = 0;
-    uint32_t a = 0, r = 0;
-    std::vector<uint32_t> c;
+    uint32_t a = 0, r = 0, n = 0;
   };
+  std::vector<uint32_t> assign(nc);
   std::vector<Bin> bins(std::max<uint32_t>(n_bins, 1));
-  std::set<std::pair<uint64_t, uint32_t>> open;  // (work, bin) of bins with room
-  for (uint32_t b = 0; b < bins.size(); ++b) open.insert({0, b});
+  // min-heap of (work, bin) over groups with room; a group that cannot take
+  // the current candidate is set aside and returned to the heap afterwards
+  using Key = std::pair<uint64_t, uint32_t>;
+  std::vector<Key> heap, aside;
+  heap.reserve(bins.size() + 8);
+  for (uint32_t b = 0; b < bins.size(); ++b) heap.push_back({0, b});
+  std::make_heap(heap.begin(), heap.end(), std::greater<Key>());
   for (uint32_t c : order) {
     const uint32_t slots = (ao[c + 1] - ao[c] + kWave - 1) & ~(kWave - 1);
     const uint32_t nres = ro[c + 1] - ro[c];
     auto fits = [&](const Bin& B) {
       const uint32_t a = B.a + slots, r = B.r + nres;
-      return B.c.size() < cpw && a / kWave <= kTrajMaxTiles && r <= 0xFFFFu &&
+      return a / kWave <= kTrajMaxTiles && r <= 0xFFFFu &&
              traj_lds_bytes(a, r) + stat <= kTrajLdsLimit;
     };
     uint32_t pick = UINT32_MAX;
-    for (const auto& e : open) {
-      if (fits(bins[e.second])) {
-        pick = e.second;
+    while (!heap.empty()) {
+      std::pop_heap(heap.begin(), heap.end(), std::greater<Key>());
+      const Key top = heap.back();
+      heap.pop_back();
+      if (fits(bins[top.second])) {
+        pick = top.second;
         break;
       }
+      aside.push_back(top);
     }
+    for (const Key& k : aside) {
+      heap.push_back(k);
+      std::push_heap(heap.begin(), heap.end(), std::greater<Key>());
+    }
+    aside.clear();
     if (pick == UINT32_MAX) {  // no open group has room: a new one
       pick = static_cast<uint32_t>(bins.size());
       bins.emplace_back();
-      open.insert({0, pick});
     }
     Bin& B = bins[pick];
-    open.erase({B.w, pick});
     B.w += work[c];
     B.a += slots;
     B.r += nres;
-    B.c.push_back(c);
-    if (B.c.size() < cpw) open.insert({B.w, pick});
+    ++B.n;
+    assign[c] = pick;
+    if (B.n < cpw) {
+      heap.push_back({B.w, pick});
+      std::push_heap(heap.begin(), heap.end(), std::greater<Key>());
+    }
   }
   std::vector<uint32_t> ids;
   for (uint32_t b = 0; b < bins.size(); ++b)
-    if (!bins[b].c.empty()) ids.push_back(b);
+    if (bins[b].n) ids.push_back(b);
   std::stable_sort(ids.begin(), ids.end(), [&](uint32_t x, uint32_t y) { return bins[x].w > bins[y].w; });
-  off.assign(1, 0);
-  cand.clear();
+  // group g = bin ids[g]: offsets, then the candidates in ascending index
+  std::vector<uint32_t> slot(bins.size());
+  off.assign(ids.size() + 1, 0);
   uint32_t a_cap = kWave, r_cap = 1, most = 1;
-  for (uint32_t b : ids) {
-    const Bin& B = bins[b];
-    cand.insert(cand.end(), B.c.begin(), B.c.end());
-    off.push_back(static_cast<uint32_t>(cand.size()));
+  for (size_t gi = 0; gi < ids.size(); ++gi) {
+    const Bin& B = bins[ids[gi]];
+    slot[ids[gi]] = off[gi];
+    off[gi + 1] = off[gi] + B.n;
     a_cap = std::max(a_cap, B.a);
     r_cap = std::max(r_cap, B.r);
-    most = std::max<uint32_t>(most, static_cast<uint32_t>(B.c.size()));
+    most = std::max(most, B.n);
   }
+  cand.resize(nc);
+  for (size_t c = 0; c < nc; ++c) cand[slot[assign[c]]++] = static_cast<uint32_t>(c);
   const size_t lds = traj_lds_bytes(a_cap, r_cap);
   if (lds + stat > kTrajLdsLimit) return false;
   pl.waves = waves;
@@ -753,7 +795,9 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
     DNAB_HIP_CHECK(ctx, launch_plan(e, pl, false, ctx->stream));
     std::vector<uint32_t> work(nc);
     DNAB_RETURN_IF(download(ctx, work.data(), pop->traj_work.p, nc * sizeof(uint32_t)));
-    std::vector<uint32_t> off, cand;
+    // kept in the population: the asynchronous copies read them after this returns
+    std::vector<uint32_t>& off = pop->h_group_off;
+    std::vector<uint32_t>& cand = pop->h_group_cand;
     TrajPlan bal;
     const uint32_t bins = std::max<uint32_t>(static_cast<uint32_t>(ctx->cu_count),
                                              static_cast<uint32_t>((nc + pop->traj_cpw - 1) / pop->traj_cpw));
