@@ -28,22 +28,34 @@ dnab_antigen_desc antigen_desc(const dnab_atom_cloud* ag, float cutoff) {
   return d;
 }
 
-// One candidate holding every antibody atom as a single residue.
+// One candidate of the given residues (or, with n_res == 0 and no residue
+// arrays, every antibody atom as a single residue).
 int single_candidate(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
-                     const dnab_atom_cloud* antibody, float cutoff, AgGuard& ag, PopGuard& pop) {
+                     const dnab_atom_cloud* antibody, float cutoff, AgGuard& ag, PopGuard& pop,
+                     size_t n_res = 0, const uint32_t* res_atom_off = nullptr,
+                     const uint32_t* res_ca_atom = nullptr, const uint8_t* amino_acid = nullptr) {
   if (!(cutoff > 0.0f) || !std::isfinite(cutoff))
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "cutoff must be positive");
+  if (n_res >= 0xFFFFFF00ull)
+    return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "antibody: too many residues");
   dnab_antigen_desc ad = antigen_desc(antigen, cutoff);
   DNAB_RETURN_IF(dnab_antigen_create(ctx, &ad, &ag.p));
   const uint32_t n = static_cast<uint32_t>(antibody->n);
-  uint32_t cand_ag = 0, cand_res_off[2] = {0, n ? 1u : 0u}, res_atom_off[2] = {0, n}, res_ca = 0;
+  uint32_t cand_ag = 0, cand_res_off[2] = {0, n ? 1u : 0u}, one_res[2] = {0, n}, res_ca = 0;
   dnab_population_desc pd{};
   pd.n_cand = 1;
   pd.n_res = n ? 1 : 0;
   pd.cand_antigen = &cand_ag;
   pd.cand_res_off = cand_res_off;
-  pd.res_atom_off = res_atom_off;
+  pd.res_atom_off = one_res;
   pd.res_ca_atom = &res_ca;
+  if (res_atom_off) {
+    cand_res_off[1] = static_cast<uint32_t>(n_res);
+    pd.n_res = n_res;
+    pd.res_atom_off = res_atom_off;
+    pd.res_ca_atom = res_ca_atom;
+    pd.res_amino_acid = amino_acid;
+  }
   pd.atoms = *antibody;
   dnab_antigen* ags[1] = {ag.p};
   return dnab_population_create(ctx, ags, 1, &pd, &pop.p);
@@ -66,13 +78,13 @@ int dnab_interaction_energy_atoms_breakdown(dnab_ctx* ctx, const dnab_atom_cloud
   PopGuard pop;
   DNAB_RETURN_IF(single_candidate(ctx, antigen, antibody, cutoff, ag, pop));
   DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, DNAB_EVAL_BREAKDOWN));
-  double lj = 0, co = 0, hy = 0;
-  DNAB_RETURN_IF(dnab_population_read_breakdown(ctx, pop.p, &lj, &co, &hy));
-  out->lj = lj;
-  out->coulomb = co;
-  out->hydrophobic = hy;
-  out->hbond = 0.0;
-  out->solvation = 0.0;
+  double t[6];
+  DNAB_RETURN_IF(dnab_population_read_terms(ctx, pop.p, t));
+  out->lj = t[0];
+  out->coulomb = t[1];
+  out->hydrophobic = t[2];
+  out->hbond = t[3];
+  out->solvation = t[4];
   return DNAB_OK;
 }
 
@@ -85,6 +97,26 @@ int dnab_interaction_energy_atoms(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
   dnab_energy_breakdown b{};
   DNAB_RETURN_IF(dnab_interaction_energy_atoms_breakdown(ctx, antigen, antibody, cutoff, &b));
   *energy = (((b.lj + b.coulomb) + b.hydrophobic) + b.hbond) + b.solvation;
+  return DNAB_OK;
+}
+
+int dnab_allatom_energy(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                        const dnab_atom_cloud* antibody, const uint32_t* res_atom_off,
+                        const uint32_t* res_ca_atom, const uint8_t* amino_acid, size_t n_res,
+                        float cutoff, double* energy, double* disulfide) {
+  // src/diffusion.rs:281-283: interaction_energy_atoms + the antibody's
+  // disulfide_energy (src/allatom.rs:155-180), which needs its residues.
+  if (!ctx || !cloud_ok(antigen) || !cloud_ok(antibody) || !energy) return DNAB_ERR_INVALID;
+  if (!res_atom_off || (n_res && (!res_ca_atom || !amino_acid))) return DNAB_ERR_INVALID;
+  AgGuard ag;
+  PopGuard pop;
+  DNAB_RETURN_IF(single_candidate(ctx, antigen, antibody, cutoff, ag, pop, n_res, res_atom_off,
+                                  res_ca_atom, amino_acid));
+  DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, DNAB_EVAL_BREAKDOWN));
+  double t[6];
+  DNAB_RETURN_IF(dnab_population_read_terms(ctx, pop.p, t));
+  *energy = ((((t[0] + t[1]) + t[2]) + t[3]) + t[4]) + t[5];
+  if (disulfide) *disulfide = t[5];
   return DNAB_OK;
 }
 

@@ -169,6 +169,17 @@ int dnab_residue_cg_params(uint8_t amino_acid, float* charge, float* epsilon, fl
   return DNAB_OK;
 }
 
+int dnab_ff_type_params(uint8_t atom_type, float* dg_ref, float* volume, float* lambda,
+                        uint8_t* hbond_donor, uint8_t* hbond_acceptor) {
+  if (atom_type >= kNumAtomTypes) return DNAB_ERR_INVALID;
+  if (dg_ref) *dg_ref = kEef1[atom_type].dg_ref;
+  if (volume) *volume = kEef1[atom_type].vol;
+  if (lambda) *lambda = kEef1[atom_type].lambda;
+  if (hbond_donor) *hbond_donor = (kHbondDonorMask >> atom_type) & 1u;
+  if (hbond_acceptor) *hbond_acceptor = (kHbondAcceptorMask >> atom_type) & 1u;
+  return DNAB_OK;
+}
+
 int dnab_ff_table(uint8_t amino_acid, int slot, uint8_t* atom_type, float* charge,
                   uint8_t* hydrophobic, float* r_min_half, float* epsilon) {
   if (amino_acid >= kNumAminoAcids) return DNAB_ERR_INVALID;

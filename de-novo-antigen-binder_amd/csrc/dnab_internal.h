@@ -127,8 +127,13 @@ struct PairTuning {
 
 struct AgAtom {
   float x, y, z, q332;
-  float rmh, sqe, hyd, pad;
+  float rmh, sqe, hyd, type;  // type: AtomType as a float, -1 when not given
 };
+
+constexpr int kNumAtomTypesDev = 16;  // AtomType values (src/amber.rs:13-32)
+// Per-atom and per-candidate energy terms of a breakdown evaluation: LJ,
+// Coulomb, hydrophobic, H-bond, EEF1 solvation.
+constexpr int kBdTerms = 5;
 
 // Per-antigen constants read by the pair kernels.
 struct AgView {
@@ -217,7 +222,8 @@ struct dnab_population {
   std::vector<dnab_antigen*> antigens;
   std::vector<dnab::AgView> views_host;
   dnab::DevBuf<dnab::AgView> views;
-  // candidate atoms: pos = (x, y, z, q), par = (r_min_half, epsilon, hydrophobic, 0)
+  // candidate atoms: pos = (x, y, z, q), par = (r_min_half, epsilon, hydrophobic,
+  // AtomType as a float or -1)
   dnab::DevBuf<float4> pos, par;
   dnab::DevBuf<uint32_t> atom_ag;    // per atom
   dnab::DevBuf<uint32_t> atom_cand;  // per atom
@@ -229,6 +235,7 @@ struct dnab_population {
   // fused trajectory kernel: workgroup shape, grouping by measured work
   uint32_t traj_waves = 16, traj_cpw = 4, traj_cells = 2, traj_chunks = 2;
   bool traj_balance = true;
+  uint32_t traj_rebalance = 0;   // steps per launch between regroupings (0: one launch)
   dnab::DevBuf<uint32_t> traj_work, traj_group_off, traj_group_cand;
   std::vector<uint32_t> h_group_off, h_group_cand;  // sources of the group-table copies
   std::vector<float> h_sigma;                       // source of the noise-table copy
@@ -260,7 +267,8 @@ struct dnab_population {
   dnab::DevBuf<double> f64_atom;  // per atom FP64 oracle (fx, fy, fz, e)
   dnab::DevBuf<double> energy;    // per candidate
   dnab::DevBuf<float> err_bound;  // per candidate
-  dnab::DevBuf<double> breakdown; // per candidate (lj, coulomb, hydrophobic)
+  dnab::DevBuf<double> breakdown; // per candidate (lj, coulomb, hydrophobic, hbond, solvation)
+  dnab::DevBuf<double> disulfide; // per candidate, the binder's S-S term (breakdown runs)
   unsigned last_flags = 0;
   bool evaluated = false;
   // gradient-step state

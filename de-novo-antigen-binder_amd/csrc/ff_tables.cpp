@@ -23,6 +23,11 @@ const NbParams kAmberNb[kNumAtomTypes] = {
     {2.0000f, 0.2500f},  // SH
 };
 
+// EEF1 (dg_ref, volume, lambda) per AtomType, src/amber.rs:414-431.
+#define DNAB_EEF1_ROW(dg, vol, lam) {dg, vol, lam},
+const Eef1Params kEef1[kNumAtomTypes] = {DNAB_EEF1_ROWS(DNAB_EEF1_ROW)};
+#undef DNAB_EEF1_ROW
+
 namespace {
 
 // Heavy atoms per residue in topology order (backbone N, CA, C, O, CB, then

@@ -8,8 +8,11 @@
 //   backbone-independent rotamers     src/rotamer.rs:33-184
 //   side-chain NERF build recipes     src/rotamer.rs:288-430
 //   residue-level (Calpha) parameters src/atom.rs:55-104
+//   EEF1 solvation parameters         src/amber.rs:396-431 (numbers in ff_terms.h)
 #pragma once
 #include <cstdint>
+
+#include "ff_terms.h"
 
 namespace dnab {
 
@@ -62,7 +65,17 @@ struct BondDef {
   uint8_t chi_idx;
 };
 
+// EEF1 parameters per AtomType (Lazaridis & Karplus 1999), src/amber.rs:406-431.
+struct Eef1Params {
+  float dg_ref;  // reference solvation free energy, kcal/mol
+  float vol;     // volume, A^3
+  float lambda;  // correlation length, A
+};
+
 extern const NbParams kAmberNb[kNumAtomTypes];
+extern const Eef1Params kEef1[kNumAtomTypes];
+// Topology slot of the Cys SG atom (the disulfide term, src/allatom.rs:139-150).
+constexpr int kCysSgSlot = 5;
 extern const ResTopology kResidueTopology[kNumAminoAcids];
 extern const RotamerDef* const kRotamerLib[kNumAminoAcids];
 extern const uint8_t kRotamerCount[kNumAminoAcids];

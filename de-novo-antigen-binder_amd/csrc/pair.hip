@@ -37,6 +37,8 @@
 
 #include "device_common.h"
 #include "dnab_internal.h"
+#include "ff_tables.h"
+#include "ff_terms.h"
 #include "grid_device.h"
 #include "pair_device.h"
 #include "sort_device.h"
@@ -64,8 +66,8 @@ __global__ void k_ag_gather(const uint32_t* __restrict__ sorted, uint32_t n,
                             const float* __restrict__ x, const float* __restrict__ y,
                             const float* __restrict__ z, const float* __restrict__ q,
                             const float* __restrict__ rmh, const float* __restrict__ eps,
-                            const uint8_t* __restrict__ hyd, AgAtom* __restrict__ out,
-                            float2* __restrict__ raw) {
+                            const uint8_t* __restrict__ hyd, const uint8_t* __restrict__ type,
+                            AgAtom* __restrict__ out, float2* __restrict__ raw) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   uint32_t i = sorted[s];
@@ -77,7 +79,7 @@ __global__ void k_ag_gather(const uint32_t* __restrict__ sorted, uint32_t n,
   a.rmh = rmh[i];
   a.sqe = sqrtf(eps[i]);
   a.hyd = hyd[i] ? 1.0f : 0.0f;
-  a.pad = 0.0f;
+  a.type = type ? static_cast<float>(type[i]) : -1.0f;  // AtomType, -1 when not given
   out[s] = a;
   raw[s] = make_float2(q[i], eps[i]);
 }
@@ -199,6 +201,34 @@ __global__ void k_sort_scatter(const uint32_t* __restrict__ code_in,
 
 // ---- the pair tile kernel ---------------------------------------------------------
 
+#define DNAB_EEF1_DG(dg, vol, lam) dg,
+#define DNAB_EEF1_VOL(dg, vol, lam) vol,
+#define DNAB_EEF1_LAM(dg, vol, lam) lam,
+__constant__ float c_eef1_dg[kNumAtomTypesDev] = {DNAB_EEF1_ROWS(DNAB_EEF1_DG)};
+__constant__ float c_eef1_vol[kNumAtomTypesDev] = {DNAB_EEF1_ROWS(DNAB_EEF1_VOL)};
+__constant__ float c_eef1_lam[kNumAtomTypesDev] = {DNAB_EEF1_ROWS(DNAB_EEF1_LAM)};
+#undef DNAB_EEF1_DG
+#undef DNAB_EEF1_VOL
+#undef DNAB_EEF1_LAM
+
+// H-bond term of one pair (src/energy.rs:32-39), f32 in the reference's order.
+__device__ inline float hbond_term(float r2) {
+  const float r = sqrtf(r2);
+  const float z = (r - kHbR0) / kHbSigma;
+  return -kHbDepth * expf(-0.5f * z * z);
+}
+
+// EEF1 cross term of one pair (src/solvation.rs:34-44): i the antigen atom,
+// j the binder atom; powi(2) and powi(3) as repeated products.
+__device__ inline float solvation_term(uint32_t ti, uint32_t tj, float r2) {
+  const float r = sqrtf(r2);
+  const float li = c_eef1_lam[ti], lj = c_eef1_lam[tj];
+  const float ui = r / li, uj = r / lj;
+  const float fi = expf(-(ui * ui)) / (kPi32 * (li * li * li));
+  const float fj = expf(-(uj * uj)) / (kPi32 * (lj * lj * lj));
+  return -(c_eef1_dg[ti] * c_eef1_vol[tj] * fi + c_eef1_dg[tj] * c_eef1_vol[ti] * fj);
+}
+
 struct TileOut {
   float4* force;      // per atom (fx, fy, fz, sum |e|)
   double* e_atom;     // per atom energy
@@ -248,6 +278,8 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
   const float sqeb = sqrtf(Q.y);
   const float sqe12b = 12.0f * sqeb;
   const bool hydb = Q.z != 0.0f;
+  const bool typed_b = Q.w >= 0.0f;  // the binder atom's AtomType is known
+  const uint32_t tb = typed_b ? static_cast<uint32_t>(Q.w) : 0u;
 
   const float inf = __builtin_inff();
   const float lox = wave_reduce_f32<OpMin>(valid ? xb : inf);
@@ -270,6 +302,7 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
   float fx = 0.f, fy = 0.f, fz = 0.f;
   float es = 0.f, ec = 0.f, eabs = 0.f;        // TwoSum (sum, compensation), sum |e|
   float ljs = 0.f, ljc = 0.f, cos_ = 0.f, coc = 0.f;  // BREAKDOWN
+  float hbs = 0.f, hbc = 0.f, svs = 0.f, svc = 0.f;   // BREAKDOWN: H-bond, solvation
   uint32_t nh = 0;
   uint32_t n_in = 0, n_tested = 0;
   (void)bits;
@@ -327,6 +360,7 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
             A.rmh = readlane_f32(A1.x, k);
             A.sqe = readlane_f32(A1.y, k);
             A.hyd = readlane_f32(A1.z, k);
+            A.type = BREAKDOWN ? readlane_f32(A1.w, k) : -1.0f;
             // Pair predicate in f32, exactly the reference's (src/energy.rs:279-284).
             const float dx = xb - A.x;
             const float dy = yb - A.y;
@@ -370,6 +404,22 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
               two_sum(cos_, ecl, s, e);
               cos_ = s;
               coc += e;
+              if (A.type >= 0.0f && typed_b) {  // both atom types known
+                const uint32_t ta = static_cast<uint32_t>(A.type);
+                // H-bond inside the pair predicate (src/energy.rs:303-307)
+                const bool hb_types = (((kHbondDonorMask >> tb) & (kHbondAcceptorMask >> ta)) |
+                                       ((kHbondAcceptorMask >> tb) & (kHbondDonorMask >> ta))) & 1u;
+                const float ehb = (in && hb_types && r2 <= kHbCutoffSq) ? hbond_term(r2) : 0.0f;
+                two_sum(hbs, ehb, s, e);
+                hbs = s;
+                hbc += e;
+                // EEF1 with its own cutoff (src/solvation.rs:71-78)
+                const bool sv = valid && r2 >= kMinRSq && r2 < kSolvCutoffSq;
+                const float esv = sv ? solvation_term(ta, tb, r2) : 0.0f;
+                two_sum(svs, esv, s, e);
+                svs = s;
+                svc += e;
+              }
             } else {
               const float ep = elj + ecl;
               float s, e;
@@ -386,7 +436,7 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
 
   // Combine the four waves' partial sums in wave order.
   __shared__ float sh_f[kTileWaves][3][kWave];
-  __shared__ float sh_e[kTileWaves][4][kWave];
+  __shared__ float sh_e[kTileWaves][8][kWave];
   __shared__ uint32_t sh_nh[kTileWaves][kWave];
   if (FORCES) {
     sh_f[wave][0][lane] = fx;
@@ -398,6 +448,10 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
     sh_e[wave][1][lane] = ljc;
     sh_e[wave][2][lane] = cos_;
     sh_e[wave][3][lane] = coc;
+    sh_e[wave][4][lane] = hbs;
+    sh_e[wave][5][lane] = hbc;
+    sh_e[wave][6][lane] = svs;
+    sh_e[wave][7][lane] = svc;
   } else {
     sh_e[wave][0][lane] = es;
     sh_e[wave][1][lane] = ec;
@@ -425,15 +479,23 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
     }
   }
   if (BREAKDOWN) {
-    double lj = 0.0, co = 0.0;
+    double lj = 0.0, co = 0.0, hb = 0.0, sv = 0.0;
     for (int w = 0; w < kTileWaves; ++w) {
       lj += static_cast<double>(sh_e[w][0][lane]) + static_cast<double>(sh_e[w][1][lane]);
       co += static_cast<double>(sh_e[w][2][lane]) + static_cast<double>(sh_e[w][3][lane]);
+      hb += static_cast<double>(sh_e[w][4][lane]) +
+            static_cast<double>(sh_e[w][5][lane]);
+      sv += static_cast<double>(sh_e[w][6][lane]) +
+            static_cast<double>(sh_e[w][7][lane]);
     }
-    out.bd_atom[3 * static_cast<size_t>(j)] = lj;
-    out.bd_atom[3 * static_cast<size_t>(j) + 1] = co;
-    out.bd_atom[3 * static_cast<size_t>(j) + 2] = ehyd;
-    out.e_atom[j] = (lj + co) + ehyd;
+    double* bd = out.bd_atom + static_cast<size_t>(kBdTerms) * j;
+    bd[0] = lj;
+    bd[1] = co;
+    bd[2] = ehyd;
+    bd[3] = hb;
+    bd[4] = sv;
+    // the reference's total(): lj + coulomb + hydrophobic + hbond + solvation
+    out.e_atom[j] = (((lj + co) + ehyd) + hb) + sv;
   } else {
     float S = sh_e[0][0][lane], C = sh_e[0][1][lane], Aabs = sh_e[0][2][lane];
     for (int w = 1; w < kTileWaves; ++w) {
@@ -881,9 +943,12 @@ k_pair_tile_f64(const AgView* __restrict__ views, uint32_t n_ag,
   f64_atom[3 * jj] = acc[0];
   f64_atom[3 * jj + 1] = acc[1];
   f64_atom[3 * jj + 2] = acc[2];
-  bd_atom[3 * jj] = acc[3];
-  bd_atom[3 * jj + 1] = acc[4];
-  bd_atom[3 * jj + 2] = -0.5 * static_cast<double>(nh_t);
+  double* bd = bd_atom + static_cast<size_t>(kBdTerms) * jj;
+  bd[0] = acc[3];
+  bd[1] = acc[4];
+  bd[2] = -0.5 * static_cast<double>(nh_t);
+  bd[3] = 0.0;  // the FP64 kernel checks the three terms of the E+F path
+  bd[4] = 0.0;
   e_atom[jj] = (acc[3] + acc[4]) + (-0.5 * static_cast<double>(nh_t));
 }
 
@@ -929,20 +994,45 @@ k_reduce_breakdown(const uint32_t* __restrict__ cand_atom_off, uint32_t n_cand,
   const uint32_t lane = threadIdx.x & 63u;
   if (c >= n_cand) return;
   const uint32_t b = cand_atom_off[c], e = cand_atom_off[c + 1];
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (uint32_t k = b + lane; k < e; k += kWave) {
-    s0 += bd_atom[3 * static_cast<size_t>(k)];
-    s1 += bd_atom[3 * static_cast<size_t>(k) + 1];
-    s2 += bd_atom[3 * static_cast<size_t>(k) + 2];
+  double s[kBdTerms] = {0, 0, 0, 0, 0};
+  for (uint32_t k = b + lane; k < e; k += kWave)
+    for (int t = 0; t < kBdTerms; ++t) s[t] += bd_atom[static_cast<size_t>(kBdTerms) * k + t];
+  for (int t = 0; t < kBdTerms; ++t) s[t] = wave_sum_f64(s[t]);
+  if (lane == 0)
+    for (int t = 0; t < kBdTerms; ++t) bd[static_cast<size_t>(kBdTerms) * c + t] = s[t];
+}
+
+// disulfide_energy (src/allatom.rs:155-180) of each candidate: Gaussian wells
+// over the pairs of SG atoms (topology slot 5) of its Cys residues, pairs in
+// residue order, f32 in the reference's order. One thread per candidate: a
+// binder has few Cys. Without residue types the term is 0.
+__global__ void __launch_bounds__(256)
+k_disulfide(const uint32_t* __restrict__ cand_res_off, const uint32_t* __restrict__ res_atom_off,
+            const uint8_t* __restrict__ res_aa, const float4* __restrict__ pos, uint32_t n_cand,
+            double* __restrict__ out) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cand) return;
+  float e = 0.0f;
+  if (res_aa) {
+    const uint32_t r0 = cand_res_off[c], r1 = cand_res_off[c + 1];
+    auto has_sg = [&](uint32_t r) {
+      return res_aa[r] == kCys && res_atom_off[r + 1] - res_atom_off[r] > kCysSgSlot;
+    };
+    for (uint32_t ri = r0; ri < r1; ++ri) {
+      if (!has_sg(ri)) continue;
+      const float4 a = pos[res_atom_off[ri] + kCysSgSlot];
+      for (uint32_t rj = ri + 1; rj < r1; ++rj) {
+        if (!has_sg(rj)) continue;
+        const float4 b = pos[res_atom_off[rj] + kCysSgSlot];
+        const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (d > kSsCutoff) continue;
+        const float z = (d - kSsD0) / kSsSigma;
+        e -= kSsDepth * expf(-0.5f * z * z);
+      }
+    }
   }
-  s0 = wave_sum_f64(s0);
-  s1 = wave_sum_f64(s1);
-  s2 = wave_sum_f64(s2);
-  if (lane == 0) {
-    bd[3 * static_cast<size_t>(c)] = s0;
-    bd[3 * static_cast<size_t>(c) + 1] = s1;
-    bd[3 * static_cast<size_t>(c) + 2] = s2;
-  }
+  out[c] = static_cast<double>(e);
 }
 
 // ---- host side ------------------------------------------------------------------
@@ -1086,6 +1176,12 @@ int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned fla
       hipLaunchKernelGGL(k_reduce_breakdown, dim3(nblocks(nc, 4)), dim3(256), 0, ctx->stream,
                          pop->cand_atom_off.p, nc, pop->bd_atom.p, pop->breakdown.p);
     }
+    if (flags & DNAB_EVAL_BREAKDOWN) {
+      hipLaunchKernelGGL(k_disulfide, dim3(nblocks(nc, 256)), dim3(256), 0, ctx->stream,
+                         pop->cand_res_off.p, pop->res_atom_off.p,
+                         pop->has_res_aa ? pop->res_aa.p : nullptr, pop->pos.p, nc,
+                         pop->disulfide.p);
+    }
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   pop->last_flags = (flags & kEvalCount) ? (flags | DNAB_EVAL_FORCES) & ~kEvalCount : flags;
@@ -1193,11 +1289,16 @@ int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* d, dnab_antigen*
   g.cell = d->cell_size;
   g.inv = 1.0f / d->cell_size;
   DevBuf<float> q, rmh, eps;
-  DevBuf<uint8_t> hyd;
+  DevBuf<uint8_t> hyd, type;
+  if (c.atom_type)
+    for (size_t i = 0; i < n; ++i)
+      if (c.atom_type[i] >= kNumAtomTypesDev)
+        return fail(dnab_set_error(ctx, DNAB_ERR_INVALID, "antigen: bad atom type"));
   int rc = DNAB_OK;
   if ((rc = up(ctx, g.x, c.x, n)) || (rc = up(ctx, g.y, c.y, n)) || (rc = up(ctx, g.z, c.z, n)) ||
       (rc = up(ctx, q, c.charge, n)) || (rc = up(ctx, rmh, c.r_min_half, n)) ||
-      (rc = up(ctx, eps, c.epsilon, n)) || (rc = up(ctx, hyd, c.hydrophobic, n)))
+      (rc = up(ctx, eps, c.epsilon, n)) || (rc = up(ctx, hyd, c.hydrophobic, n)) ||
+      (c.atom_type && (rc = up(ctx, type, c.atom_type, n))))
     return fail(rc);
   if ((rc = grid_build_device(ctx, g, g.x.p, g.y.p, g.z.p, n))) return fail(rc);
   if (ag->atoms.alloc(n) != hipSuccess || ag->raw.alloc(n) != hipSuccess)
@@ -1205,7 +1306,7 @@ int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* d, dnab_antigen*
   if (n) {
     hipLaunchKernelGGL(k_ag_gather, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
                        g.sorted_idx.p, static_cast<uint32_t>(n), g.x.p, g.y.p, g.z.p, q.p, rmh.p,
-                       eps.p, hyd.p, ag->atoms.p, ag->raw.p);
+                       eps.p, hyd.p, c.atom_type ? type.p : nullptr, ag->atoms.p, ag->raw.p);
   }
   if (hipGetLastError() != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "antigen: gather launch failed"));
@@ -1301,8 +1402,11 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   std::vector<float4> hres_cg(nr);
   for (size_t i = 0; i < na; ++i) {
     hpos[i] = make_float4(d->atoms.x[i], d->atoms.y[i], d->atoms.z[i], d->atoms.charge[i]);
+    if (d->atoms.atom_type && d->atoms.atom_type[i] >= kNumAtomTypesDev)
+      return fail(dnab_set_error(ctx, DNAB_ERR_INVALID, "population: bad atom type"));
     hpar[i] = make_float4(d->atoms.r_min_half[i], d->atoms.epsilon[i],
-                          d->atoms.hydrophobic[i] ? 1.0f : 0.0f, 0.0f);
+                          d->atoms.hydrophobic[i] ? 1.0f : 0.0f,
+                          d->atoms.atom_type ? static_cast<float>(d->atoms.atom_type[i]) : -1.0f);
   }
   for (size_t c = 0; c < nc; ++c) {
     uint32_t r0 = d->cand_res_off[c], r1 = d->cand_res_off[c + 1];
@@ -1428,9 +1532,10 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
       pop->ho_chunks.alloc(pop->tile_cap * kHandoverCap) != hipSuccess ||
       pop->ho_part.alloc(5 * pop->tile_cap * kWave) != hipSuccess ||
       pop->ag_tile_start.alloc(n_antigens + 1) != hipSuccess || pop->force.alloc(na) != hipSuccess ||
-      pop->e_atom.alloc(na) != hipSuccess || pop->bd_atom.alloc(3 * na) != hipSuccess ||
+      pop->e_atom.alloc(na) != hipSuccess || pop->bd_atom.alloc(kBdTerms * na) != hipSuccess ||
       pop->f64_atom.alloc(3 * na) != hipSuccess || pop->energy.alloc(nc) != hipSuccess ||
-      pop->err_bound.alloc(nc) != hipSuccess || pop->breakdown.alloc(3 * nc) != hipSuccess ||
+      pop->err_bound.alloc(nc) != hipSuccess || pop->breakdown.alloc(kBdTerms * nc) != hipSuccess ||
+      pop->disulfide.alloc(nc) != hipSuccess ||
       pop->res_delta.alloc(nr) != hipSuccess || pop->step_counter.alloc(4) != hipSuccess ||
       pop->pair_stats.alloc(2) != hipSuccess || pop->cand_mask.alloc(nc) != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
@@ -1490,12 +1595,27 @@ int dnab_population_read_breakdown(dnab_ctx* ctx, const dnab_population* pop, do
   if (!ctx || !pop) return DNAB_ERR_INVALID;
   if (!pop->evaluated || !(pop->last_flags & (DNAB_EVAL_BREAKDOWN | DNAB_EVAL_FP64)))
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "no breakdown evaluation");
-  std::vector<double> h(3 * pop->n_cand);
+  std::vector<double> h(kBdTerms * pop->n_cand);
   DNAB_RETURN_IF(download(ctx, h.data(), pop->breakdown.p, h.size() * sizeof(double)));
   for (size_t c = 0; c < pop->n_cand; ++c) {
-    if (lj) lj[c] = h[3 * c];
-    if (coulomb) coulomb[c] = h[3 * c + 1];
-    if (hydrophobic) hydrophobic[c] = h[3 * c + 2];
+    if (lj) lj[c] = h[kBdTerms * c];
+    if (coulomb) coulomb[c] = h[kBdTerms * c + 1];
+    if (hydrophobic) hydrophobic[c] = h[kBdTerms * c + 2];
+  }
+  return DNAB_OK;
+}
+
+int dnab_population_read_terms(dnab_ctx* ctx, const dnab_population* pop, double* terms) {
+  if (!ctx || !pop || (pop->n_cand && !terms)) return DNAB_ERR_INVALID;
+  if (!pop->evaluated || !(pop->last_flags & DNAB_EVAL_BREAKDOWN))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "no breakdown evaluation");
+  const size_t nc = pop->n_cand;
+  std::vector<double> h(kBdTerms * nc), ss(nc);
+  DNAB_RETURN_IF(download(ctx, h.data(), pop->breakdown.p, h.size() * sizeof(double)));
+  DNAB_RETURN_IF(download(ctx, ss.data(), pop->disulfide.p, nc * sizeof(double)));
+  for (size_t c = 0; c < nc; ++c) {
+    for (int t = 0; t < kBdTerms; ++t) terms[6 * c + t] = h[kBdTerms * c + t];
+    terms[6 * c + 5] = ss[c];
   }
   return DNAB_OK;
 }

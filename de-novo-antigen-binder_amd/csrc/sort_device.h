@@ -33,8 +33,9 @@ __device__ inline void zero_atom_outputs(uint32_t j, float4* force, double* e_at
                                          double* bd_atom, double* f64_atom) {
   force[j] = make_float4(0.f, 0.f, 0.f, 0.f);
   e_atom[j] = 0.0;
+  const size_t kb = static_cast<size_t>(kBdTerms) * j;
+  for (int t = 0; t < kBdTerms; ++t) bd_atom[kb + t] = 0.0;
   const size_t k = 3 * static_cast<size_t>(j);
-  bd_atom[k] = bd_atom[k + 1] = bd_atom[k + 2] = 0.0;
   f64_atom[k] = f64_atom[k + 1] = f64_atom[k + 2] = 0.0;
 }
 

@@ -96,6 +96,9 @@ struct TrajArgs {
   const uint32_t* group_cand;
   uint32_t* work;   // optional: per candidate, antigen atoms taken into pair loops (all steps)
   uint32_t n_cand, n_steps, score_every;
+  uint32_t s0;              // loop index of this launch's first step
+  uint32_t n_total;         // steps of the whole loop (the final scoring's row)
+  int32_t final_eval;       // this launch ends the loop: evaluate the final state
   uint32_t n_groups;        // with a table
   int32_t cells;            // grid cells per work item (tiles whose chunk list overflowed)
   uint32_t chunks;          // listed chunks per work item (1..kLoadAhead)
@@ -253,8 +256,10 @@ k_trajectory(TrajArgs g) {
 
   uint32_t n_in = 0, n_tested = 0;
   for (uint32_t s = 0;; ++s) {
+    if (s == g.n_steps && !g.final_eval) break;  // a later launch continues the loop
+    const uint32_t it = g.s0 + s;                // the loop's step index
     // this step's noise amplitude, loaded early (used in phase 4)
-    const float sigma = (g.sp.with_noise && s < g.n_steps) ? g.sigma_table[s] : 0.0f;
+    const float sigma = (g.sp.with_noise && s < g.n_steps) ? g.sigma_table[it] : 0.0f;
     // 1. tile boxes, the chunks of the grid cells within the cutoff of
     //    each box (listed in LDS), and cleared accumulators
     for (uint32_t t = wave; t < T; t += WAVES) {
@@ -511,11 +516,11 @@ k_trajectory(TrajArgs g) {
     __syncthreads();
 
     if (s == g.n_steps) break;
-    if (g.score_every && s % g.score_every == 0) {
+    if (g.score_every && it % g.score_every == 0) {
       for (uint32_t j = wave; j < NC; j += WAVES) {
         double se, sa;
         candidate_sums(j, se, sa);
-        if (lane == 0) g.score_e[static_cast<size_t>(s / g.score_every) * g.n_cand + s_cand[j].id] = se;
+        if (lane == 0) g.score_e[static_cast<size_t>(it / g.score_every) * g.n_cand + s_cand[j].id] = se;
       }
     }
 
@@ -532,7 +537,7 @@ k_trajectory(TrajArgs g) {
         fz += f.z;
       }
       s_delta[r] = residue_delta(C.center[0], C.center[1], C.center[2], g.sp, C.r_restr,
-                                 s_pos[ri.y], fx, fy, fz, sigma, s, ri.z);
+                                 s_pos[ri.y], fx, fy, fz, sigma, it, ri.z);
     }
     __syncthreads();
 
@@ -550,16 +555,18 @@ k_trajectory(TrajArgs g) {
     __syncthreads();
   }
 
-  // the final state's candidate energies
-  const uint32_t n_mid = g.score_every ? (g.n_steps + g.score_every - 1) / g.score_every : 0;
+  // the final state's candidate energies (when this launch ends the loop)
+  const uint32_t n_mid = g.score_every ? (g.n_total + g.score_every - 1) / g.score_every : 0;
   for (uint32_t j = wave; j < NC; j += WAVES) {
-    double se, sa;
-    candidate_sums(j, se, sa);
+    double se = 0.0, sa = 0.0;
+    if (g.final_eval) candidate_sums(j, se, sa);
     if (lane == 0) {
       const uint32_t c = s_cand[j].id;
-      g.energy[c] = se;
-      g.err[c] = static_cast<float>(sa) * kTrajErrPerAbs;
-      g.score_e[static_cast<size_t>(n_mid) * g.n_cand + c] = se;
+      if (g.final_eval) {
+        g.energy[c] = se;
+        g.err[c] = static_cast<float>(sa) * kTrajErrPerAbs;
+        g.score_e[static_cast<size_t>(n_mid) * g.n_cand + c] = se;
+      }
       if (g.work) g.work[c] = s_work[j];
     }
   }
@@ -568,8 +575,10 @@ k_trajectory(TrajArgs g) {
     const uint32_t i = k - C.a_off;
     if (i < C.n) {
       g.pos[C.a0 + i] = s_pos[k];
-      g.force[C.a0 + i] = s_force[k];
-      g.e_atom[C.a0 + i] = s_e[k];
+      if (g.final_eval) {
+        g.force[C.a0 + i] = s_force[k];
+        g.e_atom[C.a0 + i] = s_e[k];
+      }
     }
   }
   if (COUNT) {
@@ -782,45 +791,71 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
   a.a_cap = pl.a_cap;
   a.r_cap = pl.r_cap;
   a.sp = sp;
+  a.s0 = 0;
+  a.n_total = p->n_steps;
+  a.final_eval = 1;
   const bool count = p->count_pairs != 0;
-  if (pop->traj_balance && pop->n_cand > 1 && p->n_steps > 0) {
-    // measure each candidate's work on the start state (one evaluation, whose
-    // outputs the run overwrites), then group candidates by it
-    const size_t nc = pop->n_cand;
-    DNAB_ALLOC_CHECK(ctx, pop->traj_work.ensure(nc));
-    TrajArgs e = a;
-    e.n_steps = 0;
-    e.score_every = 0;
-    e.work = pop->traj_work.p;
-    DNAB_HIP_CHECK(ctx, launch_plan(e, pl, false, ctx->stream));
-    std::vector<uint32_t> work(nc);
-    DNAB_RETURN_IF(download(ctx, work.data(), pop->traj_work.p, nc * sizeof(uint32_t)));
-    // kept in the population: the asynchronous copies read them after this returns
-    std::vector<uint32_t>& off = pop->h_group_off;
-    std::vector<uint32_t>& cand = pop->h_group_cand;
+  const size_t nc = pop->n_cand;
+  const bool balance = pop->traj_balance && nc > 1 && p->n_steps > 0;
+  std::vector<uint32_t> work;
+  // group candidates by the work in `work` (per candidate, staged antigen
+  // atoms); on failure the launch keeps consecutive groups
+  auto regroup = [&]() -> int {
+    std::vector<uint32_t>& off = pop->h_group_off;   // kept: the copies read them
+    std::vector<uint32_t>& cand = pop->h_group_cand; // after this returns
     TrajPlan bal;
     const uint32_t bins = std::max<uint32_t>(static_cast<uint32_t>(ctx->cu_count),
                                              static_cast<uint32_t>((nc + pop->traj_cpw - 1) / pop->traj_cpw));
-    if (traj_balance(pop, pop->traj_waves, pop->traj_cpw, bins, work, off, cand, bal)) {
-      DNAB_ALLOC_CHECK(ctx, pop->traj_group_off.ensure(off.size()));
-      DNAB_ALLOC_CHECK(ctx, pop->traj_group_cand.ensure(cand.size()));
-      DNAB_RETURN_IF(upload(ctx, pop->traj_group_off.p, off.data(), off.size() * sizeof(uint32_t)));
-      DNAB_RETURN_IF(upload(ctx, pop->traj_group_cand.p, cand.data(), cand.size() * sizeof(uint32_t)));
-      a.group_off = pop->traj_group_off.p;
-      a.group_cand = pop->traj_group_cand.p;
-      a.n_groups = static_cast<uint32_t>(off.size() - 1);
-      a.cpw = bal.cpw;
-      a.a_cap = bal.a_cap;
-      a.r_cap = bal.r_cap;
-      pl = bal;
-    }
+    if (!traj_balance(pop, pop->traj_waves, pop->traj_cpw, bins, work, off, cand, bal)) return DNAB_OK;
+    DNAB_ALLOC_CHECK(ctx, pop->traj_group_off.ensure(off.size()));
+    DNAB_ALLOC_CHECK(ctx, pop->traj_group_cand.ensure(cand.size()));
+    DNAB_RETURN_IF(upload(ctx, pop->traj_group_off.p, off.data(), off.size() * sizeof(uint32_t)));
+    DNAB_RETURN_IF(upload(ctx, pop->traj_group_cand.p, cand.data(), cand.size() * sizeof(uint32_t)));
+    a.group_off = pop->traj_group_off.p;
+    a.group_cand = pop->traj_group_cand.p;
+    a.n_groups = static_cast<uint32_t>(off.size() - 1);
+    a.cpw = bal.cpw;
+    a.a_cap = bal.a_cap;
+    a.r_cap = bal.r_cap;
+    pl = bal;
+    return DNAB_OK;
+  };
+  if (balance) {
+    // measure each candidate's work on the start state (one evaluation, whose
+    // outputs the run overwrites), then group candidates by it
+    DNAB_ALLOC_CHECK(ctx, pop->traj_work.ensure(nc));
+    work.resize(nc);
+    TrajArgs e = a;
+    e.n_steps = 0;
+    e.n_total = 0;
+    e.score_every = 0;
+    e.work = pop->traj_work.p;
+    DNAB_HIP_CHECK(ctx, launch_plan(e, pl, false, ctx->stream));
+    DNAB_RETURN_IF(download(ctx, work.data(), pop->traj_work.p, nc * sizeof(uint32_t)));
+    DNAB_RETURN_IF(regroup());
   }
-  hipEvent_t ev_end;
-  hipEvent_t ev = timer_begin(ctx, "trajectory", &ev_end);
-  hipError_t e = launch_plan(a, pl, count, ctx->stream);
-  timer_end(ctx, "trajectory", ev, ev_end);
-  if (e != hipSuccess)
-    return dnab_set_error(ctx, DNAB_ERR_HIP, "trajectory launch failed: %s", hipGetErrorString(e));
+  // the loop in segments of pop->traj_rebalance steps (all at once when 0);
+  // between segments the groups follow the work measured in the last one
+  const uint32_t seg = (balance && pop->traj_rebalance) ? pop->traj_rebalance : std::max<uint32_t>(p->n_steps, 1);
+  uint32_t done = 0;
+  do {
+    const uint32_t len = std::min(seg, p->n_steps - done);
+    a.s0 = done;
+    a.n_steps = len;
+    a.final_eval = done + len == p->n_steps ? 1 : 0;
+    a.work = balance && !a.final_eval ? pop->traj_work.p : nullptr;
+    hipEvent_t ev_end;
+    hipEvent_t ev = timer_begin(ctx, "trajectory", &ev_end);
+    hipError_t e = launch_plan(a, pl, count, ctx->stream);
+    timer_end(ctx, "trajectory", ev, ev_end);
+    if (e != hipSuccess)
+      return dnab_set_error(ctx, DNAB_ERR_HIP, "trajectory launch failed: %s", hipGetErrorString(e));
+    done += len;
+    if (a.work) {
+      DNAB_RETURN_IF(download(ctx, work.data(), pop->traj_work.p, nc * sizeof(uint32_t)));
+      DNAB_RETURN_IF(regroup());
+    }
+  } while (done < p->n_steps);
   return DNAB_OK;
 }
 
@@ -837,6 +872,7 @@ int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_
   t->balance = pop->traj_balance ? 1u : 0u;
   t->cells_per_item = pop->traj_cells;
   t->chunks_per_item = pop->traj_chunks;
+  t->rebalance_steps = pop->traj_rebalance;
   return DNAB_OK;
 }
 
@@ -854,6 +890,7 @@ int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "trajectory tuning: chunks_per_item must be 1..%d",
                           kLoadAhead);
   pop->traj_chunks = t->chunks_per_item;
+  pop->traj_rebalance = t->rebalance_steps;
   pop->traj_waves = t->waves;
   pop->traj_cells = t->cells_per_item;
   pop->traj_cpw = t->cand_per_group;

@@ -72,7 +72,8 @@ class PairTuningC(C.Structure):
 
 class TrajTuningC(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("cand_per_group", C.c_uint32), ("balance", C.c_uint32),
-                ("cells_per_item", C.c_uint32), ("chunks_per_item", C.c_uint32)]
+                ("cells_per_item", C.c_uint32), ("chunks_per_item", C.c_uint32),
+                ("rebalance_steps", C.c_uint32)]
 
 
 class ProteinOutC(C.Structure):
@@ -143,6 +144,7 @@ _PROTOS = {
     "dnab_residue_atom_count": (C.c_int, [C.c_uint8]),
     "dnab_residue_cg_params": (C.c_int, [C.c_uint8, f32p, f32p, f32p, u8p]),
     "dnab_ff_table": (C.c_int, [C.c_uint8, C.c_int, u8p, f32p, u8p, f32p, f32p]),
+    "dnab_ff_type_params": (C.c_int, [C.c_uint8, f32p, f32p, f32p, u8p, u8p]),
     "dnab_synth_globule_trace": (C.c_int, [C.c_size_t, C.c_float, C.c_uint64, f32p, f32p, f32p,
                                            u8p, u8p]),
     "dnab_synth_binder_ring": (C.c_int, [C.c_size_t, f32p, C.c_float, C.c_uint64, f32p, f32p,
@@ -216,6 +218,9 @@ _PROTOS = {
     "dnab_langevin_step": (C.c_int, [vp, vp, C.POINTER(LangevinParamsC), C.c_uint32]),
     "dnab_langevin_run": (C.c_int, [vp, vp, C.POINTER(LangevinParamsC), u32p, f64p, u32p]),
     "dnab_population_read_scores": (C.c_int, [vp, vp, u32p, u32p, u32p, f64p]),
+    "dnab_population_read_terms": (C.c_int, [vp, vp, f64p]),
+    "dnab_allatom_energy": (C.c_int, [vp, C.POINTER(AtomCloudC), C.POINTER(AtomCloudC), u32p,
+                                      u32p, u8p, C.c_size_t, C.c_float, f64p, f64p]),
 }
 for _name, (_res, _args) in _PROTOS.items():
     _fn = getattr(lib, _name)

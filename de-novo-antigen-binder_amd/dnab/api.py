@@ -127,6 +127,21 @@ class GpuContext:
                                                        C.byref(e)))
         return e.value
 
+    def allatom_energy(self, antigen, antibody: AtomProtein, cutoff: float = 10.0,
+                       with_disulfide: bool = False):
+        """allatom_energy (src/diffusion.rs:281-283): interaction_energy_atoms
+        plus the antibody's disulfide_energy. antigen: AtomProtein or AtomCloud."""
+        ag = antigen.atoms if isinstance(antigen, AtomProtein) else antigen
+        a, c = ag.c(), antibody.atoms.c()
+        rs = np.ascontiguousarray(antibody.res_start, np.uint32)
+        ca = np.ascontiguousarray(antibody.ca_atom_idx, np.uint32)
+        aa = np.ascontiguousarray(antibody.amino_acid, np.uint8)
+        e, ds = C.c_double(), C.c_double()
+        self.check(N.lib.dnab_allatom_energy(self._h, C.byref(a), C.byref(c), N.ptr(rs, C.c_uint32),
+                                             N.ptr(ca, C.c_uint32), N.ptr(aa, C.c_uint8),
+                                             len(aa), cutoff, C.byref(e), C.byref(ds)))
+        return (e.value, ds.value) if with_disulfide else e.value
+
     def atom_forces(self, antigen: AtomCloud, antibody: AtomCloud, cutoff: float = 10.0):
         n = len(antibody)
         fx, fy, fz = (np.zeros(max(n, 1), np.float32) for _ in range(3))
@@ -321,6 +336,14 @@ class DevicePopulation:
                                                            N.ptr(err, C.c_float)))
         return e[:self.n_cand], err[:self.n_cand]
 
+    def terms(self):
+        """(n_cand, 6) of the last breakdown evaluation: LJ, Coulomb,
+        hydrophobic, H-bond, solvation, disulfide."""
+        out = np.zeros((max(self.n_cand, 1), 6), np.float64)
+        self.ctx.check(N.lib.dnab_population_read_terms(self.ctx.handle, self._h,
+                                                        N.ptr(out, C.c_double)))
+        return out[:self.n_cand]
+
     def breakdown(self):
         out = [np.zeros(max(self.n_cand, 1), np.float64) for _ in range(3)]
         self.ctx.check(N.lib.dnab_population_read_breakdown(
@@ -366,11 +389,14 @@ class DevicePopulation:
         t = N.TrajTuningC()
         N.check(N.lib.dnab_population_get_trajectory_tuning(self._h, C.byref(t)))
         return {"waves": t.waves, "cand_per_group": t.cand_per_group, "balance": t.balance,
-                "cells_per_item": t.cells_per_item, "chunks_per_item": t.chunks_per_item}
+                "cells_per_item": t.cells_per_item, "chunks_per_item": t.chunks_per_item,
+                "rebalance_steps": t.rebalance_steps}
 
     def set_trajectory_tuning(self, waves: int, cand_per_group: int, balance: int = 1,
-                              cells_per_item: int = 2, chunks_per_item: int = 2):
-        t = N.TrajTuningC(waves, cand_per_group, balance, cells_per_item, chunks_per_item)
+                              cells_per_item: int = 2, chunks_per_item: int = 2,
+                              rebalance_steps: int = 0):
+        t = N.TrajTuningC(waves, cand_per_group, balance, cells_per_item, chunks_per_item,
+                          rebalance_steps)
         self.ctx.check(N.lib.dnab_population_set_trajectory_tuning(self.ctx.handle, self._h,
                                                                    C.byref(t)))
 

@@ -11,6 +11,9 @@
 //   binder::GpuContext::score_batch        src/gpu.rs:166-267
 //   binder::interaction_energy_atoms[_breakdown], compute_forces[_with_grid]
 //                                          src/energy.rs:154-398
+//   binder::allatom_energy, disulfide_energy
+//                                          src/diffusion.rs:281-283,
+//                                          src/allatom.rs:155-180
 //   binder::run_allatom_gpu_phase          src/diffusion.rs:1008-1062 (GPU phase,
 //                                          gradient steps + scoring, no MC moves)
 //
@@ -220,8 +223,9 @@ inline SpatialHashGrid::SpatialHashGrid(const GpuContext& ctx, float cell_size)
   check(dnab_grid_create(ctx_, cell_size, &g_), ctx_);
 }
 
-// interaction_energy_atoms_breakdown (src/energy.rs:261-314): LJ, Coulomb and
-// hydrophobic terms (hbond, solvation = 0) at the reference cutoff of 10 A.
+// interaction_energy_atoms_breakdown (src/energy.rs:261-314): LJ, Coulomb,
+// hydrophobic, and H-bond and EEF1 solvation when both clouds carry atom
+// types, at the reference cutoff of 10 A.
 inline AtomEnergyBreakdown interaction_energy_atoms_breakdown(const GpuContext& gpu,
                                                               const AtomCloud& antigen,
                                                               const AtomCloud& antibody,
@@ -239,6 +243,27 @@ inline double interaction_energy_atoms(const GpuContext& gpu, const AtomCloud& a
   double e = 0;
   check(dnab_interaction_energy_atoms(gpu.raw(), &a, &b, cutoff, &e), gpu.raw());
   return e;
+}
+
+// allatom_energy (src/diffusion.rs:281-283): interaction_energy_atoms plus the
+// antibody's disulfide_energy (src/allatom.rs:155-180).
+inline double allatom_energy(const GpuContext& gpu, const AtomProtein& ag, const AtomProtein& ab,
+                             double* disulfide = nullptr, float cutoff = 10.0f) {
+  dnab_atom_cloud a = ag.atoms.view(), b = ab.atoms.view();
+  std::vector<uint8_t> aa(ab.amino_acid.size());
+  for (size_t i = 0; i < aa.size(); ++i) aa[i] = static_cast<uint8_t>(ab.amino_acid[i]);
+  double e = 0;
+  check(dnab_allatom_energy(gpu.raw(), &a, &b, ab.res_start.data(), ab.ca_atom_idx.data(),
+                            aa.data(), aa.size(), cutoff, &e, disulfide), gpu.raw());
+  return e;
+}
+
+// AtomProtein::disulfide_energy (src/allatom.rs:155-180), evaluated on the
+// device against an empty antigen.
+inline double disulfide_energy(const GpuContext& gpu, const AtomProtein& p) {
+  double ds = 0;
+  allatom_energy(gpu, AtomProtein{}, p, &ds);
+  return ds;
 }
 
 // compute_forces_with_grid (src/energy.rs:322-398): outputs sized >= n_ab.

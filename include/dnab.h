@@ -75,8 +75,8 @@ typedef struct dnab_residue_cloud {
   const uint8_t* hydrophobic;
 } dnab_residue_cloud;
 
-/* AtomEnergyBreakdown (src/energy.rs:238-250). The GPU path computes the LJ,
- * Coulomb and hydrophobic terms; hbond and solvation are returned as 0. */
+/* AtomEnergyBreakdown (src/energy.rs:238-250). hbond and solvation need the
+ * atom types of both clouds (atom_type); without them they are 0. */
 typedef struct dnab_energy_breakdown {
   double lj, coulomb, hydrophobic, hbond, solvation;
 } dnab_energy_breakdown;
@@ -131,6 +131,10 @@ int dnab_residue_cg_params(uint8_t amino_acid, float* charge, float* epsilon, fl
  * (src/amber.rs:46-63). */
 int dnab_ff_table(uint8_t amino_acid, int slot, uint8_t* atom_type, float* charge,
                   uint8_t* hydrophobic, float* r_min_half, float* epsilon);
+/* Per AtomType (0..15): EEF1 (dg_ref, volume, lambda), src/amber.rs:406-431,
+ * and the H-bond donor / acceptor classes, src/amber.rs:72-84. */
+int dnab_ff_type_params(uint8_t atom_type, float* dg_ref, float* volume, float* lambda,
+                        uint8_t* hbond_donor, uint8_t* hbond_acceptor);
 /* Synthetic Calpha traces (no reference counterpart except the ring layout,
  * which follows random_allatom_antibody, src/diffusion.rs:264-277). */
 int dnab_synth_globule_trace(size_t n_res, float radius, uint64_t seed, float* x, float* y,
@@ -185,16 +189,28 @@ int dnab_cg_forces(dnab_ctx* ctx, const dnab_residue_cloud* antigen,
                    const dnab_residue_cloud* antibody, float* fx, float* fy, float* fz);
 
 /* ---- all-atom energies and forces (host buffers) ---------------------------
- * interaction_energy_atoms_breakdown (src/energy.rs:261-314), LJ + Coulomb +
- * hydrophobic terms with the reference pair predicate 0.25 <= r^2 <= cutoff^2
- * (cutoff 10 in the reference, src/energy.rs:16). */
+ * interaction_energy_atoms_breakdown (src/energy.rs:261-314): LJ + Coulomb +
+ * hydrophobic + H-bond with the reference pair predicate 0.25 <= r^2 <=
+ * cutoff^2 (cutoff 10 in the reference, src/energy.rs:16), and EEF1 solvation
+ * over 0.25 <= r^2 < 81 (src/solvation.rs:52-84, counted within the cutoff).
+ * H-bond and solvation need both clouds' atom_type. */
 int dnab_interaction_energy_atoms_breakdown(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
                           const dnab_atom_cloud* antibody, float cutoff,
                           dnab_energy_breakdown* out);
-/* interaction_energy_atoms (src/energy.rs:256-258), same three terms. */
+/* interaction_energy_atoms (src/energy.rs:256-258), the total of those terms. */
 int dnab_interaction_energy_atoms(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
                                   const dnab_atom_cloud* antibody, float cutoff,
                                   double* energy);
+/* allatom_energy (src/diffusion.rs:281-283), the score run_allatom ranks its
+ * refined candidates by (src/diffusion.rs:1092): interaction_energy_atoms plus
+ * the antibody's disulfide_energy (src/allatom.rs:155-180) over its Cys SG
+ * atoms (slot 5 of each Cys residue). The antibody is given as residues:
+ * res_atom_off (n_res + 1, spanning 0..antibody->n), res_ca_atom (n_res) and
+ * amino_acid (n_res). disulfide (nullable) receives the disulfide term. */
+int dnab_allatom_energy(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
+                        const dnab_atom_cloud* antibody, const uint32_t* res_atom_off,
+                        const uint32_t* res_ca_atom, const uint8_t* amino_acid, size_t n_res,
+                        float cutoff, double* energy, double* disulfide);
 /* All-atom AMBER forces on the antibody atoms (no reference counterpart: the
  * reference has residue-level forces only, src/energy.rs:154-234). Force on
  * atom j is sum_i [12 eps (s^12 - s^6)/r^2 + 332 q_i q_j / r^3 + f_h] (p_j -
@@ -257,7 +273,14 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
 void dnab_population_destroy(dnab_population* pop);
 
 #define DNAB_EVAL_FORCES 1u    /* also compute per-atom forces */
-#define DNAB_EVAL_BREAKDOWN 2u /* keep LJ / Coulomb / hydrophobic separately */
+#define DNAB_EVAL_BREAKDOWN 2u /* energy-only, the reference's five terms kept
+                                  separately: LJ, Coulomb, hydrophobic, and, when
+                                  both sides carry atom types, H-bond and EEF1
+                                  solvation (src/energy.rs:261-314,
+                                  src/solvation.rs:52-84); the candidate energy
+                                  is their total; plus the binder's disulfide
+                                  term (src/allatom.rs:155-180) when residue
+                                  types are given */
 #define DNAB_EVAL_FP64 4u      /* all-FP64 oracle kernel instead of FP32 */
 /* One evaluation of every candidate against its antigen: per-candidate
  * energies (FP32 pair terms, TwoSum-compensated per atom, FP64 total) and,
@@ -267,6 +290,11 @@ int dnab_population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags
  * a-priori bound on |E - E_fp64| from the FP32 pair terms. */
 int dnab_population_read_energies(dnab_ctx* ctx, const dnab_population* pop, double* energy,
                                   float* err_bound);
+/* Per candidate, 6 values of the last DNAB_EVAL_BREAKDOWN run: LJ, Coulomb,
+ * hydrophobic, H-bond, solvation (their sum is the candidate energy,
+ * interaction_energy_atoms) and the binder's disulfide term; allatom_energy
+ * (src/diffusion.rs:281-283) is the sum of all six. */
+int dnab_population_read_terms(dnab_ctx* ctx, const dnab_population* pop, double* terms);
 /* Per-candidate LJ, Coulomb, hydrophobic of the last DNAB_EVAL_BREAKDOWN run. */
 int dnab_population_read_breakdown(dnab_ctx* ctx, const dnab_population* pop, double* lj,
                                    double* coulomb, double* hydrophobic);
@@ -372,7 +400,8 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
  * of one grid cell) of the cells within the cutoff of the tile's box; a work
  * item is a run of `chunks_per_item` of them (or of `cells_per_item` cells
  * when the list overflows). Default 16 waves, 4 candidates, balance on,
- * 2 cells, 2 chunks. Results do not depend on it. */
+ * 2 cells, 2 chunks, one launch (no regrouping). Results do not depend on
+ * it. */
 typedef struct dnab_traj_tuning {
   uint32_t waves;
   uint32_t cand_per_group;
@@ -380,6 +409,9 @@ typedef struct dnab_traj_tuning {
   uint32_t cells_per_item;  /* grid cells per work item, for a tile whose
                                chunk list overflowed (1..64) */
   uint32_t chunks_per_item; /* listed chunks per work item (1..4) */
+  uint32_t rebalance_steps; /* with balance: the loop runs in launches of this
+                               many steps, regrouping between them by the work
+                               each candidate had in the last (0: one launch) */
 } dnab_traj_tuning;
 int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_tuning* t);
 int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,

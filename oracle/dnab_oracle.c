@@ -19,6 +19,17 @@
  *                               forces (negative gradient of LJ + Coulomb, the
  *                               hydrophobic pull of src/energy.rs:382-386)
  *   or_population_f64           or_energy_f64 over a ragged batch, threaded
+ *   or_hbond_energy_f32/_f64    hbond_energy                  src/energy.rs:32-39
+ *   or_pair_solvation_f32/_f64  pair_solvation (EEF1)         src/solvation.rs:34-44
+ *   or_energy_full_f32          interaction_energy_atoms_breakdown, all five terms
+ *                               (LJ, Coulomb, hydrophobic, H-bond in the j-i loop,
+ *                               solvation_interaction after it) in f32 and the
+ *                               reference's order   src/energy.rs:261-314,
+ *                                                   src/solvation.rs:52-84
+ *   or_energy_full_f64          the same pair sets (f32 predicates), every term and
+ *                               sum in double
+ *   or_disulfide_f32/_f64       disulfide_energy over Cys SG positions
+ *                                                   src/allatom.rs:155-180
  *   or_cg_forces(_with_grid)    compute_forces(_with_grid)    src/energy.rs:154-234, 322-398
  *   or_langevin_step            allatom_diffusion_step lines  src/diffusion.rs:300-339
  *                               with set_ca_pos               src/allatom.rs:279-293
@@ -200,6 +211,166 @@ void or_energy_f64(const or_cloud* ag, const or_cloud* ab, float cutoff, double 
   out[1] = co;
   out[2] = hy;
   if (n_pairs) *n_pairs = np;
+}
+
+/* ---- the other two interaction terms and the disulfide term ---------------- */
+
+#define HB_R0 2.9f         /* src/energy.rs:24-27 */
+#define HB_SIGMA 0.35f
+#define HB_DEPTH 1.5f
+#define HB_CUTOFF_SQ 25.0f
+#define PI32 5.568328f     /* pi^(3/2), src/solvation.rs:22 */
+#define SOLV_CUTOFF_SQ 81.0f  /* (9 A)^2, src/solvation.rs:25-26 */
+
+/* Per-atom EEF1 parameters and H-bond classes (src/amber.rs:72-84, 414-431),
+ * passed in by the caller (the tests take them from the reference's tables). */
+typedef struct {
+  const float *dg, *vol, *lam;
+  const uint8_t *don, *acc;
+} or_ext;
+
+float or_hbond_energy_f32(float r_sq) {
+  if (r_sq > HB_CUTOFF_SQ) return 0.0f;
+  float r = sqrtf(r_sq);
+  float z = (r - HB_R0) / HB_SIGMA;
+  return -HB_DEPTH * expf(-0.5f * z * z);
+}
+
+/* the double oracles take the reference's f32 constants, widened */
+double or_hbond_energy_f64(double r_sq) {
+  double r = sqrt(r_sq);
+  double z = (r - (double)HB_R0) / (double)HB_SIGMA;
+  return -(double)HB_DEPTH * exp(-0.5 * z * z);
+}
+
+/* powi(2) and powi(3) as repeated products, in the reference's order */
+float or_pair_solvation_f32(float dgi, float voli, float lami, float dgj, float volj, float lamj,
+                            float r_sq) {
+  float r = sqrtf(r_sq);
+  float ti = r / lami, tj = r / lamj;
+  float fi = expf(-(ti * ti)) / (PI32 * (lami * lami * lami));
+  float fj = expf(-(tj * tj)) / (PI32 * (lamj * lamj * lamj));
+  return -(dgi * volj * fi + dgj * voli * fj);
+}
+
+double or_pair_solvation_f64(double dgi, double voli, double lami, double dgj, double volj,
+                             double lamj, double r_sq) {
+  const double pi32 = (double)PI32;
+  double r = sqrt(r_sq);
+  double fi = exp(-(r / lami) * (r / lami)) / (pi32 * lami * lami * lami);
+  double fj = exp(-(r / lamj) * (r / lamj)) / (pi32 * lamj * lamj * lamj);
+  return -(dgi * volj * fi + dgj * voli * fj);
+}
+
+static int hb_pair(const or_ext* ga, size_t i, const or_ext* gb, size_t j) {
+  return (gb->don[j] && ga->acc[i]) || (gb->acc[j] && ga->don[i]);
+}
+
+/* All five terms in f32, the reference's order: the j-i loop with four
+ * accumulators, then the solvation loop summed per antibody atom. out =
+ * (lj, coulomb, hydrophobic, hbond, solvation). */
+void or_energy_full_f32(const or_cloud* ag, const or_cloud* ab, const or_ext* ga,
+                        const or_ext* gb, float cutoff, float out[5]) {
+  float three[3];
+  or_energy_f32(ag, ab, cutoff, three);
+  const float cut2 = cutoff * cutoff;
+  float hb = 0.0f, solv = 0.0f;
+  for (size_t j = 0; j < ab->n; ++j) {
+    for (size_t i = 0; i < ag->n; ++i) {
+      float dx = ab->x[j] - ag->x[i];
+      float dy = ab->y[j] - ag->y[i];
+      float dz = ab->z[j] - ag->z[i];
+      float r_sq = dx * dx + dy * dy + dz * dz;
+      if (r_sq > cut2 || r_sq < MIN_R_SQ) continue;
+      if (hb_pair(ga, i, gb, j)) hb += or_hbond_energy_f32(r_sq);
+    }
+  }
+  for (size_t j = 0; j < ab->n; ++j) {
+    float local = 0.0f;
+    for (size_t i = 0; i < ag->n; ++i) {
+      float dx = ab->x[j] - ag->x[i];
+      float dy = ab->y[j] - ag->y[i];
+      float dz = ab->z[j] - ag->z[i];
+      float r_sq = dx * dx + dy * dy + dz * dz;
+      if (r_sq >= SOLV_CUTOFF_SQ || r_sq < MIN_R_SQ) continue;
+      local += or_pair_solvation_f32(ga->dg[i], ga->vol[i], ga->lam[i], gb->dg[j], gb->vol[j],
+                                     gb->lam[j], r_sq);
+    }
+    solv += local;
+  }
+  out[0] = three[0];
+  out[1] = three[1];
+  out[2] = three[2];
+  out[3] = hb;
+  out[4] = solv;
+}
+
+/* The same pair sets (f32 predicates), terms and sums in double; per
+ * antibody atom into e5[5 * j .. 5 * j + 4] when e5 is given. */
+void or_energy_full_f64(const or_cloud* ag, const or_cloud* ab, const or_ext* ga,
+                        const or_ext* gb, float cutoff, double out[5], double* e5) {
+  const float cut2 = cutoff * cutoff;
+  double tot[5] = {0, 0, 0, 0, 0};
+  for (size_t j = 0; j < ab->n; ++j) {
+    double a[6], hb = 0.0, solv = 0.0;
+    atom_f64(ag, ab, j, cut2, a, NULL);
+    for (size_t i = 0; i < ag->n; ++i) {
+      float dx = ab->x[j] - ag->x[i];
+      float dy = ab->y[j] - ag->y[i];
+      float dz = ab->z[j] - ag->z[i];
+      float r_sq = dx * dx + dy * dy + dz * dz;
+      if (r_sq < MIN_R_SQ) continue;
+      double ddx = (double)ab->x[j] - (double)ag->x[i];
+      double ddy = (double)ab->y[j] - (double)ag->y[i];
+      double ddz = (double)ab->z[j] - (double)ag->z[i];
+      double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      if (r_sq <= cut2 && r_sq <= HB_CUTOFF_SQ && hb_pair(ga, i, gb, j))
+        hb += or_hbond_energy_f64(r2);
+      if (r_sq < SOLV_CUTOFF_SQ)
+        solv += or_pair_solvation_f64(ga->dg[i], ga->vol[i], ga->lam[i], gb->dg[j], gb->vol[j],
+                                      gb->lam[j], r2);
+    }
+    const double t[5] = {a[0], a[1], a[2], hb, solv};
+    for (int k = 0; k < 5; ++k) {
+      tot[k] += t[k];
+      if (e5) e5[5 * j + k] = t[k];
+    }
+  }
+  for (int k = 0; k < 5; ++k) out[k] = tot[k];
+}
+
+/* disulfide_energy over the given SG positions (src/allatom.rs:155-180):
+ * pairs i < j in order, d = sqrt(dx*dx + dy*dy + dz*dz), skipped beyond 4 A,
+ * e -= 4 exp(-z^2 / 2), z = (d - 2.05) / 0.4. */
+float or_disulfide_f32(const float* x, const float* y, const float* z, size_t n) {
+  float e = 0.0f;
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = i + 1; j < n; ++j) {
+      float dx = x[i] - x[j];
+      float dy = y[i] - y[j];
+      float dz = z[i] - z[j];
+      float d = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (d > 4.0f) continue;
+      float t = (d - 2.05f) / 0.4f;
+      e -= 4.0f * expf(-0.5f * t * t);
+    }
+  return e;
+}
+
+double or_disulfide_f64(const float* x, const float* y, const float* z, size_t n) {
+  double e = 0.0;
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = i + 1; j < n; ++j) {
+      float dx = x[i] - x[j];
+      float dy = y[i] - y[j];
+      float dz = z[i] - z[j];
+      float d32 = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (d32 > 4.0f) continue;  /* the reference's f32 cutoff decides the pair set */
+      double ddx = (double)x[i] - x[j], ddy = (double)y[i] - y[j], ddz = (double)z[i] - z[j];
+      double t = (sqrt(ddx * ddx + ddy * ddy + ddz * ddz) - (double)2.05f) / (double)0.4f;
+      e -= 4.0 * exp(-0.5 * t * t);
+    }
+  return e;
 }
 
 /* ---- threaded batch helpers ----------------------------------------------- */

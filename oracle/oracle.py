@@ -36,6 +36,11 @@ class OrRCloud(C.Structure):
                 ("eps", f32p), ("sig", f32p), ("hyd", u8p)]
 
 
+class OrExt(C.Structure):
+    """Per-atom EEF1 parameters and H-bond classes."""
+    _fields_ = [("dg", f32p), ("vol", f32p), ("lam", f32p), ("don", u8p), ("acc", u8p)]
+
+
 class OrStepParams(C.Structure):
     _fields_ = [("step", C.c_float), ("max_disp", C.c_float), ("restraint_k", C.c_float),
                 ("restraint_radius", C.c_float), ("sigma", C.c_float), ("with_noise", C.c_int),
@@ -66,6 +71,16 @@ def _load():
         "or_cg_forces": (None, [C.POINTER(OrRCloud), C.POINTER(OrRCloud), f32p, f32p, f32p]),
         "or_cg_forces_with_grid": (C.c_int, [C.POINTER(OrRCloud), C.c_float,
                                              C.POINTER(OrRCloud), f32p, f32p, f32p]),
+        "or_hbond_energy_f32": (C.c_float, [C.c_float]),
+        "or_hbond_energy_f64": (C.c_double, [C.c_double]),
+        "or_pair_solvation_f32": (C.c_float, [C.c_float] * 7),
+        "or_pair_solvation_f64": (C.c_double, [C.c_double] * 7),
+        "or_energy_full_f32": (None, [C.POINTER(OrCloud), C.POINTER(OrCloud), C.POINTER(OrExt),
+                                      C.POINTER(OrExt), C.c_float, f32p]),
+        "or_energy_full_f64": (None, [C.POINTER(OrCloud), C.POINTER(OrCloud), C.POINTER(OrExt),
+                                      C.POINTER(OrExt), C.c_float, f64p, f64p]),
+        "or_disulfide_f32": (C.c_float, [f32p, f32p, f32p, C.c_size_t]),
+        "or_disulfide_f64": (C.c_double, [f32p, f32p, f32p, C.c_size_t]),
         "or_philox4x32_10": (None, [u32p, u32p, u32p]),
         "or_sigma": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]),
         "or_langevin_step": (None, [f32p, f32p, f32p, C.c_size_t, u32p, u32p, f32p, f32p, f32p,
@@ -169,6 +184,97 @@ def population_f64(ag, atoms, cand_atom_off, cutoff=10.0, forces=True, nthreads=
     if forces:
         res["forces"] = tuple(v[:na] for v in f)
     return res
+
+
+# ---- H-bond, EEF1 solvation, disulfide (the reference's other energy terms) ----
+
+def reference_tables():
+    """The reference's force-field tables as recorded in the committed fixture
+    tests/golden/ff_tables.json (EEF1 rows and H-bond classes per AtomType)."""
+    import json
+    return json.loads((HERE.parent / "tests" / "golden" / "ff_tables.json").read_text())
+
+
+def ext(types) -> tuple[OrExt, list]:
+    """Per-atom EEF1 parameters and H-bond classes from atom types."""
+    tab = reference_tables()
+    t = np.asarray(types, np.int64)
+    eef = np.asarray(tab["eef1"], np.float32).reshape(-1, 3)
+    don = np.asarray(tab["hbond_donor"], np.uint8)
+    acc = np.asarray(tab["hbond_acceptor"], np.uint8)
+    keep = [np.ascontiguousarray(eef[t, 0]), np.ascontiguousarray(eef[t, 1]),
+            np.ascontiguousarray(eef[t, 2]), np.ascontiguousarray(don[t]),
+            np.ascontiguousarray(acc[t])]
+    oe = OrExt(*[_p(k, C.c_float) for k in keep[:3]], _p(keep[3], C.c_uint8),
+               _p(keep[4], C.c_uint8))
+    return oe, keep
+
+
+def hbond_energy_f32(r_sq):
+    return float(np.float32(lib.or_hbond_energy_f32(float(r_sq))))
+
+
+def hbond_energy_f64(r_sq):
+    return lib.or_hbond_energy_f64(float(r_sq))
+
+
+def pair_solvation_f32(pi, pj, r_sq):
+    """pi, pj: (dg_ref, vol, lambda) of the two atoms."""
+    return float(np.float32(lib.or_pair_solvation_f32(*map(float, pi), *map(float, pj),
+                                                      float(r_sq))))
+
+
+def pair_solvation_f64(pi, pj, r_sq):
+    return lib.or_pair_solvation_f64(*map(float, pi), *map(float, pj), float(r_sq))
+
+
+def energy_full_f32(ag, ab, cutoff=10.0):
+    """(lj, coulomb, hydrophobic, hbond, solvation) in f32, the reference's order."""
+    a, ka = cloud(ag)
+    b, kb = cloud(ab)
+    ea, kea = ext(ag.atom_type)
+    eb, keb = ext(ab.atom_type)
+    out = np.zeros(5, np.float32)
+    lib.or_energy_full_f32(C.byref(a), C.byref(b), C.byref(ea), C.byref(eb), cutoff,
+                           _p(out, C.c_float))
+    return out
+
+
+def energy_full_f64(ag, ab, cutoff=10.0):
+    """FP64 (lj, coulomb, hydrophobic, hbond, solvation) and the per-atom (n, 5) terms."""
+    a, ka = cloud(ag)
+    b, kb = cloud(ab)
+    ea, kea = ext(ag.atom_type)
+    eb, keb = ext(ab.atom_type)
+    n = len(kb[0])
+    out = np.zeros(5, np.float64)
+    per = np.zeros((max(n, 1), 5), np.float64)
+    lib.or_energy_full_f64(C.byref(a), C.byref(b), C.byref(ea), C.byref(eb), cutoff,
+                           _p(out, C.c_double), _p(per, C.c_double))
+    return out, per[:n]
+
+
+def disulfide_f32(x, y, z):
+    x, y, z = _f32(x), _f32(y), _f32(z)
+    return float(np.float32(lib.or_disulfide_f32(_p(x, C.c_float), _p(y, C.c_float),
+                                                 _p(z, C.c_float), len(x))))
+
+
+def disulfide_f64(x, y, z):
+    x, y, z = _f32(x), _f32(y), _f32(z)
+    return lib.or_disulfide_f64(_p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), len(x))
+
+
+CYS, CYS_SG_SLOT = 4, 5   # AminoAcid::Cys; SG's slot in the Cys topology (src/amber.rs)
+
+
+def sg_positions(x, y, z, res_atom_off, res_amino_acid, r0, r1):
+    """SG atoms of the Cys residues r0..r1 in residue order (sg_pos,
+    src/allatom.rs:139-150)."""
+    sel = [res_atom_off[r] + CYS_SG_SLOT for r in range(r0, r1)
+           if res_amino_acid[r] == CYS and res_atom_off[r + 1] - res_atom_off[r] > CYS_SG_SLOT]
+    sel = np.asarray(sel, np.int64)
+    return np.asarray(x)[sel], np.asarray(y)[sel], np.asarray(z)[sel]
 
 
 def baseline_energy_brute(ag, atoms, cand_atom_off, cutoff, nthreads):

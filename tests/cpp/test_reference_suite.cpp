@@ -2,6 +2,7 @@
 // the binder:: host interface (de-novo-antigen-binder_amd/host/binder.hpp)
 // and run on the GPU. Each test names the reference test it restates.
 // Exit status 0 when every check passes.
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <functional>
@@ -107,7 +108,58 @@ int main() {
     CHECK(protein_from_ca_trace({0.0f}, {0.0f}, {0.0f}, {AminoAcid::Gly}).n_atoms() == 4);
   });
 
-  run("gpu::score_batch uniform candidates (src/gpu.rs:166) vs interaction_energy_atoms", [&] {
+  run("allatom::disulfide_energy_zero_without_a_partner (src/allatom.rs:591)", [&] {
+    auto prot = protein_from_ca_trace({0.0f}, {0.0f}, {0.0f}, {AminoAcid::Cys});
+    CHECK(disulfide_energy(gpu, prot) == 0.0);
+  });
+
+  run("allatom::disulfide_energy_zero_beyond_cutoff (src/allatom.rs:600)", [&] {
+    auto prot = protein_from_ca_trace({0.0f, 50.0f}, {0.0f, 0.0f}, {0.0f, 0.0f},
+                                      {AminoAcid::Cys, AminoAcid::Cys});
+    CHECK(disulfide_energy(gpu, prot) == 0.0);
+  });
+
+  run("allatom::disulfide_energy_matches_pairwise_gaussian_sum (src/allatom.rs:611)", [&] {
+    auto prot = protein_from_ca_trace({0.0f, 3.8f, 7.6f}, {0.0f, 0.0f, 0.0f},
+                                      {0.0f, 0.0f, 0.0f},
+                                      {AminoAcid::Cys, AminoAcid::Cys, AminoAcid::Cys});
+    // the Gaussian well written out: d0 2.05, sigma 0.4, depth 4, cutoff 4 A,
+    // over the SG atom (slot 5) of each Cys
+    std::vector<std::array<float, 3>> sg;
+    for (size_t r = 0; r < prot.n_residues(); ++r) {
+      const uint32_t i = prot.res_start[r] + 5;
+      if (prot.amino_acid[r] == AminoAcid::Cys && i < prot.res_start[r + 1])
+        sg.push_back({prot.atoms.x[i], prot.atoms.y[i], prot.atoms.z[i]});
+    }
+    CHECK(sg.size() == 3);
+    float expected = 0.0f;
+    for (size_t i = 0; i < sg.size(); ++i)
+      for (size_t j = i + 1; j < sg.size(); ++j) {
+        const float dx = sg[i][0] - sg[j][0], dy = sg[i][1] - sg[j][1], dz = sg[i][2] - sg[j][2];
+        const float d = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (d > 4.0f) continue;
+        const float zz = (d - 2.05f) / 0.4f;
+        expected -= 4.0f * std::exp(-0.5f * zz * zz);
+      }
+    CHECK(std::fabs(disulfide_energy(gpu, prot) - expected) < 1e-5);
+  });
+
+  run("diffusion::allatom_energy (src/diffusion.rs:281) = interaction + disulfide", [&] {
+    auto ag = protein_from_ca_trace({0.0f, 3.8f}, {0.0f, 0.0f}, {0.0f, 0.0f},
+                                    {AminoAcid::Asp, AminoAcid::Ser});
+    auto ab = protein_from_ca_trace({0.0f, 3.8f, 7.6f}, {6.0f, 6.0f, 6.0f},
+                                    {0.0f, 0.0f, 0.0f},
+                                    {AminoAcid::Cys, AminoAcid::Lys, AminoAcid::Cys});
+    double ds = 0;
+    const double e = allatom_energy(gpu, ag, ab, &ds);
+    const double inter = interaction_energy_atoms(gpu, ag.atoms, ab.atoms);
+    CHECK(std::fabs(e - (inter + ds)) < 1e-6 * (1 + std::fabs(e)));
+    CHECK(ds == disulfide_energy(gpu, ab));
+    auto b = interaction_energy_atoms_breakdown(gpu, ag.atoms, ab.atoms);
+    CHECK(b.solvation != 0.0);   // typed atoms on both sides
+  });
+
+  run("gpu::score_batch uniform candidates (src/gpu.rs:166) vs the breakdown's shader terms", [&] {
     auto ag = protein_from_ca_trace({0.0f, 3.8f, 7.6f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f},
                                     {AminoAcid::Asp, AminoAcid::Lys, AminoAcid::Leu});
     std::vector<AtomProtein> cands;
@@ -123,7 +175,9 @@ int main() {
     auto e = gpu.score_batch(ag.atoms, flat, cands[0].n_atoms());
     CHECK(e.size() == 3);
     for (size_t k = 0; k < e.size(); ++k) {
-      double ref = interaction_energy_atoms(gpu, ag.atoms, cands[k].atoms);
+      // the shader scores LJ + Coulomb + hydrophobic (shaders/energy.wgsl:1,74-95)
+      auto b = interaction_energy_atoms_breakdown(gpu, ag.atoms, cands[k].atoms);
+      double ref = (b.lj + b.coulomb) + b.hydrophobic;
       CHECK(std::fabs(e[k] - ref) <= 1e-3 + 1e-6 * std::fabs(ref));
     }
     CHECK(gpu.score_batch(ag.atoms, flat, 0).empty());  // src/gpu.rs:172-174

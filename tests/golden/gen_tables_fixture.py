@@ -1,7 +1,9 @@
 """Writes tests/golden/ff_tables.json from the reference's force-field source
-(src/amber.rs, read as text): the AMBER_NB table (src/amber.rs:46-63) and,
-per residue in RESIDUE_TOPOLOGY order (src/amber.rs:357-378), each heavy
-atom's (AtomType index, charge, hydrophobic flag) (src/amber.rs:126-351).
+(src/amber.rs, read as text): the AMBER_NB table (src/amber.rs:46-63), the
+H-bond donor / acceptor classes (src/amber.rs:72-84), the EEF1 table
+(src/amber.rs:414-431) and, per residue in RESIDUE_TOPOLOGY order
+(src/amber.rs:357-378), each heavy atom's (AtomType index, charge,
+hydrophobic flag) and name (src/amber.rs:126-351).
 
 Run in the build container (where the reference is mounted):
     python tests/golden/gen_tables_fixture.py /root/reference
@@ -22,17 +24,30 @@ nb_block = re.search(r"pub static AMBER_NB.*?=\s*\[(.*?)\];", src, re.S).group(1
 amber_nb = [[float(a), float(b)] for a, b in
             re.findall(r"r_min_half:\s*([-\d.]+),\s*epsilon:\s*([-\d.]+)", nb_block)]
 
-residues = {}
+def bools(name):
+    block = re.search(r"pub static " + name + r".*?=\s*\[(.*?)\];", src, re.S).group(1)
+    block = re.sub(r"//[^\n]*", "", block)
+    return [1 if v == "true" else 0 for v in re.findall(r"\b(true|false)\b", block)]
+
+eef1_block = re.search(r"pub static EEF1.*?=\s*\[(.*?)\];", src, re.S).group(1)
+eef1 = [[float(a), float(b), float(c)] for a, b, c in re.findall(
+    r"dg_ref:\s*([-\d.]+),\s*vol:\s*([-\d.]+),\s*lambda:\s*([-\d.]+)", eef1_block)]
+
+residues, names = {}, {}
 for m in re.finditer(r"static (\w+)_ATOMS: &\[ResAtom\] = &\[(.*?)\];", src, re.S):
-    atoms = []
-    for a in re.finditer(r'atom!\("[^"]*",\s*(\w+),\s*([-\d.]+)(,\s*H)?\)', m.group(2)):
-        atoms.append([types[a.group(1)], float(a.group(2)), 1 if a.group(3) else 0])
+    atoms, nm = [], []
+    for a in re.finditer(r'atom!\("([^"]*)",\s*(\w+),\s*([-\d.]+)(,\s*H)?\)', m.group(2)):
+        atoms.append([types[a.group(2)], float(a.group(3)), 1 if a.group(4) else 0])
+        nm.append(a.group(1).strip())
     residues[m.group(1)] = atoms
+    names[m.group(1)] = nm
 
 topo = re.search(r"pub static RESIDUE_TOPOLOGY.*?=\s*\[(.*?)\];", src, re.S).group(1)
 order = re.findall(r"atoms:\s*(\w+)_ATOMS", topo)
 out = {"amber_nb": amber_nb, "residues": [residues[name] for name in order],
-       "residue_order": order}
+       "residue_order": order, "atom_names": [names[name] for name in order],
+       "hbond_donor": bools("HBOND_DONOR"), "hbond_acceptor": bools("HBOND_ACCEPTOR"),
+       "eef1": eef1}
 dst = Path(__file__).resolve().parent / "ff_tables.json"
 dst.write_text(json.dumps(out, indent=1) + "\n")
 print(f"wrote {dst}: {len(amber_nb)} atom types, {len(order)} residues, "

@@ -57,8 +57,146 @@ def test_atom_breakdown_sums_to_total(ctx):
     b = ctx.interaction_energy_atoms_breakdown(ag.atoms, ab.atoms)
     total = ctx.interaction_energy_atoms(ag.atoms, ab.atoms)
     assert abs(b.total() - total) < 1e-4
-    e64 = O.energy_f64(ag.atoms, ab.atoms)[0]
-    assert tol(total, e64.sum())
+    t64, _ = O.energy_full_f64(ag.atoms, ab.atoms)
+    assert tol(total, t64.sum())
+    got = [b.lj, b.coulomb, b.hydrophobic, b.hbond, b.solvation]
+    assert tol(got, t64).all(), (got, t64)
+    assert b.solvation != 0.0          # typed atoms: EEF1 is live
+
+
+def typed_cloud(xyz, types, q=0.0, rmh=1.9, eps=0.1, hyd=0):
+    c = cloud(xyz, q, rmh, eps, hyd)
+    c.atom_type = np.ascontiguousarray(types, np.uint8)
+    return c
+
+
+def test_hbond_and_solvation_two_atoms(ctx):
+    # backbone N (AtomType 5, donor) and O (10, acceptor) at the well's centre
+    # (src/energy.rs:477-491): -1.5 kcal/mol; EEF1 (src/solvation.rs:52-84)
+    # from the tables of the two types
+    tab = O.reference_tables()
+    eef = np.asarray(tab["eef1"], np.float32).reshape(-1, 3)
+    r = np.float32(2.9)
+    r2 = float(r * r)
+    ag, ab = typed_cloud([0, 0, 0], [5]), typed_cloud([r, 0, 0], [10])
+    b = ctx.interaction_energy_atoms_breakdown(ag, ab)
+    assert abs(b.hbond - O.hbond_energy_f64(r2)) < 1e-6 and abs(b.hbond + 1.5) < 1e-5
+    sv64 = O.pair_solvation_f64(eef[5], eef[10], r2)
+    assert abs(b.solvation - sv64) < 1e-6 * (1 + abs(sv64)) and sv64 != 0.0
+    # donor-donor (types 5 and 6): no H-bond, solvation stays (type 5 has zero
+    # volume, so only type 6's volume enters)
+    b2 = ctx.interaction_energy_atoms_breakdown(ag, typed_cloud([r, 0, 0], [6]))
+    assert b2.hbond == 0.0 and b2.solvation != 0.0
+    sv2 = O.pair_solvation_f64(eef[5], eef[6], r2)
+    assert abs(b2.solvation - sv2) < 1e-6 * (1 + abs(sv2))
+    # acceptor on the antigen side, donor on the binder: the term is symmetric
+    b3 = ctx.interaction_energy_atoms_breakdown(typed_cloud([0, 0, 0], [10]), ab.subset([0]))
+    assert b3.hbond == 0.0
+    b4 = ctx.interaction_energy_atoms_breakdown(typed_cloud([0, 0, 0], [10]),
+                                                typed_cloud([r, 0, 0], [5]))
+    assert abs(b4.hbond - b.hbond) < 1e-7
+    # the H-bond cutoff is r^2 <= 25, solvation's r^2 < 81; both need both types
+    far = np.float32(5.0)
+    b5 = ctx.interaction_energy_atoms_breakdown(ag, typed_cloud([far, 0, 0], [10]))
+    assert b5.hbond != 0.0
+    b6 = ctx.interaction_energy_atoms_breakdown(
+        ag, typed_cloud([np.nextafter(far, np.float32(6)), 0, 0], [10]))
+    assert b6.hbond == 0.0
+    b7 = ctx.interaction_energy_atoms_breakdown(ag, typed_cloud([9.0, 0, 0], [10]))
+    assert b7.solvation == 0.0
+    nine = np.nextafter(np.float32(9), np.float32(0))
+    b9 = ctx.interaction_energy_atoms_breakdown(ag, typed_cloud([nine, 0, 0], [10]))
+    assert b9.solvation != 0.0
+    untyped = cloud([r, 0, 0], 0.0, 1.9, 0.1, 0)
+    b8 = ctx.interaction_energy_atoms_breakdown(ag, untyped)
+    assert b8.hbond == 0.0 and b8.solvation == 0.0
+
+
+@pytest.mark.parametrize("make", [lambda: W.config1(), lambda: W.config2(n_cand=96),
+                                  lambda: W.config4(n_cand=12)],
+                         ids=["c1", "c2-96", "c4-12"])
+def test_all_terms_parity(ctx, make):
+    # allatom_energy (src/diffusion.rs:281-283) per candidate: the five pair
+    # terms vs the FP64 oracle and the binder's disulfide term vs its oracle
+    w = make()
+    ags, pop = W.device_objects(ctx, w)
+    pop.evaluate(forces=False, breakdown=True)
+    t = pop.terms()
+    e, _ = pop.energies()
+    p = w.pop
+    off = p.cand_atom_off()
+    ag = w.antigens[0].atoms
+    n_hb = 0
+    tot64 = np.zeros(p.n_cand)
+    for c in range(p.n_cand):
+        t64, _ = O.energy_full_f64(ag, p.candidate_cloud(c), w.cutoff)
+        tot64[c] = t64.sum()
+        ok = tol(t[c, :5], t64)
+        assert ok.all(), (c, t[c, :5], t64)
+        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
+        sg = O.sg_positions(p.atoms.x, p.atoms.y, p.atoms.z, p.res_atom_off,
+                            p.res_amino_acid, r0, r1)
+        ds = O.disulfide_f64(*sg)
+        assert abs(t[c, 5] - ds) <= 1e-5 * (1 + abs(ds)), (c, t[c, 5], ds)
+        n_hb += t64[3] != 0.0
+    assert tol(e, tot64).all()           # the candidate energy is the five terms' total
+    assert n_hb > 0                      # the workload exercises the H-bond term
+    # a forces evaluation keeps LJ + Coulomb + hydrophobic, the terms its forces derive from
+    pop.evaluate(forces=True)
+    e3, _ = pop.energies()
+    r = O.population_f64(ag, p.atoms, off, w.cutoff, forces=False)
+    assert tol(e3, r["energy"]).all()
+
+
+def test_allatom_energy_entry_point(ctx):
+    # allatom_energy (src/diffusion.rs:281-283) through its own entry point
+    # equals the population's six terms for the same candidate
+    from dnab.structure import AtomProtein
+    w = W.config2(n_cand=8)
+    ags, pop = W.device_objects(ctx, w)
+    pop.evaluate(forces=False, breakdown=True)
+    t = pop.terms()
+    p = w.pop
+    off = p.cand_atom_off()
+    ag = w.antigens[0]
+    for c in range(p.n_cand):
+        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
+        prot = AtomProtein(p.candidate_cloud(c), (p.res_atom_off[r0:r1 + 1] - off[c]).astype(np.uint32),
+                           p.res_amino_acid[r0:r1].copy(),
+                           (p.res_ca_atom[r0:r1] - off[c]).astype(np.uint32))
+        e, ds = ctx.allatom_energy(ag, prot, w.cutoff, with_disulfide=True)
+        assert ds == t[c, 5]
+        assert abs(e - t[c].sum()) <= 1e-9 * (1 + np.abs(t[c]).sum()), (c, e, t[c])
+        inter = ctx.interaction_energy_atoms(ag.atoms, prot.atoms, w.cutoff)
+        assert abs(e - (inter + ds)) <= 1e-9 * (1 + abs(e))
+
+
+def test_disulfide_closed_form(ctx):
+    # two Cys whose SG atoms sit 2.05 A apart: the well's bottom, -4 kcal/mol
+    # (src/allatom.rs:155-180); beyond 4 A nothing; a third SG adds its pairs
+    prot = protein_from_ca_trace(np.arange(4, dtype=np.float32) * 3.8, np.zeros(4), np.zeros(4),
+                                 aa_from_sequence("ACCA"))
+    host = population_from_proteins([prot, prot])
+    ags = [DeviceAntigen(ctx, W.globule_antigen(20, 8.0, 0), cutoff=10.0)]
+    pop = DevicePopulation(ctx, ags, host)
+    x, y, z = (v.copy() for v in pop.coords())
+    sg = [int(host.res_atom_off[r]) + O.CYS_SG_SLOT for r in (1, 2)]
+    base = int(host.res_atom_off[4])
+    d0 = np.float32(2.05)
+    for k, d in ((0, d0), (1, np.float32(4.5))):
+        a, b = sg[0] + k * base, sg[1] + k * base
+        y[b], z[b] = y[a], z[a]
+        x[b] = x[a] + d
+    pop.set_coords(x, y, z)
+    pop.evaluate(forces=False, breakdown=True)
+    t = pop.terms()
+    assert abs(t[0, 5] + 4.0) < 1e-5, t[:, 5]
+    assert t[1, 5] == 0.0
+    for c in range(2):
+        xs = x[[sg[0] + c * base, sg[1] + c * base]]
+        ys = y[[sg[0] + c * base, sg[1] + c * base]]
+        zs = z[[sg[0] + c * base, sg[1] + c * base]]
+        assert abs(t[c, 5] - O.disulfide_f64(xs, ys, zs)) < 1e-6
 
 
 def eval_vs_oracle(ctx, w, n_check=None):

@@ -123,12 +123,16 @@ def test_trajectory_tuning_changes_speed_not_results(ctx):
     w = W.config3(n_cand=80)
     kw = dict(n_steps=12, score_every=4, top_k=8, with_noise=1, seed=9)
     ref = None
-    for waves, cpw, bal, cells, chunks in ((4, 1, 0, 16, 4), (8, 3, 0, 5, 1), (16, 4, 0, 64, 3),
-                                           (16, 16, 0, 1, 2), (16, 6, 1, 16, 2), (4, 2, 1, 3, 1)):
+    # (rebalance 5: the 12 steps run as launches of 5, 5 and 2 steps, regrouped between them)
+    for waves, cpw, bal, cells, chunks, reb in ((4, 1, 0, 16, 4, 0), (8, 3, 0, 5, 1, 0),
+                                                (16, 4, 0, 64, 3, 0), (16, 16, 0, 1, 2, 0),
+                                                (16, 6, 1, 16, 2, 0), (4, 2, 1, 3, 1, 5),
+                                                (16, 4, 1, 2, 2, 5)):
         ags, pop = W.device_objects(ctx, w)
-        pop.set_trajectory_tuning(waves, cpw, bal, cells, chunks)
+        pop.set_trajectory_tuning(waves, cpw, bal, cells, chunks, reb)
         assert pop.trajectory_tuning() == {"waves": waves, "cand_per_group": cpw, "balance": bal,
-                                           "cells_per_item": cells, "chunks_per_item": chunks}
+                                           "cells_per_item": cells, "chunks_per_item": chunks,
+                                           "rebalance_steps": reb}
         p = langevin_params(restraint_radius=w.restraint_radius[0], **kw)
         idx, e = pop.run(p)
         hi, he = pop.scores()

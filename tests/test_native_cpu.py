@@ -99,6 +99,17 @@ def test_ff_tables_match_reference_fixture():
             assert q.value == np.float32(charge)
             assert rmh.value == np.float32(fx["amber_nb"][atype][0])
             assert eps.value == np.float32(fx["amber_nb"][atype][1])
+    for t in range(16):
+        dg, vol, lam = C.c_float(), C.c_float(), C.c_float()
+        don, acc = C.c_uint8(), C.c_uint8()
+        N.check(N.lib.dnab_ff_type_params(t, C.byref(dg), C.byref(vol), C.byref(lam), C.byref(don),
+                                          C.byref(acc)))
+        assert [dg.value, vol.value, lam.value] == [np.float32(v) for v in fx["eef1"][t]]
+        assert don.value == fx["hbond_donor"][t] and acc.value == fx["hbond_acceptor"][t]
+    assert N.lib.dnab_ff_type_params(16, None, None, None, None, None) != 0
+    # the disulfide term reads the Cys SG atom at topology slot 5
+    cys = fx["residue_order"].index("CYS")
+    assert fx["atom_names"][cys].index("SG") == 5
 
 
 def test_parse_peptide_layout():

@@ -295,3 +295,124 @@ def test_step_clamp_and_noise_bounds():
         xs.append(float(x[0]) - 20.0)
     xs = np.array(xs)
     assert (np.abs(xs) <= 0.5).all() and xs.std() > 0.2      # U(-s, s)
+
+
+# ---- H-bond, EEF1 solvation and disulfide terms -------------------------------
+
+# the reference's f32 constants (src/energy.rs:24-25), as the FP64 oracle widens them
+HB_R0, HB_SIGMA = float(np.float32(2.9)), float(np.float32(0.35))
+
+
+def test_hbond_energy_closed_forms():
+    # src/energy.rs:477-491: deepest at the ideal distance, decays, zero beyond 5 A
+    assert abs(O.hbond_energy_f32(np.float32(HB_R0) * np.float32(HB_R0)) - (-1.5)) < 1e-5
+    assert O.hbond_energy_f32(2.9 ** 2) < O.hbond_energy_f32(3.9 ** 2)
+    assert O.hbond_energy_f32(25.0 + 1.0) == 0.0
+    assert O.hbond_energy_f32(25.0) != 0.0          # the cutoff is inclusive
+    # the Gaussian well: symmetric about r0, half depth at r0 +- sigma sqrt(2 ln 2)
+    for d in (0.1, 0.3, 0.7):
+        assert abs(O.hbond_energy_f64((HB_R0 - d) ** 2) - O.hbond_energy_f64((HB_R0 + d) ** 2)) < 1e-12
+    w = HB_SIGMA * math.sqrt(2 * math.log(2))
+    assert abs(O.hbond_energy_f64((HB_R0 + w) ** 2) - (-0.75)) < 1e-6
+    assert abs(O.hbond_energy_f64(HB_R0 ** 2) - (-1.5)) < 1e-6
+    # the f32 restatement follows the f64 one
+    for r in np.linspace(2.0, 5.0, 31):
+        assert abs(O.hbond_energy_f32(np.float32(r) ** 2) - O.hbond_energy_f64(float(np.float32(r) ** 2))) < 1e-5
+
+
+def test_pair_solvation_signs_and_shape():
+    # src/solvation.rs:92-117: hydrophobic burial lowers, polar burial raises the energy
+    hyd = (1.0, 10.0, 3.5)
+    pol = (-5.0, 14.0, 3.15)
+    assert O.pair_solvation_f32(hyd, hyd, 9.0) < 0.0
+    assert O.pair_solvation_f32(pol, pol, 9.0) > 0.0
+    # symmetric in the two atoms; linear in dg_ref; exp(-(r/lambda)^2) decay
+    a, b = (-0.187, 11.7, 3.7), (-5.33, 14.2, 3.15)
+    assert abs(O.pair_solvation_f64(a, b, 16.0) - O.pair_solvation_f64(b, a, 16.0)) < 1e-15
+    two = (2 * hyd[0], hyd[1], hyd[2])
+    assert abs(O.pair_solvation_f64(two, two, 9.0) - 2 * O.pair_solvation_f64(hyd, hyd, 9.0)) < 1e-15
+    lam = hyd[2]
+    ratio = O.pair_solvation_f64(hyd, hyd, lam * lam) / O.pair_solvation_f64(hyd, hyd, 1e-12)
+    assert abs(ratio - math.exp(-1.0)) < 1e-9
+    # f(0) = 1 / (pi^(3/2) lambda^3) per burial direction
+    f0 = -O.pair_solvation_f64(hyd, hyd, 1e-18) / (2 * hyd[0] * hyd[1])
+    assert abs(f0 - 1.0 / (float(np.float32(5.568328)) * lam ** 3)) < 1e-9
+
+
+def _cys_protein(xs):
+    return protein_from_ca_trace(list(xs), [0.0] * len(xs), [0.0] * len(xs), A("C" * len(xs)))
+
+
+def _sg(p):
+    n = len(p.amino_acid)
+    return O.sg_positions(p.atoms.x, p.atoms.y, p.atoms.z, p.res_start, p.amino_acid, 0, n)
+
+
+def test_disulfide_energy_reference_cases():
+    # src/allatom.rs:590-639
+    assert O.disulfide_f32(*_sg(_cys_protein([0.0]))) == 0.0           # no partner
+    assert O.disulfide_f32(*_sg(_cys_protein([0.0, 50.0]))) == 0.0     # beyond 4 A
+    p = _cys_protein([0.0, 3.8, 7.6])
+    x, y, z = _sg(p)
+    assert len(x) == 3
+    exp = np.float32(0.0)
+    for i in range(3):
+        for j in range(i + 1, 3):
+            d = np.float32(np.sqrt(np.float32((x[i] - x[j]) ** 2 + (y[i] - y[j]) ** 2 + (z[i] - z[j]) ** 2)))
+            if d > 4.0:
+                continue
+            t = (d - np.float32(2.05)) / np.float32(0.4)
+            exp -= np.float32(4.0) * np.float32(np.exp(-0.5 * t * t))
+    assert abs(O.disulfide_f32(x, y, z) - float(exp)) < 1e-5
+
+
+def test_disulfide_well_closed_forms():
+    # deepest (-4) at the ideal S-S distance, Gaussian in d, zero past 4 A
+    x = np.array([0.0, 2.05], np.float32)
+    z0 = np.zeros(2, np.float32)
+    assert abs(O.disulfide_f64(x, z0, z0) - (-4.0)) < 1e-6
+    w = 0.4 * math.sqrt(2 * math.log(2))
+    assert abs(O.disulfide_f64(np.array([0.0, 2.05 + w], np.float32), z0, z0) - (-2.0)) < 1e-5
+    assert O.disulfide_f64(np.array([0.0, 4.01], np.float32), z0, z0) == 0.0
+    # three mutual partners add pairwise
+    t = np.array([0.0, 2.05, 1.025], np.float32)
+    u = np.array([0.0, 0.0, float(np.float32(2.05 * math.sqrt(3) / 2))], np.float32)
+    e3 = O.disulfide_f64(t, u, np.zeros(3, np.float32))
+    assert abs(e3 - 3 * (-4.0)) < 1e-4
+
+
+def test_full_energy_f32_follows_f64_and_sums():
+    # five terms of interaction_energy_atoms_breakdown (src/energy.rs:261-314)
+    ag = protein_from_ca_trace([0.0, 3.8, 7.6, 11.4], [0.0] * 4, [0.0] * 4, A("DKSN"))
+    ab = protein_from_ca_trace([1.0, 4.8, 8.6], [5.0] * 3, [0.5] * 3, A("ERT"))
+    e32 = O.energy_full_f32(ag.atoms, ab.atoms)
+    e64, per = O.energy_full_f64(ag.atoms, ab.atoms)
+    assert e64[3] < 0.0 and e64[4] != 0.0           # H-bonds and solvation present
+    for a, b in zip(e32, e64):
+        assert abs(float(a) - b) <= 1e-3 + 1e-4 * abs(b)
+    assert np.allclose(per.sum(0), e64, rtol=1e-12, atol=1e-12)
+    # the first three terms are those of the three-term oracle
+    three = O.energy_f64(ag.atoms, ab.atoms)[0]
+    assert np.allclose(e64[:3], three, rtol=1e-12, atol=1e-12)
+    # H-bond and solvation: brute force over all pairs with the pair functions
+    # pinned above, the reference's pair sets (src/energy.rs:284-305: inside
+    # the 0.25 <= r^2 <= 100 predicate, donor-acceptor either way;
+    # src/solvation.rs:75-80: 0.25 <= r^2 < 81) and the type tables
+    tab = O.reference_tables()
+    eef = np.asarray(tab["eef1"], np.float32).reshape(-1, 3)    # the f32 table values
+    don, acc = tab["hbond_donor"], tab["hbond_acceptor"]
+    a, b = ag.atoms, ab.atoms
+    hb = sv = 0.0
+    for j in range(len(b)):
+        for i in range(len(a)):
+            dx, dy, dz = (np.float32(getattr(b, k)[j] - getattr(a, k)[i]) for k in "xyz")
+            r2f = float(np.float32(np.float32(dx * dx + dy * dy) + np.float32(dz * dz)))
+            # the sets by the f32 predicate, the values at the exact r^2
+            r2 = sum((float(getattr(b, k)[j]) - float(getattr(a, k)[i])) ** 2 for k in "xyz")
+            ta, tb = int(a.atom_type[i]), int(b.atom_type[j])
+            if 0.25 <= r2f <= 25.0 and ((don[tb] and acc[ta]) or (acc[tb] and don[ta])):
+                hb += O.hbond_energy_f64(r2)
+            if 0.25 <= r2f < 81.0:
+                sv += O.pair_solvation_f64(eef[ta], eef[tb], r2)
+    assert abs(e64[3] - hb) <= 1e-9 * (1 + abs(hb)), (e64[3], hb)
+    assert abs(e64[4] - sv) <= 1e-9 * (1 + abs(sv)), (e64[4], sv)
