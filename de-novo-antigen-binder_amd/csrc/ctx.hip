@@ -75,6 +75,25 @@ void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end
   ctx->timers.push_back(std::move(t));
 }
 
+dnab_ctx* ctx_retain(dnab_ctx* ctx) {
+  if (ctx) ++ctx->refs;
+  return ctx;
+}
+
+void ctx_release(dnab_ctx* ctx) {
+  if (!ctx || --ctx->refs > 0) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& t : ctx->timers)
+    for (auto& pr : t.pending) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
 }  // namespace dnab
 
 using namespace dnab;
@@ -127,19 +146,10 @@ int dnab_ctx_create(int device, dnab_ctx** out) {
   return DNAB_OK;
 }
 
-void dnab_ctx_destroy(dnab_ctx* ctx) {
-  if (!ctx) return;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (auto& t : ctx->timers)
-    for (auto& pr : t.pending) {
-      (void)hipEventDestroy(pr.first);
-      (void)hipEventDestroy(pr.second);
-    }
-  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
-}
+
+// Drops the caller's reference: the context is freed once the objects made on
+// it are destroyed too, whatever the order.
+void dnab_ctx_destroy(dnab_ctx* ctx) { dnab::ctx_release(ctx); }
 
 const char* dnab_ctx_last_error(const dnab_ctx* ctx) {
   return ctx ? ctx->last_error.c_str() : "null context";

@@ -183,6 +183,10 @@ struct KernelTimer {
 }  // namespace dnab
 
 struct dnab_ctx {
+  // One reference for the caller (dropped by dnab_ctx_destroy) and one per
+  // object made on the context (dropped when the object is destroyed): the
+  // stream and the struct outlive every object, in any destroy order.
+  int refs = 1;
   int device = 0;
   hipStream_t stream = nullptr;
   std::string last_error;
@@ -328,6 +332,8 @@ constexpr uint32_t kExcluded = 0xFFFFFFFFu;
 int grid_build_device(dnab_ctx* ctx, GridDev& g, const float* d_x, const float* d_y,
                       const float* d_z, size_t n);
 // ctx.hip
+dnab_ctx* ctx_retain(dnab_ctx* ctx);   // an object made on ctx keeps it alive
+void ctx_release(dnab_ctx* ctx);       // the last release frees the stream and ctx
 hipEvent_t timer_begin(dnab_ctx* ctx, const char* name, hipEvent_t* end_out);
 void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end);
 int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);

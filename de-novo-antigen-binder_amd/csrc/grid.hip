@@ -359,14 +359,19 @@ int dnab_grid_create(dnab_ctx* ctx, float cell_size, dnab_grid** out) {
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "grid: cell_size must be positive and finite");
   dnab_grid* g = new (std::nothrow) dnab_grid();
   if (!g) return DNAB_ERR_ALLOC;
-  g->ctx = ctx;
+  g->ctx = ctx_retain(ctx);
   g->g.cell = cell_size;
   g->g.inv = 1.0f / cell_size;  // src/spatial.rs:26, IEEE f32 division on the host
   *out = g;
   return DNAB_OK;
 }
 
-void dnab_grid_destroy(dnab_grid* g) { delete g; }
+void dnab_grid_destroy(dnab_grid* g) {
+  if (!g) return;
+  dnab_ctx* ctx = g->ctx;
+  delete g;
+  ctx_release(ctx);
+}
 
 int dnab_grid_build(dnab_ctx* ctx, dnab_grid* grid, const float* x, const float* y,
                     const float* z, size_t n) {

@@ -495,10 +495,10 @@ int dnab_mlp_create(dnab_ctx* ctx, uint64_t seed, dnab_mlp** out) {
   (void)hipSetDevice(ctx->device);
   dnab_mlp* n = new (std::nothrow) dnab_mlp();
   if (!n) return DNAB_ERR_ALLOC;
-  n->ctx = ctx;
+  n->ctx = ctx_retain(ctx);
   if (n->p.alloc(kParams) || n->g.alloc(kParams) || n->m.alloc(kParams) || n->v.alloc(kParams) ||
       n->pbf.alloc(kParams)) {
-    delete n;
+    dnab_mlp_destroy(n);
     return dnab_set_error(ctx, DNAB_ERR_ALLOC, "mlp: allocation failed");
   }
   // Glorot-uniform weights, zero biases (host SplitMix64).
@@ -522,7 +522,7 @@ int dnab_mlp_create(dnab_ctx* ctx, uint64_t seed, dnab_mlp** out) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = DNAB_ERR_HIP;
   }
   if (rc) {
-    delete n;
+    dnab_mlp_destroy(n);
     return rc;
   }
   *out = n;
@@ -531,8 +531,10 @@ int dnab_mlp_create(dnab_ctx* ctx, uint64_t seed, dnab_mlp** out) {
 
 void dnab_mlp_destroy(dnab_mlp* n) {
   if (!n) return;
-  if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
+  dnab_ctx* ctx = n->ctx;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
   delete n;
+  ctx_release(ctx);
 }
 
 int dnab_mlp_get_params(dnab_ctx* ctx, const dnab_mlp* n, float* out, size_t count) {
@@ -578,12 +580,12 @@ int dnab_dataset_create(dnab_ctx* ctx, size_t capacity, float energy_scale, floa
   (void)hipSetDevice(ctx->device);
   dnab_dataset* d = new (std::nothrow) dnab_dataset();
   if (!d) return DNAB_ERR_ALLOC;
-  d->ctx = ctx;
+  d->ctx = ctx_retain(ctx);
   d->capacity = capacity;
   d->energy_scale = energy_scale;
   d->energy_cap = energy_cap;
   if (d->X.alloc(capacity * kD) || d->t.alloc(capacity) || d->e.alloc(capacity)) {
-    delete d;
+    dnab_dataset_destroy(d);
     return dnab_set_error(ctx, DNAB_ERR_ALLOC, "dataset: allocation of %zu rows failed", capacity);
   }
   *out = d;
@@ -592,8 +594,10 @@ int dnab_dataset_create(dnab_ctx* ctx, size_t capacity, float energy_scale, floa
 
 void dnab_dataset_destroy(dnab_dataset* d) {
   if (!d) return;
-  if (d->ctx) (void)hipStreamSynchronize(d->ctx->stream);
+  dnab_ctx* ctx = d->ctx;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
   delete d;
+  ctx_release(ctx);
 }
 
 int dnab_dataset_size(const dnab_dataset* d, size_t* n) {

@@ -1142,9 +1142,12 @@ int population_pair(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
                        pop->ag_tile_start.p, pop->perm.p, pop->pos.p, pop->par.p,
                        pop->f64_atom.p, pop->bd_atom.p, pop->e_atom.p);
   } else if (flags & DNAB_EVAL_BREAKDOWN) {
+    hipEvent_t ev_end;
+    hipEvent_t ev = timer_begin(ctx, "pair_breakdown", &ev_end);
     hipLaunchKernelGGL((k_pair_tile<false, true, false>), grid, block, 0, ctx->stream,
                        pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->pos.p,
                        pop->par.p, o);
+    timer_end(ctx, "pair_breakdown", ev, ev_end);
   } else if (flags & kEvalCount) {
     launch_pair<true, true>(ctx, pop, o);
   } else if (flags & DNAB_EVAL_FORCES) {
@@ -1258,15 +1261,14 @@ int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* d, dnab_antigen*
   (void)hipSetDevice(ctx->device);
   dnab_antigen* ag = new (std::nothrow) dnab_antigen();
   if (!ag) return DNAB_ERR_ALLOC;
-  ag->ctx = ctx;
+  ag->ctx = ctx_retain(ctx);
   ag->cutoff = d->cutoff;
   ag->restraint_radius = d->restraint_radius;
   for (int a = 0; a < 3; ++a) ag->center[a] = d->center[a];
   const size_t n = d->atoms.n;
   const dnab_atom_cloud& c = d->atoms;
   auto fail = [&](int rc) {
-    (void)hipStreamSynchronize(ctx->stream);
-    delete ag;
+    dnab_antigen_destroy(ag);
     return rc;
   };
   // bounding box of the atoms (host)
@@ -1333,8 +1335,10 @@ int dnab_antigen_info(const dnab_antigen* ag, size_t* n_atoms, size_t* n_cells_n
 
 void dnab_antigen_destroy(dnab_antigen* ag) {
   if (!ag) return;
-  if (ag->ctx) (void)hipStreamSynchronize(ag->ctx->stream);
+  dnab_ctx* ctx = ag->ctx;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
   delete ag;
+  ctx_release(ctx);
 }
 
 int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t n_antigens,
@@ -1384,15 +1388,14 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   (void)hipSetDevice(ctx->device);
   dnab_population* pop = new (std::nothrow) dnab_population();
   if (!pop) return DNAB_ERR_ALLOC;
-  pop->ctx = ctx;
+  pop->ctx = ctx_retain(ctx);
   pop->n_cand = nc;
   pop->n_res = nr;
   pop->n_atoms = na;
   pop->n_ag = n_antigens;
   pop->antigens.assign(antigens, antigens + n_antigens);
   auto fail = [&](int rc) {
-    (void)hipStreamSynchronize(ctx->stream);
-    delete pop;
+    dnab_population_destroy(pop);
     return rc;
   };
   // host staging
@@ -1561,8 +1564,10 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
 
 void dnab_population_destroy(dnab_population* pop) {
   if (!pop) return;
-  if (pop->ctx) (void)hipStreamSynchronize(pop->ctx->stream);
+  dnab_ctx* ctx = pop->ctx;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
   delete pop;
+  ctx_release(ctx);
 }
 
 int dnab_population_info(const dnab_population* pop, size_t* n_cand, size_t* n_res,

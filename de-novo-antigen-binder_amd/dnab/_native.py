@@ -221,6 +221,10 @@ _PROTOS = {
     "dnab_population_read_terms": (C.c_int, [vp, vp, f64p]),
     "dnab_allatom_energy": (C.c_int, [vp, C.POINTER(AtomCloudC), C.POINTER(AtomCloudC), u32p,
                                       u32p, u8p, C.c_size_t, C.c_float, f64p, f64p]),
+    "dnab_affinity_maturation": (C.c_int, [vp, C.POINTER(AtomCloudC), C.c_float,
+                                           C.POINTER(PopulationDescC), u8p, u32p, u32p,
+                                           C.POINTER(ProteinOutC), u8p, u8p, f64p,
+                                           C.POINTER(C.c_size_t)]),
 }
 for _name, (_res, _args) in _PROTOS.items():
     _fn = getattr(lib, _name)

@@ -14,7 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
-from .structure import AtomCloud, AtomProtein, PopulationHost, ResidueCloud
+from .structure import (MAX_ATOMS_PER_RESIDUE, AtomCloud, AtomProtein, PopulationHost,
+                        ResidueCloud, population_from_proteins)
 
 vp = C.c_void_p
 
@@ -141,6 +142,63 @@ class GpuContext:
                                              N.ptr(ca, C.c_uint32), N.ptr(aa, C.c_uint8),
                                              len(aa), cutoff, C.byref(e), C.byref(ds)))
         return (e.value, ds.value) if with_disulfide else e.value
+
+    def affinity_maturation(self, antigen, antibodies: list[AtomProtein], mutable,
+                            rotamers=None, cutoff: float = 10.0):
+        """affinity_maturation_allatom (src/diffusion.rs:955-961) for every
+        antibody in one call. mutable[a]: ascending residue indices of antibody
+        a; rotamers[a] (optional): each residue's rotamer index. Returns the
+        final proteins, the scores (the reference's return value), and per
+        antibody the rotamer index and rebuilt flag of every residue."""
+        ag = antigen.atoms if isinstance(antigen, AtomProtein) else antigen
+        antibodies = list(antibodies)
+        n_ab = len(antibodies)
+        host = population_from_proteins(antibodies)
+        n_res = host.n_res
+        moff = np.zeros(n_ab + 1, np.uint32)
+        for a, m in enumerate(mutable):
+            moff[a + 1] = moff[a] + len(m)
+        mres = np.ascontiguousarray(np.concatenate([np.asarray(m, np.int64) for m in mutable])
+                                    if n_ab else np.zeros(0), np.uint32)
+        rot = None
+        if rotamers is not None:
+            rot = np.ascontiguousarray(np.concatenate([np.asarray(r, np.uint8) for r in rotamers]),
+                                       np.uint8)
+        cap = max(1, MAX_ATOMS_PER_RESIDUE * n_res)
+        f = {k: np.zeros(cap, np.float32) for k in ("x", "y", "z", "charge", "rmh", "eps")}
+        hyd, at = np.zeros(cap, np.uint8), np.zeros(cap, np.uint8)
+        ri = np.zeros(cap, np.uint32)
+        rs = np.zeros(n_res + 1, np.uint32)
+        ca = np.zeros(max(n_res, 1), np.uint32)
+        aa = np.zeros(max(n_res, 1), np.uint8)
+        out = N.ProteinOutC(cap, N.ptr(f["x"], C.c_float), N.ptr(f["y"], C.c_float),
+                            N.ptr(f["z"], C.c_float), N.ptr(f["charge"], C.c_float),
+                            N.ptr(f["rmh"], C.c_float), N.ptr(f["eps"], C.c_float),
+                            N.ptr(hyd, C.c_uint8), N.ptr(at, C.c_uint8), N.ptr(ri, C.c_uint32),
+                            N.ptr(rs, C.c_uint32), N.ptr(ca, C.c_uint32), N.ptr(aa, C.c_uint8))
+        rot_out = np.zeros(max(n_res, 1), np.uint8)
+        rebuilt = np.zeros(max(n_res, 1), np.uint8)
+        energy = np.zeros(max(n_ab, 1), np.float64)
+        na = C.c_size_t(0)
+        agc, desc = ag.c(), host.c()
+        self.check(N.lib.dnab_affinity_maturation(
+            self._h, C.byref(agc), cutoff, C.byref(desc), N.ptr(rot, C.c_uint8),
+            N.ptr(moff, C.c_uint32), N.ptr(mres, C.c_uint32), C.byref(out),
+            N.ptr(rot_out, C.c_uint8), N.ptr(rebuilt, C.c_uint8), N.ptr(energy, C.c_double),
+            C.byref(na)))
+        prots, rots, flags = [], [], []
+        for a in range(n_ab):
+            g0, g1 = int(host.cand_res_off[a]), int(host.cand_res_off[a + 1])
+            a0, a1 = int(rs[g0]), int(rs[g1])
+            cloud = AtomCloud(f["x"][a0:a1].copy(), f["y"][a0:a1].copy(), f["z"][a0:a1].copy(),
+                              f["charge"][a0:a1].copy(), f["rmh"][a0:a1].copy(),
+                              f["eps"][a0:a1].copy(), hyd[a0:a1].copy(), at[a0:a1].copy(),
+                              ri[a0:a1].copy())
+            prots.append(AtomProtein(cloud, (rs[g0:g1 + 1] - a0).astype(np.uint32),
+                                     aa[g0:g1].copy(), (ca[g0:g1] - a0).astype(np.uint32)))
+            rots.append(rot_out[g0:g1].copy())
+            flags.append(rebuilt[g0:g1].astype(bool))
+        return prots, energy[:n_ab], rots, flags
 
     def atom_forces(self, antigen: AtomCloud, antibody: AtomCloud, cutoff: float = 10.0):
         n = len(antibody)

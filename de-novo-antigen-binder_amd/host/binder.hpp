@@ -14,6 +14,7 @@
 //   binder::allatom_energy, disulfide_energy
 //                                          src/diffusion.rs:281-283,
 //                                          src/allatom.rs:155-180
+//   binder::affinity_maturation_allatom    src/diffusion.rs:924-961
 //   binder::run_allatom_gpu_phase          src/diffusion.rs:1008-1062 (GPU phase,
 //                                          gradient steps + scoring, no MC moves)
 //
@@ -264,6 +265,55 @@ inline double disulfide_energy(const GpuContext& gpu, const AtomProtein& p) {
   double ds = 0;
   allatom_energy(gpu, AtomProtein{}, p, &ds);
   return ds;
+}
+
+// affinity_maturation_allatom (src/diffusion.rs:955-961): greedy point
+// mutations at the `mutable` residues (ascending), scored by allatom_energy;
+// ab is replaced by the final protein and the best score returned. Residues
+// carry the most probable rotamer (this interface keeps no chi angles).
+inline double affinity_maturation_allatom(const GpuContext& gpu, AtomProtein& ab,
+                                          const AtomProtein& ag,
+                                          const std::vector<uint32_t>& mutable_res,
+                                          float cutoff = 10.0f) {
+  const size_t nr = ab.n_residues();
+  std::vector<uint8_t> aa(nr);
+  for (size_t i = 0; i < nr; ++i) aa[i] = static_cast<uint8_t>(ab.amino_acid[i]);
+  uint32_t cand_ag = 0, cand_res_off[2] = {0, static_cast<uint32_t>(nr)};
+  uint32_t mut_off[2] = {0, static_cast<uint32_t>(mutable_res.size())};
+  dnab_population_desc d{};
+  d.n_cand = 1;
+  d.n_res = nr;
+  d.cand_antigen = &cand_ag;
+  d.cand_res_off = cand_res_off;
+  d.res_atom_off = ab.res_start.data();
+  d.res_ca_atom = ab.ca_atom_idx.data();
+  d.res_amino_acid = aa.data();
+  d.atoms = ab.atoms.view();
+  dnab_atom_cloud a = ag.atoms.view();
+  const size_t cap = 14 * nr + 1;
+  AtomProtein out;
+  auto& o = out.atoms;
+  o.x.resize(cap); o.y.resize(cap); o.z.resize(cap); o.charge.resize(cap);
+  o.r_min_half.resize(cap); o.epsilon.resize(cap); o.hydrophobic.resize(cap);
+  o.atom_type.resize(cap); o.residue_idx.resize(cap);
+  out.res_start.assign(nr + 1, 0);
+  out.ca_atom_idx.assign(nr, 0);
+  std::vector<uint8_t> aa_out(nr + 1);
+  dnab_protein_out po{cap, o.x.data(), o.y.data(), o.z.data(), o.charge.data(),
+                      o.r_min_half.data(), o.epsilon.data(), o.hydrophobic.data(),
+                      o.atom_type.data(), o.residue_idx.data(), out.res_start.data(),
+                      out.ca_atom_idx.data(), aa_out.data()};
+  double e = 0;
+  size_t na = 0;
+  check(dnab_affinity_maturation(gpu.raw(), &a, cutoff, &d, nullptr, mut_off,
+                                 mutable_res.data(), &po, nullptr, nullptr, &e, &na),
+        gpu.raw());
+  for (auto* v : {&o.x, &o.y, &o.z, &o.charge, &o.r_min_half, &o.epsilon}) v->resize(na);
+  o.hydrophobic.resize(na); o.atom_type.resize(na); o.residue_idx.resize(na);
+  out.amino_acid.resize(nr);
+  for (size_t i = 0; i < nr; ++i) out.amino_acid[i] = static_cast<AminoAcid>(aa_out[i]);
+  ab = std::move(out);
+  return e;
 }
 
 // compute_forces_with_grid (src/energy.rs:322-398): outputs sized >= n_ab.

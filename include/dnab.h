@@ -98,6 +98,11 @@ typedef struct dnab_protein_out {
  * dnab_ctx_create replaces GpuContext::try_init (src/gpu.rs:143-243): a
  * non-zero status (DNAB_ERR_NO_DEVICE) takes the place of `None`. */
 int dnab_ctx_create(int device, dnab_ctx** out);
+/* Drops the caller's hold on the context. Objects made on it (grids,
+ * antigens, populations, networks, datasets) keep it alive until they are
+ * destroyed, so objects and context may be destroyed in any order (as Rust
+ * drops or a garbage collector's finalizers order them); the context must
+ * not be passed to any other call after this one. */
 void dnab_ctx_destroy(dnab_ctx* ctx);
 const char* dnab_ctx_last_error(const dnab_ctx* ctx);
 const char* dnab_status_string(int status);
@@ -271,6 +276,36 @@ typedef struct dnab_population_desc {
 int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t n_antigens,
                            const dnab_population_desc* desc, dnab_population** out);
 void dnab_population_destroy(dnab_population* pop);
+/* Affinity maturation of a batch of antibodies against one antigen:
+ * affinity_maturation_allatom (src/diffusion.rs:924-961), called by the
+ * reference once per result (src/main.rs:341-347), here for all of them in one
+ * call. Per antibody, for each mutable position r in order, the amino acids
+ * are tried in ALL_AA order (src/atom.rs:29-50), each mutant built as
+ * mutate_residue does (ideal backbone around r's Calpha, the type's first
+ * rotamer, src/allatom.rs:299-384); the first whose allatom_energy is strictly
+ * below the best so far is kept, and a non-improving trial is undone by
+ * rebuilding the old type on an ideal backbone with its rotamer.
+ *   ab: the antibodies as a population description (n_cand antibodies,
+ *       cand_antigen ignored); res_amino_acid and res_ca_atom required;
+ *       atom_type needed for the H-bond and solvation terms.
+ *   res_rotamer (nullable, n_res): each residue's rotamer index, whose chi the
+ *       undo rebuild uses (0, the most probable, when NULL).
+ *   mutable_off (n_cand + 1), mutable_res: per antibody, strictly ascending
+ *       residue indices within it (the reference's interface filter,
+ *       src/main.rs:342-344).
+ * Outputs: out (nullable: then only sizes and per-residue outputs) receives
+ * the final antibodies concatenated, residues in input order, res_start and
+ * ca_atom_idx as global atom indices; out_rotamer and out_rebuilt (nullable,
+ * n_res) the rotamer index and whether the residue was rebuilt; out_energy
+ * (n_cand) the returned score of the reference, the best allatom_energy
+ * accepted (FP64); n_atoms_out the final atom count. Scores are taken from one
+ * scoring batch of single residues: the trial scores equal full re-scoring to
+ * FP64 rounding. */
+int dnab_affinity_maturation(dnab_ctx* ctx, const dnab_atom_cloud* antigen, float cutoff,
+                             const dnab_population_desc* ab, const uint8_t* res_rotamer,
+                             const uint32_t* mutable_off, const uint32_t* mutable_res,
+                             dnab_protein_out* out, uint8_t* out_rotamer, uint8_t* out_rebuilt,
+                             double* out_energy, size_t* n_atoms_out);
 
 #define DNAB_EVAL_FORCES 1u    /* also compute per-atom forces */
 #define DNAB_EVAL_BREAKDOWN 2u /* energy-only, the reference's five terms kept

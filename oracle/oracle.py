@@ -362,3 +362,51 @@ def langevin_step(x, y, z, res_off, res_ca, params: dict, forces=None, ag_ca=Non
                          _p(fy, C.c_float), _p(fz, C.c_float), agp, _p(rq, C.c_float),
                          _p(re, C.c_float), _p(rs, C.c_float), _p(rh, C.c_uint8), res_id0,
                          C.byref(p))
+
+
+# ---- affinity maturation: the greedy scan restated literally ----
+
+_ATOM_FIELDS = ("x", "y", "z", "charge", "r_min_half", "epsilon", "hydrophobic", "atom_type")
+
+
+def concat_residues(residues):
+    """One atom cloud of per-residue clouds, in residue order."""
+    from types import SimpleNamespace
+    return SimpleNamespace(**{k: np.concatenate([np.asarray(getattr(r, k)) for r in residues])
+                              for k in _ATOM_FIELDS})
+
+
+def allatom_energy_f64(ag, residues, aa, cutoff=10.0):
+    """allatom_energy (src/diffusion.rs:281-283) in FP64: the five pair terms
+    in total() order plus the disulfide wells over the SG atom (slot 5) of each
+    Cys residue, in residue order."""
+    t, _ = energy_full_f64(ag, concat_residues(residues), cutoff)
+    sg = [(r.x[CYS_SG_SLOT], r.y[CYS_SG_SLOT], r.z[CYS_SG_SLOT])
+          for r, a in zip(residues, aa) if a == CYS and len(r.x) > CYS_SG_SLOT]
+    xs, ys, zs = (np.array([p[k] for p in sg], np.float32) for k in range(3))
+    return ((((t[0] + t[1]) + t[2]) + t[3]) + t[4]) + disulfide_f64(xs, ys, zs)
+
+
+def affinity_maturation_f64(ag, residues, aa, rot, ca, mutable, build_residue, cutoff=10.0,
+                            trace=None):
+    """affinity_maturation (src/diffusion.rs:924-961) with every trial scored
+    by a full FP64 allatom_energy. residues (per-residue atom clouds), aa and
+    rot (rotamer index per residue) are lists updated in place; ca[r] is
+    residue r's Calpha; build_residue(aa, rot, ca) gives the atoms
+    mutate_residue places (src/allatom.rs:299-384). Returns the best score;
+    trace (a list) receives (position, type, score, best) of every trial."""
+    best = allatom_energy_f64(ag, residues, aa, cutoff)
+    for r in mutable:
+        old, old_rot = aa[r], rot[r]
+        for cand in range(20):                     # ALL_AA order, src/atom.rs:29-50
+            if cand == old:
+                continue
+            residues[r], aa[r], rot[r] = build_residue(cand, 0, ca[r]), cand, 0
+            e = allatom_energy_f64(ag, residues, aa, cutoff)
+            if trace is not None:
+                trace.append((r, cand, e, best))
+            if e < best:                           # first strict improvement is kept
+                best = e
+                break
+            residues[r], aa[r], rot[r] = build_residue(old, old_rot, ca[r]), old, old_rot
+    return best

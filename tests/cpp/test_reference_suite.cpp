@@ -159,6 +159,22 @@ int main() {
     CHECK(b.solvation != 0.0);   // typed atoms on both sides
   });
 
+  run("diffusion::affinity_maturation_never_worsens_the_score (src/diffusion.rs:1157)", [&] {
+    auto chain = [](float x0, int n) {
+      std::vector<float> xs, ys(n, 0.0f), zs(n, 0.0f);
+      for (int i = 0; i < n; ++i) xs.push_back(x0 + i * 3.8f);
+      return protein_from_ca_trace(xs, ys, zs, std::vector<AminoAcid>(n, AminoAcid::Ala));
+    };
+    auto ag = chain(0.0f, 6);
+    auto ab = chain(8.0f, 6);
+    std::vector<uint32_t> mutable_res = {0, 1, 2, 3, 4, 5};
+    const double before = allatom_energy(gpu, ag, ab);
+    const double after = affinity_maturation_allatom(gpu, ab, ag, mutable_res);
+    CHECK(after <= before + 1e-3);
+    CHECK(std::fabs(after - allatom_energy(gpu, ag, ab)) < 1e-3 * (1 + std::fabs(after)));
+    CHECK(ab.n_residues() == 6 && ab.res_start.back() == ab.n_atoms());
+  });
+
   run("gpu::score_batch uniform candidates (src/gpu.rs:166) vs the breakdown's shader terms", [&] {
     auto ag = protein_from_ca_trace({0.0f, 3.8f, 7.6f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f},
                                     {AminoAcid::Asp, AminoAcid::Lys, AminoAcid::Leu});

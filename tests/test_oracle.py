@@ -416,3 +416,82 @@ def test_full_energy_f32_follows_f64_and_sums():
                 sv += O.pair_solvation_f64(eef[ta], eef[tb], r2)
     assert abs(e64[3] - hb) <= 1e-9 * (1 + abs(hb)), (e64[3], hb)
     assert abs(e64[4] - sv) <= 1e-9 * (1 + abs(sv)), (e64[4], sv)
+
+
+# ---- affinity maturation (src/diffusion.rs:924-961) ----
+
+def _residues(p):
+    rs = p.res_start
+    return [p.atoms.subset(slice(int(rs[r]), int(rs[r + 1]))) for r in range(len(p.amino_acid))]
+
+
+def _build(aa, rot, ca):
+    return protein_from_ca_trace([ca[0]], [ca[1]], [ca[2]], np.array([aa], np.uint8),
+                                 np.array([rot], np.uint8)).atoms
+
+
+def _state(p):
+    ca = [np.array([p.atoms.x[i], p.atoms.y[i], p.atoms.z[i]], np.float32) for i in p.ca_atom_idx]
+    return _residues(p), [int(a) for a in p.amino_acid], [0] * len(p.amino_acid), ca
+
+
+def _straight(start, n):
+    # straight_chain_at (src/diffusion.rs:1116-1122): Ala every 3.8 A along x
+    xs = [start[0] + i * np.float32(3.8) for i in range(n)]
+    return protein_from_ca_trace(xs, [start[1]] * n, [start[2]] * n, A("A" * n))
+
+
+def test_affinity_maturation_never_worsens_the_score():
+    # affinity_maturation_never_worsens_the_score (src/diffusion.rs:1157-1172)
+    ag = _straight([0.0, 0.0, 0.0], 6)
+    ab = _straight([8.0, 0.0, 0.0], 6)
+    res, aa, rot, ca = _state(ab)
+    before = O.allatom_energy_f64(ag.atoms, res, aa)
+    after = O.affinity_maturation_f64(ag.atoms, res, aa, rot, ca, range(6), _build)
+    assert after <= before + 1e-3
+    assert abs(after - O.allatom_energy_f64(ag.atoms, res, aa)) < 1e-3
+    assert aa != [0] * 6                       # some mutation was accepted
+
+
+def test_affinity_maturation_single_position_by_enumeration():
+    # One position: the trials all see the other residues unchanged, so the
+    # kept type is the first in ALL_AA order whose full score is strictly
+    # below the start; with none, the undo leaves the old type rebuilt on an
+    # ideal backbone (mutate_residue, src/allatom.rs:299-384), not the input
+    # atoms.
+    ag = protein_from_ca_trace([0.0, 3.8, 7.6, 11.4], [0.0] * 4, [0.0] * 4, A("DKSE"))
+    ab = protein_from_ca_trace([1.0, 4.8, 8.6, 12.4], [6.0] * 4, [0.0] * 4, A("RLNT"))
+    rng = np.random.default_rng(5)
+    for k in "xyz":                           # off-ideal input geometry
+        getattr(ab.atoms, k)[:] += rng.uniform(-0.3, 0.3, len(ab.atoms)).astype(np.float32)
+    for r, far in ((1, False), (2, False)):
+        res, aa, rot, ca = _state(ab)
+        e0 = O.allatom_energy_f64(ag.atoms, res, aa)
+        trials = []
+        for cand in range(20):
+            if cand == aa[r]:
+                continue
+            t = list(res)
+            t[r] = _build(cand, 0, ca[r])
+            ta = list(aa)
+            ta[r] = cand
+            trials.append((cand, O.allatom_energy_f64(ag.atoms, t, ta)))
+        better = [(c, e) for c, e in trials if e < e0]
+        got = O.affinity_maturation_f64(ag.atoms, res, aa, rot, ca, [r], _build)
+        if better:
+            assert aa[r] == better[0][0] and got == better[0][1]
+        else:
+            assert got == e0 and aa[r] == int(ab.amino_acid[r])
+            ideal = _build(aa[r], 0, ca[r])
+            assert np.array_equal(res[r].x, ideal.x)
+    # a residue out of every cutoff: all trials tie with the start, none is
+    # kept, and the residue comes back rebuilt (its jittered atoms are gone)
+    far = protein_from_ca_trace([40.0], [0.0], [0.0], A("L"))
+    far.atoms.x[4:] += np.float32(0.2)        # side chain off its ideal place, Calpha kept
+    res, aa, rot, ca = _state(far)
+    e0 = O.allatom_energy_f64(ag.atoms, res, aa)
+    assert e0 == 0.0
+    assert O.affinity_maturation_f64(ag.atoms, res, aa, rot, ca, [0], _build) == 0.0
+    assert aa == [int(far.amino_acid[0])]
+    assert np.array_equal(res[0].x, _build(aa[0], 0, ca[0]).x)
+    assert not np.array_equal(res[0].x, far.atoms.x)
