@@ -168,6 +168,107 @@ def saturated_trajectory_rate(ctx, n_cand, steps=10):
             "frac": achieved / PEAK_FP32_TFLOPS, "pairs_per_s": n_in / (ms * 1e-3)}
 
 
+def interface_positions(ab, ag):
+    """is_interface (src/filters.rs:96-105): Calpha within 8 A of an antigen Calpha."""
+    import numpy as np
+    p, q = ab.ca_pos(), ag.ca_pos()
+    d = p[:, None, :] - q[None, :, :]
+    r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    return [int(r) for r in np.nonzero((r2 < np.float32(64.0)).any(axis=1))[0]]
+
+
+def maturation_bench(ctx, w, ag_dev, coords, reps=5, cpu_budget_s=4.0):
+    """Batched affinity maturation (dnab_affinity_maturation) of every binder
+    the loop refined, at its interface positions, as src/main.rs:341-347 runs
+    it per result. Timed: the C call alone, inputs packed beforehand. The CPU
+    baseline restates the reference's cost: one f32 allatom_energy (the
+    oracle's C restatement of interaction_energy_atoms_breakdown plus the
+    disulfide term) per trial the reference's scan makes."""
+    import numpy as np
+    from dnab.api import MaturationCall
+    from dnab.structure import AtomProtein, population_from_proteins
+    p = w.pop
+    off = p.cand_atom_off()
+    x, y, z = coords
+    ag = w.antigens[0]
+    abs_, muts = [], []
+    for c in range(p.n_cand):
+        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
+        cl = p.candidate_cloud(c)
+        a0, a1 = int(off[c]), int(off[c + 1])
+        cl.x, cl.y, cl.z = x[a0:a1].copy(), y[a0:a1].copy(), z[a0:a1].copy()
+        prot = AtomProtein(cl, (p.res_atom_off[r0:r1 + 1] - off[c]).astype(np.uint32),
+                           p.res_amino_acid[r0:r1].copy(),
+                           (p.res_ca_atom[r0:r1] - off[c]).astype(np.uint32))
+        abs_.append(prot)
+        muts.append(interface_positions(prot, ag))
+    npos = sum(len(m) for m in muts)
+    call = MaturationCall(ctx, ag_dev, population_from_proteins(abs_), muts)
+    call.run()                                     # warm-up
+    phases = ("maturation_host_build", "maturation_host_upload", "maturation_host_device",
+              "maturation_host_assemble", "pair_breakdown", "maturation_scan",
+              "population_create_pack", "population_create_upload", "population_create_sort")
+    ctx.set_timing(True)
+    k0 = {n: ctx.kernel_time(n) for n in phases}
+    times = []
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        call.run()
+        times.append(time.perf_counter() - t0)
+    k1 = {n: ctx.kernel_time(n) for n in phases}
+    ctx.set_timing(False)
+    dt = float(np.median(times))
+    prots, e, _, _ = call.results()
+    # trials the reference's scan makes: up to the first improving type, else 19
+    trials = 0
+    for a, m in enumerate(muts):
+        old, new = abs_[a].amino_acid, prots[a].amino_acid
+        for r in m:
+            if new[r] != old[r]:
+                trials += [t for t in range(20) if t != old[r]].index(int(new[r])) + 1
+            else:
+                trials += 19
+    out = {
+        "workload": ("affinity maturation (greedy point-mutation scan scored by allatom_energy) "
+                     "of the 1,024 binders after the 200-step loop, at their interface positions"),
+        "antibodies": len(abs_),
+        "antibodies_with_positions": sum(1 for m in muts if m),
+        "positions": npos,
+        "residues_scored": int(call.host.n_res + 20 * npos),
+        "reference_trials": trials,
+        "ms_per_call": dt * 1e3,
+        "value": npos / dt,
+        "unit": "positions/s",
+        "phase_ms": {n: (k1[n][0] - k0[n][0]) / reps for n in phases},
+        "mutations_kept": int(sum(int((a.amino_acid != b.amino_acid).sum())
+                                  for a, b in zip(abs_, prots))),
+    }
+    # CPU baseline: f32 allatom_energy per trial, one thread, on a bounded sample
+    from oracle import oracle as O
+    sample = [a for a, m in zip(abs_, muts) if m] or abs_[:1]
+    n_sc, t0 = 0, time.perf_counter()
+    while True:
+        ab = sample[n_sc % len(sample)]
+        O.energy_full_f32(ag.atoms, ab.atoms, w.cutoff)
+        sg = O.sg_positions(ab.atoms.x, ab.atoms.y, ab.atoms.z, ab.res_start, ab.amino_acid,
+                            0, len(ab.amino_acid))
+        O.disulfide_f32(*sg)
+        n_sc += 1
+        el = time.perf_counter() - t0
+        if el >= cpu_budget_s or n_sc >= 200000:
+            break
+    t_score = el / n_sc
+    cpu_val = npos / (trials * t_score) if trials else None
+    out["cpu_baseline"] = {
+        "value": cpu_val, "unit": "positions/s", "cores": 1, "kind": "port",
+        "sample": (f"{n_sc} f32 allatom_energy scorings of the workload's antibodies with "
+                   f"interface positions in {el:.1f} s ({t_score * 1e3:.3f} ms each), times the "
+                   f"{trials} trials the reference's scan makes")}
+    out["speedup_vs_cpu"] = (out["value"] / cpu_val) if cpu_val else None
+    return out
+
+
 def prescreen_bench(ctx, args):
     """Learned prescreen at the config-5 shape: train on exact energies of
     shell binders around antigens of 100..1,000 residues, then screen a pool
@@ -244,6 +345,7 @@ def main():
     ap.add_argument("--no-prescreen", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-saturated", action="store_true")
+    ap.add_argument("--no-maturation", action="store_true")
     ap.add_argument("--tuning", default="",
                     help="fixed pair work split 'survivors,chunks,waves' instead of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -402,6 +504,11 @@ def main():
             "flop_per_pair": FLOP_PER_PAIR_EF,
         },
     }
+    if not args.no_maturation:
+        log("[bench] batched affinity maturation of the refined binders")
+        out["maturation"] = maturation_bench(ctx, w, ags[0], (x1, y1, z1))
+        log(f"[bench] maturation: {out['maturation']['ms_per_call']:.2f} ms per call, "
+            f"{out['maturation']['positions']} positions")
     if not args.no_saturated:
         log("[bench] kernels with enough work (16,384 binders in contact)")
         out["roofline"]["saturated"] = {

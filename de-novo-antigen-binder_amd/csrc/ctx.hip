@@ -6,6 +6,7 @@
 // DNAB_ERR_NO_DEVICE and leaves the fallback decision to the caller.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 
@@ -25,16 +26,75 @@ int dnab_set_error(dnab_ctx* ctx, int code, const char* fmt, ...) {
 
 namespace dnab {
 
+// Host<->device copies. A copy from pageable memory is staged by the runtime
+// through its own buffer, synchronously and with a large fixed cost per call;
+// copies here go through pinned buffers of the context instead. An upload
+// copies the source into the next slot of a pinned ring (so the caller may
+// reuse its buffer at once, as before) and queues the DMA; the ring restarts
+// only after the event recorded behind the last queued copy. Very large
+// copies, and copies while the stream is being captured into a graph (whose
+// nodes would keep pointing at the ring), go direct.
+constexpr size_t kStageMax = size_t(64) << 20;
+constexpr size_t kStageAlign = 256;
+
+static bool stream_capturing(dnab_ctx* ctx) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+static bool ensure_pinned(char*& buf, size_t& cap, size_t need) {
+  if (need <= cap) return true;
+  if (buf) (void)hipHostFree(buf);
+  buf = nullptr;
+  cap = 0;
+  size_t sz = std::max<size_t>(need, size_t(1) << 20);
+  if (hipHostMalloc(reinterpret_cast<void**>(&buf), sz, hipHostMallocDefault) != hipSuccess) {
+    buf = nullptr;
+    return false;
+  }
+  cap = sz;
+  return true;
+}
+
 int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return DNAB_OK;
-  DNAB_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  const size_t need = (bytes + kStageAlign - 1) / kStageAlign * kStageAlign;
+  bool staged = bytes <= kStageMax && !stream_capturing(ctx);
+  if (staged && !ctx->stage_event &&
+      hipEventCreateWithFlags(&ctx->stage_event, hipEventDisableTiming) != hipSuccess) {
+    ctx->stage_event = nullptr;
+    staged = false;
+  }
+  if (staged && ctx->stage_off + need > ctx->stage_cap) {
+    // every copy queued from the ring so far has finished reading it
+    DNAB_HIP_CHECK(ctx, hipEventSynchronize(ctx->stage_event));
+    ctx->stage_off = 0;
+    if (need > ctx->stage_cap)  // grow (to at most twice the largest staged copy)
+      staged = ensure_pinned(ctx->stage, ctx->stage_cap,
+                             std::max(need, std::min(2 * ctx->stage_cap, kStageMax)));
+  }
+  if (!staged) {
+    DNAB_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return DNAB_OK;
+  }
+  char* slot = ctx->stage + ctx->stage_off;
+  std::memcpy(slot, src, bytes);
+  DNAB_HIP_CHECK(ctx, hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, ctx->stream));
+  DNAB_HIP_CHECK(ctx, hipEventRecord(ctx->stage_event, ctx->stream));
+  ctx->stage_off += need;
   return DNAB_OK;
 }
 
 int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return DNAB_OK;
-  DNAB_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (bytes > kStageMax || stream_capturing(ctx) || !ensure_pinned(ctx->dl_stage, ctx->dl_cap, bytes)) {
+    DNAB_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return DNAB_OK;
+  }
+  DNAB_HIP_CHECK(ctx, hipMemcpyAsync(ctx->dl_stage, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(dst, ctx->dl_stage, bytes);
   return DNAB_OK;
 }
 
@@ -75,6 +135,22 @@ void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end
   ctx->timers.push_back(std::move(t));
 }
 
+void timer_host(dnab_ctx* ctx, const char* name, double ms) {
+  if (!ctx->timing) return;
+  for (auto& t : ctx->timers) {
+    if (t.name == name) {
+      t.total_ms += ms;
+      t.launches += 1;
+      return;
+    }
+  }
+  KernelTimer t;
+  t.name = name;
+  t.total_ms = ms;
+  t.launches = 1;
+  ctx->timers.push_back(std::move(t));
+}
+
 dnab_ctx* ctx_retain(dnab_ctx* ctx) {
   if (ctx) ++ctx->refs;
   return ctx;
@@ -90,6 +166,9 @@ void ctx_release(dnab_ctx* ctx) {
       (void)hipEventDestroy(pr.second);
     }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->stage_event) (void)hipEventDestroy(ctx->stage_event);
+  if (ctx->stage) (void)hipHostFree(ctx->stage);
+  if (ctx->dl_stage) (void)hipHostFree(ctx->dl_stage);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

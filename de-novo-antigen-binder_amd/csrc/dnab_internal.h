@@ -196,6 +196,13 @@ struct dnab_ctx {
   bool timing = false;
   std::vector<dnab::KernelTimer> timers;
   std::vector<hipEvent_t> event_pool;
+  // pinned staging for host<->device copies (ctx.hip): uploads fill a ring
+  // whose slots are reused behind stage_event; downloads land in dl_stage
+  char* stage = nullptr;
+  size_t stage_cap = 0, stage_off = 0;
+  hipEvent_t stage_event = nullptr;
+  char* dl_stage = nullptr;
+  size_t dl_cap = 0;
 };
 
 struct dnab_grid {
@@ -336,6 +343,7 @@ dnab_ctx* ctx_retain(dnab_ctx* ctx);   // an object made on ctx keeps it alive
 void ctx_release(dnab_ctx* ctx);       // the last release frees the stream and ctx
 hipEvent_t timer_begin(dnab_ctx* ctx, const char* name, hipEvent_t* end_out);
 void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end);
+void timer_host(dnab_ctx* ctx, const char* name, double ms);  // host-clock phase, when timing
 int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 // step.hip: multi-pass top-k (ascending, NaN last, ties by index)

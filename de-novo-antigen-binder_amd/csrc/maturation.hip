@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -40,10 +41,6 @@ namespace {
 // with its rotamer, slots 1..19 the other types in ALL_AA order, rotamer 0.
 constexpr uint32_t kSlots = kNumAminoAcids;
 
-struct AgGuard {
-  dnab_antigen* p = nullptr;
-  ~AgGuard() { dnab_antigen_destroy(p); }
-};
 struct PopGuard {
   dnab_population* p = nullptr;
   ~PopGuard() { dnab_population_destroy(p); }
@@ -145,6 +142,10 @@ k_maturation_scan(const uint32_t* __restrict__ ab_res_off, const uint32_t* __res
   energy[a] = best;
 }
 
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 template <class F>
 void parallel_for(size_t n, F fn) {
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -169,18 +170,16 @@ using namespace dnab;
 
 extern "C" {
 
-int dnab_affinity_maturation(dnab_ctx* ctx, const dnab_atom_cloud* antigen, float cutoff,
+int dnab_affinity_maturation(dnab_ctx* ctx, dnab_antigen* antigen,
                              const dnab_population_desc* ab, const uint8_t* res_rotamer,
                              const uint32_t* mutable_off, const uint32_t* mutable_res,
                              dnab_protein_out* out, uint8_t* out_rotamer, uint8_t* out_rebuilt,
                              double* out_energy, size_t* n_atoms_out) {
-  if (!ctx || !antigen || !ab || !mutable_off) return DNAB_ERR_INVALID;
+  if (!ctx || !antigen || !ab || !mutable_off || antigen->ctx != ctx) return DNAB_ERR_INVALID;
   const size_t n_ab = ab->n_cand, n_res = ab->n_res, n_in = ab->atoms.n;
   if (n_ab && (!ab->cand_res_off || !out_energy)) return DNAB_ERR_INVALID;
   if (n_res && (!ab->res_atom_off || !ab->res_ca_atom || !ab->res_amino_acid))
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "maturation: residues need offsets, Calpha and types");
-  if (!(cutoff > 0.0f) || !std::isfinite(cutoff))
-    return dnab_set_error(ctx, DNAB_ERR_INVALID, "cutoff must be positive");
   if (n_ab >= 0xFFFFFF00ull || n_res >= 0xFFFFFF00ull)
     return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "maturation: too large");
   if (n_ab && (ab->cand_res_off[0] != 0 || ab->cand_res_off[n_ab] != n_res))
@@ -226,6 +225,7 @@ int dnab_affinity_maturation(dnab_ctx* ctx, const dnab_atom_cloud* antigen, floa
     return DNAB_ERR_INVALID;
 
   // ---- the scoring batch: input residues, then kSlots built residues per position
+  auto t0 = std::chrono::steady_clock::now();
   const size_t nc = n_res + kSlots * n_mut;
   std::vector<uint8_t> c_aa(nc), c_rot(nc, 0);
   std::vector<uint32_t> c_atom_off(nc + 1, 0);
@@ -292,14 +292,10 @@ int dnab_affinity_maturation(dnab_ctx* ctx, const dnab_atom_cloud* antigen, floa
   });
   if (status.load() != DNAB_OK)
     return dnab_set_error(ctx, status.load(), "maturation: residue build failed");
+  timer_host(ctx, "maturation_host_build", ms_since(t0));
+  t0 = std::chrono::steady_clock::now();
 
-  AgGuard ag;
   PopGuard pop;
-  dnab_antigen_desc ad{};
-  ad.atoms = *antigen;
-  ad.cell_size = cutoff;
-  ad.cutoff = cutoff;
-  DNAB_RETURN_IF(dnab_antigen_create(ctx, &ad, &ag.p));
   dnab_population_desc pd{};
   pd.n_cand = nc;
   pd.n_res = nc;
@@ -317,8 +313,11 @@ int dnab_affinity_maturation(dnab_ctx* ctx, const dnab_atom_cloud* antigen, floa
   pd.atoms.epsilon = eps.data();
   pd.atoms.hydrophobic = hyd.data();
   pd.atoms.atom_type = typed ? typ.data() : nullptr;
-  dnab_antigen* ags[1] = {ag.p};
+  dnab_antigen* ags[1] = {antigen};
   DNAB_RETURN_IF(dnab_population_create(ctx, ags, 1, &pd, &pop.p));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  timer_host(ctx, "maturation_host_upload", ms_since(t0));
+  t0 = std::chrono::steady_clock::now();
   DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, DNAB_EVAL_BREAKDOWN));
 
   // ---- the scan
@@ -344,6 +343,8 @@ int dnab_affinity_maturation(dnab_ctx* ctx, const dnab_atom_cloud* antigen, floa
   std::vector<uint32_t> state(n_res);
   if (n_res) DNAB_RETURN_IF(download(ctx, state.data(), d_state.p, n_res * sizeof(uint32_t)));
   DNAB_RETURN_IF(download(ctx, out_energy, d_energy.p, n_ab * sizeof(double)));
+  timer_host(ctx, "maturation_host_device", ms_since(t0));
+  t0 = std::chrono::steady_clock::now();
 
   // ---- the final proteins: each residue's input atoms or its chosen built set
   size_t n_out = 0;
@@ -379,6 +380,7 @@ int dnab_affinity_maturation(dnab_ctx* ctx, const dnab_atom_cloud* antigen, floa
       if (out->res_start) out->res_start[g + 1] = static_cast<uint32_t>(o);
     }
   }
+  timer_host(ctx, "maturation_host_assemble", ms_since(t0));
   return DNAB_OK;
 }
 

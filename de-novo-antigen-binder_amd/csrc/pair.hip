@@ -30,6 +30,8 @@
 //    with DPP wave reductions.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -53,6 +55,36 @@ __global__ void k_fill_u32b(uint32_t* p, uint64_t n, uint32_t v) {
   uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
+}
+
+// Packs the per-atom SoA (x, y, z, charge, r_min_half, epsilon as float
+// planes of stride nf, then hydrophobic and atom type as byte planes) into
+// pos = (x, y, z, q) and par = (r_min_half, epsilon, hydrophobic, type or -1),
+// and finds each atom's residue by binary search in res_atom_off, with its
+// candidate and antigen.
+__global__ void __launch_bounds__(256)
+k_pack_atoms(const float* __restrict__ soa, size_t nf, int typed, uint32_t na, uint32_t nr,
+             const uint32_t* __restrict__ res_atom_off, const uint32_t* __restrict__ res_cand,
+             const uint32_t* __restrict__ cand_ag, float4* __restrict__ pos,
+             float4* __restrict__ par, uint32_t* __restrict__ atom_ag,
+             uint32_t* __restrict__ atom_cand, uint32_t* __restrict__ atom_res) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  const uint8_t* bytes = reinterpret_cast<const uint8_t*>(soa + 6 * nf);
+  pos[i] = make_float4(soa[i], soa[nf + i], soa[2 * nf + i], soa[3 * nf + i]);
+  par[i] = make_float4(soa[4 * nf + i], soa[5 * nf + i], bytes[i] ? 1.0f : 0.0f,
+                       typed ? static_cast<float>(bytes[nf + i]) : -1.0f);
+  // last residue r with res_atom_off[r] <= i (residues are non-empty)
+  uint32_t lo = 0, hi = nr;  // res_atom_off[lo] <= i < res_atom_off[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (res_atom_off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t c = res_cand[lo];
+  atom_res[i] = lo;
+  atom_cand[i] = c;
+  atom_ag[i] = cand_ag[c];
 }
 
 static inline uint32_t nblocks(uint64_t n, uint32_t t) {
@@ -1385,6 +1417,10 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
     if (!std::isfinite(d->atoms.x[i]) || !std::isfinite(d->atoms.y[i]) ||
         !std::isfinite(d->atoms.z[i]))
       return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: non-finite coordinate");
+  if (d->atoms.atom_type)
+    for (size_t i = 0; i < na; ++i)
+      if (d->atoms.atom_type[i] >= kNumAtomTypesDev)
+        return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: bad atom type");
   (void)hipSetDevice(ctx->device);
   dnab_population* pop = new (std::nothrow) dnab_population();
   if (!pop) return DNAB_ERR_ALLOC;
@@ -1398,30 +1434,20 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
     dnab_population_destroy(pop);
     return rc;
   };
-  // host staging
-  std::vector<float4> hpos(na), hpar(na);
-  std::vector<uint32_t> hatom_ag(na), hatom_cand(na), hatom_res(na), hcand_atom_off(nc + 1),
-      hres_cand(nr);
+  auto t_phase = std::chrono::steady_clock::now();
+  auto phase = [&](const char* name) {
+    const auto now = std::chrono::steady_clock::now();
+    timer_host(ctx, name, std::chrono::duration<double, std::milli>(now - t_phase).count());
+    t_phase = now;
+  };
+  // host staging of the per-candidate and per-residue tables; the per-atom
+  // arrays are packed on the device from the caller's SoA (k_pack_atoms)
+  std::vector<uint32_t> hcand_atom_off(nc + 1), hres_cand(nr);
   std::vector<float4> hres_cg(nr);
-  for (size_t i = 0; i < na; ++i) {
-    hpos[i] = make_float4(d->atoms.x[i], d->atoms.y[i], d->atoms.z[i], d->atoms.charge[i]);
-    if (d->atoms.atom_type && d->atoms.atom_type[i] >= kNumAtomTypesDev)
-      return fail(dnab_set_error(ctx, DNAB_ERR_INVALID, "population: bad atom type"));
-    hpar[i] = make_float4(d->atoms.r_min_half[i], d->atoms.epsilon[i],
-                          d->atoms.hydrophobic[i] ? 1.0f : 0.0f,
-                          d->atoms.atom_type ? static_cast<float>(d->atoms.atom_type[i]) : -1.0f);
-  }
   for (size_t c = 0; c < nc; ++c) {
     uint32_t r0 = d->cand_res_off[c], r1 = d->cand_res_off[c + 1];
     hcand_atom_off[c] = nr ? d->res_atom_off[r0] : 0;
-    for (uint32_t r = r0; r < r1; ++r) {
-      hres_cand[r] = static_cast<uint32_t>(c);
-      for (uint32_t k = d->res_atom_off[r]; k < d->res_atom_off[r + 1]; ++k) {
-        hatom_ag[k] = d->cand_antigen[c];
-        hatom_cand[k] = static_cast<uint32_t>(c);
-        hatom_res[k] = r;
-      }
-    }
+    for (uint32_t r = r0; r < r1; ++r) hres_cand[r] = static_cast<uint32_t>(c);
   }
   if (nc) hcand_atom_off[nc] = static_cast<uint32_t>(na);
   else hcand_atom_off[0] = 0;
@@ -1504,12 +1530,9 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   // every non-empty 16 A brick adds at most one partly filled tile
   pop->tile_cap = (na + kWave - 1) / kWave +
                   std::min<size_t>(na, static_cast<size_t>(pop->sort_codes / kWave)) + 1;
+  phase("population_create_pack");
   int rc = DNAB_OK;
   if ((rc = up(ctx, pop->views, pop->views_host.data(), n_antigens)) ||
-      (rc = up(ctx, pop->pos, hpos.data(), na)) || (rc = up(ctx, pop->par, hpar.data(), na)) ||
-      (rc = up(ctx, pop->atom_ag, hatom_ag.data(), na)) ||
-      (rc = up(ctx, pop->atom_cand, hatom_cand.data(), na)) ||
-      (rc = up(ctx, pop->atom_res, hatom_res.data(), na)) ||
       (rc = up(ctx, pop->cand_atom_off, hcand_atom_off.data(), nc + 1)) ||
       (rc = up(ctx, pop->res_cand, hres_cand.data(), nr)) ||
       (rc = up(ctx, pop->res_cg, hres_cg.data(), nr)))
@@ -1520,6 +1543,34 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   if (nr && ((rc = up(ctx, pop->res_atom_off, d->res_atom_off, nr + 1)) ||
              (rc = up(ctx, pop->res_ca, d->res_ca_atom, nr))))
     return fail(rc);
+  {
+    // the caller's per-atom SoA, 26 bytes per atom, packed into pos / par
+    // and the per-atom owner indices on the device
+    const size_t nf = (na + 63) / 64 * 64;
+    DevBuf<float> soa;
+    if (soa.alloc(6 * nf + (2 * nf + 3) / 4) != hipSuccess || pop->pos.alloc(na) != hipSuccess ||
+        pop->par.alloc(na) != hipSuccess || pop->atom_ag.alloc(na) != hipSuccess ||
+        pop->atom_cand.alloc(na) != hipSuccess || pop->atom_res.alloc(na) != hipSuccess)
+      return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
+    const float* src[6] = {d->atoms.x, d->atoms.y, d->atoms.z, d->atoms.charge,
+                           d->atoms.r_min_half, d->atoms.epsilon};
+    for (int k = 0; k < 6; ++k)
+      if ((rc = upload(ctx, soa.p + k * nf, src[k], na * sizeof(float)))) return fail(rc);
+    uint8_t* bytes = reinterpret_cast<uint8_t*>(soa.p + 6 * nf);
+    if ((rc = upload(ctx, bytes, d->atoms.hydrophobic, na))) return fail(rc);
+    if (d->atoms.atom_type && (rc = upload(ctx, bytes + nf, d->atoms.atom_type, na)))
+      return fail(rc);
+    if (na) {
+      hipLaunchKernelGGL(k_pack_atoms, dim3(nblocks(na, 256)), dim3(256), 0, ctx->stream,
+                         soa.p, nf, d->atoms.atom_type ? 1 : 0, static_cast<uint32_t>(na),
+                         static_cast<uint32_t>(nr), pop->res_atom_off.p, pop->res_cand.p,
+                         pop->cand_ag.p, pop->pos.p, pop->par.p, pop->atom_ag.p,
+                         pop->atom_cand.p, pop->atom_res.p);
+      DNAB_HIP_CHECK(ctx, hipGetLastError());
+    }
+    // the scratch is freed at the end of this block: order it behind the kernel
+    DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  }
   if (nr && d->res_amino_acid) {  // types checked above (< 20)
     if ((rc = up(ctx, pop->res_aa, d->res_amino_acid, nr))) return fail(rc);
     pop->has_res_aa = true;
@@ -1555,9 +1606,11 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
         hipMemsetAsync(pop->ho_list.p, 0, (pop->tile_cap + 4) * sizeof(uint32_t), ctx->stream) != hipSuccess)
       return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "population: clearing the handover table failed"));
   }
+  phase("population_create_upload");
   if ((rc = population_sort(ctx, pop))) return fail(rc);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "population: synchronize failed"));
+  phase("population_create_sort");
   *out = pop;
   return DNAB_OK;
 }

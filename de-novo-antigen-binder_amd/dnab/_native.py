@@ -221,8 +221,7 @@ _PROTOS = {
     "dnab_population_read_terms": (C.c_int, [vp, vp, f64p]),
     "dnab_allatom_energy": (C.c_int, [vp, C.POINTER(AtomCloudC), C.POINTER(AtomCloudC), u32p,
                                       u32p, u8p, C.c_size_t, C.c_float, f64p, f64p]),
-    "dnab_affinity_maturation": (C.c_int, [vp, C.POINTER(AtomCloudC), C.c_float,
-                                           C.POINTER(PopulationDescC), u8p, u32p, u32p,
+    "dnab_affinity_maturation": (C.c_int, [vp, vp, C.POINTER(PopulationDescC), u8p, u32p, u32p,
                                            C.POINTER(ProteinOutC), u8p, u8p, f64p,
                                            C.POINTER(C.c_size_t)]),
 }

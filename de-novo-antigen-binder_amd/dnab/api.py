@@ -150,55 +150,11 @@ class GpuContext:
         a; rotamers[a] (optional): each residue's rotamer index. Returns the
         final proteins, the scores (the reference's return value), and per
         antibody the rotamer index and rebuilt flag of every residue."""
-        ag = antigen.atoms if isinstance(antigen, AtomProtein) else antigen
         antibodies = list(antibodies)
-        n_ab = len(antibodies)
-        host = population_from_proteins(antibodies)
-        n_res = host.n_res
-        moff = np.zeros(n_ab + 1, np.uint32)
-        for a, m in enumerate(mutable):
-            moff[a + 1] = moff[a] + len(m)
-        mres = np.ascontiguousarray(np.concatenate([np.asarray(m, np.int64) for m in mutable])
-                                    if n_ab else np.zeros(0), np.uint32)
-        rot = None
-        if rotamers is not None:
-            rot = np.ascontiguousarray(np.concatenate([np.asarray(r, np.uint8) for r in rotamers]),
-                                       np.uint8)
-        cap = max(1, MAX_ATOMS_PER_RESIDUE * n_res)
-        f = {k: np.zeros(cap, np.float32) for k in ("x", "y", "z", "charge", "rmh", "eps")}
-        hyd, at = np.zeros(cap, np.uint8), np.zeros(cap, np.uint8)
-        ri = np.zeros(cap, np.uint32)
-        rs = np.zeros(n_res + 1, np.uint32)
-        ca = np.zeros(max(n_res, 1), np.uint32)
-        aa = np.zeros(max(n_res, 1), np.uint8)
-        out = N.ProteinOutC(cap, N.ptr(f["x"], C.c_float), N.ptr(f["y"], C.c_float),
-                            N.ptr(f["z"], C.c_float), N.ptr(f["charge"], C.c_float),
-                            N.ptr(f["rmh"], C.c_float), N.ptr(f["eps"], C.c_float),
-                            N.ptr(hyd, C.c_uint8), N.ptr(at, C.c_uint8), N.ptr(ri, C.c_uint32),
-                            N.ptr(rs, C.c_uint32), N.ptr(ca, C.c_uint32), N.ptr(aa, C.c_uint8))
-        rot_out = np.zeros(max(n_res, 1), np.uint8)
-        rebuilt = np.zeros(max(n_res, 1), np.uint8)
-        energy = np.zeros(max(n_ab, 1), np.float64)
-        na = C.c_size_t(0)
-        agc, desc = ag.c(), host.c()
-        self.check(N.lib.dnab_affinity_maturation(
-            self._h, C.byref(agc), cutoff, C.byref(desc), N.ptr(rot, C.c_uint8),
-            N.ptr(moff, C.c_uint32), N.ptr(mres, C.c_uint32), C.byref(out),
-            N.ptr(rot_out, C.c_uint8), N.ptr(rebuilt, C.c_uint8), N.ptr(energy, C.c_double),
-            C.byref(na)))
-        prots, rots, flags = [], [], []
-        for a in range(n_ab):
-            g0, g1 = int(host.cand_res_off[a]), int(host.cand_res_off[a + 1])
-            a0, a1 = int(rs[g0]), int(rs[g1])
-            cloud = AtomCloud(f["x"][a0:a1].copy(), f["y"][a0:a1].copy(), f["z"][a0:a1].copy(),
-                              f["charge"][a0:a1].copy(), f["rmh"][a0:a1].copy(),
-                              f["eps"][a0:a1].copy(), hyd[a0:a1].copy(), at[a0:a1].copy(),
-                              ri[a0:a1].copy())
-            prots.append(AtomProtein(cloud, (rs[g0:g1 + 1] - a0).astype(np.uint32),
-                                     aa[g0:g1].copy(), (ca[g0:g1] - a0).astype(np.uint32)))
-            rots.append(rot_out[g0:g1].copy())
-            flags.append(rebuilt[g0:g1].astype(bool))
-        return prots, energy[:n_ab], rots, flags
+        call = MaturationCall(self, antigen, population_from_proteins(antibodies), mutable,
+                              rotamers, cutoff)
+        call.run()
+        return call.results()
 
     def atom_forces(self, antigen: AtomCloud, antibody: AtomCloud, cutoff: float = 10.0):
         n = len(antibody)
@@ -349,6 +305,72 @@ class DeviceAntigen:
             self.close()
         except Exception:
             pass
+
+
+class MaturationCall:
+    """One dnab_affinity_maturation call with its inputs packed and its output
+    buffers allocated up front, so that run() is the C call alone."""
+
+    def __init__(self, ctx: "GpuContext", antigen, host: PopulationHost, mutable,
+                 rotamers=None, cutoff: float = 10.0):
+        """antigen: a DeviceAntigen (its cutoff applies), or an AtomProtein /
+        AtomCloud placed on the device here with `cutoff`."""
+        self.ctx, self.host = ctx, host
+        if not isinstance(antigen, DeviceAntigen):
+            antigen = DeviceAntigen(ctx, antigen, cutoff=cutoff, with_ca=False)
+        self.antigen = antigen
+        n_ab, n_res = host.n_cand, host.n_res
+        self.moff = np.zeros(n_ab + 1, np.uint32)
+        for a, m in enumerate(mutable):
+            self.moff[a + 1] = self.moff[a] + len(m)
+        self.mres = np.ascontiguousarray(
+            np.concatenate([np.asarray(m, np.int64) for m in mutable]) if n_ab else np.zeros(0),
+            np.uint32)
+        self.rot = None if rotamers is None else np.ascontiguousarray(
+            np.concatenate([np.asarray(r, np.uint8) for r in rotamers]), np.uint8)
+        cap = max(1, MAX_ATOMS_PER_RESIDUE * n_res)
+        self.f = {k: np.zeros(cap, np.float32) for k in ("x", "y", "z", "charge", "rmh", "eps")}
+        self.hyd, self.at = np.zeros(cap, np.uint8), np.zeros(cap, np.uint8)
+        self.ri = np.zeros(cap, np.uint32)
+        self.rs = np.zeros(n_res + 1, np.uint32)
+        self.ca = np.zeros(max(n_res, 1), np.uint32)
+        self.aa = np.zeros(max(n_res, 1), np.uint8)
+        f = self.f
+        self.out = N.ProteinOutC(cap, N.ptr(f["x"], C.c_float), N.ptr(f["y"], C.c_float),
+                                 N.ptr(f["z"], C.c_float), N.ptr(f["charge"], C.c_float),
+                                 N.ptr(f["rmh"], C.c_float), N.ptr(f["eps"], C.c_float),
+                                 N.ptr(self.hyd, C.c_uint8), N.ptr(self.at, C.c_uint8),
+                                 N.ptr(self.ri, C.c_uint32), N.ptr(self.rs, C.c_uint32),
+                                 N.ptr(self.ca, C.c_uint32), N.ptr(self.aa, C.c_uint8))
+        self.rot_out = np.zeros(max(n_res, 1), np.uint8)
+        self.rebuilt = np.zeros(max(n_res, 1), np.uint8)
+        self.energy = np.zeros(max(n_ab, 1), np.float64)
+        self.na = C.c_size_t(0)
+        self.desc = host.c()
+
+    def run(self):
+        self.ctx.check(N.lib.dnab_affinity_maturation(
+            self.ctx.handle, self.antigen.handle, C.byref(self.desc),
+            N.ptr(self.rot, C.c_uint8), N.ptr(self.moff, C.c_uint32),
+            N.ptr(self.mres, C.c_uint32), C.byref(self.out), N.ptr(self.rot_out, C.c_uint8),
+            N.ptr(self.rebuilt, C.c_uint8), N.ptr(self.energy, C.c_double), C.byref(self.na)))
+
+    def results(self):
+        host, f, rs = self.host, self.f, self.rs
+        prots, rots, flags = [], [], []
+        for a in range(host.n_cand):
+            g0, g1 = int(host.cand_res_off[a]), int(host.cand_res_off[a + 1])
+            a0, a1 = int(rs[g0]), int(rs[g1])
+            cloud = AtomCloud(f["x"][a0:a1].copy(), f["y"][a0:a1].copy(), f["z"][a0:a1].copy(),
+                              f["charge"][a0:a1].copy(), f["rmh"][a0:a1].copy(),
+                              f["eps"][a0:a1].copy(), self.hyd[a0:a1].copy(),
+                              self.at[a0:a1].copy(), self.ri[a0:a1].copy())
+            prots.append(AtomProtein(cloud, (rs[g0:g1 + 1] - a0).astype(np.uint32),
+                                     self.aa[g0:g1].copy(),
+                                     (self.ca[g0:g1] - a0).astype(np.uint32)))
+            rots.append(self.rot_out[g0:g1].copy())
+            flags.append(self.rebuilt[g0:g1].astype(bool))
+        return prots, self.energy[:host.n_cand].copy(), rots, flags
 
 
 def langevin_params(**kw) -> N.LangevinParamsC:

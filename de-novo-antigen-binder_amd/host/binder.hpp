@@ -289,7 +289,13 @@ inline double affinity_maturation_allatom(const GpuContext& gpu, AtomProtein& ab
   d.res_ca_atom = ab.ca_atom_idx.data();
   d.res_amino_acid = aa.data();
   d.atoms = ab.atoms.view();
-  dnab_atom_cloud a = ag.atoms.view();
+  dnab_antigen_desc ad{};
+  ad.atoms = ag.atoms.view();
+  ad.cell_size = cutoff;
+  ad.cutoff = cutoff;
+  dnab_antigen* dag = nullptr;
+  check(dnab_antigen_create(gpu.raw(), &ad, &dag), gpu.raw());
+  std::unique_ptr<dnab_antigen, void (*)(dnab_antigen*)> hold(dag, dnab_antigen_destroy);
   const size_t cap = 14 * nr + 1;
   AtomProtein out;
   auto& o = out.atoms;
@@ -305,7 +311,7 @@ inline double affinity_maturation_allatom(const GpuContext& gpu, AtomProtein& ab
                       out.ca_atom_idx.data(), aa_out.data()};
   double e = 0;
   size_t na = 0;
-  check(dnab_affinity_maturation(gpu.raw(), &a, cutoff, &d, nullptr, mut_off,
+  check(dnab_affinity_maturation(gpu.raw(), dag, &d, nullptr, mut_off,
                                  mutable_res.data(), &po, nullptr, nullptr, &e, &na),
         gpu.raw());
   for (auto* v : {&o.x, &o.y, &o.z, &o.charge, &o.r_min_half, &o.epsilon}) v->resize(na);

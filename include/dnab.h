@@ -276,7 +276,8 @@ typedef struct dnab_population_desc {
 int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t n_antigens,
                            const dnab_population_desc* desc, dnab_population** out);
 void dnab_population_destroy(dnab_population* pop);
-/* Affinity maturation of a batch of antibodies against one antigen:
+/* Affinity maturation of a batch of antibodies against one device antigen
+ * (its cutoff applies):
  * affinity_maturation_allatom (src/diffusion.rs:924-961), called by the
  * reference once per result (src/main.rs:341-347), here for all of them in one
  * call. Per antibody, for each mutable position r in order, the amino acids
@@ -301,7 +302,7 @@ void dnab_population_destroy(dnab_population* pop);
  * accepted (FP64); n_atoms_out the final atom count. Scores are taken from one
  * scoring batch of single residues: the trial scores equal full re-scoring to
  * FP64 rounding. */
-int dnab_affinity_maturation(dnab_ctx* ctx, const dnab_atom_cloud* antigen, float cutoff,
+int dnab_affinity_maturation(dnab_ctx* ctx, dnab_antigen* antigen,
                              const dnab_population_desc* ab, const uint8_t* res_rotamer,
                              const uint32_t* mutable_off, const uint32_t* mutable_res,
                              dnab_protein_out* out, uint8_t* out_rotamer, uint8_t* out_rebuilt,
