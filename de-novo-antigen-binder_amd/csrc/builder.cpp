@@ -15,52 +15,35 @@
 #include <vector>
 
 #include "dnab_internal.h"
+#include "geometry.h"
 #include "ff_tables.h"
 
 namespace dnab {
 
-namespace {
-
-struct V3 {
-  float x, y, z;
-};
-inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline V3 cross(V3 a, V3 b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-inline V3 normalize(V3 v) {
-  float l = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
-  if (l < 1e-8f) l = 1e-8f;
-  return {v.x / l, v.y / l, v.z / l};
-}
-inline float deg2rad(float d) { return d * (3.14159265358979323846f / 180.0f); }
-
-}  // namespace
+using geo::V3;
 
 // NERF (Parsons et al. 2005, J. Comput. Chem. 26(10)): place D so that
 // |CD| = bond_len, angle BCD = angle_deg, dihedral ABCD = dihedral_deg.
 void place_atom(const float a[3], const float b[3], const float c[3], float bond_len,
                 float angle_deg, float dihedral_deg, float out[3]) {
-  V3 A{a[0], a[1], a[2]}, B{b[0], b[1], b[2]}, C{c[0], c[1], c[2]};
-  float ang = deg2rad(angle_deg), dih = deg2rad(dihedral_deg);
-  V3 bc = normalize(sub(B, C));
-  V3 n = normalize(cross(sub(B, A), bc));
-  V3 m = cross(bc, n);
-  float sa = std::sin(ang), ca = std::cos(ang), sd = std::sin(dih), cd = std::cos(dih);
-  out[0] = C.x + bond_len * (-ca * bc.x + sa * cd * m.x + sa * sd * n.x);
-  out[1] = C.y + bond_len * (-ca * bc.y + sa * cd * m.y + sa * sd * n.y);
-  out[2] = C.z + bond_len * (-ca * bc.z + sa * cd * m.z + sa * sd * n.z);
+  const float ang = geo::deg2rad(angle_deg), dih = geo::deg2rad(dihedral_deg);
+  const V3 d = geo::place({a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]}, bond_len,
+                          std::sin(ang), std::cos(ang), std::sin(dih), std::cos(dih));
+  out[0] = d.x;
+  out[1] = d.y;
+  out[2] = d.z;
 }
 
 // Axis-aligned ideal backbone around a Calpha, src/allatom.rs:502-517.
 int ideal_backbone(const float ca[3], int aa, float bb[5][3]) {
-  const float kNCa = 1.46f, kCaC = 1.52f, kCO = 1.23f, kCaCb = 1.52f;
-  bb[0][0] = ca[0] - kNCa; bb[0][1] = ca[1]; bb[0][2] = ca[2];
-  bb[1][0] = ca[0];        bb[1][1] = ca[1]; bb[1][2] = ca[2];
-  bb[2][0] = ca[0] + kCaC; bb[2][1] = ca[1]; bb[2][2] = ca[2];
-  bb[3][0] = bb[2][0] + 0.5f; bb[3][1] = bb[2][1] + kCO * 0.9f; bb[3][2] = bb[2][2];
-  bb[4][0] = ca[0]; bb[4][1] = ca[1] - kCaCb; bb[4][2] = ca[2];
-  return aa == kGly ? 4 : 5;
+  V3 v[5];
+  const int n = geo::ideal_backbone(ca[0], ca[1], ca[2], aa == kGly, v);
+  for (int k = 0; k < 5; ++k) {
+    bb[k][0] = v[k].x;
+    bb[k][1] = v[k].y;
+    bb[k][2] = v[k].z;
+  }
+  return n;
 }
 
 // Append all heavy atoms of one residue; returns the number appended.
@@ -145,6 +128,29 @@ int dnab_build_protein(const float* ca_x, const float* ca_y, const float* ca_z,
     }
     if (out->res_start) out->res_start[r + 1] = static_cast<uint32_t>(a);
   }
+  return DNAB_OK;
+}
+
+int dnab_rebuild_side_chain(uint8_t amino_acid, uint8_t rotamer, float* xyz, size_t n_atoms) {
+  if (!xyz || amino_acid >= kNumAminoAcids) return DNAB_ERR_INVALID;
+  const int aa = amino_acid;
+  const ResTopology& topo = kResidueTopology[aa];
+  if (n_atoms != topo.n_atoms) return DNAB_ERR_INVALID;
+  if (kRotamerCount[aa] ? rotamer >= kRotamerCount[aa] : rotamer != 0) return DNAB_ERR_INVALID;
+  float chi[4] = {0, 0, 0, 0};
+  if (kRotamerCount[aa]) std::memcpy(chi, kRotamerLib[aa][rotamer].chi, sizeof(chi));
+  float atoms[16][3];
+  std::memcpy(atoms, xyz, sizeof(float) * 3 * n_atoms);
+  int n = aa == kGly ? 4 : 5;  // the backbone kept (src/allatom.rs:248)
+  const BondDef* defs = kSidechainDefs[aa];
+  for (int k = 0; k < kSidechainCount[aa] && n < topo.n_atoms; ++k) {
+    const BondDef& d = defs[k];
+    const float dih = d.chi_idx == 255 ? d.dihedral_fixed : chi[d.chi_idx];
+    place_atom(atoms[d.ref_a], atoms[d.ref_b], atoms[d.ref_c], d.bond_len, d.angle_deg, dih,
+               atoms[n]);
+    ++n;
+  }
+  std::memcpy(xyz, atoms, sizeof(float) * 3 * n_atoms);
   return DNAB_OK;
 }
 

@@ -1,0 +1,676 @@
+// All-atom refinement with Monte-Carlo moves on the device: every iteration
+// of allatom_diffusion_step (src/diffusion.rs:290-416) for a batch of
+// candidates in one launch, one wave per candidate:
+//   1. residue-level forces on each Calpha from the antigen Calpha grid
+//      (compute_forces_with_grid, src/energy.rs:322-398), the radial
+//      restraint, clamp, optional noise, set_ca_pos (the gradient step);
+//   2. the interface mask: Calpha within 8 A of an antigen Calpha;
+//   3. per residue in order, a Metropolis rotamer move or amino-acid
+//      mutation scored by residue_contribution_allatom (src/diffusion.rs:
+//      422-456) plus disulfide_energy (src/allatom.rs:155-180);
+//   4. per residue in order, a Metropolis phi/psi torsion move of the chain
+//      downstream (perturb_phi / perturb_psi, src/allatom.rs:392-456).
+// Each residue holds up to 14 atom slots in LDS in topology order, so a
+// mutation rewrites the slots of one residue instead of splicing arrays
+// (mutate_residue, src/allatom.rs:299-384). Residues are rebuilt with the
+// host builder's NERF (geometry.h) and sines/cosines the host tabulates, so
+// they match host-built ones bit for bit. Random draws are Philox4x32-10 by
+// (iteration, residue, kind); the energies of the Metropolis tests are FP64
+// sums of the reference's f32 pair terms.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_common.h"
+#include "dnab_internal.h"
+#include "ff_tables.h"
+#include "ff_terms.h"
+#include "geometry.h"
+#include "grid_device.h"
+#include "step_device.h"
+
+namespace dnab {
+namespace {
+
+constexpr int kSlots = 14;        // atom slots per residue (the largest topology, Trp)
+constexpr int kMaxDefs = 12;      // side-chain placements per residue type
+constexpr uint32_t kMaxRefineRes = 512;
+constexpr uint32_t kTagMc = 0x6d636d76u;  // draws of the side-chain moves
+constexpr uint32_t kTagBb = 0x62626d76u;  // draws of the torsion moves
+
+struct RefDef {
+  uint8_t a, b, c, chi;  // reference atoms; chi index or 255 (fixed dihedral)
+  float bond, sa, ca;    // bond length, sine and cosine of the bond angle
+  float sd, cd;          // sine and cosine of the fixed dihedral
+};
+
+// Per residue type: side-chain placements, chi sines/cosines per rotamer, the
+// Calpha atom's AMBER parameters and the residue-level (CG) parameters.
+struct RefTables {
+  RefDef defs[kNumAminoAcids][kMaxDefs];
+  uint8_t ndef[kNumAminoAcids], natoms[kNumAminoAcids], nrot[kNumAminoAcids];
+  float chi_sc[kNumAminoAcids][5][4][2];  // (sin, cos) of chi k of rotamer j
+  float ca_rmh[kNumAminoAcids], ca_eps[kNumAminoAcids], ca_q[kNumAminoAcids];
+  uint8_t ca_hyd[kNumAminoAcids], ca_type[kNumAminoAcids];
+  float cg_q[kNumAminoAcids], cg_eps[kNumAminoAcids], cg_sig[kNumAminoAcids];
+  uint8_t cg_hyd[kNumAminoAcids];
+};
+
+struct RefineArgs {
+  const RefTables* tab;
+  // antigen Calphas: positions, CA-atom parameters (rmh, eps, q, hyd + 256 type)
+  const float *ag_x, *ag_y, *ag_z;
+  const float4* ag_par;
+  uint32_t n_ag;
+  GridView ag_grid;
+  CgCloud ag_cg;
+  float cx, cy, cz, r0;
+  // candidates: residue ranges, per residue type / rotamer, 14 slots per residue
+  const uint32_t* cand_res_off;
+  uint8_t* res_aa;
+  uint8_t* res_rot;
+  float4* slots;
+  uint32_t n_cand, max_res;
+  // schedule, per step
+  const float* temp;
+  const float* sigma;
+  uint32_t n_steps, iter0;
+  StepParams sp;
+  float mut_p, rot_p, bb_p, max_torsion, iface_sq;
+  uint32_t moves;
+};
+
+__device__ inline float4 u4(uint32_t it, uint32_t r, uint32_t tag, const StepParams& sp) {
+  const Philox4 u = philox4x32_10(it, r, tag, 0u, sp.seed_lo, sp.seed_hi);
+  return make_float4(u01_from_u32(u.v[0]), u01_from_u32(u.v[1]), u01_from_u32(u.v[2]),
+                     u01_from_u32(u.v[3]));
+}
+
+// residue_contribution_allatom (src/diffusion.rs:422-456) of a Calpha of type
+// `aa` at p, lane-split over the antigen residues, FP64 total on every lane.
+__device__ inline double contribution(const RefineArgs& a, uint32_t aa, float px, float py,
+                                      float pz, uint32_t lane) {
+  const RefTables& T = *a.tab;
+  const float brm = T.ca_rmh[aa], be = T.ca_eps[aa], bq = T.ca_q[aa];
+  const uint8_t bh = T.ca_hyd[aa];
+  const uint32_t bt = T.ca_type[aa];
+  double e = 0.0;
+  for (uint32_t i = lane; i < a.n_ag; i += kWave) {
+    const float dx = px - a.ag_x[i], dy = py - a.ag_y[i], dz = pz - a.ag_z[i];
+    const float r_sq = dx * dx + dy * dy + dz * dz;
+    if (r_sq > 100.0f || r_sq < 0.25f) continue;
+    const float4 q = a.ag_par[i];
+    const float r_ij = brm + q.x;
+    const float eps = sqrtf(be * q.y);
+    const float r2 = (r_ij * r_ij) / r_sq;
+    const float r6 = r2 * r2 * r2;
+    e += static_cast<double>(eps * (r6 * r6 - 2.0f * r6));
+    if (bq != 0.0f && q.z != 0.0f) e += static_cast<double>(332.0f * bq * q.z / sqrtf(r_sq));
+    const uint32_t packed = static_cast<uint32_t>(q.w);
+    const uint32_t ah = packed & 1u, at = packed >> 8;
+    if (bh == 1 && ah == 1 && r_sq < 36.0f) e -= 0.5;
+    const bool hb = (((kHbondDonorMask >> bt) & (kHbondAcceptorMask >> at)) |
+                     ((kHbondAcceptorMask >> bt) & (kHbondDonorMask >> at))) & 1u;
+    if (hb && r_sq <= kHbCutoffSq) {
+      const float z = (sqrtf(r_sq) - kHbR0) / kHbSigma;
+      e += static_cast<double>(-kHbDepth * expf(-0.5f * z * z));
+    }
+  }
+  return wave_sum_f64(e);
+}
+
+// Side chain of residue slots X/Y/Z (type aa, rotamer rot) from its
+// backbone atoms 0..nb-1 (rebuild_side_chain, src/allatom.rs:244-271).
+__device__ inline void side_chain(const RefTables& T, uint32_t aa, uint32_t rot, int nb, float* X,
+                                  float* Y, float* Z) {
+  int n = nb;
+  for (int k = 0; k < T.ndef[aa] && n < T.natoms[aa]; ++k) {
+    const RefDef& d = T.defs[aa][k];
+    float sd = d.sd, cd = d.cd;
+    if (d.chi != 255) {
+      if (T.nrot[aa]) {
+        sd = T.chi_sc[aa][rot][d.chi][0];
+        cd = T.chi_sc[aa][rot][d.chi][1];
+      } else {
+        sd = 0.0f;
+        cd = 1.0f;
+      }
+    }
+    const geo::V3 p = geo::place({X[d.a], Y[d.a], Z[d.a]}, {X[d.b], Y[d.b], Z[d.b]},
+                                 {X[d.c], Y[d.c], Z[d.c]}, d.bond, d.sa, d.ca, sd, cd);
+    X[n] = p.x;
+    Y[n] = p.y;
+    Z[n] = p.z;
+    ++n;
+  }
+}
+
+// mutate_residue (src/allatom.rs:299-384) on one residue's slots: the same
+// type keeps the backbone and rebuilds the side chain; another type is built
+// anew on an ideal backbone around the Calpha.
+__device__ inline void mutate(const RefTables& T, uint8_t* aa, uint8_t* rot, uint32_t r,
+                              uint32_t new_aa, uint32_t new_rot, float* X, float* Y, float* Z) {
+  float* x = X + r * kSlots;
+  float* y = Y + r * kSlots;
+  float* z = Z + r * kSlots;
+  if (aa[r] == new_aa) {
+    rot[r] = static_cast<uint8_t>(new_rot);
+    side_chain(T, new_aa, new_rot, new_aa == kGly ? 4 : 5, x, y, z);
+    return;
+  }
+  geo::V3 bb[5];
+  int nb = geo::ideal_backbone(x[1], y[1], z[1], new_aa == kGly, bb);
+  if (nb > T.natoms[new_aa]) nb = T.natoms[new_aa];
+  for (int k = 0; k < nb; ++k) {
+    x[k] = bb[k].x;
+    y[k] = bb[k].y;
+    z[k] = bb[k].z;
+  }
+  aa[r] = static_cast<uint8_t>(new_aa);
+  rot[r] = static_cast<uint8_t>(new_rot);
+  side_chain(T, new_aa, new_rot, nb, x, y, z);
+}
+
+// disulfide_energy over the Cys SG slots (slot 5), pairs in residue order,
+// the pair set by the f32 distance and each well in FP64.
+__device__ inline double disulfide(const uint8_t* aa, const float* X, const float* Y,
+                                   const float* Z, uint32_t nr) {
+  double e = 0.0;
+  for (uint32_t i = 0; i < nr; ++i) {
+    if (aa[i] != kCys) continue;
+    const uint32_t si = i * kSlots + kCysSgSlot;
+    for (uint32_t j = i + 1; j < nr; ++j) {
+      if (aa[j] != kCys) continue;
+      const uint32_t sj = j * kSlots + kCysSgSlot;
+      const float dx = X[si] - X[sj], dy = Y[si] - Y[sj], dz = Z[si] - Z[sj];
+      if (sqrtf(dx * dx + dy * dy + dz * dz) > kSsCutoff) continue;
+      const double ex = static_cast<double>(X[si]) - X[sj], ey = static_cast<double>(Y[si]) - Y[sj],
+                   ez = static_cast<double>(Z[si]) - Z[sj];
+      const double t = (sqrt(ex * ex + ey * ey + ez * ez) - static_cast<double>(kSsD0)) /
+                       static_cast<double>(kSsSigma);
+      e -= static_cast<double>(kSsDepth) * exp(-0.5 * t * t);
+    }
+  }
+  return e;
+}
+
+// Sum of residue_contribution_allatom over residues r..nr-1 plus the
+// disulfide term: the energy a torsion move at r is judged by.
+__device__ inline double chain_energy(const RefineArgs& a, const uint8_t* aa, const float* X,
+                                      const float* Y, const float* Z, uint32_t r, uint32_t nr,
+                                      uint32_t lane) {
+  double sum = 0.0;
+  for (uint32_t i = r; i < nr; ++i) {
+    const uint32_t t = i * kSlots + 1;
+    sum += contribution(a, aa[i], X[t], Y[t], Z[t], lane);
+  }
+  return sum + disulfide(aa, X, Y, Z, nr);
+}
+
+// perturb_phi / perturb_psi (src/allatom.rs:392-456) by rad: phi turns the
+// atoms of residue r after its Calpha and every later residue about the N-CA
+// axis through CA; psi turns r's O and every later residue about the CA-C
+// axis through C. Lane-split over the slots; the axis atoms do not move.
+__device__ inline void torsion(const RefTables& T, const uint8_t* aa, float* X, float* Y,
+                               float* Z, uint32_t r, uint32_t nr, float rad, bool phi,
+                               uint32_t lane) {
+  const uint32_t b = r * kSlots;
+  const geo::V3 p0 = phi ? geo::V3{X[b], Y[b], Z[b]} : geo::V3{X[b + 1], Y[b + 1], Z[b + 1]};
+  const geo::V3 p1 = phi ? geo::V3{X[b + 1], Y[b + 1], Z[b + 1]} : geo::V3{X[b + 2], Y[b + 2], Z[b + 2]};
+  const geo::V3 axis = geo::normalize(geo::sub(p1, p0));
+  const float cs = static_cast<float>(cos(static_cast<double>(rad)));
+  const float sn = static_cast<float>(sin(static_cast<double>(rad)));
+  __syncthreads();
+  for (uint32_t q = b + lane; q < nr * kSlots; q += kWave) {
+    const uint32_t rr = q / kSlots, k = q - rr * kSlots;
+    if (k >= T.natoms[aa[rr]]) continue;
+    if (rr == r && (phi ? k < 2 : k != 3)) continue;
+    const geo::V3 o = geo::rodrigues({X[q], Y[q], Z[q]}, p1, axis, cs, sn);
+    X[q] = o.x;
+    Y[q] = o.y;
+    Z[q] = o.z;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
+  extern __shared__ float lds[];
+  const uint32_t c = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  if (c >= a.n_cand) return;
+  const RefTables& T = *a.tab;
+  const uint32_t g0 = a.cand_res_off[c], nr = a.cand_res_off[c + 1] - g0;
+  const uint32_t ms = a.max_res * kSlots;
+  float* X = lds;
+  float* Y = X + ms;
+  float* Z = Y + ms;
+  float* F = Z + ms;  // 3 * max_res forces
+  uint8_t* aa = reinterpret_cast<uint8_t*>(F + 3 * a.max_res);
+  uint8_t* rot = aa + a.max_res;
+  uint8_t* iface = rot + a.max_res;
+  for (uint32_t t = lane; t < nr * kSlots; t += kWave) {
+    const float4 v = a.slots[static_cast<size_t>(g0) * kSlots + t];
+    X[t] = v.x;
+    Y[t] = v.y;
+    Z[t] = v.z;
+  }
+  for (uint32_t r = lane; r < nr; r += kWave) {
+    aa[r] = a.res_aa[g0 + r];
+    rot[r] = a.res_rot[g0 + r];
+  }
+  __syncthreads();
+
+  for (uint32_t s = 0; s < a.n_steps; ++s) {
+    const uint32_t it = a.iter0 + s;
+    const float temp = a.temp[s];
+    // 1. the gradient step (src/diffusion.rs:305-339)
+    for (uint32_t r = lane; r < nr; r += kWave) {
+      const uint32_t t = r * kSlots + 1;
+      const uint32_t k = aa[r];
+      float fx, fy, fz;
+      cg_force_point(a.ag_grid, a.ag_cg, X[t], Y[t], Z[t], T.cg_q[k], T.cg_eps[k], T.cg_sig[k],
+                     T.cg_hyd[k], fx, fy, fz);
+      F[3 * r] = fx;
+      F[3 * r + 1] = fy;
+      F[3 * r + 2] = fz;
+    }
+    __syncthreads();
+    for (uint32_t r = lane; r < nr; r += kWave) {
+      const uint32_t t = r * kSlots + 1;
+      const float4 d = residue_delta(a.cx, a.cy, a.cz, a.sp, a.r0,
+                                     make_float4(X[t], Y[t], Z[t], 0.0f), F[3 * r], F[3 * r + 1],
+                                     F[3 * r + 2], a.sigma[s], it, g0 + r);
+      for (uint32_t k = 0; k < T.natoms[aa[r]]; ++k) {
+        X[r * kSlots + k] += d.x;
+        Y[r * kSlots + k] += d.y;
+        Z[r * kSlots + k] += d.z;
+      }
+    }
+    __syncthreads();
+    // 2. interface mask (src/diffusion.rs:341-351)
+    for (uint32_t r = lane; r < nr; r += kWave) {
+      const uint32_t t = r * kSlots + 1;
+      bool in = false;
+      for (uint32_t i = 0; i < a.n_ag && !in; ++i) {
+        const float dx = X[t] - a.ag_x[i], dy = Y[t] - a.ag_y[i], dz = Z[t] - a.ag_z[i];
+        in = dx * dx + dy * dy + dz * dz < a.iface_sq;
+      }
+      iface[r] = in ? 1 : 0;
+    }
+    __syncthreads();
+    // 3. side-chain moves (src/diffusion.rs:353-398)
+    if (a.moves & DNAB_MC_SIDECHAIN) {
+      for (uint32_t r = 0; r < nr; ++r) {
+        const float4 u = u4(it, g0 + r, kTagMc, a.sp);
+        const bool in = iface[r] != 0;
+        const float eff_mut = in ? a.mut_p * 3.0f : a.mut_p * 0.4f;
+        const float eff_rot = in ? a.rot_p * 2.0f : a.rot_p;
+        if (!(u.x < eff_mut)) continue;
+        const uint32_t t = r * kSlots + 1;
+        const uint32_t old_aa = aa[r], old_rot = rot[r];
+        uint32_t new_aa = old_aa, new_rot;
+        if (u.y < eff_rot) {  // rotamer move
+          const uint32_t nrot = T.nrot[old_aa];
+          if (nrot == 0) continue;
+          new_rot = min(static_cast<uint32_t>(u.z * static_cast<float>(nrot)), nrot - 1);
+        } else {  // mutation: the new type's first rotamer
+          new_aa = min(static_cast<uint32_t>(u.z * static_cast<float>(kNumAminoAcids)),
+                       static_cast<uint32_t>(kNumAminoAcids - 1));
+          new_rot = 0;
+        }
+        const double old_e = contribution(a, old_aa, X[t], Y[t], Z[t], lane) +
+                             disulfide(aa, X, Y, Z, nr);
+        __syncthreads();
+        if (lane == 0) {
+          if (new_aa == old_aa && u.y < eff_rot) {  // apply_rotamer
+            rot[r] = static_cast<uint8_t>(new_rot);
+            side_chain(T, old_aa, new_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
+                       Y + r * kSlots, Z + r * kSlots);
+          } else {
+            mutate(T, aa, rot, r, new_aa, new_rot, X, Y, Z);
+          }
+        }
+        __syncthreads();
+        const double new_e = contribution(a, aa[r], X[t], Y[t], Z[t], lane) +
+                             disulfide(aa, X, Y, Z, nr);
+        const double delta = new_e - old_e;
+        const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
+        __syncthreads();
+        if (!accept && lane == 0) {
+          if (u.y < eff_rot) {  // the old chi, rebuilt on the current backbone
+            rot[r] = static_cast<uint8_t>(old_rot);
+            side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
+                       Y + r * kSlots, Z + r * kSlots);
+          } else {
+            mutate(T, aa, rot, r, old_aa, old_rot, X, Y, Z);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // 4. torsion moves (src/diffusion.rs:400-415)
+    if (a.moves & DNAB_MC_BACKBONE) {
+      for (uint32_t r = 0; r < nr; ++r) {
+        const float4 u = u4(it, g0 + r, kTagBb, a.sp);
+        if (!(u.x < a.bb_p)) continue;
+        const float deg = -a.max_torsion + (a.max_torsion + a.max_torsion) * u.y;
+        const float rad = geo::deg2rad(deg);
+        const bool phi = u.z < 0.5f;
+        const double old_e = chain_energy(a, aa, X, Y, Z, r, nr, lane);
+        torsion(T, aa, X, Y, Z, r, nr, rad, phi, lane);
+        const double new_e = chain_energy(a, aa, X, Y, Z, r, nr, lane);
+        const double delta = new_e - old_e;
+        const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
+        if (!accept) torsion(T, aa, X, Y, Z, r, nr, -rad, phi, lane);  // rotate back
+      }
+    }
+  }
+  for (uint32_t t = lane; t < nr * kSlots; t += kWave)
+    a.slots[static_cast<size_t>(g0) * kSlots + t] = make_float4(X[t], Y[t], Z[t], 0.0f);
+  for (uint32_t r = lane; r < nr; r += kWave) {
+    a.res_aa[g0 + r] = aa[r];
+    a.res_rot[g0 + r] = rot[r];
+  }
+}
+
+void fill_tables(RefTables& T) {
+  std::memset(&T, 0, sizeof(T));
+  for (int aa = 0; aa < kNumAminoAcids; ++aa) {
+    const ResTopology& topo = kResidueTopology[aa];
+    T.natoms[aa] = topo.n_atoms;
+    T.ndef[aa] = kSidechainCount[aa];
+    T.nrot[aa] = kRotamerCount[aa];
+    for (int k = 0; k < kSidechainCount[aa]; ++k) {
+      const BondDef& d = kSidechainDefs[aa][k];
+      RefDef& o = T.defs[aa][k];
+      o.a = d.ref_a;
+      o.b = d.ref_b;
+      o.c = d.ref_c;
+      o.chi = d.chi_idx;
+      o.bond = d.bond_len;
+      // the host builder's sines and cosines (builder.cpp place_atom)
+      const float ang = geo::deg2rad(d.angle_deg), dih = geo::deg2rad(d.dihedral_fixed);
+      o.sa = std::sin(ang);
+      o.ca = std::cos(ang);
+      o.sd = std::sin(dih);
+      o.cd = std::cos(dih);
+    }
+    for (int j = 0; j < kRotamerCount[aa] && j < 5; ++j)
+      for (int k = 0; k < 4; ++k) {
+        const float dih = geo::deg2rad(kRotamerLib[aa][j].chi[k]);
+        T.chi_sc[aa][j][k][0] = std::sin(dih);
+        T.chi_sc[aa][j][k][1] = std::cos(dih);
+      }
+    const ResAtomDef& cad = topo.atoms[topo.ca_idx];
+    T.ca_rmh[aa] = kAmberNb[cad.atom_type].r_min_half;
+    T.ca_eps[aa] = kAmberNb[cad.atom_type].epsilon;
+    T.ca_q[aa] = cad.charge;
+    T.ca_hyd[aa] = cad.hydrophobic;
+    T.ca_type[aa] = cad.atom_type;
+    uint8_t h = 0;
+    (void)dnab_residue_cg_params(static_cast<uint8_t>(aa), &T.cg_q[aa], &T.cg_eps[aa],
+                                 &T.cg_sig[aa], &h);
+    T.cg_hyd[aa] = h;
+  }
+}
+
+struct PopGuard {
+  dnab_population* p = nullptr;
+  ~PopGuard() { dnab_population_destroy(p); }
+};
+
+}  // namespace
+}  // namespace dnab
+
+using namespace dnab;
+
+extern "C" {
+
+int dnab_mc_default_params(dnab_mc_params* p) {
+  if (!p) return DNAB_ERR_INVALID;
+  p->mutation_p = 0.08f;       // src/diffusion.rs:41
+  p->rotamer_move_p = 0.12f;   // src/diffusion.rs:43
+  p->backbone_move_p = 0.04f;  // src/diffusion.rs:45
+  p->max_torsion_deg = 15.0f;  // src/diffusion.rs:47
+  p->iface_cutoff_sq = 64.0f;  // src/diffusion.rs:51
+  p->moves = DNAB_MC_SIDECHAIN | DNAB_MC_BACKBONE;
+  return DNAB_OK;
+}
+
+int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
+                const uint8_t* res_rotamer, const dnab_langevin_params* lp,
+                const dnab_mc_params* mc, dnab_protein_out* out, uint8_t* out_rotamer,
+                double* out_energy, size_t* n_atoms_out) {
+  if (!ctx || !ag || !ab || !lp || !mc || ag->ctx != ctx) return DNAB_ERR_INVALID;
+  const size_t nc = ab->n_cand, nr = ab->n_res, na = ab->atoms.n;
+  if (nc && !ab->cand_res_off) return DNAB_ERR_INVALID;
+  if (nr && (!ab->res_atom_off || !ab->res_ca_atom || !ab->res_amino_acid))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: residues need offsets, Calpha and types");
+  if (!ag->has_ca_aa || ag->n_ca == 0)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: the antigen needs its Calpha cloud and types");
+  if (lp->schedule_total == 0 || !(lp->t_hot > 0.0f) || !(lp->t_cold > 0.0f))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: bad temperature schedule");
+  if (nc >= 0xFFFFFF00ull || nr >= 0xFFFFFF00ull / kSlots)
+    return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: too large");
+  if (nc && (ab->cand_res_off[0] != 0 || ab->cand_res_off[nc] != nr))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: cand_res_off must span 0..n_res");
+  if (nr && (ab->res_atom_off[0] != 0 || ab->res_atom_off[nr] != na))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: res_atom_off must span 0..n_atoms");
+  if (na && (!ab->atoms.x || !ab->atoms.y || !ab->atoms.z)) return DNAB_ERR_INVALID;
+  uint32_t max_res = 1;
+  for (size_t c = 0; c < nc; ++c) {
+    if (ab->cand_res_off[c + 1] < ab->cand_res_off[c])
+      return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: cand_res_off not monotone");
+    max_res = std::max(max_res, ab->cand_res_off[c + 1] - ab->cand_res_off[c]);
+  }
+  for (int t = 0; t < kNumAminoAcids; ++t)
+    if (kSidechainCount[t] > kMaxDefs || kRotamerCount[t] > 5 || kResidueTopology[t].n_atoms > kSlots)
+      return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: residue tables exceed the slots");
+  if (max_res > kMaxRefineRes)
+    return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: more than %u residues in a candidate",
+                          kMaxRefineRes);
+  // every residue carries its topology's atoms in topology order (the slots)
+  std::vector<float4> slots(std::max<size_t>(nr * kSlots, 1), make_float4(0, 0, 0, 0));
+  std::vector<uint8_t> raa(std::max<size_t>(nr, 1)), rrot(std::max<size_t>(nr, 1), 0);
+  for (size_t r = 0; r < nr; ++r) {
+    const uint8_t t = ab->res_amino_acid[r];
+    if (t >= kNumAminoAcids)
+      return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: residue %zu type out of range", r);
+    const uint32_t b = ab->res_atom_off[r], e = ab->res_atom_off[r + 1];
+    if (e < b || e - b != kResidueTopology[t].n_atoms || ab->res_ca_atom[r] != b + 1)
+      return dnab_set_error(ctx, DNAB_ERR_INVALID,
+                            "refine: residue %zu does not hold its topology's atoms", r);
+    raa[r] = t;
+    if (res_rotamer) {
+      if (kRotamerCount[t] ? res_rotamer[r] >= kRotamerCount[t] : res_rotamer[r] != 0)
+        return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: residue %zu rotamer out of range", r);
+      rrot[r] = res_rotamer[r];
+    }
+    for (uint32_t k = 0; k < e - b; ++k)
+      slots[r * kSlots + k] = make_float4(ab->atoms.x[b + k], ab->atoms.y[b + k], ab->atoms.z[b + k], 0.0f);
+  }
+  if (nc == 0) {
+    if (n_atoms_out) *n_atoms_out = 0;
+    if (out && out->res_start) out->res_start[0] = 0;
+    return DNAB_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  // tables, antigen Calpha parameters, schedule
+  RefTables tab;
+  fill_tables(tab);
+  const size_t n_ag = ag->n_ca;
+  std::vector<uint8_t> ag_aa(n_ag);
+  DNAB_RETURN_IF(download(ctx, ag_aa.data(), ag->ca_aa.p, n_ag));
+  std::vector<float4> ag_par(n_ag);
+  for (size_t i = 0; i < n_ag; ++i) {
+    const uint8_t t = ag_aa[i] < kNumAminoAcids ? ag_aa[i] : 0;
+    ag_par[i] = make_float4(tab.ca_rmh[t], tab.ca_eps[t], tab.ca_q[t],
+                            static_cast<float>(tab.ca_hyd[t] | (tab.ca_type[t] << 8)));
+  }
+  const uint32_t n_steps = lp->n_steps;
+  std::vector<float> temp(std::max<uint32_t>(n_steps, 1)), sigma(std::max<uint32_t>(n_steps, 1));
+  for (uint32_t s = 0; s < n_steps; ++s) {
+    // temperature (src/diffusion.rs:102-104) and the noise scale (src/diffusion.rs:315)
+    const float it = static_cast<float>(s + lp->schedule_offset);
+    temp[s] = lp->t_hot * std::pow(lp->t_cold / lp->t_hot, it / static_cast<float>(lp->schedule_total));
+    sigma[s] = lp->with_noise ? lp->noise_base * std::sqrt(temp[s]) : 0.0f;
+  }
+  DevBuf<RefTables> d_tab;
+  DevBuf<float4> d_slots, d_agpar;
+  DevBuf<uint8_t> d_aa, d_rot;
+  DevBuf<uint32_t> d_off;
+  DevBuf<float> d_temp, d_sigma;
+  DNAB_ALLOC_CHECK(ctx, d_tab.alloc(1));
+  DNAB_ALLOC_CHECK(ctx, d_slots.alloc(slots.size()));
+  DNAB_ALLOC_CHECK(ctx, d_agpar.alloc(n_ag));
+  DNAB_ALLOC_CHECK(ctx, d_aa.alloc(raa.size()));
+  DNAB_ALLOC_CHECK(ctx, d_rot.alloc(rrot.size()));
+  DNAB_ALLOC_CHECK(ctx, d_off.alloc(nc + 1));
+  DNAB_ALLOC_CHECK(ctx, d_temp.alloc(temp.size()));
+  DNAB_ALLOC_CHECK(ctx, d_sigma.alloc(sigma.size()));
+  DNAB_RETURN_IF(upload(ctx, d_tab.p, &tab, sizeof(tab)));
+  DNAB_RETURN_IF(upload(ctx, d_slots.p, slots.data(), slots.size() * sizeof(float4)));
+  DNAB_RETURN_IF(upload(ctx, d_agpar.p, ag_par.data(), n_ag * sizeof(float4)));
+  DNAB_RETURN_IF(upload(ctx, d_aa.p, raa.data(), raa.size()));
+  DNAB_RETURN_IF(upload(ctx, d_rot.p, rrot.data(), rrot.size()));
+  DNAB_RETURN_IF(upload(ctx, d_off.p, ab->cand_res_off, (nc + 1) * sizeof(uint32_t)));
+  DNAB_RETURN_IF(upload(ctx, d_temp.p, temp.data(), temp.size() * sizeof(float)));
+  DNAB_RETURN_IF(upload(ctx, d_sigma.p, sigma.data(), sigma.size() * sizeof(float)));
+
+  RefineArgs a{};
+  a.tab = d_tab.p;
+  a.ag_x = ag->ca_x.p;
+  a.ag_y = ag->ca_y.p;
+  a.ag_z = ag->ca_z.p;
+  a.ag_par = d_agpar.p;
+  a.n_ag = static_cast<uint32_t>(n_ag);
+  a.ag_grid = make_view(ag->ca_grid);
+  a.ag_cg = CgCloud{ag->ca_x.p, ag->ca_y.p, ag->ca_z.p, ag->ca_q.p, ag->ca_eps.p, ag->ca_sig.p,
+                    ag->ca_hyd.p};
+  a.cx = ag->center[0];
+  a.cy = ag->center[1];
+  a.cz = ag->center[2];
+  a.r0 = ag->restraint_radius > 0.0f ? ag->restraint_radius : lp->restraint_radius;
+  a.cand_res_off = d_off.p;
+  a.res_aa = d_aa.p;
+  a.res_rot = d_rot.p;
+  a.slots = d_slots.p;
+  a.n_cand = static_cast<uint32_t>(nc);
+  a.max_res = max_res;
+  a.temp = d_temp.p;
+  a.sigma = d_sigma.p;
+  a.n_steps = n_steps;
+  a.iter0 = lp->schedule_offset;
+  a.sp.step = lp->step;
+  a.sp.max_disp = lp->max_disp;
+  a.sp.restraint_k = lp->restraint_k;
+  a.sp.restraint_radius = lp->restraint_radius;
+  a.sp.with_noise = lp->with_noise;
+  a.sp.force_model = DNAB_FORCE_CG;
+  a.sp.seed_lo = static_cast<uint32_t>(lp->seed);
+  a.sp.seed_hi = static_cast<uint32_t>(lp->seed >> 32);
+  a.mut_p = mc->mutation_p;
+  a.rot_p = mc->rotamer_move_p;
+  a.bb_p = mc->backbone_move_p;
+  a.max_torsion = mc->max_torsion_deg;
+  a.iface_sq = mc->iface_cutoff_sq;
+  a.moves = mc->moves;
+  const size_t lds = static_cast<size_t>(max_res) * (3 * kSlots + 3) * sizeof(float) + 3 * max_res;
+  if (lds > 65536)
+    DNAB_HIP_CHECK(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_refine),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(lds)));
+  hipEvent_t ev_end = nullptr;
+  hipEvent_t ev = timer_begin(ctx, "refine", &ev_end);
+  hipLaunchKernelGGL(k_refine, dim3(static_cast<uint32_t>(nc)), dim3(kWave), lds, ctx->stream, a);
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  timer_end(ctx, "refine", ev, ev_end);
+  DNAB_RETURN_IF(download(ctx, slots.data(), d_slots.p, slots.size() * sizeof(float4)));
+  DNAB_RETURN_IF(download(ctx, raa.data(), d_aa.p, raa.size()));
+  DNAB_RETURN_IF(download(ctx, rrot.data(), d_rot.p, rrot.size()));
+
+  // the final proteins, residue by residue in topology order
+  size_t n_out = 0;
+  std::vector<uint32_t> res_off(nr + 1, 0);
+  for (size_t r = 0; r < nr; ++r) {
+    n_out += kResidueTopology[raa[r]].n_atoms;
+    res_off[r + 1] = static_cast<uint32_t>(n_out);
+  }
+  if (n_atoms_out) *n_atoms_out = n_out;
+  if (out_rotamer)
+    for (size_t r = 0; r < nr; ++r) out_rotamer[r] = rrot[r];
+  std::vector<float> fx(n_out), fy(n_out), fz(n_out), fq(n_out), frm(n_out), fe(n_out);
+  std::vector<uint8_t> fh(n_out), ft(n_out);
+  std::vector<uint32_t> fca(std::max<size_t>(nr, 1)), fri(n_out);
+  for (size_t c = 0; c < nc; ++c)
+    for (uint32_t r = ab->cand_res_off[c]; r < ab->cand_res_off[c + 1]; ++r) {
+      const ResTopology& topo = kResidueTopology[raa[r]];
+      fca[r] = res_off[r] + topo.ca_idx;
+      for (uint32_t k = 0; k < topo.n_atoms; ++k) {
+        const size_t o = res_off[r] + k;
+        const float4 v = slots[r * kSlots + k];
+        const ResAtomDef& d = topo.atoms[k];
+        fx[o] = v.x;
+        fy[o] = v.y;
+        fz[o] = v.z;
+        fq[o] = d.charge;
+        frm[o] = kAmberNb[d.atom_type].r_min_half;
+        fe[o] = kAmberNb[d.atom_type].epsilon;
+        fh[o] = d.hydrophobic;
+        ft[o] = d.atom_type;
+        fri[o] = r - ab->cand_res_off[c];
+      }
+    }
+  if (out_energy) {
+    // allatom_energy of each refined candidate (src/diffusion.rs:1092)
+    std::vector<uint32_t> cand_ag(nc, 0);
+    dnab_population_desc pd{};
+    pd.n_cand = nc;
+    pd.n_res = nr;
+    pd.cand_antigen = cand_ag.data();
+    pd.cand_res_off = ab->cand_res_off;
+    pd.res_atom_off = res_off.data();
+    pd.res_ca_atom = fca.data();
+    pd.res_amino_acid = raa.data();
+    pd.atoms.n = n_out;
+    pd.atoms.x = fx.data();
+    pd.atoms.y = fy.data();
+    pd.atoms.z = fz.data();
+    pd.atoms.charge = fq.data();
+    pd.atoms.r_min_half = frm.data();
+    pd.atoms.epsilon = fe.data();
+    pd.atoms.hydrophobic = fh.data();
+    pd.atoms.atom_type = ft.data();
+    PopGuard pop;
+    dnab_antigen* ags[1] = {ag};
+    DNAB_RETURN_IF(dnab_population_create(ctx, ags, 1, &pd, &pop.p));
+    DNAB_RETURN_IF(dnab_population_evaluate(ctx, pop.p, DNAB_EVAL_BREAKDOWN));
+    std::vector<double> terms(6 * nc);
+    DNAB_RETURN_IF(dnab_population_read_terms(ctx, pop.p, terms.data()));
+    for (size_t c = 0; c < nc; ++c) {
+      const double* t = &terms[6 * c];
+      out_energy[c] = ((((t[0] + t[1]) + t[2]) + t[3]) + t[4]) + t[5];
+    }
+  }
+  if (!out) return DNAB_OK;
+  if (!out->x || !out->y || !out->z) return DNAB_ERR_INVALID;
+  if (n_out > out->atom_capacity) return DNAB_ERR_CAPACITY;
+  std::memcpy(out->x, fx.data(), n_out * sizeof(float));
+  std::memcpy(out->y, fy.data(), n_out * sizeof(float));
+  std::memcpy(out->z, fz.data(), n_out * sizeof(float));
+  if (out->charge) std::memcpy(out->charge, fq.data(), n_out * sizeof(float));
+  if (out->r_min_half) std::memcpy(out->r_min_half, frm.data(), n_out * sizeof(float));
+  if (out->epsilon) std::memcpy(out->epsilon, fe.data(), n_out * sizeof(float));
+  if (out->hydrophobic) std::memcpy(out->hydrophobic, fh.data(), n_out);
+  if (out->atom_type) std::memcpy(out->atom_type, ft.data(), n_out);
+  if (out->residue_idx) std::memcpy(out->residue_idx, fri.data(), n_out * sizeof(uint32_t));
+  if (out->res_start) std::memcpy(out->res_start, res_off.data(), (nr + 1) * sizeof(uint32_t));
+  if (out->ca_atom_idx && nr) std::memcpy(out->ca_atom_idx, fca.data(), nr * sizeof(uint32_t));
+  if (out->amino_acid && nr) std::memcpy(out->amino_acid, raa.data(), nr);
+  return DNAB_OK;
+}
+
+}  // extern "C"

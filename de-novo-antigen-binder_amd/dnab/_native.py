@@ -95,6 +95,15 @@ class PopulationDescC(C.Structure):
                 ("res_amino_acid", u8p), ("atoms", AtomCloudC)]
 
 
+DNAB_MC_SIDECHAIN, DNAB_MC_BACKBONE = 1, 2
+
+
+class McParamsC(C.Structure):
+    _fields_ = [("mutation_p", C.c_float), ("rotamer_move_p", C.c_float),
+                ("backbone_move_p", C.c_float), ("max_torsion_deg", C.c_float),
+                ("iface_cutoff_sq", C.c_float), ("moves", C.c_uint32)]
+
+
 class LangevinParamsC(C.Structure):
     _fields_ = [("n_steps", C.c_uint32), ("score_every", C.c_uint32), ("top_k", C.c_uint32),
                 ("seed", C.c_uint64), ("with_noise", C.c_int32),
@@ -221,6 +230,11 @@ _PROTOS = {
     "dnab_population_read_terms": (C.c_int, [vp, vp, f64p]),
     "dnab_allatom_energy": (C.c_int, [vp, C.POINTER(AtomCloudC), C.POINTER(AtomCloudC), u32p,
                                       u32p, u8p, C.c_size_t, C.c_float, f64p, f64p]),
+    "dnab_mc_default_params": (C.c_int, [C.POINTER(McParamsC)]),
+    "dnab_refine": (C.c_int, [vp, vp, C.POINTER(PopulationDescC), u8p, C.POINTER(LangevinParamsC),
+                              C.POINTER(McParamsC), C.POINTER(ProteinOutC), u8p, f64p,
+                              C.POINTER(C.c_size_t)]),
+    "dnab_rebuild_side_chain": (C.c_int, [C.c_uint8, C.c_uint8, f32p, C.c_size_t]),
     "dnab_affinity_maturation": (C.c_int, [vp, vp, C.POINTER(PopulationDescC), u8p, u32p, u32p,
                                            C.POINTER(ProteinOutC), u8p, u8p, f64p,
                                            C.POINTER(C.c_size_t)]),

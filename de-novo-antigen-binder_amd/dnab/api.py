@@ -156,6 +156,16 @@ class GpuContext:
         call.run()
         return call.results()
 
+    def refine(self, antigen: "DeviceAntigen", antibodies: list[AtomProtein], lp=None, mc=None,
+               rotamers=None, energies: bool = True):
+        """allatom_diffusion_step (src/diffusion.rs:290-416) lp.n_steps times
+        for every antibody; returns the refined proteins, their rotamer
+        indices and (energies) their allatom_energy."""
+        call = RefineCall(self, antigen, population_from_proteins(list(antibodies)),
+                          lp if lp is not None else langevin_params(), mc, rotamers, energies)
+        call.run()
+        return call.results()
+
     def atom_forces(self, antigen: AtomCloud, antibody: AtomCloud, cutoff: float = 10.0):
         n = len(antibody)
         fx, fy, fz = (np.zeros(max(n, 1), np.float32) for _ in range(3))
@@ -307,6 +317,84 @@ class DeviceAntigen:
             pass
 
 
+class _ProteinOut:
+    """Caller-allocated dnab_protein_out buffers for n_res residues, and their
+    split into one AtomProtein per candidate of a population description."""
+
+    def __init__(self, n_res: int):
+        cap = max(1, MAX_ATOMS_PER_RESIDUE * n_res)
+        self.f = {k: np.zeros(cap, np.float32) for k in ("x", "y", "z", "charge", "rmh", "eps")}
+        self.hyd, self.at = np.zeros(cap, np.uint8), np.zeros(cap, np.uint8)
+        self.ri = np.zeros(cap, np.uint32)
+        self.rs = np.zeros(n_res + 1, np.uint32)
+        self.ca = np.zeros(max(n_res, 1), np.uint32)
+        self.aa = np.zeros(max(n_res, 1), np.uint8)
+        f = self.f
+        self.c = N.ProteinOutC(cap, N.ptr(f["x"], C.c_float), N.ptr(f["y"], C.c_float),
+                               N.ptr(f["z"], C.c_float), N.ptr(f["charge"], C.c_float),
+                               N.ptr(f["rmh"], C.c_float), N.ptr(f["eps"], C.c_float),
+                               N.ptr(self.hyd, C.c_uint8), N.ptr(self.at, C.c_uint8),
+                               N.ptr(self.ri, C.c_uint32), N.ptr(self.rs, C.c_uint32),
+                               N.ptr(self.ca, C.c_uint32), N.ptr(self.aa, C.c_uint8))
+
+    def proteins(self, cand_res_off) -> list[AtomProtein]:
+        f, rs = self.f, self.rs
+        out = []
+        for a in range(len(cand_res_off) - 1):
+            g0, g1 = int(cand_res_off[a]), int(cand_res_off[a + 1])
+            a0, a1 = int(rs[g0]), int(rs[g1])
+            cloud = AtomCloud(f["x"][a0:a1].copy(), f["y"][a0:a1].copy(), f["z"][a0:a1].copy(),
+                              f["charge"][a0:a1].copy(), f["rmh"][a0:a1].copy(),
+                              f["eps"][a0:a1].copy(), self.hyd[a0:a1].copy(),
+                              self.at[a0:a1].copy(), self.ri[a0:a1].copy())
+            out.append(AtomProtein(cloud, (rs[g0:g1 + 1] - a0).astype(np.uint32),
+                                   self.aa[g0:g1].copy(), (self.ca[g0:g1] - a0).astype(np.uint32)))
+        return out
+
+
+def mc_params(**kw) -> N.McParamsC:
+    """Monte-Carlo move rates (the reference's by default)."""
+    p = N.McParamsC()
+    N.check(N.lib.dnab_mc_default_params(C.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"unknown Monte-Carlo parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+class RefineCall:
+    """One dnab_refine call (allatom_diffusion_step for every candidate, all
+    iterations in one launch) with inputs packed and outputs allocated up
+    front, so that run() is the C call alone."""
+
+    def __init__(self, ctx: "GpuContext", antigen: "DeviceAntigen", host: PopulationHost,
+                 lp: N.LangevinParamsC, mc: N.McParamsC | None = None, rotamers=None,
+                 energies: bool = True):
+        self.ctx, self.host, self.antigen, self.lp = ctx, host, antigen, lp
+        self.mc = mc if mc is not None else mc_params()
+        n_res = host.n_res
+        self.rot = None if rotamers is None else np.ascontiguousarray(
+            np.concatenate([np.asarray(r, np.uint8) for r in rotamers]), np.uint8)
+        self.buf = _ProteinOut(n_res)
+        self.rot_out = np.zeros(max(n_res, 1), np.uint8)
+        self.energy = np.zeros(max(host.n_cand, 1), np.float64) if energies else None
+        self.na = C.c_size_t(0)
+        self.desc = host.c()
+
+    def run(self):
+        self.ctx.check(N.lib.dnab_refine(
+            self.ctx.handle, self.antigen.handle, C.byref(self.desc), N.ptr(self.rot, C.c_uint8),
+            C.byref(self.lp), C.byref(self.mc), C.byref(self.buf.c),
+            N.ptr(self.rot_out, C.c_uint8), N.ptr(self.energy, C.c_double), C.byref(self.na)))
+
+    def results(self):
+        off = self.host.cand_res_off
+        rots = [self.rot_out[int(off[a]):int(off[a + 1])].copy() for a in range(self.host.n_cand)]
+        e = None if self.energy is None else self.energy[:self.host.n_cand].copy()
+        return self.buf.proteins(off), rots, e
+
+
 class MaturationCall:
     """One dnab_affinity_maturation call with its inputs packed and its output
     buffers allocated up front, so that run() is the C call alone."""
@@ -328,20 +416,7 @@ class MaturationCall:
             np.uint32)
         self.rot = None if rotamers is None else np.ascontiguousarray(
             np.concatenate([np.asarray(r, np.uint8) for r in rotamers]), np.uint8)
-        cap = max(1, MAX_ATOMS_PER_RESIDUE * n_res)
-        self.f = {k: np.zeros(cap, np.float32) for k in ("x", "y", "z", "charge", "rmh", "eps")}
-        self.hyd, self.at = np.zeros(cap, np.uint8), np.zeros(cap, np.uint8)
-        self.ri = np.zeros(cap, np.uint32)
-        self.rs = np.zeros(n_res + 1, np.uint32)
-        self.ca = np.zeros(max(n_res, 1), np.uint32)
-        self.aa = np.zeros(max(n_res, 1), np.uint8)
-        f = self.f
-        self.out = N.ProteinOutC(cap, N.ptr(f["x"], C.c_float), N.ptr(f["y"], C.c_float),
-                                 N.ptr(f["z"], C.c_float), N.ptr(f["charge"], C.c_float),
-                                 N.ptr(f["rmh"], C.c_float), N.ptr(f["eps"], C.c_float),
-                                 N.ptr(self.hyd, C.c_uint8), N.ptr(self.at, C.c_uint8),
-                                 N.ptr(self.ri, C.c_uint32), N.ptr(self.rs, C.c_uint32),
-                                 N.ptr(self.ca, C.c_uint32), N.ptr(self.aa, C.c_uint8))
+        self.buf = _ProteinOut(n_res)
         self.rot_out = np.zeros(max(n_res, 1), np.uint8)
         self.rebuilt = np.zeros(max(n_res, 1), np.uint8)
         self.energy = np.zeros(max(n_ab, 1), np.float64)
@@ -352,25 +427,15 @@ class MaturationCall:
         self.ctx.check(N.lib.dnab_affinity_maturation(
             self.ctx.handle, self.antigen.handle, C.byref(self.desc),
             N.ptr(self.rot, C.c_uint8), N.ptr(self.moff, C.c_uint32),
-            N.ptr(self.mres, C.c_uint32), C.byref(self.out), N.ptr(self.rot_out, C.c_uint8),
+            N.ptr(self.mres, C.c_uint32), C.byref(self.buf.c), N.ptr(self.rot_out, C.c_uint8),
             N.ptr(self.rebuilt, C.c_uint8), N.ptr(self.energy, C.c_double), C.byref(self.na)))
 
     def results(self):
-        host, f, rs = self.host, self.f, self.rs
-        prots, rots, flags = [], [], []
-        for a in range(host.n_cand):
-            g0, g1 = int(host.cand_res_off[a]), int(host.cand_res_off[a + 1])
-            a0, a1 = int(rs[g0]), int(rs[g1])
-            cloud = AtomCloud(f["x"][a0:a1].copy(), f["y"][a0:a1].copy(), f["z"][a0:a1].copy(),
-                              f["charge"][a0:a1].copy(), f["rmh"][a0:a1].copy(),
-                              f["eps"][a0:a1].copy(), self.hyd[a0:a1].copy(),
-                              self.at[a0:a1].copy(), self.ri[a0:a1].copy())
-            prots.append(AtomProtein(cloud, (rs[g0:g1 + 1] - a0).astype(np.uint32),
-                                     self.aa[g0:g1].copy(),
-                                     (self.ca[g0:g1] - a0).astype(np.uint32)))
-            rots.append(self.rot_out[g0:g1].copy())
-            flags.append(self.rebuilt[g0:g1].astype(bool))
-        return prots, self.energy[:host.n_cand].copy(), rots, flags
+        host = self.host
+        off = host.cand_res_off
+        rots = [self.rot_out[int(off[a]):int(off[a + 1])].copy() for a in range(host.n_cand)]
+        flags = [self.rebuilt[int(off[a]):int(off[a + 1])].astype(bool) for a in range(host.n_cand)]
+        return self.buf.proteins(off), self.energy[:host.n_cand].copy(), rots, flags
 
 
 def langevin_params(**kw) -> N.LangevinParamsC:

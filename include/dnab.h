@@ -124,6 +124,11 @@ int dnab_ctx_kernel_time(dnab_ctx* ctx, const char* kernel, double* total_ms, ui
 int dnab_build_protein(const float* ca_x, const float* ca_y, const float* ca_z,
                        const uint8_t* amino_acid, const uint8_t* rotamer_idx, size_t n_res,
                        dnab_protein_out* out, size_t* n_atoms_out);
+/* rebuild_side_chain (src/allatom.rs:244-271) of one residue: xyz holds the
+ * residue's atoms in topology order (3 floats each); the side chain is placed
+ * from its backbone atoms (N, CA, C, O and, except Gly, CB) with the chi of
+ * rotamer `rotamer` (0 for types without rotamers) and written after them. */
+int dnab_rebuild_side_chain(uint8_t amino_acid, uint8_t rotamer, float* xyz, size_t n_atoms);
 /* place_atom (src/rotamer.rs:198-232), NERF. */
 int dnab_place_atom(const float a[3], const float b[3], const float c[3], float bond_len,
                     float angle_deg, float dihedral_deg, float out[3]);
@@ -415,6 +420,43 @@ typedef struct dnab_langevin_params {
                                in the cutoff, not on pairs tested. */
 } dnab_langevin_params;
 int dnab_langevin_default_params(dnab_langevin_params* p);
+
+/* ---- refinement with Monte-Carlo moves (allatom_diffusion_step) ----------
+ * Rates of the Metropolis moves of allatom_diffusion_step
+ * (src/diffusion.rs:353-415); the defaults are the reference's. */
+#define DNAB_MC_SIDECHAIN 1u /* rotamer moves and amino-acid mutations */
+#define DNAB_MC_BACKBONE 2u  /* phi/psi torsion moves of the downstream chain */
+typedef struct dnab_mc_params {
+  float mutation_p;      /* 0.08 (src/diffusion.rs:41); x3 at the interface, x0.4 elsewhere */
+  float rotamer_move_p;  /* 0.12 (src/diffusion.rs:43); x2 at the interface */
+  float backbone_move_p; /* 0.04 (src/diffusion.rs:45) */
+  float max_torsion_deg; /* 15 (src/diffusion.rs:47) */
+  float iface_cutoff_sq; /* 64, an 8 A Calpha-Calpha interface (src/diffusion.rs:51) */
+  uint32_t moves;        /* DNAB_MC_SIDECHAIN | DNAB_MC_BACKBONE (the reference) */
+} dnab_mc_params;
+int dnab_mc_default_params(dnab_mc_params* p);
+/* Every iteration of allatom_diffusion_step (src/diffusion.rs:290-416) for a
+ * batch of candidates, all iterations in one launch: residue-level forces from
+ * the antigen Calpha grid, restraint, clamp, noise (with_noise) and rigid
+ * residue moves, then Metropolis rotamer/mutation moves and phi/psi torsion
+ * moves scored by residue_contribution_allatom plus disulfide_energy. This is
+ * the per-candidate work of run_allatom's refinement phase
+ * (src/diffusion.rs:1083-1095, with_noise = 1) and, with with_noise = 0, of
+ * its broad-sampling steps (src/diffusion.rs:1017-1024). lp supplies n_steps,
+ * seed, with_noise, step, max_disp, noise_base, t_hot, t_cold,
+ * schedule_offset/total (temperature(offset + s, total)), restraint_k and
+ * restraint_radius (or the antigen's own). The antigen needs its Calpha cloud
+ * and residue types. Each input residue holds its topology's atoms in
+ * topology order (as protein_from_ca_trace builds them); res_rotamer
+ * (nullable) gives each residue's rotamer index. Random draws are
+ * Philox4x32-10 by (iteration, residue, kind): runs repeat exactly, but the
+ * reference's SmallRng streams are not reproduced. Outputs as
+ * dnab_affinity_maturation; out_energy (nullable, n_cand) receives each
+ * refined candidate's allatom_energy. Candidates of up to 512 residues. */
+int dnab_refine(dnab_ctx* ctx, dnab_antigen* antigen, const dnab_population_desc* ab,
+                const uint8_t* res_rotamer, const dnab_langevin_params* lp,
+                const dnab_mc_params* mc, dnab_protein_out* out, uint8_t* out_rotamer,
+                double* out_energy, size_t* n_atoms_out);
 /* One step at iteration `iter` (forces from a fresh evaluation). */
 int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                        uint32_t iter);

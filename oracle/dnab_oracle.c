@@ -33,7 +33,7 @@
  *   or_cg_forces(_with_grid)    compute_forces(_with_grid)    src/energy.rs:154-234, 322-398
  *   or_langevin_step            allatom_diffusion_step lines  src/diffusion.rs:300-339
  *                               with set_ca_pos               src/allatom.rs:279-293
- *   or_sigma                    temperature + noise scale     src/diffusion.rs:102-104, 315
+ *   or_temperature, or_sigma    temperature + noise scale     src/diffusion.rs:102-104, 315
  *   or_philox4x32_10            counter RNG of the step's noise (Salmon et al. 2011)
  *   or_baseline_energy_brute    timed CPU baseline: src/energy.rs:269-301 pattern,
  *                               parallel over candidates like the Rayon loop of
@@ -755,6 +755,11 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 }
 
 /* noise scale at iteration it: NOISE_BASE * sqrt(T_HOT (T_COLD/T_HOT)^(it/total)) */
+/* temperature (src/diffusion.rs:102-104) in f32. */
+float or_temperature(uint32_t it, uint32_t total, float t_hot, float t_cold) {
+  return t_hot * powf(t_cold / t_hot, (float)it / (float)total);
+}
+
 float or_sigma(uint32_t it, uint32_t total, float t_hot, float t_cold, float noise_base) {
   float temp = t_hot * powf(t_cold / t_hot, (float)it / (float)total);
   return noise_base * sqrtf(temp);

@@ -83,6 +83,7 @@ def _load():
         "or_disulfide_f64": (C.c_double, [f32p, f32p, f32p, C.c_size_t]),
         "or_philox4x32_10": (None, [u32p, u32p, u32p]),
         "or_sigma": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+        "or_temperature": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
         "or_langevin_step": (None, [f32p, f32p, f32p, C.c_size_t, u32p, u32p, f32p, f32p, f32p,
                                     C.POINTER(OrRCloud), f32p, f32p, f32p, u8p, C.c_uint32,
                                     C.POINTER(OrStepParams)]),
@@ -325,6 +326,10 @@ def philox(ctr, key):
     o = np.zeros(4, np.uint32)
     lib.or_philox4x32_10(_p(c, C.c_uint32), _p(k, C.c_uint32), _p(o, C.c_uint32))
     return o
+
+
+def temperature(it, total, t_hot=3.0, t_cold=0.02):
+    return float(lib.or_temperature(it, total, t_hot, t_cold))
 
 
 def sigma(it, total, t_hot=3.0, t_cold=0.02, noise_base=0.25):
