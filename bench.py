@@ -269,6 +269,96 @@ def maturation_bench(ctx, w, ag_dev, coords, reps=5, cpu_budget_s=4.0):
     return out
 
 
+def refine_bench(ctx, w, ag_dev, coords, survivors, cpu_budget_s=5.0):
+    """dnab_refine: allatom_diffusion_step with its Monte-Carlo moves
+    (src/diffusion.rs:290-416), all iterations in one launch. Two workloads
+    of run_allatom: the refinement phase (the loop's 64 best binders, 600
+    iterations with noise, src/diffusion.rs:1083-1095) and the broad-sampling
+    step count with the moves on all 1,024 binders (200 iterations, no noise,
+    src/diffusion.rs:1017-1024). CPU baseline: the oracle's C port of the
+    same step, threaded over candidates like the reference's Rayon loop,
+    timed on a bounded sample."""
+    import numpy as np
+    import dnab
+    from dnab.api import RefineCall, mc_params
+    from dnab.structure import AtomProtein, population_from_proteins
+    p = w.pop
+    off = p.cand_atom_off()
+    x, y, z = coords
+    prots = []
+    for c in range(p.n_cand):
+        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
+        cl = p.candidate_cloud(c)
+        a0, a1 = int(off[c]), int(off[c + 1])
+        cl.x, cl.y, cl.z = x[a0:a1].copy(), y[a0:a1].copy(), z[a0:a1].copy()
+        prots.append(AtomProtein(cl, (p.res_atom_off[r0:r1 + 1] - off[c]).astype(np.uint32),
+                                 p.res_amino_acid[r0:r1].copy(),
+                                 (p.res_ca_atom[r0:r1] - off[c]).astype(np.uint32)))
+    out = {}
+    for name, cands, steps, noise, total in (
+            ("refinement_64x600", [prots[int(i)] for i in survivors[:64]], 600, 1, 600),
+            ("broad_1024x200", prots, 200, 0, 800)):
+        lp = dnab.langevin_params(n_steps=steps, with_noise=noise, schedule_offset=0,
+                                  schedule_total=total, restraint_radius=w.restraint_radius[0],
+                                  seed=2024)
+        host = population_from_proteins(cands)
+        call = RefineCall(ctx, ag_dev, host, lp, mc_params(), energies=False)
+        call.run()                                       # warm-up (and the kernel's first load)
+        ctx.set_timing(True)
+        k0 = ctx.kernel_time("refine")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        call.run()
+        dt = time.perf_counter() - t0
+        k1 = ctx.kernel_time("refine")
+        ctx.set_timing(False)
+        cs = len(cands) * steps
+        out[name] = {"candidates": len(cands), "iterations": steps, "noise": bool(noise),
+                     "ms_per_call": dt * 1e3, "kernel_ms": k1[0] - k0[0],
+                     "value": cs / dt, "unit": "candidate-iterations/s"}
+    # CPU baseline: the C port on the host cores, 64 candidates, a bounded number of iterations
+    from oracle import oracle as O
+    import ctypes as C
+    from dnab import _native as N
+    cg = []
+    for a in range(20):
+        q, e, sg, h = C.c_float(), C.c_float(), C.c_float(), C.c_uint8()
+        N.lib.dnab_residue_cg_params(a, C.byref(q), C.byref(e), C.byref(sg), C.byref(h))
+        cg.append((q.value, e.value, sg.value, h.value))
+    tables = O.refine_tables(np.array(cg, np.float32))
+    cands = [prots[int(i)] for i in survivors[:64]]
+    host = population_from_proteins(cands)
+    slots = np.zeros((host.n_res, 14, 3), np.float32)
+    for g in range(host.n_res):
+        b, e = int(host.res_atom_off[g]), int(host.res_atom_off[g + 1])
+        slots[g, :e - b] = np.stack([host.atoms.x[b:e], host.atoms.y[b:e], host.atoms.z[b:e]], 1)
+    aa = host.res_amino_acid.copy()
+    rot = np.zeros(host.n_res, np.uint8)
+    ag = w.antigens[0]
+    threads = int(os.environ.get("RAYON_NUM_THREADS", "0")) or os.cpu_count() or 1
+    prm = dict(step=0.08, max_disp=2.0, restraint_k=0.02, restraint_radius=w.restraint_radius[0],
+               with_noise=1, seed_lo=2024, seed_hi=0, mut_p=0.08, rot_p=0.12, bb_p=0.04,
+               max_torsion=15.0, iface_sq=64.0, moves=3, center=ag.ca_center_of_mass())
+    done, t0 = 0, time.perf_counter()
+    while True:
+        k = 10
+        temps = [O.temperature(done + i, 600) for i in range(k)]
+        sigs = [O.sigma(done + i, 600) for i in range(k)]
+        O.refine_run(tables, ag.ca_residue_cloud(), ag.amino_acid, host.cand_res_off, aa, rot,
+                     slots, temps, sigs, done, prm, threads)
+        done += k
+        el = time.perf_counter() - t0
+        if el >= cpu_budget_s or done >= 600:
+            break
+    out["cpu_baseline"] = {
+        "value": 64 * done / el, "unit": "candidate-iterations/s", "cores": threads,
+        "kind": "port",
+        "sample": (f"the oracle's C port of the step with its moves, 64 candidates x {done} "
+                   f"iterations of the refinement phase in {el:.1f} s on {threads} threads")}
+    out["speedup_vs_cpu"] = out["refinement_64x600"]["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def prescreen_bench(ctx, args):
     """Learned prescreen at the config-5 shape: train on exact energies of
     shell binders around antigens of 100..1,000 residues, then screen a pool
@@ -346,6 +436,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-saturated", action="store_true")
     ap.add_argument("--no-maturation", action="store_true")
+    ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--tuning", default="",
                     help="fixed pair work split 'survivors,chunks,waves' instead of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -509,6 +600,11 @@ def main():
         out["maturation"] = maturation_bench(ctx, w, ags[0], (x1, y1, z1))
         log(f"[bench] maturation: {out['maturation']['ms_per_call']:.2f} ms per call, "
             f"{out['maturation']['positions']} positions")
+    if not args.no_refine:
+        log("[bench] refinement with Monte-Carlo moves")
+        out["refine"] = refine_bench(ctx, w, ags[0], (x1, y1, z1), idx)
+        log(f"[bench] refine: 64 x 600 in {out['refine']['refinement_64x600']['ms_per_call']:.1f} ms, "
+            f"1024 x 200 in {out['refine']['broad_1024x200']['ms_per_call']:.1f} ms")
     if not args.no_saturated:
         log("[bench] kernels with enough work (16,384 binders in contact)")
         out["roofline"]["saturated"] = {

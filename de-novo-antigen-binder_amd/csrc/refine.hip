@@ -59,12 +59,15 @@ struct RefTables {
   uint8_t cg_hyd[kNumAminoAcids];
 };
 
+static_assert(sizeof(RefTables) % 4 == 0, "tables are copied to LDS in 4-byte words");
+
 struct RefineArgs {
   const RefTables* tab;
   // antigen Calphas: positions, CA-atom parameters (rmh, eps, q, hyd + 256 type)
   const float *ag_x, *ag_y, *ag_z;
   const float4* ag_par;
-  uint32_t n_ag;
+  uint32_t n_ag, n_ag_lds;  // n_ag_lds: n_ag when the Calphas are copied to LDS, else 0
+  uint32_t ncells_lds;      // entries of the grid's cell_start copied with them
   GridView ag_grid;
   CgCloud ag_cg;
   float cx, cy, cz, r0;
@@ -89,20 +92,26 @@ __device__ inline float4 u4(uint32_t it, uint32_t r, uint32_t tag, const StepPar
                      u01_from_u32(u.v[3]));
 }
 
-// residue_contribution_allatom (src/diffusion.rs:422-456) of a Calpha of type
-// `aa` at p, lane-split over the antigen residues, FP64 total on every lane.
-__device__ inline double contribution(const RefineArgs& a, uint32_t aa, float px, float py,
-                                      float pz, uint32_t lane) {
-  const RefTables& T = *a.tab;
+// The antigen Calphas the residue energy runs over (in LDS when they fit).
+struct CaSet {
+  const float *x, *y, *z;
+  const float4* par;  // r_min_half, epsilon, charge, hydrophobic + 256 * type
+  uint32_t n;
+};
+
+// This lane's part of residue_contribution_allatom (src/diffusion.rs:422-456)
+// for a Calpha of type `aa` at p: the antigen Calphas lane, lane + 64, ...
+__device__ inline double pair_terms(const RefTables& T, const CaSet& g, uint32_t aa, float px,
+                                    float py, float pz, uint32_t lane) {
   const float brm = T.ca_rmh[aa], be = T.ca_eps[aa], bq = T.ca_q[aa];
   const uint8_t bh = T.ca_hyd[aa];
   const uint32_t bt = T.ca_type[aa];
   double e = 0.0;
-  for (uint32_t i = lane; i < a.n_ag; i += kWave) {
-    const float dx = px - a.ag_x[i], dy = py - a.ag_y[i], dz = pz - a.ag_z[i];
+  for (uint32_t i = lane; i < g.n; i += kWave) {
+    const float dx = px - g.x[i], dy = py - g.y[i], dz = pz - g.z[i];
     const float r_sq = dx * dx + dy * dy + dz * dz;
     if (r_sq > 100.0f || r_sq < 0.25f) continue;
-    const float4 q = a.ag_par[i];
+    const float4 q = g.par[i];
     const float r_ij = brm + q.x;
     const float eps = sqrtf(be * q.y);
     const float r2 = (r_ij * r_ij) / r_sq;
@@ -119,7 +128,13 @@ __device__ inline double contribution(const RefineArgs& a, uint32_t aa, float px
       e += static_cast<double>(-kHbDepth * expf(-0.5f * z * z));
     }
   }
-  return wave_sum_f64(e);
+  return e;
+}
+
+// residue_contribution_allatom, FP64 total on every lane.
+__device__ inline double contribution(const RefTables& T, const CaSet& g, uint32_t aa, float px,
+                                      float py, float pz, uint32_t lane) {
+  return wave_sum_f64(pair_terms(T, g, aa, px, py, pz, lane));
 }
 
 // Side chain of residue slots X/Y/Z (type aa, rotamer rot) from its
@@ -199,15 +214,15 @@ __device__ inline double disulfide(const uint8_t* aa, const float* X, const floa
 
 // Sum of residue_contribution_allatom over residues r..nr-1 plus the
 // disulfide term: the energy a torsion move at r is judged by.
-__device__ inline double chain_energy(const RefineArgs& a, const uint8_t* aa, const float* X,
-                                      const float* Y, const float* Z, uint32_t r, uint32_t nr,
-                                      uint32_t lane) {
-  double sum = 0.0;
+__device__ inline double chain_energy(const RefTables& T, const CaSet& g, const uint8_t* aa,
+                                      const float* X, const float* Y, const float* Z, uint32_t r,
+                                      uint32_t nr, uint32_t lane) {
+  double part = 0.0;  // one reduction for the whole chain
   for (uint32_t i = r; i < nr; ++i) {
     const uint32_t t = i * kSlots + 1;
-    sum += contribution(a, aa[i], X[t], Y[t], Z[t], lane);
+    part += pair_terms(T, g, aa[i], X[t], Y[t], Z[t], lane);
   }
-  return sum + disulfide(aa, X, Y, Z, nr);
+  return wave_sum_f64(part) + disulfide(aa, X, Y, Z, nr);
 }
 
 // perturb_phi / perturb_psi (src/allatom.rs:392-456) by rad: phi turns the
@@ -236,21 +251,69 @@ __device__ inline void torsion(const RefTables& T, const uint8_t* aa, float* X, 
   __syncthreads();
 }
 
+// kLds: the antigen Calpha set and grid live in LDS for the whole launch
+// (their addresses then resolve to LDS at compile time; a pointer that may
+// be either would turn every access into a flat one).
+template <bool kLds>
 __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
-  extern __shared__ float lds[];
+  extern __shared__ float lds_all[];
   const uint32_t c = blockIdx.x;
   const uint32_t lane = threadIdx.x;
   if (c >= a.n_cand) return;
-  const RefTables& T = *a.tab;
+  // the residue tables first (read throughout: NERF, energies), then the rest
+  constexpr uint32_t kTabWords = (sizeof(RefTables) + 15) / 16 * 4;
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.tab);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(lds_all);
+    for (uint32_t k = lane; k < sizeof(RefTables) / 4; k += kWave) dst[k] = src[k];
+  }
+  const RefTables& T = *reinterpret_cast<const RefTables*>(lds_all);
+  float* lds = lds_all + kTabWords;
   const uint32_t g0 = a.cand_res_off[c], nr = a.cand_res_off[c + 1] - g0;
   const uint32_t ms = a.max_res * kSlots;
-  float* X = lds;
+  // LDS: the antigen Calpha set, its residue-level parameters and grid (when
+  // they fit, n_ag_lds = n_ag), then the candidate's slots, forces, the
+  // step's random draws and per-residue bytes.
+  const uint32_t na_l = kLds ? a.n_ag : 0, nc_l = kLds ? a.ncells_lds : 0;
+  float4* gpar = reinterpret_cast<float4*>(lds);  // 16-byte entries first
+  float4* U = gpar + na_l;                         // 2 * max_res draws
+  float* gx = reinterpret_cast<float*>(U + 2 * a.max_res);
+  float* gy = gx + na_l;
+  float* gz = gy + na_l;
+  float* gq = gz + na_l;
+  float* ge = gq + na_l;
+  float* gs = ge + na_l;
+  uint32_t* gcs = reinterpret_cast<uint32_t*>(gs + na_l);
+  uint32_t* gsi = gcs + nc_l;
+  float* X = reinterpret_cast<float*>(gsi + na_l);
   float* Y = X + ms;
   float* Z = Y + ms;
-  float* F = Z + ms;  // 3 * max_res forces
-  uint8_t* aa = reinterpret_cast<uint8_t*>(F + 3 * a.max_res);
+  float* F = Z + ms;                 // 3 * max_res forces
+  uint8_t* gh = reinterpret_cast<uint8_t*>(F + 3 * a.max_res);
+  uint8_t* aa = gh + na_l;
   uint8_t* rot = aa + a.max_res;
   uint8_t* iface = rot + a.max_res;
+  if (kLds) {
+    for (uint32_t i = lane; i < a.n_ag; i += kWave) {
+      gpar[i] = a.ag_par[i];
+      gx[i] = a.ag_x[i];
+      gy[i] = a.ag_y[i];
+      gz[i] = a.ag_z[i];
+      gq[i] = a.ag_cg.q[i];
+      ge[i] = a.ag_cg.eps[i];
+      gs[i] = a.ag_cg.sig[i];
+      gh[i] = a.ag_cg.hyd[i];
+      gsi[i] = a.ag_grid.sorted_idx[i];
+    }
+    for (uint32_t k = lane; k < nc_l; k += kWave) gcs[k] = a.ag_grid.cell_start[k];
+  }
+  const CaSet g = kLds ? CaSet{gx, gy, gz, gpar, a.n_ag} : CaSet{a.ag_x, a.ag_y, a.ag_z, a.ag_par, a.n_ag};
+  const CgCloud cg = kLds ? CgCloud{gx, gy, gz, gq, ge, gs, gh} : a.ag_cg;
+  GridView grid = a.ag_grid;
+  if (kLds) {
+    grid.cell_start = gcs;
+    grid.sorted_idx = gsi;
+  }
   for (uint32_t t = lane; t < nr * kSlots; t += kWave) {
     const float4 v = a.slots[static_cast<size_t>(g0) * kSlots + t];
     X[t] = v.x;
@@ -271,7 +334,7 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
       const uint32_t t = r * kSlots + 1;
       const uint32_t k = aa[r];
       float fx, fy, fz;
-      cg_force_point(a.ag_grid, a.ag_cg, X[t], Y[t], Z[t], T.cg_q[k], T.cg_eps[k], T.cg_sig[k],
+      cg_force_point(grid, cg, X[t], Y[t], Z[t], T.cg_q[k], T.cg_eps[k], T.cg_sig[k],
                      T.cg_hyd[k], fx, fy, fz);
       F[3 * r] = fx;
       F[3 * r + 1] = fy;
@@ -294,17 +357,23 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
     for (uint32_t r = lane; r < nr; r += kWave) {
       const uint32_t t = r * kSlots + 1;
       bool in = false;
-      for (uint32_t i = 0; i < a.n_ag && !in; ++i) {
-        const float dx = X[t] - a.ag_x[i], dy = Y[t] - a.ag_y[i], dz = Z[t] - a.ag_z[i];
+      for (uint32_t i = 0; i < g.n && !in; ++i) {
+        const float dx = X[t] - g.x[i], dy = Y[t] - g.y[i], dz = Z[t] - g.z[i];
         in = dx * dx + dy * dy + dz * dz < a.iface_sq;
       }
       iface[r] = in ? 1 : 0;
     }
     __syncthreads();
+    // this step's draws of both move kinds, one residue per lane
+    for (uint32_t r = lane; r < nr; r += kWave) {
+      U[2 * r] = u4(it, g0 + r, kTagMc, a.sp);
+      U[2 * r + 1] = u4(it, g0 + r, kTagBb, a.sp);
+    }
+    __syncthreads();
     // 3. side-chain moves (src/diffusion.rs:353-398)
     if (a.moves & DNAB_MC_SIDECHAIN) {
       for (uint32_t r = 0; r < nr; ++r) {
-        const float4 u = u4(it, g0 + r, kTagMc, a.sp);
+        const float4 u = U[2 * r];
         const bool in = iface[r] != 0;
         const float eff_mut = in ? a.mut_p * 3.0f : a.mut_p * 0.4f;
         const float eff_rot = in ? a.rot_p * 2.0f : a.rot_p;
@@ -321,7 +390,7 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
                        static_cast<uint32_t>(kNumAminoAcids - 1));
           new_rot = 0;
         }
-        const double old_e = contribution(a, old_aa, X[t], Y[t], Z[t], lane) +
+        const double old_e = contribution(T, g, old_aa, X[t], Y[t], Z[t], lane) +
                              disulfide(aa, X, Y, Z, nr);
         __syncthreads();
         if (lane == 0) {
@@ -334,7 +403,7 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
           }
         }
         __syncthreads();
-        const double new_e = contribution(a, aa[r], X[t], Y[t], Z[t], lane) +
+        const double new_e = contribution(T, g, aa[r], X[t], Y[t], Z[t], lane) +
                              disulfide(aa, X, Y, Z, nr);
         const double delta = new_e - old_e;
         const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
@@ -354,14 +423,14 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
     // 4. torsion moves (src/diffusion.rs:400-415)
     if (a.moves & DNAB_MC_BACKBONE) {
       for (uint32_t r = 0; r < nr; ++r) {
-        const float4 u = u4(it, g0 + r, kTagBb, a.sp);
+        const float4 u = U[2 * r + 1];
         if (!(u.x < a.bb_p)) continue;
         const float deg = -a.max_torsion + (a.max_torsion + a.max_torsion) * u.y;
         const float rad = geo::deg2rad(deg);
         const bool phi = u.z < 0.5f;
-        const double old_e = chain_energy(a, aa, X, Y, Z, r, nr, lane);
+        const double old_e = chain_energy(T, g, aa, X, Y, Z, r, nr, lane);
         torsion(T, aa, X, Y, Z, r, nr, rad, phi, lane);
-        const double new_e = chain_energy(a, aa, X, Y, Z, r, nr, lane);
+        const double new_e = chain_energy(T, g, aa, X, Y, Z, r, nr, lane);
         const double delta = new_e - old_e;
         const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
         if (!accept) torsion(T, aa, X, Y, Z, r, nr, -rad, phi, lane);  // rotate back
@@ -578,14 +647,28 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   a.max_torsion = mc->max_torsion_deg;
   a.iface_sq = mc->iface_cutoff_sq;
   a.moves = mc->moves;
-  const size_t lds = static_cast<size_t>(max_res) * (3 * kSlots + 3) * sizeof(float) + 3 * max_res;
+  size_t lds = (sizeof(RefTables) + 15) / 16 * 16 +
+               static_cast<size_t>(max_res) * ((3 * kSlots + 3) * sizeof(float) + 2 * 16 + 3);
+  const size_t ncells = ag->ca_grid.n ? ag->ca_grid.ncodes + 1 : 0;
+  const size_t ag_bytes = n_ag * (16 + 6 * 4 + 4 + 1) + ncells * 4;
+  if (ncells && lds + ag_bytes <= size_t(120) << 10) {
+    a.n_ag_lds = static_cast<uint32_t>(n_ag);
+    a.ncells_lds = static_cast<uint32_t>(ncells);
+    lds += ag_bytes;
+  }
+  const void* kfn = a.n_ag_lds ? reinterpret_cast<const void*>(&k_refine<true>)
+                              : reinterpret_cast<const void*>(&k_refine<false>);
   if (lds > 65536)
-    DNAB_HIP_CHECK(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_refine),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+    DNAB_HIP_CHECK(ctx, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             static_cast<int>(lds)));
   hipEvent_t ev_end = nullptr;
   hipEvent_t ev = timer_begin(ctx, "refine", &ev_end);
-  hipLaunchKernelGGL(k_refine, dim3(static_cast<uint32_t>(nc)), dim3(kWave), lds, ctx->stream, a);
+  if (a.n_ag_lds)
+    hipLaunchKernelGGL(k_refine<true>, dim3(static_cast<uint32_t>(nc)), dim3(kWave), lds,
+                       ctx->stream, a);
+  else
+    hipLaunchKernelGGL(k_refine<false>, dim3(static_cast<uint32_t>(nc)), dim3(kWave), lds,
+                       ctx->stream, a);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   timer_end(ctx, "refine", ev, ev_end);
   DNAB_RETURN_IF(download(ctx, slots.data(), d_slots.p, slots.size() * sizeof(float4)));

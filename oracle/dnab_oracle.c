@@ -844,3 +844,408 @@ void or_langevin_step(float* x, float* y, float* z, size_t n_res, const uint32_t
   }
   free(keys);
 }
+
+/* ---- allatom_diffusion_step with Monte-Carlo moves, C port -----------------
+ * The same algorithm as oracle/refine_oracle.py (src/diffusion.rs:290-416),
+ * written for speed as the CPU baseline of the refinement: candidates in
+ * parallel (the Rayon loop of src/diffusion.rs:1083-1095), residues in 14
+ * atom slots (the splice of mutate_residue, src/allatom.rs:299-384), the
+ * antigen Calpha grid as cell lists (src/spatial.rs:12-68), NERF from the
+ * tables the caller passes (src/rotamer.rs), Philox draws by (iteration,
+ * residue, kind), Metropolis energies as FP64 sums of f32 terms. */
+
+#define RF_SLOTS 14
+#define RF_DEFS 12
+
+typedef struct {
+  const uint8_t* natoms;  /* 20: atoms per residue type */
+  const uint8_t* ndef;    /* 20: side-chain placements */
+  const float* defs;      /* 20 x 12 x 7: a, b, c, bond, angle, fixed dihedral, chi (255 fixed) */
+  const uint8_t* nrot;    /* 20 */
+  const float* chi;       /* 20 x 5 x 4 degrees */
+  const float* ca_par;    /* 20 x 5: Calpha r_min_half, epsilon, charge, hydrophobic, type */
+  const float* cg_par;    /* 20 x 4: residue-level charge, epsilon, sigma, hydrophobic */
+  const uint8_t* hb_don;  /* 16 */
+  const uint8_t* hb_acc;  /* 16 */
+} or_rf_tables;
+
+typedef struct {
+  float step, max_disp, restraint_k, restraint_radius;
+  int with_noise;
+  uint32_t seed_lo, seed_hi;
+  float mut_p, rot_p, bb_p, max_torsion, iface_sq;
+  uint32_t moves;
+  float center[3];
+} or_rf_params;
+
+typedef struct {
+  float x, y, z;
+} rf_v3;
+
+static rf_v3 rf_sub(rf_v3 a, rf_v3 b) { rf_v3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static rf_v3 rf_cross(rf_v3 a, rf_v3 b) {
+  rf_v3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  return r;
+}
+static rf_v3 rf_norm(rf_v3 v) {
+  float l = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+  if (l < 1e-8f) l = 1e-8f;
+  rf_v3 r = {v.x / l, v.y / l, v.z / l};
+  return r;
+}
+static float rf_rad(float d) { return d * (3.14159265358979323846f / 180.0f); }
+
+/* place_atom (src/rotamer.rs:198-232) */
+static rf_v3 rf_place(rf_v3 a, rf_v3 b, rf_v3 c, float bond, float ang_deg, float dih_deg) {
+  float ang = rf_rad(ang_deg), dih = rf_rad(dih_deg);
+  rf_v3 bc = rf_norm(rf_sub(b, c));
+  rf_v3 n = rf_norm(rf_cross(rf_sub(b, a), bc));
+  rf_v3 m = rf_cross(bc, n);
+  float sa = sinf(ang), ca = cosf(ang), sd = sinf(dih), cd = cosf(dih);
+  rf_v3 d = {c.x + bond * (-ca * bc.x + sa * cd * m.x + sa * sd * n.x),
+             c.y + bond * (-ca * bc.y + sa * cd * m.y + sa * sd * n.y),
+             c.z + bond * (-ca * bc.z + sa * cd * m.z + sa * sd * n.z)};
+  return d;
+}
+
+static void rf_side_chain(const or_rf_tables* T, int aa, int rot, int nb, rf_v3* s) {
+  int n = nb;
+  for (int k = 0; k < T->ndef[aa] && n < T->natoms[aa]; ++k) {
+    const float* d = T->defs + ((size_t)aa * RF_DEFS + k) * 7;
+    int ci = (int)d[6];
+    float dih = ci == 255 ? d[5] : (T->nrot[aa] ? T->chi[((size_t)aa * 5 + rot) * 4 + ci] : 0.0f);
+    s[n] = rf_place(s[(int)d[0]], s[(int)d[1]], s[(int)d[2]], d[3], d[4], dih);
+    ++n;
+  }
+}
+
+/* mutate_residue (src/allatom.rs:299-384) on one residue's slots */
+static void rf_mutate(const or_rf_tables* T, uint8_t* aa, uint8_t* rot, int r, int new_aa,
+                      int new_rot, rf_v3* slots) {
+  rf_v3* s = slots + (size_t)r * RF_SLOTS;
+  if (aa[r] == new_aa) {
+    rot[r] = (uint8_t)new_rot;
+    rf_side_chain(T, new_aa, new_rot, new_aa == 7 ? 4 : 5, s);
+    return;
+  }
+  const float cx = s[1].x, cy = s[1].y, cz = s[1].z;
+  rf_v3 bb[5] = {{cx - 1.46f, cy, cz}, {cx, cy, cz}, {cx + 1.52f, cy, cz}, {0, 0, 0}, {cx, cy - 1.52f, cz}};
+  bb[3].x = bb[2].x + 0.5f;
+  bb[3].y = bb[2].y + 1.23f * 0.9f;
+  bb[3].z = bb[2].z;
+  int nb = new_aa == 7 ? 4 : 5;
+  if (nb > T->natoms[new_aa]) nb = T->natoms[new_aa];
+  for (int k = 0; k < nb; ++k) s[k] = bb[k];
+  aa[r] = (uint8_t)new_aa;
+  rot[r] = (uint8_t)new_rot;
+  rf_side_chain(T, new_aa, new_rot, nb, s);
+}
+
+/* the antigen Calpha grid as cell lists over the key bounding box */
+typedef struct {
+  int32_t kmin[3], dim[3];
+  uint32_t* start; /* cells + 1 */
+  uint32_t* idx;
+  float inv;
+} rf_grid;
+
+static int rf_grid_build(rf_grid* g, const or_rcloud* ag) {
+  g->inv = 1.0f / 10.0f;
+  int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  int32_t* keys = (int32_t*)malloc(3 * (ag->n ? ag->n : 1) * sizeof(int32_t));
+  if (!keys) return -1;
+  for (size_t i = 0; i < ag->n; ++i) {
+    const float* c[3] = {ag->x, ag->y, ag->z};
+    for (int k = 0; k < 3; ++k) {
+      keys[3 * i + k] = or_cell_of(c[k][i], g->inv);
+      if (keys[3 * i + k] < lo[k]) lo[k] = keys[3 * i + k];
+      if (keys[3 * i + k] > hi[k]) hi[k] = keys[3 * i + k];
+    }
+  }
+  size_t cells = 1;
+  for (int k = 0; k < 3; ++k) {
+    g->kmin[k] = ag->n ? lo[k] : 0;
+    g->dim[k] = ag->n ? hi[k] - lo[k] + 1 : 1;
+    cells *= (size_t)g->dim[k];
+  }
+  g->start = (uint32_t*)calloc(cells + 1, sizeof(uint32_t));
+  g->idx = (uint32_t*)malloc((ag->n ? ag->n : 1) * sizeof(uint32_t));
+  if (!g->start || !g->idx) { free(keys); return -1; }
+  for (size_t i = 0; i < ag->n; ++i) {
+    size_t c = ((size_t)(keys[3 * i] - g->kmin[0]) * g->dim[1] + (keys[3 * i + 1] - g->kmin[1])) *
+                   g->dim[2] + (keys[3 * i + 2] - g->kmin[2]);
+    g->start[c + 1]++;
+  }
+  for (size_t c = 0; c < cells; ++c) g->start[c + 1] += g->start[c];
+  uint32_t* fill = (uint32_t*)calloc(cells, sizeof(uint32_t));
+  for (size_t i = 0; i < ag->n; ++i) {  /* ascending index within a cell (src/spatial.rs:35-38) */
+    size_t c = ((size_t)(keys[3 * i] - g->kmin[0]) * g->dim[1] + (keys[3 * i + 1] - g->kmin[1])) *
+                   g->dim[2] + (keys[3 * i + 2] - g->kmin[2]);
+    g->idx[g->start[c] + fill[c]++] = (uint32_t)i;
+  }
+  free(fill);
+  free(keys);
+  return 0;
+}
+
+/* compute_forces_with_grid for one Calpha (src/energy.rs:322-398), the
+ * reference's 27-cell order */
+static void rf_cg_force(const rf_grid* g, const or_rcloud* ag, float bx, float by, float bz,
+                        const float* cg, float* fx, float* fy, float* fz) {
+  float ax = 0, ay = 0, az = 0;
+  int32_t c0 = or_cell_of(bx, g->inv), c1 = or_cell_of(by, g->inv), c2 = or_cell_of(bz, g->inv);
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        int64_t k0 = (int64_t)wrap_add(c0, dx) - g->kmin[0], k1 = (int64_t)wrap_add(c1, dy) - g->kmin[1],
+                k2 = (int64_t)wrap_add(c2, dz) - g->kmin[2];
+        if (k0 < 0 || k1 < 0 || k2 < 0 || k0 >= g->dim[0] || k1 >= g->dim[1] || k2 >= g->dim[2]) continue;
+        size_t c = ((size_t)k0 * g->dim[1] + (size_t)k1) * g->dim[2] + (size_t)k2;
+        for (uint32_t k = g->start[c]; k < g->start[c + 1]; ++k)
+          cg_pair(ag, g->idx[k], bx, by, bz, cg[0], cg[1], cg[2], (uint8_t)cg[3], &ax, &ay, &az);
+      }
+  *fx = ax;
+  *fy = ay;
+  *fz = az;
+}
+
+/* residue_contribution_allatom (src/diffusion.rs:422-456) */
+static double rf_contribution(const or_rf_tables* T, const or_rcloud* ag, const uint8_t* ag_aa,
+                              int aa, rf_v3 p) {
+  const float* b = T->ca_par + (size_t)aa * 5;
+  const float brm = b[0], be = b[1], bq = b[2];
+  const int bh = (int)b[3], bt = (int)b[4];
+  double e = 0.0;
+  for (size_t i = 0; i < ag->n; ++i) {
+    float dx = p.x - ag->x[i], dy = p.y - ag->y[i], dz = p.z - ag->z[i];
+    float r_sq = dx * dx + dy * dy + dz * dz;
+    if (r_sq > 100.0f || r_sq < 0.25f) continue;
+    const float* a = T->ca_par + (size_t)ag_aa[i] * 5;
+    float r_ij = brm + a[0];
+    float eps = sqrtf(be * a[1]);
+    float r2 = (r_ij * r_ij) / r_sq;
+    float r6 = r2 * r2 * r2;
+    e += (double)(eps * (r6 * r6 - 2.0f * r6));
+    if (bq != 0.0f && a[2] != 0.0f) e += (double)(332.0f * bq * a[2] / sqrtf(r_sq));
+    if (bh == 1 && (int)a[3] == 1 && r_sq < 36.0f) e -= 0.5;
+    int at = (int)a[4];
+    if ((T->hb_don[bt] && T->hb_acc[at]) || (T->hb_acc[bt] && T->hb_don[at])) {
+      if (r_sq <= 25.0f) {
+        float z = (sqrtf(r_sq) - 2.9f) / 0.35f;
+        e += (double)(-1.5f * expf(-0.5f * z * z));
+      }
+    }
+  }
+  return e;
+}
+
+static double rf_disulfide(const uint8_t* aa, const rf_v3* s, int nr) {
+  double e = 0.0;
+  for (int i = 0; i < nr; ++i) {
+    if (aa[i] != 4) continue;
+    rf_v3 a = s[(size_t)i * RF_SLOTS + 5];
+    for (int j = i + 1; j < nr; ++j) {
+      if (aa[j] != 4) continue;
+      rf_v3 b = s[(size_t)j * RF_SLOTS + 5];
+      float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+      if (sqrtf(dx * dx + dy * dy + dz * dz) > 4.0f) continue;
+      double ex = (double)a.x - b.x, ey = (double)a.y - b.y, ez = (double)a.z - b.z;
+      double t = (sqrt(ex * ex + ey * ey + ez * ez) - (double)2.05f) / (double)0.4f;
+      e -= 4.0 * exp(-0.5 * t * t);
+    }
+  }
+  return e;
+}
+
+static void rf_u4(uint32_t it, uint32_t r, uint32_t tag, const or_rf_params* p, float u[4]) {
+  uint32_t ctr[4] = {it, r, tag, 0u}, key[2] = {p->seed_lo, p->seed_hi}, o[4];
+  or_philox4x32_10(ctr, key, o);
+  for (int k = 0; k < 4; ++k) u[k] = (float)(o[k] >> 8) * (1.0f / 16777216.0f);
+}
+
+/* perturb_phi / perturb_psi (src/allatom.rs:392-456) with rodrigues (467-479) */
+static void rf_torsion(const or_rf_tables* T, const uint8_t* aa, rf_v3* s, int r, int nr,
+                       float rad, int phi) {
+  const rf_v3* b = s + (size_t)r * RF_SLOTS;
+  rf_v3 p0 = phi ? b[0] : b[1], p1 = phi ? b[1] : b[2];
+  rf_v3 axis = rf_norm(rf_sub(p1, p0));
+  float cs = (float)cos((double)rad), sn = (float)sin((double)rad);
+  for (int rr = r; rr < nr; ++rr)
+    for (int k = 0; k < T->natoms[aa[rr]]; ++k) {
+      if (rr == r && (phi ? k < 2 : k != 3)) continue;
+      rf_v3* q = s + (size_t)rr * RF_SLOTS + k;
+      rf_v3 v = rf_sub(*q, p1);
+      float dot = v.x * axis.x + v.y * axis.y + v.z * axis.z;
+      rf_v3 cx = rf_cross(axis, v);
+      float omc = 1.0f - cs;
+      q->x = (v.x * cs + cx.x * sn + axis.x * dot * omc) + p1.x;
+      q->y = (v.y * cs + cx.y * sn + axis.y * dot * omc) + p1.y;
+      q->z = (v.z * cs + cx.z * sn + axis.z * dot * omc) + p1.z;
+    }
+}
+
+typedef struct {
+  const or_rf_tables* T;
+  const or_rcloud* ag;
+  const uint8_t* ag_aa;
+  const rf_grid* grid;
+  const uint32_t* cand_res_off;
+  uint8_t* aa;
+  uint8_t* rot;
+  rf_v3* slots;
+  const float* temp;
+  const float* sigma;
+  uint32_t n_steps, iter0;
+  const or_rf_params* p;
+  size_t c0, c1;
+} rf_job;
+
+static void rf_candidate(const rf_job* J, size_t c) {
+  const or_rf_tables* T = J->T;
+  const or_rf_params* p = J->p;
+  const uint32_t g0 = J->cand_res_off[c];
+  const int nr = (int)(J->cand_res_off[c + 1] - g0);
+  uint8_t* aa = J->aa + g0;
+  uint8_t* rot = J->rot + g0;
+  rf_v3* s = J->slots + (size_t)g0 * RF_SLOTS;
+  uint8_t* iface = (uint8_t*)malloc(nr ? nr : 1);
+  float* F = (float*)malloc(3 * (nr ? nr : 1) * sizeof(float));
+  for (uint32_t st = 0; st < J->n_steps; ++st) {
+    const uint32_t it = J->iter0 + st;
+    const float temp = J->temp[st], sigma = J->sigma[st];
+    for (int r = 0; r < nr; ++r) {
+      rf_v3 ca = s[(size_t)r * RF_SLOTS + 1];
+      rf_cg_force(J->grid, J->ag, ca.x, ca.y, ca.z, T->cg_par + (size_t)aa[r] * 4, &F[3 * r],
+                  &F[3 * r + 1], &F[3 * r + 2]);
+    }
+    for (int r = 0; r < nr; ++r) {  /* src/diffusion.rs:318-338 */
+      rf_v3 ca = s[(size_t)r * RF_SLOTS + 1];
+      float fx = F[3 * r], fy = F[3 * r + 1], fz = F[3 * r + 2];
+      float dx_c = ca.x - p->center[0], dy_c = ca.y - p->center[1], dz_c = ca.z - p->center[2];
+      float d = sqrtf(dx_c * dx_c + dy_c * dy_c + dz_c * dz_c);
+      if (d < 0.001f) d = 0.001f;
+      float f_r = -p->restraint_k * (d - p->restraint_radius) / d;
+      fx += f_r * dx_c;
+      fy += f_r * dy_c;
+      fz += f_r * dz_c;
+      float nx = 0, ny = 0, nz = 0;
+      if (p->with_noise) {
+        float u[4];
+        rf_u4(it, g0 + (uint32_t)r, 0x6e6f6973u, p, u);
+        float two = sigma + sigma;
+        nx = u[0] * two - sigma;
+        ny = u[1] * two - sigma;
+        nz = u[2] * two - sigma;
+      }
+      float ddx = (ca.x + clampf_rust(p->step * fx, -p->max_disp, p->max_disp) + nx) - ca.x;
+      float ddy = (ca.y + clampf_rust(p->step * fy, -p->max_disp, p->max_disp) + ny) - ca.y;
+      float ddz = (ca.z + clampf_rust(p->step * fz, -p->max_disp, p->max_disp) + nz) - ca.z;
+      for (int k = 0; k < T->natoms[aa[r]]; ++k) {
+        rf_v3* q = s + (size_t)r * RF_SLOTS + k;
+        q->x += ddx;
+        q->y += ddy;
+        q->z += ddz;
+      }
+    }
+    for (int r = 0; r < nr; ++r) {  /* src/diffusion.rs:341-351 */
+      rf_v3 ca = s[(size_t)r * RF_SLOTS + 1];
+      int in = 0;
+      for (size_t i = 0; i < J->ag->n && !in; ++i) {
+        float dx = ca.x - J->ag->x[i], dy = ca.y - J->ag->y[i], dz = ca.z - J->ag->z[i];
+        in = dx * dx + dy * dy + dz * dz < p->iface_sq;
+      }
+      iface[r] = (uint8_t)in;
+    }
+    if (p->moves & 1u) {  /* src/diffusion.rs:353-398 */
+      for (int r = 0; r < nr; ++r) {
+        float u[4];
+        rf_u4(it, g0 + (uint32_t)r, 0x6d636d76u, p, u);
+        float eff_mut = iface[r] ? p->mut_p * 3.0f : p->mut_p * 0.4f;
+        float eff_rot = iface[r] ? p->rot_p * 2.0f : p->rot_p;
+        if (!(u[0] < eff_mut)) continue;
+        int old_aa = aa[r], old_rot = rot[r];
+        rf_v3* sr = s + (size_t)r * RF_SLOTS;
+        if (u[1] < eff_rot) {
+          int nrot = T->nrot[old_aa];
+          if (!nrot) continue;
+          int nrt = (int)(u[2] * (float)nrot);
+          if (nrt > nrot - 1) nrt = nrot - 1;
+          double old_e = rf_contribution(T, J->ag, J->ag_aa, old_aa, sr[1]) + rf_disulfide(aa, s, nr);
+          rot[r] = (uint8_t)nrt;
+          rf_side_chain(T, old_aa, nrt, old_aa == 7 ? 4 : 5, sr);
+          double new_e = rf_contribution(T, J->ag, J->ag_aa, old_aa, sr[1]) + rf_disulfide(aa, s, nr);
+          double delta = new_e - old_e;
+          if (!(delta <= 0.0 || (double)u[3] < exp(-delta / temp))) {
+            rot[r] = (uint8_t)old_rot;
+            rf_side_chain(T, old_aa, old_rot, old_aa == 7 ? 4 : 5, sr);
+          }
+        } else {
+          int new_aa = (int)(u[2] * 20.0f);
+          if (new_aa > 19) new_aa = 19;
+          double old_e = rf_contribution(T, J->ag, J->ag_aa, old_aa, sr[1]) + rf_disulfide(aa, s, nr);
+          rf_mutate(T, aa, rot, r, new_aa, 0, s);
+          double new_e = rf_contribution(T, J->ag, J->ag_aa, aa[r], sr[1]) + rf_disulfide(aa, s, nr);
+          double delta = new_e - old_e;
+          if (!(delta <= 0.0 || (double)u[3] < exp(-delta / temp)))
+            rf_mutate(T, aa, rot, r, old_aa, old_rot, s);
+        }
+      }
+    }
+    if (p->moves & 2u) {  /* src/diffusion.rs:400-415 */
+      for (int r = 0; r < nr; ++r) {
+        float u[4];
+        rf_u4(it, g0 + (uint32_t)r, 0x62626d76u, p, u);
+        if (!(u[0] < p->bb_p)) continue;
+        float rad = rf_rad(-p->max_torsion + (p->max_torsion + p->max_torsion) * u[1]);
+        int phi = u[2] < 0.5f;
+        double e[2];
+        for (int pass = 0; pass < 2; ++pass) {
+          double sum = 0.0;
+          for (int i = r; i < nr; ++i)
+            sum += rf_contribution(T, J->ag, J->ag_aa, aa[i], s[(size_t)i * RF_SLOTS + 1]);
+          e[pass] = sum + rf_disulfide(aa, s, nr);
+          if (pass == 0) rf_torsion(T, aa, s, r, nr, rad, phi);
+        }
+        double delta = e[1] - e[0];
+        if (!(delta <= 0.0 || (double)u[3] < exp(-delta / temp))) rf_torsion(T, aa, s, r, nr, -rad, phi);
+      }
+    }
+  }
+  free(iface);
+  free(F);
+}
+
+static void* rf_worker(void* arg) {
+  const rf_job* J = (const rf_job*)arg;
+  for (size_t c = J->c0; c < J->c1; ++c) rf_candidate(J, c);
+  return NULL;
+}
+
+/* Refines n_cand candidates in place: slots (n_res x 14 x 3 floats), residue
+ * types and rotamers; temp/sigma per step. Returns 0 or -1. */
+int or_refine_run(const or_rf_tables* T, const or_rcloud* ag, const uint8_t* ag_aa,
+                  size_t n_cand, const uint32_t* cand_res_off, uint8_t* res_aa, uint8_t* res_rot,
+                  float* slots, const float* temp, const float* sigma, uint32_t n_steps,
+                  uint32_t iter0, const or_rf_params* p, int nthreads) {
+  rf_grid g;
+  memset(&g, 0, sizeof(g));
+  if (rf_grid_build(&g, ag)) return -1;
+  if (nthreads < 1) nthreads = 1;
+  if ((size_t)nthreads > n_cand) nthreads = n_cand ? (int)n_cand : 1;
+  pthread_t th[256];
+  rf_job jobs[256];
+  if (nthreads > 256) nthreads = 256;
+  size_t per = (n_cand + nthreads - 1) / nthreads;
+  for (int k = 0; k < nthreads; ++k) {
+    rf_job J = {T, ag, ag_aa, &g, cand_res_off, res_aa, res_rot, (rf_v3*)slots, temp, sigma,
+                n_steps, iter0, p, 0, 0};
+    J.c0 = (size_t)k * per < n_cand ? (size_t)k * per : n_cand;
+    J.c1 = J.c0 + per < n_cand ? J.c0 + per : n_cand;
+    jobs[k] = J;
+    pthread_create(&th[k], NULL, rf_worker, &jobs[k]);
+  }
+  for (int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
+  free(g.start);
+  free(g.idx);
+  return 0;
+}

@@ -84,6 +84,9 @@ def _load():
         "or_philox4x32_10": (None, [u32p, u32p, u32p]),
         "or_sigma": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]),
         "or_temperature": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
+        "or_refine_run": (C.c_int, [C.c_void_p, C.POINTER(OrRCloud), u8p, C.c_size_t, u32p, u8p,
+                                    u8p, f32p, f32p, f32p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                    C.c_int]),
         "or_langevin_step": (None, [f32p, f32p, f32p, C.c_size_t, u32p, u32p, f32p, f32p, f32p,
                                     C.POINTER(OrRCloud), f32p, f32p, f32p, u8p, C.c_uint32,
                                     C.POINTER(OrStepParams)]),
@@ -415,3 +418,72 @@ def affinity_maturation_f64(ag, residues, aa, rot, ca, mutable, build_residue, c
                 break
             residues[r], aa[r], rot[r] = build_residue(old, old_rot, ca[r]), old, old_rot
     return best
+
+
+# ---- refinement (allatom_diffusion_step), the C port ----
+
+class OrRfTables(C.Structure):
+    _fields_ = [("natoms", u8p), ("ndef", u8p), ("defs", f32p), ("nrot", u8p), ("chi", f32p),
+                ("ca_par", f32p), ("cg_par", f32p), ("hb_don", u8p), ("hb_acc", u8p)]
+
+
+class OrRfParams(C.Structure):
+    _fields_ = [("step", C.c_float), ("max_disp", C.c_float), ("restraint_k", C.c_float),
+                ("restraint_radius", C.c_float), ("with_noise", C.c_int), ("seed_lo", C.c_uint32),
+                ("seed_hi", C.c_uint32), ("mut_p", C.c_float), ("rot_p", C.c_float),
+                ("bb_p", C.c_float), ("max_torsion", C.c_float), ("iface_sq", C.c_float),
+                ("moves", C.c_uint32), ("center", C.c_float * 3)]
+
+
+def refine_tables(cg_par):
+    """Tables of the refinement port from the committed fixture (residue
+    topologies, rotamers, side-chain placements, H-bond classes) and the
+    residue-level parameters cg_par (20 x 4: charge, epsilon, sigma,
+    hydrophobic) the caller supplies."""
+    tab = reference_tables()
+    natoms = np.array([len(r) for r in tab["residues"]], np.uint8)
+    ndef = np.array([len(d) for d in tab["sidechain"]], np.uint8)
+    defs = np.zeros((20, 12, 7), np.float32)
+    for a, d in enumerate(tab["sidechain"]):
+        if d:
+            defs[a, :len(d)] = np.asarray(d, np.float32)
+    nrot = np.array([len(r) for r in tab["rotamers"]], np.uint8)
+    chi = np.zeros((20, 5, 4), np.float32)
+    for a, r in enumerate(tab["rotamers"]):
+        if r:
+            chi[a, :len(r)] = np.asarray(r, np.float32)
+    nb = np.asarray(tab["amber_nb"], np.float32)
+    ca = np.zeros((20, 5), np.float32)
+    for a, atoms in enumerate(tab["residues"]):
+        t, q, h = atoms[1]
+        ca[a] = (nb[t, 0], nb[t, 1], q, h, t)
+    keep = [natoms, ndef, np.ascontiguousarray(defs), nrot, np.ascontiguousarray(chi),
+            np.ascontiguousarray(ca), np.ascontiguousarray(cg_par, np.float32),
+            np.asarray(tab["hbond_donor"], np.uint8), np.asarray(tab["hbond_acceptor"], np.uint8)]
+    t = OrRfTables(_p(keep[0], C.c_uint8), _p(keep[1], C.c_uint8), _p(keep[2], C.c_float),
+                   _p(keep[3], C.c_uint8), _p(keep[4], C.c_float), _p(keep[5], C.c_float),
+                   _p(keep[6], C.c_float), _p(keep[7], C.c_uint8), _p(keep[8], C.c_uint8))
+    return t, keep
+
+
+def refine_run(tables, ag_ca, ag_aa, cand_res_off, res_aa, res_rot, slots, temp, sigma, iter0,
+               params: dict, nthreads=1):
+    """The C port of allatom_diffusion_step over len(temp) iterations, in
+    place on res_aa, res_rot (uint8) and slots (n_res x 14 x 3 float32)."""
+    t, _keep = tables
+    oc, k1 = rcloud(ag_ca)
+    p = OrRfParams()
+    for k, v in params.items():
+        if k == "center":
+            for i in range(3):
+                p.center[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    ag_aa = np.ascontiguousarray(ag_aa, np.uint8)
+    off = np.ascontiguousarray(cand_res_off, np.uint32)
+    temp, sigma = _f32(temp), _f32(sigma)
+    rc = lib.or_refine_run(C.addressof(t), C.byref(oc), _p(ag_aa, C.c_uint8), len(off) - 1,
+                           _p(off, C.c_uint32), _p(res_aa, C.c_uint8), _p(res_rot, C.c_uint8),
+                           _p(slots, C.c_float), _p(temp, C.c_float), _p(sigma, C.c_float),
+                           len(temp), iter0, C.addressof(p), int(nthreads))
+    assert rc == 0

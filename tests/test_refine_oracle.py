@@ -178,3 +178,42 @@ def test_metropolis_moves_happen_and_keep_residues_whole():
         assert len(r) == len(tab["residues"][a])   # mutate_residue keeps topology counts
         # bond N-CA of an ideal or rotated backbone keeps its length
         assert abs(np.linalg.norm(r[1].astype(np.float64) - r[0]) - 1.46) < 1e-3
+
+
+def _cg_table():
+    return np.array([_cg(a) for a in range(20)], np.float32)
+
+
+def _slots_of(res):
+    s = np.zeros((len(res), 14, 3), np.float32)
+    for r, xyz in enumerate(res):
+        s[r, :len(xyz)] = xyz
+    return s
+
+
+def test_c_port_matches_the_literal_restatement():
+    # the C port of the step (the CPU baseline) against refine_oracle.step:
+    # same draws, same f32 geometry, same decisions
+    tables = O.refine_tables(_cg_table())
+    for moves, noise in ((3, 1), (1, 0), (2, 1)):
+        agd, state = _setup(nb=6, seed=moves * 7 + noise)
+        mc = dict(mutation_p=0.5, rotamer_move_p=0.4, backbone_move_p=0.4, max_torsion_deg=15.0,
+                  iface_cutoff_sq=400.0, moves=moves)
+        lp = dict(LP, with_noise=noise)
+        slots = _slots_of(state["res"])
+        aa = np.array(state["aa"], np.uint8)
+        rot = np.array(state["rot"], np.uint8)
+        steps, total = 5, 600
+        temps = [O.temperature(it, total) for it in range(steps)]
+        sigs = [O.sigma(it, total) if noise else 0.0 for it in range(steps)]
+        for it in range(steps):
+            R.step(state, it, np.float32(temps[it]), np.float32(sigs[it]), agd, mc, lp, build,
+                   _rebuild_cb, _cg)
+        O.refine_run(tables, agd["ca"], agd["aa"], [0, len(aa)], aa, rot, slots, temps, sigs, 0,
+                     dict(step=0.08, max_disp=2.0, restraint_k=0.02, restraint_radius=20.0,
+                          with_noise=noise, seed_lo=11, seed_hi=0, mut_p=0.5, rot_p=0.4,
+                          bb_p=0.4, max_torsion=15.0, iface_sq=400.0, moves=moves,
+                          center=agd["center"]))
+        assert list(aa) == state["aa"] and list(rot) == state["rot"], moves
+        for r, xyz in enumerate(state["res"]):
+            assert np.array_equal(slots[r, :len(xyz)].view(np.uint32), xyz.view(np.uint32)), (moves, r)
