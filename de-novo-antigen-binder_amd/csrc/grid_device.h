@@ -29,15 +29,18 @@ struct CgCloud {
   const uint8_t* hyd;
 };
 
-__device__ inline void cg_pair(const CgCloud& ag, uint32_t i, float bx, float by, float bz,
-                               float bq, float be, float bs, uint8_t bh, float& ax, float& ay,
-                               float& az) {
+// The force term of antigen point i on the antibody point b: false when the
+// pair is outside the reference's predicate; else (tx, ty, tz) = f * d, the
+// products the reference adds (src/energy.rs:353-391).
+__device__ inline bool cg_term(const CgCloud& ag, uint32_t i, float bx, float by, float bz,
+                               float bq, float be, float bs, uint8_t bh, float& tx, float& ty,
+                               float& tz) {
   const float kCutoffSq = 100.0f, kMinRSq = 0.25f, kHydroSq = 36.0f, kCoulomb = 332.0f;
   float dx = bx - ag.x[i];
   float dy = by - ag.y[i];
   float dz = bz - ag.z[i];
   float r_sq = dx * dx + dy * dy + dz * dz;
-  if (r_sq > kCutoffSq || r_sq < kMinRSq) return;
+  if (r_sq > kCutoffSq || r_sq < kMinRSq) return false;
   float eps_ij = sqrtf(be * ag.eps[i]);
   float sig_ij = 0.5f * (bs + ag.sig[i]);
   float sigma_sq_ij = sig_ij * sig_ij;
@@ -54,9 +57,20 @@ __device__ inline void cg_pair(const CgCloud& ag, uint32_t i, float bx, float by
   float f_hydro = 0.0f;
   if (bh == 1 && ag.hyd[i] == 1 && r_sq < kHydroSq) f_hydro = -0.1f / sqrtf(r_sq);
   float f_total = f_lj + f_coul + f_hydro;
-  ax += f_total * dx;
-  ay += f_total * dy;
-  az += f_total * dz;
+  tx = f_total * dx;
+  ty = f_total * dy;
+  tz = f_total * dz;
+  return true;
+}
+
+__device__ inline void cg_pair(const CgCloud& ag, uint32_t i, float bx, float by, float bz,
+                               float bq, float be, float bs, uint8_t bh, float& ax, float& ay,
+                               float& az) {
+  float tx, ty, tz;
+  if (!cg_term(ag, i, bx, by, bz, bq, be, bs, bh, tx, ty, tz)) return;
+  ax += tx;
+  ay += ty;
+  az += tz;
 }
 
 // Device-side residue-level force for one Calpha (used by the gradient step).

@@ -38,6 +38,7 @@ namespace {
 constexpr int kSlots = 14;        // atom slots per residue (the largest topology, Trp)
 constexpr int kMaxDefs = 12;      // side-chain placements per residue type
 constexpr uint32_t kMaxRefineRes = 512;
+constexpr uint32_t kPairCap = 1024;       // residue-antigen pairs evaluated per batch
 constexpr uint32_t kTagMc = 0x6d636d76u;  // draws of the side-chain moves
 constexpr uint32_t kTagBb = 0x62626d76u;  // draws of the torsion moves
 
@@ -251,6 +252,96 @@ __device__ inline void torsion(const RefTables& T, const uint8_t* aa, float* X, 
   __syncthreads();
 }
 
+// compute_forces_with_grid (src/energy.rs:322-398) for every residue's
+// Calpha: the 27-cell neighbour lists of a batch of residues are flattened in
+// the reference's order, each pair's force term is evaluated lane-parallel,
+// then each residue adds its terms in list order, so every residue gets the
+// reference's f32 sum. A residue whose list alone exceeds the buffer is
+// evaluated by one lane in place.
+__device__ inline void cg_forces(const RefTables& T, const GridView& grid, const CgCloud& cg,
+                                 const uint8_t* aa, const float* X, const float* Y,
+                                 const float* Z, uint32_t nr, uint32_t lane, float* F,
+                                 uint32_t* PC, uint32_t* PS, uint32_t* PI, uint32_t* PO,
+                                 float* PX, float* PY, float* PZ) {
+  auto for_cells = [&](uint32_t r, auto&& fn) {
+    if (grid.n == 0) return;
+    const uint32_t t = r * kSlots + 1;
+    const int32_t cx = cell_key(X[t], grid.inv), cy = cell_key(Y[t], grid.inv),
+                  cz = cell_key(Z[t], grid.inv);
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          uint32_t b, e;
+          if (cell_range(grid, wrap_add(cx, dx), wrap_add(cy, dy), wrap_add(cz, dz), b, e))
+            fn(b, e);
+        }
+  };
+  for (uint32_t r = lane; r < nr; r += kWave) {
+    uint32_t n = 0;
+    for_cells(r, [&](uint32_t b, uint32_t e) { n += e - b; });
+    PC[r] = n;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    uint32_t acc = 0;
+    for (uint32_t r = 0; r < nr; ++r) {
+      PS[r] = acc;
+      acc += PC[r];
+    }
+  }
+  __syncthreads();
+  for (uint32_t rb = 0; rb < nr;) {
+    uint32_t re = rb + 1, tot = PC[rb];
+    while (re < nr && tot + PC[re] <= kPairCap) tot += PC[re++];
+    const uint32_t base = PS[rb];
+    if (tot > kPairCap) {  // one residue over the buffer: its own lane, in place
+      if (lane == 0) {
+        const uint32_t t = rb * kSlots + 1, k = aa[rb];
+        cg_force_point(grid, cg, X[t], Y[t], Z[t], T.cg_q[k], T.cg_eps[k], T.cg_sig[k],
+                       T.cg_hyd[k], F[3 * rb], F[3 * rb + 1], F[3 * rb + 2]);
+      }
+    } else {
+      for (uint32_t r = rb + lane; r < re; r += kWave) {
+        uint32_t k = PS[r] - base;
+        for_cells(r, [&](uint32_t b, uint32_t e) {
+          for (uint32_t q = b; q < e; ++q, ++k) {
+            PI[k] = grid.sorted_idx[q];
+            PO[k] = r;
+          }
+        });
+      }
+      __syncthreads();
+      for (uint32_t k = lane; k < tot; k += kWave) {
+        const uint32_t r = PO[k], t = r * kSlots + 1, ty = aa[r];
+        float tx, tyy, tz;
+        if (cg_term(cg, PI[k], X[t], Y[t], Z[t], T.cg_q[ty], T.cg_eps[ty], T.cg_sig[ty],
+                    T.cg_hyd[ty], tx, tyy, tz)) {
+          PX[k] = tx;
+          PY[k] = tyy;
+          PZ[k] = tz;
+          PI[k] |= 0x80000000u;
+        }
+      }
+      __syncthreads();
+      for (uint32_t r = rb + lane; r < re; r += kWave) {
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        const uint32_t k0 = PS[r] - base, k1 = k0 + PC[r];
+        for (uint32_t k = k0; k < k1; ++k)
+          if (PI[k] & 0x80000000u) {
+            ax += PX[k];
+            ay += PY[k];
+            az += PZ[k];
+          }
+        F[3 * r] = ax;
+        F[3 * r + 1] = ay;
+        F[3 * r + 2] = az;
+      }
+    }
+    __syncthreads();
+    rb = re;
+  }
+}
+
 // kLds: the antigen Calpha set and grid live in LDS for the whole launch
 // (their addresses then resolve to LDS at compile time; a pointer that may
 // be either would turn every access into a flat one).
@@ -289,7 +380,14 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   float* Y = X + ms;
   float* Z = Y + ms;
   float* F = Z + ms;                 // 3 * max_res forces
-  uint8_t* gh = reinterpret_cast<uint8_t*>(F + 3 * a.max_res);
+  uint32_t* PC = reinterpret_cast<uint32_t*>(F + 3 * a.max_res);  // pairs per residue
+  uint32_t* PS = PC + a.max_res;                                   // their list offsets
+  uint32_t* PI = PS + a.max_res;     // kPairCap: antigen index | in-predicate bit
+  uint32_t* PO = PI + kPairCap;      // kPairCap: owning residue
+  float* PX = reinterpret_cast<float*>(PO + kPairCap);
+  float* PY = PX + kPairCap;
+  float* PZ = PY + kPairCap;
+  uint8_t* gh = reinterpret_cast<uint8_t*>(PZ + kPairCap);
   uint8_t* aa = gh + na_l;
   uint8_t* rot = aa + a.max_res;
   uint8_t* iface = rot + a.max_res;
@@ -329,18 +427,9 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   for (uint32_t s = 0; s < a.n_steps; ++s) {
     const uint32_t it = a.iter0 + s;
     const float temp = a.temp[s];
-    // 1. the gradient step (src/diffusion.rs:305-339)
-    for (uint32_t r = lane; r < nr; r += kWave) {
-      const uint32_t t = r * kSlots + 1;
-      const uint32_t k = aa[r];
-      float fx, fy, fz;
-      cg_force_point(grid, cg, X[t], Y[t], Z[t], T.cg_q[k], T.cg_eps[k], T.cg_sig[k],
-                     T.cg_hyd[k], fx, fy, fz);
-      F[3 * r] = fx;
-      F[3 * r + 1] = fy;
-      F[3 * r + 2] = fz;
-    }
-    __syncthreads();
+    // 1. the gradient step (src/diffusion.rs:305-339): residue-level forces
+    //    on the Calphas (compute_forces_with_grid)
+    cg_forces(T, grid, cg, aa, X, Y, Z, nr, lane, F, PC, PS, PI, PO, PX, PY, PZ);
     for (uint32_t r = lane; r < nr; r += kWave) {
       const uint32_t t = r * kSlots + 1;
       const float4 d = residue_delta(a.cx, a.cy, a.cz, a.sp, a.r0,
@@ -353,15 +442,18 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
       }
     }
     __syncthreads();
-    // 2. interface mask (src/diffusion.rs:341-351)
-    for (uint32_t r = lane; r < nr; r += kWave) {
+    // 2. interface mask (src/diffusion.rs:341-351), lane-split over the
+    //    antigen Calphas residue by residue
+    for (uint32_t r = 0; r < nr; ++r) {
       const uint32_t t = r * kSlots + 1;
+      const float px = X[t], py = Y[t], pz = Z[t];
       bool in = false;
-      for (uint32_t i = 0; i < g.n && !in; ++i) {
-        const float dx = X[t] - g.x[i], dy = Y[t] - g.y[i], dz = Z[t] - g.z[i];
-        in = dx * dx + dy * dy + dz * dz < a.iface_sq;
+      for (uint32_t i = lane; i < g.n; i += kWave) {
+        const float dx = px - g.x[i], dy = py - g.y[i], dz = pz - g.z[i];
+        in = in || dx * dx + dy * dy + dz * dz < a.iface_sq;
       }
-      iface[r] = in ? 1 : 0;
+      const bool any = ballot64(in) != 0;
+      if (lane == 0) iface[r] = any ? 1 : 0;
     }
     __syncthreads();
     // this step's draws of both move kinds, one residue per lane
@@ -647,8 +739,8 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   a.max_torsion = mc->max_torsion_deg;
   a.iface_sq = mc->iface_cutoff_sq;
   a.moves = mc->moves;
-  size_t lds = (sizeof(RefTables) + 15) / 16 * 16 +
-               static_cast<size_t>(max_res) * ((3 * kSlots + 3) * sizeof(float) + 2 * 16 + 3);
+  size_t lds = (sizeof(RefTables) + 15) / 16 * 16 + kPairCap * 5 * sizeof(float) +
+               static_cast<size_t>(max_res) * ((3 * kSlots + 3 + 2) * sizeof(float) + 2 * 16 + 3);
   const size_t ncells = ag->ca_grid.n ? ag->ca_grid.ncodes + 1 : 0;
   const size_t ag_bytes = n_ag * (16 + 6 * 4 + 4 + 1) + ncells * 4;
   if (ncells && lds + ag_bytes <= size_t(120) << 10) {
