@@ -98,6 +98,11 @@ class PopulationDescC(C.Structure):
 DNAB_MC_SIDECHAIN, DNAB_MC_BACKBONE = 1, 2
 
 
+class PdbChainC(C.Structure):
+    _fields_ = [("x", f32p), ("y", f32p), ("z", f32p), ("res_start", u32p), ("amino_acid", u8p),
+                ("n_res", C.c_size_t), ("chain_id", C.c_char)]
+
+
 class McParamsC(C.Structure):
     _fields_ = [("mutation_p", C.c_float), ("rotamer_move_p", C.c_float),
                 ("backbone_move_p", C.c_float), ("max_torsion_deg", C.c_float),
@@ -231,6 +236,12 @@ _PROTOS = {
     "dnab_allatom_energy": (C.c_int, [vp, C.POINTER(AtomCloudC), C.POINTER(AtomCloudC), u32p,
                                       u32p, u8p, C.c_size_t, C.c_float, f64p, f64p]),
     "dnab_mc_default_params": (C.c_int, [C.POINTER(McParamsC)]),
+    "dnab_parse_pdb": (C.c_int, [C.c_char_p, C.c_size_t, f32p, f32p, f32p, u8p, C.c_size_t,
+                                 C.POINTER(C.c_size_t)]),
+    "dnab_write_pdb_ca": (C.c_int, [f32p, f32p, f32p, u8p, C.c_size_t, C.c_char, C.c_char_p,
+                                    C.c_size_t, C.POINTER(C.c_size_t)]),
+    "dnab_write_pdb_chains": (C.c_int, [C.POINTER(PdbChainC), C.c_size_t, C.c_int, C.c_char_p,
+                                        C.c_size_t, C.POINTER(C.c_size_t)]),
     "dnab_refine": (C.c_int, [vp, vp, C.POINTER(PopulationDescC), u8p, C.POINTER(LangevinParamsC),
                               C.POINTER(McParamsC), C.POINTER(ProteinOutC), u8p, f64p,
                               C.POINTER(C.c_size_t)]),

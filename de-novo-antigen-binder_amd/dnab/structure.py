@@ -271,3 +271,55 @@ def population_from_proteins(proteins: list[AtomProtein], antigen_ids=None) -> P
         np.ascontiguousarray(np.concatenate(res_ca) if res_ca else np.zeros(0), np.uint32),
         np.ascontiguousarray(np.concatenate(res_aa) if res_aa else np.zeros(0), np.uint8),
         AtomCloud.concat(clouds))
+
+
+# ---- PDB text I/O (src/pdb.rs) ----
+
+def parse_pdb(text: str | bytes) -> ResidueCloud:
+    """parse_pdb (src/pdb.rs:19-57) on a PDB file's text: the CA atoms."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    n = C.c_size_t(0)
+    st = N.lib.dnab_parse_pdb(data, len(data), None, None, None, None, 0, C.byref(n))
+    if st not in (N.DNAB_OK, N.DNAB_ERR_CAPACITY):
+        raise ValueError("no CA atom in the PDB text")
+    m = n.value
+    x, y, z = (np.zeros(m, np.float32) for _ in range(3))
+    aa = np.zeros(m, np.uint8)
+    N.check(N.lib.dnab_parse_pdb(data, len(data), N.ptr(x, C.c_float), N.ptr(y, C.c_float),
+                                 N.ptr(z, C.c_float), N.ptr(aa, C.c_uint8), m, C.byref(n)))
+    return ResidueCloud.from_points(x, y, z, aa)
+
+
+def _text(fn):
+    n = C.c_size_t(0)
+    N.check(fn(None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    N.check(fn(buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()
+
+
+def write_pdb(cloud: ResidueCloud, chain_id: str = "B") -> str:
+    """write_pdb (src/pdb.rs:94-119): CA-only ATOM records."""
+    return _text(lambda o, cap, n: N.lib.dnab_write_pdb_ca(
+        N.ptr(cloud.x, C.c_float), N.ptr(cloud.y, C.c_float), N.ptr(cloud.z, C.c_float),
+        N.ptr(cloud.amino_acid, C.c_uint8), len(cloud), chain_id.encode(), o, cap, n))
+
+
+def write_pdb_multi(chains: list[tuple[AtomProtein, str]], ter: bool = True) -> str:
+    """write_pdb_multi (src/pdb.rs:157-191); ter=False with one chain is
+    write_pdb_allatom (src/pdb.rs:121-155)."""
+    arr = (N.PdbChainC * max(len(chains), 1))()
+    keep = []
+    for k, (p, cid) in enumerate(chains):
+        rs = np.ascontiguousarray(p.res_start, np.uint32)
+        aa = np.ascontiguousarray(p.amino_acid, np.uint8)
+        keep.append((rs, aa))
+        arr[k] = N.PdbChainC(N.ptr(p.atoms.x, C.c_float), N.ptr(p.atoms.y, C.c_float),
+                             N.ptr(p.atoms.z, C.c_float), N.ptr(rs, C.c_uint32),
+                             N.ptr(aa, C.c_uint8), len(aa), cid.encode())
+    return _text(lambda o, cap, n: N.lib.dnab_write_pdb_chains(arr, len(chains), 1 if ter else 0,
+                                                              o, cap, n))
+
+
+def write_pdb_allatom(p: AtomProtein, chain_id: str = "B") -> str:
+    return write_pdb_multi([(p, chain_id)], ter=False)

@@ -15,6 +15,8 @@
 //                                          src/diffusion.rs:281-283,
 //                                          src/allatom.rs:155-180
 //   binder::affinity_maturation_allatom    src/diffusion.rs:924-961
+//   binder::parse_pdb, write_pdb_allatom, write_pdb_multi
+//                                          src/pdb.rs:19-57, 121-191
 //   binder::run_allatom_gpu_phase          src/diffusion.rs:1008-1062 (GPU phase,
 //                                          gradient steps + scoring, no MC moves)
 //
@@ -26,11 +28,13 @@
 #include <algorithm>
 #include <array>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/dnab.h"
@@ -320,6 +324,50 @@ inline double affinity_maturation_allatom(const GpuContext& gpu, AtomProtein& ab
   for (size_t i = 0; i < nr; ++i) out.amino_acid[i] = static_cast<AminoAcid>(aa_out[i]);
   ab = std::move(out);
   return e;
+}
+
+// parse_pdb (src/pdb.rs:19-57): the CA atoms of a PDB file as a residue cloud.
+inline ResidueCloud parse_pdb(const std::string& path) {
+  std::FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw Error(DNAB_ERR_INVALID, "cannot open " + path);
+  std::string text;
+  char buf[1 << 16];
+  size_t got;
+  while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+  std::fclose(f);
+  size_t n = 0;
+  const int st = dnab_parse_pdb(text.data(), text.size(), nullptr, nullptr, nullptr, nullptr, 0, &n);
+  if (st != DNAB_OK && st != DNAB_ERR_CAPACITY) check(st);
+  std::vector<float> x(n), y(n), z(n);
+  std::vector<uint8_t> aa(n);
+  check(dnab_parse_pdb(text.data(), text.size(), x.data(), y.data(), z.data(), aa.data(), n, &n));
+  ResidueCloud c;
+  for (size_t i = 0; i < n; ++i) c.push(x[i], y[i], z[i], static_cast<AminoAcid>(aa[i]));
+  return c;
+}
+
+// write_pdb_multi (src/pdb.rs:157-191); with one chain and ter = false,
+// write_pdb_allatom (src/pdb.rs:121-155).
+inline std::string write_pdb_multi(const std::vector<std::pair<const AtomProtein*, char>>& chains,
+                                   bool ter = true) {
+  std::vector<dnab_pdb_chain> cs;
+  std::vector<std::vector<uint8_t>> aas;
+  aas.reserve(chains.size());
+  for (const auto& [p, id] : chains) {
+    aas.emplace_back(p->amino_acid.size());
+    for (size_t i = 0; i < p->amino_acid.size(); ++i) aas.back()[i] = static_cast<uint8_t>(p->amino_acid[i]);
+    cs.push_back({p->atoms.x.data(), p->atoms.y.data(), p->atoms.z.data(), p->res_start.data(),
+                  aas.back().data(), p->n_residues(), id});
+  }
+  size_t n = 0;
+  check(dnab_write_pdb_chains(cs.data(), cs.size(), ter ? 1 : 0, nullptr, 0, &n));
+  std::string out(n + 1, '\0');
+  check(dnab_write_pdb_chains(cs.data(), cs.size(), ter ? 1 : 0, out.data(), out.size(), &n));
+  out.resize(n);
+  return out;
+}
+inline std::string write_pdb_allatom(const AtomProtein& p, char chain_id) {
+  return write_pdb_multi({{&p, chain_id}}, false);
 }
 
 // compute_forces_with_grid (src/energy.rs:322-398): outputs sized >= n_ab.

@@ -129,6 +129,35 @@ int dnab_build_protein(const float* ca_x, const float* ca_y, const float* ca_z,
  * from its backbone atoms (N, CA, C, O and, except Gly, CB) with the chi of
  * rotamer `rotamer` (0 for types without rotamers) and written after them. */
 int dnab_rebuild_side_chain(uint8_t amino_acid, uint8_t rotamer, float* xyz, size_t n_atoms);
+/* ---- PDB text I/O (host only) ---------------------------------------------
+ * parse_pdb (src/pdb.rs:19-57) on a PDB file's text: one residue per CA atom
+ * of an ATOM/HETATM record, in file order; x, y, z from columns 31-54 (0
+ * where a field is missing or not a number), the residue from columns 18-20
+ * (from_three_letter: HID/HIE/HIP are His, unknown names Gly). Writes up to
+ * `capacity` residues; n_out = the count. No CA atom: DNAB_ERR_INVALID (the
+ * reference's EmptyInput); more than capacity: DNAB_ERR_CAPACITY. */
+int dnab_parse_pdb(const char* text, size_t len, float* x, float* y, float* z,
+                   uint8_t* amino_acid, size_t capacity, size_t* n_out);
+/* write_pdb (src/pdb.rs:94-119): CA-only ATOM records of a residue cloud and
+ * END. The writers fill `out` (capacity bytes, NUL-terminated) and set
+ * n_chars to the text length; out = NULL only queries the length. */
+int dnab_write_pdb_ca(const float* x, const float* y, const float* z, const uint8_t* amino_acid,
+                      size_t n, char chain_id, char* out, size_t capacity, size_t* n_chars);
+/* One all-atom chain: atoms in residue order (res_start, n_res + 1), each
+ * residue's atoms in topology order. */
+typedef struct dnab_pdb_chain {
+  const float *x, *y, *z;
+  const uint32_t* res_start;
+  const uint8_t* amino_acid;
+  size_t n_res;
+  char chain_id;
+} dnab_pdb_chain;
+/* write_pdb_allatom (src/pdb.rs:121-155, one chain, ter = 0) and
+ * write_pdb_multi (src/pdb.rs:157-191, ter = 1: a TER record after each
+ * chain): ATOM records with the topology's atom names, serials continuing
+ * across chains, then END. */
+int dnab_write_pdb_chains(const dnab_pdb_chain* chains, size_t n_chains, int ter, char* out,
+                          size_t capacity, size_t* n_chars);
 /* place_atom (src/rotamer.rs:198-232), NERF. */
 int dnab_place_atom(const float a[3], const float b[3], const float c[3], float bond_len,
                     float angle_deg, float dihedral_deg, float out[3]);

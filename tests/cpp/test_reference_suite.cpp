@@ -175,6 +175,28 @@ int main() {
     CHECK(ab.n_residues() == 6 && ab.res_start.back() == ab.n_atoms());
   });
 
+  run("pdb::write_pdb_allatom / parse_pdb round trip (src/pdb.rs:19-57, 121-155)", [&] {
+    auto p = protein_from_ca_trace({0.0f, 3.8f, 7.6f}, {1.0f, 1.0f, 1.0f}, {0.0f, 0.5f, 1.0f},
+                                   {AminoAcid::Trp, AminoAcid::Gly, AminoAcid::His});
+    const std::string text = write_pdb_allatom(p, 'B');
+    CHECK(text.substr(0, 6) == "ATOM  " && text.size() >= 4 &&
+          text.compare(text.size() - 4, 4, "END\n") == 0);
+    const std::string path = "/tmp/dnab_suite_roundtrip.pdb";
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    CHECK(f != nullptr);
+    if (!f) return;
+    std::fwrite(text.data(), 1, text.size(), f);
+    std::fclose(f);
+    auto c = parse_pdb(path);
+    std::remove(path.c_str());
+    CHECK(c.len() == 3);
+    for (size_t r = 0; r < c.len() && r < 3; ++r) {
+      const auto ca = p.ca_atom_idx[r];
+      CHECK(std::fabs(c.x[r] - p.atoms.x[ca]) <= 5e-4f && std::fabs(c.z[r] - p.atoms.z[ca]) <= 5e-4f);
+      CHECK(c.amino_acid[r] == p.amino_acid[r]);
+    }
+  });
+
   run("gpu::score_batch uniform candidates (src/gpu.rs:166) vs the breakdown's shader terms", [&] {
     auto ag = protein_from_ca_trace({0.0f, 3.8f, 7.6f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f},
                                     {AminoAcid::Asp, AminoAcid::Lys, AminoAcid::Leu});
