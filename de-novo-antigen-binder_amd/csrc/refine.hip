@@ -38,7 +38,6 @@ namespace {
 constexpr int kSlots = 14;        // atom slots per residue (the largest topology, Trp)
 constexpr int kMaxDefs = 12;      // side-chain placements per residue type
 constexpr uint32_t kMaxRefineRes = 512;
-constexpr uint32_t kPairCap = 1024;       // residue-antigen pairs evaluated per batch
 constexpr uint32_t kTagMc = 0x6d636d76u;  // draws of the side-chain moves
 constexpr uint32_t kTagBb = 0x62626d76u;  // draws of the torsion moves
 
@@ -78,6 +77,7 @@ struct RefineArgs {
   uint8_t* res_rot;
   float4* slots;
   uint32_t n_cand, max_res;
+  uint32_t pair_cap;  // force-pass list entries (a multiple of 64)
   // schedule, per step
   const float* temp;
   const float* sigma;
@@ -252,93 +252,167 @@ __device__ inline void torsion(const RefTables& T, const uint8_t* aa, float* X, 
   __syncthreads();
 }
 
+// Scratch of the residue-level force pass, in LDS: a flattened pair list of
+// `cap` entries and the per-residue rows of one chunk of 64 residues.
+struct CgScratch {
+  float4* CT;    // cap: in-predicate force terms, compacted, in list order
+  float4* RQ;    // 64: Calpha x, y, z and the residue-level charge
+  float4* RP;    // 64: epsilon, sigma, hydrophobic flag
+  uint32_t* H;   // cap: at each cell's first pair, its packed entry (below)
+  uint32_t* HM;  // cap / 32: which list positions start a cell
+  uint32_t* CS;  // 64: each residue's first compacted term
+  uint32_t cap;  // a multiple of 64, with n + cap < 2^25 for the grid's n points
+};
+
+// A cell's entry in the list: bits 0-24 hold (first sorted index) - (first
+// list position) + cap, bits 25-30 the residue's lane, bit 31 marks the
+// residue's first non-empty cell.
+constexpr uint32_t kEntryOffMask = 0x1FFFFFFu;
+constexpr uint32_t kEntryFirst = 0x80000000u;
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+  for (uint32_t d = 1; d < kWave; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
 // compute_forces_with_grid (src/energy.rs:322-398) for every residue's
-// Calpha: the 27-cell neighbour lists of a batch of residues are flattened in
-// the reference's order, each pair's force term is evaluated lane-parallel,
-// then each residue adds its terms in list order, so every residue gets the
-// reference's f32 sum. A residue whose list alone exceeds the buffer is
-// evaluated by one lane in place.
+// Calpha. The neighbour pairs of a chunk of 64 residues (one per lane) form a
+// flat list in the reference's order: residues in order, the 27 cells in the
+// reference's dx, dy, dz order, each cell's points in sorted order. Each lane
+// evaluates list positions lane, lane + 64, ...: the cell a position belongs
+// to is the last cell start at or before it (a bit scan over the head bits),
+// so no lane walks a residue's list. In-predicate terms are compacted in list
+// order; then each residue adds its terms one by one, so every force is the
+// reference's f32 sum in the reference's order. A chunk whose list exceeds
+// `cap` is split into batches of whole residues; a residue whose list alone
+// exceeds it is summed by one lane in place.
 __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const CgCloud& cg,
                                  const uint8_t* aa, const float* X, const float* Y,
                                  const float* Z, uint32_t nr, uint32_t lane, float* F,
-                                 uint32_t* PC, uint32_t* PS, uint32_t* PI, uint32_t* PO,
-                                 float* PX, float* PY, float* PZ) {
-  auto for_cells = [&](uint32_t r, auto&& fn) {
-    if (grid.n == 0) return;
-    const uint32_t t = r * kSlots + 1;
-    const int32_t cx = cell_key(X[t], grid.inv), cy = cell_key(Y[t], grid.inv),
-                  cz = cell_key(Z[t], grid.inv);
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dz = -1; dz <= 1; ++dz) {
-          uint32_t b, e;
-          if (cell_range(grid, wrap_add(cx, dx), wrap_add(cy, dy), wrap_add(cz, dz), b, e))
-            fn(b, e);
-        }
-  };
-  for (uint32_t r = lane; r < nr; r += kWave) {
+                                 const CgScratch& s) {
+  for (uint32_t cb = 0; cb < nr; cb += kWave) {
+    const uint32_t nrc = min(nr - cb, static_cast<uint32_t>(kWave));
+    const bool act = lane < nrc;
+    // this lane's residue: its 27 cell ranges, kept in registers
+    uint32_t cbeg[27], ccnt[27];
     uint32_t n = 0;
-    for_cells(r, [&](uint32_t b, uint32_t e) { n += e - b; });
-    PC[r] = n;
-  }
-  __syncthreads();
-  if (lane == 0) {
-    uint32_t acc = 0;
-    for (uint32_t r = 0; r < nr; ++r) {
-      PS[r] = acc;
-      acc += PC[r];
+#pragma unroll
+    for (int c = 0; c < 27; ++c) ccnt[c] = cbeg[c] = 0;
+    if (act) {
+      const uint32_t r = cb + lane, t = r * kSlots + 1, ty = aa[r];
+      const float px = X[t], py = Y[t], pz = Z[t];
+      s.RQ[lane] = make_float4(px, py, pz, T.cg_q[ty]);
+      s.RP[lane] = make_float4(T.cg_eps[ty], T.cg_sig[ty], static_cast<float>(T.cg_hyd[ty]), 0.0f);
+      if (grid.n) {
+        const int32_t kx = cell_key(px, grid.inv), ky = cell_key(py, grid.inv),
+                      kz = cell_key(pz, grid.inv);
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+          for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) {
+              const int c = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+              uint32_t b = 0, e = 0;
+              if (cell_range(grid, wrap_add(kx, dx), wrap_add(ky, dy), wrap_add(kz, dz), b, e)) {
+                cbeg[c] = b;
+                ccnt[c] = e - b;
+              }
+              n += ccnt[c];
+            }
+      }
     }
-  }
-  __syncthreads();
-  for (uint32_t rb = 0; rb < nr;) {
-    uint32_t re = rb + 1, tot = PC[rb];
-    while (re < nr && tot + PC[re] <= kPairCap) tot += PC[re++];
-    const uint32_t base = PS[rb];
-    if (tot > kPairCap) {  // one residue over the buffer: its own lane, in place
-      if (lane == 0) {
-        const uint32_t t = rb * kSlots + 1, k = aa[rb];
-        cg_force_point(grid, cg, X[t], Y[t], Z[t], T.cg_q[k], T.cg_eps[k], T.cg_sig[k],
-                       T.cg_hyd[k], F[3 * rb], F[3 * rb + 1], F[3 * rb + 2]);
-      }
-    } else {
-      for (uint32_t r = rb + lane; r < re; r += kWave) {
-        uint32_t k = PS[r] - base;
-        for_cells(r, [&](uint32_t b, uint32_t e) {
-          for (uint32_t q = b; q < e; ++q, ++k) {
-            PI[k] = grid.sorted_idx[q];
-            PO[k] = r;
-          }
-        });
-      }
-      __syncthreads();
-      for (uint32_t k = lane; k < tot; k += kWave) {
-        const uint32_t r = PO[k], t = r * kSlots + 1, ty = aa[r];
-        float tx, tyy, tz;
-        if (cg_term(cg, PI[k], X[t], Y[t], Z[t], T.cg_q[ty], T.cg_eps[ty], T.cg_sig[ty],
-                    T.cg_hyd[ty], tx, tyy, tz)) {
-          PX[k] = tx;
-          PY[k] = tyy;
-          PZ[k] = tz;
-          PI[k] |= 0x80000000u;
+    const uint32_t incl = wave_incl_scan(n, lane), excl = incl - n;
+    __syncthreads();  // RQ, RP
+    for (uint32_t ls = 0; ls < nrc;) {
+      // the batch [ls, le): the residues from ls whose lists fit together
+      const uint32_t base = __shfl(excl, static_cast<int>(ls));
+      const uint32_t le = static_cast<uint32_t>(
+          __popcll(ballot64(lane < ls || (act && incl - base <= s.cap))));
+      if (le == ls) {  // one residue over the buffer: its own lane, in place
+        if (lane == 0) {
+          const uint32_t r = cb + ls, t = r * kSlots + 1, k = aa[r];
+          cg_force_point(grid, cg, X[t], Y[t], Z[t], T.cg_q[k], T.cg_eps[k], T.cg_sig[k],
+                         T.cg_hyd[k], F[3 * r], F[3 * r + 1], F[3 * r + 2]);
         }
+        __syncthreads();
+        ++ls;
+        continue;
+      }
+      const uint32_t tot = __shfl(incl, static_cast<int>(le - 1)) - base;
+      for (uint32_t w = lane; w < (tot + 31) / 32; w += kWave) s.HM[w] = 0u;
+      __syncthreads();
+      const bool mine = lane >= ls && lane < le;
+      if (mine && n) {
+        uint32_t k0 = excl - base;
+        uint32_t first = kEntryFirst;
+#pragma unroll
+        for (int c = 0; c < 27; ++c)
+          if (ccnt[c]) {
+            s.H[k0] = (cbeg[c] + s.cap - k0) | (lane << 25) | first;
+            atomicOr(&s.HM[k0 >> 5], 1u << (k0 & 31u));
+            first = 0u;
+            k0 += ccnt[c];
+          }
       }
       __syncthreads();
-      for (uint32_t r = rb + lane; r < re; r += kWave) {
-        float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        const uint32_t k0 = PS[r] - base, k1 = k0 + PC[r];
-        for (uint32_t k = k0; k < k1; ++k)
-          if (PI[k] & 0x80000000u) {
-            ax += PX[k];
-            ay += PY[k];
-            az += PZ[k];
+      uint32_t cnt = 0, carry = 0;
+      for (uint32_t kb = 0; kb < tot; kb += kWave) {
+        const uint32_t k = kb + lane;
+        const bool live = k < tot;
+        const uint64_t heads =
+            (static_cast<uint64_t>(s.HM[kb >> 5]) | (static_cast<uint64_t>(s.HM[(kb >> 5) + 1]) << 32)) &
+            ballot64(live);
+        const uint64_t upto = heads & ((2ull << lane) - 1ull);  // heads at lanes <= this one
+        const uint32_t hk = s.H[live ? k : kb];
+        const uint32_t hv0 = __shfl(hk, upto ? 63 - __builtin_clzll(upto) : 0);
+        const uint32_t hv = upto ? hv0 : carry;
+        carry = __shfl(hv, 63);
+        float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+        bool in = false;
+        const uint32_t l = (hv >> 25) & 63u;
+        if (live) {
+          const uint32_t q = (hv & kEntryOffMask) + k - s.cap;
+          if (q < grid.n) {
+            const float4 rq = s.RQ[l], rp = s.RP[l];
+            in = cg_term(cg, grid.sorted_idx[q], rq.x, rq.y, rq.z, rq.w, rp.x, rp.y,
+                         static_cast<uint8_t>(rp.z), tx, ty, tz);
           }
+        }
+        const uint64_t inb = ballot64(in);
+        const uint32_t pos = cnt + static_cast<uint32_t>(__popcll(inb & ((1ull << lane) - 1ull)));
+        if (in) s.CT[pos] = make_float4(tx, ty, tz, 0.0f);
+        if (live && ((heads >> lane) & 1ull) && (hk & kEntryFirst)) s.CS[l] = pos;
+        cnt += static_cast<uint32_t>(__popcll(inb));
+      }
+      __syncthreads();
+      // each residue's terms [CS[l], next residue's CS) added in list order
+      const bool has = mine && n;
+      const uint32_t st = has ? s.CS[lane] : 0u;
+      const uint64_t later = ballot64(has) & ~((2ull << lane) - 1ull);
+      const uint32_t stn = __shfl(st, later ? __builtin_ctzll(later) : 0);
+      const uint32_t en = has ? (later ? stn : cnt) : st;
+      if (mine) {
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll 4
+        for (uint32_t k = st; k < en; ++k) {
+          const float4 v = s.CT[k];
+          ax += v.x;
+          ay += v.y;
+          az += v.z;
+        }
+        const uint32_t r = cb + lane;
         F[3 * r] = ax;
         F[3 * r + 1] = ay;
         F[3 * r + 2] = az;
       }
+      __syncthreads();
+      ls = le;
     }
-    __syncthreads();
-    rb = re;
   }
 }
 
@@ -368,7 +442,12 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   const uint32_t na_l = kLds ? a.n_ag : 0, nc_l = kLds ? a.ncells_lds : 0;
   float4* gpar = reinterpret_cast<float4*>(lds);  // 16-byte entries first
   float4* U = gpar + na_l;                         // 2 * max_res draws
-  float* gx = reinterpret_cast<float*>(U + 2 * a.max_res);
+  CgScratch cs;
+  cs.cap = a.pair_cap;
+  cs.CT = U + 2 * a.max_res;
+  cs.RQ = cs.CT + a.pair_cap;
+  cs.RP = cs.RQ + kWave;
+  float* gx = reinterpret_cast<float*>(cs.RP + kWave);
   float* gy = gx + na_l;
   float* gz = gy + na_l;
   float* gq = gz + na_l;
@@ -380,14 +459,10 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   float* Y = X + ms;
   float* Z = Y + ms;
   float* F = Z + ms;                 // 3 * max_res forces
-  uint32_t* PC = reinterpret_cast<uint32_t*>(F + 3 * a.max_res);  // pairs per residue
-  uint32_t* PS = PC + a.max_res;                                   // their list offsets
-  uint32_t* PI = PS + a.max_res;     // kPairCap: antigen index | in-predicate bit
-  uint32_t* PO = PI + kPairCap;      // kPairCap: owning residue
-  float* PX = reinterpret_cast<float*>(PO + kPairCap);
-  float* PY = PX + kPairCap;
-  float* PZ = PY + kPairCap;
-  uint8_t* gh = reinterpret_cast<uint8_t*>(PZ + kPairCap);
+  cs.H = reinterpret_cast<uint32_t*>(F + 3 * a.max_res);
+  cs.HM = cs.H + a.pair_cap;
+  cs.CS = cs.HM + a.pair_cap / 32;
+  uint8_t* gh = reinterpret_cast<uint8_t*>(cs.CS + kWave);
   uint8_t* aa = gh + na_l;
   uint8_t* rot = aa + a.max_res;
   uint8_t* iface = rot + a.max_res;
@@ -429,7 +504,7 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
     const float temp = a.temp[s];
     // 1. the gradient step (src/diffusion.rs:305-339): residue-level forces
     //    on the Calphas (compute_forces_with_grid)
-    cg_forces(T, grid, cg, aa, X, Y, Z, nr, lane, F, PC, PS, PI, PO, PX, PY, PZ);
+    cg_forces(T, grid, cg, aa, X, Y, Z, nr, lane, F, cs);
     for (uint32_t r = lane; r < nr; r += kWave) {
       const uint32_t t = r * kSlots + 1;
       const float4 d = residue_delta(a.cx, a.cy, a.cz, a.sp, a.r0,
@@ -739,11 +814,38 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   a.max_torsion = mc->max_torsion_deg;
   a.iface_sq = mc->iface_cutoff_sq;
   a.moves = mc->moves;
-  size_t lds = (sizeof(RefTables) + 15) / 16 * 16 + kPairCap * 5 * sizeof(float) +
-               static_cast<size_t>(max_res) * ((3 * kSlots + 3 + 2) * sizeof(float) + 2 * 16 + 3);
+  // LDS: the tables, the candidate's rows, the force pass's pair list, and
+  // the antigen Calpha set with its grid when that leaves room for a list of
+  // 1024 pairs. Blocks resident per CU: as many as the batch needs to fill
+  // the device once (up to 4), fewer when the list would drop below 1024.
+  int lds_max = 0;
+  DNAB_HIP_CHECK(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                            ctx->device));
+  const size_t lds_cu = static_cast<size_t>(lds_max);
+  const size_t fixed = (sizeof(RefTables) + 15) / 16 * 16 + 2 * kWave * sizeof(float4) +
+                       kWave * sizeof(uint32_t) +
+                       static_cast<size_t>(max_res) * (2 * 16 + (3 * kSlots + 3) * sizeof(float) + 3);
   const size_t ncells = ag->ca_grid.n ? ag->ca_grid.ncodes + 1 : 0;
   const size_t ag_bytes = n_ag * (16 + 6 * 4 + 4 + 1) + ncells * 4;
-  if (ncells && lds + ag_bytes <= size_t(120) << 10) {
+  auto list_bytes = [](size_t cap) { return cap * (sizeof(float4) + sizeof(uint32_t)) + cap / 8; };
+  if (n_ag >= (size_t(1) << 24))
+    return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: more than 2^24 antigen Calphas");
+  const size_t cu = ctx->cu_count > 0 ? static_cast<size_t>(ctx->cu_count) : 256;
+  size_t cap = 0;
+  bool ag_in = false;
+  for (size_t bpc = std::min<size_t>(4, (nc + cu - 1) / cu); bpc >= 1; --bpc) {
+    const size_t budget = lds_cu / bpc;
+    ag_in = ncells && fixed + ag_bytes + list_bytes(1024) <= budget;
+    const size_t used = fixed + (ag_in ? ag_bytes : 0);
+    cap = used < budget ? (budget - used) * 8 / 161 / 64 * 64 : 0;
+    cap = std::min<size_t>(cap, 8192);
+    if (cap >= 1024) break;
+  }
+  if (cap < 64)
+    return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: candidates too long for LDS");
+  a.pair_cap = static_cast<uint32_t>(cap);
+  size_t lds = fixed + list_bytes(cap);
+  if (ag_in) {
     a.n_ag_lds = static_cast<uint32_t>(n_ag);
     a.ncells_lds = static_cast<uint32_t>(ncells);
     lds += ag_bytes;
