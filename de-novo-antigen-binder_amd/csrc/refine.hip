@@ -38,6 +38,7 @@ namespace {
 constexpr int kSlots = 14;        // atom slots per residue (the largest topology, Trp)
 constexpr int kMaxDefs = 12;      // side-chain placements per residue type
 constexpr uint32_t kMaxRefineRes = 512;
+constexpr uint8_t kBuildNone = 0, kBuildSide = 1, kBuildIdeal = 2;  // deferred side-chain builds
 constexpr uint32_t kTagMc = 0x6d636d76u;  // draws of the side-chain moves
 constexpr uint32_t kTagBb = 0x62626d76u;  // draws of the torsion moves
 
@@ -85,7 +86,15 @@ struct RefineArgs {
   StepParams sp;
   float mut_p, rot_p, bb_p, max_torsion, iface_sq;
   uint32_t moves;
+  uint32_t finite;  // every input coordinate finite: energies stay finite
 };
+
+// A value every lane holds alike, moved to a scalar register so that the
+// control flow depending on it is scalar.
+__device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float uniform_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 
 __device__ inline float4 u4(uint32_t it, uint32_t r, uint32_t tag, const StepParams& sp) {
   const Philox4 u = philox4x32_10(it, r, tag, 0u, sp.seed_lo, sp.seed_hi);
@@ -100,34 +109,109 @@ struct CaSet {
   uint32_t n;
 };
 
-// This lane's part of residue_contribution_allatom (src/diffusion.rs:422-456)
-// for a Calpha of type `aa` at p: the antigen Calphas lane, lane + 64, ...
-__device__ inline double pair_terms(const RefTables& T, const CaSet& g, uint32_t aa, float px,
-                                    float py, float pz, uint32_t lane) {
+// One antigen Calpha's term of residue_contribution_allatom (src/diffusion.rs:
+// 422-456) on a Calpha of type `aa` at squared distance r_sq, for a pair the
+// reference does not skip (0.25 <= r_sq <= 100); q holds the antigen Calpha's
+// r_min_half, epsilon, charge and hydrophobic flag + 256 * atom type.
+__device__ inline double pair_term(const RefTables& T, uint32_t aa, const float4 q, float r_sq) {
   const float brm = T.ca_rmh[aa], be = T.ca_eps[aa], bq = T.ca_q[aa];
   const uint8_t bh = T.ca_hyd[aa];
   const uint32_t bt = T.ca_type[aa];
+  const float r_ij = brm + q.x;
+  const float eps = sqrtf(be * q.y);
+  const float r2 = (r_ij * r_ij) / r_sq;
+  const float r6 = r2 * r2 * r2;
+  double e = static_cast<double>(eps * (r6 * r6 - 2.0f * r6));
+  if (bq != 0.0f && q.z != 0.0f) e += static_cast<double>(332.0f * bq * q.z / sqrtf(r_sq));
+  const uint32_t packed = static_cast<uint32_t>(q.w);
+  const uint32_t ah = packed & 1u, at = packed >> 8;
+  if (bh == 1 && ah == 1 && r_sq < 36.0f) e -= 0.5;
+  const bool hb = (((kHbondDonorMask >> bt) & (kHbondAcceptorMask >> at)) |
+                   ((kHbondAcceptorMask >> bt) & (kHbondDonorMask >> at))) & 1u;
+  if (hb && r_sq <= kHbCutoffSq) {
+    const float z = (sqrtf(r_sq) - kHbR0) / kHbSigma;
+    e += static_cast<double>(-kHbDepth * expf(-0.5f * z * z));
+  }
+  return e;
+}
+
+// residue_contribution_allatom of residues [r0, r1) at their Calphas, this
+// lane's FP64 part: a cheap squared-distance pass over every antigen Calpha
+// (the reference's skip test) compacts the pairs it keeps into buf, then
+// their terms are evaluated lane-parallel. With kTwo, one residue (r0) is
+// scored as type ta into ea and as type tb into eb in the same pass; else
+// each residue is scored as its current type into ea.
+template <bool kTwo>
+__device__ inline void contrib(const RefTables& T, const CaSet& g, const uint8_t* aa,
+                               const float* X, const float* Y, const float* Z, uint32_t r0,
+                               uint32_t r1, uint32_t ta, uint32_t tb, uint32_t lane, uint2* buf,
+                               uint32_t bcap, double& ea, double& eb) {
+  ea = eb = 0.0;
+  uint32_t cnt = 0;
+  auto flush = [&]() {
+    __syncthreads();
+    for (uint32_t k = lane; k < cnt; k += kWave) {
+      const uint2 p = buf[k];
+      const uint32_t t = p.y * kSlots + 1;
+      const float dx = X[t] - g.x[p.x], dy = Y[t] - g.y[p.x], dz = Z[t] - g.z[p.x];
+      const float r_sq = dx * dx + dy * dy + dz * dz;
+      const float4 q = g.par[p.x];
+      if (kTwo) {
+        ea += pair_term(T, ta, q, r_sq);
+        eb += pair_term(T, tb, q, r_sq);
+      } else {
+        ea += pair_term(T, aa[p.y], q, r_sq);
+      }
+    }
+    __syncthreads();
+    cnt = 0;
+  };
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t cb = r0; cb < r1; cb += kWave) {
+    // the chunk's Calphas, residue cb + j in lane j, broadcast by readlane
+    const uint32_t nrc = min(r1 - cb, static_cast<uint32_t>(kWave));
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (lane < nrc) {
+      const uint32_t t = (cb + lane) * kSlots + 1;
+      px = X[t];
+      py = Y[t];
+      pz = Z[t];
+    }
+    for (uint32_t i0 = 0; i0 < g.n; i0 += kWave) {
+      const uint32_t i = i0 + lane;
+      const bool live = i < g.n;
+      const float ax = live ? g.x[i] : 0.0f, ay = live ? g.y[i] : 0.0f,
+                  az = live ? g.z[i] : 0.0f;
+      // four residues per round (independent chains); room for their pairs first
+      for (uint32_t j0 = 0; j0 < nrc; j0 += 4) {
+        if (cnt + 4 * kWave > bcap) flush();
+#pragma unroll
+        for (uint32_t v = 0; v < 4; ++v) {
+          const uint32_t j = j0 + v, jl = min(j, nrc - 1);
+          const float dx = readlane_f32(px, jl) - ax, dy = readlane_f32(py, jl) - ay,
+                      dz = readlane_f32(pz, jl) - az;
+          const float r_sq = dx * dx + dy * dy + dz * dz;
+          const bool keep = live & (j < nrc) & !(r_sq > 100.0f) & !(r_sq < 0.25f);  // NaN kept
+          const uint64_t b = ballot64(keep);
+          if (keep) buf[cnt + static_cast<uint32_t>(__popcll(b & below))] = make_uint2(i, cb + j);
+          cnt += static_cast<uint32_t>(__popcll(b));
+        }
+      }
+    }
+  }
+  flush();
+}
+
+// This lane's part of residue_contribution_allatom for a Calpha of type `aa`
+// at p: the antigen Calphas lane, lane + 64, ...
+__device__ inline double pair_terms(const RefTables& T, const CaSet& g, uint32_t aa, float px,
+                                    float py, float pz, uint32_t lane) {
   double e = 0.0;
   for (uint32_t i = lane; i < g.n; i += kWave) {
     const float dx = px - g.x[i], dy = py - g.y[i], dz = pz - g.z[i];
     const float r_sq = dx * dx + dy * dy + dz * dz;
     if (r_sq > 100.0f || r_sq < 0.25f) continue;
-    const float4 q = g.par[i];
-    const float r_ij = brm + q.x;
-    const float eps = sqrtf(be * q.y);
-    const float r2 = (r_ij * r_ij) / r_sq;
-    const float r6 = r2 * r2 * r2;
-    e += static_cast<double>(eps * (r6 * r6 - 2.0f * r6));
-    if (bq != 0.0f && q.z != 0.0f) e += static_cast<double>(332.0f * bq * q.z / sqrtf(r_sq));
-    const uint32_t packed = static_cast<uint32_t>(q.w);
-    const uint32_t ah = packed & 1u, at = packed >> 8;
-    if (bh == 1 && ah == 1 && r_sq < 36.0f) e -= 0.5;
-    const bool hb = (((kHbondDonorMask >> bt) & (kHbondAcceptorMask >> at)) |
-                     ((kHbondAcceptorMask >> bt) & (kHbondDonorMask >> at))) & 1u;
-    if (hb && r_sq <= kHbCutoffSq) {
-      const float z = (sqrtf(r_sq) - kHbR0) / kHbSigma;
-      e += static_cast<double>(-kHbDepth * expf(-0.5f * z * z));
-    }
+    e += pair_term(T, aa, g.par[i], r_sq);
   }
   return e;
 }
@@ -164,6 +248,28 @@ __device__ inline void side_chain(const RefTables& T, uint32_t aa, uint32_t rot,
   }
 }
 
+// A residue of type new_aa, rotamer new_rot, built anew on an ideal backbone
+// around its Calpha (the other-type branch of mutate_residue,
+// src/allatom.rs:316-384).
+__device__ inline void rebuild_ideal(const RefTables& T, uint8_t* aa, uint8_t* rot, uint32_t r,
+                                     uint32_t new_aa, uint32_t new_rot, float* X, float* Y,
+                                     float* Z) {
+  float* x = X + r * kSlots;
+  float* y = Y + r * kSlots;
+  float* z = Z + r * kSlots;
+  geo::V3 bb[5];
+  int nb = geo::ideal_backbone(x[1], y[1], z[1], new_aa == kGly, bb);
+  if (nb > T.natoms[new_aa]) nb = T.natoms[new_aa];
+  for (int k = 0; k < nb; ++k) {
+    x[k] = bb[k].x;
+    y[k] = bb[k].y;
+    z[k] = bb[k].z;
+  }
+  aa[r] = static_cast<uint8_t>(new_aa);
+  rot[r] = static_cast<uint8_t>(new_rot);
+  side_chain(T, new_aa, new_rot, nb, x, y, z);
+}
+
 // mutate_residue (src/allatom.rs:299-384) on one residue's slots: the same
 // type keeps the backbone and rebuilds the side chain; another type is built
 // anew on an ideal backbone around the Calpha.
@@ -177,17 +283,7 @@ __device__ inline void mutate(const RefTables& T, uint8_t* aa, uint8_t* rot, uin
     side_chain(T, new_aa, new_rot, new_aa == kGly ? 4 : 5, x, y, z);
     return;
   }
-  geo::V3 bb[5];
-  int nb = geo::ideal_backbone(x[1], y[1], z[1], new_aa == kGly, bb);
-  if (nb > T.natoms[new_aa]) nb = T.natoms[new_aa];
-  for (int k = 0; k < nb; ++k) {
-    x[k] = bb[k].x;
-    y[k] = bb[k].y;
-    z[k] = bb[k].z;
-  }
-  aa[r] = static_cast<uint8_t>(new_aa);
-  rot[r] = static_cast<uint8_t>(new_rot);
-  side_chain(T, new_aa, new_rot, nb, x, y, z);
+  rebuild_ideal(T, aa, rot, r, new_aa, new_rot, X, Y, Z);
 }
 
 // disulfide_energy over the Cys SG slots (slot 5), pairs in residue order,
@@ -214,16 +310,15 @@ __device__ inline double disulfide(const uint8_t* aa, const float* X, const floa
 }
 
 // Sum of residue_contribution_allatom over residues r..nr-1 plus the
-// disulfide term: the energy a torsion move at r is judged by.
+// disulfide term: the energy a torsion move at r is judged by. ncys: the
+// candidate's Cys count (no pair, no disulfide term below two).
 __device__ inline double chain_energy(const RefTables& T, const CaSet& g, const uint8_t* aa,
                                       const float* X, const float* Y, const float* Z, uint32_t r,
-                                      uint32_t nr, uint32_t lane) {
-  double part = 0.0;  // one reduction for the whole chain
-  for (uint32_t i = r; i < nr; ++i) {
-    const uint32_t t = i * kSlots + 1;
-    part += pair_terms(T, g, aa[i], X[t], Y[t], Z[t], lane);
-  }
-  return wave_sum_f64(part) + disulfide(aa, X, Y, Z, nr);
+                                      uint32_t nr, uint32_t ncys, uint32_t lane, uint2* buf,
+                                      uint32_t bcap) {
+  double part, unused;
+  contrib<false>(T, g, aa, X, Y, Z, r, nr, 0, 0, lane, buf, bcap, part, unused);
+  return wave_sum_f64(part) + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
 }
 
 // perturb_phi / perturb_psi (src/allatom.rs:392-456) by rad: phi turns the
@@ -242,8 +337,7 @@ __device__ inline void torsion(const RefTables& T, const uint8_t* aa, float* X, 
   __syncthreads();
   for (uint32_t q = b + lane; q < nr * kSlots; q += kWave) {
     const uint32_t rr = q / kSlots, k = q - rr * kSlots;
-    if (k >= T.natoms[aa[rr]]) continue;
-    if (rr == r && (phi ? k < 2 : k != 3)) continue;
+    if (rr == r && (phi ? k < 2 : k != 3)) continue;  // slots past the atoms turn too, unread
     const geo::V3 o = geo::rodrigues({X[q], Y[q], Z[q]}, p1, axis, cs, sn);
     X[q] = o.x;
     Y[q] = o.y;
@@ -308,19 +402,34 @@ __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const
       s.RQ[lane] = make_float4(px, py, pz, T.cg_q[ty]);
       s.RP[lane] = make_float4(T.cg_eps[ty], T.cg_sig[ty], static_cast<float>(T.cg_hyd[ty]), 0.0f);
       if (grid.n) {
-        const int32_t kx = cell_key(px, grid.inv), ky = cell_key(py, grid.inv),
-                      kz = cell_key(pz, grid.inv);
+        // cell_range of the 27 neighbours (dx, dy, dz order), with the key
+        // range test and Morton spread done once per axis offset
+        const int32_t k[3] = {cell_key(px, grid.inv), cell_key(py, grid.inv),
+                              cell_key(pz, grid.inv)};
+        uint32_t sp[3][3];
+        bool ok[3][3];
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx)
+        for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-          for (int dy = -1; dy <= 1; ++dy)
+          for (int d = 0; d < 3; ++d) {
+            const int32_t kk = wrap_add(k[ax], d - 1);
+            ok[ax][d] = kk >= grid.kmin[ax] && kk <= grid.kmax[ax];
+            sp[ax][d] = spread3(static_cast<uint32_t>(kk) - static_cast<uint32_t>(grid.kmin[ax])) << ax;
+          }
 #pragma unroll
-            for (int dz = -1; dz <= 1; ++dz) {
-              const int c = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-              uint32_t b = 0, e = 0;
-              if (cell_range(grid, wrap_add(kx, dx), wrap_add(ky, dy), wrap_add(kz, dz), b, e)) {
-                cbeg[c] = b;
-                ccnt[c] = e - b;
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+          for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dz = 0; dz < 3; ++dz) {
+              const int c = dx * 9 + dy * 3 + dz;
+              if (ok[0][dx] & ok[1][dy] & ok[2][dz]) {
+                const uint32_t code = sp[0][dx] | sp[1][dy] | sp[2][dz];
+                const uint32_t b = grid.cell_start[code], e = grid.cell_start[code + 1];
+                if (e > b) {
+                  cbeg[c] = b;
+                  ccnt[c] = e - b;
+                }
               }
               n += ccnt[c];
             }
@@ -434,7 +543,8 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   }
   const RefTables& T = *reinterpret_cast<const RefTables*>(lds_all);
   float* lds = lds_all + kTabWords;
-  const uint32_t g0 = a.cand_res_off[c], nr = a.cand_res_off[c + 1] - g0;
+  const uint32_t g0 = uniform_u32(a.cand_res_off[c]),
+                 nr = uniform_u32(a.cand_res_off[c + 1]) - g0;
   const uint32_t ms = a.max_res * kSlots;
   // LDS: the antigen Calpha set, its residue-level parameters and grid (when
   // they fit, n_ag_lds = n_ag), then the candidate's slots, forces, the
@@ -466,6 +576,7 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   uint8_t* aa = gh + na_l;
   uint8_t* rot = aa + a.max_res;
   uint8_t* iface = rot + a.max_res;
+  uint8_t* pend = iface + a.max_res;  // deferred build of each residue
   if (kLds) {
     for (uint32_t i = lane; i < a.n_ag; i += kWave) {
       gpar[i] = a.ag_par[i];
@@ -496,8 +607,17 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   for (uint32_t r = lane; r < nr; r += kWave) {
     aa[r] = a.res_aa[g0 + r];
     rot[r] = a.res_rot[g0 + r];
+    pend[r] = kBuildNone;
   }
   __syncthreads();
+
+  // the pair list of the filtered contributions reuses the force pass's
+  // term list (free outside it); the candidate's Cys count (uniform)
+  uint2* lbuf = reinterpret_cast<uint2*>(cs.CT);
+  const uint32_t lcap = 2 * a.pair_cap;
+  uint32_t ncys = 0;
+  for (uint32_t cb = 0; cb < nr; cb += kWave)
+    ncys += static_cast<uint32_t>(__popcll(ballot64(cb + lane < nr && aa[cb + lane] == kCys)));
 
   for (uint32_t s = 0; s < a.n_steps; ++s) {
     const uint32_t it = a.iter0 + s;
@@ -510,25 +630,50 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
       const float4 d = residue_delta(a.cx, a.cy, a.cz, a.sp, a.r0,
                                      make_float4(X[t], Y[t], Z[t], 0.0f), F[3 * r], F[3 * r + 1],
                                      F[3 * r + 2], a.sigma[s], it, g0 + r);
-      for (uint32_t k = 0; k < T.natoms[aa[r]]; ++k) {
-        X[r * kSlots + k] += d.x;
-        Y[r * kSlots + k] += d.y;
-        Z[r * kSlots + k] += d.z;
-      }
+      F[3 * r] = d.x;
+      F[3 * r + 1] = d.y;
+      F[3 * r + 2] = d.z;
     }
     __syncthreads();
-    // 2. interface mask (src/diffusion.rs:341-351), lane-split over the
-    //    antigen Calphas residue by residue
-    for (uint32_t r = 0; r < nr; ++r) {
-      const uint32_t t = r * kSlots + 1;
-      const float px = X[t], py = Y[t], pz = Z[t];
-      bool in = false;
-      for (uint32_t i = lane; i < g.n; i += kWave) {
-        const float dx = px - g.x[i], dy = py - g.y[i], dz = pz - g.z[i];
-        in = in || dx * dx + dy * dy + dz * dz < a.iface_sq;
+    // set_ca_pos: every slot moves by its residue's displacement (slots past
+    // the residue's atoms are never read before a rebuild writes them)
+    for (uint32_t q = lane; q < nr * kSlots; q += kWave) {
+      const uint32_t r = q / kSlots;
+      X[q] += F[3 * r];
+      Y[q] += F[3 * r + 1];
+      Z[q] += F[3 * r + 2];
+    }
+    __syncthreads();
+    // 2. interface mask (src/diffusion.rs:341-351): per chunk of 64
+    //    residues (Calphas in lanes, broadcast by readlane), each lane tests
+    //    its antigen Calphas against every residue; the wave's compare masks
+    //    set the residue's bit
+    for (uint32_t cb = 0; cb < nr; cb += kWave) {
+      const uint32_t nrc = min(nr - cb, static_cast<uint32_t>(kWave));
+      float px = 0.0f, py = 0.0f, pz = 0.0f;
+      if (lane < nrc) {
+        const uint32_t t = (cb + lane) * kSlots + 1;
+        px = X[t];
+        py = Y[t];
+        pz = Z[t];
       }
-      const bool any = ballot64(in) != 0;
-      if (lane == 0) iface[r] = any ? 1 : 0;
+      uint64_t m = 0;
+      for (uint32_t i0 = 0; i0 < g.n; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        const bool live = i < g.n;
+        const float ax = live ? g.x[i] : 0.0f, ay = live ? g.y[i] : 0.0f,
+                    az = live ? g.z[i] : 0.0f;
+        for (uint32_t j0 = 0; j0 < nrc; j0 += 4) {
+#pragma unroll
+          for (uint32_t v = 0; v < 4; ++v) {
+            const uint32_t j = j0 + v, jl = min(j, nrc - 1);
+            const float dx = readlane_f32(px, jl) - ax, dy = readlane_f32(py, jl) - ay,
+                        dz = readlane_f32(pz, jl) - az;
+            if (ballot64(live & (dx * dx + dy * dy + dz * dz < a.iface_sq))) m |= 1ull << jl;
+          }
+        }
+      }
+      if (lane < nrc) iface[cb + lane] = static_cast<uint8_t>((m >> lane) & 1ull);
     }
     __syncthreads();
     // this step's draws of both move kinds, one residue per lane
@@ -540,16 +685,19 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
     // 3. side-chain moves (src/diffusion.rs:353-398)
     if (a.moves & DNAB_MC_SIDECHAIN) {
       for (uint32_t r = 0; r < nr; ++r) {
-        const float4 u = U[2 * r];
-        const bool in = iface[r] != 0;
+        const float4 uu = U[2 * r];
+        const float4 u = make_float4(uniform_f32(uu.x), uniform_f32(uu.y), uniform_f32(uu.z),
+                                     uniform_f32(uu.w));
+        const bool in = uniform_u32(iface[r]) != 0;
         const float eff_mut = in ? a.mut_p * 3.0f : a.mut_p * 0.4f;
         const float eff_rot = in ? a.rot_p * 2.0f : a.rot_p;
         if (!(u.x < eff_mut)) continue;
         const uint32_t t = r * kSlots + 1;
-        const uint32_t old_aa = aa[r], old_rot = rot[r];
+        const uint32_t old_aa = uniform_u32(aa[r]), old_rot = uniform_u32(rot[r]);
+        const bool rot_move = u.y < eff_rot;
         uint32_t new_aa = old_aa, new_rot;
-        if (u.y < eff_rot) {  // rotamer move
-          const uint32_t nrot = T.nrot[old_aa];
+        if (rot_move) {  // rotamer move
+          const uint32_t nrot = uniform_u32(T.nrot[old_aa]);
           if (nrot == 0) continue;
           new_rot = min(static_cast<uint32_t>(u.z * static_cast<float>(nrot)), nrot - 1);
         } else {  // mutation: the new type's first rotamer
@@ -557,11 +705,49 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
                        static_cast<uint32_t>(kNumAminoAcids - 1));
           new_rot = 0;
         }
+        if (a.finite && old_aa != kCys && new_aa != kCys) {
+          // Neither the Calpha nor any Cys SG moves, so the reference's
+          // energies before and after differ only in residue r's type: a
+          // rotamer move scores the same finite energy twice (delta 0,
+          // accepted) and a mutation is decided on both types' energies
+          // before anything is rebuilt; then the outcome the reference ends
+          // with is built once. (From finite coordinates every step stays
+          // finite; a non-finite input takes the literal path below.)
+          bool accept = true;
+          if (!rot_move) {
+            double e_old, e_new;
+            contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, lbuf, lcap, e_old,
+                          e_new);
+            e_old = wave_sum_f64(e_old);
+            e_new = wave_sum_f64(e_new);
+            const double ds = ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0;
+            const double delta = (e_new + ds) - (e_old + ds);
+            accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
+          }
+          // The build itself waits for the end of the phase: until then
+          // only this residue's Calpha (unchanged by any build) and type are
+          // read, and no other move touches it this step.
+          if (lane == 0) {
+            if (rot_move) {  // apply_rotamer: the new chi on the current backbone
+              rot[r] = static_cast<uint8_t>(new_rot);
+              pend[r] = kBuildSide;
+            } else if (accept) {  // mutate_residue to (new_aa, rotamer 0)
+              pend[r] = new_aa == old_aa ? kBuildSide : kBuildIdeal;
+              aa[r] = static_cast<uint8_t>(new_aa);
+              rot[r] = 0;
+            } else {  // undone: the old type and rotamer, rebuilt as the reference does
+              pend[r] = new_aa == old_aa ? kBuildSide : kBuildIdeal;
+            }
+          }
+          __syncthreads();
+          continue;
+        }
+        // a Cys gains, loses or moves its SG: score the rebuilt residue
         const double old_e = contribution(T, g, old_aa, X[t], Y[t], Z[t], lane) +
-                             disulfide(aa, X, Y, Z, nr);
+                             (ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
         __syncthreads();
         if (lane == 0) {
-          if (new_aa == old_aa && u.y < eff_rot) {  // apply_rotamer
+          if (rot_move) {  // apply_rotamer
             rot[r] = static_cast<uint8_t>(new_rot);
             side_chain(T, old_aa, new_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
                        Y + r * kSlots, Z + r * kSlots);
@@ -570,13 +756,16 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
           }
         }
         __syncthreads();
+        const uint32_t ncys_new = ncys + (uniform_u32(aa[r]) == kCys) - (old_aa == kCys);
         const double new_e = contribution(T, g, aa[r], X[t], Y[t], Z[t], lane) +
-                             disulfide(aa, X, Y, Z, nr);
+                             (ncys_new >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
         const double delta = new_e - old_e;
         const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
         __syncthreads();
-        if (!accept && lane == 0) {
-          if (u.y < eff_rot) {  // the old chi, rebuilt on the current backbone
+        if (accept) {
+          ncys = ncys_new;
+        } else if (lane == 0) {
+          if (rot_move) {  // the old chi, rebuilt on the current backbone
             rot[r] = static_cast<uint8_t>(old_rot);
             side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
                        Y + r * kSlots, Z + r * kSlots);
@@ -587,17 +776,30 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
         __syncthreads();
       }
     }
+    // the side chains the moves above decided, built lane-parallel
+    for (uint32_t r = lane; r < nr; r += kWave) {
+      const uint8_t b = pend[r];
+      if (b == kBuildSide) {
+        side_chain(T, aa[r], rot[r], aa[r] == kGly ? 4 : 5, X + r * kSlots, Y + r * kSlots,
+                   Z + r * kSlots);
+      } else if (b == kBuildIdeal) {
+        rebuild_ideal(T, aa, rot, r, aa[r], rot[r], X, Y, Z);
+      }
+      pend[r] = kBuildNone;
+    }
+    __syncthreads();
     // 4. torsion moves (src/diffusion.rs:400-415)
     if (a.moves & DNAB_MC_BACKBONE) {
       for (uint32_t r = 0; r < nr; ++r) {
-        const float4 u = U[2 * r + 1];
-        if (!(u.x < a.bb_p)) continue;
+        const float4 uu = U[2 * r + 1];
+        if (!(uniform_f32(uu.x) < a.bb_p)) continue;
+        const float4 u = make_float4(uu.x, uniform_f32(uu.y), uniform_f32(uu.z), uniform_f32(uu.w));
         const float deg = -a.max_torsion + (a.max_torsion + a.max_torsion) * u.y;
         const float rad = geo::deg2rad(deg);
         const bool phi = u.z < 0.5f;
-        const double old_e = chain_energy(T, g, aa, X, Y, Z, r, nr, lane);
+        const double old_e = chain_energy(T, g, aa, X, Y, Z, r, nr, ncys, lane, lbuf, lcap);
         torsion(T, aa, X, Y, Z, r, nr, rad, phi, lane);
-        const double new_e = chain_energy(T, g, aa, X, Y, Z, r, nr, lane);
+        const double new_e = chain_energy(T, g, aa, X, Y, Z, r, nr, ncys, lane, lbuf, lcap);
         const double delta = new_e - old_e;
         const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
         if (!accept) torsion(T, aa, X, Y, Z, r, nr, -rad, phi, lane);  // rotate back
@@ -711,6 +913,7 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   // every residue carries its topology's atoms in topology order (the slots)
   std::vector<float4> slots(std::max<size_t>(nr * kSlots, 1), make_float4(0, 0, 0, 0));
   std::vector<uint8_t> raa(std::max<size_t>(nr, 1)), rrot(std::max<size_t>(nr, 1), 0);
+  bool finite = true;
   for (size_t r = 0; r < nr; ++r) {
     const uint8_t t = ab->res_amino_acid[r];
     if (t >= kNumAminoAcids)
@@ -725,8 +928,11 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
         return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: residue %zu rotamer out of range", r);
       rrot[r] = res_rotamer[r];
     }
-    for (uint32_t k = 0; k < e - b; ++k)
-      slots[r * kSlots + k] = make_float4(ab->atoms.x[b + k], ab->atoms.y[b + k], ab->atoms.z[b + k], 0.0f);
+    for (uint32_t k = 0; k < e - b; ++k) {
+      const float x = ab->atoms.x[b + k], y = ab->atoms.y[b + k], z = ab->atoms.z[b + k];
+      finite = finite && std::isfinite(x) && std::isfinite(y) && std::isfinite(z);
+      slots[r * kSlots + k] = make_float4(x, y, z, 0.0f);
+    }
   }
   if (nc == 0) {
     if (n_atoms_out) *n_atoms_out = 0;
@@ -740,6 +946,13 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   const size_t n_ag = ag->n_ca;
   std::vector<uint8_t> ag_aa(n_ag);
   DNAB_RETURN_IF(download(ctx, ag_aa.data(), ag->ca_aa.p, n_ag));
+  {
+    std::vector<float> c(3 * n_ag);
+    DNAB_RETURN_IF(download(ctx, c.data(), ag->ca_x.p, n_ag * sizeof(float)));
+    DNAB_RETURN_IF(download(ctx, c.data() + n_ag, ag->ca_y.p, n_ag * sizeof(float)));
+    DNAB_RETURN_IF(download(ctx, c.data() + 2 * n_ag, ag->ca_z.p, n_ag * sizeof(float)));
+    for (float v : c) finite = finite && std::isfinite(v);
+  }
   std::vector<float4> ag_par(n_ag);
   for (size_t i = 0; i < n_ag; ++i) {
     const uint8_t t = ag_aa[i] < kNumAminoAcids ? ag_aa[i] : 0;
@@ -814,6 +1027,7 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   a.max_torsion = mc->max_torsion_deg;
   a.iface_sq = mc->iface_cutoff_sq;
   a.moves = mc->moves;
+  a.finite = finite ? 1u : 0u;
   // LDS: the tables, the candidate's rows, the force pass's pair list, and
   // the antigen Calpha set with its grid when that leaves room for a list of
   // 1024 pairs. Blocks resident per CU: as many as the batch needs to fill
@@ -824,7 +1038,7 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   const size_t lds_cu = static_cast<size_t>(lds_max);
   const size_t fixed = (sizeof(RefTables) + 15) / 16 * 16 + 2 * kWave * sizeof(float4) +
                        kWave * sizeof(uint32_t) +
-                       static_cast<size_t>(max_res) * (2 * 16 + (3 * kSlots + 3) * sizeof(float) + 3);
+                       static_cast<size_t>(max_res) * (2 * 16 + (3 * kSlots + 3) * sizeof(float) + 4);
   const size_t ncells = ag->ca_grid.n ? ag->ca_grid.ncodes + 1 : 0;
   const size_t ag_bytes = n_ag * (16 + 6 * 4 + 4 + 1) + ncells * 4;
   auto list_bytes = [](size_t cap) { return cap * (sizeof(float4) + sizeof(uint32_t)) + cap / 8; };
@@ -841,7 +1055,7 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
     cap = std::min<size_t>(cap, 8192);
     if (cap >= 1024) break;
   }
-  if (cap < 64)
+  if (cap < 128)  // the filtered contributions stage up to 4 x 64 pairs in 2 * cap entries
     return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: candidates too long for LDS");
   a.pair_cap = static_cast<uint32_t>(cap);
   size_t lds = fixed + list_bytes(cap);

@@ -118,6 +118,32 @@ def test_refine_matches_the_oracle(ctx, noise, moves):
         assert n_mut > 0
 
 
+def test_refine_literal_moves_match_the_oracle(ctx):
+    """A non-finite input coordinate sends every move down the literal path
+    (rebuild, score, undo) instead of deciding on the residue types first; a
+    NaN on a side-chain atom of a non-Cys residue leaves every energy finite,
+    so both paths must end where the oracle does."""
+    ag, abs_, rots = _workload(n_cand=3, seed=6)
+    ab = abs_[1]
+    r = next(i for i, a in enumerate(ab.amino_acid) if a not in (0, 4, 7))
+    ab.atoms.x[int(ab.res_start[r]) + 5] = np.nan
+    agd = DeviceAntigen(ctx, ag, cutoff=10.0)
+    lp = langevin_params(n_steps=6, with_noise=1, schedule_offset=50, schedule_total=800, seed=4242)
+    mc = mc_params(mutation_p=0.35, rotamer_move_p=0.4, backbone_move_p=0.25)
+    prots, rot_out, _ = ctx.refine(agd, abs_, lp, mc, rots)
+    g0 = 0
+    for c, (ab, rot) in enumerate(zip(abs_, rots)):
+        st = _oracle_run(ag, ab, rot, g0, lp, mc)
+        g0 += len(ab.amino_acid)
+        p = prots[c]
+        assert list(p.amino_acid) == st["aa"] and list(rot_out[c]) == st["rot"], c
+        want = np.concatenate(st["res"]).astype(np.float64)
+        got = np.stack([p.atoms.x, p.atoms.y, p.atoms.z], 1).astype(np.float64)
+        assert np.array_equal(np.isnan(got), np.isnan(want)), c
+        ok = ~np.isnan(want)
+        assert np.abs(got[ok] - want[ok]).max() <= 1e-4, c
+
+
 def _cloud(xyz, aa):
     tab = O.reference_tables()
     from types import SimpleNamespace
