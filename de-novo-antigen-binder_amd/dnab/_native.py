@@ -103,6 +103,14 @@ class PdbChainC(C.Structure):
                 ("n_res", C.c_size_t), ("chain_id", C.c_char)]
 
 
+class CandidateReportC(C.Structure):
+    _fields_ = [("rank", C.c_size_t), ("sequence", C.c_char_p), ("light_sequence", C.c_char_p),
+                ("energy", C.c_float), ("entropy_penalty", C.c_float),
+                ("dg_corrected", C.c_float), ("net_charge", C.c_int32),
+                ("aggregation_risk", C.c_int), ("n_interface", C.c_size_t),
+                ("clashes", C.c_int64), ("rama_outliers", C.c_int64)]
+
+
 class McParamsC(C.Structure):
     _fields_ = [("mutation_p", C.c_float), ("rotamer_move_p", C.c_float),
                 ("backbone_move_p", C.c_float), ("max_torsion_deg", C.c_float),
@@ -242,6 +250,8 @@ _PROTOS = {
                                     C.c_size_t, C.POINTER(C.c_size_t)]),
     "dnab_write_pdb_chains": (C.c_int, [C.POINTER(PdbChainC), C.c_size_t, C.c_int, C.c_char_p,
                                         C.c_size_t, C.POINTER(C.c_size_t)]),
+    "dnab_write_report": (C.c_int, [C.POINTER(CandidateReportC), C.c_size_t, C.c_int, C.c_char_p,
+                                    C.c_size_t, C.POINTER(C.c_size_t)]),
     "dnab_refine": (C.c_int, [vp, vp, C.POINTER(PopulationDescC), u8p, C.POINTER(LangevinParamsC),
                               C.POINTER(McParamsC), C.POINTER(ProteinOutC), u8p, f64p,
                               C.POINTER(C.c_size_t)]),

@@ -17,6 +17,8 @@
 //   binder::affinity_maturation_allatom    src/diffusion.rs:924-961
 //   binder::parse_pdb, write_pdb_allatom, write_pdb_multi
 //                                          src/pdb.rs:19-57, 121-191
+//   binder::CandidateReport, write_fasta, write_json, write_csv
+//                                          src/report.rs:8-113
 //   binder::run_allatom_gpu_phase          src/diffusion.rs:1008-1062 (GPU phase,
 //                                          gradient steps + scoring, no MC moves)
 //
@@ -368,6 +370,44 @@ inline std::string write_pdb_multi(const std::vector<std::pair<const AtomProtein
 }
 inline std::string write_pdb_allatom(const AtomProtein& p, char chain_id) {
   return write_pdb_multi({{&p, chain_id}}, false);
+}
+
+// CandidateReport (src/report.rs:8-20)
+struct CandidateReport {
+  size_t rank = 0;
+  std::string sequence;
+  std::optional<std::string> light_sequence;
+  float energy = 0.0f, entropy_penalty = 0.0f, dg_corrected = 0.0f;
+  int32_t net_charge = 0;
+  bool aggregation_risk = false;
+  size_t n_interface = 0;
+  std::optional<size_t> clashes, rama_outliers;
+};
+
+inline std::string write_report(const std::vector<CandidateReport>& reports, int format) {
+  std::vector<dnab_candidate_report> rs;
+  for (const auto& r : reports)
+    rs.push_back({r.rank, r.sequence.c_str(), r.light_sequence ? r.light_sequence->c_str() : nullptr,
+                  r.energy, r.entropy_penalty, r.dg_corrected, r.net_charge,
+                  r.aggregation_risk ? 1 : 0, r.n_interface,
+                  r.clashes ? static_cast<int64_t>(*r.clashes) : -1,
+                  r.rama_outliers ? static_cast<int64_t>(*r.rama_outliers) : -1});
+  size_t n = 0;
+  check(dnab_write_report(rs.data(), rs.size(), format, nullptr, 0, &n));
+  std::string out(n + 1, '\0');
+  check(dnab_write_report(rs.data(), rs.size(), format, out.data(), out.size(), &n));
+  out.resize(n);
+  return out;
+}
+// write_fasta / write_json / write_csv (src/report.rs:41-113)
+inline std::string write_fasta(const std::vector<CandidateReport>& r) {
+  return write_report(r, DNAB_REPORT_FASTA);
+}
+inline std::string write_json(const std::vector<CandidateReport>& r) {
+  return write_report(r, DNAB_REPORT_JSON);
+}
+inline std::string write_csv(const std::vector<CandidateReport>& r) {
+  return write_report(r, DNAB_REPORT_CSV);
 }
 
 // compute_forces_with_grid (src/energy.rs:322-398): outputs sized >= n_ab.

@@ -158,6 +158,28 @@ typedef struct dnab_pdb_chain {
  * across chains, then END. */
 int dnab_write_pdb_chains(const dnab_pdb_chain* chains, size_t n_chains, int ter, char* out,
                           size_t capacity, size_t* n_chars);
+/* One designed candidate's summary (CandidateReport, src/report.rs:8-20).
+ * Strings are NUL-terminated; light_sequence = NULL for a single chain;
+ * clashes / rama_outliers < 0 for none. */
+typedef struct dnab_candidate_report {
+  size_t rank;
+  const char* sequence;
+  const char* light_sequence;
+  float energy, entropy_penalty, dg_corrected;
+  int32_t net_charge;
+  int aggregation_risk;
+  size_t n_interface;
+  int64_t clashes;
+  int64_t rama_outliers;
+} dnab_candidate_report;
+#define DNAB_REPORT_FASTA 0
+#define DNAB_REPORT_JSON 1
+#define DNAB_REPORT_CSV 2
+/* write_fasta / write_json / write_csv (src/report.rs:41-113) of n reports
+ * in order, into `out` as the PDB writers do (out = NULL queries the
+ * length). */
+int dnab_write_report(const dnab_candidate_report* reports, size_t n, int format, char* out,
+                      size_t capacity, size_t* n_chars);
 /* place_atom (src/rotamer.rs:198-232), NERF. */
 int dnab_place_atom(const float a[3], const float b[3], const float c[3], float bond_len,
                     float angle_deg, float dihedral_deg, float out[3]);

@@ -197,6 +197,31 @@ int main() {
     }
   });
 
+  run("report::write_fasta / write_json / write_csv (src/report.rs:135-171)", [&] {
+    CandidateReport r;
+    r.rank = 1;
+    r.sequence = "ACDEFG";
+    r.energy = -12.5f;
+    r.entropy_penalty = 5.7f;
+    r.dg_corrected = -6.8f;
+    r.net_charge = 2;
+    r.n_interface = 4;
+    r.clashes = 0;
+    r.rama_outliers = 1;
+    const std::string fa = write_fasta({r});
+    CHECK(fa.find(">candidate_1") != std::string::npos && fa.find("ACDEFG") != std::string::npos);
+    CandidateReport fv = r;
+    fv.light_sequence = "HIKLMN";
+    const std::string fa2 = write_fasta({fv});
+    CHECK(fa2.find("_VH") != std::string::npos && fa2.find("_VL") != std::string::npos &&
+          fa2.find("HIKLMN") != std::string::npos);
+    const std::string js = write_json({r, r});
+    CHECK(js.front() == '[' && js.compare(js.size() - 2, 2, "]\n") == 0);
+    CHECK(std::count(js.begin(), js.end(), '{') == 2 && std::count(js.begin(), js.end(), '}') == 2);
+    const std::string cs = write_csv({r, r});
+    CHECK(std::count(cs.begin(), cs.end(), '\n') == 3 && cs.compare(0, 5, "rank,") == 0);
+  });
+
   run("gpu::score_batch uniform candidates (src/gpu.rs:166) vs the breakdown's shader terms", [&] {
     auto ag = protein_from_ca_trace({0.0f, 3.8f, 7.6f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f},
                                     {AminoAcid::Asp, AminoAcid::Lys, AminoAcid::Leu});
