@@ -89,6 +89,52 @@ struct RefineArgs {
   uint32_t finite;  // every input coordinate finite: energies stay finite
 };
 
+// Orders this wave's LDS accesses around it (a wave's LDS operations complete
+// in issue order): the synchronisation of work that one wave owns alone.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// FP64 sums over the workgroup's NW waves: each wave's DPP sum, then the
+// waves' sums in wave order (red: NW doubles of LDS), the same on every lane.
+template <int NW>
+__device__ inline double block_sum(double v, double* red, uint32_t lane, uint32_t wv) {
+  v = wave_sum_f64(v);
+  if constexpr (NW == 1) {
+    return v;
+  } else {
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double t = red[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) t += red[w];
+    __syncthreads();
+    return t;
+  }
+}
+template <int NW>
+__device__ inline void block_sum2(double& a, double& b, double* red, uint32_t lane, uint32_t wv) {
+  a = wave_sum_f64(a);
+  b = wave_sum_f64(b);
+  if constexpr (NW > 1) {
+    if (lane == 0) {
+      red[wv] = a;
+      red[NW + wv] = b;
+    }
+    __syncthreads();
+    a = red[0];
+    b = red[NW];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      a += red[w];
+      b += red[NW + w];
+    }
+    __syncthreads();
+  }
+}
+
 // A value every lane holds alike, moved to a scalar register so that the
 // control flow depending on it is scalar.
 __device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -136,20 +182,23 @@ __device__ inline double pair_term(const RefTables& T, uint32_t aa, const float4
 }
 
 // residue_contribution_allatom of residues [r0, r1) at their Calphas, this
-// lane's FP64 part: a cheap squared-distance pass over every antigen Calpha
-// (the reference's skip test) compacts the pairs it keeps into buf, then
-// their terms are evaluated lane-parallel. With kTwo, one residue (r0) is
-// scored as type ta into ea and as type tb into eb in the same pass; else
-// each residue is scored as its current type into ea.
+// lane's FP64 part over the antigen Calphas of chunks i_first, i_first +
+// i_step, ... (64 each; a workgroup's waves split them): a cheap
+// squared-distance pass (the reference's skip test) compacts the pairs it
+// keeps into the wave's buf, then their terms are evaluated lane-parallel.
+// With kTwo, one residue (r0) is scored as type ta into ea and as type tb
+// into eb in the same pass; else each residue is scored as its current type
+// into ea.
 template <bool kTwo>
 __device__ inline void contrib(const RefTables& T, const CaSet& g, const uint8_t* aa,
                                const float* X, const float* Y, const float* Z, uint32_t r0,
-                               uint32_t r1, uint32_t ta, uint32_t tb, uint32_t lane, uint2* buf,
-                               uint32_t bcap, double& ea, double& eb) {
+                               uint32_t r1, uint32_t ta, uint32_t tb, uint32_t lane,
+                               uint32_t i_first, uint32_t i_step, uint2* buf, uint32_t bcap,
+                               double& ea, double& eb) {
   ea = eb = 0.0;
   uint32_t cnt = 0;
   auto flush = [&]() {
-    __syncthreads();
+    wave_sync();
     for (uint32_t k = lane; k < cnt; k += kWave) {
       const uint2 p = buf[k];
       const uint32_t t = p.y * kSlots + 1;
@@ -163,7 +212,7 @@ __device__ inline void contrib(const RefTables& T, const CaSet& g, const uint8_t
         ea += pair_term(T, aa[p.y], q, r_sq);
       }
     }
-    __syncthreads();
+    wave_sync();
     cnt = 0;
   };
   const uint64_t below = (1ull << lane) - 1ull;
@@ -177,7 +226,7 @@ __device__ inline void contrib(const RefTables& T, const CaSet& g, const uint8_t
       py = Y[t];
       pz = Z[t];
     }
-    for (uint32_t i0 = 0; i0 < g.n; i0 += kWave) {
+    for (uint32_t i0 = i_first; i0 < g.n; i0 += i_step) {
       const uint32_t i = i0 + lane;
       const bool live = i < g.n;
       const float ax = live ? g.x[i] : 0.0f, ay = live ? g.y[i] : 0.0f,
@@ -202,12 +251,12 @@ __device__ inline void contrib(const RefTables& T, const CaSet& g, const uint8_t
   flush();
 }
 
-// This lane's part of residue_contribution_allatom for a Calpha of type `aa`
-// at p: the antigen Calphas lane, lane + 64, ...
+// This thread's part of residue_contribution_allatom for a Calpha of type
+// `aa` at p: the antigen Calphas tid, tid + nt, ...
 __device__ inline double pair_terms(const RefTables& T, const CaSet& g, uint32_t aa, float px,
-                                    float py, float pz, uint32_t lane) {
+                                    float py, float pz, uint32_t tid, uint32_t nt) {
   double e = 0.0;
-  for (uint32_t i = lane; i < g.n; i += kWave) {
+  for (uint32_t i = tid; i < g.n; i += nt) {
     const float dx = px - g.x[i], dy = py - g.y[i], dz = pz - g.z[i];
     const float r_sq = dx * dx + dy * dy + dz * dz;
     if (r_sq > 100.0f || r_sq < 0.25f) continue;
@@ -216,10 +265,13 @@ __device__ inline double pair_terms(const RefTables& T, const CaSet& g, uint32_t
   return e;
 }
 
-// residue_contribution_allatom, FP64 total on every lane.
+// residue_contribution_allatom, FP64 total on every thread.
+template <int NW>
 __device__ inline double contribution(const RefTables& T, const CaSet& g, uint32_t aa, float px,
-                                      float py, float pz, uint32_t lane) {
-  return wave_sum_f64(pair_terms(T, g, aa, px, py, pz, lane));
+                                      float py, float pz, uint32_t lane, uint32_t wv,
+                                      double* red) {
+  return block_sum<NW>(pair_terms(T, g, aa, px, py, pz, wv * kWave + lane, NW * kWave), red,
+                       lane, wv);
 }
 
 // Side chain of residue slots X/Y/Z (type aa, rotamer rot) from its
@@ -312,22 +364,23 @@ __device__ inline double disulfide(const uint8_t* aa, const float* X, const floa
 // Sum of residue_contribution_allatom over residues r..nr-1 plus the
 // disulfide term: the energy a torsion move at r is judged by. ncys: the
 // candidate's Cys count (no pair, no disulfide term below two).
+template <int NW>
 __device__ inline double chain_energy(const RefTables& T, const CaSet& g, const uint8_t* aa,
                                       const float* X, const float* Y, const float* Z, uint32_t r,
-                                      uint32_t nr, uint32_t ncys, uint32_t lane, uint2* buf,
-                                      uint32_t bcap) {
+                                      uint32_t nr, uint32_t ncys, uint32_t lane, uint32_t wv,
+                                      uint2* buf, uint32_t bcap, double* red) {
   double part, unused;
-  contrib<false>(T, g, aa, X, Y, Z, r, nr, 0, 0, lane, buf, bcap, part, unused);
-  return wave_sum_f64(part) + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
+  contrib<false>(T, g, aa, X, Y, Z, r, nr, 0, 0, lane, wv * kWave, NW * kWave, buf, bcap, part,
+                 unused);
+  return block_sum<NW>(part, red, lane, wv) + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
 }
 
 // perturb_phi / perturb_psi (src/allatom.rs:392-456) by rad: phi turns the
 // atoms of residue r after its Calpha and every later residue about the N-CA
 // axis through CA; psi turns r's O and every later residue about the CA-C
 // axis through C. Lane-split over the slots; the axis atoms do not move.
-__device__ inline void torsion(const RefTables& T, const uint8_t* aa, float* X, float* Y,
-                               float* Z, uint32_t r, uint32_t nr, float rad, bool phi,
-                               uint32_t lane) {
+__device__ inline void torsion(float* X, float* Y, float* Z, uint32_t r, uint32_t nr, float rad,
+                               bool phi, uint32_t tid, uint32_t nt) {
   const uint32_t b = r * kSlots;
   const geo::V3 p0 = phi ? geo::V3{X[b], Y[b], Z[b]} : geo::V3{X[b + 1], Y[b + 1], Z[b + 1]};
   const geo::V3 p1 = phi ? geo::V3{X[b + 1], Y[b + 1], Z[b + 1]} : geo::V3{X[b + 2], Y[b + 2], Z[b + 2]};
@@ -335,7 +388,7 @@ __device__ inline void torsion(const RefTables& T, const uint8_t* aa, float* X, 
   const float cs = static_cast<float>(cos(static_cast<double>(rad)));
   const float sn = static_cast<float>(sin(static_cast<double>(rad)));
   __syncthreads();
-  for (uint32_t q = b + lane; q < nr * kSlots; q += kWave) {
+  for (uint32_t q = b + tid; q < nr * kSlots; q += nt) {
     const uint32_t rr = q / kSlots, k = q - rr * kSlots;
     if (rr == r && (phi ? k < 2 : k != 3)) continue;  // slots past the atoms turn too, unread
     const geo::V3 o = geo::rodrigues({X[q], Y[q], Z[q]}, p1, axis, cs, sn);
@@ -346,8 +399,8 @@ __device__ inline void torsion(const RefTables& T, const uint8_t* aa, float* X, 
   __syncthreads();
 }
 
-// Scratch of the residue-level force pass, in LDS: a flattened pair list of
-// `cap` entries and the per-residue rows of one chunk of 64 residues.
+// Scratch of one wave's residue-level force pass, in LDS: a flattened pair
+// list of `cap` entries and the per-residue rows of one chunk of 64 residues.
 struct CgScratch {
   float4* CT;    // cap: in-predicate force terms, compacted, in list order
   float4* RQ;    // 64: Calpha x, y, z and the residue-level charge
@@ -373,8 +426,11 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
   return v;
 }
 
-// compute_forces_with_grid (src/energy.rs:322-398) for every residue's
-// Calpha. The neighbour pairs of a chunk of 64 residues (one per lane) form a
+// compute_forces_with_grid (src/energy.rs:322-398) for the Calphas of the
+// residues r_first, r_first + r_step, ... < nr, by one wave with its own
+// scratch (a workgroup's waves take interleaved residues; each residue's force
+// is one wave's work). The neighbour pairs of a chunk of 64 of them (one per
+// lane) form a
 // flat list in the reference's order: residues in order, the 27 cells in the
 // reference's dx, dy, dz order, each cell's points in sorted order. Each lane
 // evaluates list positions lane, lane + 64, ...: the cell a position belongs
@@ -386,10 +442,12 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
 // exceeds it is summed by one lane in place.
 __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const CgCloud& cg,
                                  const uint8_t* aa, const float* X, const float* Y,
-                                 const float* Z, uint32_t nr, uint32_t lane, float* F,
-                                 const CgScratch& s) {
-  for (uint32_t cb = 0; cb < nr; cb += kWave) {
-    const uint32_t nrc = min(nr - cb, static_cast<uint32_t>(kWave));
+                                 const float* Z, uint32_t nr, uint32_t r_first, uint32_t r_step,
+                                 uint32_t lane, float* F, const CgScratch& s) {
+  const uint32_t nmine = nr > r_first ? (nr - r_first + r_step - 1) / r_step : 0;
+  auto res = [&](uint32_t m) { return r_first + m * r_step; };
+  for (uint32_t cb = 0; cb < nmine; cb += kWave) {
+    const uint32_t nrc = min(nmine - cb, static_cast<uint32_t>(kWave));
     const bool act = lane < nrc;
     // this lane's residue: its 27 cell ranges, kept in registers
     uint32_t cbeg[27], ccnt[27];
@@ -397,7 +455,7 @@ __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const
 #pragma unroll
     for (int c = 0; c < 27; ++c) ccnt[c] = cbeg[c] = 0;
     if (act) {
-      const uint32_t r = cb + lane, t = r * kSlots + 1, ty = aa[r];
+      const uint32_t r = res(cb + lane), t = r * kSlots + 1, ty = aa[r];
       const float px = X[t], py = Y[t], pz = Z[t];
       s.RQ[lane] = make_float4(px, py, pz, T.cg_q[ty]);
       s.RP[lane] = make_float4(T.cg_eps[ty], T.cg_sig[ty], static_cast<float>(T.cg_hyd[ty]), 0.0f);
@@ -436,7 +494,7 @@ __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const
       }
     }
     const uint32_t incl = wave_incl_scan(n, lane), excl = incl - n;
-    __syncthreads();  // RQ, RP
+    wave_sync();  // RQ, RP
     for (uint32_t ls = 0; ls < nrc;) {
       // the batch [ls, le): the residues from ls whose lists fit together
       const uint32_t base = __shfl(excl, static_cast<int>(ls));
@@ -444,17 +502,17 @@ __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const
           __popcll(ballot64(lane < ls || (act && incl - base <= s.cap))));
       if (le == ls) {  // one residue over the buffer: its own lane, in place
         if (lane == 0) {
-          const uint32_t r = cb + ls, t = r * kSlots + 1, k = aa[r];
+          const uint32_t r = res(cb + ls), t = r * kSlots + 1, k = aa[r];
           cg_force_point(grid, cg, X[t], Y[t], Z[t], T.cg_q[k], T.cg_eps[k], T.cg_sig[k],
                          T.cg_hyd[k], F[3 * r], F[3 * r + 1], F[3 * r + 2]);
         }
-        __syncthreads();
+        wave_sync();
         ++ls;
         continue;
       }
       const uint32_t tot = __shfl(incl, static_cast<int>(le - 1)) - base;
       for (uint32_t w = lane; w < (tot + 31) / 32; w += kWave) s.HM[w] = 0u;
-      __syncthreads();
+      wave_sync();
       const bool mine = lane >= ls && lane < le;
       if (mine && n) {
         uint32_t k0 = excl - base;
@@ -468,7 +526,7 @@ __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const
             k0 += ccnt[c];
           }
       }
-      __syncthreads();
+      wave_sync();
       uint32_t cnt = 0, carry = 0;
       for (uint32_t kb = 0; kb < tot; kb += kWave) {
         const uint32_t k = kb + lane;
@@ -498,7 +556,7 @@ __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const
         if (live && ((heads >> lane) & 1ull) && (hk & kEntryFirst)) s.CS[l] = pos;
         cnt += static_cast<uint32_t>(__popcll(inb));
       }
-      __syncthreads();
+      wave_sync();
       // each residue's terms [CS[l], next residue's CS) added in list order
       const bool has = mine && n;
       const uint32_t st = has ? s.CS[lane] : 0u;
@@ -514,12 +572,12 @@ __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const
           ay += v.y;
           az += v.z;
         }
-        const uint32_t r = cb + lane;
+        const uint32_t r = res(cb + lane);
         F[3 * r] = ax;
         F[3 * r + 1] = ay;
         F[3 * r + 2] = az;
       }
-      __syncthreads();
+      wave_sync();
       ls = le;
     }
   }
@@ -527,19 +585,24 @@ __device__ inline void cg_forces(const RefTables& T, const GridView& grid, const
 
 // kLds: the antigen Calpha set and grid live in LDS for the whole launch
 // (their addresses then resolve to LDS at compile time; a pointer that may
-// be either would turn every access into a flat one).
-template <bool kLds>
-__global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
+// be either would turn every access into a flat one). NW: waves per
+// candidate. Loops over residues, slots and antigen Calphas are split over
+// the workgroup's threads; the force pass and the interface mask give each
+// wave interleaved residues; the move decisions are scalar code every wave
+// runs alike on the same LDS state, and one thread writes their outcome.
+template <bool kLds, int NW>
+__global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
   extern __shared__ float lds_all[];
+  constexpr uint32_t kThreads = NW * kWave;
   const uint32_t c = blockIdx.x;
-  const uint32_t lane = threadIdx.x;
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   if (c >= a.n_cand) return;
   // the residue tables first (read throughout: NERF, energies), then the rest
   constexpr uint32_t kTabWords = (sizeof(RefTables) + 15) / 16 * 4;
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(a.tab);
     uint32_t* dst = reinterpret_cast<uint32_t*>(lds_all);
-    for (uint32_t k = lane; k < sizeof(RefTables) / 4; k += kWave) dst[k] = src[k];
+    for (uint32_t k = tid; k < sizeof(RefTables) / 4; k += kThreads) dst[k] = src[k];
   }
   const RefTables& T = *reinterpret_cast<const RefTables*>(lds_all);
   float* lds = lds_all + kTabWords;
@@ -552,12 +615,17 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   const uint32_t na_l = kLds ? a.n_ag : 0, nc_l = kLds ? a.ncells_lds : 0;
   float4* gpar = reinterpret_cast<float4*>(lds);  // 16-byte entries first
   float4* U = gpar + na_l;                         // 2 * max_res draws
+  float4* CT = U + 2 * a.max_res;                  // pair_cap terms, a slice per wave
+  float4* RQ = CT + a.pair_cap;                    // 64 rows per wave
+  float4* RP = RQ + NW * kWave;
+  double* red = reinterpret_cast<double*>(RP + NW * kWave);  // 16: the block sums
+  const uint32_t wcap = a.pair_cap / NW;           // each wave's list
   CgScratch cs;
-  cs.cap = a.pair_cap;
-  cs.CT = U + 2 * a.max_res;
-  cs.RQ = cs.CT + a.pair_cap;
-  cs.RP = cs.RQ + kWave;
-  float* gx = reinterpret_cast<float*>(cs.RP + kWave);
+  cs.cap = wcap;
+  cs.CT = CT + wv * wcap;
+  cs.RQ = RQ + wv * kWave;
+  cs.RP = RP + wv * kWave;
+  float* gx = reinterpret_cast<float*>(red + 16);
   float* gy = gx + na_l;
   float* gz = gy + na_l;
   float* gq = gz + na_l;
@@ -569,16 +637,19 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
   float* Y = X + ms;
   float* Z = Y + ms;
   float* F = Z + ms;                 // 3 * max_res forces
-  cs.H = reinterpret_cast<uint32_t*>(F + 3 * a.max_res);
-  cs.HM = cs.H + a.pair_cap;
-  cs.CS = cs.HM + a.pair_cap / 32;
-  uint8_t* gh = reinterpret_cast<uint8_t*>(cs.CS + kWave);
+  uint32_t* H = reinterpret_cast<uint32_t*>(F + 3 * a.max_res);
+  uint32_t* HM = H + a.pair_cap;
+  uint32_t* CS = HM + a.pair_cap / 32;
+  cs.H = H + wv * wcap;
+  cs.HM = HM + wv * (wcap / 32);
+  cs.CS = CS + wv * kWave;
+  uint8_t* gh = reinterpret_cast<uint8_t*>(CS + NW * kWave);
   uint8_t* aa = gh + na_l;
   uint8_t* rot = aa + a.max_res;
   uint8_t* iface = rot + a.max_res;
   uint8_t* pend = iface + a.max_res;  // deferred build of each residue
   if (kLds) {
-    for (uint32_t i = lane; i < a.n_ag; i += kWave) {
+    for (uint32_t i = tid; i < a.n_ag; i += kThreads) {
       gpar[i] = a.ag_par[i];
       gx[i] = a.ag_x[i];
       gy[i] = a.ag_y[i];
@@ -589,7 +660,7 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
       gh[i] = a.ag_cg.hyd[i];
       gsi[i] = a.ag_grid.sorted_idx[i];
     }
-    for (uint32_t k = lane; k < nc_l; k += kWave) gcs[k] = a.ag_grid.cell_start[k];
+    for (uint32_t k = tid; k < nc_l; k += kThreads) gcs[k] = a.ag_grid.cell_start[k];
   }
   const CaSet g = kLds ? CaSet{gx, gy, gz, gpar, a.n_ag} : CaSet{a.ag_x, a.ag_y, a.ag_z, a.ag_par, a.n_ag};
   const CgCloud cg = kLds ? CgCloud{gx, gy, gz, gq, ge, gs, gh} : a.ag_cg;
@@ -598,23 +669,24 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
     grid.cell_start = gcs;
     grid.sorted_idx = gsi;
   }
-  for (uint32_t t = lane; t < nr * kSlots; t += kWave) {
+  for (uint32_t t = tid; t < nr * kSlots; t += kThreads) {
     const float4 v = a.slots[static_cast<size_t>(g0) * kSlots + t];
     X[t] = v.x;
     Y[t] = v.y;
     Z[t] = v.z;
   }
-  for (uint32_t r = lane; r < nr; r += kWave) {
+  for (uint32_t r = tid; r < nr; r += kThreads) {
     aa[r] = a.res_aa[g0 + r];
     rot[r] = a.res_rot[g0 + r];
     pend[r] = kBuildNone;
   }
   __syncthreads();
 
-  // the pair list of the filtered contributions reuses the force pass's
-  // term list (free outside it); the candidate's Cys count (uniform)
+  // each wave's pair list of the filtered contributions reuses its slice of
+  // the force pass's term list (free outside it); the candidate's Cys count
+  // (uniform)
   uint2* lbuf = reinterpret_cast<uint2*>(cs.CT);
-  const uint32_t lcap = 2 * a.pair_cap;
+  const uint32_t lcap = 2 * wcap;
   uint32_t ncys = 0;
   for (uint32_t cb = 0; cb < nr; cb += kWave)
     ncys += static_cast<uint32_t>(__popcll(ballot64(cb + lane < nr && aa[cb + lane] == kCys)));
@@ -624,8 +696,9 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
     const float temp = a.temp[s];
     // 1. the gradient step (src/diffusion.rs:305-339): residue-level forces
     //    on the Calphas (compute_forces_with_grid)
-    cg_forces(T, grid, cg, aa, X, Y, Z, nr, lane, F, cs);
-    for (uint32_t r = lane; r < nr; r += kWave) {
+    cg_forces(T, grid, cg, aa, X, Y, Z, nr, wv, NW, lane, F, cs);
+    __syncthreads();
+    for (uint32_t r = tid; r < nr; r += kThreads) {
       const uint32_t t = r * kSlots + 1;
       const float4 d = residue_delta(a.cx, a.cy, a.cz, a.sp, a.r0,
                                      make_float4(X[t], Y[t], Z[t], 0.0f), F[3 * r], F[3 * r + 1],
@@ -637,22 +710,23 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
     __syncthreads();
     // set_ca_pos: every slot moves by its residue's displacement (slots past
     // the residue's atoms are never read before a rebuild writes them)
-    for (uint32_t q = lane; q < nr * kSlots; q += kWave) {
+    for (uint32_t q = tid; q < nr * kSlots; q += kThreads) {
       const uint32_t r = q / kSlots;
       X[q] += F[3 * r];
       Y[q] += F[3 * r + 1];
       Z[q] += F[3 * r + 2];
     }
     __syncthreads();
-    // 2. interface mask (src/diffusion.rs:341-351): per chunk of 64
-    //    residues (Calphas in lanes, broadcast by readlane), each lane tests
-    //    its antigen Calphas against every residue; the wave's compare masks
-    //    set the residue's bit
-    for (uint32_t cb = 0; cb < nr; cb += kWave) {
-      const uint32_t nrc = min(nr - cb, static_cast<uint32_t>(kWave));
+    // 2. interface mask (src/diffusion.rs:341-351): each wave takes the
+    //    residues wv, wv + NW, ...; per chunk of 64 of them (Calphas in
+    //    lanes, broadcast by readlane) each lane tests its antigen Calphas
+    //    against every residue; the wave's compare masks set the residue bits
+    const uint32_t nmine = nr > wv ? (nr - wv + NW - 1) / NW : 0;
+    for (uint32_t cb = 0; cb < nmine; cb += kWave) {
+      const uint32_t nrc = min(nmine - cb, static_cast<uint32_t>(kWave));
       float px = 0.0f, py = 0.0f, pz = 0.0f;
       if (lane < nrc) {
-        const uint32_t t = (cb + lane) * kSlots + 1;
+        const uint32_t t = (wv + (cb + lane) * NW) * kSlots + 1;
         px = X[t];
         py = Y[t];
         pz = Z[t];
@@ -673,11 +747,11 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
           }
         }
       }
-      if (lane < nrc) iface[cb + lane] = static_cast<uint8_t>((m >> lane) & 1ull);
+      if (lane < nrc) iface[wv + (cb + lane) * NW] = static_cast<uint8_t>((m >> lane) & 1ull);
     }
     __syncthreads();
     // this step's draws of both move kinds, one residue per lane
-    for (uint32_t r = lane; r < nr; r += kWave) {
+    for (uint32_t r = tid; r < nr; r += kThreads) {
       U[2 * r] = u4(it, g0 + r, kTagMc, a.sp);
       U[2 * r + 1] = u4(it, g0 + r, kTagBb, a.sp);
     }
@@ -716,10 +790,9 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
           bool accept = true;
           if (!rot_move) {
             double e_old, e_new;
-            contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, lbuf, lcap, e_old,
-                          e_new);
-            e_old = wave_sum_f64(e_old);
-            e_new = wave_sum_f64(e_new);
+            contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, wv * kWave,
+                          kThreads, lbuf, lcap, e_old, e_new);
+            block_sum2<NW>(e_old, e_new, red, lane, wv);
             const double ds = ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0;
             const double delta = (e_new + ds) - (e_old + ds);
             accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
@@ -727,7 +800,7 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
           // The build itself waits for the end of the phase: until then
           // only this residue's Calpha (unchanged by any build) and type are
           // read, and no other move touches it this step.
-          if (lane == 0) {
+          if (tid == 0) {
             if (rot_move) {  // apply_rotamer: the new chi on the current backbone
               rot[r] = static_cast<uint8_t>(new_rot);
               pend[r] = kBuildSide;
@@ -743,10 +816,10 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
           continue;
         }
         // a Cys gains, loses or moves its SG: score the rebuilt residue
-        const double old_e = contribution(T, g, old_aa, X[t], Y[t], Z[t], lane) +
+        const double old_e = contribution<NW>(T, g, old_aa, X[t], Y[t], Z[t], lane, wv, red) +
                              (ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
         __syncthreads();
-        if (lane == 0) {
+        if (tid == 0) {
           if (rot_move) {  // apply_rotamer
             rot[r] = static_cast<uint8_t>(new_rot);
             side_chain(T, old_aa, new_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
@@ -757,14 +830,15 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
         }
         __syncthreads();
         const uint32_t ncys_new = ncys + (uniform_u32(aa[r]) == kCys) - (old_aa == kCys);
-        const double new_e = contribution(T, g, aa[r], X[t], Y[t], Z[t], lane) +
-                             (ncys_new >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
+        const double new_e =
+            contribution<NW>(T, g, uniform_u32(aa[r]), X[t], Y[t], Z[t], lane, wv, red) +
+            (ncys_new >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
         const double delta = new_e - old_e;
         const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
         __syncthreads();
         if (accept) {
           ncys = ncys_new;
-        } else if (lane == 0) {
+        } else if (tid == 0) {
           if (rot_move) {  // the old chi, rebuilt on the current backbone
             rot[r] = static_cast<uint8_t>(old_rot);
             side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
@@ -776,8 +850,8 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
         __syncthreads();
       }
     }
-    // the side chains the moves above decided, built lane-parallel
-    for (uint32_t r = lane; r < nr; r += kWave) {
+    // the side chains the moves above decided, built in parallel
+    for (uint32_t r = tid; r < nr; r += kThreads) {
       const uint8_t b = pend[r];
       if (b == kBuildSide) {
         side_chain(T, aa[r], rot[r], aa[r] == kGly ? 4 : 5, X + r * kSlots, Y + r * kSlots,
@@ -797,18 +871,20 @@ __global__ void __launch_bounds__(64) k_refine(RefineArgs a) {
         const float deg = -a.max_torsion + (a.max_torsion + a.max_torsion) * u.y;
         const float rad = geo::deg2rad(deg);
         const bool phi = u.z < 0.5f;
-        const double old_e = chain_energy(T, g, aa, X, Y, Z, r, nr, ncys, lane, lbuf, lcap);
-        torsion(T, aa, X, Y, Z, r, nr, rad, phi, lane);
-        const double new_e = chain_energy(T, g, aa, X, Y, Z, r, nr, ncys, lane, lbuf, lcap);
+        const double old_e =
+            chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red);
+        torsion(X, Y, Z, r, nr, rad, phi, tid, kThreads);
+        const double new_e =
+            chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red);
         const double delta = new_e - old_e;
         const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
-        if (!accept) torsion(T, aa, X, Y, Z, r, nr, -rad, phi, lane);  // rotate back
+        if (!accept) torsion(X, Y, Z, r, nr, -rad, phi, tid, kThreads);  // rotate back
       }
     }
   }
-  for (uint32_t t = lane; t < nr * kSlots; t += kWave)
+  for (uint32_t t = tid; t < nr * kSlots; t += kThreads)
     a.slots[static_cast<size_t>(g0) * kSlots + t] = make_float4(X[t], Y[t], Z[t], 0.0f);
-  for (uint32_t r = lane; r < nr; r += kWave) {
+  for (uint32_t r = tid; r < nr; r += kThreads) {
     a.res_aa[g0 + r] = aa[r];
     a.res_rot[g0 + r] = rot[r];
   }
@@ -875,6 +951,7 @@ int dnab_mc_default_params(dnab_mc_params* p) {
   p->max_torsion_deg = 15.0f;  // src/diffusion.rs:47
   p->iface_cutoff_sq = 64.0f;  // src/diffusion.rs:51
   p->moves = DNAB_MC_SIDECHAIN | DNAB_MC_BACKBONE;
+  p->waves = 0;
   return DNAB_OK;
 }
 
@@ -1036,26 +1113,31 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   DNAB_HIP_CHECK(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                             ctx->device));
   const size_t lds_cu = static_cast<size_t>(lds_max);
-  const size_t fixed = (sizeof(RefTables) + 15) / 16 * 16 + 2 * kWave * sizeof(float4) +
-                       kWave * sizeof(uint32_t) +
+  if (mc->waves != 0 && mc->waves != 1 && mc->waves != 4)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: waves must be 0, 1 or 4");
+  const size_t cu = ctx->cu_count > 0 ? static_cast<size_t>(ctx->cu_count) : 256;
+  // small batches leave CUs idle: four waves per candidate split its work
+  const uint32_t nw = mc->waves ? mc->waves : (nc <= 2 * cu ? 4u : 1u);
+  const size_t fixed = (sizeof(RefTables) + 15) / 16 * 16 +
+                       (2 * kWave * nw + 8) * sizeof(float4) + kWave * nw * sizeof(uint32_t) +
                        static_cast<size_t>(max_res) * (2 * 16 + (3 * kSlots + 3) * sizeof(float) + 4);
   const size_t ncells = ag->ca_grid.n ? ag->ca_grid.ncodes + 1 : 0;
   const size_t ag_bytes = n_ag * (16 + 6 * 4 + 4 + 1) + ncells * 4;
   auto list_bytes = [](size_t cap) { return cap * (sizeof(float4) + sizeof(uint32_t)) + cap / 8; };
   if (n_ag >= (size_t(1) << 24))
     return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: more than 2^24 antigen Calphas");
-  const size_t cu = ctx->cu_count > 0 ? static_cast<size_t>(ctx->cu_count) : 256;
   size_t cap = 0;
   bool ag_in = false;
   for (size_t bpc = std::min<size_t>(4, (nc + cu - 1) / cu); bpc >= 1; --bpc) {
     const size_t budget = lds_cu / bpc;
     ag_in = ncells && fixed + ag_bytes + list_bytes(1024) <= budget;
     const size_t used = fixed + (ag_in ? ag_bytes : 0);
-    cap = used < budget ? (budget - used) * 8 / 161 / 64 * 64 : 0;
+    const size_t unit = 64 * nw;  // each wave's list a multiple of 64
+    cap = used < budget ? (budget - used) * 8 / 161 / unit * unit : 0;
     cap = std::min<size_t>(cap, 8192);
     if (cap >= 1024) break;
   }
-  if (cap < 128)  // the filtered contributions stage up to 4 x 64 pairs in 2 * cap entries
+  if (cap < 128 * nw)  // a wave's filtered contributions stage up to 4 x 64 pairs in 2 x its list
     return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: candidates too long for LDS");
   a.pair_cap = static_cast<uint32_t>(cap);
   size_t lds = fixed + list_bytes(cap);
@@ -1064,19 +1146,25 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
     a.ncells_lds = static_cast<uint32_t>(ncells);
     lds += ag_bytes;
   }
-  const void* kfn = a.n_ag_lds ? reinterpret_cast<const void*>(&k_refine<true>)
-                              : reinterpret_cast<const void*>(&k_refine<false>);
+  const void* kfn =
+      a.n_ag_lds ? (nw == 4 ? reinterpret_cast<const void*>(&k_refine<true, 4>)
+                            : reinterpret_cast<const void*>(&k_refine<true, 1>))
+                 : (nw == 4 ? reinterpret_cast<const void*>(&k_refine<false, 4>)
+                            : reinterpret_cast<const void*>(&k_refine<false, 1>));
   if (lds > 65536)
     DNAB_HIP_CHECK(ctx, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             static_cast<int>(lds)));
   hipEvent_t ev_end = nullptr;
   hipEvent_t ev = timer_begin(ctx, "refine", &ev_end);
-  if (a.n_ag_lds)
-    hipLaunchKernelGGL(k_refine<true>, dim3(static_cast<uint32_t>(nc)), dim3(kWave), lds,
-                       ctx->stream, a);
+  const dim3 grid_dim(static_cast<uint32_t>(nc)), block_dim(kWave * nw);
+  if (a.n_ag_lds && nw == 4)
+    hipLaunchKernelGGL((k_refine<true, 4>), grid_dim, block_dim, lds, ctx->stream, a);
+  else if (a.n_ag_lds)
+    hipLaunchKernelGGL((k_refine<true, 1>), grid_dim, block_dim, lds, ctx->stream, a);
+  else if (nw == 4)
+    hipLaunchKernelGGL((k_refine<false, 4>), grid_dim, block_dim, lds, ctx->stream, a);
   else
-    hipLaunchKernelGGL(k_refine<false>, dim3(static_cast<uint32_t>(nc)), dim3(kWave), lds,
-                       ctx->stream, a);
+    hipLaunchKernelGGL((k_refine<false, 1>), grid_dim, block_dim, lds, ctx->stream, a);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   timer_end(ctx, "refine", ev, ev_end);
   DNAB_RETURN_IF(download(ctx, slots.data(), d_slots.p, slots.size() * sizeof(float4)));

@@ -114,7 +114,7 @@ class CandidateReportC(C.Structure):
 class McParamsC(C.Structure):
     _fields_ = [("mutation_p", C.c_float), ("rotamer_move_p", C.c_float),
                 ("backbone_move_p", C.c_float), ("max_torsion_deg", C.c_float),
-                ("iface_cutoff_sq", C.c_float), ("moves", C.c_uint32)]
+                ("iface_cutoff_sq", C.c_float), ("moves", C.c_uint32), ("waves", C.c_uint32)]
 
 
 class LangevinParamsC(C.Structure):
