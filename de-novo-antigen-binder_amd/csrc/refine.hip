@@ -265,6 +265,24 @@ __device__ inline double pair_terms(const RefTables& T, const CaSet& g, uint32_t
   return e;
 }
 
+// residue_contribution_allatom of a Calpha at p as type ta and as type tb in
+// one pass, FP64 totals on every thread.
+template <int NW>
+__device__ inline void contribution2(const RefTables& T, const CaSet& g, uint32_t ta, uint32_t tb,
+                                     float px, float py, float pz, uint32_t lane, uint32_t wv,
+                                     double* red, double& ea, double& eb) {
+  ea = eb = 0.0;
+  for (uint32_t i = wv * kWave + lane; i < g.n; i += NW * kWave) {
+    const float dx = px - g.x[i], dy = py - g.y[i], dz = pz - g.z[i];
+    const float r_sq = dx * dx + dy * dy + dz * dz;
+    if (r_sq > 100.0f || r_sq < 0.25f) continue;
+    const float4 q = g.par[i];
+    ea += pair_term(T, ta, q, r_sq);
+    eb += pair_term(T, tb, q, r_sq);
+  }
+  block_sum2<NW>(ea, eb, red, lane, wv);
+}
+
 // residue_contribution_allatom, FP64 total on every thread.
 template <int NW>
 __device__ inline double contribution(const RefTables& T, const CaSet& g, uint32_t aa, float px,
@@ -301,14 +319,10 @@ __device__ inline void side_chain(const RefTables& T, uint32_t aa, uint32_t rot,
 }
 
 // A residue of type new_aa, rotamer new_rot, built anew on an ideal backbone
-// around its Calpha (the other-type branch of mutate_residue,
-// src/allatom.rs:316-384).
-__device__ inline void rebuild_ideal(const RefTables& T, uint8_t* aa, uint8_t* rot, uint32_t r,
-                                     uint32_t new_aa, uint32_t new_rot, float* X, float* Y,
-                                     float* Z) {
-  float* x = X + r * kSlots;
-  float* y = Y + r * kSlots;
-  float* z = Z + r * kSlots;
+// around the Calpha in slot 1 of x/y/z (the other-type branch of
+// mutate_residue, src/allatom.rs:316-384).
+__device__ inline void build_ideal(const RefTables& T, uint32_t new_aa, uint32_t new_rot, float* x,
+                                   float* y, float* z) {
   geo::V3 bb[5];
   int nb = geo::ideal_backbone(x[1], y[1], z[1], new_aa == kGly, bb);
   if (nb > T.natoms[new_aa]) nb = T.natoms[new_aa];
@@ -317,9 +331,16 @@ __device__ inline void rebuild_ideal(const RefTables& T, uint8_t* aa, uint8_t* r
     y[k] = bb[k].y;
     z[k] = bb[k].z;
   }
+  side_chain(T, new_aa, new_rot, nb, x, y, z);
+}
+
+// build_ideal on residue r's slots, which then has type new_aa.
+__device__ inline void rebuild_ideal(const RefTables& T, uint8_t* aa, uint8_t* rot, uint32_t r,
+                                     uint32_t new_aa, uint32_t new_rot, float* X, float* Y,
+                                     float* Z) {
   aa[r] = static_cast<uint8_t>(new_aa);
   rot[r] = static_cast<uint8_t>(new_rot);
-  side_chain(T, new_aa, new_rot, nb, x, y, z);
+  build_ideal(T, new_aa, new_rot, X + r * kSlots, Y + r * kSlots, Z + r * kSlots);
 }
 
 // mutate_residue (src/allatom.rs:299-384) on one residue's slots: the same
@@ -338,17 +359,29 @@ __device__ inline void mutate(const RefTables& T, uint8_t* aa, uint8_t* rot, uin
   rebuild_ideal(T, aa, rot, r, new_aa, new_rot, X, Y, Z);
 }
 
-// disulfide_energy over the Cys SG slots (slot 5), pairs in residue order,
-// the pair set by the f32 distance and each well in FP64.
+// disulfide_energy (src/allatom.rs:155-180) over the Cys SG slots (slot 5):
+// the pair set by the f32 distance and each well in FP64, as there. The
+// wave lists its Cys residues in order (cys: one entry per residue of LDS,
+// written alike by every wave), then lane 8a + b takes the pair (8 ta + a,
+// 8 tb + b) of each 8 x 8 tile; the wells are summed by DPP.
 __device__ inline double disulfide(const uint8_t* aa, const float* X, const float* Y,
-                                   const float* Z, uint32_t nr) {
+                                   const float* Z, uint32_t nr, uint32_t lane, uint16_t* cys) {
+  uint32_t k = 0;
+  for (uint32_t cb = 0; cb < nr; cb += kWave) {
+    const bool is = cb + lane < nr && aa[cb + lane] == kCys;
+    const uint64_t m = ballot64(is);
+    if (is) cys[k + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)))] =
+        static_cast<uint16_t>(cb + lane);
+    k += static_cast<uint32_t>(__popcll(m));
+  }
+  wave_sync();
   double e = 0.0;
-  for (uint32_t i = 0; i < nr; ++i) {
-    if (aa[i] != kCys) continue;
-    const uint32_t si = i * kSlots + kCysSgSlot;
-    for (uint32_t j = i + 1; j < nr; ++j) {
-      if (aa[j] != kCys) continue;
-      const uint32_t sj = j * kSlots + kCysSgSlot;
+  const uint32_t nt = (k + 7) / 8, a = lane >> 3, b = lane & 7;
+  for (uint32_t ta = 0; ta < nt; ++ta)
+    for (uint32_t tb = ta; tb < nt; ++tb) {
+      const uint32_t pi = ta * 8 + a, pj = tb * 8 + b;
+      if (!(pi < pj && pj < k)) continue;
+      const uint32_t si = cys[pi] * kSlots + kCysSgSlot, sj = cys[pj] * kSlots + kCysSgSlot;
       const float dx = X[si] - X[sj], dy = Y[si] - Y[sj], dz = Z[si] - Z[sj];
       if (sqrtf(dx * dx + dy * dy + dz * dz) > kSsCutoff) continue;
       const double ex = static_cast<double>(X[si]) - X[sj], ey = static_cast<double>(Y[si]) - Y[sj],
@@ -357,8 +390,8 @@ __device__ inline double disulfide(const uint8_t* aa, const float* X, const floa
                        static_cast<double>(kSsSigma);
       e -= static_cast<double>(kSsDepth) * exp(-0.5 * t * t);
     }
-  }
-  return e;
+  wave_sync();
+  return wave_sum_f64(e);
 }
 
 // Sum of residue_contribution_allatom over residues r..nr-1 plus the
@@ -368,11 +401,11 @@ template <int NW>
 __device__ inline double chain_energy(const RefTables& T, const CaSet& g, const uint8_t* aa,
                                       const float* X, const float* Y, const float* Z, uint32_t r,
                                       uint32_t nr, uint32_t ncys, uint32_t lane, uint32_t wv,
-                                      uint2* buf, uint32_t bcap, double* red) {
+                                      uint2* buf, uint32_t bcap, double* red, uint16_t* cysl) {
   double part, unused;
   contrib<false>(T, g, aa, X, Y, Z, r, nr, 0, 0, lane, wv * kWave, NW * kWave, buf, bcap, part,
                  unused);
-  return block_sum<NW>(part, red, lane, wv) + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
+  return block_sum<NW>(part, red, lane, wv) + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0);
 }
 
 // perturb_phi / perturb_psi (src/allatom.rs:392-456) by rad: phi turns the
@@ -625,6 +658,13 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
   cs.CT = CT + wv * wcap;
   cs.RQ = RQ + wv * kWave;
   cs.RP = RP + wv * kWave;
+  // outside the force pass: the undo build's slots (in RQ) and the Cys list
+  // (in RP); the undo is built by another wave's thread when there is one
+  float* SX = reinterpret_cast<float*>(RQ);
+  float* SY = SX + kSlots;
+  float* SZ = SY + kSlots;
+  uint16_t* cysl = reinterpret_cast<uint16_t*>(RP);
+  constexpr uint32_t kUndoThread = NW > 1 ? kWave : 1;
   float* gx = reinterpret_cast<float*>(red + 16);
   float* gy = gx + na_l;
   float* gz = gy + na_l;
@@ -793,7 +833,7 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
             contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, wv * kWave,
                           kThreads, lbuf, lcap, e_old, e_new);
             block_sum2<NW>(e_old, e_new, red, lane, wv);
-            const double ds = ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0;
+            const double ds = ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0;
             const double delta = (e_new + ds) - (e_old + ds);
             accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
           }
@@ -815,9 +855,19 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
           __syncthreads();
           continue;
         }
-        // a Cys gains, loses or moves its SG: score the rebuilt residue
-        const double old_e = contribution<NW>(T, g, old_aa, X[t], Y[t], Z[t], lane, wv, red) +
-                             (ncys >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
+        // A Cys gains, loses or moves its SG: the disulfide term needs the
+        // built residue. The Calpha does not move, so both types' energies
+        // come from one pass first; thread 0 builds the move while another
+        // wave's thread builds the reference's undo into scratch (SX/SY/SZ),
+        // from a copy of the residue taken before either starts.
+        double c_old, c_new;
+        contribution2<NW>(T, g, old_aa, new_aa, X[t], Y[t], Z[t], lane, wv, red, c_old, c_new);
+        const double old_e = c_old + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0);
+        for (uint32_t q = tid; q < kSlots; q += kThreads) {
+          SX[q] = X[r * kSlots + q];
+          SY[q] = Y[r * kSlots + q];
+          SZ[q] = Z[r * kSlots + q];
+        }
         __syncthreads();
         if (tid == 0) {
           if (rot_move) {  // apply_rotamer
@@ -828,23 +878,30 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
             mutate(T, aa, rot, r, new_aa, new_rot, X, Y, Z);
           }
         }
+        if (tid == kUndoThread) {
+          if (rot_move || new_aa == old_aa)  // the old chi on the same backbone
+            side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, SX, SY, SZ);
+          else  // mutate_residue back: the old type on an ideal backbone
+            build_ideal(T, old_aa, old_rot, SX, SY, SZ);
+        }
         __syncthreads();
         const uint32_t ncys_new = ncys + (uniform_u32(aa[r]) == kCys) - (old_aa == kCys);
         const double new_e =
-            contribution<NW>(T, g, uniform_u32(aa[r]), X[t], Y[t], Z[t], lane, wv, red) +
-            (ncys_new >= 2 ? disulfide(aa, X, Y, Z, nr) : 0.0);
+            c_new + (ncys_new >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0);
         const double delta = new_e - old_e;
         const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
         __syncthreads();
         if (accept) {
           ncys = ncys_new;
-        } else if (tid == 0) {
-          if (rot_move) {  // the old chi, rebuilt on the current backbone
+        } else {
+          for (uint32_t q = tid; q < kSlots; q += kThreads) {
+            X[r * kSlots + q] = SX[q];
+            Y[r * kSlots + q] = SY[q];
+            Z[r * kSlots + q] = SZ[q];
+          }
+          if (tid == 0) {
+            aa[r] = static_cast<uint8_t>(old_aa);
             rot[r] = static_cast<uint8_t>(old_rot);
-            side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
-                       Y + r * kSlots, Z + r * kSlots);
-          } else {
-            mutate(T, aa, rot, r, old_aa, old_rot, X, Y, Z);
           }
         }
         __syncthreads();
@@ -872,10 +929,10 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
         const float rad = geo::deg2rad(deg);
         const bool phi = u.z < 0.5f;
         const double old_e =
-            chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red);
+            chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red, cysl);
         torsion(X, Y, Z, r, nr, rad, phi, tid, kThreads);
         const double new_e =
-            chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red);
+            chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red, cysl);
         const double delta = new_e - old_e;
         const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
         if (!accept) torsion(X, Y, Z, r, nr, -rad, phi, tid, kThreads);  // rotate back
