@@ -1170,11 +1170,11 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   DNAB_HIP_CHECK(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                             ctx->device));
   const size_t lds_cu = static_cast<size_t>(lds_max);
-  if (mc->waves != 0 && mc->waves != 1 && mc->waves != 4)
-    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: waves must be 0, 1 or 4");
+  if (mc->waves != 0 && mc->waves != 1 && mc->waves != 4 && mc->waves != 8)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: waves must be 0, 1, 4 or 8");
   const size_t cu = ctx->cu_count > 0 ? static_cast<size_t>(ctx->cu_count) : 256;
-  // small batches leave CUs idle: four waves per candidate split its work
-  const uint32_t nw = mc->waves ? mc->waves : (nc <= 2 * cu ? 4u : 1u);
+  // small batches leave CUs idle: eight or four waves per candidate split its work
+  const uint32_t nw = mc->waves ? mc->waves : (nc <= cu ? 8u : nc <= 2 * cu ? 4u : 1u);
   const size_t fixed = (sizeof(RefTables) + 15) / 16 * 16 +
                        (2 * kWave * nw + 8) * sizeof(float4) + kWave * nw * sizeof(uint32_t) +
                        static_cast<size_t>(max_res) * (2 * 16 + (3 * kSlots + 3) * sizeof(float) + 4);
@@ -1203,21 +1203,29 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
     a.ncells_lds = static_cast<uint32_t>(ncells);
     lds += ag_bytes;
   }
-  const void* kfn =
-      a.n_ag_lds ? (nw == 4 ? reinterpret_cast<const void*>(&k_refine<true, 4>)
-                            : reinterpret_cast<const void*>(&k_refine<true, 1>))
-                 : (nw == 4 ? reinterpret_cast<const void*>(&k_refine<false, 4>)
-                            : reinterpret_cast<const void*>(&k_refine<false, 1>));
+  const void* kfn = nullptr;
+  if (a.n_ag_lds)
+    kfn = nw == 8 ? reinterpret_cast<const void*>(&k_refine<true, 8>)
+        : nw == 4 ? reinterpret_cast<const void*>(&k_refine<true, 4>)
+                  : reinterpret_cast<const void*>(&k_refine<true, 1>);
+  else
+    kfn = nw == 8 ? reinterpret_cast<const void*>(&k_refine<false, 8>)
+        : nw == 4 ? reinterpret_cast<const void*>(&k_refine<false, 4>)
+                  : reinterpret_cast<const void*>(&k_refine<false, 1>);
   if (lds > 65536)
     DNAB_HIP_CHECK(ctx, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             static_cast<int>(lds)));
   hipEvent_t ev_end = nullptr;
   hipEvent_t ev = timer_begin(ctx, "refine", &ev_end);
   const dim3 grid_dim(static_cast<uint32_t>(nc)), block_dim(kWave * nw);
-  if (a.n_ag_lds && nw == 4)
+  if (a.n_ag_lds && nw == 8)
+    hipLaunchKernelGGL((k_refine<true, 8>), grid_dim, block_dim, lds, ctx->stream, a);
+  else if (a.n_ag_lds && nw == 4)
     hipLaunchKernelGGL((k_refine<true, 4>), grid_dim, block_dim, lds, ctx->stream, a);
   else if (a.n_ag_lds)
     hipLaunchKernelGGL((k_refine<true, 1>), grid_dim, block_dim, lds, ctx->stream, a);
+  else if (nw == 8)
+    hipLaunchKernelGGL((k_refine<false, 8>), grid_dim, block_dim, lds, ctx->stream, a);
   else if (nw == 4)
     hipLaunchKernelGGL((k_refine<false, 4>), grid_dim, block_dim, lds, ctx->stream, a);
   else

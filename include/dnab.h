@@ -484,9 +484,9 @@ typedef struct dnab_mc_params {
   float max_torsion_deg; /* 15 (src/diffusion.rs:47) */
   float iface_cutoff_sq; /* 64, an 8 A Calpha-Calpha interface (src/diffusion.rs:51) */
   uint32_t moves;        /* DNAB_MC_SIDECHAIN | DNAB_MC_BACKBONE (the reference) */
-  uint32_t waves;        /* waves per candidate: 0 = chosen by batch size (4 for batches up
-                            to twice the CU count, else 1), or 1 / 4; no effect on results
-                            beyond the order of FP64 energy sums */
+  uint32_t waves;        /* waves per candidate: 0 = chosen by batch size (8 for batches up
+                            to the CU count, 4 up to twice it, else 1), or 1 / 4 / 8; no
+                            effect on results beyond the order of FP64 energy sums */
 } dnab_mc_params;
 int dnab_mc_default_params(dnab_mc_params* p);
 /* Every iteration of allatom_diffusion_step (src/diffusion.rs:290-416) for a
