@@ -45,14 +45,15 @@ def _rebuild(aa, rot, xyz):
     xyz[:] = buf.reshape(-1, 3)
 
 
-def _workload(n_cand=4, n_res=10, seed=3):
-    ag = W.globule_antigen(60, W.globule_radius(60), 900 + seed)
+def _workload(n_cand=4, n_res=10, seed=3, n_ag=60, inside=False):
+    ag = W.globule_antigen(n_ag, W.globule_radius(n_ag), 900 + seed)
     ctr = ag.ca_center_of_mass()
-    r = W.globule_radius(60)
+    r = W.globule_radius(n_ag)
     rng = np.random.default_rng(seed)
     abs_, rots = [], []
     for c in range(n_cand):
-        sb = W.shell_binder(n_res, ctr, r + 6.0, r, 70 + seed * 100 + c)
+        shell, rmin = (0.5 * r, 0.0) if inside else (r + 6.0, r)
+        sb = W.shell_binder(n_res, ctr, shell, rmin, 70 + seed * 100 + c)
         ca = sb.ca_pos()
         aa = sb.amino_acid
         rot = np.where((aa == 0) | (aa == 7), 0, rng.integers(0, 5, n_res)).astype(np.uint8)
@@ -120,6 +121,23 @@ def test_refine_matches_the_oracle(ctx, noise, moves, waves):
     print(f"atoms bit-identical {n_exact}/{n_atoms}, mutations kept {n_mut}")
     if moves & 1:
         assert n_mut > 0
+
+
+@pytest.mark.parametrize("waves", [1, 8])
+def test_refine_long_binder_matches_the_oracle(ctx, waves):
+    """A 150-residue binder threaded through a 300-residue antigen: more than
+    64 residues per wave (chunked force pass and interface mask at one wave)
+    and 27-cell pair lists longer than a wave's list (batches of residues)."""
+    ag, abs_, rots = _workload(n_cand=1, n_res=150, seed=11, n_ag=300, inside=True)
+    agd = DeviceAntigen(ctx, ag, cutoff=10.0)
+    lp = langevin_params(n_steps=3, with_noise=1, schedule_offset=20, schedule_total=800, seed=99)
+    mc = mc_params(mutation_p=0.2, rotamer_move_p=0.3, backbone_move_p=0.01, waves=waves)
+    prots, rot_out, _ = ctx.refine(agd, abs_, lp, mc, rots)
+    st = _oracle_run(ag, abs_[0], rots[0], 0, lp, mc)
+    assert list(prots[0].amino_acid) == st["aa"] and list(rot_out[0]) == st["rot"]
+    want = np.concatenate(st["res"]).astype(np.float64)
+    got = np.stack([prots[0].atoms.x, prots[0].atoms.y, prots[0].atoms.z], 1).astype(np.float64)
+    assert got.shape == want.shape and np.abs(got - want).max() <= 1e-4
 
 
 def test_refine_literal_moves_match_the_oracle(ctx):
