@@ -797,114 +797,122 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
     }
     __syncthreads();
     // 3. side-chain moves (src/diffusion.rs:353-398)
+    //    (the residues whose first draw asks for a move, from a ballot per 64,
+    //    visited in order)
     if (a.moves & DNAB_MC_SIDECHAIN) {
-      for (uint32_t r = 0; r < nr; ++r) {
-        const float4 uu = U[2 * r];
-        const float4 u = make_float4(uniform_f32(uu.x), uniform_f32(uu.y), uniform_f32(uu.z),
-                                     uniform_f32(uu.w));
-        const bool in = uniform_u32(iface[r]) != 0;
-        const float eff_mut = in ? a.mut_p * 3.0f : a.mut_p * 0.4f;
-        const float eff_rot = in ? a.rot_p * 2.0f : a.rot_p;
-        if (!(u.x < eff_mut)) continue;
-        const uint32_t t = r * kSlots + 1;
-        const uint32_t old_aa = uniform_u32(aa[r]), old_rot = uniform_u32(rot[r]);
-        const bool rot_move = u.y < eff_rot;
-        uint32_t new_aa = old_aa, new_rot;
-        if (rot_move) {  // rotamer move
-          const uint32_t nrot = uniform_u32(T.nrot[old_aa]);
-          if (nrot == 0) continue;
-          new_rot = min(static_cast<uint32_t>(u.z * static_cast<float>(nrot)), nrot - 1);
-        } else {  // mutation: the new type's first rotamer
-          new_aa = min(static_cast<uint32_t>(u.z * static_cast<float>(kNumAminoAcids)),
-                       static_cast<uint32_t>(kNumAminoAcids - 1));
-          new_rot = 0;
+      for (uint32_t cb = 0; cb < nr; cb += kWave) {
+        bool want = false;
+        if (cb + lane < nr) {
+          const bool in = iface[cb + lane] != 0;
+          want = U[2 * (cb + lane)].x < (in ? a.mut_p * 3.0f : a.mut_p * 0.4f);
         }
-        if (a.finite && old_aa != kCys && new_aa != kCys) {
-          // Neither the Calpha nor any Cys SG moves, so the reference's
-          // energies before and after differ only in residue r's type: a
-          // rotamer move scores the same finite energy twice (delta 0,
-          // accepted) and a mutation is decided on both types' energies
-          // before anything is rebuilt; then the outcome the reference ends
-          // with is built once. (From finite coordinates every step stays
-          // finite; a non-finite input takes the literal path below.)
-          bool accept = true;
-          if (!rot_move) {
-            double e_old, e_new;
-            contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, wv * kWave,
-                          kThreads, lbuf, lcap, e_old, e_new);
-            block_sum2<NW>(e_old, e_new, red, lane, wv);
-            const double ds = ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0;
-            const double delta = (e_new + ds) - (e_old + ds);
-            accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
+        for (uint64_t mv = ballot64(want); mv; mv &= mv - 1) {
+          const uint32_t r = cb + static_cast<uint32_t>(__builtin_ctzll(mv));
+          const float4 uu = U[2 * r];
+          const float4 u = make_float4(uniform_f32(uu.x), uniform_f32(uu.y), uniform_f32(uu.z),
+                                       uniform_f32(uu.w));
+          const bool in = uniform_u32(iface[r]) != 0;
+          const float eff_rot = in ? a.rot_p * 2.0f : a.rot_p;
+          const uint32_t t = r * kSlots + 1;
+          const uint32_t old_aa = uniform_u32(aa[r]), old_rot = uniform_u32(rot[r]);
+          const bool rot_move = u.y < eff_rot;
+          uint32_t new_aa = old_aa, new_rot;
+          if (rot_move) {  // rotamer move
+            const uint32_t nrot = uniform_u32(T.nrot[old_aa]);
+            if (nrot == 0) continue;
+            new_rot = min(static_cast<uint32_t>(u.z * static_cast<float>(nrot)), nrot - 1);
+          } else {  // mutation: the new type's first rotamer
+            new_aa = min(static_cast<uint32_t>(u.z * static_cast<float>(kNumAminoAcids)),
+                         static_cast<uint32_t>(kNumAminoAcids - 1));
+            new_rot = 0;
           }
-          // The build itself waits for the end of the phase: until then
-          // only this residue's Calpha (unchanged by any build) and type are
-          // read, and no other move touches it this step.
+          if (a.finite && old_aa != kCys && new_aa != kCys) {
+            // Neither the Calpha nor any Cys SG moves, so the reference's
+            // energies before and after differ only in residue r's type: a
+            // rotamer move scores the same finite energy twice (delta 0,
+            // accepted) and a mutation is decided on both types' energies
+            // before anything is rebuilt; then the outcome the reference ends
+            // with is built once. (From finite coordinates every step stays
+            // finite; a non-finite input takes the literal path below.)
+            bool accept = true;
+            if (!rot_move) {
+              double e_old, e_new;
+              contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, wv * kWave,
+                            kThreads, lbuf, lcap, e_old, e_new);
+              block_sum2<NW>(e_old, e_new, red, lane, wv);
+              const double ds = ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0;
+              const double delta = (e_new + ds) - (e_old + ds);
+              accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
+            }
+            // The build itself waits for the end of the phase: until then
+            // only this residue's Calpha (unchanged by any build) and type are
+            // read, and no other move touches it this step.
+            if (tid == 0) {
+              if (rot_move) {  // apply_rotamer: the new chi on the current backbone
+                rot[r] = static_cast<uint8_t>(new_rot);
+                pend[r] = kBuildSide;
+              } else if (accept) {  // mutate_residue to (new_aa, rotamer 0)
+                pend[r] = new_aa == old_aa ? kBuildSide : kBuildIdeal;
+                aa[r] = static_cast<uint8_t>(new_aa);
+                rot[r] = 0;
+              } else {  // undone: the old type and rotamer, rebuilt as the reference does
+                pend[r] = new_aa == old_aa ? kBuildSide : kBuildIdeal;
+              }
+            }
+            __syncthreads();
+            continue;
+          }
+          // A Cys gains, loses or moves its SG: the disulfide term needs the
+          // built residue. The Calpha does not move, so both types' energies
+          // come from one pass first; thread 0 builds the move while another
+          // wave's thread builds the reference's undo into scratch (SX/SY/SZ),
+          // from a copy of the residue taken before either starts.
+          double c_old, c_new;
+          contribution2<NW>(T, g, old_aa, new_aa, X[t], Y[t], Z[t], lane, wv, red, c_old, c_new);
+          const double old_e = c_old + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0);
+          for (uint32_t q = tid; q < kSlots; q += kThreads) {
+            SX[q] = X[r * kSlots + q];
+            SY[q] = Y[r * kSlots + q];
+            SZ[q] = Z[r * kSlots + q];
+          }
+          __syncthreads();
           if (tid == 0) {
-            if (rot_move) {  // apply_rotamer: the new chi on the current backbone
+            if (rot_move) {  // apply_rotamer
               rot[r] = static_cast<uint8_t>(new_rot);
-              pend[r] = kBuildSide;
-            } else if (accept) {  // mutate_residue to (new_aa, rotamer 0)
-              pend[r] = new_aa == old_aa ? kBuildSide : kBuildIdeal;
-              aa[r] = static_cast<uint8_t>(new_aa);
-              rot[r] = 0;
-            } else {  // undone: the old type and rotamer, rebuilt as the reference does
-              pend[r] = new_aa == old_aa ? kBuildSide : kBuildIdeal;
+              side_chain(T, old_aa, new_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
+                         Y + r * kSlots, Z + r * kSlots);
+            } else {
+              mutate(T, aa, rot, r, new_aa, new_rot, X, Y, Z);
+            }
+          }
+          if (tid == kUndoThread) {
+            if (rot_move || new_aa == old_aa)  // the old chi on the same backbone
+              side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, SX, SY, SZ);
+            else  // mutate_residue back: the old type on an ideal backbone
+              build_ideal(T, old_aa, old_rot, SX, SY, SZ);
+          }
+          __syncthreads();
+          const uint32_t ncys_new = ncys + (uniform_u32(aa[r]) == kCys) - (old_aa == kCys);
+          const double new_e =
+              c_new + (ncys_new >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0);
+          const double delta = new_e - old_e;
+          const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
+          __syncthreads();
+          if (accept) {
+            ncys = ncys_new;
+          } else {
+            for (uint32_t q = tid; q < kSlots; q += kThreads) {
+              X[r * kSlots + q] = SX[q];
+              Y[r * kSlots + q] = SY[q];
+              Z[r * kSlots + q] = SZ[q];
+            }
+            if (tid == 0) {
+              aa[r] = static_cast<uint8_t>(old_aa);
+              rot[r] = static_cast<uint8_t>(old_rot);
             }
           }
           __syncthreads();
-          continue;
         }
-        // A Cys gains, loses or moves its SG: the disulfide term needs the
-        // built residue. The Calpha does not move, so both types' energies
-        // come from one pass first; thread 0 builds the move while another
-        // wave's thread builds the reference's undo into scratch (SX/SY/SZ),
-        // from a copy of the residue taken before either starts.
-        double c_old, c_new;
-        contribution2<NW>(T, g, old_aa, new_aa, X[t], Y[t], Z[t], lane, wv, red, c_old, c_new);
-        const double old_e = c_old + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0);
-        for (uint32_t q = tid; q < kSlots; q += kThreads) {
-          SX[q] = X[r * kSlots + q];
-          SY[q] = Y[r * kSlots + q];
-          SZ[q] = Z[r * kSlots + q];
-        }
-        __syncthreads();
-        if (tid == 0) {
-          if (rot_move) {  // apply_rotamer
-            rot[r] = static_cast<uint8_t>(new_rot);
-            side_chain(T, old_aa, new_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
-                       Y + r * kSlots, Z + r * kSlots);
-          } else {
-            mutate(T, aa, rot, r, new_aa, new_rot, X, Y, Z);
-          }
-        }
-        if (tid == kUndoThread) {
-          if (rot_move || new_aa == old_aa)  // the old chi on the same backbone
-            side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, SX, SY, SZ);
-          else  // mutate_residue back: the old type on an ideal backbone
-            build_ideal(T, old_aa, old_rot, SX, SY, SZ);
-        }
-        __syncthreads();
-        const uint32_t ncys_new = ncys + (uniform_u32(aa[r]) == kCys) - (old_aa == kCys);
-        const double new_e =
-            c_new + (ncys_new >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0);
-        const double delta = new_e - old_e;
-        const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
-        __syncthreads();
-        if (accept) {
-          ncys = ncys_new;
-        } else {
-          for (uint32_t q = tid; q < kSlots; q += kThreads) {
-            X[r * kSlots + q] = SX[q];
-            Y[r * kSlots + q] = SY[q];
-            Z[r * kSlots + q] = SZ[q];
-          }
-          if (tid == 0) {
-            aa[r] = static_cast<uint8_t>(old_aa);
-            rot[r] = static_cast<uint8_t>(old_rot);
-          }
-        }
-        __syncthreads();
       }
     }
     // the side chains the moves above decided, built in parallel
@@ -921,21 +929,24 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
     __syncthreads();
     // 4. torsion moves (src/diffusion.rs:400-415)
     if (a.moves & DNAB_MC_BACKBONE) {
-      for (uint32_t r = 0; r < nr; ++r) {
-        const float4 uu = U[2 * r + 1];
-        if (!(uniform_f32(uu.x) < a.bb_p)) continue;
-        const float4 u = make_float4(uu.x, uniform_f32(uu.y), uniform_f32(uu.z), uniform_f32(uu.w));
-        const float deg = -a.max_torsion + (a.max_torsion + a.max_torsion) * u.y;
-        const float rad = geo::deg2rad(deg);
-        const bool phi = u.z < 0.5f;
-        const double old_e =
-            chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red, cysl);
-        torsion(X, Y, Z, r, nr, rad, phi, tid, kThreads);
-        const double new_e =
-            chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red, cysl);
-        const double delta = new_e - old_e;
-        const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
-        if (!accept) torsion(X, Y, Z, r, nr, -rad, phi, tid, kThreads);  // rotate back
+      for (uint32_t cb = 0; cb < nr; cb += kWave) {
+        const bool want = cb + lane < nr && U[2 * (cb + lane) + 1].x < a.bb_p;
+        for (uint64_t mv = ballot64(want); mv; mv &= mv - 1) {
+          const uint32_t r = cb + static_cast<uint32_t>(__builtin_ctzll(mv));
+          const float4 uu = U[2 * r + 1];
+          const float4 u = make_float4(uu.x, uniform_f32(uu.y), uniform_f32(uu.z), uniform_f32(uu.w));
+          const float deg = -a.max_torsion + (a.max_torsion + a.max_torsion) * u.y;
+          const float rad = geo::deg2rad(deg);
+          const bool phi = u.z < 0.5f;
+          const double old_e =
+              chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red, cysl);
+          torsion(X, Y, Z, r, nr, rad, phi, tid, kThreads);
+          const double new_e =
+              chain_energy<NW>(T, g, aa, X, Y, Z, r, nr, ncys, lane, wv, lbuf, lcap, red, cysl);
+          const double delta = new_e - old_e;
+          const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
+          if (!accept) torsion(X, Y, Z, r, nr, -rad, phi, tid, kThreads);  // rotate back
+        }
       }
     }
   }
