@@ -1,6 +1,6 @@
 // All-atom refinement with Monte-Carlo moves on the device: every iteration
 // of allatom_diffusion_step (src/diffusion.rs:290-416) for a batch of
-// candidates in one launch, one wave per candidate:
+// candidates in one launch, one workgroup of 1, 4 or 8 waves per candidate:
 //   1. residue-level forces on each Calpha from the antigen Calpha grid
 //      (compute_forces_with_grid, src/energy.rs:322-398), the radial
 //      restraint, clamp, optional noise, set_ca_pos (the gradient step);
@@ -251,20 +251,6 @@ __device__ inline void contrib(const RefTables& T, const CaSet& g, const uint8_t
   flush();
 }
 
-// This thread's part of residue_contribution_allatom for a Calpha of type
-// `aa` at p: the antigen Calphas tid, tid + nt, ...
-__device__ inline double pair_terms(const RefTables& T, const CaSet& g, uint32_t aa, float px,
-                                    float py, float pz, uint32_t tid, uint32_t nt) {
-  double e = 0.0;
-  for (uint32_t i = tid; i < g.n; i += nt) {
-    const float dx = px - g.x[i], dy = py - g.y[i], dz = pz - g.z[i];
-    const float r_sq = dx * dx + dy * dy + dz * dz;
-    if (r_sq > 100.0f || r_sq < 0.25f) continue;
-    e += pair_term(T, aa, g.par[i], r_sq);
-  }
-  return e;
-}
-
 // residue_contribution_allatom of a Calpha at p as type ta and as type tb in
 // one pass, FP64 totals on every thread.
 template <int NW>
@@ -281,15 +267,6 @@ __device__ inline void contribution2(const RefTables& T, const CaSet& g, uint32_
     eb += pair_term(T, tb, q, r_sq);
   }
   block_sum2<NW>(ea, eb, red, lane, wv);
-}
-
-// residue_contribution_allatom, FP64 total on every thread.
-template <int NW>
-__device__ inline double contribution(const RefTables& T, const CaSet& g, uint32_t aa, float px,
-                                      float py, float pz, uint32_t lane, uint32_t wv,
-                                      double* red) {
-  return block_sum<NW>(pair_terms(T, g, aa, px, py, pz, wv * kWave + lane, NW * kWave), red,
-                       lane, wv);
 }
 
 // Side chain of residue slots X/Y/Z (type aa, rotamer rot) from its
@@ -411,7 +388,7 @@ __device__ inline double chain_energy(const RefTables& T, const CaSet& g, const 
 // perturb_phi / perturb_psi (src/allatom.rs:392-456) by rad: phi turns the
 // atoms of residue r after its Calpha and every later residue about the N-CA
 // axis through CA; psi turns r's O and every later residue about the CA-C
-// axis through C. Lane-split over the slots; the axis atoms do not move.
+// axis through C. Split over the threads by slot; the axis atoms do not move.
 __device__ inline void torsion(float* X, float* Y, float* Z, uint32_t r, uint32_t nr, float rad,
                                bool phi, uint32_t tid, uint32_t nt) {
   const uint32_t b = r * kSlots;
