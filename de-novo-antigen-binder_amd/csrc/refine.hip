@@ -20,8 +20,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "device_common.h"
@@ -30,6 +32,7 @@
 #include "ff_terms.h"
 #include "geometry.h"
 #include "grid_device.h"
+#include "host_parallel.h"
 #include "step_device.h"
 
 namespace dnab {
@@ -1033,6 +1036,7 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
     return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "refine: more than %u residues in a candidate",
                           kMaxRefineRes);
   // every residue carries its topology's atoms in topology order (the slots)
+  auto t_pack = std::chrono::steady_clock::now();
   std::vector<float4> slots(std::max<size_t>(nr * kSlots, 1), make_float4(0, 0, 0, 0));
   std::vector<uint8_t> raa(std::max<size_t>(nr, 1)), rrot(std::max<size_t>(nr, 1), 0);
   bool finite = true;
@@ -1050,12 +1054,24 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
         return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: residue %zu rotamer out of range", r);
       rrot[r] = res_rotamer[r];
     }
-    for (uint32_t k = 0; k < e - b; ++k) {
-      const float x = ab->atoms.x[b + k], y = ab->atoms.y[b + k], z = ab->atoms.z[b + k];
-      finite = finite && std::isfinite(x) && std::isfinite(y) && std::isfinite(z);
-      slots[r * kSlots + k] = make_float4(x, y, z, 0.0f);
-    }
   }
+  {
+    std::atomic<bool> all_finite{true};
+    parallel_for(nr, [&](size_t rb, size_t re) {
+      bool ok = true;
+      for (size_t r = rb; r < re; ++r) {
+        const uint32_t b = ab->res_atom_off[r], e = ab->res_atom_off[r + 1];
+        for (uint32_t k = 0; k < e - b; ++k) {
+          const float x = ab->atoms.x[b + k], y = ab->atoms.y[b + k], z = ab->atoms.z[b + k];
+          ok = ok && std::isfinite(x) && std::isfinite(y) && std::isfinite(z);
+          slots[r * kSlots + k] = make_float4(x, y, z, 0.0f);
+        }
+      }
+      if (!ok) all_finite.store(false, std::memory_order_relaxed);
+    });
+    finite = all_finite.load();
+  }
+  timer_host(ctx, "refine_host_pack", ms_since(t_pack));
   if (nc == 0) {
     if (n_atoms_out) *n_atoms_out = 0;
     if (out && out->res_start) out->res_start[0] = 0;
@@ -1234,28 +1250,34 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   if (n_atoms_out) *n_atoms_out = n_out;
   if (out_rotamer)
     for (size_t r = 0; r < nr; ++r) out_rotamer[r] = rrot[r];
-  std::vector<float> fx(n_out), fy(n_out), fz(n_out), fq(n_out), frm(n_out), fe(n_out);
-  std::vector<uint8_t> fh(n_out), ft(n_out);
-  std::vector<uint32_t> fca(std::max<size_t>(nr, 1)), fri(n_out);
-  for (size_t c = 0; c < nc; ++c)
-    for (uint32_t r = ab->cand_res_off[c]; r < ab->cand_res_off[c + 1]; ++r) {
-      const ResTopology& topo = kResidueTopology[raa[r]];
-      fca[r] = res_off[r] + topo.ca_idx;
-      for (uint32_t k = 0; k < topo.n_atoms; ++k) {
-        const size_t o = res_off[r] + k;
-        const float4 v = slots[r * kSlots + k];
-        const ResAtomDef& d = topo.atoms[k];
-        fx[o] = v.x;
-        fy[o] = v.y;
-        fz[o] = v.z;
-        fq[o] = d.charge;
-        frm[o] = kAmberNb[d.atom_type].r_min_half;
-        fe[o] = kAmberNb[d.atom_type].epsilon;
-        fh[o] = d.hydrophobic;
-        ft[o] = d.atom_type;
-        fri[o] = r - ab->cand_res_off[c];
+  auto t_asm = std::chrono::steady_clock::now();
+  const size_t no = std::max<size_t>(n_out, 1);
+  std::unique_ptr<float[]> fx(new float[no]), fy(new float[no]), fz(new float[no]),
+      fq(new float[no]), frm(new float[no]), fe(new float[no]);
+  std::unique_ptr<uint8_t[]> fh(new uint8_t[no]), ft(new uint8_t[no]);
+  std::unique_ptr<uint32_t[]> fca(new uint32_t[std::max<size_t>(nr, 1)]), fri(new uint32_t[no]);
+  parallel_for(nc, [&](size_t cb, size_t ce) {
+    for (size_t c = cb; c < ce; ++c)
+      for (uint32_t r = ab->cand_res_off[c]; r < ab->cand_res_off[c + 1]; ++r) {
+        const ResTopology& topo = kResidueTopology[raa[r]];
+        fca[r] = res_off[r] + topo.ca_idx;
+        for (uint32_t k = 0; k < topo.n_atoms; ++k) {
+          const size_t o = res_off[r] + k;
+          const float4 v = slots[r * kSlots + k];
+          const ResAtomDef& d = topo.atoms[k];
+          fx[o] = v.x;
+          fy[o] = v.y;
+          fz[o] = v.z;
+          fq[o] = d.charge;
+          frm[o] = kAmberNb[d.atom_type].r_min_half;
+          fe[o] = kAmberNb[d.atom_type].epsilon;
+          fh[o] = d.hydrophobic;
+          ft[o] = d.atom_type;
+          fri[o] = r - ab->cand_res_off[c];
+        }
       }
-    }
+  });
+  timer_host(ctx, "refine_host_assemble", ms_since(t_asm));
   if (out_energy) {
     // allatom_energy of each refined candidate (src/diffusion.rs:1092)
     std::vector<uint32_t> cand_ag(nc, 0);
@@ -1265,17 +1287,17 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
     pd.cand_antigen = cand_ag.data();
     pd.cand_res_off = ab->cand_res_off;
     pd.res_atom_off = res_off.data();
-    pd.res_ca_atom = fca.data();
+    pd.res_ca_atom = fca.get();
     pd.res_amino_acid = raa.data();
     pd.atoms.n = n_out;
-    pd.atoms.x = fx.data();
-    pd.atoms.y = fy.data();
-    pd.atoms.z = fz.data();
-    pd.atoms.charge = fq.data();
-    pd.atoms.r_min_half = frm.data();
-    pd.atoms.epsilon = fe.data();
-    pd.atoms.hydrophobic = fh.data();
-    pd.atoms.atom_type = ft.data();
+    pd.atoms.x = fx.get();
+    pd.atoms.y = fy.get();
+    pd.atoms.z = fz.get();
+    pd.atoms.charge = fq.get();
+    pd.atoms.r_min_half = frm.get();
+    pd.atoms.epsilon = fe.get();
+    pd.atoms.hydrophobic = fh.get();
+    pd.atoms.atom_type = ft.get();
     PopGuard pop;
     dnab_antigen* ags[1] = {ag};
     DNAB_RETURN_IF(dnab_population_create(ctx, ags, 1, &pd, &pop.p));
@@ -1290,17 +1312,17 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   if (!out) return DNAB_OK;
   if (!out->x || !out->y || !out->z) return DNAB_ERR_INVALID;
   if (n_out > out->atom_capacity) return DNAB_ERR_CAPACITY;
-  std::memcpy(out->x, fx.data(), n_out * sizeof(float));
-  std::memcpy(out->y, fy.data(), n_out * sizeof(float));
-  std::memcpy(out->z, fz.data(), n_out * sizeof(float));
-  if (out->charge) std::memcpy(out->charge, fq.data(), n_out * sizeof(float));
-  if (out->r_min_half) std::memcpy(out->r_min_half, frm.data(), n_out * sizeof(float));
-  if (out->epsilon) std::memcpy(out->epsilon, fe.data(), n_out * sizeof(float));
-  if (out->hydrophobic) std::memcpy(out->hydrophobic, fh.data(), n_out);
-  if (out->atom_type) std::memcpy(out->atom_type, ft.data(), n_out);
-  if (out->residue_idx) std::memcpy(out->residue_idx, fri.data(), n_out * sizeof(uint32_t));
+  std::memcpy(out->x, fx.get(), n_out * sizeof(float));
+  std::memcpy(out->y, fy.get(), n_out * sizeof(float));
+  std::memcpy(out->z, fz.get(), n_out * sizeof(float));
+  if (out->charge) std::memcpy(out->charge, fq.get(), n_out * sizeof(float));
+  if (out->r_min_half) std::memcpy(out->r_min_half, frm.get(), n_out * sizeof(float));
+  if (out->epsilon) std::memcpy(out->epsilon, fe.get(), n_out * sizeof(float));
+  if (out->hydrophobic) std::memcpy(out->hydrophobic, fh.get(), n_out);
+  if (out->atom_type) std::memcpy(out->atom_type, ft.get(), n_out);
+  if (out->residue_idx) std::memcpy(out->residue_idx, fri.get(), n_out * sizeof(uint32_t));
   if (out->res_start) std::memcpy(out->res_start, res_off.data(), (nr + 1) * sizeof(uint32_t));
-  if (out->ca_atom_idx && nr) std::memcpy(out->ca_atom_idx, fca.data(), nr * sizeof(uint32_t));
+  if (out->ca_atom_idx && nr) std::memcpy(out->ca_atom_idx, fca.get(), nr * sizeof(uint32_t));
   if (out->amino_acid && nr) std::memcpy(out->amino_acid, raa.data(), nr);
   return DNAB_OK;
 }
