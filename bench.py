@@ -177,6 +177,63 @@ def interface_positions(ab, ag):
     return [int(r) for r in np.nonzero((r2 < np.float32(64.0)).any(axis=1))[0]]
 
 
+def allatom_energy_bench(ctx, w, pop, coords, reps=10, cpu_budget_s=3.0):
+    """allatom_energy (src/diffusion.rs:281-283: the five pair terms of
+    interaction_energy_atoms_breakdown plus the disulfide term) of every
+    binder of the workload at its start state, one breakdown evaluation of the
+    population per call. The CPU baseline is the oracle's f32 restatement of
+    the same sum, one thread, on a bounded sample of the binders."""
+    import numpy as np
+    pop.set_coords(*coords)
+    pairs, _ = pop.pair_count()
+    pop.evaluate(forces=False, breakdown=True)     # warm-up
+    ctx.synchronize()
+    ctx.set_timing(True)
+    k0 = ctx.kernel_time("pair_breakdown")
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        pop.evaluate(forces=False, breakdown=True)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+    k1 = ctx.kernel_time("pair_breakdown")
+    ctx.set_timing(False)
+    dt = float(np.median(times))
+    terms = pop.terms()
+    out = {
+        "workload": ("allatom_energy (LJ, Coulomb, hydrophobic, H-bond, EEF1 solvation, disulfide) "
+                     "of the 1,024 binders at their start state"),
+        "candidates": int(pop.n_cand),
+        "pairs_in_cutoff": int(pairs),
+        "ms_per_call": dt * 1e3,
+        "kernel_ms": (k1[0] - k0[0]) / reps,
+        "value": pop.n_cand / dt,
+        "unit": "candidates/s",
+        "pairs_per_s": pairs / dt,
+        "mean_energy": float(terms.sum(1).mean()),
+    }
+    from oracle import oracle as O
+    p = w.pop
+    ag = w.antigens[0]
+    n, t0 = 0, time.perf_counter()
+    while True:
+        c = n % p.n_cand
+        cl = p.candidate_cloud(c)
+        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
+        O.energy_full_f32(ag.atoms, cl, w.cutoff)
+        rs = (p.res_atom_off[r0:r1 + 1] - p.res_atom_off[r0]).astype(np.uint32)
+        O.disulfide_f32(*O.sg_positions(cl.x, cl.y, cl.z, rs, p.res_amino_acid[r0:r1], 0, r1 - r0))
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= cpu_budget_s or n >= 100000:
+            break
+    out["cpu_baseline"] = {
+        "value": n / el, "unit": "candidates/s", "cores": 1, "kind": "port",
+        "sample": f"{n} f32 allatom_energy scorings of the workload's binders in {el:.1f} s"}
+    out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def maturation_bench(ctx, w, ag_dev, coords, reps=5, cpu_budget_s=4.0):
     """Batched affinity maturation (dnab_affinity_maturation) of every binder
     the loop refined, at its interface positions, as src/main.rs:341-347 runs
@@ -436,6 +493,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-saturated", action="store_true")
     ap.add_argument("--no-maturation", action="store_true")
+    ap.add_argument("--no-allatom", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--tuning", default="",
                     help="fixed pair work split 'survivors,chunks,waves' instead of autotuning")
@@ -595,6 +653,10 @@ def main():
             "flop_per_pair": FLOP_PER_PAIR_EF,
         },
     }
+    if not args.no_allatom:
+        log("[bench] allatom_energy of the binders at their start state")
+        out["allatom_energy"] = allatom_energy_bench(ctx, w, pop, (x0, y0, z0))
+        log(f"[bench] allatom_energy: {out['allatom_energy']['ms_per_call']:.3f} ms per call")
     if not args.no_maturation:
         log("[bench] batched affinity maturation of the refined binders")
         out["maturation"] = maturation_bench(ctx, w, ags[0], (x1, y1, z1))
