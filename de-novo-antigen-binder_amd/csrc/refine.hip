@@ -185,10 +185,10 @@ __device__ inline double pair_term(const RefTables& T, uint32_t aa, const float4
 }
 
 // residue_contribution_allatom of residues [r0, r1) at their Calphas, this
-// lane's FP64 part over the antigen Calphas of chunks i_first, i_first +
-// i_step, ... (64 each; a workgroup's waves split them): a cheap
-// squared-distance pass (the reference's skip test) compacts the pairs it
-// keeps into the wave's buf, then their terms are evaluated lane-parallel.
+// lane's FP64 part over the units u_first, u_first + u_step, ... of (antigen
+// chunk of 64, group of 4 residues) pairs (a workgroup's waves split them): a
+// cheap squared-distance pass (the reference's skip test) compacts the pairs
+// it keeps into the wave's buf, then their terms are evaluated lane-parallel.
 // With kTwo, one residue (r0) is scored as type ta into ea and as type tb
 // into eb in the same pass; else each residue is scored as its current type
 // into ea.
@@ -196,7 +196,7 @@ template <bool kTwo>
 __device__ inline void contrib(const RefTables& T, const CaSet& g, const uint8_t* aa,
                                const float* X, const float* Y, const float* Z, uint32_t r0,
                                uint32_t r1, uint32_t ta, uint32_t tb, uint32_t lane,
-                               uint32_t i_first, uint32_t i_step, uint2* buf, uint32_t bcap,
+                               uint32_t u_first, uint32_t u_step, uint2* buf, uint32_t bcap,
                                double& ea, double& eb) {
   ea = eb = 0.0;
   uint32_t cnt = 0;
@@ -229,25 +229,24 @@ __device__ inline void contrib(const RefTables& T, const CaSet& g, const uint8_t
       py = Y[t];
       pz = Z[t];
     }
-    for (uint32_t i0 = i_first; i0 < g.n; i0 += i_step) {
-      const uint32_t i = i0 + lane;
+    const uint32_t ngr = (nrc + 3) / 4, nunits = ngr * ((g.n + kWave - 1) / kWave);
+    for (uint32_t u = u_first; u < nunits; u += u_step) {
+      const uint32_t i = (u / ngr) * kWave + lane, j0 = (u % ngr) * 4;
       const bool live = i < g.n;
       const float ax = live ? g.x[i] : 0.0f, ay = live ? g.y[i] : 0.0f,
                   az = live ? g.z[i] : 0.0f;
-      // four residues per round (independent chains); room for their pairs first
-      for (uint32_t j0 = 0; j0 < nrc; j0 += 4) {
-        if (cnt + 4 * kWave > bcap) flush();
+      // four residues per unit (independent chains); room for their pairs first
+      if (cnt + 4 * kWave > bcap) flush();
 #pragma unroll
-        for (uint32_t v = 0; v < 4; ++v) {
-          const uint32_t j = j0 + v, jl = min(j, nrc - 1);
-          const float dx = readlane_f32(px, jl) - ax, dy = readlane_f32(py, jl) - ay,
-                      dz = readlane_f32(pz, jl) - az;
-          const float r_sq = dx * dx + dy * dy + dz * dz;
-          const bool keep = live & (j < nrc) & !(r_sq > 100.0f) & !(r_sq < 0.25f);  // NaN kept
-          const uint64_t b = ballot64(keep);
-          if (keep) buf[cnt + static_cast<uint32_t>(__popcll(b & below))] = make_uint2(i, cb + j);
-          cnt += static_cast<uint32_t>(__popcll(b));
-        }
+      for (uint32_t v = 0; v < 4; ++v) {
+        const uint32_t j = j0 + v, jl = min(j, nrc - 1);
+        const float dx = readlane_f32(px, jl) - ax, dy = readlane_f32(py, jl) - ay,
+                    dz = readlane_f32(pz, jl) - az;
+        const float r_sq = dx * dx + dy * dy + dz * dz;
+        const bool keep = live & (j < nrc) & !(r_sq > 100.0f) & !(r_sq < 0.25f);  // NaN kept
+        const uint64_t b = ballot64(keep);
+        if (keep) buf[cnt + static_cast<uint32_t>(__popcll(b & below))] = make_uint2(i, cb + j);
+        cnt += static_cast<uint32_t>(__popcll(b));
       }
     }
   }
@@ -383,8 +382,7 @@ __device__ inline double chain_energy(const RefTables& T, const CaSet& g, const 
                                       uint32_t nr, uint32_t ncys, uint32_t lane, uint32_t wv,
                                       uint2* buf, uint32_t bcap, double* red, uint16_t* cysl) {
   double part, unused;
-  contrib<false>(T, g, aa, X, Y, Z, r, nr, 0, 0, lane, wv * kWave, NW * kWave, buf, bcap, part,
-                 unused);
+  contrib<false>(T, g, aa, X, Y, Z, r, nr, 0, 0, lane, wv, NW, buf, bcap, part, unused);
   return block_sum<NW>(part, red, lane, wv) + (ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0);
 }
 
@@ -817,8 +815,8 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
             bool accept = true;
             if (!rot_move) {
               double e_old, e_new;
-              contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, wv * kWave,
-                            kThreads, lbuf, lcap, e_old, e_new);
+              contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, wv, NW, lbuf,
+                            lcap, e_old, e_new);
               block_sum2<NW>(e_old, e_new, red, lane, wv);
               const double ds = ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0;
               const double delta = (e_new + ds) - (e_old + ds);
