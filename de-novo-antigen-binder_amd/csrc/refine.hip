@@ -1172,11 +1172,13 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   DNAB_HIP_CHECK(ctx, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                             ctx->device));
   const size_t lds_cu = static_cast<size_t>(lds_max);
-  if (mc->waves != 0 && mc->waves != 1 && mc->waves != 4 && mc->waves != 8)
-    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: waves must be 0, 1, 4 or 8");
+  if (mc->waves != 0 && mc->waves != 1 && mc->waves != 2 && mc->waves != 4 && mc->waves != 8)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "refine: waves must be 0, 1, 2, 4 or 8");
   const size_t cu = ctx->cu_count > 0 ? static_cast<size_t>(ctx->cu_count) : 256;
-  // small batches leave CUs idle: eight or four waves per candidate split its work
-  const uint32_t nw = mc->waves ? mc->waves : (nc <= cu ? 8u : nc <= 2 * cu ? 4u : 1u);
+  // a lone wave per SIMD is latency-bound: small batches split each candidate
+  // over eight or four waves; larger ones over two (two waves per SIMD, which
+  // the kernel's ~210 VGPRs allow)
+  const uint32_t nw = mc->waves ? mc->waves : (nc <= cu ? 8u : nc <= 2 * cu ? 4u : 2u);
   const size_t fixed = (sizeof(RefTables) + 15) / 16 * 16 +
                        (2 * kWave * nw + 8) * sizeof(float4) + kWave * nw * sizeof(uint32_t) +
                        static_cast<size_t>(max_res) * (2 * 16 + (3 * kSlots + 3) * sizeof(float) + 4);
@@ -1209,10 +1211,12 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   if (a.n_ag_lds)
     kfn = nw == 8 ? reinterpret_cast<const void*>(&k_refine<true, 8>)
         : nw == 4 ? reinterpret_cast<const void*>(&k_refine<true, 4>)
+        : nw == 2 ? reinterpret_cast<const void*>(&k_refine<true, 2>)
                   : reinterpret_cast<const void*>(&k_refine<true, 1>);
   else
     kfn = nw == 8 ? reinterpret_cast<const void*>(&k_refine<false, 8>)
         : nw == 4 ? reinterpret_cast<const void*>(&k_refine<false, 4>)
+        : nw == 2 ? reinterpret_cast<const void*>(&k_refine<false, 2>)
                   : reinterpret_cast<const void*>(&k_refine<false, 1>);
   if (lds > 65536)
     DNAB_HIP_CHECK(ctx, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1224,12 +1228,16 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
     hipLaunchKernelGGL((k_refine<true, 8>), grid_dim, block_dim, lds, ctx->stream, a);
   else if (a.n_ag_lds && nw == 4)
     hipLaunchKernelGGL((k_refine<true, 4>), grid_dim, block_dim, lds, ctx->stream, a);
+  else if (a.n_ag_lds && nw == 2)
+    hipLaunchKernelGGL((k_refine<true, 2>), grid_dim, block_dim, lds, ctx->stream, a);
   else if (a.n_ag_lds)
     hipLaunchKernelGGL((k_refine<true, 1>), grid_dim, block_dim, lds, ctx->stream, a);
   else if (nw == 8)
     hipLaunchKernelGGL((k_refine<false, 8>), grid_dim, block_dim, lds, ctx->stream, a);
   else if (nw == 4)
     hipLaunchKernelGGL((k_refine<false, 4>), grid_dim, block_dim, lds, ctx->stream, a);
+  else if (nw == 2)
+    hipLaunchKernelGGL((k_refine<false, 2>), grid_dim, block_dim, lds, ctx->stream, a);
   else
     hipLaunchKernelGGL((k_refine<false, 1>), grid_dim, block_dim, lds, ctx->stream, a);
   DNAB_HIP_CHECK(ctx, hipGetLastError());

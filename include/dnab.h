@@ -485,7 +485,7 @@ typedef struct dnab_mc_params {
   float iface_cutoff_sq; /* 64, an 8 A Calpha-Calpha interface (src/diffusion.rs:51) */
   uint32_t moves;        /* DNAB_MC_SIDECHAIN | DNAB_MC_BACKBONE (the reference) */
   uint32_t waves;        /* waves per candidate: 0 = chosen by batch size (8 for batches up
-                            to the CU count, 4 up to twice it, else 1), or 1 / 4 / 8; no
+                            to the CU count, 4 up to twice it, else 2), or 1 / 2 / 4 / 8; no
                             effect on results beyond the order of FP64 energy sums */
 } dnab_mc_params;
 int dnab_mc_default_params(dnab_mc_params* p);

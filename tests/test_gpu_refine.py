@@ -84,9 +84,10 @@ def _oracle_run(ag, ab, rot, g0, lp, mc):
 
 
 @pytest.mark.parametrize("noise,moves,waves", [(1, 3, 0), (0, 3, 1), (1, 1, 4), (1, 2, 1),
-                                               (1, 3, 1), (1, 3, 4)],
+                                               (1, 3, 1), (1, 3, 4), (1, 3, 2)],
                          ids=["noise-all", "quiet-all-1wave", "noise-sidechain-4waves",
-                              "noise-torsion-1wave", "noise-all-1wave", "noise-all-4waves"])
+                              "noise-torsion-1wave", "noise-all-1wave", "noise-all-4waves",
+                              "noise-all-2waves"])
 def test_refine_matches_the_oracle(ctx, noise, moves, waves):
     """waves: per candidate, 0 = the library's choice (8 for this batch)."""
     ag, abs_, rots = _workload(seed=3 + moves)
@@ -189,7 +190,7 @@ def test_refine_repeats_and_seeds_differ(ctx):
         assert np.array_equal(p.atoms.x, q.atoms.x) and list(p.amino_acid) == list(q.amino_acid)
     assert np.array_equal(a[2], b[2])
     # one wave or four per candidate: the same moves and coordinates
-    for w in (1, 4, 8):
+    for w in (1, 2, 4, 8):
         d = ctx.refine(agd, abs_, lp, mc_params(mutation_p=0.3, backbone_move_p=0.2, waves=w), rots)
         for p, q in zip(a[0], d[0]):
             assert np.array_equal(p.atoms.x, q.atoms.x) and np.array_equal(p.atoms.z, q.atoms.z)
@@ -206,7 +207,7 @@ def test_refine_rejects_bad_inputs(ctx):
         ctx.refine(DeviceAntigen(ctx, ag, cutoff=10.0, with_ca=False), abs_)
     with pytest.raises(Exception):          # rotamer index out of range
         ctx.refine(agd, abs_, rotamers=[np.full(len(abs_[0].amino_acid), 7, np.uint8)])
-    with pytest.raises(Exception):          # waves per candidate other than 0, 1, 4, 8
-        ctx.refine(agd, abs_, langevin_params(n_steps=2), mc_params(waves=2))
+    with pytest.raises(Exception):          # waves per candidate other than 0, 1, 2, 4, 8
+        ctx.refine(agd, abs_, langevin_params(n_steps=2), mc_params(waves=3))
     prots, _, e = ctx.refine(agd, [], langevin_params(n_steps=2))
     assert prots == [] and len(e) == 0
