@@ -19,6 +19,8 @@
 //                                          src/pdb.rs:19-57, 121-191
 //   binder::CandidateReport, write_fasta, write_json, write_csv
 //                                          src/report.rs:8-113
+//   binder::allatom_diffusion_steps        src/diffusion.rs:290-416 (with the moves,
+//                                          a batch of antibodies in one call)
 //   binder::run_allatom_gpu_phase          src/diffusion.rs:1008-1062 (GPU phase,
 //                                          gradient steps + scoring, no MC moves)
 //
@@ -499,6 +501,105 @@ inline std::vector<AllAtomResult> run_allatom_gpu_phase(
   std::vector<AllAtomResult> out(n);
   for (uint32_t k = 0; k < n; ++k) out[k] = {idx[k], e[k]};
   return out;
+}
+
+
+// allatom_diffusion_step (src/diffusion.rs:290-416) lp.n_steps times for every
+// antibody against the antigen, in one dnab_refine call (mc: the move rates,
+// NULL for the reference's). Each antibody is replaced by its refined protein;
+// rotamers (optional, one vector per antibody) carry each residue's rotamer
+// index in and out. Returns each refined antibody's allatom_energy.
+inline std::vector<double> allatom_diffusion_steps(
+    const GpuContext& gpu, const AtomProtein& antigen, std::vector<AtomProtein>& abs,
+    const dnab_langevin_params& lp, const dnab_mc_params* mc = nullptr,
+    std::vector<std::vector<uint8_t>>* rotamers = nullptr, float cutoff = 10.0f) {
+  dnab_ctx* c = gpu.raw();
+  ResidueCloud ca;
+  std::vector<uint8_t> ca_aa;
+  for (size_t r = 0; r < antigen.n_residues(); ++r) {
+    const uint32_t i = antigen.ca_atom_idx[r];
+    ca.push(antigen.atoms.x[i], antigen.atoms.y[i], antigen.atoms.z[i], antigen.amino_acid[r]);
+    ca_aa.push_back(static_cast<uint8_t>(antigen.amino_acid[r]));
+  }
+  dnab_antigen_desc ad{};
+  ad.atoms = antigen.atoms.view();
+  ad.ca = ca.view();
+  ad.ca_amino_acid = ca_aa.data();
+  const auto ctr = antigen.ca_center_of_mass();
+  std::memcpy(ad.center, ctr.data(), sizeof(ad.center));
+  ad.cell_size = cutoff;
+  ad.cutoff = cutoff;
+  dnab_antigen* ag = nullptr;
+  check(dnab_antigen_create(c, &ad, &ag), c);
+  std::unique_ptr<dnab_antigen, void (*)(dnab_antigen*)> ag_guard(ag, dnab_antigen_destroy);
+  AtomCloud all;
+  std::vector<uint32_t> cand_ag(abs.size(), 0), cand_res_off{0}, res_atom_off{0}, res_ca;
+  std::vector<uint8_t> res_aa, res_rot;
+  for (size_t a = 0; a < abs.size(); ++a) {
+    const AtomProtein& p = abs[a];
+    const uint32_t base = static_cast<uint32_t>(all.len());
+    for (size_t r = 0; r < p.n_residues(); ++r) {
+      res_atom_off.push_back(base + p.res_start[r + 1]);
+      res_ca.push_back(base + p.ca_atom_idx[r]);
+      res_aa.push_back(static_cast<uint8_t>(p.amino_acid[r]));
+      res_rot.push_back(rotamers && a < rotamers->size() && r < (*rotamers)[a].size()
+                            ? (*rotamers)[a][r]
+                            : 0);
+    }
+    cand_res_off.push_back(static_cast<uint32_t>(res_ca.size()));
+    auto app = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
+    app(all.x, p.atoms.x); app(all.y, p.atoms.y); app(all.z, p.atoms.z);
+    app(all.charge, p.atoms.charge); app(all.r_min_half, p.atoms.r_min_half);
+    app(all.epsilon, p.atoms.epsilon); app(all.hydrophobic, p.atoms.hydrophobic);
+  }
+  const size_t nr = res_ca.size(), cap = 14 * nr + 1;
+  dnab_population_desc pd{};
+  pd.n_cand = abs.size();
+  pd.n_res = nr;
+  pd.cand_antigen = cand_ag.data();
+  pd.cand_res_off = cand_res_off.data();
+  pd.res_atom_off = res_atom_off.data();
+  pd.res_ca_atom = res_ca.data();
+  pd.res_amino_acid = res_aa.data();
+  pd.atoms = all.view();
+  AtomCloud o;
+  o.x.resize(cap); o.y.resize(cap); o.z.resize(cap); o.charge.resize(cap);
+  o.r_min_half.resize(cap); o.epsilon.resize(cap); o.hydrophobic.resize(cap);
+  o.atom_type.resize(cap); o.residue_idx.resize(cap);
+  std::vector<uint32_t> rs(nr + 1), cai(std::max<size_t>(nr, 1));
+  std::vector<uint8_t> aa_out(nr + 1), rot_out(nr + 1);
+  dnab_protein_out po{cap, o.x.data(), o.y.data(), o.z.data(), o.charge.data(),
+                      o.r_min_half.data(), o.epsilon.data(), o.hydrophobic.data(),
+                      o.atom_type.data(), o.residue_idx.data(), rs.data(), cai.data(),
+                      aa_out.data()};
+  dnab_mc_params mp{};
+  if (mc) mp = *mc;
+  else check(dnab_mc_default_params(&mp));
+  std::vector<double> e(std::max<size_t>(abs.size(), 1));
+  size_t na = 0;
+  check(dnab_refine(c, ag, &pd, res_rot.data(), &lp, &mp, &po, rot_out.data(), e.data(), &na), c);
+  for (size_t a = 0; a < abs.size(); ++a) {
+    const uint32_t r0 = cand_res_off[a], r1 = cand_res_off[a + 1], a0 = rs[r0], a1 = rs[r1];
+    AtomProtein q;
+    auto cut = [&](auto& dst, const auto& src) { dst.assign(src.begin() + a0, src.begin() + a1); };
+    cut(q.atoms.x, o.x); cut(q.atoms.y, o.y); cut(q.atoms.z, o.z); cut(q.atoms.charge, o.charge);
+    cut(q.atoms.r_min_half, o.r_min_half); cut(q.atoms.epsilon, o.epsilon);
+    cut(q.atoms.hydrophobic, o.hydrophobic); cut(q.atoms.atom_type, o.atom_type);
+    cut(q.atoms.residue_idx, o.residue_idx);
+    q.res_start.assign(1, 0);
+    for (uint32_t r = r0; r < r1; ++r) {
+      q.res_start.push_back(rs[r + 1] - a0);
+      q.ca_atom_idx.push_back(cai[r] - a0);
+      q.amino_acid.push_back(static_cast<AminoAcid>(aa_out[r]));
+    }
+    if (rotamers) {
+      if ((*rotamers).size() < abs.size()) (*rotamers).resize(abs.size());
+      (*rotamers)[a].assign(rot_out.begin() + r0, rot_out.begin() + r1);
+    }
+    abs[a] = std::move(q);
+  }
+  e.resize(abs.size());
+  return e;
 }
 
 }  // namespace binder

@@ -175,6 +175,48 @@ int main() {
     CHECK(ab.n_residues() == 6 && ab.res_start.back() == ab.n_atoms());
   });
 
+  run("diffusion::allatom_diffusion_step with moves, a batch (src/diffusion.rs:290-416)", [&] {
+    // an antigen chain and two binder chains 8 A beside it, 5 iterations with
+    // the reference's move rates raised so that moves happen
+    auto chain = [](float y0, int n, AminoAcid a) {
+      std::vector<float> xs, ys(n, y0), zs(n, 0.0f);
+      for (int i = 0; i < n; ++i) xs.push_back(i * 3.8f);
+      return protein_from_ca_trace(xs, ys, zs, std::vector<AminoAcid>(n, a));
+    };
+    const auto ag = chain(0.0f, 12, AminoAcid::Leu);
+    std::vector<AtomProtein> abs = {chain(8.0f, 6, AminoAcid::Ser), chain(-8.0f, 5, AminoAcid::Lys)};
+    dnab_langevin_params lp{};
+    check(dnab_langevin_default_params(&lp));
+    lp.n_steps = 5;
+    lp.with_noise = 1;
+    lp.seed = 12345;
+    dnab_mc_params mc{};
+    check(dnab_mc_default_params(&mc));
+    mc.mutation_p = 0.3f;
+    mc.backbone_move_p = 0.2f;
+    auto run_once = [&]() {
+      std::vector<AtomProtein> b = abs;
+      std::vector<std::vector<uint8_t>> rot;
+      const auto e = allatom_diffusion_steps(gpu, ag, b, lp, &mc, &rot);
+      return std::make_tuple(b, rot, e);
+    };
+    const auto [b1, rot1, e1] = run_once();
+    const auto [b2, rot2, e2] = run_once();
+    CHECK(b1.size() == 2 && e1.size() == 2 && rot1.size() == 2);
+    for (size_t a = 0; a < b1.size(); ++a) {
+      CHECK(b1[a].n_residues() == abs[a].n_residues());
+      CHECK(b1[a].res_start.back() == b1[a].n_atoms());
+      for (size_t r = 0; r < b1[a].n_residues(); ++r)
+        CHECK(b1[a].res_start[r + 1] - b1[a].res_start[r] ==
+              static_cast<uint32_t>(dnab_residue_atom_count(static_cast<uint8_t>(b1[a].amino_acid[r]))));
+      // the returned score is allatom_energy of the returned protein
+      const double e = allatom_energy(gpu, ag, b1[a]);
+      CHECK(std::fabs(e - e1[a]) <= 1e-3 + 1e-4 * std::fabs(e));
+      // the same seed repeats exactly
+      CHECK(b1[a].atoms.x == b2[a].atoms.x && rot1[a] == rot2[a] && e1[a] == e2[a]);
+    }
+  });
+
   run("pdb::write_pdb_allatom / parse_pdb round trip (src/pdb.rs:19-57, 121-155)", [&] {
     auto p = protein_from_ca_trace({0.0f, 3.8f, 7.6f}, {1.0f, 1.0f, 1.0f}, {0.0f, 0.5f, 1.0f},
                                    {AminoAcid::Trp, AminoAcid::Gly, AminoAcid::His});
