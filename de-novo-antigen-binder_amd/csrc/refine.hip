@@ -271,10 +271,39 @@ __device__ inline void contribution2(const RefTables& T, const CaSet& g, uint32_
   block_sum2<NW>(ea, eb, red, lane, wv);
 }
 
-// Side chain of residue slots X/Y/Z (type aa, rotamer rot) from its
-// backbone atoms 0..nb-1 (rebuild_side_chain, src/allatom.rs:244-271).
-__device__ inline void side_chain(const RefTables& T, uint32_t aa, uint32_t rot, int nb, float* X,
-                                  float* Y, float* Z) {
+// ---- residue builds on a quad of lanes ------------------------------------
+// Lane q = 0, 1, 2 of a quad holds component q (x, y, z) of every vector and
+// lane 3 follows along without writing. A lane takes the components it needs
+// from the others by DPP quad permutes and applies the host builder's f32
+// operations in the host builder's order, so a residue equals geo::place's
+// bit for bit, with one division per lane where geo::normalize does three.
+constexpr int kQuadFrom0 = 0x00, kQuadFrom1 = 0x55, kQuadFrom2 = 0xAA;
+constexpr int kQuadNext = 0xC9;   // lane q reads lane (q + 1) mod 3; lane 3 itself
+constexpr int kQuadNext2 = 0xD2;  // lane q reads lane (q + 2) mod 3; lane 3 itself
+
+__device__ inline float q_normalize(float v) {
+  const float s = v * v;
+  float l = sqrtf((dpp_f32<kQuadFrom0>(s) + dpp_f32<kQuadFrom1>(s)) + dpp_f32<kQuadFrom2>(s));
+  if (l < 1e-8f) l = 1e-8f;
+  return v / l;
+}
+__device__ inline float q_cross(float a, float b) {
+  return dpp_f32<kQuadNext>(a) * dpp_f32<kQuadNext2>(b) -
+         dpp_f32<kQuadNext2>(a) * dpp_f32<kQuadNext>(b);
+}
+// geo::place, component q of D
+__device__ inline float q_place(float A, float B, float C, float bond_len, float sa, float ca,
+                                float sd, float cd) {
+  const float bc = q_normalize(B - C);
+  const float n = q_normalize(q_cross(B - A, bc));
+  const float m = q_cross(bc, n);
+  return C + bond_len * (-ca * bc + sa * cd * m + sa * sd * n);
+}
+// rebuild_side_chain (src/allatom.rs:244-271) on a quad: the side chain of
+// type aa, rotamer rot from the backbone atoms 0..nb-1; P = this lane's
+// component array of the residue
+__device__ inline void q_side_chain(const RefTables& T, uint32_t aa, uint32_t rot, int nb,
+                                    float* P, uint32_t q) {
   int n = nb;
   for (int k = 0; k < T.ndef[aa] && n < T.natoms[aa]; ++k) {
     const RefDef& d = T.defs[aa][k];
@@ -288,54 +317,24 @@ __device__ inline void side_chain(const RefTables& T, uint32_t aa, uint32_t rot,
         cd = 1.0f;
       }
     }
-    const geo::V3 p = geo::place({X[d.a], Y[d.a], Z[d.a]}, {X[d.b], Y[d.b], Z[d.b]},
-                                 {X[d.c], Y[d.c], Z[d.c]}, d.bond, d.sa, d.ca, sd, cd);
-    X[n] = p.x;
-    Y[n] = p.y;
-    Z[n] = p.z;
+    const float v = q_place(P[d.a], P[d.b], P[d.c], d.bond, d.sa, d.ca, sd, cd);
+    if (q < 3) P[n] = v;
     ++n;
   }
 }
-
-// A residue of type new_aa, rotamer new_rot, built anew on an ideal backbone
-// around the Calpha in slot 1 of x/y/z (the other-type branch of
-// mutate_residue, src/allatom.rs:316-384).
-__device__ inline void build_ideal(const RefTables& T, uint32_t new_aa, uint32_t new_rot, float* x,
-                                   float* y, float* z) {
+// The other-type branch of mutate_residue (src/allatom.rs:316-384) on a
+// quad: the ideal backbone around the Calpha in P[1] (geo::ideal_backbone on
+// every lane), then the side chain
+__device__ inline void q_build_ideal(const RefTables& T, uint32_t new_aa, uint32_t new_rot,
+                                     float* P, uint32_t q) {
+  const float c = P[1];
+  const float cx = dpp_f32<kQuadFrom0>(c), cy = dpp_f32<kQuadFrom1>(c), cz = dpp_f32<kQuadFrom2>(c);
   geo::V3 bb[5];
-  int nb = geo::ideal_backbone(x[1], y[1], z[1], new_aa == kGly, bb);
+  int nb = geo::ideal_backbone(cx, cy, cz, new_aa == kGly, bb);
   if (nb > T.natoms[new_aa]) nb = T.natoms[new_aa];
-  for (int k = 0; k < nb; ++k) {
-    x[k] = bb[k].x;
-    y[k] = bb[k].y;
-    z[k] = bb[k].z;
-  }
-  side_chain(T, new_aa, new_rot, nb, x, y, z);
-}
-
-// build_ideal on residue r's slots, which then has type new_aa.
-__device__ inline void rebuild_ideal(const RefTables& T, uint8_t* aa, uint8_t* rot, uint32_t r,
-                                     uint32_t new_aa, uint32_t new_rot, float* X, float* Y,
-                                     float* Z) {
-  aa[r] = static_cast<uint8_t>(new_aa);
-  rot[r] = static_cast<uint8_t>(new_rot);
-  build_ideal(T, new_aa, new_rot, X + r * kSlots, Y + r * kSlots, Z + r * kSlots);
-}
-
-// mutate_residue (src/allatom.rs:299-384) on one residue's slots: the same
-// type keeps the backbone and rebuilds the side chain; another type is built
-// anew on an ideal backbone around the Calpha.
-__device__ inline void mutate(const RefTables& T, uint8_t* aa, uint8_t* rot, uint32_t r,
-                              uint32_t new_aa, uint32_t new_rot, float* X, float* Y, float* Z) {
-  float* x = X + r * kSlots;
-  float* y = Y + r * kSlots;
-  float* z = Z + r * kSlots;
-  if (aa[r] == new_aa) {
-    rot[r] = static_cast<uint8_t>(new_rot);
-    side_chain(T, new_aa, new_rot, new_aa == kGly ? 4 : 5, x, y, z);
-    return;
-  }
-  rebuild_ideal(T, aa, rot, r, new_aa, new_rot, X, Y, Z);
+  for (int k = 0; k < nb; ++k)
+    if (q < 3) P[k] = q == 0 ? bb[k].x : q == 1 ? bb[k].y : bb[k].z;
+  q_side_chain(T, new_aa, new_rot, nb, P, q);
 }
 
 // disulfide_energy (src/allatom.rs:155-180) over the Cys SG slots (slot 5):
@@ -642,7 +641,7 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
   float* SY = SX + kSlots;
   float* SZ = SY + kSlots;
   uint16_t* cysl = reinterpret_cast<uint16_t*>(RP);
-  constexpr uint32_t kUndoThread = NW > 1 ? kWave : 1;
+  constexpr uint32_t kUndoThread = NW > 1 ? kWave : 4;  // first lane of the undo's quad
   float* gx = reinterpret_cast<float*>(red + 16);
   float* gy = gx + na_l;
   float* gz = gy + na_l;
@@ -854,20 +853,25 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
             SZ[q] = Z[r * kSlots + q];
           }
           __syncthreads();
-          if (tid == 0) {
-            if (rot_move) {  // apply_rotamer
+          if (tid < 4) {  // the move, on quad 0
+            const uint32_t q = tid;
+            float* P = (q == 1 ? Y : q == 2 ? Z : X) + r * kSlots;
+            if (rot_move || new_aa == old_aa)  // apply_rotamer, or mutate_residue to the same type
+              q_side_chain(T, old_aa, new_rot, old_aa == kGly ? 4 : 5, P, q);
+            else  // mutate_residue to another type
+              q_build_ideal(T, new_aa, new_rot, P, q);
+            if (q == 0) {
+              aa[r] = static_cast<uint8_t>(new_aa);
               rot[r] = static_cast<uint8_t>(new_rot);
-              side_chain(T, old_aa, new_rot, old_aa == kGly ? 4 : 5, X + r * kSlots,
-                         Y + r * kSlots, Z + r * kSlots);
-            } else {
-              mutate(T, aa, rot, r, new_aa, new_rot, X, Y, Z);
             }
           }
-          if (tid == kUndoThread) {
+          if (tid >= kUndoThread && tid < kUndoThread + 4) {  // the undo, on another quad
+            const uint32_t q = tid - kUndoThread;
+            float* P = q == 1 ? SY : q == 2 ? SZ : SX;
             if (rot_move || new_aa == old_aa)  // the old chi on the same backbone
-              side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, SX, SY, SZ);
+              q_side_chain(T, old_aa, old_rot, old_aa == kGly ? 4 : 5, P, q);
             else  // mutate_residue back: the old type on an ideal backbone
-              build_ideal(T, old_aa, old_rot, SX, SY, SZ);
+              q_build_ideal(T, old_aa, old_rot, P, q);
           }
           __syncthreads();
           const uint32_t ncys_new = ncys + (uniform_u32(aa[r]) == kCys) - (old_aa == kCys);
@@ -893,18 +897,20 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
         }
       }
     }
-    // the side chains the moves above decided, built in parallel
-    for (uint32_t r = tid; r < nr; r += kThreads) {
+    // the side chains the moves above decided, built in parallel, a quad of
+    // lanes per residue
+    for (uint32_t r = tid / 4; r < nr; r += kThreads / 4) {
+      const uint32_t q = tid & 3u;
+      float* P = (q == 1 ? Y : q == 2 ? Z : X) + r * kSlots;
       const uint8_t b = pend[r];
-      if (b == kBuildSide) {
-        side_chain(T, aa[r], rot[r], aa[r] == kGly ? 4 : 5, X + r * kSlots, Y + r * kSlots,
-                   Z + r * kSlots);
-      } else if (b == kBuildIdeal) {
-        rebuild_ideal(T, aa, rot, r, aa[r], rot[r], X, Y, Z);
-      }
-      pend[r] = kBuildNone;
+      const uint32_t ra = aa[r], rr = rot[r];
+      if (b == kBuildSide)
+        q_side_chain(T, ra, rr, ra == kGly ? 4 : 5, P, q);
+      else if (b == kBuildIdeal)
+        q_build_ideal(T, ra, rr, P, q);
     }
     __syncthreads();
+    for (uint32_t r = tid; r < nr; r += kThreads) pend[r] = kBuildNone;
     // 4. torsion moves (src/diffusion.rs:400-415)
     if (a.moves & DNAB_MC_BACKBONE) {
       for (uint32_t cb = 0; cb < nr; cb += kWave) {
