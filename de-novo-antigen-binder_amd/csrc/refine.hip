@@ -642,7 +642,8 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
   float* SZ = SY + kSlots;
   uint16_t* cysl = reinterpret_cast<uint16_t*>(RP);
   constexpr uint32_t kUndoThread = NW > 1 ? kWave : 4;  // first lane of the undo's quad
-  float* gx = reinterpret_cast<float*>(red + 16);
+  double* EC = red + 16;                           // 2 * max_res move energies
+  float* gx = reinterpret_cast<float*>(EC + 2 * a.max_res);
   float* gy = gx + na_l;
   float* gz = gy + na_l;
   float* gq = gz + na_l;
@@ -777,6 +778,45 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
     //    (the residues whose first draw asks for a move, from a ballot per 64,
     //    visited in order)
     if (a.moves & DNAB_MC_SIDECHAIN) {
+      // The energies of this step's mutations that involve no Cys, first and
+      // all at once: each depends only on its residue's Calpha and two types,
+      // which no decision of this phase changes (a residue moves once per
+      // step). Proposal k is scored by wave k mod NW, its FP64 sums inside
+      // that wave; EC holds (old type, new type) per residue.
+      if (a.finite) {
+        uint32_t k = 0;
+        for (uint32_t cb = 0; cb < nr; cb += kWave) {
+          bool mut = false;
+          if (cb + lane < nr) {
+            const uint32_t rl = cb + lane;
+            const float4 ul = U[2 * rl];
+            const bool in = iface[rl] != 0;
+            const uint32_t na = min(static_cast<uint32_t>(ul.z * static_cast<float>(kNumAminoAcids)),
+                                    static_cast<uint32_t>(kNumAminoAcids - 1));
+            mut = ul.x < (in ? a.mut_p * 3.0f : a.mut_p * 0.4f) &&
+                  !(ul.y < (in ? a.rot_p * 2.0f : a.rot_p)) && aa[rl] != kCys && na != kCys;
+          }
+          for (uint64_t mv = ballot64(mut); mv; mv &= mv - 1, ++k) {
+            if (k % NW != wv) continue;
+            const uint32_t r = cb + static_cast<uint32_t>(__builtin_ctzll(mv));
+            const uint32_t oa = uniform_u32(aa[r]);
+            const uint32_t na =
+                min(static_cast<uint32_t>(uniform_f32(U[2 * r].z) * static_cast<float>(kNumAminoAcids)),
+                    static_cast<uint32_t>(kNumAminoAcids - 1));
+            double eo, en;
+            contrib<true>(T, g, aa, X, Y, Z, r, r + 1, oa, na, lane, 0, 1, lbuf, lcap, eo, en);
+            eo = wave_sum_f64(eo);
+            en = wave_sum_f64(en);
+            if (lane == 0) {
+              EC[2 * r] = eo;
+              EC[2 * r + 1] = en;
+            }
+          }
+        }
+        __syncthreads();
+      }
+      double ds_fast = 0.0;  // the disulfide term while no Cys move intervenes
+      bool ds_known = false;
       for (uint32_t cb = 0; cb < nr; cb += kWave) {
         bool want = false;
         if (cb + lane < nr) {
@@ -813,17 +853,19 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
             // finite; a non-finite input takes the literal path below.)
             bool accept = true;
             if (!rot_move) {
-              double e_old, e_new;
-              contrib<true>(T, g, aa, X, Y, Z, r, r + 1, old_aa, new_aa, lane, wv, NW, lbuf,
-                            lcap, e_old, e_new);
-              block_sum2<NW>(e_old, e_new, red, lane, wv);
-              const double ds = ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0;
-              const double delta = (e_new + ds) - (e_old + ds);
+              const double e_old = EC[2 * r], e_new = EC[2 * r + 1];
+              if (!ds_known) {
+                ds_fast = ncys >= 2 ? disulfide(aa, X, Y, Z, nr, lane, cysl) : 0.0;
+                ds_known = true;
+              }
+              const double delta = (e_new + ds_fast) - (e_old + ds_fast);
               accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
             }
             // The build itself waits for the end of the phase: until then
             // only this residue's Calpha (unchanged by any build) and type are
-            // read, and no other move touches it this step.
+            // read, and no other move touches it this step. (No barrier: what
+            // thread 0 writes here is read after the phase, and no write here
+            // changes which residues are Cys.)
             if (tid == 0) {
               if (rot_move) {  // apply_rotamer: the new chi on the current backbone
                 rot[r] = static_cast<uint8_t>(new_rot);
@@ -836,7 +878,6 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
                 pend[r] = new_aa == old_aa ? kBuildSide : kBuildIdeal;
               }
             }
-            __syncthreads();
             continue;
           }
           // A Cys gains, loses or moves its SG: the disulfide term needs the
@@ -880,6 +921,7 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
           const double delta = new_e - old_e;
           const bool accept = delta <= 0.0 || static_cast<double>(u.w) < exp(-delta / temp);
           __syncthreads();
+          ds_known = false;
           if (accept) {
             ncys = ncys_new;
           } else {
@@ -1187,7 +1229,7 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
   const uint32_t nw = mc->waves ? mc->waves : (nc <= cu ? 8u : nc <= 2 * cu ? 4u : 2u);
   const size_t fixed = (sizeof(RefTables) + 15) / 16 * 16 +
                        (2 * kWave * nw + 8) * sizeof(float4) + kWave * nw * sizeof(uint32_t) +
-                       static_cast<size_t>(max_res) * (2 * 16 + (3 * kSlots + 3) * sizeof(float) + 4);
+                       static_cast<size_t>(max_res) * (2 * 16 + 2 * 8 + (3 * kSlots + 3) * sizeof(float) + 4);
   const size_t ncells = ag->ca_grid.n ? ag->ca_grid.ncodes + 1 : 0;
   const size_t ag_bytes = n_ag * (16 + 6 * 4 + 4 + 1) + ncells * 4;
   auto list_bytes = [](size_t cap) { return cap * (sizeof(float4) + sizeof(uint32_t)) + cap / 8; };
