@@ -863,9 +863,11 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
             }
             // The build itself waits for the end of the phase: until then
             // only this residue's Calpha (unchanged by any build) and type are
-            // read, and no other move touches it this step. (No barrier: what
-            // thread 0 writes here is read after the phase, and no write here
-            // changes which residues are Cys.)
+            // read, and no other move touches it this step. No barrier per
+            // move: what thread 0 writes here (pend, aa, rot) is published to
+            // the other waves by the barrier that closes the phase, before the
+            // build loop reads it; no write here changes which residues are
+            // Cys.
             if (tid == 0) {
               if (rot_move) {  // apply_rotamer: the new chi on the current backbone
                 rot[r] = static_cast<uint8_t>(new_rot);
@@ -939,6 +941,9 @@ __global__ void __launch_bounds__(NW * 64) k_refine(RefineArgs a) {
         }
       }
     }
+    // Close the phase: thread 0's pend/aa/rot writes of the fast moves become
+    // visible to every wave before the build loop reads them.
+    __syncthreads();
     // the side chains the moves above decided, built in parallel, a quad of
     // lanes per residue
     for (uint32_t r = tid / 4; r < nr; r += kThreads / 4) {
