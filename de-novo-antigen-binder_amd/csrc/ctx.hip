@@ -1,7 +1,7 @@
 // Context, errors, transfers and kernel timing.
 //
-// dnab_ctx_create replaces GpuContext::try_init (src/gpu.rs:143-243): where
-// the reference returns None when no adapter exists (src/gpu.rs:153-159) and
+// dnab_ctx_create replaces GpuContext::try_init (src/gpu.rs:55-57; async body 59-155): where
+// the reference returns None when no adapter exists (src/gpu.rs:65-71) and
 // the caller falls back to the CPU (src/main.rs:319-327), this returns
 // DNAB_ERR_NO_DEVICE and leaves the fallback decision to the caller.
 #include <hip/hip_runtime.h>

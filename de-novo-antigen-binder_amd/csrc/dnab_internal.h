@@ -114,6 +114,7 @@ struct PairHandover {
   uint32_t* n_list;   // cleared by the sort scan (or a reset) before each launch
   uint2* chunks;      // per tile: kHandoverCap entries
   long long* part;    // 5 x n_slots partial sums (energy, sum |e|, fx, fy, fz)
+  uint32_t* status;   // n_slots: the partial sums' sticky status bits (pair_device.h)
   uint32_t n_slots;
 };
 
@@ -125,10 +126,15 @@ struct PairTuning {
   uint32_t rest_waves;        // waves per workgroup for handed tiles (4, 8, 16)
 };
 
+// Antigen atom record of the pair kernels, 32 bytes: position and 332 q;
+// Rmin/2, sqrt(eps), the raw eps (clash pairs are evaluated from it in
+// FP64), and a tag holding the AtomType and the hydrophobic flag (ag_tag,
+// pair_device.h).
 struct AgAtom {
   float x, y, z, q332;
-  float rmh, sqe, hyd, type;  // type: AtomType as a float, -1 when not given
+  float rmh, sqe, eps, tag;
 };
+static_assert(sizeof(AgAtom) == 32, "AgAtom is two float4 loads");
 
 constexpr int kNumAtomTypesDev = 16;  // AtomType values (src/amber.rs:13-32)
 // Per-atom and per-candidate energy terms of a breakdown evaluation: LJ,
@@ -269,6 +275,7 @@ struct dnab_population {
   bool ho_dirty = false;                  // count left set by the last pair launch
   dnab::DevBuf<uint2> ho_chunks;          // handover: tile_cap * kHandoverCap
   dnab::DevBuf<long long> ho_part;        // handover: 5 * tile_cap * 64
+  dnab::DevBuf<uint32_t> ho_status;       // handover: tile_cap * 64 status words
   dnab::DevBuf<uint32_t> ag_tile_start;  // n_ag + 1
   bool sorted = false;
   // evaluation outputs
@@ -298,6 +305,11 @@ struct dnab_population {
   // candidate mask: only masked-in candidates are sorted and scored
   dnab::DevBuf<uint8_t> cand_mask;
   bool masked = false;
+  // some candidate atom has a non-finite coordinate: the population can be
+  // evaluated (its candidates score NaN, as the reference's sums do) but not
+  // stepped, refined or screened
+  bool nonfinite = false;
+  dnab::DevBuf<uint8_t> cand_status;  // per candidate: DNAB_STATUS_* of the last evaluation
   // prescreen
   dnab::DevBuf<uint16_t> feat;   // n_cand x 3360 bf16
   dnab::DevBuf<float> scores;    // n_cand
@@ -334,7 +346,8 @@ int dnab_set_error(dnab_ctx* ctx, int code, const char* fmt, ...);
 namespace dnab {
 // internal evaluate flag: the counting variant of the E+F kernel
 constexpr unsigned kEvalCount = 1u << 8;
-constexpr uint32_t kExcluded = 0xFFFFFFFFu;
+constexpr uint32_t kExcluded = 0xFFFFFFFFu;   // sort code: no pair, zero outputs
+constexpr uint32_t kNonFinite = 0xFFFFFFFEu;  // sort code: a NaN coordinate, NaN outputs
 // grid.hip
 int grid_build_device(dnab_ctx* ctx, GridDev& g, const float* d_x, const float* d_y,
                       const float* d_z, size_t n);
@@ -358,4 +371,8 @@ int population_pair(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
 int discard_pending_counts(dnab_ctx* ctx, dnab_population* pop);
 int population_count(dnab_ctx* ctx, dnab_population* pop);
 int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
+// DNAB_ERR_INVALID naming `what` when the population has a non-finite coordinate
+int require_finite(dnab_ctx* ctx, const dnab_population* pop, const char* what);
+// parameters of an atom cloud: finite charge, r_min_half >= 0 and epsilon >= 0
+bool atom_params_ok(const dnab_atom_cloud* c);
 }  // namespace dnab

@@ -608,6 +608,7 @@ int dnab_dataset_size(const dnab_dataset* d, size_t* n) {
 
 int dnab_dataset_add_population(dnab_ctx* ctx, dnab_dataset* d, dnab_population* pop) {
   if (!ctx || !d || !pop || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "dataset_add_population"));
   if (d->size + pop->n_cand > d->capacity)
     return dnab_set_error(ctx, DNAB_ERR_CAPACITY, "dataset full (%zu + %zu > %zu)", d->size,
                           pop->n_cand, d->capacity);
@@ -683,6 +684,7 @@ int dnab_mlp_train(dnab_ctx* ctx, dnab_mlp* n, dnab_dataset* d, const dnab_train
 
 int dnab_population_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* host) {
   if (!ctx || !pop || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "population_features"));
   (void)hipSetDevice(ctx->device);
   DNAB_ALLOC_CHECK(ctx, pop->feat.ensure(pop->n_cand * kD));
   DNAB_RETURN_IF(population_features(ctx, pop, pop->feat.p));
@@ -693,6 +695,7 @@ int dnab_population_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* host
 
 int dnab_mlp_predict(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop, float* scores) {
   if (!ctx || !n || !pop || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "mlp_predict"));
   (void)hipSetDevice(ctx->device);
   DNAB_RETURN_IF(predict_device(ctx, n, pop));
   if (scores) DNAB_RETURN_IF(download(ctx, scores, pop->scores.p, pop->n_cand * 4));
@@ -703,6 +706,7 @@ int dnab_mlp_predict(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop, float* sc
 int dnab_prescreen(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop, uint32_t n_keep,
                    uint32_t top_k, uint32_t* idx, double* energy, uint32_t* n_out) {
   if (!ctx || !n || !pop || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "prescreen"));
   if (n_keep == 0 || top_k == 0 || n_keep > 2048 || top_k > n_keep)
     return dnab_set_error(ctx, DNAB_ERR_INVALID,
                           "prescreen: need 1 <= top_k <= n_keep <= 2048 (got %u, %u)", top_k, n_keep);
@@ -749,6 +753,7 @@ int dnab_prescreen(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop, uint32_t n_
 int dnab_exact_topk(dnab_ctx* ctx, dnab_population* pop, uint32_t top_k, uint32_t* idx,
                     double* energy, uint32_t* n_out) {
   if (!ctx || !pop || pop->ctx != ctx || top_k == 0 || top_k > 4096) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "exact_topk"));
   (void)hipSetDevice(ctx->device);
   const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
   const uint32_t k = std::min(top_k, nc);

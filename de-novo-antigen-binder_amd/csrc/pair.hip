@@ -110,8 +110,8 @@ __global__ void k_ag_gather(const uint32_t* __restrict__ sorted, uint32_t n,
   a.q332 = 332.0f * q[i];
   a.rmh = rmh[i];
   a.sqe = sqrtf(eps[i]);
-  a.hyd = hyd[i] ? 1.0f : 0.0f;
-  a.type = type ? static_cast<float>(type[i]) : -1.0f;  // AtomType, -1 when not given
+  a.eps = eps[i];
+  a.tag = ag_tag(hyd[i] != 0, type ? static_cast<int>(type[i]) : -1);  // AtomType, when given
   out[s] = a;
   raw[s] = make_float2(q[i], eps[i]);
 }
@@ -141,7 +141,7 @@ k_sort_count(const AgView* __restrict__ views, const uint32_t* __restrict__ cand
       code = sort_code_of(V, p.x, p.y, p.z);
     }
     code_out[j] = code;
-    if (code == kExcluded) zero_atom_outputs(j, force, e_atom, bd_atom, f64_atom);
+    if (code >= kNonFinite) fill_atom_outputs(j, code, V, force, e_atom, bd_atom, f64_atom);
     else rank_out[j] = atomicAdd(&count[code], 1u);
   }
 }
@@ -223,7 +223,7 @@ __global__ void k_sort_scatter(const uint32_t* __restrict__ code_in,
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t code = code_in[j];
-  if (code != kExcluded) {
+  if (code < kNonFinite) {
     const uint32_t slot = start[code] + rank[j];
     perm[slot] = j;
     spos[slot] = pos[j];
@@ -391,14 +391,14 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
             A.q332 = readlane_f32(A0.w, k);
             A.rmh = readlane_f32(A1.x, k);
             A.sqe = readlane_f32(A1.y, k);
-            A.hyd = readlane_f32(A1.z, k);
-            A.type = BREAKDOWN ? readlane_f32(A1.w, k) : -1.0f;
-            // Pair predicate in f32, exactly the reference's (src/energy.rs:279-284).
+            A.tag = readlane_f32(A1.w, k);
+            // Pair predicate in f32: the reference's skip test, negated
+            // (src/energy.rs:279-284).
             const float dx = xb - A.x;
             const float dy = yb - A.y;
             const float dz = zb - A.z;
             const float r2 = (dx * dx + dy * dy) + dz * dz;
-            const bool in = valid && r2 <= rc2 && r2 >= kMinRSq;
+            const bool in = valid && !(r2 > rc2 || r2 < kMinRSq);
             if (COUNT) {
               n_tested += valid ? 1u : 0u;
               n_in += in ? 1u : 0u;
@@ -418,7 +418,7 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
               const float flj = ((A.sqe * sqe12b) * (s12 - s6)) * rinv2;
               f = __builtin_fmaf(ecl, rinv2, flj);
             }
-            if (A.hyd != 0.0f) {
+            if (tag_hyd(A.tag)) {
               const bool h = in && hydb && r2 < kHydroSq;
               nh += h ? 1u : 0u;
               if (FORCES) f = h ? __builtin_fmaf(-0.1f, rinv, f) : f;
@@ -436,8 +436,9 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
               two_sum(cos_, ecl, s, e);
               cos_ = s;
               coc += e;
-              if (A.type >= 0.0f && typed_b) {  // both atom types known
-                const uint32_t ta = static_cast<uint32_t>(A.type);
+              const int ta_i = tag_type(A.tag);
+              if (ta_i >= 0 && typed_b) {  // both atom types known
+                const uint32_t ta = static_cast<uint32_t>(ta_i);
                 // H-bond inside the pair predicate (src/energy.rs:303-307)
                 const bool hb_types = (((kHbondDonorMask >> tb) & (kHbondAcceptorMask >> ta)) |
                                        ((kHbondAcceptorMask >> tb) & (kHbondDonorMask >> ta))) & 1u;
@@ -445,8 +446,8 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
                 two_sum(hbs, ehb, s, e);
                 hbs = s;
                 hbc += e;
-                // EEF1 with its own cutoff (src/solvation.rs:71-78)
-                const bool sv = valid && r2 >= kMinRSq && r2 < kSolvCutoffSq;
+                // EEF1 with its own skip test, negated (src/solvation.rs:71-78)
+                const bool sv = valid && !(r2 >= kSolvCutoffSq || r2 < kMinRSq);
                 const float esv = sv ? solvation_term(ta, tb, r2) : 0.0f;
                 two_sum(svs, esv, s, e);
                 svs = s;
@@ -566,11 +567,13 @@ k_pair_tile(const AgView* __restrict__ views, uint32_t n_ag,
 // kernels meet through plain stores and loads in device memory (the kernel
 // boundary orders them), with no device-scope atomics or fences, which on
 // this GPU would reach across the XCDs' separate L2 caches.
+// An atom's outputs from its integer sums: a component whose sum saturated
+// is written as +-inf (fix_value), so the candidate's energy and status show it.
 __device__ inline void write_fixed(const FixAcc& acc, uint32_t j, TileOut out) {
-  out.e_atom[j] = static_cast<double>(acc.e) * kFixUnit;
-  out.force[j] = make_float4(static_cast<float>(static_cast<double>(acc.x) * kFixUnit),
-                             static_cast<float>(static_cast<double>(acc.y) * kFixUnit),
-                             static_cast<float>(static_cast<double>(acc.z) * kFixUnit),
+  out.e_atom[j] = fix_value(acc.e, acc.st, kStE);
+  out.force[j] = make_float4(static_cast<float>(fix_value(acc.x, acc.st, kStX)),
+                             static_cast<float>(fix_value(acc.y, acc.st, kStY)),
+                             static_cast<float>(fix_value(acc.z, acc.st, kStZ)),
                              static_cast<float>(static_cast<double>(acc.a) * kFixUnit));
 }
 
@@ -604,6 +607,7 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
   const float rmhb = Q.x;
   const float sqeb = sqrtf(Q.y);
   const float sqe12b = 12.0f * sqeb;
+  const float epsb = Q.y;
   const bool hydb = Q.z != 0.0f;
   const float rc2_lane = valid ? rc2 : -1.0f;  // an empty slot has no pair
 
@@ -700,8 +704,8 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
         if (n_s == 0) continue;
         taken_atoms += n_s;
         stage_chunk(s_atoms[wave], mask, lane, L0[u], L1[u]);
-        chunk_pairs<FORCES, COUNT>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb,
-                                   rc2_lane, valid, acc, n_in, n_tested);
+        chunk_pairs<FORCES, COUNT>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
+                                   hydb, rc2_lane, valid, acc, n_in, n_tested);
       }
     }
     if (k_hand < total) {
@@ -742,6 +746,7 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
   ho.part[2 * ns + slot_id] = acc.x;
   ho.part[3 * ns + slot_id] = acc.y;
   ho.part[4 * ns + slot_id] = acc.z;
+  ho.status[slot_id] = acc.st;
 }
 
 // The chunks a tile's wave handed over: WAVES waves take them from an
@@ -755,6 +760,7 @@ k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
             PairHandover ho) {
   __shared__ float4 s_atoms[WAVES][kWave][2];
   __shared__ long long s_sum[5][kWave];
+  __shared__ uint32_t s_st[kWave];
   __shared__ uint32_t s_next;
   const uint32_t n_heavy = min(*ho.n_list, ho.n_slots / kWave);
   for (uint32_t h = blockIdx.x; h < n_heavy; h += gridDim.x) {  // workgroup-uniform
@@ -774,6 +780,7 @@ k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
   const float rmhb = Q.x;
   const float sqeb = sqrtf(Q.y);
   const float sqe12b = 12.0f * sqeb;
+  const float epsb = Q.y;
   const bool hydb = Q.z != 0.0f;
   const float rc2_lane = valid ? rc2 : -1.0f;
   const float inf = __builtin_inff();
@@ -785,6 +792,7 @@ k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
   const float hiz = wave_reduce_f32<OpMax>(valid ? zb : -inf);
   __syncthreads();  // the previous tile's LDS is no longer read
   for (uint32_t i = tid; i < 5 * kWave; i += WAVES * kWave) s_sum[i >> 6][i & 63u] = 0;
+  if (tid < kWave) s_st[tid] = 0;
   if (tid == 0) s_next = 0;
   __syncthreads();
 
@@ -814,8 +822,8 @@ k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
     const uint32_t n_s = static_cast<uint32_t>(__builtin_popcountll(mask));
     if (n_s) {
       stage_chunk(s_atoms[wave], mask, lane, A0, A1);
-      chunk_pairs<FORCES, COUNT>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb,
-                                 rc2_lane, valid, acc, n_in, n_tested);
+      chunk_pairs<FORCES, COUNT>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
+                                 hydb, rc2_lane, valid, acc, n_in, n_tested);
     }
     k = k2;
     ch = ch2;
@@ -828,6 +836,7 @@ k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
     atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[3][lane]), static_cast<unsigned long long>(acc.y));
     atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[4][lane]), static_cast<unsigned long long>(acc.z));
   }
+  if (acc.st) atomicOr(&s_st[lane], acc.st);
   if (COUNT) {
     const uint32_t ai = wave_sum_u32(n_in), at = wave_sum_u32(n_tested);
     if (lane == 0 && (ai | at)) {
@@ -844,11 +853,13 @@ k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
     tot.x = s_sum[2][lane] + ho.part[2 * ns + slot_id];
     tot.y = s_sum[3][lane] + ho.part[3 * ns + slot_id];
     tot.z = s_sum[4][lane] + ho.part[4 * ns + slot_id];
-    tot.e = min(max(tot.e, -kFixLim), kFixLim);
-    tot.a = min(max(tot.a, -kFixLim), kFixLim);
-    tot.x = min(max(tot.x, -kFixLim), kFixLim);
-    tot.y = min(max(tot.y, -kFixLim), kFixLim);
-    tot.z = min(max(tot.z, -kFixLim), kFixLim);
+    tot.st = s_st[lane] | ho.status[slot_id];
+    uint32_t ignore = 0;
+    tot.e = fix_add(tot.e, 0, tot.st, kStE);
+    tot.a = fix_add(tot.a, 0, ignore, 0u);
+    tot.x = fix_add(tot.x, 0, tot.st, kStX);
+    tot.y = fix_add(tot.y, 0, tot.st, kStY);
+    tot.z = fix_add(tot.z, 0, tot.st, kStZ);
     write_fixed(tot, j, out);
   }
   }  // next heavy tile
@@ -928,7 +939,7 @@ k_pair_tile_f64(const AgView* __restrict__ views, uint32_t n_ag,
             const AgAtom A = atoms[base + k];
             const float dx = xb - A.x, dy = yb - A.y, dz = zb - A.z;
             const float r2 = (dx * dx + dy * dy) + dz * dz;
-            if (!(valid && r2 <= rc2 && r2 >= kMinRSq)) continue;
+            if (!valid || r2 > rc2 || r2 < kMinRSq) continue;
             const float2 rw = raw[base + k];
             const double ddx = static_cast<double>(xb) - static_cast<double>(A.x);
             const double ddy = static_cast<double>(yb) - static_cast<double>(A.y);
@@ -945,7 +956,7 @@ k_pair_tile_f64(const AgView* __restrict__ views, uint32_t n_ag,
             const double ecl = 332.0 * qb * static_cast<double>(rw.x) / r;
             co += ecl;
             f += ecl / r2d;
-            if (A.hyd != 0.0f && hydb && r2 < kHydroSq) {
+            if (tag_hyd(A.tag) && hydb && r2 < kHydroSq) {
               ++nh;
               f += -0.1 / r;
             }
@@ -1132,6 +1143,7 @@ static PairHandover pair_handover(dnab_population* pop) {
   h.n_list = pop->ho_list.p + pop->tile_cap;
   h.chunks = pop->ho_chunks.p;
   h.part = pop->ho_part.p;
+  h.status = pop->ho_status.p;
   h.n_slots = static_cast<uint32_t>(pop->tile_cap * kWave);
   return h;
 }
@@ -1251,6 +1263,57 @@ bool atom_cloud_ok(const dnab_atom_cloud* c) {
                              c->hydrophobic));
 }
 
+}  // namespace
+
+namespace dnab {
+// Finite charges and non-negative finite r_min_half and epsilon: the pair
+// kernels take sqrt(epsilon), and a clash pair's FP64 terms are finite for
+// any such values, so a NaN sum can only come from a non-finite coordinate.
+bool atom_params_ok(const dnab_atom_cloud* c) {
+  for (size_t i = 0; i < c->n; ++i)
+    if (!std::isfinite(c->charge[i]) || !(c->r_min_half[i] >= 0.0f) ||
+        !std::isfinite(c->r_min_half[i]) || !(c->epsilon[i] >= 0.0f) ||
+        !std::isfinite(c->epsilon[i]))
+      return false;
+  return true;
+}
+
+int require_finite(dnab_ctx* ctx, const dnab_population* pop, const char* what) {
+  if (!pop->nonfinite) return DNAB_OK;
+  return dnab_set_error(ctx, DNAB_ERR_INVALID,
+                        "%s: the population has a non-finite coordinate (evaluation only)", what);
+}
+}  // namespace dnab
+
+namespace {
+
+// Per-candidate status of the last evaluation from its per-atom outputs:
+// DNAB_STATUS_NAN when the energy sum is NaN (a non-finite coordinate),
+// DNAB_STATUS_SATURATED when an atom's energy or force sum reached the
+// fixed-point bound (written as +-inf).
+__global__ void __launch_bounds__(256)
+k_status(const uint32_t* __restrict__ cand_atom_off, uint32_t n_cand,
+         const double* __restrict__ e_atom, const float4* __restrict__ force, int with_force,
+         uint8_t* __restrict__ status) {
+  const uint32_t c = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  if (c >= n_cand) return;
+  bool nan = false, inf = false;
+  for (uint32_t k = cand_atom_off[c] + lane; k < cand_atom_off[c + 1]; k += kWave) {
+    const double e = e_atom[k];
+    nan = nan || isnan(e);
+    inf = inf || isinf(e);
+    if (with_force) {
+      const float4 f = force[k];
+      inf = inf || isinf(f.x) || isinf(f.y) || isinf(f.z);
+    }
+  }
+  const bool any_nan = ballot64(nan) != 0, any_inf = ballot64(inf) != 0;
+  if (lane == 0)
+    status[c] = static_cast<uint8_t>((any_nan ? DNAB_STATUS_NAN : 0) |
+                                     (any_inf ? DNAB_STATUS_SATURATED : 0));
+}
+
 template <class T>
 int up(dnab_ctx* ctx, DevBuf<T>& b, const T* src, size_t n) {
   DNAB_ALLOC_CHECK(ctx, b.alloc(n));
@@ -1303,6 +1366,9 @@ int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* d, dnab_antigen*
     dnab_antigen_destroy(ag);
     return rc;
   };
+  if (!atom_params_ok(&c))
+    return fail(dnab_set_error(ctx, DNAB_ERR_INVALID,
+                               "antigen: non-finite charge, or negative r_min_half or epsilon"));
   // bounding box of the atoms (host)
   if (n) {
     for (int a = 0; a < 3; ++a) {
@@ -1413,10 +1479,13 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   } else if (na) {
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: atoms without residues");
   }
+  if (!atom_params_ok(&d->atoms))
+    return dnab_set_error(ctx, DNAB_ERR_INVALID,
+                          "population: non-finite charge, or negative r_min_half or epsilon");
+  bool nonfinite = false;  // allowed: such a candidate evaluates to NaN
   for (size_t i = 0; i < na; ++i)
-    if (!std::isfinite(d->atoms.x[i]) || !std::isfinite(d->atoms.y[i]) ||
-        !std::isfinite(d->atoms.z[i]))
-      return dnab_set_error(ctx, DNAB_ERR_INVALID, "population: non-finite coordinate");
+    nonfinite = nonfinite || !std::isfinite(d->atoms.x[i]) || !std::isfinite(d->atoms.y[i]) ||
+                !std::isfinite(d->atoms.z[i]);
   if (d->atoms.atom_type)
     for (size_t i = 0; i < na; ++i)
       if (d->atoms.atom_type[i] >= kNumAtomTypesDev)
@@ -1426,6 +1495,7 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   if (!pop) return DNAB_ERR_ALLOC;
   pop->ctx = ctx_retain(ctx);
   pop->n_cand = nc;
+  pop->nonfinite = nonfinite;
   pop->n_res = nr;
   pop->n_atoms = na;
   pop->n_ag = n_antigens;
@@ -1585,13 +1655,15 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
       pop->ho_list.alloc(pop->tile_cap + 4) != hipSuccess ||
       pop->ho_chunks.alloc(pop->tile_cap * kHandoverCap) != hipSuccess ||
       pop->ho_part.alloc(5 * pop->tile_cap * kWave) != hipSuccess ||
+      pop->ho_status.alloc(pop->tile_cap * kWave) != hipSuccess ||
       pop->ag_tile_start.alloc(n_antigens + 1) != hipSuccess || pop->force.alloc(na) != hipSuccess ||
       pop->e_atom.alloc(na) != hipSuccess || pop->bd_atom.alloc(kBdTerms * na) != hipSuccess ||
       pop->f64_atom.alloc(3 * na) != hipSuccess || pop->energy.alloc(nc) != hipSuccess ||
       pop->err_bound.alloc(nc) != hipSuccess || pop->breakdown.alloc(kBdTerms * nc) != hipSuccess ||
       pop->disulfide.alloc(nc) != hipSuccess ||
       pop->res_delta.alloc(nr) != hipSuccess || pop->step_counter.alloc(4) != hipSuccess ||
-      pop->pair_stats.alloc(2) != hipSuccess || pop->cand_mask.alloc(nc) != hipSuccess)
+      pop->pair_stats.alloc(2) != hipSuccess || pop->cand_mask.alloc(nc) != hipSuccess ||
+      pop->cand_status.alloc(nc) != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
   {
     std::vector<uint8_t> ones(nc ? nc : 1, 1);
@@ -1646,6 +1718,20 @@ int dnab_population_read_energies(dnab_ctx* ctx, const dnab_population* pop, dou
   DNAB_RETURN_IF(download(ctx, energy, pop->energy.p, pop->n_cand * sizeof(double)));
   if (err_bound) DNAB_RETURN_IF(download(ctx, err_bound, pop->err_bound.p, pop->n_cand * sizeof(float)));
   return DNAB_OK;
+}
+
+int dnab_population_read_status(dnab_ctx* ctx, dnab_population* pop, uint8_t* status) {
+  if (!ctx || !pop || pop->ctx != ctx || (pop->n_cand && !status)) return DNAB_ERR_INVALID;
+  if (!pop->evaluated) return dnab_set_error(ctx, DNAB_ERR_INVALID, "population not evaluated");
+  const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
+  if (!nc) return DNAB_OK;
+  const bool f64 = (pop->last_flags & DNAB_EVAL_FP64) != 0;
+  const bool forces = (pop->last_flags & DNAB_EVAL_FORCES) != 0 && !f64;
+  hipLaunchKernelGGL(k_status, dim3(nblocks(nc, 4)), dim3(256), 0, ctx->stream,
+                     pop->cand_atom_off.p, nc, pop->e_atom.p, pop->force.p, forces ? 1 : 0,
+                     pop->cand_status.p);
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  return download(ctx, status, pop->cand_status.p, nc);
 }
 
 int dnab_population_read_breakdown(dnab_ctx* ctx, const dnab_population* pop, double* lj,
@@ -1741,15 +1827,16 @@ int dnab_population_write_coords(dnab_ctx* ctx, dnab_population* pop, const floa
   const size_t n = pop->n_atoms;
   std::vector<float4> h(n);
   DNAB_RETURN_IF(download(ctx, h.data(), pop->pos.p, n * sizeof(float4)));
+  bool nonfinite = false;  // allowed: such a candidate evaluates to NaN
   for (size_t i = 0; i < n; ++i) {
-    if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]))
-      return dnab_set_error(ctx, DNAB_ERR_INVALID, "write_coords: non-finite coordinate");
+    nonfinite = nonfinite || !std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]);
     h[i].x = x[i];
     h[i].y = y[i];
     h[i].z = z[i];
   }
   DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
   DNAB_RETURN_IF(upload(ctx, pop->pos.p, h.data(), n * sizeof(float4)));
+  pop->nonfinite = nonfinite;
   pop->evaluated = false;
   DNAB_RETURN_IF(population_sort(ctx, pop));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1777,6 +1864,7 @@ int dnab_population_set_tuning(dnab_ctx* ctx, dnab_population* pop, const dnab_p
 int dnab_population_autotune(dnab_ctx* ctx, dnab_population* pop, uint32_t reps,
                              dnab_pair_tuning* best, double* best_ms) {
   if (!ctx || !pop || pop->ctx != ctx || reps == 0) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "autotune"));
   (void)hipSetDevice(ctx->device);
   pop->masked = false;
   if (!pop->sorted) DNAB_RETURN_IF(population_sort(ctx, pop));

@@ -8,6 +8,18 @@
 // (atom, chunk) sum is formed in FP32 in chunk order and rounded once to a
 // 64-bit integer in units of 2^-20, so sums over chunks are exact and do not
 // depend on how chunks are assigned to waves or kernels.
+//
+// Clash pairs. A pair deep inside the repulsive wall, (R/r)^2 > kHotS2, is
+// evaluated in FP64 from the exact coordinate differences and the raw
+// epsilons: its LJ terms reach 1e10 kcal/mol and 1e11 kcal/mol/A, where the
+// FP32 terms' own rounding (~1e-6 of each) would exceed the tolerance of a
+// force component in which two such pairs nearly cancel. Every other pair has
+// |F| below ~30 kcal/mol/A and stays in FP32.
+//
+// Saturation. An integer sum is bounded by +-2^59 units (+-5.5e11); a sum
+// that reaches the bound is clamped and its component is marked in the
+// accumulator's sticky status, which the writers turn into +-inf, so a
+// saturated result is never reported as a finite number.
 #pragma once
 #include "device_common.h"
 
@@ -15,45 +27,83 @@ namespace dnab {
 
 constexpr float kMinRSq = 0.25f;   // src/energy.rs:17
 constexpr float kHydroSq = 36.0f;  // src/energy.rs:20
+constexpr float kHotS2 = 1.5f;     // (R/r)^2 above which a pair is evaluated in FP64
 constexpr int kLoadAhead = 4;   // chunk loads in flight per round trip
 constexpr double kFixScale = 1048576.0;          // 2^20
 constexpr double kFixUnit = 1.0 / 1048576.0;
 constexpr long long kFixLim = 1LL << 59;         // accumulator bound; sums of 4 stay below 2^61
+// error estimate per energy rounding, in units of sum |e| (times 32 ulp(1) = 2^-19
+// gives 2^-21, half a fixed-point unit)
+constexpr float kRoundAbs = 0.25f;
 
-__device__ inline long long fix_round(double v) {
-  const double lim = static_cast<double>(kFixLim);
-  v = fmin(fmax(v * kFixScale, -lim), lim);   // NaN -> -lim (fmax drops NaN)
-  return __double2ll_rn(v);
+// Sticky status bits of an integer accumulator: the component reached the
+// bound (energy, fx, fy, fz), or a non-finite value was rounded.
+constexpr uint32_t kStE = 1u, kStX = 2u, kStY = 4u, kStZ = 8u, kStNaN = 16u;
+
+// The antigen atom record's last float: AtomType (0..15, or 16 when not
+// given) plus kTagHyd when the atom is hydrophobic.
+constexpr float kTagHyd = 32.0f;
+constexpr float kTagUntyped = 16.0f;
+__host__ __device__ inline float ag_tag(bool hyd, int type) {
+  return (type >= 0 && type < 16 ? static_cast<float>(type) : kTagUntyped) + (hyd ? kTagHyd : 0.0f);
 }
-__device__ inline long long fix_add(long long a, long long b) {
+__host__ __device__ inline bool tag_hyd(float tag) { return tag >= kTagHyd; }
+// AtomType, or -1 when not given
+__host__ __device__ inline int tag_type(float tag) {
+  const int t = static_cast<int>(tag >= kTagHyd ? tag - kTagHyd : tag);
+  return t < 16 ? t : -1;
+}
+
+__device__ inline long long fix_round(double v, uint32_t& st, uint32_t bit) {
+  const double lim = static_cast<double>(kFixLim);
+  const double s = v * kFixScale;
+  if (!(fabs(s) < lim)) st |= (s != s) ? (bit | kStNaN) : bit;
+  return __double2ll_rn(fmin(fmax(s, -lim), lim));   // NaN -> -lim (fmax drops NaN), flagged
+}
+__device__ inline long long fix_add(long long a, long long b, uint32_t& st, uint32_t bit) {
   const long long s = a + b;  // |a|, |b| <= 2^59 + 2^25: no overflow
+  if (s >= kFixLim || s <= -kFixLim) st |= bit;
   return s > kFixLim ? kFixLim : (s < -kFixLim ? -kFixLim : s);
+}
+// A clamped integer sum as a reported value: +-inf when its status bit is set
+// (NaN with kStNaN), else v units.
+__device__ inline double fix_value(long long v, uint32_t st, uint32_t bit) {
+  if (st & kStNaN) return __builtin_nan("");
+  if (st & bit) return v < 0 ? -__builtin_huge_val() : __builtin_huge_val();
+  return static_cast<double>(v) * kFixUnit;
 }
 
 // Per-lane integer accumulators of one tile slot.
 struct FixAcc {
   long long e = 0, a = 0, x = 0, y = 0, z = 0;
+  uint32_t st = 0;
 };
 
-// The pair predicate in f32, exactly the reference's (src/energy.rs:279-284).
+// The pair predicate in f32: r^2 = (dx*dx + dy*dy) + dz*dz and the
+// reference's skip test `r_sq > CUTOFF_SQ || r_sq < MIN_R_SQ`
+// (src/energy.rs:279-284), negated, so a NaN distance is kept as the CPU
+// reference keeps it (the WGSL kernel, shaders/energy.wgsl:74, drops it).
+// An empty slot passes rc2_lane = -1 and has no pair.
 __device__ inline void pair_pred(float4 B0, float xb, float yb, float zb, float rc2_lane,
                                  float& dx, float& dy, float& dz, float& r2, bool& in) {
   dx = xb - B0.x;
   dy = yb - B0.y;
   dz = zb - B0.z;
   r2 = (dx * dx + dy * dy) + dz * dz;
-  in = r2 <= rc2_lane && r2 >= kMinRSq;
+  in = !(r2 > rc2_lane || r2 < kMinRSq);
 }
 
 // One pair term (the reference's formulas, src/energy.rs:261-314) after the
 // predicate: the f32 energy ep and, with FORCES, the scalar f with
-// F = f (dx, dy, dz); both exactly zero when the pair is out of range.
+// F = f (dx, dy, dz); both exactly zero when the pair is out of range or
+// `hot` (evaluated in FP64 instead). h: the pair takes the hydrophobic step.
+// B0 = (x, y, z, 332 q), B1 = (Rmin/2, sqrt(eps), eps, tag) of the antigen atom.
 template <bool FORCES>
 __device__ inline void pair_value(float4 B0, float4 B1, float qb, float rmhb, float sqeb,
-                                  float sqe12b, bool hydb, float r2, bool in, float& ep, float& f,
-                                  uint32_t& h) {
+                                  float sqe12b, bool hydb, float r2, bool in, bool hot, float& ep,
+                                  float& f, uint32_t& h) {
   float rinv = __builtin_amdgcn_rsqf(r2);
-  rinv = in ? rinv : 0.0f;
+  rinv = (in && !hot) ? rinv : 0.0f;
   const float rinv2 = rinv * rinv;
   const float R = B1.x + rmhb;
   const float s2 = (R * R) * rinv2;
@@ -67,33 +117,72 @@ __device__ inline void pair_value(float4 B0, float4 B1, float qb, float rmhb, fl
     f = __builtin_fmaf(ecl, rinv2, flj);
   }
   h = 0;
-  if (B1.z != 0.0f) {  // uniform: the antigen atom is the same in every lane
+  if (tag_hyd(B1.w)) {  // uniform: the antigen atom is the same in every lane
     h = (in && hydb && r2 < kHydroSq) ? 1u : 0u;
     if (FORCES) f = h ? __builtin_fmaf(-0.1f, rinv, f) : f;
   }
   ep = elj + ecl;
 }
 
+// The FP64 evaluation of a clash pair: energy and, with FORCES, the force
+// added to (hx, hy, hz), from the exact differences of the f32 coordinates,
+// eps_ij = sqrt(eps_i eps_j) from the raw epsilons, as the all-FP64 kernel
+// forms them. Returns the pair's contribution to the error estimate's
+// sum |e| (the FP32 rounding of 332 q, and a margin on the LJ term).
+template <bool FORCES>
+__device__ inline float hot_pair(float4 B0, float4 B1, float xb, float yb, float zb, float qb,
+                                 float rmhb, float epsb, uint32_t h, double& he, double& hx,
+                                 double& hy, double& hz) {
+  const double ddx = static_cast<double>(xb) - static_cast<double>(B0.x);
+  const double ddy = static_cast<double>(yb) - static_cast<double>(B0.y);
+  const double ddz = static_cast<double>(zb) - static_cast<double>(B0.z);
+  const double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
+  const double rinv2 = 1.0 / r2d;
+  const double rinv = sqrt(rinv2);
+  const double R = static_cast<double>(B1.x) + static_cast<double>(rmhb);
+  const double s2 = (R * R) * rinv2;
+  const double s6 = (s2 * s2) * s2;
+  const double s12 = s6 * s6;
+  const double eps = sqrt(static_cast<double>(B1.z) * static_cast<double>(epsb));
+  const double elj = eps * (s12 - 2.0 * s6);
+  const double ecl = (static_cast<double>(B0.w) * static_cast<double>(qb)) * rinv;
+  he += elj + ecl;
+  if (FORCES) {
+    double f = (12.0 * eps * (s12 - s6) + ecl) * rinv2;
+    if (h) f -= 0.1 * rinv;
+    hx += f * ddx;
+    hy += f * ddy;
+    hz += f * ddz;
+  }
+  return static_cast<float>(fabs(ecl) * (1.0 / 32.0) + fabs(elj) * 0x1p-26);
+}
+
 // Pairs of the lane's atom with the `n_s` antigen atoms staged in `st` (in
 // chunk order); folds the chunk's sums into `acc`. The predicate is formed
 // for two atoms at a time; an atom in range of no lane of the wave is
 // skipped (each of its terms would add an exact zero), the others are
-// added in chunk order, which keeps the sums those of a plain loop. A chunk
-// with no pair in range in the whole wave leaves `acc` as it is (its sums
-// are exact zeros), without the integer conversions.
+// added in chunk order, which keeps the sums those of a plain loop. Clash
+// pairs go to hot_pair (a branch taken by the wave only when one of its
+// lanes has one). A chunk with no pair in range in the whole wave leaves
+// `acc` as it is (its sums are exact zeros), without the integer conversions.
 template <bool FORCES, bool COUNT>
 __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb, float yb,
                                    float zb, float qb, float rmhb, float sqeb, float sqe12b,
-                                   bool hydb, float rc2_lane, bool valid, FixAcc& acc,
+                                   float epsb, bool hydb, float rc2_lane, bool valid, FixAcc& acc,
                                    uint32_t& n_in, uint32_t& n_tested) {
   float fx = 0.f, fy = 0.f, fz = 0.f, es = 0.f, ec = 0.f, eabs = 0.f;
+  double he = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;  // clash pairs, FP64
   uint32_t nh = 0;
-  uint64_t any = 0;  // wave-uniform
+  uint64_t any = 0;   // wave-uniform
+  bool mine = false;  // the lane's atom has a pair in this chunk
   auto add = [&](const float4& B0, const float4& B1, float dx, float dy, float dz, float r2,
                  bool in) {
+    mine = mine || in;
+    const float R = B1.x + rmhb;
+    const bool hot = in && (R * R) > kHotS2 * r2;
     float ep, f;
     uint32_t h;
-    pair_value<FORCES>(B0, B1, qb, rmhb, sqeb, sqe12b, hydb, r2, in, ep, f, h);
+    pair_value<FORCES>(B0, B1, qb, rmhb, sqeb, sqe12b, hydb, r2, in, hot, ep, f, h);
     nh += h;
     if (FORCES) {
       fx = __builtin_fmaf(f, dx, fx);
@@ -105,6 +194,9 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
     es = sum;
     ec += err;
     eabs += fabsf(ep);
+    if (ballot64(hot)) {
+      if (hot) eabs += hot_pair<FORCES>(B0, B1, xb, yb, zb, qb, rmhb, epsb, h, he, hx, hy, hz);
+    }
   };
   uint32_t q = 0;
   for (; q + 2 <= n_s; q += 2) {
@@ -138,12 +230,18 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
   if (!any) return;
   // hydrophobic: -0.5 per pair, exactly -2^19 in fixed point
   const long long eh = -static_cast<long long>(nh) << 19;
-  acc.e = fix_add(acc.e, fix_round(static_cast<double>(es) + static_cast<double>(ec)) + eh);
-  acc.a = fix_add(acc.a, fix_round(static_cast<double>(eabs)));
+  const double e_chunk = (static_cast<double>(es) + static_cast<double>(ec)) + he;
+  acc.e = fix_add(acc.e, fix_round(e_chunk, acc.st, kStE) + eh, acc.st, kStE);
+  // sum |e| feeds the error estimate only; an atom's energy rounding can err
+  // only in a chunk where it has a pair (a chunk is fixed by the antigen, so
+  // this count does not depend on the tiles)
+  uint32_t ignore = 0;
+  const float ea = mine ? eabs + kRoundAbs : eabs;
+  acc.a = fix_add(acc.a, fix_round(static_cast<double>(ea), ignore, 0u), ignore, 0u);
   if (FORCES) {
-    acc.x = fix_add(acc.x, fix_round(static_cast<double>(fx)));
-    acc.y = fix_add(acc.y, fix_round(static_cast<double>(fy)));
-    acc.z = fix_add(acc.z, fix_round(static_cast<double>(fz)));
+    acc.x = fix_add(acc.x, fix_round(static_cast<double>(fx) + hx, acc.st, kStX), acc.st, kStX);
+    acc.y = fix_add(acc.y, fix_round(static_cast<double>(fy) + hy, acc.st, kStY), acc.st, kStY);
+    acc.z = fix_add(acc.z, fix_round(static_cast<double>(fz) + hz, acc.st, kStZ), acc.st, kStZ);
   }
 }
 

@@ -8,8 +8,12 @@ namespace dnab {
 
 // Sort code of a candidate atom, or kExcluded when it lies farther than the
 // cutoff from the antigen's atom bounding box (then no antigen atom is within
-// the cutoff: the f32 box distance never exceeds the f32 pair distance).
+// the cutoff: the f32 box distance never exceeds the f32 pair distance; an
+// infinite coordinate lands here, its r^2 being +inf), or kNonFinite for a
+// NaN coordinate (then every pair's r^2 is NaN, which the reference's skip
+// test keeps, src/energy.rs:284: its terms and the atom's sums are NaN).
 __device__ inline uint32_t sort_code_of(const AgView& V, float x, float y, float z) {
+  if (isnan(x) || isnan(y) || isnan(z)) return kNonFinite;
   const float bx = fmaxf(fmaxf(V.excl_lo[0] - x, x - V.excl_hi[0]), 0.0f);
   const float by = fmaxf(fmaxf(V.excl_lo[1] - y, y - V.excl_hi[1]), 0.0f);
   const float bz = fmaxf(fmaxf(V.excl_lo[2] - z, z - V.excl_hi[2]), 0.0f);
@@ -28,15 +32,20 @@ __device__ inline uint32_t sort_code_of(const AgView& V, float x, float y, float
   return V.sort_code_base + brick * 64u + morton3(c[0] & 3u, c[1] & 3u, c[2] & 3u);
 }
 
-// Zero an excluded atom's per-atom outputs (it has no pair).
-__device__ inline void zero_atom_outputs(uint32_t j, float4* force, double* e_atom,
-                                         double* bd_atom, double* f64_atom) {
-  force[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-  e_atom[j] = 0.0;
+// The per-atom outputs of an atom left out of the tiles: zeros when it has no
+// pair (kExcluded), NaN when a coordinate is NaN and the antigen has atoms
+// (kNonFinite: its sums over the antigen are NaN).
+__device__ inline void fill_atom_outputs(uint32_t j, uint32_t code, const AgView& V, float4* force,
+                                         double* e_atom, double* bd_atom, double* f64_atom) {
+  const bool nan = code == kNonFinite && V.n_atoms > 0;
+  const float vf = nan ? __builtin_nanf("") : 0.0f;
+  const double vd = nan ? __builtin_nan("") : 0.0;
+  force[j] = make_float4(vf, vf, vf, vf);
+  e_atom[j] = vd;
   const size_t kb = static_cast<size_t>(kBdTerms) * j;
-  for (int t = 0; t < kBdTerms; ++t) bd_atom[kb + t] = 0.0;
+  for (int t = 0; t < kBdTerms; ++t) bd_atom[kb + t] = vd;
   const size_t k = 3 * static_cast<size_t>(j);
-  f64_atom[k] = f64_atom[k + 1] = f64_atom[k + 2] = 0.0;
+  f64_atom[k] = f64_atom[k + 1] = f64_atom[k + 2] = vd;
 }
 
 }  // namespace dnab

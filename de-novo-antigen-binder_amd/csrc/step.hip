@@ -70,16 +70,16 @@ k_candidate_update(const AgView* __restrict__ views, const uint32_t* __restrict_
       uint32_t k = a0;
       for (; k + 4 <= a1; k += 4) {
         const float4 f0 = force[k], f1 = force[k + 1], f2 = force[k + 2], f3 = force[k + 3];
-        fx += f0.x; fy += f0.y; fz += f0.z;
-        fx += f1.x; fy += f1.y; fz += f1.z;
-        fx += f2.x; fy += f2.y; fz += f2.z;
-        fx += f3.x; fy += f3.y; fz += f3.z;
+        fx += dyn_force(f0.x); fy += dyn_force(f0.y); fz += dyn_force(f0.z);
+        fx += dyn_force(f1.x); fy += dyn_force(f1.y); fz += dyn_force(f1.z);
+        fx += dyn_force(f2.x); fy += dyn_force(f2.y); fz += dyn_force(f2.z);
+        fx += dyn_force(f3.x); fy += dyn_force(f3.y); fz += dyn_force(f3.z);
       }
       for (; k < a1; ++k) {
         const float4 f = force[k];
-        fx += f.x;
-        fy += f.y;
-        fz += f.z;
+        fx += dyn_force(f.x);
+        fy += dyn_force(f.y);
+        fz += dyn_force(f.z);
       }
     }
     const float sigma = sp.with_noise ? sigma_table[iter] : 0.0f;
@@ -111,13 +111,14 @@ k_atom_move(const AgView* __restrict__ views, const uint32_t* __restrict__ atom_
     q.z += d.z;
     pos[k] = q;
     if (do_count) {
-      code = sort_code_of(views[atom_ag[k]], q.x, q.y, q.z);
+      const AgView& V = views[atom_ag[k]];
+      code = sort_code_of(V, q.x, q.y, q.z);
       code_out[k] = code;
-      if (code == kExcluded) zero_atom_outputs(k, force, e_atom, bd_atom, f64_atom);
+      if (code >= kNonFinite) fill_atom_outputs(k, code, V, force, e_atom, bd_atom, f64_atom);
     }
   }
   if (!do_count) return;  // uniform
-  const bool need = live && code != kExcluded;
+  const bool need = live && code < kNonFinite;
   // group the wave's lanes by code: a leader per distinct code
   uint64_t rem = ballot64(need);
   uint32_t leader = 0, below = 0, size = 0;
@@ -400,6 +401,7 @@ int dnab_langevin_default_params(dnab_langevin_params* p) {
 int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                        uint32_t iter) {
   if (!ctx || !pop || !p || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "langevin_step"));
   DNAB_RETURN_IF(validate_params(ctx, p));
   (void)hipSetDevice(ctx->device);
   DNAB_RETURN_IF(upload_sigma(ctx, pop, p, iter));
@@ -426,6 +428,7 @@ int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_
 int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                       uint32_t* topk_idx, double* topk_energy, uint32_t* n_topk) {
   if (!ctx || !pop || !p || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "langevin_run"));
   DNAB_RETURN_IF(validate_params(ctx, p));
   if (p->top_k == 0 || p->top_k > 4096)
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: top_k must be 1..4096");

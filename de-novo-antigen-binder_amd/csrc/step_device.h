@@ -17,6 +17,15 @@ struct StepParams {
   uint32_t seed_lo, seed_hi;
 };
 
+// An atom's force component as the residue update reads it: a saturated
+// component (+-inf, pair_device.h) counts as the saturation bound 2^39, so
+// opposite saturated atoms of one residue do not make a NaN step; every
+// other value passes unchanged.
+__device__ inline float dyn_force(float v) {
+  constexpr float kSat = 549755813888.0f;  // 2^59 units of 2^-20
+  return fminf(fmaxf(v, -kSat), kSat);
+}
+
 __device__ inline float clamp_rust(float v, float lo, float hi) {
   // f32::clamp: NaN passes through.
   if (v < lo) v = lo;

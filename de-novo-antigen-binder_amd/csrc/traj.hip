@@ -108,13 +108,13 @@ struct TrajArgs {
 };
 
 // Bytes of dynamic LDS for a capacity of a_cap atom slots and r_cap residues:
-// 5 integer sums, energy, position, parameters, force per atom slot; the
+// 5 integer sums, energy, position, parameters, force, status per atom slot; the
 // displacement and the (atom range, Calpha, index, candidate) record per
 // residue; the residue slot per atom slot; the chunk list of each tile
 // (a_cap / 64 tiles).
 __host__ __device__ inline size_t traj_lds_bytes(uint32_t a_cap, uint32_t r_cap) {
   const size_t res = (static_cast<size_t>(a_cap) * 2 + 3) & ~static_cast<size_t>(3);
-  return static_cast<size_t>(a_cap) * (5 * 8 + 8 + 3 * 16) + static_cast<size_t>(r_cap) * 32 + res +
+  return static_cast<size_t>(a_cap) * (5 * 8 + 8 + 3 * 16 + 4) + static_cast<size_t>(r_cap) * 32 + res +
          static_cast<size_t>(a_cap / kWave) * kTileChunks * 4;
 }
 // Static LDS of a workgroup of `waves` waves (staged chunks, tiles,
@@ -153,7 +153,8 @@ k_trajectory(TrajArgs g) {
   float4* s_pos = reinterpret_cast<float4*>(s_e + AC);      // [a_cap]
   float4* s_par = s_pos + AC;                               // [a_cap]
   float4* s_force = s_par + AC;                             // [a_cap]
-  float4* s_delta = s_force + AC;                           // [r_cap]
+  uint32_t* s_st = reinterpret_cast<uint32_t*>(s_force + AC);  // [a_cap] sticky status
+  float4* s_delta = reinterpret_cast<float4*>(s_st + AC);      // [r_cap] (a_cap: a multiple of 64)
   uint4* s_rinfo = reinterpret_cast<uint4*>(s_delta + g.r_cap);       // [r_cap]
   uint16_t* s_res = reinterpret_cast<uint16_t*>(s_rinfo + g.r_cap);  // [a_cap]
   uint32_t* s_desc = reinterpret_cast<uint32_t*>(s_dyn + ((reinterpret_cast<uintptr_t>(s_res + AC) -
@@ -345,6 +346,7 @@ k_trajectory(TrajArgs g) {
         tb.nchunk = over ? kTileOverflow : nt;
       }
       for (int q = 0; q < 5; ++q) s_acc[q * AC + k] = 0;
+      s_st[k] = 0;
     }
     if (tid == 0) s_next = 0;
     __syncthreads();
@@ -373,7 +375,7 @@ k_trajectory(TrajArgs g) {
 
       FixAcc acc;
       int32_t cur = -1;
-      float xb = 0.f, yb = 0.f, zb = 0.f, qb = 0.f, rmhb = 0.f, sqeb = 0.f, sqe12b = 0.f;
+      float xb = 0.f, yb = 0.f, zb = 0.f, qb = 0.f, rmhb = 0.f, sqeb = 0.f, sqe12b = 0.f, epsb = 0.f;
       bool hydb = false, valid = false;
       float rc2_lane = -1.0f;
       auto flush = [&]() {
@@ -386,6 +388,7 @@ k_trajectory(TrajArgs g) {
           atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[3 * AC + k]), static_cast<unsigned long long>(acc.y));
           atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[4 * AC + k]), static_cast<unsigned long long>(acc.z));
         }
+        if (acc.st) atomicOr(&s_st[k], acc.st);
         acc = FixAcc();
       };
 
@@ -407,6 +410,7 @@ k_trajectory(TrajArgs g) {
           rmhb = Q.x;
           sqeb = sqrtf(Q.y);
           sqe12b = 12.0f * sqeb;
+          epsb = Q.y;
           hydb = Q.z != 0.0f;
           rc2_lane = valid ? V.rc2 : -1.0f;  // an atom without pairs takes none
         }
@@ -437,8 +441,8 @@ k_trajectory(TrajArgs g) {
             if (n_s == 0) continue;
             if (lane == 0 && g.work) atomicAdd(&s_work[tb.cand], n_s);
             stage_chunk(s_stage[wave], mask, lane, L0[u], L1[u]);
-            chunk_pairs<true, COUNT>(s_stage[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, hydb,
-                                     rc2_lane, valid, acc, n_in, n_tested);
+            chunk_pairs<true, COUNT>(s_stage[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
+                                     hydb, rc2_lane, valid, acc, n_in, n_tested);
           }
         };
         uint32_t cbv[kLoadAhead], cev[kLoadAhead];
@@ -504,14 +508,19 @@ k_trajectory(TrajArgs g) {
     __syncthreads();
 
     // 3. per-atom energy and force from the integer sums
+    //    (a saturated component as +-inf, as the population kernels write it)
     for (uint32_t k = tid; k < A; k += WAVES * kWave) {
-      long long v[5];
-      for (int q = 0; q < 5; ++q) v[q] = min(max(s_acc[q * AC + k], -kFixLim), kFixLim);
-      s_e[k] = static_cast<double>(v[0]) * kFixUnit;
-      s_force[k] = make_float4(static_cast<float>(static_cast<double>(v[2]) * kFixUnit),
-                               static_cast<float>(static_cast<double>(v[3]) * kFixUnit),
-                               static_cast<float>(static_cast<double>(v[4]) * kFixUnit),
-                               static_cast<float>(static_cast<double>(v[1]) * kFixUnit));
+      uint32_t st = s_st[k], ignore = 0;
+      const long long ve = fix_add(s_acc[0 * AC + k], 0, st, kStE);
+      const long long va = fix_add(s_acc[1 * AC + k], 0, ignore, 0u);
+      const long long vx = fix_add(s_acc[2 * AC + k], 0, st, kStX);
+      const long long vy = fix_add(s_acc[3 * AC + k], 0, st, kStY);
+      const long long vz = fix_add(s_acc[4 * AC + k], 0, st, kStZ);
+      s_e[k] = fix_value(ve, st, kStE);
+      s_force[k] = make_float4(static_cast<float>(fix_value(vx, st, kStX)),
+                               static_cast<float>(fix_value(vy, st, kStY)),
+                               static_cast<float>(fix_value(vz, st, kStZ)),
+                               static_cast<float>(static_cast<double>(va) * kFixUnit));
     }
     __syncthreads();
 
@@ -532,9 +541,9 @@ k_trajectory(TrajArgs g) {
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
       for (uint32_t k = b0; k < b1; ++k) {  // in atom order
         const float4 f = s_force[k];
-        fx += f.x;
-        fy += f.y;
-        fz += f.z;
+        fx += dyn_force(f.x);
+        fy += dyn_force(f.y);
+        fz += dyn_force(f.z);
       }
       s_delta[r] = residue_delta(C.center[0], C.center[1], C.center[2], g.sp, C.r_restr,
                                  s_pos[ri.y], fx, fy, fz, sigma, it, ri.z);

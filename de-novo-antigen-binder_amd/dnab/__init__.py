@@ -5,6 +5,8 @@ include/dnab.h). This package binds it with ctypes for tests, benchmarks and
 scripting; it raises on import when the library is not built.
 """
 from . import _native
+from ._native import DNAB_STATUS_NAN as STATUS_NAN
+from ._native import DNAB_STATUS_SATURATED as STATUS_SATURATED
 from ._native import DnabError, device_count, visible_gpu
 from .api import (AtomEnergyBreakdown, Dataset, DeviceAntigen, DevicePopulation, GpuContext,
                   Mlp, SpatialHashGrid, exact_topk, langevin_params, population_features,
@@ -17,6 +19,7 @@ __all__ = [
     "AMINO_ONE_LETTER", "AtomCloud", "AtomEnergyBreakdown", "AtomProtein", "Dataset",
     "DeviceAntigen", "Mlp", "exact_topk", "population_features", "precision_study",
     "DevicePopulation", "DnabError", "GpuContext", "PopulationHost", "ResidueCloud",
+    "STATUS_NAN", "STATUS_SATURATED",
     "SpatialHashGrid", "aa_from_sequence", "device_count", "langevin_params", "parse_peptide",
     "population_from_proteins", "protein_from_ca_trace", "visible_gpu", "_native",
 ]

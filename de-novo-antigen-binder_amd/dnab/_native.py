@@ -26,6 +26,8 @@ DNAB_ERR_UNSUPPORTED = 6
 DNAB_EVAL_FORCES = 1
 DNAB_EVAL_BREAKDOWN = 2
 DNAB_EVAL_FP64 = 4
+DNAB_STATUS_NAN = 1
+DNAB_STATUS_SATURATED = 2
 DNAB_FORCE_ALLATOM = 0
 DNAB_FORCE_CG = 1
 
@@ -210,6 +212,7 @@ _PROTOS = {
     "dnab_population_destroy": (None, [vp]),
     "dnab_population_evaluate": (C.c_int, [vp, vp, C.c_uint]),
     "dnab_population_read_energies": (C.c_int, [vp, vp, f64p, f32p]),
+    "dnab_population_read_status": (C.c_int, [vp, vp, u8p]),
     "dnab_population_read_breakdown": (C.c_int, [vp, vp, f64p, f64p, f64p]),
     "dnab_population_read_forces": (C.c_int, [vp, vp, f32p, f32p, f32p]),
     "dnab_population_read_forces_f64": (C.c_int, [vp, vp, f64p, f64p, f64p]),

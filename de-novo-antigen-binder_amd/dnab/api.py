@@ -1,8 +1,8 @@
 """Python handles over the device-side C ABI.
 
-GpuContext mirrors the reference's GpuContext (src/gpu.rs:134-243):
+GpuContext mirrors the reference's GpuContext (src/gpu.rs:46-57):
 ``GpuContext.try_init()`` returns None when no usable GPU exists
-(src/gpu.rs:143-159), and ``score_batch`` keeps the reference's signature
+(src/gpu.rs:55-57, its async body src/gpu.rs:59-155 returns None at 65-71), and ``score_batch`` keeps the reference's signature
 (src/gpu.rs:166-171). The energy/force methods mirror src/energy.rs.
 """
 from __future__ import annotations
@@ -480,6 +480,14 @@ class DevicePopulation:
                                                            N.ptr(e, C.c_double),
                                                            N.ptr(err, C.c_float)))
         return e[:self.n_cand], err[:self.n_cand]
+
+    def status(self):
+        """Per candidate, DNAB_STATUS_* bits of the last evaluation: 1 NaN
+        energy (a non-finite coordinate), 2 a saturated sum (reported as inf)."""
+        out = np.zeros(max(self.n_cand, 1), np.uint8)
+        self.ctx.check(N.lib.dnab_population_read_status(self.ctx.handle, self._h,
+                                                         N.ptr(out, C.c_uint8)))
+        return out[:self.n_cand]
 
     def terms(self):
         """(n_cand, 6) of the last breakdown evaluation: LJ, Coulomb,

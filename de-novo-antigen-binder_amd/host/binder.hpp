@@ -7,7 +7,7 @@
 //                                          src/allatom.rs:92-101, 523-541
 //   binder::ResidueCloud                   src/atom.rs:185-257
 //   binder::SpatialHashGrid                src/spatial.rs:12-68 (device-built)
-//   binder::GpuContext::try_init           src/gpu.rs:143-243 (std::nullopt = None)
+//   binder::GpuContext::try_init           src/gpu.rs:55-57 (async body 59-155; std::nullopt = None)
 //   binder::GpuContext::score_batch        src/gpu.rs:166-267
 //   binder::interaction_energy_atoms[_breakdown], compute_forces[_with_grid]
 //                                          src/energy.rs:154-398
@@ -191,7 +191,7 @@ class SpatialHashGrid {
 
 class GpuContext {
  public:
-  // GpuContext::try_init (src/gpu.rs:143-243): nullopt when no device.
+  // GpuContext::try_init (src/gpu.rs:55-57, async body 59-155): nullopt when no device.
   static std::optional<GpuContext> try_init(int device = 0) {
     dnab_ctx* c = nullptr;
     int st = dnab_ctx_create(device, &c);

@@ -47,6 +47,22 @@ typedef struct dnab_gpu_atom {
   float x, y, z, q, r_min_half, epsilon, hydrophobic, _pad;
 } dnab_gpu_atom;
 
+/* The layout is the ABI: 32 bytes, fields at 4-byte steps in the order of
+ * the reference's #[repr(C)] struct. */
+#ifdef __cplusplus
+#define DNAB_STATIC_ASSERT(c, m) static_assert(c, m)
+#else
+#define DNAB_STATIC_ASSERT(c, m) _Static_assert(c, m)
+#endif
+DNAB_STATIC_ASSERT(sizeof(dnab_gpu_atom) == 32, "dnab_gpu_atom must be 32 bytes");
+DNAB_STATIC_ASSERT(offsetof(dnab_gpu_atom, x) == 0 && offsetof(dnab_gpu_atom, y) == 4 &&
+                       offsetof(dnab_gpu_atom, z) == 8 && offsetof(dnab_gpu_atom, q) == 12 &&
+                       offsetof(dnab_gpu_atom, r_min_half) == 16 &&
+                       offsetof(dnab_gpu_atom, epsilon) == 20 &&
+                       offsetof(dnab_gpu_atom, hydrophobic) == 24 &&
+                       offsetof(dnab_gpu_atom, _pad) == 28,
+                   "dnab_gpu_atom field offsets follow src/gpu.rs:18-29");
+
 /* Read-only SoA view of an AtomCloud (src/allatom.rs:16-31). atom_type and
  * residue_idx are not used by the LJ/Coulomb/hydrophobic terms and may be
  * NULL. */
@@ -80,6 +96,7 @@ typedef struct dnab_residue_cloud {
 typedef struct dnab_energy_breakdown {
   double lj, coulomb, hydrophobic, hbond, solvation;
 } dnab_energy_breakdown;
+DNAB_STATIC_ASSERT(sizeof(dnab_energy_breakdown) == 40, "five doubles");
 
 /* Output buffers for dnab_build_protein (caller-allocated). Per-atom arrays
  * hold atom_capacity entries; per-residue arrays n_res (res_start n_res+1).
@@ -94,8 +111,9 @@ typedef struct dnab_protein_out {
   uint8_t* amino_acid;
 } dnab_protein_out;
 
-/* ---- context (replaces GpuContext, src/gpu.rs:134-243) -------------------
- * dnab_ctx_create replaces GpuContext::try_init (src/gpu.rs:143-243): a
+/* ---- context (replaces GpuContext, src/gpu.rs:46-57) ---------------------
+ * dnab_ctx_create replaces GpuContext::try_init (src/gpu.rs:55-57, whose
+ * async body src/gpu.rs:59-155 returns None when no adapter exists): a
  * non-zero status (DNAB_ERR_NO_DEVICE) takes the place of `None`. */
 int dnab_ctx_create(int device, dnab_ctx** out);
 /* Drops the caller's hold on the context. Objects made on it (grids,
@@ -329,6 +347,17 @@ typedef struct dnab_population_desc {
   const uint8_t* res_amino_acid;/* n_res, residue types (residue-level model) */
   dnab_atom_cloud atoms;        /* all candidate atoms, n = res_atom_off[n_res] */
 } dnab_population_desc;
+/* Atom parameters must be finite with r_min_half >= 0 and epsilon >= 0
+ * (DNAB_ERR_INVALID otherwise; the same holds for dnab_antigen_create, whose
+ * coordinates must also be finite). A candidate coordinate may be NaN or
+ * infinite, as the reference's AtomCloud allows, and such a population can be
+ * evaluated with the reference's results: an atom with a NaN coordinate has
+ * NaN distances, whose pairs the reference's skip test keeps
+ * (src/energy.rs:284), so its outputs and its candidate's energy are NaN
+ * (dnab_population_read_status reports it); an atom with an infinite and no
+ * NaN coordinate has r^2 = +inf, which the test skips, so it has no pair.
+ * Every call that steps or screens such a population returns
+ * DNAB_ERR_INVALID. */
 int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t n_antigens,
                            const dnab_population_desc* desc, dnab_population** out);
 void dnab_population_destroy(dnab_population* pop);
@@ -382,6 +411,16 @@ int dnab_population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags
  * a-priori bound on |E - E_fp64| from the FP32 pair terms. */
 int dnab_population_read_energies(dnab_ctx* ctx, const dnab_population* pop, double* energy,
                                   float* err_bound);
+/* Per-candidate status of the last evaluation (no reference counterpart: the
+ * reference's f32 sums carry NaN and overflow in the value itself).
+ * DNAB_STATUS_NAN: the energy is NaN (a non-finite coordinate).
+ * DNAB_STATUS_SATURATED: an atom's energy or force sum reached the exact
+ * accumulator's bound, +-2^39 kcal/mol (or kcal/mol/A) (~5.5e11; only
+ * reachable through dozens of sub-Angstrom clashes on one atom); the value is
+ * reported as +-inf. 0: the results are finite and within the tolerance. */
+#define DNAB_STATUS_NAN 1u
+#define DNAB_STATUS_SATURATED 2u
+int dnab_population_read_status(dnab_ctx* ctx, dnab_population* pop, uint8_t* status);
 /* Per candidate, 6 values of the last DNAB_EVAL_BREAKDOWN run: LJ, Coulomb,
  * hydrophobic, H-bond, solvation (their sum is the candidate energy,
  * interaction_energy_atoms) and the binder's disulfide term; allatom_energy
