@@ -36,38 +36,131 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, budget_s=12.0):
+def host_threads():
+    """Worker threads of the CPU baselines: RAYON_NUM_THREADS when set, else
+    one per logical CPU (Rayon's default)."""
+    return int(os.environ.get("RAYON_NUM_THREADS", "0")) or os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def sample_states(pop, params, steps, every, start):
+    """The loop's states every `every` steps (and the last), stepped with the
+    per-step API, which reproduces the loop's trajectory bit for bit; each
+    state's in-cutoff pairs counted on the GPU with the reference predicate."""
+    pop.set_coords(*start)
+    states, pairs = [], []
+    for it in range(steps + 1):
+        if it % every == 0 or it == steps:
+            states.append(pop.coords())
+            pairs.append(pop.pair_count()[0])
+        if it < steps:
+            pop.step(params, it)
+    return states, pairs
+
+
+def cpu_baseline(w, states, state_pairs, budget_s=12.0):
     """The oracle's threaded C restatement of the reference's CPU path (grid
     E+F over all antigen atoms, src/energy.rs:322-398 pattern with AMBER
-    per-atom parameters), timed on this host over repeated full-batch
-    evaluations for about budget_s seconds."""
+    per-atom parameters, parallel over candidates as the Rayon loop of
+    src/diffusion.rs:1021-1024), timed on this host over the states the GPU
+    loop itself passes through, round robin, for about budget_s seconds."""
     from oracle import oracle as O
-    threads = int(os.environ.get("RAYON_NUM_THREADS", "0")) or os.cpu_count() or 1
+    from dnab.structure import AtomCloud
+    threads = host_threads()
     ag = w.antigens[0].atoms
     off = w.pop.cand_atom_off()
-    r = O.population_f64(ag, w.pop.atoms, off, w.cutoff, forces=False, nthreads=threads)
-    pairs = int(r["pairs"].sum())
-    O.baseline_grid_ef(ag, w.pop.atoms, off, w.cutoff, threads, forces=True)  # warm-up
-    reps, t0 = 0, time.perf_counter()
+    a = w.pop.atoms
+    clouds = [AtomCloud(x, y, z, a.charge, a.r_min_half, a.epsilon, a.hydrophobic, a.atom_type)
+              for x, y, z in states]
+    O.baseline_grid_ef(ag, clouds[0], off, w.cutoff, threads, forces=True)  # warm-up
+    passes, pairs, t0 = 0, 0, time.perf_counter()
     while True:
-        O.baseline_grid_ef(ag, w.pop.atoms, off, w.cutoff, threads, forces=True)
-        reps += 1
+        k = passes % len(clouds)
+        O.baseline_grid_ef(ag, clouds[k], off, w.cutoff, threads, forces=True)
+        pairs += state_pairs[k]
+        passes += 1
         dt = time.perf_counter() - t0
-        if dt >= budget_s or reps >= 100000:
+        if dt >= budget_s or passes >= 100000:
             break
-    # the reference's own energy pattern (brute force over all antigen atoms)
+    # the reference's own energy pattern (brute force over all antigen atoms), first state
     reps_b, t0 = 0, time.perf_counter()
     while True:
-        O.baseline_energy_brute(ag, w.pop.atoms, off, w.cutoff, threads)
+        O.baseline_energy_brute(ag, clouds[0], off, w.cutoff, threads)
         reps_b += 1
         dtb = time.perf_counter() - t0
         if dtb >= budget_s / 3 or reps_b >= 1000:
             break
-    return {"value": pairs * reps / dt, "unit": "atom-pairs/s", "cores": threads,
-            "kind": "port",
-            "sample": (f"{reps} full-batch grid E+F passes of the same workload "
-                       f"({pairs} in-cutoff pairs each) in {dt:.1f} s on {threads} threads; "
-                       f"brute-force energy pattern: {pairs * reps_b / dtb:.3g} pairs/s")}
+    return {"value": pairs / dt, "unit": "atom-pairs/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": (f"{passes} full-batch grid E+F passes over the {len(clouds)} states the "
+                       f"GPU loop passes through (every 20th step; {min(state_pairs)}.."
+                       f"{max(state_pairs)} in-cutoff pairs each) in {dt:.1f} s on {threads} "
+                       f"threads; brute-force energy pattern on the start state: "
+                       f"{state_pairs[0] * reps_b / dtb:.3g} pairs/s")}
+
+
+class AllatomScorer:
+    """The oracle's f32 allatom_energy (interaction_energy_atoms_breakdown
+    plus disulfide_energy, src/diffusion.rs:281-283) of fixed candidates,
+    scored by `threads` host threads: the C structs are built once, so each
+    call releases the GIL for the whole scoring (ctypes), as the reference's
+    Rayon workers score candidates independently."""
+
+    def __init__(self, ag_atoms, proteins, cutoff):
+        import ctypes as C
+        import numpy as np
+        from oracle import oracle as O
+        self.C, self.O, self.cutoff = C, O, cutoff
+        self.a, self.ka = O.cloud(ag_atoms)
+        self.ea, self.kea = O.ext(ag_atoms.atom_type)
+        self.items = []
+        for p in proteins:
+            b, kb = O.cloud(p.atoms)
+            eb, keb = O.ext(p.atoms.atom_type)
+            sg = [np.ascontiguousarray(v, np.float32)
+                  for v in O.sg_positions(p.atoms.x, p.atoms.y, p.atoms.z, p.res_start,
+                                          p.amino_acid, 0, len(p.amino_acid))]
+            self.items.append((b, kb, eb, keb, sg))
+
+    def rate(self, threads, budget_s):
+        """Scorings per second over about budget_s seconds, and the count."""
+        import threading
+        import numpy as np
+        C, lib = self.C, self.O.lib
+        counts = [0] * threads
+        deadline = time.perf_counter() + budget_s
+
+        def work(t):
+            out = np.zeros(5, np.float32)
+            op = out.ctypes.data_as(C.POINTER(C.c_float))
+            k, n = t, 0
+            while time.perf_counter() < deadline:
+                b, _, eb, _, sg = self.items[k % len(self.items)]
+                lib.or_energy_full_f32(C.byref(self.a), C.byref(b), C.byref(self.ea),
+                                       C.byref(eb), self.cutoff, op)
+                lib.or_disulfide_f32(*[v.ctypes.data_as(C.POINTER(C.c_float)) for v in sg],
+                                     len(sg[0]))
+                k += threads
+                n += 1
+            counts[t] = n
+
+        t0 = time.perf_counter()
+        ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        el = time.perf_counter() - t0
+        return sum(counts) / el, sum(counts), el
 
 
 def copy_calibration(mib):
@@ -182,7 +275,7 @@ def allatom_energy_bench(ctx, w, pop, coords, reps=10, cpu_budget_s=3.0):
     interaction_energy_atoms_breakdown plus the disulfide term) of every
     binder of the workload at its start state, one breakdown evaluation of the
     population per call. The CPU baseline is the oracle's f32 restatement of
-    the same sum, one thread, on a bounded sample of the binders."""
+    the same sum on all host threads, over the same binders."""
     import numpy as np
     pop.set_coords(*coords)
     pairs, _ = pop.pair_count()
@@ -212,26 +305,34 @@ def allatom_energy_bench(ctx, w, pop, coords, reps=10, cpu_budget_s=3.0):
         "pairs_per_s": pairs / dt,
         "mean_energy": float(terms.sum(1).mean()),
     }
-    from oracle import oracle as O
-    p = w.pop
-    ag = w.antigens[0]
-    n, t0 = 0, time.perf_counter()
-    while True:
-        c = n % p.n_cand
-        cl = p.candidate_cloud(c)
-        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
-        O.energy_full_f32(ag.atoms, cl, w.cutoff)
-        rs = (p.res_atom_off[r0:r1 + 1] - p.res_atom_off[r0]).astype(np.uint32)
-        O.disulfide_f32(*O.sg_positions(cl.x, cl.y, cl.z, rs, p.res_amino_acid[r0:r1], 0, r1 - r0))
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= cpu_budget_s or n >= 100000:
-            break
+    threads = host_threads()
+    rate, n, el = AllatomScorer(w.antigens[0].atoms, candidate_proteins(w, coords),
+                                w.cutoff).rate(threads, cpu_budget_s)
     out["cpu_baseline"] = {
-        "value": n / el, "unit": "candidates/s", "cores": 1, "kind": "port",
-        "sample": f"{n} f32 allatom_energy scorings of the workload's binders in {el:.1f} s"}
+        "value": rate, "unit": "candidates/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
+        "sample": f"{n} f32 allatom_energy scorings of the workload's binders in {el:.1f} s "
+                  f"on {threads} threads"}
     out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     return out
+
+
+def candidate_proteins(w, coords):
+    """The workload's binders as AtomProteins at the given coordinates."""
+    import numpy as np
+    from dnab.structure import AtomProtein
+    p = w.pop
+    off = p.cand_atom_off()
+    x, y, z = coords
+    prots = []
+    for c in range(p.n_cand):
+        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
+        cl = p.candidate_cloud(c)
+        a0, a1 = int(off[c]), int(off[c + 1])
+        cl.x, cl.y, cl.z = x[a0:a1].copy(), y[a0:a1].copy(), z[a0:a1].copy()
+        prots.append(AtomProtein(cl, (p.res_atom_off[r0:r1 + 1] - off[c]).astype(np.uint32),
+                                 p.res_amino_acid[r0:r1].copy(),
+                                 (p.res_ca_atom[r0:r1] - off[c]).astype(np.uint32)))
+    return prots
 
 
 def maturation_bench(ctx, w, ag_dev, coords, reps=5, cpu_budget_s=4.0):
@@ -240,7 +341,8 @@ def maturation_bench(ctx, w, ag_dev, coords, reps=5, cpu_budget_s=4.0):
     it per result. Timed: the C call alone, inputs packed beforehand. The CPU
     baseline restates the reference's cost: one f32 allatom_energy (the
     oracle's C restatement of interaction_energy_atoms_breakdown plus the
-    disulfide term) per trial the reference's scan makes."""
+    disulfide term) per trial the reference's scan makes, scored on all host
+    threads."""
     import numpy as np
     from dnab.api import MaturationCall
     from dnab.structure import AtomProtein, population_from_proteins
@@ -248,17 +350,8 @@ def maturation_bench(ctx, w, ag_dev, coords, reps=5, cpu_budget_s=4.0):
     off = p.cand_atom_off()
     x, y, z = coords
     ag = w.antigens[0]
-    abs_, muts = [], []
-    for c in range(p.n_cand):
-        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
-        cl = p.candidate_cloud(c)
-        a0, a1 = int(off[c]), int(off[c + 1])
-        cl.x, cl.y, cl.z = x[a0:a1].copy(), y[a0:a1].copy(), z[a0:a1].copy()
-        prot = AtomProtein(cl, (p.res_atom_off[r0:r1 + 1] - off[c]).astype(np.uint32),
-                           p.res_amino_acid[r0:r1].copy(),
-                           (p.res_ca_atom[r0:r1] - off[c]).astype(np.uint32))
-        abs_.append(prot)
-        muts.append(interface_positions(prot, ag))
+    abs_ = candidate_proteins(w, coords)
+    muts = [interface_positions(prot, ag) for prot in abs_]
     npos = sum(len(m) for m in muts)
     call = MaturationCall(ctx, ag_dev, population_from_proteins(abs_), muts)
     call.run()                                     # warm-up
@@ -301,27 +394,17 @@ def maturation_bench(ctx, w, ag_dev, coords, reps=5, cpu_budget_s=4.0):
         "mutations_kept": int(sum(int((a.amino_acid != b.amino_acid).sum())
                                   for a, b in zip(abs_, prots))),
     }
-    # CPU baseline: f32 allatom_energy per trial, one thread, on a bounded sample
-    from oracle import oracle as O
+    # CPU baseline: f32 allatom_energy per trial, all host threads
     sample = [a for a, m in zip(abs_, muts) if m] or abs_[:1]
-    n_sc, t0 = 0, time.perf_counter()
-    while True:
-        ab = sample[n_sc % len(sample)]
-        O.energy_full_f32(ag.atoms, ab.atoms, w.cutoff)
-        sg = O.sg_positions(ab.atoms.x, ab.atoms.y, ab.atoms.z, ab.res_start, ab.amino_acid,
-                            0, len(ab.amino_acid))
-        O.disulfide_f32(*sg)
-        n_sc += 1
-        el = time.perf_counter() - t0
-        if el >= cpu_budget_s or n_sc >= 200000:
-            break
-    t_score = el / n_sc
-    cpu_val = npos / (trials * t_score) if trials else None
+    threads = host_threads()
+    rate, n_sc, el = AllatomScorer(ag.atoms, sample, w.cutoff).rate(threads, cpu_budget_s)
+    cpu_val = npos / (trials / rate) if trials else None
     out["cpu_baseline"] = {
-        "value": cpu_val, "unit": "positions/s", "cores": 1, "kind": "port",
+        "value": cpu_val, "unit": "positions/s", "cores": threads, "kind": "port",
+        "cpu": cpu_model(),
         "sample": (f"{n_sc} f32 allatom_energy scorings of the workload's antibodies with "
-                   f"interface positions in {el:.1f} s ({t_score * 1e3:.3f} ms each), times the "
-                   f"{trials} trials the reference's scan makes")}
+                   f"interface positions in {el:.1f} s on {threads} threads "
+                   f"({rate:.0f} per s), times the {trials} trials the reference's scan makes")}
     out["speedup_vs_cpu"] = (out["value"] / cpu_val) if cpu_val else None
     return out
 
@@ -339,18 +422,7 @@ def refine_bench(ctx, w, ag_dev, coords, survivors, cpu_budget_s=5.0):
     import dnab
     from dnab.api import RefineCall, mc_params
     from dnab.structure import AtomProtein, population_from_proteins
-    p = w.pop
-    off = p.cand_atom_off()
-    x, y, z = coords
-    prots = []
-    for c in range(p.n_cand):
-        r0, r1 = int(p.cand_res_off[c]), int(p.cand_res_off[c + 1])
-        cl = p.candidate_cloud(c)
-        a0, a1 = int(off[c]), int(off[c + 1])
-        cl.x, cl.y, cl.z = x[a0:a1].copy(), y[a0:a1].copy(), z[a0:a1].copy()
-        prots.append(AtomProtein(cl, (p.res_atom_off[r0:r1 + 1] - off[c]).astype(np.uint32),
-                                 p.res_amino_acid[r0:r1].copy(),
-                                 (p.res_ca_atom[r0:r1] - off[c]).astype(np.uint32)))
+    prots = candidate_proteins(w, coords)
     out = {}
     for name, cands, steps, noise, total in (
             ("refinement_64x600", [prots[int(i)] for i in survivors[:64]], 600, 1, 600),
@@ -620,9 +692,9 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": ("exact batched scoring at GPU-phase shape: 300-residue globule "
-                         "antigen vs 1,024 x 20-residue binders, 200 gradient steps, "
-                         "scored every 10, top-64 kept"),
+            "workload": (f"exact batched scoring at GPU-phase shape: 300-residue globule "
+                         f"antigen vs {w.pop.n_cand:,} x 20-residue binders, {args.steps} "
+                         f"gradient steps, scored every {w.score_every}, top-{w.top_k} kept"),
             "antigen_atoms": int(w.antigens[0].n_atoms()),
             "candidates": int(w.pop.n_cand),
             "candidate_atoms": int(len(w.pop.atoms)),
@@ -639,13 +711,17 @@ def main():
             "trajectory_tuning": pop.trajectory_tuning(),
         },
         "roofline": {
-            "bound": "mfma",
-            "engine": "FP32 VALU (the 157.3 TFLOP/s FP32 peak is the vector and matrix figure)",
+            "bound": "valu",
+            "engine": "FP32 VALU (157.3 TFLOP/s, AMD's FP32 vector figure, assumes packed FP32)",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": measured_traffic(args.step_kernels),
+            "traffic_source": ("rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of the same kernel on "
+                               "this workload, scaled by a device copy of known size: " +
+                               ("profiles/pair_traffic_c2.txt" if args.step_kernels
+                                else "profiles/trajectory_traffic_c2.txt")),
             "kernel": kernel,
             "kernel_avg_ms": avg_ms,
             "kernel_launches": launches,
@@ -685,8 +761,11 @@ def main():
         log("[bench] learned prescreen")
         out["prescreen"] = prescreen_bench(ctx, args)
     if not args.no_cpu_baseline:
-        log("[bench] timing the CPU baseline")
-        cb = cpu_baseline(w, args.cpu_budget)
+        log("[bench] timing the CPU baseline on the loop's own states")
+        states, state_pairs = sample_states(pop, params(args.steps), args.steps, 20, (x0, y0, z0))
+        if not np.array_equal(states[-1][0], x1):
+            raise RuntimeError("the per-step API did not reproduce the loop's trajectory")
+        cb = cpu_baseline(w, states, state_pairs, args.cpu_budget)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
     print(json.dumps(out), flush=True)

@@ -346,6 +346,9 @@ int dnab_set_error(dnab_ctx* ctx, int code, const char* fmt, ...);
 namespace dnab {
 // internal evaluate flag: the counting variant of the E+F kernel
 constexpr unsigned kEvalCount = 1u << 8;
+// internal evaluate flag: a step of the loop (every pair term FP32, as in the
+// fused loop); without it clash pairs are FP64 (reported evaluations)
+constexpr unsigned kEvalLoop = 1u << 9;
 constexpr uint32_t kExcluded = 0xFFFFFFFFu;   // sort code: no pair, zero outputs
 constexpr uint32_t kNonFinite = 0xFFFFFFFEu;  // sort code: a NaN coordinate, NaN outputs
 // grid.hip

@@ -577,7 +577,7 @@ __device__ inline void write_fixed(const FixAcc& acc, uint32_t j, TileOut out) {
                              static_cast<float>(static_cast<double>(acc.a) * kFixUnit));
 }
 
-template <bool FORCES, bool COUNT>
+template <bool FORCES, bool COUNT, bool CLASH>
 __global__ void __launch_bounds__(256)
 k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
               const uint32_t* __restrict__ ag_tile_start, const uint32_t* __restrict__ perm,
@@ -667,6 +667,7 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
       }
       uint32_t cbv[kLoadAhead], cev[kLoadAhead];
       float4 L0[kLoadAhead], L1[kLoadAhead];
+      #pragma unroll
       for (int u = 0; u < kLoadAhead; ++u) {
         const uint32_t kk = k + u;
         cbv[u] = 0;
@@ -685,6 +686,7 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
           }
         }
       }
+      #pragma unroll
       for (int u = 0; u < kLoadAhead; ++u) {
         if (cev[u] <= cbv[u]) continue;
         // stop at this chunk once the tile has had its share (and the rest fits)
@@ -704,7 +706,7 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
         if (n_s == 0) continue;
         taken_atoms += n_s;
         stage_chunk(s_atoms[wave], mask, lane, L0[u], L1[u]);
-        chunk_pairs<FORCES, COUNT>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
+        chunk_pairs<FORCES, COUNT, CLASH>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
                                    hydb, rc2_lane, valid, acc, n_in, n_tested);
       }
     }
@@ -752,7 +754,7 @@ k_pair_chunks(const AgView* __restrict__ views, uint32_t n_ag,
 // The chunks a tile's wave handed over: WAVES waves take them from an
 // LDS counter, sum per chunk into fixed-point integers, combine in LDS, add
 // the wave's partial sums and write the tile's outputs.
-template <bool FORCES, bool COUNT, int WAVES>
+template <bool FORCES, bool COUNT, bool CLASH, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64)
 k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
             const uint32_t* __restrict__ ag_tile_start, const uint32_t* __restrict__ perm,
@@ -822,7 +824,7 @@ k_pair_rest(const AgView* __restrict__ views, uint32_t n_ag,
     const uint32_t n_s = static_cast<uint32_t>(__builtin_popcountll(mask));
     if (n_s) {
       stage_chunk(s_atoms[wave], mask, lane, A0, A1);
-      chunk_pairs<FORCES, COUNT>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
+      chunk_pairs<FORCES, COUNT, CLASH>(s_atoms[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
                                  hydb, rc2_lane, valid, acc, n_in, n_tested);
     }
     k = k2;
@@ -1148,30 +1150,32 @@ static PairHandover pair_handover(dnab_population* pop) {
   return h;
 }
 
-template <bool FORCES, bool COUNT>
+// CLASH: clash pairs in FP64 (evaluations whose results are reported);
+// without it every pair term is FP32 (the per-step loop, as the fused one).
+template <bool FORCES, bool COUNT, bool CLASH>
 static void launch_pair(dnab_ctx* ctx, dnab_population* pop, const TileOut& o) {
   const uint32_t n_ag = static_cast<uint32_t>(pop->n_ag);
   const PairHandover h = pair_handover(pop);
   const uint32_t cap = static_cast<uint32_t>(pop->tile_cap ? pop->tile_cap : 1);
   if (pop->ho_dirty) hipLaunchKernelGGL(k_pair_list_reset, dim3(1), dim3(64), 0, ctx->stream, h.n_list);
   pop->ho_dirty = true;  // cleared by the next sort
-  hipLaunchKernelGGL((k_pair_chunks<FORCES, COUNT>), dim3((cap + kTileWaves - 1) / kTileWaves),
+  hipLaunchKernelGGL((k_pair_chunks<FORCES, COUNT, CLASH>), dim3((cap + kTileWaves - 1) / kTileWaves),
                      dim3(256), 0, ctx->stream, pop->views.p, n_ag, pop->ag_tile_start.p,
                      pop->perm.p, pop->spos.p, pop->spar.p, o, h, pop->tune);
   const dim3 rest_grid(static_cast<uint32_t>(ctx->cu_count * 2));
   switch (pop->tune.rest_waves) {
     case 4:
-      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, 4>), rest_grid, dim3(4 * 64), 0, ctx->stream,
+      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, CLASH, 4>), rest_grid, dim3(4 * 64), 0, ctx->stream,
                          pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->spos.p,
                          pop->spar.p, o, h);
       break;
     case 16:
-      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, 16>), rest_grid, dim3(16 * 64), 0, ctx->stream,
+      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, CLASH, 16>), rest_grid, dim3(16 * 64), 0, ctx->stream,
                          pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->spos.p,
                          pop->spar.p, o, h);
       break;
     default:
-      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, 8>), rest_grid, dim3(8 * 64), 0, ctx->stream,
+      hipLaunchKernelGGL((k_pair_rest<FORCES, COUNT, CLASH, 8>), rest_grid, dim3(8 * 64), 0, ctx->stream,
                          pop->views.p, n_ag, pop->ag_tile_start.p, pop->perm.p, pop->spos.p,
                          pop->spar.p, o, h);
   }
@@ -1193,20 +1197,22 @@ int population_pair(dnab_ctx* ctx, dnab_population* pop, unsigned flags) {
                        pop->par.p, o);
     timer_end(ctx, "pair_breakdown", ev, ev_end);
   } else if (flags & kEvalCount) {
-    launch_pair<true, true>(ctx, pop, o);
+    if (flags & kEvalLoop) launch_pair<true, true, false>(ctx, pop, o);
+    else launch_pair<true, true, true>(ctx, pop, o);
   } else if (flags & DNAB_EVAL_FORCES) {
     hipEvent_t ev_end;
     hipEvent_t ev = timer_begin(ctx, "pair_ef", &ev_end);
-    launch_pair<true, false>(ctx, pop, o);
+    if (flags & kEvalLoop) launch_pair<true, false, false>(ctx, pop, o);
+    else launch_pair<true, false, true>(ctx, pop, o);
     timer_end(ctx, "pair_ef", ev, ev_end);
   } else {
     hipEvent_t ev_end;
     hipEvent_t ev = timer_begin(ctx, "pair_e", &ev_end);
-    launch_pair<false, false>(ctx, pop, o);
+    launch_pair<false, false, true>(ctx, pop, o);
     timer_end(ctx, "pair_e", ev, ev_end);
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
-  pop->last_flags = (flags & kEvalCount) ? (flags | DNAB_EVAL_FORCES) & ~kEvalCount : flags;
+  pop->last_flags = ((flags & kEvalCount) ? (flags | DNAB_EVAL_FORCES) : flags) & ~(kEvalCount | kEvalLoop);
   pop->evaluated = true;
   return DNAB_OK;
 }
@@ -1231,7 +1237,7 @@ int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned fla
     }
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
-  pop->last_flags = (flags & kEvalCount) ? (flags | DNAB_EVAL_FORCES) & ~kEvalCount : flags;
+  pop->last_flags = ((flags & kEvalCount) ? (flags | DNAB_EVAL_FORCES) : flags) & ~(kEvalCount | kEvalLoop);
   pop->evaluated = true;
   return DNAB_OK;
 }
@@ -1242,7 +1248,7 @@ static int pair_count(dnab_ctx* ctx, dnab_population* pop, uint64_t* in_cutoff,
   unsigned long long zero[2] = {0, 0};
   DNAB_RETURN_IF(upload(ctx, pop->pair_stats.p, zero, sizeof(zero)));
   TileOut o{pop->force.p, pop->e_atom.p, pop->bd_atom.p, pop->pair_stats.p};
-  launch_pair<true, true>(ctx, pop, o);
+  launch_pair<true, true, true>(ctx, pop, o);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   unsigned long long h[2];
   DNAB_RETURN_IF(download(ctx, h, pop->pair_stats.p, sizeof(h)));
@@ -1880,13 +1886,13 @@ int dnab_population_autotune(dnab_ctx* ctx, dnab_population* pop, uint32_t reps,
   PairTuning win = saved;
   double win_ms = 1e30;
   int rc = DNAB_OK;
-  launch_pair<true, false>(ctx, pop, o);  // warm-up
+  launch_pair<true, false, true>(ctx, pop, o);  // warm-up
   for (uint32_t w : waves)
     for (uint32_t c : chunks)
       for (uint32_t sv : surv) {
         pop->tune = PairTuning{sv, c, w};
         (void)hipEventRecord(e0, ctx->stream);
-        for (uint32_t r = 0; r < reps; ++r) launch_pair<true, false>(ctx, pop, o);
+        for (uint32_t r = 0; r < reps; ++r) launch_pair<true, false, true>(ctx, pop, o);
         (void)hipEventRecord(e1, ctx->stream);
         if (hipEventSynchronize(e1) != hipSuccess) {
           rc = dnab_set_error(ctx, DNAB_ERR_HIP, "autotune: launch failed");

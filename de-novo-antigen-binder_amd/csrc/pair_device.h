@@ -14,7 +14,12 @@
 // epsilons: its LJ terms reach 1e10 kcal/mol and 1e11 kcal/mol/A, where the
 // FP32 terms' own rounding (~1e-6 of each) would exceed the tolerance of a
 // force component in which two such pairs nearly cancel. Every other pair has
-// |F| below ~30 kcal/mol/A and stays in FP32.
+// |F| below ~70 kcal/mol/A and stays in FP32. Per (atom, chunk) the FP32 sum
+// and the FP64 sum of the clash pairs are each rounded once to the integer
+// unit; the clash pairs are found in the pair loop at the cost of two
+// instructions per pair and evaluated after it (hot_chunk). Evaluations
+// whose results are reported take this path (CLASH); the loop's steps do not
+// need it and keep every pair in FP32.
 //
 // Saturation. An integer sum is bounded by +-2^59 units (+-5.5e11); a sum
 // that reaches the bound is clamped and its component is marked in the
@@ -27,7 +32,7 @@ namespace dnab {
 
 constexpr float kMinRSq = 0.25f;   // src/energy.rs:17
 constexpr float kHydroSq = 36.0f;  // src/energy.rs:20
-constexpr float kHotS2 = 1.5f;     // (R/r)^2 above which a pair is evaluated in FP64
+constexpr float kHotS2 = 2.0f;     // (R/r)^2 above which a pair is evaluated in FP64
 constexpr int kLoadAhead = 4;   // chunk loads in flight per round trip
 constexpr double kFixScale = 1048576.0;          // 2^20
 constexpr double kFixUnit = 1.0 / 1048576.0;
@@ -124,26 +129,38 @@ __device__ inline void pair_value(float4 B0, float4 B1, float qb, float rmhb, fl
   ep = elj + ecl;
 }
 
-// The FP64 evaluation of a clash pair: energy and, with FORCES, the force
-// added to (hx, hy, hz), from the exact differences of the f32 coordinates,
-// eps_ij = sqrt(eps_i eps_j) from the raw epsilons, as the all-FP64 kernel
-// forms them. Returns the pair's contribution to the error estimate's
-// sum |e| (the FP32 rounding of 332 q, and a margin on the LJ term).
+// 1/sqrt(x) in FP64 for x > 0 (normal): the hardware estimate refined by
+// two Newton steps y <- y (3 - x y^2) / 2, each of which squares the relative
+// error, to below 1e-16 relative from the estimate's ~2^-26.
+__device__ inline double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * __builtin_fma(-0.5 * x, y * y, 1.5);
+  y = y * __builtin_fma(-0.5 * x, y * y, 1.5);
+  return y;
+}
+
+// The FP64 evaluation of a clash pair, from the exact differences of the
+// f32 coordinates and eps_ij = sqrt(eps_i eps_j) from the raw epsilons, as
+// the all-FP64 kernel forms them (1/r and the square root through
+// rsqrt_f64): adds its energy to he and, with FORCES, its force to
+// (hx, hy, hz); returns its share of the error estimate's sum |e| (the FP32
+// rounding of 332 q and a margin on the LJ term).
 template <bool FORCES>
-__device__ inline float hot_pair(float4 B0, float4 B1, float xb, float yb, float zb, float qb,
-                                 float rmhb, float epsb, uint32_t h, double& he, double& hx,
-                                 double& hy, double& hz) {
+__device__ inline double hot_pair(float4 B0, float4 B1, float xb, float yb, float zb, float qb,
+                                  float rmhb, float epsb, uint32_t h, double& he, double& hx,
+                                  double& hy, double& hz) {
   const double ddx = static_cast<double>(xb) - static_cast<double>(B0.x);
   const double ddy = static_cast<double>(yb) - static_cast<double>(B0.y);
   const double ddz = static_cast<double>(zb) - static_cast<double>(B0.z);
-  const double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;
-  const double rinv2 = 1.0 / r2d;
-  const double rinv = sqrt(rinv2);
+  const double r2d = (ddx * ddx + ddy * ddy) + ddz * ddz;  // >= 0.25 (the predicate)
+  const double rinv = rsqrt_f64(r2d);
+  const double rinv2 = rinv * rinv;
   const double R = static_cast<double>(B1.x) + static_cast<double>(rmhb);
   const double s2 = (R * R) * rinv2;
   const double s6 = (s2 * s2) * s2;
   const double s12 = s6 * s6;
-  const double eps = sqrt(static_cast<double>(B1.z) * static_cast<double>(epsb));
+  const double pe = static_cast<double>(B1.z) * static_cast<double>(epsb);  // exact
+  const double eps = pe > 0.0 ? pe * rsqrt_f64(pe) : 0.0;
   const double elj = eps * (s12 - 2.0 * s6);
   const double ecl = (static_cast<double>(B0.w) * static_cast<double>(qb)) * rinv;
   he += elj + ecl;
@@ -154,32 +171,96 @@ __device__ inline float hot_pair(float4 B0, float4 B1, float xb, float yb, float
     hy += f * ddy;
     hz += f * ddz;
   }
-  return static_cast<float>(fabs(ecl) * (1.0 / 32.0) + fabs(elj) * 0x1p-26);
+  return fabs(ecl) * (1.0 / 32.0) + fabs(elj) * 0x1p-26;
+}
+
+// FP64 sums of one lane's clash pairs with one chunk, in chunk order.
+struct HotSum {
+  double e = 0.0, x = 0.0, y = 0.0, z = 0.0, a = 0.0;
+  bool mine = false;  // the lane has a clash pair in the chunk
+};
+
+// The pair of the lane's atom with antigen atom (B0, B1) when it is a clash
+// pair: the same f32 predicate and test as the pair loop, then hot_pair.
+template <bool FORCES>
+__device__ inline void hot_add(float4 B0, float4 B1, float xb, float yb, float zb, float qb,
+                               float rmhb, float epsb, bool hydb, float rc2_lane, HotSum& hs) {
+  float dx, dy, dz, r2;
+  bool in;
+  pair_pred(B0, xb, yb, zb, rc2_lane, dx, dy, dz, r2, in);
+  const float R = B1.x + rmhb;
+  const bool hot = in && (R * R) > kHotS2 * r2;
+  if (ballot64(hot)) {
+    if (hot) {
+      const uint32_t h = (tag_hyd(B1.w) && hydb && r2 < kHydroSq) ? 1u : 0u;
+      hs.a += hot_pair<FORCES>(B0, B1, xb, yb, zb, qb, rmhb, epsb, h, hs.e, hs.x, hs.y, hs.z);
+      hs.mine = true;
+    }
+  }
+}
+
+// Rounds a chunk's clash sums once into `acc` (separately from the chunk's
+// FP32 sum) and clears them.
+template <bool FORCES>
+__device__ inline void hot_fold(HotSum& hs, FixAcc& acc) {
+  if (hs.mine) {
+    acc.e = fix_add(acc.e, fix_round(hs.e, acc.st, kStE), acc.st, kStE);
+    uint32_t ignore = 0;
+    acc.a = fix_add(acc.a, fix_round(hs.a + kRoundAbs, ignore, 0u), ignore, 0u);
+    if (FORCES) {
+      acc.x = fix_add(acc.x, fix_round(hs.x, acc.st, kStX), acc.st, kStX);
+      acc.y = fix_add(acc.y, fix_round(hs.y, acc.st, kStY), acc.st, kStY);
+      acc.z = fix_add(acc.z, fix_round(hs.z, acc.st, kStZ), acc.st, kStZ);
+    }
+  }
+  hs = HotSum();
+}
+
+// The clash pairs of the lane's atom with the `n_s` antigen atoms staged in
+// `st`: summed in FP64 in chunk order and rounded once per (atom, chunk) into
+// `acc`, separately from the chunk's FP32 sum. A clash pair is always among
+// the staged atoms, whichever tile box staged them, so every path that sums
+// a chunk's clash pairs in chunk order forms the same sum.
+template <bool FORCES>
+__device__ inline void hot_chunk(const float4 (*st)[2], uint32_t n_s, float xb, float yb, float zb,
+                                 float qb, float rmhb, float epsb, bool hydb, float rc2_lane,
+                                 FixAcc& acc) {
+  HotSum hs;
+  for (uint32_t k = 0; k < n_s; ++k)
+    hot_add<FORCES>(st[k][0], st[k][1], xb, yb, zb, qb, rmhb, epsb, hydb, rc2_lane, hs);
+  hot_fold<FORCES>(hs, acc);
 }
 
 // Pairs of the lane's atom with the `n_s` antigen atoms staged in `st` (in
 // chunk order); folds the chunk's sums into `acc`. The predicate is formed
 // for two atoms at a time; an atom in range of no lane of the wave is
 // skipped (each of its terms would add an exact zero), the others are
-// added in chunk order, which keeps the sums those of a plain loop. Clash
-// pairs go to hot_pair (a branch taken by the wave only when one of its
-// lanes has one). A chunk with no pair in range in the whole wave leaves
-// `acc` as it is (its sums are exact zeros), without the integer conversions.
-template <bool FORCES, bool COUNT>
+// added in chunk order, which keeps the sums those of a plain loop. With
+// CLASH a clash pair adds exact zeros to these FP32 sums and the chunk's
+// clash pairs are then summed in FP64 by hot_chunk; without it every pair is
+// an FP32 term (the loop's steps: a clash force moves its residue by the
+// clamped step whatever its rounding). A chunk with no pair in range in the
+// whole wave leaves `acc` as it is (its sums are exact zeros), without the
+// integer conversions.
+template <bool FORCES, bool COUNT, bool CLASH>
 __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb, float yb,
                                    float zb, float qb, float rmhb, float sqeb, float sqe12b,
                                    float epsb, bool hydb, float rc2_lane, bool valid, FixAcc& acc,
                                    uint32_t& n_in, uint32_t& n_tested) {
   float fx = 0.f, fy = 0.f, fz = 0.f, es = 0.f, ec = 0.f, eabs = 0.f;
-  double he = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;  // clash pairs, FP64
   uint32_t nh = 0;
-  uint64_t any = 0;   // wave-uniform
-  bool mine = false;  // the lane's atom has a pair in this chunk
+  uint64_t any = 0;     // wave-uniform
+  bool mine = false;    // the lane's atom has a pair in this chunk
+  bool anyhot = false;  // ... a clash pair (CLASH)
   auto add = [&](const float4& B0, const float4& B1, float dx, float dy, float dz, float r2,
                  bool in) {
+    bool hot = false;
+    if (CLASH) {
+      const float R = B1.x + rmhb;
+      hot = in && (R * R) > kHotS2 * r2;
+      anyhot = anyhot || hot;
+    }
     mine = mine || in;
-    const float R = B1.x + rmhb;
-    const bool hot = in && (R * R) > kHotS2 * r2;
     float ep, f;
     uint32_t h;
     pair_value<FORCES>(B0, B1, qb, rmhb, sqeb, sqe12b, hydb, r2, in, hot, ep, f, h);
@@ -194,9 +275,6 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
     es = sum;
     ec += err;
     eabs += fabsf(ep);
-    if (ballot64(hot)) {
-      if (hot) eabs += hot_pair<FORCES>(B0, B1, xb, yb, zb, qb, rmhb, epsb, h, he, hx, hy, hz);
-    }
   };
   uint32_t q = 0;
   for (; q + 2 <= n_s; q += 2) {
@@ -230,19 +308,21 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
   if (!any) return;
   // hydrophobic: -0.5 per pair, exactly -2^19 in fixed point
   const long long eh = -static_cast<long long>(nh) << 19;
-  const double e_chunk = (static_cast<double>(es) + static_cast<double>(ec)) + he;
+  const double e_chunk = static_cast<double>(es) + static_cast<double>(ec);
   acc.e = fix_add(acc.e, fix_round(e_chunk, acc.st, kStE) + eh, acc.st, kStE);
   // sum |e| feeds the error estimate only; an atom's energy rounding can err
   // only in a chunk where it has a pair (a chunk is fixed by the antigen, so
   // this count does not depend on the tiles)
   uint32_t ignore = 0;
-  const float ea = mine ? eabs + kRoundAbs : eabs;
-  acc.a = fix_add(acc.a, fix_round(static_cast<double>(ea), ignore, 0u), ignore, 0u);
+  acc.a = fix_add(acc.a, fix_round(static_cast<double>(mine ? eabs + kRoundAbs : eabs), ignore, 0u),
+                  ignore, 0u);
   if (FORCES) {
-    acc.x = fix_add(acc.x, fix_round(static_cast<double>(fx) + hx, acc.st, kStX), acc.st, kStX);
-    acc.y = fix_add(acc.y, fix_round(static_cast<double>(fy) + hy, acc.st, kStY), acc.st, kStY);
-    acc.z = fix_add(acc.z, fix_round(static_cast<double>(fz) + hz, acc.st, kStZ), acc.st, kStZ);
+    acc.x = fix_add(acc.x, fix_round(static_cast<double>(fx), acc.st, kStX), acc.st, kStX);
+    acc.y = fix_add(acc.y, fix_round(static_cast<double>(fy), acc.st, kStY), acc.st, kStY);
+    acc.z = fix_add(acc.z, fix_round(static_cast<double>(fz), acc.st, kStZ), acc.st, kStZ);
   }
+  if (CLASH && ballot64(anyhot))
+    hot_chunk<FORCES>(st, n_s, xb, yb, zb, qb, rmhb, epsb, hydb, rc2_lane, acc);
 }
 
 // Stage the chunk atoms selected by `mask` (in chunk order) into `st`.

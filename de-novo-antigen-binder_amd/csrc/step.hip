@@ -357,7 +357,7 @@ static int enqueue_chunk(dnab_ctx* ctx, dnab_population* pop, const dnab_langevi
     const bool eval = allatom || scoring;
     if (eval) {
       DNAB_RETURN_IF(population_sort(ctx, pop));  // also advances the step counter
-      DNAB_RETURN_IF(population_pair(ctx, pop, p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES));
+      DNAB_RETURN_IF(population_pair(ctx, pop, kEvalLoop | (p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES)));
     } else {
       hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p);
     }
@@ -410,7 +410,7 @@ int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_
   const bool allatom = p->force_model == DNAB_FORCE_ALLATOM;
   if (allatom) {
     DNAB_RETURN_IF(population_sort(ctx, pop));  // counter[1] = iter
-    DNAB_RETURN_IF(population_pair(ctx, pop, DNAB_EVAL_FORCES));
+    DNAB_RETURN_IF(population_pair(ctx, pop, kEvalLoop | DNAB_EVAL_FORCES));
   } else {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p);
   }
@@ -536,7 +536,8 @@ int step_kernel_loop(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_pa
     const uint32_t len = std::min(L, p->n_steps - done);
     DNAB_RETURN_IF(enqueue_chunk(ctx, pop, p, sp, len, &tk, p->score_every != 0));
   }
-  // final scoring after the last update
+  // final scoring after the last update (clash pairs in FP64: the reported
+  // energies and forces)
   DNAB_RETURN_IF(population_sort(ctx, pop));
   DNAB_RETURN_IF(population_pair(ctx, pop, p->count_pairs ? kEvalCount : DNAB_EVAL_FORCES));
   DNAB_RETURN_IF(population_reduce_energies(ctx, pop, DNAB_EVAL_FORCES));

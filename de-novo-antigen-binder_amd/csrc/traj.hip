@@ -16,13 +16,16 @@
 //     population pair kernels do (chunk_pairs, pair_device.h) and adds its
 //     integer sums into the tile's LDS accumulators. Because every (atom,
 //     chunk) sum is rounded once to 2^-20 and chunk sums add exactly, the
-//     per-atom results are bit-identical to the population kernels';
+//     per-atom results are bit-identical to the population kernels' on the
+//     loop's steps (whose pair terms are all FP32: a clash force moves its
+//     residue by the clamped step whatever its rounding);
 //  3. per-atom energies and forces from the integer sums;
 //  4. the residue update (residue_delta, step_device.h), one thread per residue;
 //  5. the rigid residue moves, one thread per atom.
-// After the last step the state is evaluated once more, the candidate energy
-// is reduced in the order of the population reduction kernel, and positions,
-// forces and energies are written back.
+// After the last step the positions are written back; a launch with no
+// steps, whose clash pairs are FP64 (CLASH), evaluates the final state and
+// writes the reported energies and forces, the candidate energy reduced in
+// the order of the population reduction kernel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -98,7 +101,9 @@ struct TrajArgs {
   uint32_t n_cand, n_steps, score_every;
   uint32_t s0;              // loop index of this launch's first step
   uint32_t n_total;         // steps of the whole loop (the final scoring's row)
-  int32_t final_eval;       // this launch ends the loop: evaluate the final state
+  int32_t final_eval;       // evaluate the state after the last step: the zero-step launches
+                            // that measure each candidate's work and that evaluate the
+                            // loop's final state
   uint32_t n_groups;        // with a table
   int32_t cells;            // grid cells per work item (tiles whose chunk list overflowed)
   uint32_t chunks;          // listed chunks per work item (1..kLoadAhead)
@@ -135,7 +140,9 @@ __device__ inline bool traj_can_pair(const TrajAg& V, float4 p) {
 // atoms share the LDS slot space (each candidate padded to whole tiles) and
 // all WAVES waves share the pair work of all of them, so a candidate in deep
 // contact is worked on by every wave of its workgroup.
-template <bool COUNT, int WAVES>
+// CLASH: clash pairs in FP64, for the launch that evaluates the final state
+// (no steps); the loop's launches keep every pair term in FP32.
+template <bool COUNT, int WAVES, bool CLASH>
 __global__ void __launch_bounds__(WAVES * 64)
 k_trajectory(TrajArgs g) {
   extern __shared__ __align__(16) unsigned char s_dyn[];
@@ -422,6 +429,7 @@ k_trajectory(TrajArgs g) {
         // the atoms near the tile box are staged and paired in chunk order
         auto run_chunks = [&](const uint32_t (&cbv)[kLoadAhead], const uint32_t (&cev)[kLoadAhead]) {
           float4 L0[kLoadAhead], L1[kLoadAhead];
+          #pragma unroll
           for (int u = 0; u < kLoadAhead; ++u) {
             L0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             L1[u] = L0[u];
@@ -430,6 +438,7 @@ k_trajectory(TrajArgs g) {
               L1[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].rmh);
             }
           }
+          #pragma unroll
           for (int u = 0; u < kLoadAhead; ++u) {
             if (cev[u] <= cbv[u]) continue;
             const float bx = fmaxf(fmaxf(lox - L0[u].x, L0[u].x - hix), 0.0f);
@@ -441,7 +450,7 @@ k_trajectory(TrajArgs g) {
             if (n_s == 0) continue;
             if (lane == 0 && g.work) atomicAdd(&s_work[tb.cand], n_s);
             stage_chunk(s_stage[wave], mask, lane, L0[u], L1[u]);
-            chunk_pairs<true, COUNT>(s_stage[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
+            chunk_pairs<true, COUNT, CLASH>(s_stage[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
                                      hydb, rc2_lane, valid, acc, n_in, n_tested);
           }
         };
@@ -450,6 +459,7 @@ k_trajectory(TrajArgs g) {
           // listed chunks b * g.chunks ...
           const uint32_t base = b * g.chunks;
           const uint32_t* desc = s_desc + t * kTileChunks;
+          #pragma unroll
           for (int u = 0; u < kLoadAhead; ++u) {
             cbv[u] = 0;
             cev[u] = 0;
@@ -486,6 +496,7 @@ k_trajectory(TrajArgs g) {
           const uint32_t off = cincl - nch;
           const uint32_t nchunk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cincl), 63));
           for (uint32_t kc = 0; kc < nchunk; kc += kLoadAhead) {
+            #pragma unroll
             for (int u = 0; u < kLoadAhead; ++u) {
               const uint32_t kk = kc + u;
               cbv[u] = 0;
@@ -599,9 +610,9 @@ k_trajectory(TrajArgs g) {
   }
 }
 
-template <bool COUNT, int WAVES>
+template <bool COUNT, int WAVES, bool CLASH>
 static hipError_t launch_traj(const TrajArgs& a, size_t lds, hipStream_t stream) {
-  auto kern = k_trajectory<COUNT, WAVES>;
+  auto kern = k_trajectory<COUNT, WAVES, CLASH>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
   if (e != hipSuccess) return e;
@@ -756,13 +767,22 @@ static bool traj_balance(const dnab_population* pop, uint32_t waves, uint32_t cp
   return true;
 }
 
-static hipError_t launch_plan(const TrajArgs& a, const TrajPlan& pl, bool count, hipStream_t st) {
+template <bool CLASH>
+static hipError_t launch_plan_c(const TrajArgs& a, const TrajPlan& pl, bool count, hipStream_t st) {
   switch (pl.waves) {
-    case 4: return count ? launch_traj<true, 4>(a, pl.lds, st) : launch_traj<false, 4>(a, pl.lds, st);
-    case 8: return count ? launch_traj<true, 8>(a, pl.lds, st) : launch_traj<false, 8>(a, pl.lds, st);
-    case 16: return count ? launch_traj<true, 16>(a, pl.lds, st) : launch_traj<false, 16>(a, pl.lds, st);
-    default: return hipErrorInvalidValue;
+    case 4:
+      return count ? launch_traj<true, 4, CLASH>(a, pl.lds, st) : launch_traj<false, 4, CLASH>(a, pl.lds, st);
+    case 8:
+      return count ? launch_traj<true, 8, CLASH>(a, pl.lds, st) : launch_traj<false, 8, CLASH>(a, pl.lds, st);
+    case 16:
+      return count ? launch_traj<true, 16, CLASH>(a, pl.lds, st) : launch_traj<false, 16, CLASH>(a, pl.lds, st);
+    default:
+      return hipErrorInvalidValue;
   }
+}
+static hipError_t launch_plan(const TrajArgs& a, const TrajPlan& pl, bool count, bool clash,
+                              hipStream_t st) {
+  return clash ? launch_plan_c<true>(a, pl, count, st) : launch_plan_c<false>(a, pl, count, st);
 }
 
 int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
@@ -839,7 +859,7 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
     e.n_total = 0;
     e.score_every = 0;
     e.work = pop->traj_work.p;
-    DNAB_HIP_CHECK(ctx, launch_plan(e, pl, false, ctx->stream));
+    DNAB_HIP_CHECK(ctx, launch_plan(e, pl, false, false, ctx->stream));
     DNAB_RETURN_IF(download(ctx, work.data(), pop->traj_work.p, nc * sizeof(uint32_t)));
     DNAB_RETURN_IF(regroup());
   }
@@ -851,11 +871,11 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
     const uint32_t len = std::min(seg, p->n_steps - done);
     a.s0 = done;
     a.n_steps = len;
-    a.final_eval = done + len == p->n_steps ? 1 : 0;
-    a.work = balance && !a.final_eval ? pop->traj_work.p : nullptr;
+    a.final_eval = 0;
+    a.work = balance && done + len < p->n_steps ? pop->traj_work.p : nullptr;
     hipEvent_t ev_end;
     hipEvent_t ev = timer_begin(ctx, "trajectory", &ev_end);
-    hipError_t e = launch_plan(a, pl, count, ctx->stream);
+    hipError_t e = launch_plan(a, pl, count, false, ctx->stream);
     timer_end(ctx, "trajectory", ev, ev_end);
     if (e != hipSuccess)
       return dnab_set_error(ctx, DNAB_ERR_HIP, "trajectory launch failed: %s", hipGetErrorString(e));
@@ -865,6 +885,18 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
       DNAB_RETURN_IF(regroup());
     }
   } while (done < p->n_steps);
+  // the final state: energies, forces and the last scoring row, with the
+  // clash pairs in FP64 as every reported evaluation has them
+  a.s0 = p->n_steps;
+  a.n_steps = 0;
+  a.final_eval = 1;
+  a.work = nullptr;
+  hipEvent_t ev_end;
+  hipEvent_t ev = timer_begin(ctx, "trajectory_final", &ev_end);
+  hipError_t e = launch_plan(a, pl, count, true, ctx->stream);
+  timer_end(ctx, "trajectory_final", ev, ev_end);
+  if (e != hipSuccess)
+    return dnab_set_error(ctx, DNAB_ERR_HIP, "trajectory launch failed: %s", hipGetErrorString(e));
   return DNAB_OK;
 }
 
