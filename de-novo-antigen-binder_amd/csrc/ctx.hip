@@ -221,6 +221,9 @@ int dnab_ctx_create(int device, dnab_ctx** out) {
     delete ctx;
     return DNAB_ERR_HIP;
   }
+  // the no-alloc entry's scratch address on this device (once per device; on
+  // failure that entry reports it and everything else works)
+  (void)dnab::noalloc_init_device(device);
   *out = ctx;
   return DNAB_OK;
 }

@@ -378,4 +378,6 @@ int population_reduce_energies(dnab_ctx* ctx, dnab_population* pop, unsigned fla
 int require_finite(dnab_ctx* ctx, const dnab_population* pop, const char* what);
 // parameters of an atom cloud: finite charge, r_min_half >= 0 and epsilon >= 0
 bool atom_params_ok(const dnab_atom_cloud* c);
+// noalloc.hip: the no-alloc entry's scratch address on `device` (current)
+int noalloc_init_device(int device);
 }  // namespace dnab

@@ -204,6 +204,8 @@ _PROTOS = {
                                    f32p, f32p, f32p, f64p]),
     "dnab_score_batch": (C.c_int, [vp, C.POINTER(AtomCloudC), C.POINTER(GpuAtom), C.c_size_t,
                                    C.c_size_t, f32p, C.c_size_t, szp]),
+    "dnab_score_batch_noalloc": (C.c_int, [vp, vp, vp, C.c_size_t, C.c_size_t, vp, C.c_size_t,
+                                           szp]),
     "dnab_antigen_create": (C.c_int, [vp, C.POINTER(AntigenDescC), C.POINTER(vp)]),
     "dnab_antigen_info": (C.c_int, [vp, szp, szp, f32p, f32p]),
     "dnab_antigen_destroy": (None, [vp]),

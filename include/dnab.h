@@ -308,6 +308,22 @@ int dnab_atom_forces(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
 int dnab_score_batch(dnab_ctx* ctx, const dnab_atom_cloud* antigen,
                      const dnab_gpu_atom* candidates, size_t n_atoms, size_t n_ab,
                      float* energies_out, size_t capacity, size_t* n_cand);
+/* dnab_score_batch without any allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). The antigen is a
+ * dnab_antigen (its cutoff applies; dnab_score_batch's is 10 A with 10 A
+ * cells); candidates_dev (n_atoms GpuAtom records) and energies_dev
+ * (capacity floats) are device memory owned by the caller. The work is
+ * queued on the context's stream and the call returns without waiting:
+ * energies_dev is ready when the stream has completed (dnab_ctx_synchronize).
+ * Energies equal dnab_score_batch's on the same atoms and antigen; a NaN
+ * coordinate gives NaN, a saturated sum +inf. The per-candidate sums live
+ * in one fixed device array per GPU: at most 65,536 candidates per call
+ * (DNAB_ERR_UNSUPPORTED beyond); calls that overlap on one GPU (from two
+ * contexts or threads) race on that array. */
+int dnab_score_batch_noalloc(dnab_ctx* ctx, const dnab_antigen* antigen,
+                             const dnab_gpu_atom* candidates_dev, size_t n_atoms, size_t n_ab,
+                             float* energies_dev, size_t capacity, size_t* n_cand);
 
 /* ---- device-resident antigen ---------------------------------------------
  * The antigen's atoms are sorted into a Morton-ordered, cell-major SoA over
