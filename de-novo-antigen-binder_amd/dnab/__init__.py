@@ -10,7 +10,7 @@ from ._native import DNAB_STATUS_SATURATED as STATUS_SATURATED
 from ._native import DnabError, device_count, visible_gpu
 from .api import (AtomEnergyBreakdown, Dataset, DeviceAntigen, DevicePopulation, GpuContext,
                   Mlp, SpatialHashGrid, exact_topk, langevin_params, population_features,
-                  precision_study)
+                  precision_study, run_allatom)
 from .structure import (AMINO_ONE_LETTER, AtomCloud, AtomProtein, PopulationHost, ResidueCloud,
                         aa_from_sequence, parse_peptide, population_from_proteins,
                         protein_from_ca_trace)
@@ -21,5 +21,5 @@ __all__ = [
     "DevicePopulation", "DnabError", "GpuContext", "PopulationHost", "ResidueCloud",
     "STATUS_NAN", "STATUS_SATURATED",
     "SpatialHashGrid", "aa_from_sequence", "device_count", "langevin_params", "parse_peptide",
-    "population_from_proteins", "protein_from_ca_trace", "visible_gpu", "_native",
+    "population_from_proteins", "protein_from_ca_trace", "run_allatom", "visible_gpu", "_native",
 ]

@@ -119,6 +119,14 @@ class McParamsC(C.Structure):
                 ("iface_cutoff_sq", C.c_float), ("moves", C.c_uint32), ("waves", C.c_uint32)]
 
 
+class RunAllatomParamsC(C.Structure):
+    _fields_ = [("ab_length", C.c_uint32), ("population", C.c_uint32),
+                ("cpu_iterations", C.c_uint32), ("top_n", C.c_uint32), ("n_broad", C.c_uint32),
+                ("broad_steps", C.c_uint32), ("broad_moves", C.c_uint32),
+                ("_reserved", C.c_uint32), ("seed", C.c_uint64), ("ring_radius", C.c_float),
+                ("_pad", C.c_float)]
+
+
 class LangevinParamsC(C.Structure):
     _fields_ = [("n_steps", C.c_uint32), ("score_every", C.c_uint32), ("top_k", C.c_uint32),
                 ("seed", C.c_uint64), ("with_noise", C.c_int32),
@@ -260,6 +268,9 @@ _PROTOS = {
     "dnab_refine": (C.c_int, [vp, vp, C.POINTER(PopulationDescC), u8p, C.POINTER(LangevinParamsC),
                               C.POINTER(McParamsC), C.POINTER(ProteinOutC), u8p, f64p,
                               C.POINTER(C.c_size_t)]),
+    "dnab_run_allatom_default_params": (C.c_int, [C.POINTER(RunAllatomParamsC)]),
+    "dnab_run_allatom": (C.c_int, [vp, vp, C.POINTER(RunAllatomParamsC), C.POINTER(ProteinOutC),
+                                   f64p, C.POINTER(C.c_uint32), C.POINTER(C.c_size_t)]),
     "dnab_rebuild_side_chain": (C.c_int, [C.c_uint8, C.c_uint8, f32p, C.c_size_t]),
     "dnab_affinity_maturation": (C.c_int, [vp, vp, C.POINTER(PopulationDescC), u8p, u32p, u32p,
                                            C.POINTER(ProteinOutC), u8p, u8p, f64p,

@@ -363,6 +363,31 @@ def mc_params(**kw) -> N.McParamsC:
     return p
 
 
+def run_allatom(ctx: "GpuContext", antigen: "DeviceAntigen", **kw):
+    """run_allatom (src/diffusion.rs:983-1107) in one call (dnab_run_allatom):
+    keyword arguments set dnab_run_allatom_params fields (ab_length,
+    population, cpu_iterations, top_n, n_broad, broad_steps, broad_moves,
+    seed, ring_radius). Returns [(AtomProtein, allatom_energy, sequence)]
+    in ascending energy, as the reference's Vec<AllAtomResult>."""
+    p = N.RunAllatomParamsC()
+    N.check(N.lib.dnab_run_allatom_default_params(C.byref(p)))
+    for k, v in kw.items():
+        if k.startswith("_") or not hasattr(p, k):
+            raise AttributeError(f"unknown run_allatom parameter {k}")
+        setattr(p, k, v)
+    ab_len = p.ab_length or (len(antigen.ca) if antigen.ca is not None else 0)
+    pool = p.population if p.n_broad == 0 else min(p.population, p.n_broad)
+    n_keep = min(max(p.top_n, 1), pool)
+    buf = _ProteinOut(n_keep * ab_len)
+    e = np.zeros(max(n_keep, 1), np.float64)
+    n, na = C.c_uint32(0), C.c_size_t(0)
+    ctx.check(N.lib.dnab_run_allatom(ctx.handle, antigen.handle, C.byref(p), C.byref(buf.c),
+                                     N.ptr(e, C.c_double), C.byref(n), C.byref(na)))
+    off = np.arange(n.value + 1, dtype=np.uint32) * ab_len
+    prots = buf.proteins(off)
+    return [(pr, float(e[k]), pr.sequence()) for k, pr in enumerate(prots)]
+
+
 class RefineCall:
     """One dnab_refine call (allatom_diffusion_step for every candidate, all
     iterations in one launch) with inputs packed and outputs allocated up

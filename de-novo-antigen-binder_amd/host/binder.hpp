@@ -23,6 +23,7 @@
 //                                          a batch of antibodies in one call)
 //   binder::run_allatom_gpu_phase          src/diffusion.rs:1008-1062 (GPU phase,
 //                                          gradient steps + scoring, no MC moves)
+//   binder::run_allatom, AllAtomResult     src/diffusion.rs:93-97, 983-1107
 //
 // Errors: functions that the reference cannot fail return values as the
 // reference does; a failing C call throws binder::Error (the reference
@@ -436,7 +437,7 @@ inline void compute_forces(const GpuContext& gpu, const ResidueCloud& antigen,
   check(dnab_cg_forces(gpu.raw(), &a, &b, fx.data(), fy.data(), fz.data()), gpu.raw());
 }
 
-struct AllAtomResult {
+struct GpuPhaseResult {
   uint32_t candidate;  // index into the candidates passed in
   double energy;       // LJ + Coulomb + hydrophobic at the final scoring
 };
@@ -444,7 +445,7 @@ struct AllAtomResult {
 // The GPU phase of run_allatom (src/diffusion.rs:1008-1062) with every
 // candidate moving for params.n_steps gradient steps, scoring every
 // params.score_every steps and keeping params.top_k (default 64).
-inline std::vector<AllAtomResult> run_allatom_gpu_phase(
+inline std::vector<GpuPhaseResult> run_allatom_gpu_phase(
     const GpuContext& gpu, const AtomProtein& antigen, const std::vector<AtomProtein>& cands,
     const dnab_langevin_params& params, float cutoff = 10.0f, float cell_size = 10.0f) {
   dnab_ctx* c = gpu.raw();
@@ -498,7 +499,7 @@ inline std::vector<AllAtomResult> run_allatom_gpu_phase(
   std::vector<double> e(idx.size());
   uint32_t n = 0;
   check(dnab_langevin_run(c, pop, &params, idx.data(), e.data(), &n), c);
-  std::vector<AllAtomResult> out(n);
+  std::vector<GpuPhaseResult> out(n);
   for (uint32_t k = 0; k < n; ++k) out[k] = {idx[k], e[k]};
   return out;
 }
@@ -600,6 +601,92 @@ inline std::vector<double> allatom_diffusion_steps(
   }
   e.resize(abs.size());
   return e;
+}
+
+// AllAtomResult (src/diffusion.rs:93-97)
+struct AllAtomResult {
+  AtomProtein antibody;
+  double energy;         // allatom_energy
+  std::string sequence;  // one-letter codes
+};
+
+// run_allatom (src/diffusion.rs:983-1107) in one dnab_run_allatom call: the
+// broad phase of `n_broad` binders (0: none, the reference's path without a
+// GPU), the refinement of the `population` survivors over cpu_iterations
+// steps, results in ascending allatom_energy, max(top_n, 1) of them. The
+// antigen's Calpha centre of mass is the binders' ring centre.
+inline std::vector<AllAtomResult> run_allatom(const GpuContext& gpu, const AtomProtein& antigen,
+                                              size_t ab_length, size_t population,
+                                              size_t cpu_iterations, size_t top_n,
+                                              uint32_t n_broad = 1024, uint64_t seed = 0,
+                                              float cutoff = 10.0f) {
+  dnab_ctx* c = gpu.raw();
+  ResidueCloud ca;
+  std::vector<uint8_t> ca_aa;
+  for (size_t r = 0; r < antigen.n_residues(); ++r) {
+    const uint32_t i = antigen.ca_atom_idx[r];
+    ca.push(antigen.atoms.x[i], antigen.atoms.y[i], antigen.atoms.z[i], antigen.amino_acid[r]);
+    ca_aa.push_back(static_cast<uint8_t>(antigen.amino_acid[r]));
+  }
+  dnab_antigen_desc ad{};
+  ad.atoms = antigen.atoms.view();
+  ad.ca = ca.view();
+  ad.ca_amino_acid = ca_aa.data();
+  const auto ctr = antigen.ca_center_of_mass();
+  std::memcpy(ad.center, ctr.data(), sizeof(ad.center));
+  ad.cell_size = cutoff;
+  ad.cutoff = cutoff;
+  dnab_antigen* ag = nullptr;
+  check(dnab_antigen_create(c, &ad, &ag), c);
+  std::unique_ptr<dnab_antigen, void (*)(dnab_antigen*)> ag_guard(ag, dnab_antigen_destroy);
+  dnab_run_allatom_params p;
+  dnab_run_allatom_default_params(&p);
+  p.ab_length = static_cast<uint32_t>(ab_length);
+  p.population = static_cast<uint32_t>(population);
+  p.cpu_iterations = static_cast<uint32_t>(cpu_iterations);
+  p.top_n = static_cast<uint32_t>(top_n);
+  p.n_broad = n_broad;
+  p.seed = seed;
+  const size_t pool = n_broad ? std::min<size_t>(population, n_broad) : population;
+  const size_t n_keep = std::min<size_t>(std::max<size_t>(top_n, 1), pool);
+  const size_t nr = n_keep * ab_length, cap = 14 * nr + 1;
+  AtomCloud all;
+  all.x.resize(cap); all.y.resize(cap); all.z.resize(cap); all.charge.resize(cap);
+  all.r_min_half.resize(cap); all.epsilon.resize(cap); all.hydrophobic.resize(cap);
+  all.atom_type.resize(cap); all.residue_idx.resize(cap);
+  std::vector<uint32_t> rs(nr + 1), cai(std::max<size_t>(nr, 1));
+  std::vector<uint8_t> aa(std::max<size_t>(nr, 1));
+  dnab_protein_out o{cap, all.x.data(), all.y.data(), all.z.data(), all.charge.data(),
+                     all.r_min_half.data(), all.epsilon.data(), all.hydrophobic.data(),
+                     all.atom_type.data(), all.residue_idx.data(), rs.data(), cai.data(),
+                     aa.data()};
+  std::vector<double> e(std::max<size_t>(n_keep, 1));
+  uint32_t n = 0;
+  size_t na = 0;
+  check(dnab_run_allatom(c, ag, &p, &o, e.data(), &n, &na), c);
+  static const char kOne[] = "ARNDCQEGHILKMFPSTWYV";  // AminoAcid order
+  std::vector<AllAtomResult> out;
+  for (uint32_t k = 0; k < n; ++k) {
+    AllAtomResult r;
+    const size_t g0 = k * ab_length, g1 = g0 + ab_length;
+    const uint32_t a0 = rs[g0], a1 = rs[g1];
+    AtomCloud& b = r.antibody.atoms;
+    auto cut = [&](auto& dst, const auto& src) { dst.assign(src.begin() + a0, src.begin() + a1); };
+    cut(b.x, all.x); cut(b.y, all.y); cut(b.z, all.z); cut(b.charge, all.charge);
+    cut(b.r_min_half, all.r_min_half); cut(b.epsilon, all.epsilon);
+    cut(b.hydrophobic, all.hydrophobic); cut(b.atom_type, all.atom_type);
+    cut(b.residue_idx, all.residue_idx);
+    r.antibody.res_start.assign(ab_length + 1, 0);
+    for (size_t g = g0; g < g1; ++g) {
+      r.antibody.res_start[g - g0 + 1] = rs[g + 1] - a0;
+      r.antibody.ca_atom_idx.push_back(cai[g] - a0);
+      r.antibody.amino_acid.push_back(static_cast<AminoAcid>(aa[g]));
+      r.sequence.push_back(kOne[aa[g]]);
+    }
+    r.energy = e[k];
+    out.push_back(std::move(r));
+  }
+  return out;
 }
 
 }  // namespace binder

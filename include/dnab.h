@@ -566,6 +566,47 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* antigen, const dnab_population_desc
                 const uint8_t* res_rotamer, const dnab_langevin_params* lp,
                 const dnab_mc_params* mc, dnab_protein_out* out, uint8_t* out_rotamer,
                 double* out_energy, size_t* n_atoms_out);
+
+/* ---- run_allatom (src/diffusion.rs:983-1107) in one call -------------------
+ * The broad phase: n_broad binders built as random_allatom_antibody builds
+ * them (a ring of ab_length residues around the antigen's centre,
+ * src/diffusion.rs:264-277), broad_steps of allatom_diffusion_step without
+ * noise on the temperature schedule of the whole run (broad_moves = 1: with
+ * its Monte-Carlo moves, as src/diffusion.rs:1021-1024 steps them; 0: the
+ * gradient-only loop, one fused launch), then the `population` candidates
+ * with the lowest LJ + Coulomb + hydrophobic energy (score_batch's terms,
+ * src/diffusion.rs:1041-1049) survive; n_broad = 0 starts from `population`
+ * ring binders instead (the reference's path without a GPU,
+ * src/diffusion.rs:1064-1071). Then cpu_iterations steps with noise and the
+ * moves, the schedule restarted (src/diffusion.rs:1083-1095), the
+ * allatom_energy of each, the ascending sort (NaN last, ties by index) and
+ * the first max(top_n, 1) (src/diffusion.rs:1097-1098). Outputs: the result
+ * proteins concatenated in rank order into `out` (nullable: then only the
+ * sizes; res_start and ca_atom_idx are global atom indices), their
+ * allatom_energy in out_energy (nullable, max(top_n, 1) entries), the count
+ * in n_results and the atom count in n_atoms_out. The antigen needs its
+ * Calpha cloud and residue types; its centre is the ring's centre. Random
+ * streams are the library's (SplitMix64 for construction, Philox for the
+ * steps), seeded from `seed`: runs repeat exactly, the reference's SmallRng
+ * trajectories are not reproduced. */
+typedef struct dnab_run_allatom_params {
+  uint32_t ab_length;      /* binder residues (0: the antigen's residue count, the CLI default) */
+  uint32_t population;     /* survivors of the broad phase (TOP_K = 64) */
+  uint32_t cpu_iterations; /* refinement iterations (CPU_STEPS = 600) */
+  uint32_t top_n;          /* results kept (at least 1) */
+  uint32_t n_broad;        /* broad-phase binders (AA_POPULATION = 1024; 0 = no broad phase) */
+  uint32_t broad_steps;    /* broad-phase steps (GPU_STEPS = 200) */
+  uint32_t broad_moves;    /* 1: the broad phase with the Monte-Carlo moves; 0: without */
+  uint32_t _reserved;
+  uint64_t seed;
+  float ring_radius;       /* the binders' ring (INIT_RADIUS = 20 A) */
+  float _pad;
+} dnab_run_allatom_params;
+DNAB_STATIC_ASSERT(sizeof(dnab_run_allatom_params) == 48, "dnab_run_allatom_params layout");
+int dnab_run_allatom_default_params(dnab_run_allatom_params* p);
+int dnab_run_allatom(dnab_ctx* ctx, dnab_antigen* antigen, const dnab_run_allatom_params* p,
+                     dnab_protein_out* out, double* out_energy, uint32_t* n_results,
+                     size_t* n_atoms_out);
 /* One step at iteration `iter` (forces from a fresh evaluation). */
 int dnab_langevin_step(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_params* p,
                        uint32_t iter);

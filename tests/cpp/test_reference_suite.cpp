@@ -321,6 +321,28 @@ int main() {
     for (size_t k = 1; k < res.size(); ++k) CHECK(res[k - 1].energy <= res[k].energy);
   });
 
+  run("diffusion::run_allatom: sorted, max(top_n, 1) results (src/diffusion.rs:1097-1106)", [&] {
+    std::vector<float> x, y, z;
+    std::vector<AminoAcid> seq;
+    for (int i = 0; i < 12; ++i) {
+      x.push_back((i - 6.0f) * 3.8f);
+      y.push_back(0.0f);
+      z.push_back(0.0f);
+      seq.push_back(static_cast<AminoAcid>((i * 7) % 20));
+    }
+    auto ag = protein_from_ca_trace(x, y, z, seq);
+    auto res = run_allatom(gpu, ag, 8, 16, 40, 4, /*n_broad=*/64, /*seed=*/3);
+    CHECK(res.size() == 4);
+    for (size_t k = 0; k < res.size(); ++k) {
+      CHECK(res[k].antibody.n_residues() == 8 && res[k].sequence.size() == 8);
+      if (k) CHECK(res[k - 1].energy <= res[k].energy);
+      // the reported energy is the antibody's allatom_energy
+      const double e = allatom_energy(gpu, ag, res[k].antibody);
+      CHECK(std::abs(e - res[k].energy) <= 1e-6 * (1.0 + std::abs(e)));
+    }
+    CHECK(run_allatom(gpu, ag, 8, 16, 10, 0, 0, 3).size() == 1);  // top_n = 0 keeps one
+  });
+
   std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);
   return failures ? 1 : 0;
 }
