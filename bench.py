@@ -194,6 +194,12 @@ def measured_traffic(step_kernels):
     return None
 
 
+def traj_kernel_time(ctx):
+    """Event-timed milliseconds of the fused loop's launches so far: the
+    step launches and the final evaluation's."""
+    return ctx.kernel_time("trajectory")[0] + ctx.kernel_time("trajectory_final")[0]
+
+
 def saturated_pair_rate(ctx, n_cand, reps=20):
     """The pair kernels fed enough work: the c2 antigen with n_cand binders at
     their starting positions (surface contact), energies + forces evaluated
@@ -230,6 +236,7 @@ def saturated_trajectory_rate(ctx, n_cand, steps=10):
     binders at their starting positions (surface contact, where most pairs
     are), `steps` gradient steps; in-cutoff pairs counted by a replay with
     the counting variant, the launch timed with HIP events."""
+    import numpy as np
     import dnab
     from dnab import workloads as W
     w = W.config2(n_cand=n_cand, seed=77)
@@ -244,21 +251,42 @@ def saturated_trajectory_rate(ctx, n_cand, steps=10):
     pop.run(params(True))
     n_in, n_tested = pop.pair_stats()
     pop.set_coords(x0, y0, z0)
+    tuning = pop.autotune_trajectory(params(False), trial_steps=steps)
     ctx.synchronize()
     ctx.set_timing(True)
-    k0, n0 = ctx.kernel_time("trajectory")
+    k0 = traj_kernel_time(ctx)
     pop.run(params(False))
     ctx.synchronize()
-    k1, n1 = ctx.kernel_time("trajectory")
+    ms = traj_kernel_time(ctx) - k0
     ctx.set_timing(False)
-    ms = (k1 - k0) / max(1, n1 - n0)
     achieved = n_in * FLOP_PER_PAIR_EF / (ms * 1e-3) / 1e12
+    # the per-step kernels on the same trajectory (the paths agree bit for
+    # bit): their pair kernels' event time over the same evaluations
+    xf = pop.coords()
+    pop.set_coords(x0, y0, z0)
+    pop.autotune(reps=3)
+    pop.set_coords(x0, y0, z0)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    p0 = ctx.kernel_time("pair_ef")[0]
+    pop.run(dnab.langevin_params(n_steps=steps, score_every=0, top_k=64,
+                                 restraint_radius=w.restraint_radius[0], step_kernels=1,
+                                 use_graph=0))
+    ctx.synchronize()
+    pair_ms = ctx.kernel_time("pair_ef")[0] - p0
+    ctx.set_timing(False)
+    same = all(np.array_equal(u, v) for u, v in zip(xf, pop.coords()))
     pop.close()
     for a in ags:
         a.close()
-    return {"candidates": n_cand, "steps": steps, "pairs_in_cutoff": int(n_in),
-            "pairs_tested": int(n_tested), "kernel_ms": ms, "achieved": achieved,
-            "frac": achieved / PEAK_FP32_TFLOPS, "pairs_per_s": n_in / (ms * 1e-3)}
+    return {"candidates": n_cand, "steps": steps, "evaluations": steps + 1,
+            "pairs_in_cutoff": int(n_in), "pairs_tested": int(n_tested),
+            "kernel_ms": ms, "ms_per_evaluation": ms / (steps + 1), "achieved": achieved,
+            "frac": achieved / PEAK_FP32_TFLOPS, "pairs_per_s": n_in / (ms * 1e-3),
+            "trajectory_tuning": tuning,
+            "step_kernels_same_trajectory": {
+                "pair_kernel_ms": pair_ms, "pairs_per_s": n_in / (pair_ms * 1e-3),
+                "bit_identical": bool(same)}}
 
 
 def interface_positions(ab, ag):
@@ -569,6 +597,9 @@ def main():
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--tuning", default="",
                     help="fixed pair work split 'survivors,chunks,waves' instead of autotuning")
+    ap.add_argument("--traj-tuning", default="",
+                    help="fixed fused-kernel shape 'waves,cand_per_group,balance,cells,chunks"
+                         "[,rebalance]' instead of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied device-to-device first (reference point for traffic counters)")
     ap.add_argument("--prescreen-train", type=int, default=1_000_000,
@@ -616,11 +647,13 @@ def main():
         # replays carry no per-kernel events
         ctx.set_timing(not step_kernels or not args.graph)
         k0 = ctx.kernel_time(name)
+        f0 = ctx.kernel_time("trajectory_final")[0]
         t0 = time.perf_counter()
         idx, _ = pop.run(params(args.steps, step_kernels=step_kernels))
         ctx.synchronize()
         dt = time.perf_counter() - t0
         k1 = ctx.kernel_time(name)
+        f1 = ctx.kernel_time("trajectory_final")[0]
         ctx.set_timing(False)
         if step_kernels and args.graph:  # per-kernel time from an eager pass
             pop.set_coords(x0, y0, z0)
@@ -632,7 +665,9 @@ def main():
             k1 = ctx.kernel_time(name)
             ctx.set_timing(False)
         launches = max(1, k1[1] - k0[1])
-        return dt, (k1[0] - k0[0]) / launches, idx, k1[0] - k0[0], launches
+        # the fused loop's total includes its final evaluation's launch
+        total = k1[0] - k0[0] + (0.0 if step_kernels else f1 - f0)
+        return dt, (k1[0] - k0[0]) / launches, idx, total, launches
 
     # warm-up (untimed) of both paths; the autotuner picks the per-step pair
     # kernels' work split on the state the warm-up leaves (results do not
@@ -648,6 +683,18 @@ def main():
     else:
         tuning, tuning_ms = pop.autotune(reps=3)
         log(f"[bench] autotuned pair work split {tuning} ({tuning_ms * 1e3:.1f} us per evaluation)")
+    if args.traj_tuning:
+        v = [int(t) for t in args.traj_tuning.split(",")]
+        pop.set_trajectory_tuning(*v)
+        traj_tuning = pop.trajectory_tuning()
+        log(f"[bench] fixed trajectory shape {traj_tuning}")
+    else:
+        # the fused loop's shape, timed from the start state the timed run uses
+        pop.set_coords(x0, y0, z0)
+        t = time.perf_counter()
+        traj_tuning = pop.autotune_trajectory(params(min(args.steps, 50), step_kernels=0),
+                                              trial_steps=min(args.steps, 50))
+        log(f"[bench] autotuned trajectory shape {traj_tuning} in {time.perf_counter() - t:.1f}s")
     other = 1 - args.step_kernels
     dt, avg_ms, idx, total_ms, launches = timed_run(args.step_kernels)
     x1, y1, z1 = pop.coords()
@@ -708,7 +755,8 @@ def main():
             "evaluations": n_evals,
             "replay_identical": replay_ok,
             "pair_tuning": tuning,
-            "trajectory_tuning": pop.trajectory_tuning(),
+            "trajectory_tuning": traj_tuning,
+            "trajectory_tuning_source": "fixed" if args.traj_tuning else "autotuned",
         },
         "roofline": {
             "bound": "valu",

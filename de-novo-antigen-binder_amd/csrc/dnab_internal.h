@@ -248,12 +248,14 @@ struct dnab_population {
   dnab::DevBuf<uint32_t> cand_atom_off, cand_res_off, cand_ag;
   dnab::DevBuf<uint32_t> res_atom_off, res_ca, res_cand;
   uint32_t max_cand_atoms = 0, max_cand_res = 0;  // largest candidate
-  std::vector<uint32_t> h_cand_atom_off, h_cand_res_off;  // host copies
-  // fused trajectory kernel: workgroup shape, grouping by measured work
+  std::vector<uint32_t> h_cand_atom_off, h_cand_res_off, h_cand_ag;  // host copies
+  // fused trajectory kernel: workgroup shape, candidate grouping (0: consecutive,
+  // 1: by measured work, 2: by position, the group's atoms in compact tiles)
   uint32_t traj_waves = 16, traj_cpw = 4, traj_cells = 2, traj_chunks = 2;
-  bool traj_balance = true;
+  uint32_t traj_balance = 1;
   uint32_t traj_rebalance = 0;   // steps per launch between regroupings (0: one launch)
   dnab::DevBuf<uint32_t> traj_work, traj_group_off, traj_group_cand;
+  dnab::DevBuf<unsigned long long> traj_key;  // per candidate, for grouping by position
   std::vector<uint32_t> h_group_off, h_group_cand;  // sources of the group-table copies
   std::vector<float> h_sigma;                       // source of the noise-table copy
   dnab::DevBuf<uint8_t> res_aa;         // residue types, when given

@@ -1533,6 +1533,7 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   }
   pop->h_cand_atom_off = hcand_atom_off;
   pop->h_cand_res_off.assign(d->cand_res_off, d->cand_res_off + nc + 1);
+  if (nc) pop->h_cand_ag.assign(d->cand_antigen, d->cand_antigen + nc);
   for (size_t r = 0; r < nr; ++r) {
     float q = 0, e = 0, s = 0;
     uint8_t h = 0;

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -470,6 +471,96 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
   if (topk_energy) DNAB_RETURN_IF(download(ctx, topk_energy, pop->hist_e.p + last, k * sizeof(double)));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   pop->hist_rows = rows;
+  return DNAB_OK;
+}
+
+int dnab_population_autotune_trajectory(dnab_ctx* ctx, dnab_population* pop,
+                                        const dnab_langevin_params* p, uint32_t trial_steps,
+                                        dnab_traj_tuning* chosen) {
+  if (!ctx || !pop || !p || pop->ctx != ctx) return DNAB_ERR_INVALID;
+  DNAB_RETURN_IF(require_finite(ctx, pop, "autotune_trajectory"));
+  DNAB_RETURN_IF(validate_params(ctx, p));
+  if (trial_steps == 0 || trial_steps > 10000)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "autotune_trajectory: trial_steps must be 1..10000");
+  (void)hipSetDevice(ctx->device);
+  dnab_langevin_params lp = *p;
+  lp.n_steps = trial_steps;
+  lp.step_kernels = 0;
+  const uint32_t nc = static_cast<uint32_t>(pop->n_cand);
+  if (nc == 0 || !trajectory_supported(pop, &lp))
+    return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "autotune_trajectory: no fused loop for this population");
+  DNAB_RETURN_IF(upload_sigma(ctx, pop, &lp, lp.n_steps));
+  const uint32_t rows = (lp.score_every ? (lp.n_steps + lp.score_every - 1) / lp.score_every : 0) + 1;
+  DNAB_ALLOC_CHECK(ctx, pop->score_e.ensure(static_cast<size_t>(rows) * nc));
+  DNAB_RETURN_IF(discard_pending_counts(ctx, pop));
+  // every trial starts from the caller's coordinates, which are put back at the end
+  DevBuf<float4> saved;
+  DNAB_ALLOC_CHECK(ctx, saved.alloc(pop->n_atoms));
+  const size_t pos_bytes = pop->n_atoms * sizeof(float4);
+  DNAB_HIP_CHECK(ctx, hipMemcpyAsync(saved.p, pop->pos.p, pos_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  const StepParams sp = make_step_params(&lp);
+  dnab_traj_tuning best;
+  DNAB_RETURN_IF(dnab_population_get_trajectory_tuning(pop, &best));
+  // wall clock of one loop (host regrouping included), best of two runs; a
+  // shape whose groups do not fit the LDS runs as the loop runs it then
+  // (one candidate per group of 4 waves)
+  auto trial = [&](const dnab_traj_tuning& t, double* sec) -> int {
+    *sec = INFINITY;
+    DNAB_RETURN_IF(dnab_population_set_trajectory_tuning(ctx, pop, &t));
+    for (int rep = 0; rep < 2; ++rep) {
+      DNAB_HIP_CHECK(ctx, hipMemcpyAsync(pop->pos.p, saved.p, pos_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p, 0u);
+      DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      DNAB_RETURN_IF(trajectory_run(ctx, pop, &lp, sp));
+      DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      *sec = std::min(*sec, s);
+    }
+    return DNAB_OK;
+  };
+  double best_s = INFINITY;
+  DNAB_RETURN_IF(trial(best, &best_s));
+  // coordinate descent over the shape, one dimension at a time, until a
+  // pass changes nothing (at most three passes)
+  static const uint32_t waves[] = {4, 8, 16}, cpw[] = {1, 2, 4, 8}, chunks[] = {1, 2, 4},
+                        cells[] = {2, 8}, bal[] = {0, 1, 2};
+  struct Dim { const uint32_t* v; int n; uint32_t dnab_traj_tuning::*f; };
+  const Dim dims[] = {{waves, 3, &dnab_traj_tuning::waves},
+                      {cpw, 4, &dnab_traj_tuning::cand_per_group},
+                      {chunks, 3, &dnab_traj_tuning::chunks_per_item},
+                      {cells, 2, &dnab_traj_tuning::cells_per_item},
+                      {bal, 3, &dnab_traj_tuning::balance}};
+  int rc = DNAB_OK;
+  for (int pass = 0; pass < 3 && rc == DNAB_OK; ++pass) {
+    bool changed = false;
+    for (const Dim& d : dims) {
+      for (int i = 0; i < d.n && rc == DNAB_OK; ++i) {
+        if (best.*d.f == d.v[i]) continue;
+        dnab_traj_tuning t = best;
+        t.*d.f = d.v[i];
+        double s;
+        rc = trial(t, &s);
+        if (rc == DNAB_OK && s < best_s) {
+          best_s = s;
+          best = t;
+          changed = true;
+        }
+      }
+    }
+    if (!changed) break;
+  }
+  // the winner stays in the population; the coordinates are the caller's
+  // again and no evaluation is current
+  const int rc2 = dnab_population_set_trajectory_tuning(ctx, pop, &best);
+  DNAB_HIP_CHECK(ctx, hipMemcpyAsync(pop->pos.p, saved.p, pos_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  pop->sorted = false;
+  pop->evaluated = false;
+  pop->masked = false;
+  if (rc != DNAB_OK) return rc;
+  if (rc2 != DNAB_OK) return rc2;
+  if (chosen) *chosen = best;
   return DNAB_OK;
 }
 

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <functional>
 #include <vector>
 
@@ -62,7 +63,8 @@ struct TrajCand {
   uint32_t id;         // candidate index
   uint32_t a0, n;      // global atom range
   uint32_t r0, nres;   // global residue range
-  uint32_t a_off;      // first LDS atom slot (a multiple of 64)
+  uint32_t a_off;      // first index of its atoms in group order (a multiple of 64
+                       // unless packed; the slot of index i is s_perm[i])
   uint32_t r_off;      // first LDS residue slot
   uint32_t ag;         // antigen view
 };
@@ -109,16 +111,18 @@ struct TrajArgs {
   uint32_t chunks;          // listed chunks per work item (1..kLoadAhead)
   uint32_t cpw;             // candidates per workgroup (the most, with a table)
   uint32_t a_cap, r_cap;    // LDS capacity: atom slots (multiple of 64), residues
+  int32_t packed;           // with a table: the group's atoms are packed (no padding per
+                            // candidate) and bisected into compact tiles at the start
   StepParams sp;
 };
 
 // Bytes of dynamic LDS for a capacity of a_cap atom slots and r_cap residues:
 // 5 integer sums, energy, position, parameters, force, status per atom slot; the
 // displacement and the (atom range, Calpha, index, candidate) record per
-// residue; the residue slot per atom slot; the chunk list of each tile
-// (a_cap / 64 tiles).
+// residue; the residue slot and the slot of each group-order index per atom
+// slot; the chunk list of each tile (a_cap / 64 tiles).
 __host__ __device__ inline size_t traj_lds_bytes(uint32_t a_cap, uint32_t r_cap) {
-  const size_t res = (static_cast<size_t>(a_cap) * 2 + 3) & ~static_cast<size_t>(3);
+  const size_t res = (static_cast<size_t>(a_cap) * 4 + 3) & ~static_cast<size_t>(3);
   return static_cast<size_t>(a_cap) * (5 * 8 + 8 + 3 * 16 + 4) + static_cast<size_t>(r_cap) * 32 + res +
          static_cast<size_t>(a_cap / kWave) * kTileChunks * 4;
 }
@@ -127,6 +131,219 @@ __host__ __device__ inline size_t traj_lds_bytes(uint32_t a_cap, uint32_t r_cap)
 __host__ __device__ inline size_t traj_static_lds(int waves) {
   return static_cast<size_t>(waves) * kWave * 32 + kTrajMaxTiles * sizeof(TrajTile) +
          kTrajMaxCpw * (sizeof(TrajCand) + sizeof(TrajAg) + 4) + 256;
+}
+
+// Float keys that sort as unsigned integers, and back.
+__device__ inline uint32_t float_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float key_float(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Compact tiles for n points (recursive bisection): every segment, a run of
+// whole tiles of 64 slots, is cut after half of its tiles (rounded down)
+// across the longest axis of its points' box, until each segment is one
+// tile. A cut needs no sort: a 64-bin histogram along the axis gives the bin
+// holding the cut; the points below it go left, those above right, and the
+// bin's own points (within 1/64 of the box of the cut) fill the left side in
+// slot order. Deterministic (counts and scans, no ordering by atomics).
+// Returns ord or ord2, whichever holds the result: [p] (p < n) is the point
+// in slot p. Scratch: the other of ord and ord2 (n each), pbin (n), hist (64
+// per tile), and per tile seg (the first tile of its segment), bb (6: the
+// segment's box) and cnt (9). Called by every thread.
+template <int NT>
+__device__ __attribute__((noinline)) uint16_t* traj_bisect(
+    const float4* pts, uint32_t n, uint16_t* ord, uint16_t* ord2, uint8_t* pbin, uint32_t* hist,
+    uint32_t* seg, uint32_t* bb, uint32_t* cnt, uint32_t* more, uint32_t tid) {
+  constexpr uint32_t NB = 64, WAVES = NT / kWave;
+  const uint32_t lane = tid & 63u, wave = tid / kWave;
+  const uint32_t T = (n + kWave - 1) / kWave;
+  // per segment start s: cut bin, points of it going left, left size, points
+  // in the cut bin, points below it; per tile: points left of the cut bin,
+  // in it, and the counts of both in earlier tiles of the segment
+  uint32_t *c_bin = cnt, *c_r = cnt + T, *c_k = cnt + 2 * T, *c_nb = cnt + 3 * T,
+           *c_below = cnt + 4 * T, *t_l = cnt + 5 * T, *t_b = cnt + 6 * T, *o_l = cnt + 7 * T,
+           *o_b = cnt + 8 * T;
+  for (uint32_t p = tid; p < n; p += NT) ord[p] = static_cast<uint16_t>(p);
+  for (uint32_t t = tid; t < T; t += NT) seg[t] = 0;
+  if (tid == 0) *more = T > 1 ? 1u : 0u;
+  __syncthreads();
+  // the histogram bin of the point in slot p along segment s's longest axis
+  auto bin_of = [&](uint32_t p, uint32_t sg) -> uint32_t {
+    const uint32_t* b = bb + 6 * sg;
+    const float lx = key_float(b[0]), ly = key_float(b[1]), lz = key_float(b[2]);
+    const float ex = key_float(b[3]) - lx, ey = key_float(b[4]) - ly, ez = key_float(b[5]) - lz;
+    const float4 q = pts[ord[p]];
+    float c, lo, e;
+    if (ex >= ey && ex >= ez) {
+      c = q.x; lo = lx; e = ex;
+    } else if (ey >= ez) {
+      c = q.y; lo = ly; e = ey;
+    } else {
+      c = q.z; lo = lz; e = ez;
+    }
+    const float f = e > 0.0f ? (c - lo) * (static_cast<float>(NB) / e) : 0.0f;
+    return static_cast<uint32_t>(fminf(fmaxf(f, 0.0f), static_cast<float>(NB - 1)));
+  };
+  while (*more) {
+    for (uint32_t t = tid; t < T; t += NT) {
+      bb[6 * t + 0] = bb[6 * t + 1] = bb[6 * t + 2] = 0xFFFFFFFFu;
+      bb[6 * t + 3] = bb[6 * t + 4] = bb[6 * t + 5] = 0u;
+    }
+    for (uint32_t i = tid; i < T * NB; i += NT) hist[i] = 0;
+    __syncthreads();
+    // boxes: a tile's (one segment's points) by wave reductions, then per tile
+    for (uint32_t t = wave; t < T; t += WAVES) {
+      const uint32_t p = t * kWave + lane;
+      const bool live = p < n;
+      const float4 q = pts[ord[live ? p : t * kWave]];
+      const float inf = __builtin_inff();
+      const float x0 = wave_reduce_f32<OpMin>(live ? q.x : inf);
+      const float y0 = wave_reduce_f32<OpMin>(live ? q.y : inf);
+      const float z0 = wave_reduce_f32<OpMin>(live ? q.z : inf);
+      const float x1 = wave_reduce_f32<OpMax>(live ? q.x : -inf);
+      const float y1 = wave_reduce_f32<OpMax>(live ? q.y : -inf);
+      const float z1 = wave_reduce_f32<OpMax>(live ? q.z : -inf);
+      if (lane == 0) {
+        uint32_t* b = bb + 6 * seg[t];
+        atomicMin(&b[0], float_key(x0));
+        atomicMin(&b[1], float_key(y0));
+        atomicMin(&b[2], float_key(z0));
+        atomicMax(&b[3], float_key(x1));
+        atomicMax(&b[4], float_key(y1));
+        atomicMax(&b[5], float_key(z1));
+      }
+    }
+    __syncthreads();
+    // histograms: a tile's points (clustered in few bins) counted per
+    // distinct bin, one atomic each
+    for (uint32_t t = wave; t < T; t += WAVES) {
+      const uint32_t p = t * kWave + lane, sg = seg[t];
+      const bool live = p < n;
+      const uint32_t bn = live ? bin_of(p, sg) : 0u;
+      if (live) pbin[p] = static_cast<uint8_t>(bn);
+      uint64_t rem = ballot64(live);
+      while (rem) {
+        const int l = __builtin_ctzll(rem);
+        const uint32_t lb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bn), l));
+        const uint64_t grp = ballot64(live && bn == lb);
+        if (lane == static_cast<uint32_t>(l))
+          atomicAdd(&hist[sg * NB + lb], static_cast<uint32_t>(__builtin_popcountll(grp)));
+        rem &= ~grp;
+      }
+    }
+    __syncthreads();
+    // the cut of each segment of two or more tiles (one wave per segment)
+    for (uint32_t sg = wave; sg < T; sg += WAVES) {
+      if (seg[sg] != sg) continue;  // not a segment's first tile (wave-uniform)
+      const uint64_t past = ballot64(lane < T && lane > sg && seg[lane] != sg);
+      const uint32_t e = past ? static_cast<uint32_t>(__builtin_ctzll(past)) : T;
+      if (e - sg < 2) {
+        if (lane == 0) c_bin[sg] = NB;  // one tile: stays
+        continue;
+      }
+      const uint32_t k = (e - sg) / 2 * kWave;
+      const uint32_t h = hist[sg * NB + lane];
+      uint32_t incl = h;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 64);
+        if (lane >= static_cast<uint32_t>(o)) incl += u;
+      }
+      const int b = __builtin_ctzll(ballot64(incl >= k));  // exists: k < the segment's points
+      const uint32_t hb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(h), b));
+      const uint32_t below = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), b)) - hb;
+      if (lane == 0) {
+        c_bin[sg] = static_cast<uint32_t>(b);
+        c_r[sg] = k - below;
+        c_k[sg] = k;
+        c_nb[sg] = hb;
+        c_below[sg] = below;
+      }
+    }
+    __syncthreads();
+    // per tile: its points below the cut bin and in it
+    for (uint32_t t = wave; t < T; t += WAVES) {
+      const uint32_t p = t * kWave + lane, sg = seg[t], cb = c_bin[sg];
+      const bool live = p < n && cb < NB;
+      const uint32_t bn = live ? pbin[p] : 0u;
+      const uint64_t ml = ballot64(live && bn < cb), mb = ballot64(live && bn == cb);
+      if (lane == 0) {
+        t_l[t] = static_cast<uint32_t>(__builtin_popcountll(ml));
+        t_b[t] = static_cast<uint32_t>(__builtin_popcountll(mb));
+      }
+    }
+    __syncthreads();
+    if (wave == 0) {
+      // exclusive scans of the tile counts within each segment
+      const uint32_t vl = lane < T ? t_l[lane] : 0u, vb = lane < T ? t_b[lane] : 0u;
+      uint32_t il = vl, ib = vb;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t ul = __shfl_up(il, o, 64), ub = __shfl_up(ib, o, 64);
+        if (lane >= static_cast<uint32_t>(o)) {
+          il += ul;
+          ib += ub;
+        }
+      }
+      const uint32_t el = il - vl, eb = ib - vb;
+      const int first = lane < T ? static_cast<int>(seg[lane]) : 0;
+      const uint32_t sl = static_cast<uint32_t>(__shfl(static_cast<int>(el), first, 64));
+      const uint32_t sb = static_cast<uint32_t>(__shfl(static_cast<int>(eb), first, 64));
+      if (lane < T) {
+        o_l[lane] = el - sl;
+        o_b[lane] = eb - sb;
+      }
+    }
+    __syncthreads();
+    // new slots: left of the cut bin, the cut bin's first r, then its rest,
+    // then the points right of it
+    for (uint32_t t = wave; t < T; t += WAVES) {
+      const uint32_t p = t * kWave + lane, sg = seg[t], cb = c_bin[sg];
+      const bool live = p < n;
+      uint32_t dest = p;
+      const bool split = cb < NB;
+      const uint32_t bn = live && split ? pbin[p] : 0u;
+      const bool isl = live && split && bn < cb, isb = live && split && bn == cb;
+      const uint64_t ml = ballot64(isl), mb = ballot64(isb);
+      const uint32_t lr = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(ml >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ml), 0u));
+      const uint32_t br = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mb >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mb), 0u));
+      if (live && split) {
+        const uint32_t base = sg * kWave, rl = o_l[t] + lr, rb = o_b[t] + br;
+        const uint32_t k = c_k[sg], r = c_r[sg];
+        if (isl) {
+          dest = base + rl;
+        } else if (isb) {
+          dest = rb < r ? base + c_below[sg] + rb : base + k + (rb - r);
+        } else {
+          dest = base + k + (c_nb[sg] - r) + ((p - base) - rl - rb);
+        }
+      }
+      if (live) ord2[dest] = ord[p];
+    }
+    __syncthreads();
+    uint16_t* const t_ord = ord;  // (every thread swaps alike)
+    ord = ord2;
+    ord2 = t_ord;
+    if (tid == 0) {
+      uint32_t m = 0;
+      for (uint32_t s0 = 0; s0 < T;) {
+        uint32_t e = s0 + 1;
+        while (e < T && seg[e] == s0) ++e;
+        const uint32_t left = (e - s0) / 2;
+        if (left) {
+          for (uint32_t q = s0 + left; q < e; ++q) seg[q] = s0 + left;
+          if (left > 1 || e - s0 - left > 1) m = 1;
+        }
+        s0 = e;
+      }
+      *more = m;
+    }
+    __syncthreads();
+  }
+  return ord;
 }
 
 __device__ inline bool traj_can_pair(const TrajAg& V, float4 p) {
@@ -150,7 +367,7 @@ k_trajectory(TrajArgs g) {
   __shared__ TrajTile s_tile[kTrajMaxTiles];
   __shared__ TrajCand s_cand[kTrajMaxCpw];
   __shared__ TrajAg s_ag[kTrajMaxCpw];
-  __shared__ uint32_t s_next, s_ncand, s_nslot, s_nres;
+  __shared__ uint32_t s_next, s_ncand, s_nslot, s_nres, s_norder, s_more;
   __shared__ uint32_t s_work[kTrajMaxCpw];
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -164,8 +381,10 @@ k_trajectory(TrajArgs g) {
   float4* s_delta = reinterpret_cast<float4*>(s_st + AC);      // [r_cap] (a_cap: a multiple of 64)
   uint4* s_rinfo = reinterpret_cast<uint4*>(s_delta + g.r_cap);       // [r_cap]
   uint16_t* s_res = reinterpret_cast<uint16_t*>(s_rinfo + g.r_cap);  // [a_cap]
-  uint32_t* s_desc = reinterpret_cast<uint32_t*>(s_dyn + ((reinterpret_cast<uintptr_t>(s_res + AC) -
+  uint16_t* s_perm = s_res + AC;                                      // [a_cap] index -> slot
+  uint32_t* s_desc = reinterpret_cast<uint32_t*>(s_dyn + ((reinterpret_cast<uintptr_t>(s_perm + AC) -
                                                              reinterpret_cast<uintptr_t>(s_dyn) + 3) & ~uintptr_t(3)));
+  const bool packed = g.packed && g.group_off;
 
   if (tid == 0) {
     uint32_t first, nc;
@@ -208,35 +427,83 @@ k_trajectory(TrajArgs g) {
       G.inv = V.inv;
       s_work[j] = 0;
       const uint32_t slots = (C.n + kWave - 1) & ~(kWave - 1);
-      for (uint32_t t = a_off / kWave; t < (a_off + slots) / kWave; ++t) {
-        s_tile[t].cand = j;
-        s_tile[t].nx = INT32_MIN;  // no chunk list yet (no cell range has this width)
-        s_tile[t].nchunk = 0;
-      }
-      a_off += slots;
+      for (uint32_t t = a_off / kWave; !packed && t < (a_off + slots) / kWave; ++t) s_tile[t].cand = j;
+      a_off += packed ? C.n : slots;
       r_off += C.nres;
     }
+    const uint32_t slots = (a_off + kWave - 1) & ~(kWave - 1);
+    for (uint32_t t = 0; t < slots / kWave; ++t) {
+      s_tile[t].nx = INT32_MIN;  // no chunk list yet (no cell range has this width)
+      s_tile[t].nchunk = 0;
+    }
     s_ncand = nc;
-    s_nslot = a_off;
+    s_norder = a_off;
+    s_nslot = slots;
     s_nres = r_off;
   }
   __syncthreads();
   const uint32_t NC = s_ncand, A = s_nslot, R = s_nres, T = A / kWave;
+  // the candidate holding group-order index o
+  auto cand_of = [&](uint32_t o) {
+    uint32_t j = 0;
+    while (j + 1 < NC && s_cand[j + 1].a_off <= o) ++j;
+    return j;
+  };
 
-  for (uint32_t k = tid; k < A; k += WAVES * kWave) {
-    const TrajCand C = s_cand[s_tile[k / kWave].cand];
-    const uint32_t i = k - C.a_off;
-    if (i < C.n) {
-      s_pos[k] = g.pos[C.a0 + i];
-      s_par[k] = g.par[C.a0 + i];
-      s_res[k] = static_cast<uint16_t>(C.r_off + (g.atom_res[C.a0 + i] - C.r0));
-    } else {
-      s_pos[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-      s_par[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-      s_res[k] = kTrajPad;
+  if (!packed) {
+    // each candidate from a whole tile, atoms in order: slot = index
+    for (uint32_t k = tid; k < A; k += WAVES * kWave) {
+      const TrajCand C = s_cand[s_tile[k / kWave].cand];
+      const uint32_t i = k - C.a_off;
+      s_perm[k] = static_cast<uint16_t>(k);
+      if (i < C.n) {
+        s_pos[k] = g.pos[C.a0 + i];
+        s_par[k] = g.par[C.a0 + i];
+        s_res[k] = static_cast<uint16_t>(C.r_off + (g.atom_res[C.a0 + i] - C.r0));
+      } else {
+        s_pos[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        s_par[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        s_res[k] = kTrajPad;
+      }
+    }
+  } else {
+    // packed: positions in group order (in the force array, unused until
+    // the first step), compact tiles by bisection, then the slots
+    const uint32_t N = s_norder;
+    for (uint32_t o = tid; o < N; o += WAVES * kWave) {
+      const TrajCand& C = s_cand[cand_of(o)];
+      s_force[o] = g.pos[C.a0 + (o - C.a_off)];
+    }
+    __syncthreads();
+    // scratch in the sums (free until the first step: 4 bytes per slot for
+    // ord and ord2, 4 for the histograms (256 per tile), 1 for the bins,
+    // within their 40 per slot) and in the chunk lists (16 words per tile,
+    // within their 192)
+    const uint32_t TN = (N + kWave - 1) / kWave;
+    uint16_t* const ord0 = reinterpret_cast<uint16_t*>(s_acc);
+    uint32_t* const hist = reinterpret_cast<uint32_t*>(s_acc) + AC;
+    uint8_t* const pbin = reinterpret_cast<uint8_t*>(s_acc) + 8 * static_cast<size_t>(AC);
+    const uint16_t* ord = traj_bisect<WAVES * kWave>(s_force, N, ord0, ord0 + AC, pbin, hist, s_desc,
+                                                     s_desc + TN, s_desc + 7 * TN, &s_more, tid);
+    for (uint32_t k = tid; k < A; k += WAVES * kWave) {
+      if (k < N) {
+        const uint32_t o = ord[k], j = cand_of(o);
+        const TrajCand& C = s_cand[j];
+        const uint32_t i = o - C.a_off;
+        s_pos[k] = s_force[o];
+        s_par[k] = g.par[C.a0 + i];
+        s_res[k] = static_cast<uint16_t>(C.r_off + (g.atom_res[C.a0 + i] - C.r0));
+        s_perm[o] = static_cast<uint16_t>(k);
+        if (k % kWave == 0) s_tile[k / kWave].cand = j;  // the tile's antigen (one per group)
+      } else {
+        s_pos[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        s_par[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        s_res[k] = kTrajPad;
+      }
     }
   }
-  // per residue: LDS atom range and Calpha slot, global index, candidate slot
+  __syncthreads();
+  // per residue: its atoms' group-order range, Calpha slot, global index, candidate slot
   for (uint32_t r = tid; r < R; r += WAVES * kWave) {
     uint32_t j = 0;
     while (j + 1 < NC && s_cand[j + 1].r_off <= r) ++j;
@@ -244,7 +511,7 @@ k_trajectory(TrajArgs g) {
     const uint32_t rg = C.r0 + (r - C.r_off);
     const uint32_t b0 = C.a_off + (g.res_atom_off[rg] - C.a0);
     const uint32_t b1 = C.a_off + (g.res_atom_off[rg + 1] - C.a0);
-    s_rinfo[r] = make_uint4(b0 | (b1 << 16), C.a_off + (g.res_ca[rg] - C.a0), rg, j);
+    s_rinfo[r] = make_uint4(b0 | (b1 << 16), s_perm[C.a_off + (g.res_ca[rg] - C.a0)], rg, j);
   }
   __syncthreads();
 
@@ -255,8 +522,9 @@ k_trajectory(TrajArgs g) {
     se = 0.0;
     sa = 0.0;
     for (uint32_t i = lane; i < C.n; i += kWave) {
-      se += s_e[C.a_off + i];
-      sa += static_cast<double>(s_force[C.a_off + i].w);
+      const uint32_t k = s_perm[C.a_off + i];
+      se += s_e[k];
+      sa += static_cast<double>(s_force[k].w);
     }
     se = wave_sum_f64(se);
     sa = wave_sum_f64(sa);
@@ -550,8 +818,8 @@ k_trajectory(TrajArgs g) {
       const TrajCand& C = s_cand[ri.w];
       const uint32_t b0 = ri.x & 0xFFFFu, b1 = ri.x >> 16;
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-      for (uint32_t k = b0; k < b1; ++k) {  // in atom order
-        const float4 f = s_force[k];
+      for (uint32_t o = b0; o < b1; ++o) {  // in atom order
+        const float4 f = s_force[s_perm[o]];
         fx += dyn_force(f.x);
         fy += dyn_force(f.y);
         fz += dyn_force(f.z);
@@ -590,10 +858,10 @@ k_trajectory(TrajArgs g) {
       if (g.work) g.work[c] = s_work[j];
     }
   }
-  for (uint32_t k = tid; k < A; k += WAVES * kWave) {
-    const TrajCand C = s_cand[s_tile[k / kWave].cand];
-    const uint32_t i = k - C.a_off;
-    if (i < C.n) {
+  for (uint32_t j = wave; j < NC; j += WAVES) {
+    const TrajCand& C = s_cand[j];
+    for (uint32_t i = lane; i < C.n; i += kWave) {
+      const uint32_t k = s_perm[C.a_off + i];
       g.pos[C.a0 + i] = s_pos[k];
       if (g.final_eval) {
         g.force[C.a0 + i] = s_force[k];
@@ -767,6 +1035,107 @@ static bool traj_balance(const dnab_population* pop, uint32_t waves, uint32_t cp
   return true;
 }
 
+// Per candidate, for grouping candidates by position: its antigen in the
+// high word, and the Morton code of the mean of its atom positions on a grid
+// of 2 A cells from 64 A below the antigen's atom box (clamped to 1024
+// cells a side) in the low word. One wave per candidate.
+__global__ void __launch_bounds__(256)
+k_cand_key(const AgView* __restrict__ views, const uint32_t* __restrict__ cand_ag,
+           const float4* __restrict__ pos, const uint32_t* __restrict__ cand_atom_off,
+           uint32_t n_cand, unsigned long long* __restrict__ out) {
+  const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if (c >= n_cand) return;  // whole wave
+  const uint32_t a0 = cand_atom_off[c], a1 = cand_atom_off[c + 1];
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+  for (uint32_t k = a0 + lane; k < a1; k += kWave) {
+    const float4 p = pos[k];
+    x += p.x;
+    y += p.y;
+    z += p.z;
+  }
+  x = wave_reduce_f32<OpAdd>(x);
+  y = wave_reduce_f32<OpAdd>(y);
+  z = wave_reduce_f32<OpAdd>(z);
+  if (lane == 0) {
+    const float inv = 1.0f / static_cast<float>(max(a1 - a0, 1u));
+    const AgView& V = views[cand_ag[c]];
+    const float m[3] = {x * inv, y * inv, z * inv};
+    uint32_t q[3];
+    for (int d = 0; d < 3; ++d)  // (NaN to cell 0)
+      q[d] = static_cast<uint32_t>(fminf(fmaxf((m[d] - (V.excl_lo[d] - 64.0f)) * 0.5f, 0.0f), 1023.0f));
+    out[c] = (static_cast<unsigned long long>(cand_ag[c]) << 32) | morton3(q[0], q[1], q[2]);
+  }
+}
+
+// Candidate indices in ascending key order, ties by index (LSD radix sort,
+// 8-bit digits; digits equal across all keys are skipped).
+static void sort_by_key(const std::vector<uint64_t>& key, std::vector<uint32_t>& order) {
+  const size_t n = key.size();
+  std::vector<uint32_t> tmp(n);
+  order.resize(n);
+  for (size_t i = 0; i < n; ++i) order[i] = static_cast<uint32_t>(i);
+  for (int sh = 0; sh < 64; sh += 8) {
+    size_t cnt[257] = {0};
+    for (size_t i = 0; i < n; ++i) ++cnt[((key[i] >> sh) & 0xFFu) + 1];
+    if (n == 0 || cnt[((key[0] >> sh) & 0xFFu) + 1] == n) continue;
+    for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+    for (size_t i = 0; i < n; ++i) {
+      const uint32_t c = order[i];
+      tmp[cnt[(key[c] >> sh) & 0xFFu]++] = c;
+    }
+    order.swap(tmp);
+  }
+}
+
+// Groups by position: candidates of one antigen, in the order of their keys
+// (k_cand_key: antigen, then the Morton code of the centroid), up to cpw per
+// group while the group's atoms, packed without padding per candidate, fit
+// the tiles, residue slots and LDS. Groups are launched in that order.
+static bool traj_spatial(const dnab_population* pop, uint32_t waves, uint32_t cpw,
+                         const std::vector<uint64_t>& keys, std::vector<uint32_t>& off,
+                         std::vector<uint32_t>& cand, TrajPlan& pl) {
+  const size_t nc = pop->n_cand;
+  const std::vector<uint32_t>& ao = pop->h_cand_atom_off;
+  const std::vector<uint32_t>& ro = pop->h_cand_res_off;
+  const std::vector<uint32_t>& ag = pop->h_cand_ag;
+  if (ag.size() != nc || keys.size() != nc) return false;
+  const size_t stat = traj_static_lds(static_cast<int>(waves));
+  std::vector<uint32_t> order;
+  sort_by_key(keys, order);
+  off.assign(1, 0);
+  cand.clear();
+  cand.reserve(nc);
+  uint32_t a_cap = kWave, r_cap = 1, most = 1, a = 0, r = 0, n = 0;
+  for (size_t i = 0; i < nc; ++i) {
+    const uint32_t c = order[i];
+    const uint32_t na = ao[c + 1] - ao[c], nr = ro[c + 1] - ro[c];
+    const uint32_t a2 = a + na, r2 = r + nr, slots = (a2 + kWave - 1) & ~(kWave - 1);
+    const bool same = n == 0 || ag[cand.back()] == ag[c];
+    if (n && (!same || n >= cpw || slots / kWave > kTrajMaxTiles || r2 > 0xFFFFu ||
+              traj_lds_bytes(slots, r2) + stat > kTrajLdsLimit)) {
+      off.push_back(static_cast<uint32_t>(cand.size()));
+      a = r = n = 0;
+    }
+    a += na;
+    r += nr;
+    ++n;
+    cand.push_back(c);
+    a_cap = std::max(a_cap, (a + kWave - 1) & ~(kWave - 1));
+    r_cap = std::max(r_cap, r);
+    most = std::max(most, n);
+  }
+  off.push_back(static_cast<uint32_t>(cand.size()));
+  if (a_cap / kWave > kTrajMaxTiles || r_cap > 0xFFFFu ||
+      traj_lds_bytes(a_cap, r_cap) + stat > kTrajLdsLimit)
+    return false;
+  pl.waves = waves;
+  pl.cpw = most;
+  pl.a_cap = a_cap;
+  pl.r_cap = r_cap;
+  pl.lds = traj_lds_bytes(a_cap, r_cap);
+  return true;
+}
+
 template <bool CLASH>
 static hipError_t launch_plan_c(const TrajArgs& a, const TrajPlan& pl, bool count, hipStream_t st) {
   switch (pl.waves) {
@@ -819,14 +1188,51 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
   a.cpw = pl.cpw;
   a.a_cap = pl.a_cap;
   a.r_cap = pl.r_cap;
+  a.packed = 0;
   a.sp = sp;
   a.s0 = 0;
   a.n_total = p->n_steps;
   a.final_eval = 1;
   const bool count = p->count_pairs != 0;
   const size_t nc = pop->n_cand;
-  const bool balance = pop->traj_balance && nc > 1 && p->n_steps > 0;
+  // grouping: 0 consecutive candidates, 1 by measured work, 2 by position
+  const uint32_t mode = nc > 1 && p->n_steps > 0 ? pop->traj_balance : 0u;
+  const bool balance = mode == 1;
   std::vector<uint32_t> work;
+  auto set_groups = [&](const std::vector<uint32_t>& off, const std::vector<uint32_t>& cand,
+                        const TrajPlan& np) -> int {
+    DNAB_ALLOC_CHECK(ctx, pop->traj_group_off.ensure(off.size()));
+    DNAB_ALLOC_CHECK(ctx, pop->traj_group_cand.ensure(cand.size()));
+    DNAB_RETURN_IF(upload(ctx, pop->traj_group_off.p, off.data(), off.size() * sizeof(uint32_t)));
+    DNAB_RETURN_IF(upload(ctx, pop->traj_group_cand.p, cand.data(), cand.size() * sizeof(uint32_t)));
+    a.group_off = pop->traj_group_off.p;
+    a.group_cand = pop->traj_group_cand.p;
+    a.n_groups = static_cast<uint32_t>(off.size() - 1);
+    a.cpw = np.cpw;
+    a.a_cap = np.a_cap;
+    a.r_cap = np.r_cap;
+    pl = np;
+    return DNAB_OK;
+  };
+  // group candidates by their centroids now; on failure the launch keeps
+  // the groups it has
+  std::vector<uint64_t> keys;
+  auto regroup_space = [&]() -> int {
+    std::vector<uint32_t>& off = pop->h_group_off;
+    std::vector<uint32_t>& cand = pop->h_group_cand;
+    DNAB_ALLOC_CHECK(ctx, pop->traj_key.ensure(nc));
+    hipLaunchKernelGGL(k_cand_key, dim3(static_cast<uint32_t>((nc + 3) / 4)), dim3(256), 0,
+                       ctx->stream, pop->views.p, pop->cand_ag.p, pop->pos.p, pop->cand_atom_off.p,
+                       static_cast<uint32_t>(nc), pop->traj_key.p);
+    DNAB_HIP_CHECK(ctx, hipGetLastError());
+    keys.resize(nc);
+    DNAB_RETURN_IF(download(ctx, keys.data(), pop->traj_key.p, nc * sizeof(uint64_t)));
+    TrajPlan sp2;
+    if (!traj_spatial(pop, pop->traj_waves, pop->traj_cpw, keys, off, cand, sp2)) return DNAB_OK;
+    DNAB_RETURN_IF(set_groups(off, cand, sp2));
+    a.packed = 1;
+    return DNAB_OK;
+  };
   // group candidates by the work in `work` (per candidate, staged antigen
   // atoms); on failure the launch keeps consecutive groups
   auto regroup = [&]() -> int {
@@ -836,18 +1242,7 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
     const uint32_t bins = std::max<uint32_t>(static_cast<uint32_t>(ctx->cu_count),
                                              static_cast<uint32_t>((nc + pop->traj_cpw - 1) / pop->traj_cpw));
     if (!traj_balance(pop, pop->traj_waves, pop->traj_cpw, bins, work, off, cand, bal)) return DNAB_OK;
-    DNAB_ALLOC_CHECK(ctx, pop->traj_group_off.ensure(off.size()));
-    DNAB_ALLOC_CHECK(ctx, pop->traj_group_cand.ensure(cand.size()));
-    DNAB_RETURN_IF(upload(ctx, pop->traj_group_off.p, off.data(), off.size() * sizeof(uint32_t)));
-    DNAB_RETURN_IF(upload(ctx, pop->traj_group_cand.p, cand.data(), cand.size() * sizeof(uint32_t)));
-    a.group_off = pop->traj_group_off.p;
-    a.group_cand = pop->traj_group_cand.p;
-    a.n_groups = static_cast<uint32_t>(off.size() - 1);
-    a.cpw = bal.cpw;
-    a.a_cap = bal.a_cap;
-    a.r_cap = bal.r_cap;
-    pl = bal;
-    return DNAB_OK;
+    return set_groups(off, cand, bal);
   };
   if (balance) {
     // measure each candidate's work on the start state (one evaluation, whose
@@ -862,10 +1257,13 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
     DNAB_HIP_CHECK(ctx, launch_plan(e, pl, false, false, ctx->stream));
     DNAB_RETURN_IF(download(ctx, work.data(), pop->traj_work.p, nc * sizeof(uint32_t)));
     DNAB_RETURN_IF(regroup());
+  } else if (mode == 2) {
+    DNAB_RETURN_IF(regroup_space());
   }
   // the loop in segments of pop->traj_rebalance steps (all at once when 0);
-  // between segments the groups follow the work measured in the last one
-  const uint32_t seg = (balance && pop->traj_rebalance) ? pop->traj_rebalance : std::max<uint32_t>(p->n_steps, 1);
+  // between segments the groups follow the work measured in the last one,
+  // or the candidates' new positions
+  const uint32_t seg = (mode && pop->traj_rebalance) ? pop->traj_rebalance : std::max<uint32_t>(p->n_steps, 1);
   uint32_t done = 0;
   do {
     const uint32_t len = std::min(seg, p->n_steps - done);
@@ -883,10 +1281,26 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
     if (a.work) {
       DNAB_RETURN_IF(download(ctx, work.data(), pop->traj_work.p, nc * sizeof(uint32_t)));
       DNAB_RETURN_IF(regroup());
+    } else if (mode == 2 && done < p->n_steps) {
+      DNAB_RETURN_IF(regroup_space());
     }
   } while (done < p->n_steps);
   // the final state: energies, forces and the last scoring row, with the
   // clash pairs in FP64 as every reported evaluation has them
+  if (a.packed) {
+    // a population large enough for packed groups evaluates it with the
+    // population kernels (tiles sorted across candidates; the same results)
+    hipEvent_t ev_end;
+    hipEvent_t ev = timer_begin(ctx, "trajectory_final", &ev_end);
+    DNAB_RETURN_IF(population_sort(ctx, pop));
+    DNAB_RETURN_IF(population_pair(ctx, pop, count ? kEvalCount : DNAB_EVAL_FORCES));
+    DNAB_RETURN_IF(population_reduce_energies(ctx, pop, DNAB_EVAL_FORCES));
+    const uint32_t n_mid = p->score_every ? (p->n_steps + p->score_every - 1) / p->score_every : 0;
+    DNAB_HIP_CHECK(ctx, hipMemcpyAsync(pop->score_e.p + static_cast<size_t>(n_mid) * nc, pop->energy.p,
+                                       nc * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    timer_end(ctx, "trajectory_final", ev, ev_end);
+    return DNAB_OK;
+  }
   a.s0 = p->n_steps;
   a.n_steps = 0;
   a.final_eval = 1;
@@ -910,7 +1324,7 @@ int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_
   if (!pop || !t) return DNAB_ERR_INVALID;
   t->waves = pop->traj_waves;
   t->cand_per_group = pop->traj_cpw;
-  t->balance = pop->traj_balance ? 1u : 0u;
+  t->balance = pop->traj_balance;
   t->cells_per_item = pop->traj_cells;
   t->chunks_per_item = pop->traj_chunks;
   t->rebalance_steps = pop->traj_rebalance;
@@ -925,6 +1339,8 @@ int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
   if (t->cand_per_group < 1 || t->cand_per_group > kTrajMaxCpw)
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "trajectory tuning: cand_per_group must be 1..%u",
                           kTrajMaxCpw);
+  if (t->balance > 2)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "trajectory tuning: balance must be 0, 1 or 2");
   if (t->cells_per_item < 1 || t->cells_per_item > 64)
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "trajectory tuning: cells_per_item must be 1..64");
   if (t->chunks_per_item < 1 || t->chunks_per_item > static_cast<uint32_t>(kLoadAhead))
@@ -935,7 +1351,7 @@ int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
   pop->traj_waves = t->waves;
   pop->traj_cells = t->cells_per_item;
   pop->traj_cpw = t->cand_per_group;
-  pop->traj_balance = t->balance != 0;
+  pop->traj_balance = t->balance;
   return DNAB_OK;
 }
 

@@ -187,6 +187,8 @@ _PROTOS = {
     "dnab_population_set_tuning": (C.c_int, [vp, vp, C.POINTER(PairTuningC)]),
     "dnab_population_get_trajectory_tuning": (C.c_int, [vp, C.POINTER(TrajTuningC)]),
     "dnab_population_set_trajectory_tuning": (C.c_int, [vp, vp, C.POINTER(TrajTuningC)]),
+    "dnab_population_autotune_trajectory": (C.c_int, [vp, vp, C.POINTER(LangevinParamsC),
+                                                      C.c_uint32, C.POINTER(TrajTuningC)]),
     "dnab_population_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(PairTuningC),
                                            C.POINTER(C.c_double)]),
     "dnab_synth_shell_population": (C.c_int, [C.c_size_t, C.c_size_t, f32p, C.c_float, C.c_float,

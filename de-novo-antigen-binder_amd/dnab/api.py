@@ -578,6 +578,14 @@ class DevicePopulation:
         self.ctx.check(N.lib.dnab_population_set_trajectory_tuning(self.ctx.handle, self._h,
                                                                    C.byref(t)))
 
+    def autotune_trajectory(self, params: N.LangevinParamsC, trial_steps: int = 20) -> dict:
+        """Pick the fastest fused-loop shape for these loop parameters and
+        keep it (the coordinates are restored; results do not depend on it)."""
+        t = N.TrajTuningC()
+        self.ctx.check(N.lib.dnab_population_autotune_trajectory(
+            self.ctx.handle, self._h, C.byref(params), trial_steps, C.byref(t)))
+        return self.trajectory_tuning()
+
     def autotune(self, reps: int = 3) -> tuple[dict, float]:
         """Pick the fastest pair-kernel work split for the current state."""
         t = N.PairTuningC()

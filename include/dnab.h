@@ -619,11 +619,15 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
                       uint32_t* topk_idx, double* topk_energy, uint32_t* n_topk);
 /* Workgroup shape of the fused trajectory kernel (no reference counterpart):
  * `waves` (4, 8 or 16) waves run the trajectories of up to `cand_per_group`
- * (1..16) candidates together, sharing their pair work. With `balance` set,
- * a first evaluation measures each candidate's pair work and candidates are
- * grouped so that groups carry similar work (a candidate in deep contact runs
- * alone), heaviest group first; otherwise groups are consecutive candidates.
- * Groups that do not fit one workgroup's LDS hold fewer candidates.
+ * (1..16) candidates together, sharing their pair work. `balance` 0: groups
+ * are consecutive candidates, each candidate's atoms in tiles of their own;
+ * 1: a first evaluation measures each candidate's pair work and candidates
+ * are grouped so that groups carry similar work (a candidate in deep contact
+ * runs alone), heaviest group first; 2: candidates of one antigen are grouped
+ * by position (the Morton order of their centroids) and each launch cuts the
+ * group's atoms into compact tiles of 64 by recursive bisection, so a tile
+ * sees fewer antigen atoms. Groups that do not fit one workgroup's LDS hold
+ * fewer candidates.
  * Each step lists, per tile of 64 atoms, the antigen chunks (up to 64 atoms
  * of one grid cell) of the cells within the cutoff of the tile's box; a work
  * item is a run of `chunks_per_item` of them (or of `cells_per_item` cells
@@ -637,13 +641,26 @@ typedef struct dnab_traj_tuning {
   uint32_t cells_per_item;  /* grid cells per work item, for a tile whose
                                chunk list overflowed (1..64) */
   uint32_t chunks_per_item; /* listed chunks per work item (1..4) */
-  uint32_t rebalance_steps; /* with balance: the loop runs in launches of this
-                               many steps, regrouping between them by the work
-                               each candidate had in the last (0: one launch) */
+  uint32_t rebalance_steps; /* with balance 1 or 2: the loop runs in launches
+                               of this many steps, regrouping between them by
+                               the work each candidate had in the last, or by
+                               the candidates' new positions (0: one launch) */
 } dnab_traj_tuning;
 int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_tuning* t);
 int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
                                           const dnab_traj_tuning* t);
+/* Picks the fused kernel's shape for this population and these loop
+ * parameters (no reference counterpart): times trial_steps (1..10000) steps
+ * of the fused loop, wall clock with the host's regrouping, from the
+ * population's coordinates for each shape of a coordinate descent over waves
+ * {4, 8, 16}, cand_per_group {1, 2, 4, 8}, chunks_per_item {1, 2, 4},
+ * cells_per_item {2, 8} and balance {0, 1, 2} (rebalance_steps kept), and keeps
+ * the fastest in the population (also written to `chosen`, nullable). The
+ * coordinates are put back afterwards and no evaluation is current. The
+ * shape changes only the speed: every shape gives the same results. */
+int dnab_population_autotune_trajectory(dnab_ctx* ctx, dnab_population* pop,
+                                        const dnab_langevin_params* p, uint32_t trial_steps,
+                                        dnab_traj_tuning* chosen);
 /* Score history of the last dnab_langevin_run: one row per scoring (the
  * states at steps 0, score_every, 2 score_every, ... before n_steps, then the
  * final state), each its k best (ascending energy, ties by index). The last

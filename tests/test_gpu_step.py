@@ -78,8 +78,10 @@ def test_run_topk_consistent(ctx):
     assert np.array_equal(hi[-1], idx)
 
 
-def _run_state(ctx, w, **kw):
+def _run_state(ctx, w, tuning=None, **kw):
     ags, pop = W.device_objects(ctx, w)
+    if tuning:
+        pop.set_trajectory_tuning(*tuning)
     p = langevin_params(restraint_radius=w.restraint_radius[0], **kw)
     idx, e = pop.run(p)
     ef, err = pop.energies()
@@ -124,10 +126,13 @@ def test_trajectory_tuning_changes_speed_not_results(ctx):
     kw = dict(n_steps=12, score_every=4, top_k=8, with_noise=1, seed=9)
     ref = None
     # (rebalance 5: the 12 steps run as launches of 5, 5 and 2 steps, regrouped between them)
+    # (balance 2: groups by position, atoms bisected into compact tiles; with
+    # rebalance 5 regrouped by the new positions between launches)
     for waves, cpw, bal, cells, chunks, reb in ((4, 1, 0, 16, 4, 0), (8, 3, 0, 5, 1, 0),
                                                 (16, 4, 0, 64, 3, 0), (16, 16, 0, 1, 2, 0),
                                                 (16, 6, 1, 16, 2, 0), (4, 2, 1, 3, 1, 5),
-                                                (16, 4, 1, 2, 2, 5)):
+                                                (16, 4, 1, 2, 2, 5), (16, 16, 2, 2, 2, 0),
+                                                (8, 5, 2, 1, 4, 0), (4, 3, 2, 8, 1, 5)):
         ags, pop = W.device_objects(ctx, w)
         pop.set_trajectory_tuning(waves, cpw, bal, cells, chunks, reb)
         assert pop.trajectory_tuning() == {"waves": waves, "cand_per_group": cpw, "balance": bal,
@@ -144,9 +149,43 @@ def test_trajectory_tuning_changes_speed_not_results(ctx):
             _assert_same(ref, st)
     with pytest.raises(RuntimeError):
         pop.set_trajectory_tuning(12, 4)
+    with pytest.raises(RuntimeError):
+        pop.set_trajectory_tuning(16, 4, 3)
 
 
-@pytest.mark.parametrize("config", ["config2", "config3", "config2_fine_cells"])
+def test_trajectory_autotune_keeps_state_and_results(ctx):
+    # the tuner runs trial loops from the population's state and puts the
+    # coordinates back; the chosen shape is a valid one and the run after it
+    # equals a run with the default shape
+    w = W.config3(n_cand=80)
+    kw = dict(n_steps=12, score_every=4, top_k=8, with_noise=1, seed=9)
+    ags, pop = W.device_objects(ctx, w)
+    p = langevin_params(restraint_radius=w.restraint_radius[0], **kw)
+
+    def state(pop):
+        idx, e = pop.run(p)
+        hi, he = pop.scores()
+        return dict(idx=idx, e=e, coords=pop.coords(), forces=pop.forces(),
+                    energy=pop.energies()[0], err=pop.energies()[1], hist_idx=hi, hist_e=he)
+
+    ref = state(pop)
+    ags, pop = W.device_objects(ctx, w)
+    x0 = pop.coords()
+    t = pop.autotune_trajectory(p, trial_steps=6)
+    assert t["waves"] in (4, 8, 16) and t["cand_per_group"] in (1, 2, 4, 8)
+    assert t["chunks_per_item"] in (1, 2, 4) and t["cells_per_item"] in (2, 8)
+    assert t["balance"] in (0, 1, 2) and t["rebalance_steps"] == 0
+    for a, b in zip(x0, pop.coords()):
+        assert np.array_equal(a, b)
+    with pytest.raises(RuntimeError):
+        pop.forces()  # no evaluation is current after tuning
+    _assert_same(ref, state(pop))
+    with pytest.raises(RuntimeError):
+        pop.autotune_trajectory(p, trial_steps=0)
+
+
+@pytest.mark.parametrize("config", ["config2", "config3", "config2_fine_cells", "config2_packed",
+                                    "config3_packed"])
 def test_fused_trajectory_equals_step_kernels(ctx, config):
     # no noise (the reference's GPU phase), pair counting on: the in-cutoff
     # pair count is a property of the trajectory, the tested count is not.
@@ -156,10 +195,12 @@ def test_fused_trajectory_equals_step_kernels(ctx, config):
         import dataclasses
         w = dataclasses.replace(W.config2(n_cand=96), cell_size=4.0)
     else:
-        w = getattr(W, config)(n_cand=96)
+        w = getattr(W, config.replace("_packed", ""))(n_cand=96)
+    # packed: groups by position, each group's atoms in compact tiles
+    tuning = (16, 16, 2, 2, 2, 0) if config.endswith("_packed") else None
     kw = dict(n_steps=40, score_every=0, top_k=8, count_pairs=1)
     step = _run_state(ctx, w, step_kernels=1, **kw)
-    fused = _run_state(ctx, w, step_kernels=0, **kw)
+    fused = _run_state(ctx, w, step_kernels=0, tuning=tuning, **kw)
     _assert_same(step, fused)
     assert step["stats"][0] == fused["stats"][0] > 0
     print(f"{config}: {fused['stats'][0]} pairs in cutoff over 41 evaluations; tested "
