@@ -289,6 +289,138 @@ def saturated_trajectory_rate(ctx, n_cand, steps=10):
                 "bit_identical": bool(same)}}
 
 
+def loop_leg(ctx, w, trial_steps=20):
+    """One workload's gradient-step loop (w.n_steps steps, scored every
+    w.score_every, top w.top_k kept) on both engines from the same start
+    state: the fused trajectory kernel with its autotuned shape, and the
+    per-step kernels (autotuned work split, graph replay). In-cutoff pairs
+    are counted by a replay with the counting variant; the fused rate uses
+    the event time of its launches (steps and final evaluation)."""
+    import numpy as np
+    import dnab
+    from dnab import workloads as W
+    ags, pop = W.device_objects(ctx, w)
+    x0 = pop.coords()
+
+    def params(n, count=False, step_kernels=0):
+        return dnab.langevin_params(n_steps=n, score_every=w.score_every, top_k=w.top_k,
+                                    restraint_radius=w.restraint_radius[0], with_noise=w.with_noise,
+                                    count_pairs=1 if count else 0, step_kernels=step_kernels,
+                                    use_graph=1)
+    trial = min(w.n_steps, trial_steps)
+    tuning = pop.autotune_trajectory(params(trial), trial_steps=trial)
+    pop.set_coords(*x0)
+    pop.reset_pair_stats()
+    pop.run(params(w.n_steps, count=True))
+    n_in, n_tested = pop.pair_stats()
+    xc = pop.coords()
+
+    def timed(step_kernels, reps=2):
+        best = None
+        for _ in range(reps):
+            pop.set_coords(*x0)
+            ctx.synchronize()
+            ctx.set_timing(not step_kernels)
+            k0 = traj_kernel_time(ctx)
+            t0 = time.perf_counter()
+            idx, _ = pop.run(params(w.n_steps, step_kernels=step_kernels))
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            k = traj_kernel_time(ctx) - k0
+            ctx.set_timing(False)
+            if best is None or dt < best[0]:
+                best = (dt, k, idx)
+        return best
+    dt_f, k_f, idx_f = timed(0)
+    xf = pop.coords()
+    pop.set_coords(*x0)
+    pair_tuning, _ = pop.autotune(reps=3)
+    dt_s, _, idx_s = timed(1)
+    xs = pop.coords()
+    same = (all(np.array_equal(a, b) for a, b in zip(xf, xs)) and np.array_equal(idx_f, idx_s) and
+            all(np.array_equal(a, b) for a, b in zip(xf, xc)))
+    achieved = n_in * FLOP_PER_PAIR_EF / (k_f * 1e-3) / 1e12
+    out = {"workload": w.description, "antigen_atoms": [int(a.n_atoms()) for a in w.antigens],
+           "candidates": int(w.pop.n_cand), "candidate_atoms": int(len(w.pop.atoms)),
+           "cutoff_A": w.cutoff, "steps": w.n_steps, "evaluations": w.n_steps + 1,
+           "score_every": w.score_every, "pairs_in_cutoff": int(n_in), "pairs_tested": int(n_tested),
+           "tested_per_in_cutoff": n_tested / max(n_in, 1),
+           "fused": {"wall_ms": dt_f * 1e3, "kernel_ms": k_f, "pairs_per_s": n_in / dt_f,
+                     "kernel_pairs_per_s": n_in / (k_f * 1e-3), "achieved_tflops": achieved,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "trajectory_tuning": tuning},
+           "step_kernels": {"wall_ms": dt_s * 1e3, "pairs_per_s": n_in / dt_s,
+                            "pair_tuning": pair_tuning, "graph": True},
+           "paths_bit_identical": bool(same)}
+    pop.close()
+    for a in ags:
+        a.close()
+    return out
+
+
+def probe_pair_counts(ctx, ag_xyz, probe_xyz, cell, rc):
+    """In-cutoff (candidate atom, antigen atom) pairs under the reference's
+    FP32 predicate (src/energy.rs:284): all of them, and those the
+    reference's 27-cell probe (src/spatial.rs:44-58) lists on a grid of
+    `cell` over the antigen atoms."""
+    import numpy as np
+    import dnab
+    A = np.ascontiguousarray(ag_xyz, np.float32)
+    P = np.ascontiguousarray(probe_xyz, np.float32)
+
+    def keep(d):
+        r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+        return ~((r2 > np.float32(rc * rc)) | (r2 < np.float32(0.25)))
+    n_all = 0
+    for k in range(0, len(P), 2048):
+        n_all += int(keep(P[k:k + 2048, None, :] - A[None, :, :]).sum())
+    g = dnab.SpatialHashGrid(ctx, cell)
+    g.build(A[:, 0], A[:, 1], A[:, 2])
+    off, idx = g.query_neighbors(P[:, 0], P[:, 1], P[:, 2])
+    pi = np.repeat(np.arange(len(P)), np.diff(off.astype(np.int64)))
+    found = int(keep(P[pi] - A[idx.astype(np.int64)]).sum())
+    g.close()
+    return n_all, found
+
+
+def c1_leg(ctx, reps=3):
+    """run_allatom at the plain method's shape: a 16-residue peptide antigen,
+    64 binders of 12 residues on a ring, 600 refinement iterations with the
+    Monte-Carlo moves, no broad phase (the reference's path without a GPU),
+    grid cells of 8 A; and the in-cutoff pairs the reference's 27-cell probe
+    misses with 8 A cells (10 A cutoff) against 10 A cells, on the start
+    binders and on the results."""
+    import numpy as np
+    import dnab
+    from dnab import workloads as W
+    w = W.config1()
+    ag = w.antigens[0]
+    dag = dnab.DeviceAntigen(ctx, ag, cutoff=w.cutoff, cell_size=w.cell_size)
+    kw = dict(ab_length=12, population=64, cpu_iterations=600, top_n=64, n_broad=0, seed=11)
+    dnab.run_allatom(ctx, dag, **kw)  # warm-up
+    best, res = 1e9, None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = dnab.run_allatom(ctx, dag, **kw)
+        best = min(best, time.perf_counter() - t0)
+    A = np.stack([ag.atoms.x, ag.atoms.y, ag.atoms.z], 1)
+    a = w.pop.atoms
+    start = np.stack([a.x, a.y, a.z], 1)
+    final = np.concatenate([np.stack([p.atoms.x, p.atoms.y, p.atoms.z], 1) for p, _, _ in res])
+    missed = {}
+    for name, P in (("start", start), ("results", final)):
+        n_all, f8 = probe_pair_counts(ctx, A, P, 8.0, w.cutoff)
+        _, f10 = probe_pair_counts(ctx, A, P, 10.0, w.cutoff)
+        missed[name] = {"pairs_in_cutoff": n_all, "found_cell_8": f8, "missed_cell_8": n_all - f8,
+                        "found_cell_10": f10, "missed_cell_10": n_all - f10}
+    dag.close()
+    e = [r[1] for r in res]
+    return {"workload": w.description + ", run_allatom: 600 iterations with moves, no broad phase",
+            "antigen_atoms": int(ag.n_atoms()), "ms_per_call": best * 1e3,
+            "binder_iterations_per_s": 64 * 600 / best, "best_energy": e[0],
+            "energies_sorted": bool(all(x <= y for x, y in zip(e, e[1:]))),
+            "missed_pairs": missed}
+
+
 def interface_positions(ab, ag):
     """is_interface (src/filters.rs:96-105): Calpha within 8 A of an antigen Calpha."""
     import numpy as np
@@ -595,6 +727,8 @@ def main():
     ap.add_argument("--no-maturation", action="store_true")
     ap.add_argument("--no-allatom", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--no-loops", action="store_true", help="skip the c3 / c4 loop legs")
+    ap.add_argument("--no-c1", action="store_true", help="skip the c1 run_allatom leg")
     ap.add_argument("--tuning", default="",
                     help="fixed pair work split 'survivors,chunks,waves' instead of autotuning")
     ap.add_argument("--traj-tuning", default="",
@@ -796,6 +930,24 @@ def main():
         out["roofline"]["saturated"] = {
             "trajectory": saturated_trajectory_rate(ctx, 16384),
             "pair_kernels": saturated_pair_rate(ctx, 16384)}
+    if not args.no_loops:
+        log("[bench] gradient-step loops at the c3 and c4 shapes")
+        loops = {"c2": {"pairs_per_s": value, "frac": achieved / PEAK_FP32_TFLOPS,
+                        "tested_per_in_cutoff": n_tested / max(n_in, 1),
+                        "fused_ms_per_step": step_ms["fused"],
+                        "step_kernels_ms_per_step": step_ms["step_kernels"]}}
+        for name in ("config3", "config4"):
+            t = time.perf_counter()
+            leg = loop_leg(ctx, getattr(W, name)())
+            loops[name.replace("config", "c")] = leg
+            log(f"[bench] {name}: fused {leg['fused']['wall_ms']:.2f} ms, step kernels "
+                f"{leg['step_kernels']['wall_ms']:.2f} ms, {leg['pairs_in_cutoff']:,} pairs, "
+                f"identical {leg['paths_bit_identical']} ({time.perf_counter() - t:.1f}s)")
+        out["loops"] = loops
+    if not args.no_c1:
+        log("[bench] run_allatom at the c1 shape")
+        out["c1_run_allatom"] = c1_leg(ctx)
+        log(f"[bench] c1 run_allatom: {out['c1_run_allatom']['ms_per_call']:.1f} ms per call")
     if not args.no_precision:
         log("[bench] precision study at the config-4 shape")
         wp = W.config4(n_cand=1024)
