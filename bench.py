@@ -200,6 +200,22 @@ def traj_kernel_time(ctx):
     return ctx.kernel_time("trajectory")[0] + ctx.kernel_time("trajectory_final")[0]
 
 
+def measured_valu(step_kernels):
+    """VALU issue of the dominant kernel of the c2 loop from the committed
+    SQ_INSTS_VALU profile: (fraction of the wave64 issue peak, wave
+    instructions per in-cutoff pair), or None."""
+    f = ROOT / "profiles" / "valu_c2.txt"
+    if not f.exists():
+        return None
+    key = "pair_kernels" if step_kernels else "trajectory"
+    vals = {}
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 2 and parts[0].startswith(key + "_valu_"):
+            vals[parts[0][len(key) + 6:]] = float(parts[1])
+    return vals or None
+
+
 def saturated_pair_rate(ctx, n_cand, reps=20):
     """The pair kernels fed enough work: the c2 antigen with n_cand binders at
     their starting positions (surface contact), energies + forces evaluated
@@ -904,6 +920,10 @@ def main():
                                "this workload, scaled by a device copy of known size: " +
                                ("profiles/pair_traffic_c2.txt" if args.step_kernels
                                 else "profiles/trajectory_traffic_c2.txt")),
+            "valu_issue": measured_valu(args.step_kernels),
+            "valu_issue_source": ("rocprofv3 --pmc SQ_INSTS_VALU of the same kernel on this workload: "
+                                  "profiles/valu_c2.txt (fraction of 1,024 SIMDs x 2,400 MHz / 4 cycles "
+                                  "per wave64 instruction)"),
             "kernel": kernel,
             "kernel_avg_ms": avg_ms,
             "kernel_launches": launches,
