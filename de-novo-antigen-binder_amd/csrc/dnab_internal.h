@@ -179,6 +179,35 @@ struct AgView {
   float restraint_radius;  // 0 = use the step parameter
 };
 
+// Antigen maps for the learned prescreen's features: values on a regular
+// grid of spacing h over the antigen's atom box widened by the cutoff and two
+// spacings. Energy planes (fp16): the LJ energy of a probe atom of each of the
+// kMapLjClasses parameter classes, the Coulomb potential sum 332 q_i / r and
+// the hydrophobic sum (-0.5 per hydrophobic antigen atom within 6 A), each
+// stored per grid cell as its 8 corner values (one 16-byte record, corner
+// order x fastest) so that trilinear interpolation is one load per plane;
+// kMapRbf Gaussian radial densities per grid point (one 16-byte record).
+constexpr int kMapLjClasses = 7;
+constexpr int kMapPlanes = kMapLjClasses + 2;
+constexpr int kMapRbf = 8;
+struct MapDev {
+  float h = 0.0f, inv_h = 0.0f;
+  float origin[3] = {0, 0, 0};
+  int32_t n[3] = {0, 0, 0};    // grid points per axis (cells: n - 1)
+  size_t npts = 0, ncells = 0;
+  DevBuf<uint16_t> erec;       // [plane][cell][8] fp16
+  DevBuf<uint16_t> rbf;        // [point][8] fp16
+};
+struct MapView {
+  const uint16_t* erec;
+  const uint16_t* rbf;
+  float origin[3];
+  float inv_h;
+  int32_t n[3];
+  uint32_t valid;
+  unsigned long long ncells;
+};
+
 struct KernelTimer {
   std::string name;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -231,6 +260,7 @@ struct dnab_antigen {
   dnab::DevBuf<uint8_t> ca_hyd, ca_aa;
   bool has_ca_aa = false;
   dnab::GridDev ca_grid;
+  dnab::MapDev map;  // prescreen feature maps (built on first use)
 };
 
 struct dnab_population {
@@ -313,7 +343,9 @@ struct dnab_population {
   bool nonfinite = false;
   dnab::DevBuf<uint8_t> cand_status;  // per candidate: DNAB_STATUS_* of the last evaluation
   // prescreen
-  dnab::DevBuf<uint16_t> feat;   // n_cand x 3360 bf16
+  dnab::DevBuf<uint16_t> feat;   // n_cand x kPrescreenFeatures bf16
+  dnab::DevBuf<dnab::MapView> map_views;  // per antigen, refreshed per feature pass
+  dnab::DevBuf<uint32_t> sel_scratch;     // select_smallest's stage lists
   dnab::DevBuf<float> scores;    // n_cand
   dnab::DevBuf<double> scores64;
   dnab::DevBuf<uint32_t> keep_idx;
@@ -366,9 +398,15 @@ int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 // step.hip: multi-pass top-k (ascending, NaN last, ties by index)
 int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k, uint32_t rows = 1);
+// first_idx (nullable): the entries' indices (else their positions), for
+// the output and for breaking ties
 void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
                  uint32_t* final_idx, const uint32_t* counter = nullptr, uint32_t row_div = 1,
-                 uint32_t row_fixed = 0);
+                 uint32_t row_fixed = 0, const uint32_t* first_idx = nullptr);
+// step.hip: the min(m, n) smallest scores (ties by index) as indices in
+// index order, 1 <= m <= 2048
+int select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m, uint32_t* out_idx,
+                    DevBuf<uint32_t>& scratch);
 // pair.hip
 int population_sort(dnab_ctx* ctx, dnab_population* pop);
 int population_evaluate(dnab_ctx* ctx, dnab_population* pop, unsigned flags);
@@ -382,4 +420,15 @@ int require_finite(dnab_ctx* ctx, const dnab_population* pop, const char* what);
 bool atom_params_ok(const dnab_atom_cloud* c);
 // noalloc.hip: the no-alloc entry's scratch address on `device` (current)
 int noalloc_init_device(int device);
+// maps.hip: the prescreen's antigen maps and per-candidate features
+constexpr int kPrescreenFeatures = 96;
+int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing);
+// features (bf16 rows of kPrescreenFeatures, nullable) and the map energy of
+// every candidate (nullable); builds missing antigen maps first
+int population_map_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* feat, float* map_e);
+// traj.hip: final-state evaluation (energies, forces; clash pairs in FP64) of
+// the n candidates listed in d_idx (device), one workgroup each
+// (score_scratch: n_cand doubles the kernel may write)
+int trajectory_eval_subset(dnab_ctx* ctx, dnab_population* pop, const uint32_t* d_idx, uint32_t n,
+                           double* score_scratch);
 }  // namespace dnab

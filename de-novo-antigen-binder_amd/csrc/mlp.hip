@@ -1,12 +1,16 @@
-// Learned prescreen (no reference counterpart): residue-level RBF distance-
-// histogram features, a 3360 -> 256 -> 256 -> 256 -> 1 SiLU MLP with bf16
-// weights and activations and FP32 accumulation on the MFMA units, trained on
-// the device (Adam) on energies that the exact pair kernel computes, and the
-// screening pipeline features -> MLP -> top-n -> exact re-score -> top-k.
+// Learned prescreen (no reference counterpart): a 96 -> 256 -> 256 -> 256 -> 1
+// SiLU MLP over per-candidate features read from antigen maps (maps.hip),
+// with bf16 weights and activations and FP32 accumulation on the MFMA units,
+// trained on the device (Adam) on energies that the exact pair kernel
+// computes, and the screening pipeline features -> MLP -> top-n -> exact
+// re-score of the kept candidates -> top-k.
 //
-// GEMM: C[M x N] = op(A)[M x K] op(B)[K x N], bf16 operands staged through
-// LDS with k contiguous, v_mfma_f32_16x16x32_bf16 (gfx950), a workgroup of 4
-// waves computes a 64 x 64 tile, each wave 32 x 32 as 2 x 2 MFMA blocks.
+// GEMM (training): C[M x N] = op(A)[M x K] op(B)[K x N], bf16 operands staged
+// through LDS with k contiguous, v_mfma_f32_16x16x32_bf16 (gfx950), a
+// workgroup of 4 waves computes a 64 x 64 tile, each wave 32 x 32 as 2 x 2
+// MFMA blocks. Inference runs the whole network per 64-row tile in one
+// kernel (k_mlp_infer), activations in LDS, with the same MFMA sequence and
+// epilogues, so its outputs equal the layer-by-layer GEMM path bit for bit.
 // MFMA lane layout (16x16x32): lane l supplies A[row l%16][k 8(l/16)..+7]
 // and B[k 8(l/16)..+7][col l%16]; it receives C[rows 4(l/16)..+3][col l%16].
 #include <hip/hip_runtime.h>
@@ -133,75 +137,9 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
     }
 }
 
-// ---- features --------------------------------------------------------------------
+// ---- features (maps.hip) ------------------------------------------------------
 
-constexpr int kRbf = 16;
-constexpr int kPairs = 210;
-constexpr int kFeat = kPairs * kRbf;  // 3360
-constexpr float kFeatCutoff = 14.0f;
-constexpr float kRbfStart = 2.0f, kRbfStep = 0.8f, kRbfSigma = 0.8f;
-constexpr int kFeatWaves = 4;
-
-// Unordered residue-type pair (a, b) -> 0..209.
-__host__ __device__ inline int pair_index(int a, int b) {
-  if (a > b) {
-    int t = a;
-    a = b;
-    b = t;
-  }
-  return a * (41 - a) / 2 + (b - a);
-}
-
-// One wave per candidate: every (candidate residue, antigen residue) pair with
-// Calpha distance d < 14 A adds exp(-(d - mu_k)^2 / (2 sigma^2)), k = 0..15,
-// to bin [pair_index(aa_i, aa_j)][k]; the row is stored as bf16(log1p(h)).
-__global__ void __launch_bounds__(256)
-k_features(const AgView* __restrict__ views, const uint32_t* __restrict__ cand_ag,
-           const uint32_t* __restrict__ cand_res_off, const uint32_t* __restrict__ res_ca,
-           const uint8_t* __restrict__ res_aa, const float4* __restrict__ pos, uint32_t n_cand,
-           uint16_t* __restrict__ X) {
-  __shared__ float hist[kFeatWaves][kFeat];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t c = blockIdx.x * kFeatWaves + wave;
-  float* h = hist[wave];
-  for (int f = lane; f < kFeat; f += kWave) h[f] = 0.0f;
-  __syncthreads();
-  if (c < n_cand) {
-    const AgView& V = views[cand_ag[c]];
-    const uint32_t n_ag = V.ca_grid.n;
-    const float inv2s2 = 1.0f / (2.0f * kRbfSigma * kRbfSigma);
-    const float cut2 = kFeatCutoff * kFeatCutoff;
-    for (uint32_t j0 = 0; j0 < n_ag; j0 += kWave) {
-      const uint32_t j = j0 + lane;
-      float ax = 0.f, ay = 0.f, az = 0.f;
-      int aa_j = 0;
-      if (j < n_ag) {
-        ax = V.ca_x[j];
-        ay = V.ca_y[j];
-        az = V.ca_z[j];
-        aa_j = V.ca_aa[j];
-      }
-      for (uint32_t r = cand_res_off[c]; r < cand_res_off[c + 1]; ++r) {
-        const float4 p = pos[res_ca[r]];
-        const float dx = p.x - ax, dy = p.y - ay, dz = p.z - az;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (j < n_ag && d2 < cut2) {
-          const float d = sqrtf(d2);
-          float* bin = h + pair_index(res_aa[r], aa_j) * kRbf;
-          for (int k = 0; k < kRbf; ++k) {
-            const float u = d - (kRbfStart + kRbfStep * static_cast<float>(k));
-            atomicAdd(&bin[k], __expf(-u * u * inv2s2));
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-  if (c < n_cand) {
-    uint16_t* row = X + static_cast<size_t>(c) * kFeat;
-    for (int f = lane; f < kFeat; f += kWave) row[f] = f32_to_bf16(log1pf(h[f]));
-  }
-}
+constexpr int kFeat = kPrescreenFeatures;  // 96
 
 // ---- small kernels ------------------------------------------------------------------
 
@@ -254,7 +192,7 @@ __global__ void k_last_backward(const uint16_t* __restrict__ A, const float* __r
       const size_t o = static_cast<size_t>(m) * H + n;
       s += bf16_to_f32(A[o]) * dy[m];
       const float z = Z[o];
-      const float sg = 1.0f / (1.0f + __expf(-z));
+      const float sg = sigmoidf_(z);
       dZ[o] = f32_to_bf16(dy[m] * wn * (sg * (1.0f + z * (1.0f - sg))));
     }
     gw[n] = s;
@@ -307,25 +245,116 @@ __global__ void k_targets(const double* __restrict__ e, uint32_t n, float scale,
   t[i] = static_cast<float>(asinh(fmax(v, -1e30) / scale));
 }
 
-__global__ void k_mask_clear(uint8_t* __restrict__ mask, uint32_t n) {
+// ---- fused inference -----------------------------------------------------------
+
+constexpr int kInferRows = 64;
+constexpr int kHid = 256;
+constexpr int kXStride = kFeat + 8;  // bf16 per LDS row (16-byte aligned, banks spread)
+constexpr int kHStride = kHid + 8;
+
+// W[K][N] (row-major bf16) -> WT[N][K]
+__global__ void k_transpose_bf16(const uint16_t* __restrict__ W, uint32_t K, uint32_t N,
+                                 uint16_t* __restrict__ WT) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) mask[i] = 0;
+  if (i >= K * N) return;
+  const uint32_t k = i / N, n = i - k * N;
+  WT[static_cast<size_t>(n) * K + k] = W[i];
 }
 
-__global__ void k_mask_set(uint8_t* __restrict__ mask, const uint32_t* __restrict__ idx,
-                           uint32_t k) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < k && idx[i] != 0xFFFFFFFFu) mask[idx[i]] = 1;
+// One dense SiLU layer of the tile: out[64][256] = bf16(silu(in[64][K] W + b)),
+// W given transposed (WT[256][K], k contiguous, read from global memory in
+// the MFMA B layout). Wave w computes columns 64w..64w+63 as 4 x 4 blocks.
+template <int K, int IN_STRIDE>
+__device__ inline void infer_layer(const uint16_t* __restrict__ in, const uint16_t* __restrict__ WT,
+                                   const float* __restrict__ bias, uint16_t* __restrict__ out,
+                                   uint32_t lane, uint32_t wave) {
+  const int n0 = static_cast<int>(wave) * 64;
+  const int kq = static_cast<int>(lane >> 4) * 8;
+  floatx4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += kKStep) {
+    bf16x8_t af[4], bw[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      bw[j] = *reinterpret_cast<const bf16x8_t*>(WT + static_cast<size_t>(n0 + 16 * j + (lane & 15)) * K + k0 + kq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      af[i] = *reinterpret_cast<const bf16x8_t*>(in + (16 * i + (lane & 15)) * IN_STRIDE + k0 + kq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bw[j], acc[i][j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + 16 * j + static_cast<int>(lane & 15);
+      const float b = bias[n];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 16 * i + static_cast<int>(lane >> 4) * 4 + r;
+        const float z = acc[i][j][r] + b;
+        out[m * kHStride + n] = f32_to_bf16(z * sigmoidf_(z));
+      }
+    }
 }
 
-__global__ void k_fill_mask_ones(uint8_t* __restrict__ mask, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) mask[i] = 1;
+struct InferArgs {
+  const uint16_t* X;    // [M][kFeat] bf16
+  uint32_t M;
+  const uint16_t* wt1;  // [256][kFeat]
+  const uint16_t* wt2;  // [256][256]
+  const uint16_t* wt3;  // [256][256]
+  const uint16_t* w4;   // [256] bf16
+  const float* b1;
+  const float* b2;
+  const float* b3;
+  const float* b4;
+  float* y;             // [M]
+};
+
+// The whole network on 64 rows per workgroup (4 waves); the last layer as
+// k_last_forward forms it (lane-strided products, then the wave sum).
+__global__ void __launch_bounds__(256, 2) k_mlp_infer(InferArgs g) {
+  __shared__ __attribute__((aligned(16))) uint16_t sX[kInferRows * kXStride];
+  __shared__ __attribute__((aligned(16))) uint16_t sA[kInferRows * kHStride];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[kInferRows * kHStride];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t r0 = blockIdx.x * kInferRows;
+  constexpr int kVec = kFeat / 8;  // 16-byte pieces per input row
+  for (int v = static_cast<int>(t); v < kInferRows * kVec; v += 256) {
+    const int r = v / kVec, c = (v - r * kVec) * 8;
+    uint4 w = make_uint4(0u, 0u, 0u, 0u);
+    if (r0 + r < g.M) w = *reinterpret_cast<const uint4*>(g.X + static_cast<size_t>(r0 + r) * kFeat + c);
+    *reinterpret_cast<uint4*>(&sX[r * kXStride + c]) = w;
+  }
+  __syncthreads();
+  infer_layer<kFeat, kXStride>(sX, g.wt1, g.b1, sA, lane, wave);
+  __syncthreads();
+  infer_layer<kHid, kHStride>(sA, g.wt2, g.b2, sB, lane, wave);
+  __syncthreads();
+  infer_layer<kHid, kHStride>(sB, g.wt3, g.b3, sA, lane, wave);
+  __syncthreads();
+  for (uint32_t q = 0; q < kInferRows / 4; ++q) {
+    const uint32_t m = wave * (kInferRows / 4) + q;
+    float s = 0.0f;
+    for (uint32_t n = lane; n < static_cast<uint32_t>(kHid); n += kWave)
+      s += bf16_to_f32(sA[m * kHStride + n]) * bf16_to_f32(g.w4[n]);
+    s = wave_reduce_f32<OpAdd>(s);
+    if (lane == 0 && r0 + m < g.M) g.y[r0 + m] = s + g.b4[0];
+  }
 }
 
-__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, uint32_t n) {
+// kept candidates' energies (from the population's per-candidate array)
+__global__ void k_gather_energy(const double* __restrict__ e, const uint32_t* __restrict__ idx,
+                                uint32_t n, double* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = a[i];
+  if (i < n) out[i] = e[idx[i]];
 }
 
 }  // namespace dnab
@@ -338,7 +367,6 @@ using namespace dnab;
 constexpr size_t kD = kFeat, kH = 256;
 constexpr size_t oW1 = 0, ob1 = kD * kH, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, oW3 = ob2 + kH,
                  ob3 = oW3 + kH * kH, ow4 = ob3 + kH, ob4 = ow4 + kH, kParams = ob4 + 1;
-constexpr uint32_t kPredictChunk = 16384;
 
 inline uint32_t nb(uint64_t n, uint32_t t) {
   uint64_t b = (n + t - 1) / t;
@@ -365,6 +393,7 @@ struct dnab_mlp {
   DevBuf<uint16_t> A1, A2, A3, dZ1, dZ2, dZ3, Xb;
   DevBuf<uint32_t> perm;
   DevBuf<float> loss_hist;
+  DevBuf<uint16_t> wt;  // inference copies of W1..W3, transposed (k contiguous)
 };
 
 struct dnab_dataset {
@@ -450,18 +479,42 @@ __global__ void k_sum_into(const float* __restrict__ a, uint32_t n, float scale,
 }
 
 int population_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* dst) {
-  for (auto* ag : pop->antigens)
-    if (!ag->has_ca_aa)
-      return dnab_set_error(ctx, DNAB_ERR_INVALID,
-                            "features need the antigens' ca_amino_acid");
-  if (pop->n_res && !pop->has_res_aa)
-    return dnab_set_error(ctx, DNAB_ERR_INVALID, "features need the population's res_amino_acid");
-  if (pop->n_cand)
-    hipLaunchKernelGGL(k_features, dim3(nb(pop->n_cand, kFeatWaves)), dim3(256), 0, ctx->stream,
-                       pop->views.p, pop->cand_ag.p, pop->cand_res_off.p, pop->res_ca.p,
-                       pop->res_aa.p, pop->pos.p, static_cast<uint32_t>(pop->n_cand), dst);
+  return population_map_features(ctx, pop, dst, nullptr);
+}
+
+// The bf16 weights W1..W3 transposed for k_mlp_infer (after every update).
+int refresh_inference_weights(dnab_ctx* ctx, dnab_mlp* n) {
+  DNAB_ALLOC_CHECK(ctx, n->wt.ensure(kD * kH + 2 * kH * kH));
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(nb(kD * kH, 256)), dim3(256), 0, ctx->stream,
+                     n->pbf.p + oW1, static_cast<uint32_t>(kD), static_cast<uint32_t>(kH), n->wt.p);
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(nb(kH * kH, 256)), dim3(256), 0, ctx->stream,
+                     n->pbf.p + oW2, static_cast<uint32_t>(kH), static_cast<uint32_t>(kH),
+                     n->wt.p + kD * kH);
+  hipLaunchKernelGGL(k_transpose_bf16, dim3(nb(kH * kH, 256)), dim3(256), 0, ctx->stream,
+                     n->pbf.p + oW3, static_cast<uint32_t>(kH), static_cast<uint32_t>(kH),
+                     n->wt.p + kD * kH + kH * kH);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   return DNAB_OK;
+}
+
+// Predictions for M rows of X (device) into y, one fused launch.
+void infer(dnab_ctx* ctx, dnab_mlp* n, const uint16_t* X, uint32_t M, float* y) {
+  InferArgs g;
+  g.X = X;
+  g.M = M;
+  g.wt1 = n->wt.p;
+  g.wt2 = n->wt.p + kD * kH;
+  g.wt3 = n->wt.p + kD * kH + kH * kH;
+  g.w4 = n->pbf.p + ow4;
+  g.b1 = n->p.p + ob1;
+  g.b2 = n->p.p + ob2;
+  g.b3 = n->p.p + ob3;
+  g.b4 = n->p.p + ob4;
+  g.y = y;
+  hipEvent_t ev_end;
+  hipEvent_t ev = timer_begin(ctx, "mlp_infer", &ev_end);
+  hipLaunchKernelGGL(k_mlp_infer, dim3(nb(M, kInferRows)), dim3(256), 0, ctx->stream, g);
+  timer_end(ctx, "mlp_infer", ev, ev_end);
 }
 
 int predict_device(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop) {
@@ -469,13 +522,7 @@ int predict_device(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop) {
   DNAB_ALLOC_CHECK(ctx, pop->feat.ensure(nc * kD));
   DNAB_ALLOC_CHECK(ctx, pop->scores.ensure(nc));
   DNAB_RETURN_IF(population_features(ctx, pop, pop->feat.p));
-  DNAB_RETURN_IF(ensure_rows(ctx, n, std::min<uint32_t>(kPredictChunk, nc ? nc : 1)));
-  for (size_t r0 = 0; r0 < nc; r0 += kPredictChunk) {
-    const uint32_t m = static_cast<uint32_t>(std::min<size_t>(kPredictChunk, nc - r0));
-    forward(ctx, n, pop->feat.p + r0 * kD, m, false);
-    DNAB_HIP_CHECK(ctx, hipMemcpyAsync(pop->scores.p + r0, n->y.p, m * sizeof(float),
-                                       hipMemcpyDeviceToDevice, ctx->stream));
-  }
+  if (nc) infer(ctx, n, pop->feat.p, static_cast<uint32_t>(nc), pop->scores.p);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   return DNAB_OK;
 }
@@ -519,7 +566,8 @@ int dnab_mlp_create(dnab_ctx* ctx, uint64_t seed, dnab_mlp** out) {
   if (!rc) {
     hipLaunchKernelGGL(k_to_bf16, dim3(nb(kParams, 256)), dim3(256), 0, ctx->stream, n->p.p,
                        n->pbf.p, kParams);
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = DNAB_ERR_HIP;
+    rc = refresh_inference_weights(ctx, n);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = DNAB_ERR_HIP;
   }
   if (rc) {
     dnab_mlp_destroy(n);
@@ -547,6 +595,7 @@ int dnab_mlp_set_params(dnab_ctx* ctx, dnab_mlp* n, const float* in, size_t coun
   DNAB_RETURN_IF(upload(ctx, n->p.p, in, kParams * 4));
   hipLaunchKernelGGL(k_to_bf16, dim3(nb(kParams, 256)), dim3(256), 0, ctx->stream, n->p.p,
                      n->pbf.p, kParams);
+  DNAB_RETURN_IF(refresh_inference_weights(ctx, n));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DNAB_OK;
 }
@@ -676,6 +725,7 @@ int dnab_mlp_train(dnab_ctx* ctx, dnab_mlp* n, dnab_dataset* d, const dnab_train
                        c1, c2);
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
+  DNAB_RETURN_IF(refresh_inference_weights(ctx, n));
   if (loss_history && tp->n_steps)
     DNAB_RETURN_IF(download(ctx, loss_history, n->loss_hist.p, tp->n_steps * 4));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -716,33 +766,27 @@ int dnab_prescreen(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop, uint32_t n_
   const uint32_t k = std::min(top_k, keep);
   if (n_out) *n_out = k;
   if (nc == 0) return DNAB_OK;
+  // features and predictions for every candidate
   DNAB_RETURN_IF(predict_device(ctx, n, pop));
   DNAB_ALLOC_CHECK(ctx, pop->scores64.ensure(nc));
   DNAB_ALLOC_CHECK(ctx, pop->keep_idx.ensure(keep));
   DNAB_ALLOC_CHECK(ctx, pop->keep_e.ensure(keep));
   DNAB_ALLOC_CHECK(ctx, pop->topk_idx.ensure(k));
   DNAB_ALLOC_CHECK(ctx, pop->topk_e.ensure(k));
-  hipLaunchKernelGGL(k_f32_to_f64, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream, pop->scores.p,
-                     pop->scores64.p, nc);
-  TopkPlan pk, pe;
-  DNAB_RETURN_IF(topk_plan(ctx, pk, nc, keep));
-  DNAB_RETURN_IF(topk_plan(ctx, pe, nc, k));
-  topk_launch(ctx, pk, pop->scores64.p, pop->keep_e.p, pop->keep_idx.p);
-  hipLaunchKernelGGL(k_mask_clear, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream,
-                     pop->cand_mask.p, nc);
-  hipLaunchKernelGGL(k_mask_set, dim3(nb(keep, 256)), dim3(256), 0, ctx->stream,
-                     pop->cand_mask.p, pop->keep_idx.p, keep);
-  pop->masked = true;
-  int rc = population_sort(ctx, pop);
-  if (!rc) rc = population_evaluate(ctx, pop, 0);
-  if (!rc) topk_launch(ctx, pe, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
-  // leave the population unmasked for later calls
-  hipLaunchKernelGGL(k_fill_mask_ones, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream,
-                     pop->cand_mask.p, nc);
-  pop->masked = false;
-  pop->sorted = false;
+  // the n_keep best predictions (in index order)
+  DNAB_RETURN_IF(select_smallest(ctx, pop->scores.p, nc, keep, pop->keep_idx.p, pop->sel_scratch));
+  TopkPlan pe;
+  DNAB_RETURN_IF(topk_plan(ctx, pe, keep, k));
+  // their exact energies (the fused kernel's final-state evaluation, one
+  // workgroup per kept candidate: the same sums as a full evaluation), then
+  // the top_k of them, ties by candidate index
+  DNAB_RETURN_IF(trajectory_eval_subset(ctx, pop, pop->keep_idx.p, keep, pop->scores64.p));
+  hipLaunchKernelGGL(k_gather_energy, dim3(nb(keep, 256)), dim3(256), 0, ctx->stream,
+                     pop->energy.p, pop->keep_idx.p, keep, pop->keep_e.p);
+  topk_launch(ctx, pe, pop->keep_e.p, pop->topk_e.p, pop->topk_idx.p, nullptr, 1, 0,
+              pop->keep_idx.p);
+  // the other candidates' outputs were not refreshed
   pop->evaluated = false;
-  if (rc) return rc;
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   if (idx) DNAB_RETURN_IF(download(ctx, idx, pop->topk_idx.p, k * 4));
   if (energy) DNAB_RETURN_IF(download(ctx, energy, pop->topk_e.p, k * 8));
@@ -770,6 +814,24 @@ int dnab_exact_topk(dnab_ctx* ctx, dnab_population* pop, uint32_t top_k, uint32_
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   if (idx) DNAB_RETURN_IF(download(ctx, idx, pop->topk_idx.p, k * 4));
   if (energy) DNAB_RETURN_IF(download(ctx, energy, pop->topk_e.p, k * 8));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNAB_OK;
+}
+
+int dnab_select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m,
+                         uint32_t* idx, uint32_t* n_out) {
+  if (!ctx || (n && !scores) || !idx || m == 0 || m > 2048) return DNAB_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  const uint32_t k = std::min(n, m);
+  if (n_out) *n_out = k;
+  if (n == 0) return DNAB_OK;
+  DevBuf<float> d_s;
+  DevBuf<uint32_t> d_i, scratch;
+  DNAB_ALLOC_CHECK(ctx, d_s.alloc(n));
+  DNAB_ALLOC_CHECK(ctx, d_i.alloc(k));
+  DNAB_RETURN_IF(upload(ctx, d_s.p, scores, static_cast<size_t>(n) * sizeof(float)));
+  DNAB_RETURN_IF(select_smallest(ctx, d_s.p, n, m, d_i.p, scratch));
+  DNAB_RETURN_IF(download(ctx, idx, d_i.p, k * sizeof(uint32_t)));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DNAB_OK;
 }

@@ -25,7 +25,8 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
   return f;
 }
 
-__device__ inline float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// logistic function; v_rcp_f32 (1 ulp) rather than a correctly rounded division
+__device__ inline float sigmoidf_(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 enum GemmEpilogue : int {
   kEpiStoreF32 = 0,  // C32 = acc

@@ -263,9 +263,9 @@ int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k, uint32_t rows
 // final_e / final_idx + (out_row + r) * k, with out_row as in k_topk_chunk.
 void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* final_e,
                  uint32_t* final_idx, const uint32_t* counter, uint32_t row_div,
-                 uint32_t row_fixed) {
+                 uint32_t row_fixed, const uint32_t* first_idx) {
   const double* in_e = energy;
-  const uint32_t* in_i = nullptr;
+  const uint32_t* in_i = first_idx;
   for (size_t p = 0; p < pl.lens.size(); ++p) {
     const uint32_t len = pl.lens[p];
     const uint32_t chunks = nblk(len, kTopkChunk);
@@ -279,6 +279,172 @@ void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* fina
     in_e = oe;
     in_i = oi;
   }
+}
+
+// ---- selection of the m smallest scores (unordered, in index order) -----------
+//
+// A block of 1024 threads holds up to 1024 * KPT entries (thread t the
+// consecutive slots t * KPT ...). It finds the m-th smallest key by a
+// bitwise binary search with block-wide counts (T grows bit by bit while
+// fewer than m keys lie below it), then writes, in slot order, every entry
+// with a key below T and the first ties with T until m are taken. Keys are
+// the scores' order-preserving integer images (NaN last). Stage inputs are
+// scores (slot = index) or the previous stage's per-block lists (block b's
+// entries at b * m_in, count cnt_in[b]).
+constexpr uint32_t kSelBlock = 1024;
+constexpr uint32_t kSelNone = 0xFFFFFFFFu;
+
+__device__ inline uint32_t score_key(float f) {
+  if (f != f) return kSelNone;
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Block-wide sum (all threads call; result in every thread).
+__device__ inline uint32_t sel_block_sum(uint32_t v, uint32_t* s_w) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  v = wave_sum_u32(v);
+  __syncthreads();
+  if (lane == 0) s_w[wave] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (uint32_t w = 0; w < kSelBlock / kWave; ++w) t += s_w[w];
+  return t;
+}
+
+// Block-wide exclusive scan (all threads call); *total = the sum.
+__device__ inline uint32_t sel_block_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(incl, o, 64);
+    if (lane >= static_cast<uint32_t>(o)) incl += u;
+  }
+  __syncthreads();
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < kSelBlock / kWave; ++w) {
+    before += w < wave ? s_w[w] : 0u;
+    all += s_w[w];
+  }
+  *total = all;
+  return before + incl - v;
+}
+
+template <int KPT, bool FROM_SCORES>
+__global__ void __launch_bounds__(kSelBlock)
+k_select_smallest(const float* __restrict__ scores, const uint32_t* __restrict__ keys_in,
+                  const uint32_t* __restrict__ idx_in, const uint32_t* __restrict__ cnt_in,
+                  uint32_t m_in, uint32_t n_slots, uint32_t m, uint32_t* __restrict__ keys_out,
+                  uint32_t* __restrict__ idx_out, uint32_t* __restrict__ cnt_out) {
+  __shared__ uint32_t s_w[kSelBlock / kWave];
+  const uint32_t t = threadIdx.x;
+  const uint32_t base = blockIdx.x * kSelBlock * KPT + t * KPT;
+  uint32_t key[KPT], idx[KPT];
+  uint32_t nv = 0;
+#pragma unroll
+  for (int q = 0; q < KPT; ++q) {
+    const uint32_t sl = base + q;
+    key[q] = kSelNone;
+    idx[q] = kSelNone;
+    if (sl < n_slots) {
+      if (FROM_SCORES) {
+        key[q] = score_key(scores[sl]);
+        idx[q] = sl;
+      } else {
+        const uint32_t b = sl / m_in, j = sl - b * m_in;
+        if (j < cnt_in[b]) {
+          key[q] = keys_in[sl];
+          idx[q] = idx_in[sl];
+        }
+      }
+    }
+    nv += idx[q] != kSelNone ? 1u : 0u;
+  }
+  const uint32_t mm = min(m, sel_block_sum(nv, s_w));
+  if (mm > 0) {
+    uint32_t T = 0;
+    for (int b = 31; b >= 0; --b) {
+      const uint32_t cand = T | (1u << b);
+      uint32_t c = 0;
+#pragma unroll
+      for (int q = 0; q < KPT; ++q) c += (idx[q] != kSelNone && key[q] < cand) ? 1u : 0u;
+      if (sel_block_sum(c, s_w) < mm) T = cand;
+    }
+    // T is the mm-th smallest key: all below it, then ties in slot order
+    uint32_t nlt = 0, neq = 0;
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+      nlt += (idx[q] != kSelNone && key[q] < T) ? 1u : 0u;
+      neq += (idx[q] != kSelNone && key[q] == T) ? 1u : 0u;
+    }
+    uint32_t lt_all;
+    (void)sel_block_scan(nlt, s_w, &lt_all);
+    const uint32_t need = mm - lt_all;
+    uint32_t eq_all;
+    uint32_t eq_rank = sel_block_scan(neq, s_w, &eq_all);
+    uint32_t ntake = 0;
+    bool take[KPT];
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+      const bool v = idx[q] != kSelNone;
+      const bool eq = v && key[q] == T;
+      take[q] = (v && key[q] < T) || (eq && eq_rank < need);
+      eq_rank += eq ? 1u : 0u;
+      ntake += take[q] ? 1u : 0u;
+    }
+    uint32_t all;
+    uint32_t pos = sel_block_scan(ntake, s_w, &all);
+    const size_t out0 = static_cast<size_t>(blockIdx.x) * m;
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+      if (take[q]) {
+        if (keys_out) keys_out[out0 + pos] = key[q];
+        idx_out[out0 + pos] = idx[q];
+        ++pos;
+      }
+    }
+  }
+  if (t == 0 && cnt_out) cnt_out[blockIdx.x] = mm;
+}
+
+// The min(m, n) smallest of n scores (ties by index) into out_idx, in index
+// order. 1 <= m <= 2048.
+int select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m, uint32_t* out_idx,
+                    DevBuf<uint32_t>& scratch) {
+  if (m == 0 || m > 2048) return dnab_set_error(ctx, DNAB_ERR_INVALID, "select: m must be 1..2048");
+  if (n == 0) return DNAB_OK;
+  constexpr uint32_t kChunk = kSelBlock * 4;       // stage inputs per block
+  constexpr uint32_t kLast = kSelBlock * 32;       // the last stage's inputs (one block)
+  const uint32_t nb1 = (n + kChunk - 1) / kChunk;
+  if (n <= kLast && nb1 == 1) {
+    hipLaunchKernelGGL((k_select_smallest<4, true>), dim3(1), dim3(kSelBlock), 0, ctx->stream, scores,
+                       nullptr, nullptr, nullptr, 1u, n, m, nullptr, out_idx, nullptr);
+    DNAB_HIP_CHECK(ctx, hipGetLastError());
+    return DNAB_OK;
+  }
+  // scratch: two ping-pong lists of (keys, idx, counts)
+  const size_t cap = static_cast<size_t>(nb1) * m;
+  DNAB_ALLOC_CHECK(ctx, scratch.ensure(4 * cap + 2 * nb1 + 2));
+  uint32_t* kbuf[2] = {scratch.p, scratch.p + cap};
+  uint32_t* ibuf[2] = {scratch.p + 2 * cap, scratch.p + 3 * cap};
+  uint32_t* cbuf[2] = {scratch.p + 4 * cap, scratch.p + 4 * cap + nb1 + 1};
+  hipLaunchKernelGGL((k_select_smallest<4, true>), dim3(nb1), dim3(kSelBlock), 0, ctx->stream, scores,
+                     nullptr, nullptr, nullptr, 1u, n, m, kbuf[0], ibuf[0], cbuf[0]);
+  uint32_t slots = nb1 * m, cur = 0;
+  while (slots > kLast) {  // more chunked stages (each at least halves the slots)
+    const uint32_t nb = (slots + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL((k_select_smallest<4, false>), dim3(nb), dim3(kSelBlock), 0, ctx->stream,
+                       nullptr, kbuf[cur], ibuf[cur], cbuf[cur], m, slots, m, kbuf[cur ^ 1],
+                       ibuf[cur ^ 1], cbuf[cur ^ 1]);
+    cur ^= 1;
+    slots = nb * m;
+  }
+  hipLaunchKernelGGL((k_select_smallest<32, false>), dim3(1), dim3(kSelBlock), 0, ctx->stream,
+                     nullptr, kbuf[cur], ibuf[cur], cbuf[cur], m, slots, m, nullptr, out_idx, nullptr);
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  return DNAB_OK;
 }
 
 static StepParams make_step_params(const dnab_langevin_params* p) {

@@ -1314,6 +1314,75 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
   return DNAB_OK;
 }
 
+// Group table of one candidate per group from a device index list.
+__global__ void k_subset_groups(const uint32_t* __restrict__ idx, uint32_t n,
+                                uint32_t* __restrict__ off, uint32_t* __restrict__ cand) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) off[i] = i;
+  if (i < n) cand[i] = idx[i];
+}
+
+int trajectory_eval_subset(dnab_ctx* ctx, dnab_population* pop, const uint32_t* d_idx, uint32_t n,
+                           double* score_scratch) {
+  if (n == 0) return DNAB_OK;
+  for (const AgView& v : pop->views_host)
+    if (v.n_atoms >= kTrajMaxAgAtoms)
+      return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "subset evaluation: antigen too large");
+  TrajPlan pl;
+  pl.waves = 4;
+  pl.cpw = 1;
+  pl.a_cap = std::max<uint32_t>(kWave, (pop->max_cand_atoms + kWave - 1) & ~(kWave - 1));
+  pl.r_cap = std::max<uint32_t>(1, pop->max_cand_res);
+  pl.lds = traj_lds_bytes(pl.a_cap, pl.r_cap);
+  if (pl.a_cap / kWave > kTrajMaxTiles || pl.r_cap > 0xFFFFu ||
+      pl.lds + traj_static_lds(4) > kTrajLdsLimit)
+    return dnab_set_error(ctx, DNAB_ERR_UNSUPPORTED, "subset evaluation: candidate too large");
+  DNAB_ALLOC_CHECK(ctx, pop->traj_group_off.ensure(n + 1));
+  DNAB_ALLOC_CHECK(ctx, pop->traj_group_cand.ensure(n));
+  hipLaunchKernelGGL(k_subset_groups, dim3((n + 256) / 256), dim3(256), 0, ctx->stream, d_idx, n,
+                     pop->traj_group_off.p, pop->traj_group_cand.p);
+  TrajArgs a{};
+  a.views = pop->views.p;
+  a.cand_ag = pop->cand_ag.p;
+  a.cand_atom_off = pop->cand_atom_off.p;
+  a.cand_res_off = pop->cand_res_off.p;
+  a.res_atom_off = pop->res_atom_off.p;
+  a.res_ca = pop->res_ca.p;
+  a.atom_res = pop->atom_res.p;
+  a.pos = pop->pos.p;
+  a.par = pop->par.p;
+  a.force = pop->force.p;
+  a.e_atom = pop->e_atom.p;
+  a.energy = pop->energy.p;
+  a.err = pop->err_bound.p;
+  a.sigma_table = nullptr;  // no steps
+  a.stats = nullptr;
+  a.score_e = score_scratch;  // row 0 (score_every 0)
+  a.group_off = pop->traj_group_off.p;
+  a.group_cand = pop->traj_group_cand.p;
+  a.work = nullptr;
+  a.n_groups = n;
+  a.cells = static_cast<int32_t>(pop->traj_cells);
+  a.chunks = pop->traj_chunks;
+  a.n_cand = static_cast<uint32_t>(pop->n_cand);
+  a.n_steps = 0;
+  a.score_every = 0;
+  a.s0 = 0;
+  a.n_total = 0;
+  a.final_eval = 1;
+  a.cpw = 1;
+  a.a_cap = pl.a_cap;
+  a.r_cap = pl.r_cap;
+  a.packed = 0;
+  hipEvent_t ev_end;
+  hipEvent_t ev = timer_begin(ctx, "trajectory_subset", &ev_end);
+  hipError_t e = launch_plan(a, pl, false, true, ctx->stream);
+  timer_end(ctx, "trajectory_subset", ev, ev_end);
+  if (e != hipSuccess)
+    return dnab_set_error(ctx, DNAB_ERR_HIP, "subset evaluation launch failed: %s", hipGetErrorString(e));
+  return DNAB_OK;
+}
+
 }  // namespace dnab
 
 using namespace dnab;

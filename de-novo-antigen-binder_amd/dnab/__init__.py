@@ -9,7 +9,7 @@ from ._native import DNAB_STATUS_NAN as STATUS_NAN
 from ._native import DNAB_STATUS_SATURATED as STATUS_SATURATED
 from ._native import DnabError, device_count, visible_gpu
 from .api import (AtomEnergyBreakdown, Dataset, DeviceAntigen, DevicePopulation, GpuContext,
-                  Mlp, SpatialHashGrid, exact_topk, langevin_params, population_features,
+                  Mlp, SpatialHashGrid, exact_topk, langevin_params, map_energy, population_features, select_smallest,
                   precision_study, run_allatom)
 from .structure import (AMINO_ONE_LETTER, AtomCloud, AtomProtein, PopulationHost, ResidueCloud,
                         aa_from_sequence, parse_peptide, population_from_proteins,
@@ -17,7 +17,7 @@ from .structure import (AMINO_ONE_LETTER, AtomCloud, AtomProtein, PopulationHost
 
 __all__ = [
     "AMINO_ONE_LETTER", "AtomCloud", "AtomEnergyBreakdown", "AtomProtein", "Dataset",
-    "DeviceAntigen", "Mlp", "exact_topk", "population_features", "precision_study",
+    "DeviceAntigen", "Mlp", "exact_topk", "map_energy", "population_features", "select_smallest", "precision_study",
     "DevicePopulation", "DnabError", "GpuContext", "PopulationHost", "ResidueCloud",
     "STATUS_NAN", "STATUS_SATURATED",
     "SpatialHashGrid", "aa_from_sequence", "device_count", "langevin_params", "parse_peptide",

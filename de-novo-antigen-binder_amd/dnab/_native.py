@@ -144,7 +144,8 @@ class TrainParamsC(C.Structure):
                 ("seed", C.c_uint64)]
 
 
-FEATURE_DIM = 3360
+FEATURE_DIM = 96
+MAP_VALUES = 17  # per grid point: 7 LJ classes, potential, hydrophobic, 8 radial densities
 MLP_HIDDEN = 256
 
 assert C.sizeof(GpuAtom) == 32
@@ -235,6 +236,12 @@ _PROTOS = {
     "dnab_population_read_pair_stats": (C.c_int, [vp, vp, u64p, u64p]),
     "dnab_population_reset_pair_stats": (C.c_int, [vp, vp]),
     "dnab_population_features": (C.c_int, [vp, vp, C.POINTER(C.c_uint16)]),
+    "dnab_prescreen_feature_count": (C.c_int, [szp]),
+    "dnab_antigen_build_maps": (C.c_int, [vp, vp, C.c_float]),
+    "dnab_antigen_map_info": (C.c_int, [vp, f32p, f32p, C.POINTER(C.c_int32), szp]),
+    "dnab_antigen_map_points": (C.c_int, [vp, vp, C.POINTER(C.c_int32), C.c_size_t, f32p]),
+    "dnab_population_map_energy": (C.c_int, [vp, vp, f32p]),
+    "dnab_select_smallest": (C.c_int, [vp, f32p, C.c_uint32, C.c_uint32, u32p, u32p]),
     "dnab_mlp_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
     "dnab_mlp_destroy": (None, [vp]),
     "dnab_mlp_param_count": (C.c_int, [szp]),

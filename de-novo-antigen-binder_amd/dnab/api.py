@@ -305,6 +305,30 @@ class DeviceAntigen:
         return {"n_atoms": n.value, "n_cells_nonempty": ne.value, "cutoff": rc.value,
                 "cell_size": cs.value}
 
+    def build_maps(self, spacing: float = 0.0):
+        """(Re)build the prescreen's antigen maps (dnab_antigen_build_maps);
+        spacing <= 0 selects the default 0.5 A."""
+        self.ctx.check(N.lib.dnab_antigen_build_maps(self.ctx.handle, self._h, spacing))
+
+    def map_info(self):
+        h = C.c_float()
+        o = np.zeros(3, np.float32)
+        d = np.zeros(3, np.int32)
+        b = C.c_size_t()
+        self.ctx.check(N.lib.dnab_antigen_map_info(self._h, C.byref(h), N.ptr(o, C.c_float),
+                                                   N.ptr(d, C.c_int32), C.byref(b)))
+        return {"spacing": h.value, "origin": o, "dims": d, "bytes": b.value}
+
+    def map_points(self, ijk) -> np.ndarray:
+        """Map values at grid points ijk [n, 3]: [n, 17] (7 LJ classes,
+        potential, hydrophobic, 8 radial densities)."""
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+        out = np.zeros((max(len(ijk), 1), N.MAP_VALUES), np.float32)
+        self.ctx.check(N.lib.dnab_antigen_map_points(self.ctx.handle, self._h,
+                                                     N.ptr(ijk, C.c_int32), len(ijk),
+                                                     N.ptr(out, C.c_float)))
+        return out[:len(ijk)]
+
     def close(self):
         if self._h:
             N.lib.dnab_antigen_destroy(self._h)
@@ -652,7 +676,7 @@ class DevicePopulation:
 
 
 class Mlp:
-    """The prescreen MLP (3360 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights)."""
+    """The prescreen MLP (96 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights)."""
 
     def __init__(self, ctx: GpuContext, seed: int = 0):
         self.ctx = ctx
@@ -798,6 +822,26 @@ def population_features(pop: DevicePopulation) -> np.ndarray:
     out = np.zeros((max(pop.n_cand, 1), N.FEATURE_DIM), np.uint16)
     pop.ctx.check(N.lib.dnab_population_features(pop.ctx.handle, pop.handle,
                                                  N.ptr(out, C.c_uint16)))
+    return out[:pop.n_cand]
+
+
+def select_smallest(ctx: GpuContext, scores, m: int) -> np.ndarray:
+    """Indices of the min(m, n) smallest scores (ties by index, NaN last), in
+    ascending index order (dnab_select_smallest)."""
+    s = np.ascontiguousarray(scores, np.float32)
+    out = np.zeros(max(m, 1), np.uint32)
+    n = C.c_uint32()
+    ctx.check(N.lib.dnab_select_smallest(ctx.handle, N.ptr(s, C.c_float), len(s), m,
+                                         N.ptr(out, C.c_uint32), C.byref(n)))
+    return out[:n.value]
+
+
+def map_energy(pop: DevicePopulation) -> np.ndarray:
+    """Per candidate map energy (dnab_population_map_energy): the ranking
+    score of the screen without the network."""
+    out = np.zeros(max(pop.n_cand, 1), np.float32)
+    pop.ctx.check(N.lib.dnab_population_map_energy(pop.ctx.handle, pop.handle,
+                                                   N.ptr(out, C.c_float)))
     return out[:pop.n_cand]
 
 

@@ -672,25 +672,50 @@ int dnab_population_read_scores(dnab_ctx* ctx, const dnab_population* pop, uint3
 
 /* ---- learned prescreen (no reference counterpart; the reference has no
  * learned components, Cargo.toml:4) -------------------------------------------
- * Features: for every (candidate residue i, antigen residue j) with Calpha
- * distance d < 14 A, exp(-(d - mu_k)^2 / (2 * 0.8^2)), mu_k = 2 + 0.8 k,
- * k = 0..15, summed into bin [pair(aa_i, aa_j)][k] over the 210 unordered
- * residue-type pairs; a row is bf16(log1p(bins)), 3360 values.
- * Model: 3360 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights and activations,
+ * Antigen maps: on a grid of spacing h (default 0.5 A) over the antigen's atom
+ * box widened by cutoff + 2h, the sums the exact scorer forms for a probe atom
+ * at a grid point, over antigen atoms with 0.25 <= r^2 <= cutoff^2: LJ
+ * eps_ij (s^12 - 2 s^6), s = R_ij / r, for each of the 7 distinct (Rmin/2, eps)
+ * classes of the AMBER99SB table; 332 q_i / r; -0.5 per hydrophobic atom with
+ * r^2 < 36; and 8 radial densities exp(-(r - mu_k)^2 / (2 * 0.75^2)),
+ * mu_k = 2 + k A. Stored in fp16 (saturated at +-60000).
+ * Features (96 per candidate, bf16): a candidate atom reads its class's LJ
+ * plane, the potential and the hydrophobic plane by trilinear interpolation
+ * and the radial densities at the nearest grid point; per candidate the sums
+ * per atom class (7 LJ classes, hydrophobic carbons their own) of the LJ map
+ * energies and of the radial densities (asinh(x/2), log1p), the Coulomb sums
+ * over positive and negative atoms and the hydrophobic sum (asinh(x/2)), the
+ * total map energy (asinh(x/10)) and log1p counts of clashing (LJ > 1),
+ * contacting (LJ < -0.05) and in-map atoms.
+ * Model: 96 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights and activations,
  * FP32 accumulation (MFMA), fp32 master weights, Adam. Target: asinh(E / s)
  * with E the exact LJ + Coulomb + hydrophobic energy, s = energy_scale. */
-#define DNAB_FEATURE_DIM 3360
+#define DNAB_FEATURE_DIM 96
 #define DNAB_MLP_HIDDEN 256
+int dnab_prescreen_feature_count(size_t* n);
+/* Builds (or rebuilds) the antigen's maps; spacing <= 0 selects 0.5 A.
+ * Features build missing maps with the default spacing. */
+int dnab_antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing);
+/* spacing 0 and dims 0 when no maps are built; bytes of device memory held. */
+int dnab_antigen_map_info(const dnab_antigen* ag, float* spacing, float origin[3], int32_t dims[3],
+                          size_t* bytes);
+/* Map values at n grid points ijk[3 n] (test hook): out[n][17] = 7 LJ
+ * classes, potential, hydrophobic, 8 radial densities. */
+int dnab_antigen_map_points(dnab_ctx* ctx, const dnab_antigen* ag, const int32_t* ijk, size_t n,
+                            float* out);
+/* Per candidate the map energy (the sum the features' total carries): the
+ * ranking score of the method without the network. */
+int dnab_population_map_energy(dnab_ctx* ctx, dnab_population* pop, float* out);
 typedef struct dnab_mlp dnab_mlp;
 typedef struct dnab_dataset dnab_dataset;
 
-/* Features of every candidate; copied to host_bf16 (n_cand * 3360) when not
- * NULL. Requires the antigens' ca_amino_acid. */
+/* Features of every candidate; copied to host_bf16 (n_cand * 96) when not
+ * NULL. */
 int dnab_population_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* host_bf16);
 int dnab_mlp_create(dnab_ctx* ctx, uint64_t seed, dnab_mlp** out);
 void dnab_mlp_destroy(dnab_mlp* mlp);
-/* Flat parameter vector: W1[3360][256], b1[256], W2[256][256], b2, W3[256][256],
- * b3, w4[256], b4[1] (992,257 floats, fp32 master copy). */
+/* Flat parameter vector: W1[96][256], b1[256], W2[256][256], b2, W3[256][256],
+ * b3, w4[256], b4[1] (156,673 floats, fp32 master copy). */
 int dnab_mlp_param_count(size_t* n);
 int dnab_mlp_get_params(dnab_ctx* ctx, const dnab_mlp* mlp, float* out, size_t n);
 int dnab_mlp_set_params(dnab_ctx* ctx, dnab_mlp* mlp, const float* in, size_t n);
@@ -721,11 +746,17 @@ int dnab_mlp_train(dnab_ctx* ctx, dnab_mlp* mlp, dnab_dataset* ds, const dnab_tr
 /* Predicted target for every candidate (features + forward pass). */
 int dnab_mlp_predict(dnab_ctx* ctx, dnab_mlp* mlp, dnab_population* pop, float* scores);
 /* Screening: features + MLP for every candidate, the n_keep best predictions
- * re-scored by the exact FP32 kernel, the top_k of those by exact energy
- * (ascending, ties by index) written to idx / energy (*n_out entries).
- * 1 <= top_k <= n_keep <= 2048. */
+ * re-scored exactly (one workgroup per kept candidate; the same energies as a
+ * full evaluation), the top_k of those by exact energy (ascending, ties by
+ * index) written to idx / energy (*n_out entries). Other candidates' energy
+ * outputs are left stale. 1 <= top_k <= n_keep <= 2048. */
 int dnab_prescreen(dnab_ctx* ctx, dnab_mlp* mlp, dnab_population* pop, uint32_t n_keep,
                    uint32_t top_k, uint32_t* idx, double* energy, uint32_t* n_out);
+/* The min(m, n) smallest of n host scores (ties by index, NaN last) as
+ * indices in ascending index order: the prescreen's keep selection.
+ * 1 <= m <= 2048. */
+int dnab_select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m,
+                         uint32_t* idx, uint32_t* n_out);
 /* Exact top-k of every candidate, for comparison: evaluate all + top-k. */
 int dnab_exact_topk(dnab_ctx* ctx, dnab_population* pop, uint32_t top_k, uint32_t* idx,
                     double* energy, uint32_t* n_out);

@@ -3,9 +3,19 @@
 TEST INFRASTRUCTURE ONLY (tests/ and bench.py's checks import it; the
 product never does). There is no reference counterpart: the reference has no
 learned component (Cargo.toml:4). What pins this oracle: finite differences
-of its own float64 loss (tests/test_mlp_oracle.py), closed-form checks of
-the feature definition, and bf16 rounding checked against numpy's float32
-bit patterns.
+of its own float64 loss (tests/test_mlp_oracle.py), closed forms of the map
+definition (a single antigen atom: the LJ minimum -eps at r = R_ij, Coulomb
+332 q / r, the hydrophobic step, a Gaussian peak of 1 at r = mu_k), the exact
+all-atom energy that the trilinear map energy converges to at grid points
+(tests/test_mlp_oracle.py), and bf16 rounding checked against numpy's
+float32 bit patterns.
+
+Maps (maps.hip): at grid point p (float32 origin + h * index) the sums over
+antigen atoms i passing the float32 pair predicate 0.25 <= r^2 <= rc^2 (r^2 =
+(dx*dx + dy*dy) + dz*dz in float32) of, per LJ class c, sqrt(eps_c eps_i)
+(s^12 - 2 s^6) with s = (Rmin/2_c + Rmin/2_i) / r; 332 q_i / r; -0.5 for
+hydrophobic i with r^2 < 36; exp(-(r - mu_k)^2 / (2 sigma^2)). Stored as
+fp16 after clamping to +-60000.
 
 Model: x -> SiLU(x W1 + b1) -> SiLU(. W2 + b2) -> SiLU(. W3 + b3) -> . w4 + b4.
 With bf16=True the oracle rounds at the same points as the device path:
@@ -16,9 +26,13 @@ from __future__ import annotations
 
 import numpy as np
 
-D, H = 3360, 256
-RBF, PAIRS = 16, 210
-FEAT_CUTOFF, RBF_START, RBF_STEP, RBF_SIGMA = 14.0, 2.0, 0.8, 0.8
+D, H = 96, 256
+MAP_H = 0.5
+RBF_MU0, RBF_DMU, RBF_SIGMA, N_RBF = 2.0, 1.0, 0.75, 8
+MAP_CLAMP = 60000.0
+CLASH_E, CONTACT_E = 1.0, -0.05
+N_LJ = 7
+F_COUL_POS, F_COUL_NEG, F_HYD, F_TOTAL, F_CLASH, F_CONTACT, F_INMAP, F_RBF = 8, 9, 10, 11, 12, 13, 14, 16
 
 
 def bf16_round(a) -> np.ndarray:
@@ -40,23 +54,142 @@ def bf16_to_f64(bits) -> np.ndarray:
     return (np.asarray(bits, np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-def pair_index(a, b):
-    a, b = np.minimum(a, b), np.maximum(a, b)
-    return a * (41 - a) // 2 + (b - a)
+def lj_classes(amber_nb):
+    """The distinct (Rmin/2, eps) pairs of the AtomType table, in AtomType order."""
+    out = []
+    for rmh, eps in amber_nb:
+        t = (np.float32(rmh), np.float32(eps))
+        if t not in out:
+            out.append(t)
+    return out
 
 
-def features(cand_ca, cand_aa, ag_ca, ag_aa) -> np.ndarray:
-    """RBF distance-histogram of one candidate (float64, before log1p)."""
-    h = np.zeros((PAIRS, RBF))
-    mu = RBF_START + RBF_STEP * np.arange(RBF)
-    c = np.asarray(cand_ca, np.float64)
-    a = np.asarray(ag_ca, np.float64)
-    for i in range(len(c)):
-        d = np.sqrt(((a - c[i]) ** 2).sum(1))
-        for j in np.nonzero(d < FEAT_CUTOFF)[0]:
-            h[pair_index(int(cand_aa[i]), int(ag_aa[j]))] += np.exp(
-                -(d[j] - mu) ** 2 / (2 * RBF_SIGMA ** 2))
-    return h.reshape(-1)
+def map_grid(bbox_lo, bbox_hi, cutoff, h=MAP_H):
+    """Origin (float32) and points per axis of an antigen's map grid."""
+    lo = np.asarray(bbox_lo, np.float32)
+    hi = np.asarray(bbox_hi, np.float32)
+    pad = np.float32(cutoff) + np.float32(2.0) * np.float32(h)
+    origin = (lo - pad).astype(np.float32)
+    n = np.ceil(((hi - lo) + np.float32(2.0) * pad) / np.float32(h)).astype(np.int64) + 1
+    return origin, n
+
+
+def map_values(ag, ijk, origin, h, cutoff, classes, hydrophobic_flag, round16=True):
+    """Map values [n, 17] at grid points ijk [n, 3] (float64 terms, float32
+    predicate, fp16 rounding). ag: dict of float32 arrays x, y, z, charge,
+    r_min_half, epsilon and hydrophobic (0/1)."""
+    ijk = np.asarray(ijk, np.int64).reshape(-1, 3)
+    h32 = np.float32(h)
+    P = (origin[None, :] + h32 * ijk.astype(np.float32)).astype(np.float32)
+    A = np.stack([ag["x"], ag["y"], ag["z"]], 1).astype(np.float32)
+    rc2 = np.float32(cutoff) * np.float32(cutoff)
+    q332 = (np.float32(332.0) * ag["charge"].astype(np.float32)).astype(np.float64)
+    rmh = ag["r_min_half"].astype(np.float64)
+    eps = ag["epsilon"].astype(np.float64)
+    hyd = np.asarray(hydrophobic_flag, bool)
+    out = np.zeros((len(P), 2 + len(classes) + N_RBF))
+    mu = RBF_MU0 + RBF_DMU * np.arange(N_RBF)
+    for q in range(len(P)):
+        d = P[q][None, :] - A
+        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        sel = (r2 <= rc2) & (r2 >= np.float32(0.25))
+        r2d = r2[sel].astype(np.float64)
+        r = np.sqrt(r2d)
+        for c, (rc_, ec) in enumerate(classes):
+            s2 = (float(rc_) + rmh[sel]) ** 2 / r2d
+            s6 = s2 ** 3
+            out[q, c] = (np.sqrt(float(ec) * eps[sel]) * (s6 * s6 - 2.0 * s6)).sum()
+        out[q, len(classes)] = (q332[sel] / r).sum()
+        out[q, len(classes) + 1] = -0.5 * (hyd[sel] & (r2[sel] < np.float32(36.0))).sum()
+        out[q, len(classes) + 2:] = np.exp(-(r[:, None] - mu[None, :]) ** 2
+                                            / (2 * RBF_SIGMA ** 2)).sum(0)
+    return fp16_round(out) if round16 else out
+
+
+def fp16_round(v):
+    return np.clip(np.asarray(v, np.float64), -MAP_CLAMP, MAP_CLAMP).astype(np.float16).astype(np.float64)
+
+
+def atom_class(rmh, eps, classes):
+    """LJ class of an atom: its (Rmin/2, eps) pair, else the nearest one."""
+    for c, (r, e) in enumerate(classes):
+        if np.float32(rmh) == r and np.float32(eps) == e:
+            return c
+    d = [abs(float(np.float32(rmh)) - float(r)) + abs(float(np.float32(eps)) - float(e))
+         for r, e in classes]
+    return int(np.argmin(d))
+
+
+def atom_map_lookup(x, y, z, origin, h, dims):
+    """Cell (i, j, k), fractions (float32 arithmetic as the kernel) and the
+    nearest grid point of one atom; None outside the map."""
+    inv = np.float32(1.0) / np.float32(h)
+    u = ((np.array([x, y, z], np.float32) - origin) * inv).astype(np.float32)
+    f = np.floor(u)
+    if not (np.all(f >= 0) and np.all(f < (np.asarray(dims) - 1))):
+        return None
+    frac = (u - f).astype(np.float32)
+    i = f.astype(np.int64)
+    near = i + (frac >= np.float32(0.5))
+    return i, frac.astype(np.float64), near
+
+
+def trilinear(corners, frac):
+    """corners[8] (x fastest) at fractions frac (the kernel's formula)."""
+    fx, fy, fz = frac
+    gx, gy, gz = 1 - fx, 1 - fy, 1 - fz
+    v = corners
+    x00 = gx * v[0] + fx * v[1]
+    x10 = gx * v[2] + fx * v[3]
+    x01 = gx * v[4] + fx * v[5]
+    x11 = gx * v[6] + fx * v[7]
+    return gz * (gy * x00 + fy * x10) + fz * (gy * x01 + fy * x11)
+
+
+CORNERS = np.array([[c & 1, (c >> 1) & 1, c >> 2] for c in range(8)])
+
+
+def features(atoms, origin, h, dims, lookup_values, classes):
+    """Features of one candidate (float64 before bf16). atoms: dict of the
+    candidate's x, y, z, charge, r_min_half, epsilon, hydrophobic; the map
+    grid (origin, spacing, points per axis); lookup_values(ijk [m, 3]) ->
+    [m, 17] map values at grid points. An atom outside the map adds nothing.
+    Returns (row[96] transformed, total map energy)."""
+    raw = np.zeros(D)
+    n = len(atoms["x"])
+    for a in range(n):
+        lk = atom_map_lookup(atoms["x"][a], atoms["y"][a], atoms["z"][a], origin, h, dims)
+        if lk is None:
+            continue
+        i, frac, near = lk
+        lc = atom_class(atoms["r_min_half"][a], atoms["epsilon"][a], classes)
+        hydp = atoms["hydrophobic"][a] != 0
+        fc = N_LJ if hydp else lc
+        vals = lookup_values(i[None, :] + CORNERS)
+        elj = trilinear(vals[:, lc], frac)
+        phi = trilinear(vals[:, N_LJ], frac)
+        hh = trilinear(vals[:, N_LJ + 1], frac)
+        rb = lookup_values(near[None, :])[0, N_LJ + 2:]
+        q = float(atoms["charge"][a])
+        ec = q * phi
+        eh = hh if hydp else 0.0
+        raw[fc] += elj
+        if q > 0:
+            raw[F_COUL_POS] += ec
+        elif q < 0:
+            raw[F_COUL_NEG] += ec
+        raw[F_HYD] += eh
+        raw[F_TOTAL] += elj + ec + eh
+        raw[F_CLASH] += elj > CLASH_E
+        raw[F_CONTACT] += elj < CONTACT_E
+        raw[F_INMAP] += 1
+        raw[F_RBF + fc * N_RBF:F_RBF + (fc + 1) * N_RBF] += rb
+    row = np.zeros(D)
+    row[:F_HYD + 1] = np.arcsinh(raw[:F_HYD + 1] * 0.5)
+    row[F_TOTAL] = np.arcsinh(raw[F_TOTAL] * 0.1)
+    row[F_CLASH:F_INMAP + 1] = np.log1p(raw[F_CLASH:F_INMAP + 1])
+    row[F_RBF:F_RBF + (N_LJ + 1) * N_RBF] = np.log1p(raw[F_RBF:F_RBF + (N_LJ + 1) * N_RBF])
+    return row, raw[F_TOTAL]
 
 
 def unpack(p, d=D, h=H):
