@@ -741,15 +741,18 @@ def prescreen_bench(ctx, args):
             best = w.pop.n_cand / t_p
             out["speedup_vs_exact_at_95pct_recall"] = t_exact / t_p
     out["screened_cand_per_s_at_95pct_recall"] = best
-    # the fused MLP kernel against the bf16 MFMA peak (2 M (96 256 + 2 256^2 + 256) FLOP)
+    # per-stage kernel times of one screen (keep 512, HIP events), and the
+    # fused MLP kernel against the bf16 MFMA peak (2 M (32 256 + 2 256^2 + 256) FLOP)
+    names = ("map_features", "mlp_infer", "trajectory_subset")
     ctx.set_timing(True)
-    k0 = ctx.kernel_time("mlp_infer")
-    mlp.predict(pop)
-    k1 = ctx.kernel_time("mlp_infer")
+    k0 = {n: ctx.kernel_time(n) for n in names}
+    mlp.prescreen(pop, 512, 64)
+    k1 = {n: ctx.kernel_time(n) for n in names}
     ctx.set_timing(False)
-    if k1[1] > k0[1]:
-        ms = (k1[0] - k0[0]) / (k1[1] - k0[1])
-        flop = 2.0 * w.pop.n_cand * (96 * 256 + 2 * 256 * 256 + 256)
+    out["stage_kernel_ms"] = {n: k1[n][0] - k0[n][0] for n in names}
+    ms = out["stage_kernel_ms"]["mlp_infer"]
+    if ms > 0:
+        flop = 2.0 * w.pop.n_cand * (32 * 256 + 2 * 256 * 256 + 256)
         out["mlp_infer"] = {"kernel_ms": ms, "tflops": flop / (ms * 1e-3) / 1e12,
                             "frac_bf16_mfma_peak": flop / (ms * 1e-3) / 2.5e15}
     pop.close()

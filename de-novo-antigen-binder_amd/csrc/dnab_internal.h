@@ -183,29 +183,43 @@ struct AgView {
 // grid of spacing h over the antigen's atom box widened by the cutoff and two
 // spacings. Energy planes (fp16): the LJ energy of a probe atom of each of the
 // kMapLjClasses parameter classes, the Coulomb potential sum 332 q_i / r and
-// the hydrophobic sum (-0.5 per hydrophobic antigen atom within 6 A), each
-// stored per grid cell as its 8 corner values (one 16-byte record, corner
-// order x fastest) so that trilinear interpolation is one load per plane;
-// kMapRbf Gaussian radial densities per grid point (one 16-byte record).
+// the hydrophobic sum (-0.5 per hydrophobic antigen atom within 6 A), kept per
+// grid cell as one record of kMapRecSlots 16-byte slots, each the 8 corner
+// values of one plane (x fastest): potential, hydrophobic, LJ classes 0..5;
+// the rare sulfur class (6) in a plane of its own. An atom reads 3 slots; for
+// the two carbon classes they share the record's first 64 bytes. kMapRbf
+// Gaussian radial densities on the grid of spacing 2h (every second point),
+// one 16-byte record per point.
 constexpr int kMapLjClasses = 7;
 constexpr int kMapPlanes = kMapLjClasses + 2;
+constexpr int kMapRecSlots = 8;    // 128 bytes per cell
 constexpr int kMapRbf = 8;
+constexpr int kMapSulfur = 6;      // the LJ class kept outside the record
+// the record slot of plane p (planes: LJ classes 0..6, potential,
+// hydrophobic); -1 for the sulfur class
+__host__ __device__ constexpr int map_slot(int p) {
+  return p == kMapSulfur ? -1 : (p < kMapLjClasses ? 2 + p : p - kMapLjClasses);
+}
 struct MapDev {
   float h = 0.0f, inv_h = 0.0f;
   float origin[3] = {0, 0, 0};
   int32_t n[3] = {0, 0, 0};    // grid points per axis (cells: n - 1)
-  size_t npts = 0, ncells = 0;
-  DevBuf<uint16_t> erec;       // [plane][cell][8] fp16
-  DevBuf<uint16_t> rbf;        // [point][8] fp16
+  int32_t nc[3] = {0, 0, 0};   // points of the radial-density grid (every second point)
+  size_t npts = 0, ncells = 0, ncpts = 0;
+  DevBuf<uint16_t> erec;       // [cell][kMapRecSlots][8] fp16
+  DevBuf<uint16_t> esul;       // [cell][8] fp16: the sulfur class's LJ
+  DevBuf<uint16_t> rbf;        // [coarse point][8] fp16
 };
 struct MapView {
   const uint16_t* erec;
+  const uint16_t* esul;
   const uint16_t* rbf;
   float origin[3];
   float inv_h;
   int32_t n[3];
   uint32_t valid;
-  unsigned long long ncells;
+  int32_t nc[3];
+  uint32_t pad;
 };
 
 struct KernelTimer {
@@ -421,7 +435,7 @@ bool atom_params_ok(const dnab_atom_cloud* c);
 // noalloc.hip: the no-alloc entry's scratch address on `device` (current)
 int noalloc_init_device(int device);
 // maps.hip: the prescreen's antigen maps and per-candidate features
-constexpr int kPrescreenFeatures = 96;
+constexpr int kPrescreenFeatures = 32;
 int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing);
 // features (bf16 rows of kPrescreenFeatures, nullable) and the map energy of
 // every candidate (nullable); builds missing antigen maps first

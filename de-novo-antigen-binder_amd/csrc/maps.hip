@@ -12,17 +12,22 @@
 //   - the Coulomb potential sum 332 q_i / r (times the probe's charge),
 //   - the hydrophobic sum -0.5 per hydrophobic antigen atom within 6 A (for a
 //     hydrophobic probe),
-//   - kMapRbf Gaussian radial densities sum exp(-(r - mu_k)^2 / (2 sigma^2)).
+//   - kMapRbf Gaussian radial densities sum exp(-(r - mu_k)^2 / (2 sigma^2))
+//     (on every second grid point).
 // A candidate atom's map energy is the trilinear interpolation of its class's
 // LJ plane plus q times the potential plus (hydrophobic) the hydrophobic
 // plane; summed over the candidate it approximates the exact energy except
-// where the repulsive wall is steeper than the grid resolves.
+// where the repulsive wall is steeper than the grid resolves. The planes of a
+// grid cell share one record, so an atom's three reads touch one or two cache
+// lines.
 //
-// Features of a candidate (kPrescreenFeatures, bf16): per atom class, the sum
-// of its atoms' LJ map energies and of their radial densities (atom-class x
-// RBF distance histograms against the antigen's heavy atoms), the Coulomb sums
-// over positive and negative atoms, the hydrophobic sum, the total map
-// energy, and counts of clashing, contacting and in-map atoms.
+// Features of a candidate (kPrescreenFeatures = 32, bf16): the LJ map energy
+// summed per atom class (the 7 LJ parameter classes, hydrophobic atoms a class
+// of their own), the Coulomb sums over positive and negative atoms, the
+// hydrophobic sum, the total map energy, counts of clashing, contacting and
+// in-map atoms, and the radial densities summed over polar (N, O) and over
+// the other atoms: atom-class x RBF distance histograms against the antigen's
+// heavy atoms.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,20 +43,24 @@
 
 namespace dnab {
 
-constexpr float kMapDefaultSpacing = 0.5f;
+constexpr float kMapDefaultSpacing = 0.625f;
 constexpr float kRbfMu0 = 2.0f, kRbfDmu = 1.0f, kRbfSigma = 0.75f;
 constexpr float kMapClamp = 60000.0f;   // fp16 range; a wall value beyond it is saturated
 constexpr size_t kMapMaxBytes = size_t(6) << 30;
-constexpr int kFeatClasses = kMapLjClasses + 1;  // LJ classes + hydrophobic carbon
+constexpr int kFeatClasses = kMapLjClasses + 1;  // LJ classes + hydrophobic atoms
 constexpr float kClashE = 1.0f;     // an atom's LJ map energy above this is a clash
 constexpr float kContactE = -0.05f; // ... below this, a contact
 
-// Feature layout (index ranges)
+// Feature layout
 constexpr int kFLj = 0;            // [kFeatClasses] LJ map energy per atom class
 constexpr int kFCoulPos = 8, kFCoulNeg = 9, kFHyd = 10, kFTotal = 11;
 constexpr int kFClash = 12, kFContact = 13, kFInMap = 14;
-constexpr int kFRbf = 16;          // [kFeatClasses][kMapRbf]
-static_assert(kFRbf + kFeatClasses * kMapRbf <= kPrescreenFeatures, "feature layout");
+constexpr int kFRbfPolar = 16;     // [kMapRbf] radial densities at N and O atoms
+constexpr int kFRbfApolar = 24;    // [kMapRbf] ... at the other atoms
+static_assert(kFRbfApolar + kMapRbf == kPrescreenFeatures, "feature layout");
+
+// N* (class 2), O, O2 (3), OH (4), OS (5) in the class order of map_classes
+__device__ inline bool polar_class(int fc) { return fc >= 2 && fc <= 5; }
 
 struct MapClasses {
   float rmh[kMapLjClasses];
@@ -96,7 +105,8 @@ struct MapBuildArgs {
   int32_t n[3];
   MapClasses cls;
   uint16_t* pts;  // [plane][point] fp16
-  uint16_t* rbf;  // [point][kMapRbf] fp16
+  uint16_t* rbf;  // [coarse point][kMapRbf] fp16
+  int32_t nc[3];  // coarse points per axis (every second point)
   unsigned long long npts;
 };
 
@@ -213,31 +223,38 @@ __global__ void __launch_bounds__(256) k_map_points(MapBuildArgs a) {
   for (int k = 0; k < kMapLjClasses; ++k) a.pts[k * a.npts + pt] = to_half_bits(lj[k]);
   a.pts[kMapLjClasses * a.npts + pt] = to_half_bits(phi);
   a.pts[(kMapLjClasses + 1) * a.npts + pt] = to_half_bits(hyd);
+  if ((ix | iy | iz) & 1) return;  // radial densities on every second point
   uint16_t r16[kMapRbf];
 #pragma unroll
   for (int k = 0; k < kMapRbf; ++k) r16[k] = to_half_bits(rb[k]);
-  *reinterpret_cast<uint4*>(a.rbf + pt * kMapRbf) = *reinterpret_cast<const uint4*>(r16);
+  const size_t cpt = (static_cast<size_t>(iz >> 1) * a.nc[1] + (iy >> 1)) * a.nc[0] + (ix >> 1);
+  *reinterpret_cast<uint4*>(a.rbf + cpt * kMapRbf) = *reinterpret_cast<const uint4*>(r16);
 }
 
-// Cell records: for every plane and cell, its 8 corner values (x fastest).
+// Cell records: for every cell and plane, the plane's 8 corner values (x
+// fastest), in the plane's slot of the cell's record or, for the sulfur
+// class, in its own plane.
 __global__ void k_map_expand(const uint16_t* __restrict__ pts, unsigned long long npts, int32_t n0,
                              int32_t n1, int32_t n2, unsigned long long ncells,
-                             uint16_t* __restrict__ erec) {
+                             uint16_t* __restrict__ erec, uint16_t* __restrict__ esul) {
   const unsigned long long t = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x;
   if (t >= ncells * kMapPlanes) return;
-  const unsigned long long p = t / ncells, cell = t - p * ncells;
+  const unsigned long long cell = t / kMapPlanes;
+  const int p = static_cast<int>(t - cell * kMapPlanes);
   const int32_t cx = n0 - 1, cy = n1 - 1;
   const int32_t i = static_cast<int32_t>(cell % cx);
   const int32_t j = static_cast<int32_t>((cell / cx) % cy);
   const int32_t k = static_cast<int32_t>(cell / (static_cast<unsigned long long>(cx) * cy));
-  const uint16_t* src = pts + p * npts;
+  const uint16_t* src = pts + static_cast<size_t>(p) * npts;
   uint16_t v[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const size_t q = (static_cast<size_t>(k + (c >> 2)) * n1 + (j + ((c >> 1) & 1))) * n0 + (i + (c & 1));
     v[c] = src[q];
   }
-  *reinterpret_cast<uint4*>(erec + t * 8) = *reinterpret_cast<const uint4*>(v);
+  const int slot = map_slot(p);
+  uint16_t* dst = slot < 0 ? esul + cell * 8 : erec + (cell * kMapRecSlots + slot) * 8;
+  *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(v);
 }
 
 struct FeatArgs {
@@ -252,8 +269,14 @@ struct FeatArgs {
   float* map_e;    // [n_cand], nullable
 };
 
-__device__ inline float trilerp(const uint16_t* rec, float fx, float fy, float fz) {
-  const uint4 w = *reinterpret_cast<const uint4*>(rec);
+// The nearest point of the coarse grid (spacing 2h) to fine coordinate u >= 0.
+__device__ inline int32_t coarse_nearest(float u, int32_t nc) {
+  const float uc = u * 0.5f;
+  const float f = floorf(uc);
+  return min(static_cast<int32_t>(f) + (uc - f >= 0.5f ? 1 : 0), nc - 1);
+}
+
+__device__ inline float trilerp(uint4 w, float fx, float fy, float fz) {
   const uint16_t* v = reinterpret_cast<const uint16_t*>(&w);
   const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
   const float x00 = gx * half_bits_to_f32(v[0]) + fx * half_bits_to_f32(v[1]);
@@ -263,29 +286,26 @@ __device__ inline float trilerp(const uint16_t* rec, float fx, float fy, float f
   return gz * (gy * x00 + fy * x10) + fz * (gy * x01 + fy * x11);
 }
 
-// Per-lane sums: per feature class its LJ map energy and radial densities
-// (kFeatClasses x (1 + kMapRbf)), then the Coulomb sums over positive and
-// negative atoms, the hydrophobic sum, the total map energy and the clash,
-// contact and in-map counts.
-constexpr int kClassCols = kFeatClasses * (1 + kMapRbf);  // 72
-constexpr int kSumCols = kClassCols + 7;                   // 79
-constexpr int kTrCols = 40;                                // columns per transpose round
-constexpr int kTrStride = kTrCols + 1;                     // odd: conflict-free rows and columns
+// Per-lane sums, transposed through LDS into column sums: the LJ map energy
+// per class, the radial densities at polar and at other atoms, then the
+// Coulomb sums over positive and negative atoms, the hydrophobic sum, the
+// total map energy and the clash, contact and in-map counts.
+constexpr int kCols = kFeatClasses + 2 * kMapRbf + 7;  // 31
+constexpr int kTrStride = 33;                          // odd: conflict-free rows and columns
+constexpr int kAtomsPerLane = 3;                       // atoms whose loads a lane has in flight
 
-// The feature a sum column feeds.
 __device__ inline int feature_of_column(int col) {
-  if (col < kClassCols) {
-    const int cl = col / (1 + kMapRbf), v = col - cl * (1 + kMapRbf);
-    return v == 0 ? kFLj + cl : kFRbf + cl * kMapRbf + (v - 1);
-  }
-  return kFCoulPos + (col - kClassCols);  // 8..14 in column order
+  if (col < kFeatClasses) return kFLj + col;
+  if (col < kFeatClasses + kMapRbf) return kFRbfPolar + (col - kFeatClasses);
+  if (col < kFeatClasses + 2 * kMapRbf) return kFRbfApolar + (col - kFeatClasses - kMapRbf);
+  return kFCoulPos + (col - kFeatClasses - 2 * kMapRbf);  // 8..14 in column order
 }
 
-// One wave per candidate, a lane per atom. Each lane sums its own atoms'
-// values per column in registers (fixed order over the candidate's passes of
-// 64 atoms); the 64 lanes' columns are then added through an LDS transpose
-// (lane r's row, summed by the column's lane in row order), which keeps the
-// sums deterministic.
+// One wave per candidate, a lane per atom, kAtomsPerLane atoms a lane at a
+// time (all their loads issued before any is used). Each lane sums its own
+// atoms' values per column in registers in a fixed order; the lanes' columns
+// are then added through an LDS transpose (lane r's row, summed by the
+// column's lane in row order), which keeps the sums deterministic.
 __global__ void __launch_bounds__(256) k_map_features(FeatArgs g) {
   __shared__ float s_tr[4][kWave * kTrStride];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -294,143 +314,144 @@ __global__ void __launch_bounds__(256) k_map_features(FeatArgs g) {
   const MapView V = g.views[g.cand_ag[c]];
   const uint32_t a0 = g.cand_atom_off[c], a1 = g.cand_atom_off[c + 1];
   const int32_t cx = V.n[0] - 1, cy = V.n[1] - 1, cz = V.n[2] - 1;
-  float acc[kFeatClasses][1 + kMapRbf];
+  float lj[kFeatClasses], rbp[kMapRbf], rba[kMapRbf];
 #pragma unroll
-  for (int cl = 0; cl < kFeatClasses; ++cl)
+  for (int k = 0; k < kFeatClasses; ++k) lj[k] = 0.0f;
 #pragma unroll
-    for (int v = 0; v <= kMapRbf; ++v) acc[cl][v] = 0.0f;
+  for (int k = 0; k < kMapRbf; ++k) rbp[k] = rba[k] = 0.0f;
   float cpos = 0.0f, cneg = 0.0f, shyd = 0.0f, stot = 0.0f, nclash = 0.0f, ncont = 0.0f, nin = 0.0f;
-  for (uint32_t b = a0; b < a1; b += kWave) {
-    const uint32_t a = b + lane;
-    const bool live = a < a1;
-    float4 P = make_float4(0.f, 0.f, 0.f, 0.f), Q = P;
-    if (live) {
-      P = g.pos[a];
-      Q = g.par[a];
-    }
-    // the atom's LJ class: its (Rmin/2, eps) pair, else the nearest one
-    int lc = -1;
-    float best = 3.0e38f;
-    int near = 0;
+  for (uint32_t b = a0; b < a1; b += kWave * kAtomsPerLane) {
+    float4 P[kAtomsPerLane], Q[kAtomsPerLane];
 #pragma unroll
-    for (int k = 0; k < kMapLjClasses; ++k) {
-      if (Q.x == g.cls.rmh[k] && Q.y == g.cls.eps[k]) lc = k;
-      const float d = fabsf(Q.x - g.cls.rmh[k]) + fabsf(Q.y - g.cls.eps[k]);
-      if (d < best) {
-        best = d;
-        near = k;
+    for (int q = 0; q < kAtomsPerLane; ++q) {
+      const uint32_t a = b + q * kWave + lane;
+      P[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      Q[q] = P[q];
+      if (a < a1) {
+        P[q] = g.pos[a];
+        Q[q] = g.par[a];
       }
     }
-    if (lc < 0) lc = near;
-    const bool hydp = Q.z != 0.0f;
-    const int fc = hydp ? kMapLjClasses : lc;
-    float elj = 0.0f, phi = 0.0f, hh = 0.0f;
-    float rb[kMapRbf];
+    int lc[kAtomsPerLane];
+    bool in[kAtomsPerLane];
+    float fx[kAtomsPerLane], fy[kAtomsPerLane], fz[kAtomsPerLane];
+    uint4 rl[kAtomsPerLane], rp[kAtomsPerLane], rh[kAtomsPerLane], rr[kAtomsPerLane];
 #pragma unroll
-    for (int k = 0; k < kMapRbf; ++k) rb[k] = 0.0f;
-    bool inside = false;
-    if (live && V.valid) {
-      const float ux = (P.x - V.origin[0]) * V.inv_h;
-      const float uy = (P.y - V.origin[1]) * V.inv_h;
-      const float uz = (P.z - V.origin[2]) * V.inv_h;
+    for (int q = 0; q < kAtomsPerLane; ++q) {
+      const uint32_t a = b + q * kWave + lane;
+      // the atom's LJ class: its (Rmin/2, eps) pair, else the nearest one
+      int cls = -1, near = 0;
+      float best = 3.0e38f;
+#pragma unroll
+      for (int k = 0; k < kMapLjClasses; ++k) {
+        if (Q[q].x == g.cls.rmh[k] && Q[q].y == g.cls.eps[k]) cls = k;
+        const float d = fabsf(Q[q].x - g.cls.rmh[k]) + fabsf(Q[q].y - g.cls.eps[k]);
+        if (d < best) {
+          best = d;
+          near = k;
+        }
+      }
+      lc[q] = cls < 0 ? near : cls;
+      const float ux = (P[q].x - V.origin[0]) * V.inv_h;
+      const float uy = (P[q].y - V.origin[1]) * V.inv_h;
+      const float uz = (P[q].z - V.origin[2]) * V.inv_h;
       const float ffx = floorf(ux), ffy = floorf(uy), ffz = floorf(uz);
       // (a NaN coordinate fails these tests: outside)
-      inside = ffx >= 0.0f && ffy >= 0.0f && ffz >= 0.0f && ffx < static_cast<float>(cx) &&
-               ffy < static_cast<float>(cy) && ffz < static_cast<float>(cz);
-      if (inside) {
+      in[q] = a < a1 && V.valid && ffx >= 0.0f && ffy >= 0.0f && ffz >= 0.0f &&
+              ffx < static_cast<float>(cx) && ffy < static_cast<float>(cy) && ffz < static_cast<float>(cz);
+      fx[q] = ux - ffx;
+      fy[q] = uy - ffy;
+      fz[q] = uz - ffz;
+      rl[q] = rp[q] = rh[q] = rr[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (in[q]) {
         const int32_t i = static_cast<int32_t>(ffx), j = static_cast<int32_t>(ffy), k = static_cast<int32_t>(ffz);
-        const float fx = ux - ffx, fy = uy - ffy, fz = uz - ffz;
         const size_t cell = (static_cast<size_t>(k) * cy + j) * cx + i;
-        elj = trilerp(V.erec + (static_cast<size_t>(lc) * V.ncells + cell) * 8, fx, fy, fz);
-        phi = trilerp(V.erec + (static_cast<size_t>(kMapLjClasses) * V.ncells + cell) * 8, fx, fy, fz);
-        hh = trilerp(V.erec + (static_cast<size_t>(kMapLjClasses + 1) * V.ncells + cell) * 8, fx, fy, fz);
-        // the nearest grid point's radial densities
-        const int32_t ni = i + (fx >= 0.5f ? 1 : 0), nj = j + (fy >= 0.5f ? 1 : 0), nk = k + (fz >= 0.5f ? 1 : 0);
-        const size_t pt = (static_cast<size_t>(nk) * V.n[1] + nj) * V.n[0] + ni;
-        const uint4 w = *reinterpret_cast<const uint4*>(V.rbf + pt * kMapRbf);
-        const uint16_t* v = reinterpret_cast<const uint16_t*>(&w);
-#pragma unroll
-        for (int q = 0; q < kMapRbf; ++q) rb[q] = half_bits_to_f32(v[q]);
+        const uint16_t* rec = V.erec + cell * (kMapRecSlots * 8);
+        const int sl = map_slot(lc[q]);
+        rl[q] = *reinterpret_cast<const uint4*>(sl < 0 ? V.esul + cell * 8 : rec + sl * 8);
+        rp[q] = *reinterpret_cast<const uint4*>(rec + map_slot(kMapLjClasses) * 8);
+        rh[q] = *reinterpret_cast<const uint4*>(rec + map_slot(kMapLjClasses + 1) * 8);
+        // the radial densities of the nearest point of the coarse grid
+        const int32_t ni = coarse_nearest(ux, V.nc[0]), nj = coarse_nearest(uy, V.nc[1]),
+                      nk = coarse_nearest(uz, V.nc[2]);
+        const size_t pt = (static_cast<size_t>(nk) * V.nc[1] + nj) * V.nc[0] + ni;
+        rr[q] = *reinterpret_cast<const uint4*>(V.rbf + pt * kMapRbf);
       }
     }
-    const float ec = P.w * phi;
-    const float eh = hydp ? hh : 0.0f;
-    cpos += P.w > 0.0f ? ec : 0.0f;
-    cneg += P.w < 0.0f ? ec : 0.0f;
-    shyd += eh;
-    stot += elj + ec + eh;
-    nclash += (inside && elj > kClashE) ? 1.0f : 0.0f;
-    ncont += (inside && elj < kContactE) ? 1.0f : 0.0f;
-    nin += inside ? 1.0f : 0.0f;
 #pragma unroll
-    for (int cl = 0; cl < kFeatClasses; ++cl) {
-      const bool mine = inside && fc == cl;
-      if (!ballot64(mine)) continue;  // uniform: no atom of the pass in this class
-      acc[cl][0] += mine ? elj : 0.0f;
+    for (int q = 0; q < kAtomsPerLane; ++q) {
+      const bool hydp = Q[q].z != 0.0f;
+      const int fc = hydp ? kMapLjClasses : lc[q];
+      const float elj = in[q] ? trilerp(rl[q], fx[q], fy[q], fz[q]) : 0.0f;
+      const float phi = in[q] ? trilerp(rp[q], fx[q], fy[q], fz[q]) : 0.0f;
+      const float hh = in[q] ? trilerp(rh[q], fx[q], fy[q], fz[q]) : 0.0f;
+      const float ec = P[q].w * phi;
+      const float eh = hydp ? hh : 0.0f;
+      cpos += P[q].w > 0.0f ? ec : 0.0f;
+      cneg += P[q].w < 0.0f ? ec : 0.0f;
+      shyd += eh;
+      stot += elj + ec + eh;
+      nclash += (in[q] && elj > kClashE) ? 1.0f : 0.0f;
+      ncont += (in[q] && elj < kContactE) ? 1.0f : 0.0f;
+      nin += in[q] ? 1.0f : 0.0f;
 #pragma unroll
-      for (int q = 0; q < kMapRbf; ++q) acc[cl][1 + q] += mine ? rb[q] : 0.0f;
+      for (int k = 0; k < kFeatClasses; ++k) lj[k] += fc == k ? elj : 0.0f;
+      const uint16_t* v = reinterpret_cast<const uint16_t*>(&rr[q]);
+      const bool pol = polar_class(fc);
+#pragma unroll
+      for (int k = 0; k < kMapRbf; ++k) {
+        const float r = half_bits_to_f32(v[k]);  // zero outside the map
+        rbp[k] += pol ? r : 0.0f;
+        rba[k] += pol ? 0.0f : r;
+      }
     }
   }
-  // the column sums over the lanes: two transpose rounds of kTrCols columns
+  // the column sums over the lanes (one transpose)
   float* tr = s_tr[wave];
-  float colsum[2];
 #pragma unroll
-  for (int round = 0; round < 2; ++round) {
-#pragma unroll
-    for (int q = 0; q < kTrCols; ++q) {
-      const int col = round * kTrCols + q;
-      float v = 0.0f;
-      if (col < kClassCols) v = acc[col / (1 + kMapRbf)][col % (1 + kMapRbf)];
-      else if (col == kClassCols + 0) v = cpos;
-      else if (col == kClassCols + 1) v = cneg;
-      else if (col == kClassCols + 2) v = shyd;
-      else if (col == kClassCols + 3) v = stot;
-      else if (col == kClassCols + 4) v = nclash;
-      else if (col == kClassCols + 5) v = ncont;
-      else if (col == kClassCols + 6) v = nin;
-      tr[lane * kTrStride + q] = v;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    float sum = 0.0f;
-    if (lane < static_cast<uint32_t>(kTrCols))
-      for (int r = 0; r < kWave; ++r) sum += tr[r * kTrStride + lane];
-    colsum[round] = sum;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int col = 0; col < kCols; ++col) {
+    float v;
+    if (col < kFeatClasses) v = lj[col];
+    else if (col < kFeatClasses + kMapRbf) v = rbp[col - kFeatClasses];
+    else if (col < kFeatClasses + 2 * kMapRbf) v = rba[col - kFeatClasses - kMapRbf];
+    else if (col == kFeatClasses + 2 * kMapRbf + 0) v = cpos;
+    else if (col == kFeatClasses + 2 * kMapRbf + 1) v = cneg;
+    else if (col == kFeatClasses + 2 * kMapRbf + 2) v = shyd;
+    else if (col == kFeatClasses + 2 * kMapRbf + 3) v = stot;
+    else if (col == kFeatClasses + 2 * kMapRbf + 4) v = nclash;
+    else if (col == kFeatClasses + 2 * kMapRbf + 5) v = ncont;
+    else v = nin;
+    tr[lane * kTrStride + col] = v;
   }
-  // the total map energy: column kClassCols + 3 (round 1, lane kClassCols + 3 - kTrCols)
-  const float total = readlane_f32(colsum[1], kClassCols + 3 - kTrCols);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  float sum = 0.0f;
+  if (lane < static_cast<uint32_t>(kCols))
+    for (int r = 0; r < kWave; ++r) sum += tr[r * kTrStride + lane];
+  const float total = readlane_f32(sum, kFeatClasses + 2 * kMapRbf + 3);
   if (g.map_e && lane == 0) g.map_e[c] = total;
   if (g.feat) {
     uint16_t* row = g.feat + static_cast<size_t>(c) * kPrescreenFeatures;
-    auto put = [&](int k, float v) {
-      float t = 0.0f;
-      if (k <= kFHyd) t = asinhf(v * 0.5f);
-      else if (k == kFTotal) t = asinhf(v * 0.1f);
-      else if (k >= kFClash && k <= kFInMap) t = log1pf(v);
-      else if (k >= kFRbf && k < kFRbf + kFeatClasses * kMapRbf) t = log1pf(v);
+    if (lane < static_cast<uint32_t>(kCols)) {
+      const int k = feature_of_column(static_cast<int>(lane));
+      float t;
+      if (k <= kFHyd) t = asinhf(sum * 0.5f);
+      else if (k == kFTotal) t = asinhf(sum * 0.1f);
+      else t = log1pf(sum);  // counts and radial densities
       row[k] = f32_to_bf16(t);
-    };
-#pragma unroll
-    for (int round = 0; round < 2; ++round) {
-      const int col = round * kTrCols + static_cast<int>(lane);
-      if (lane < static_cast<uint32_t>(kTrCols) && col < kSumCols) put(feature_of_column(col), colsum[round]);
+    } else if (lane == static_cast<uint32_t>(kCols)) {
+      row[kFInMap + 1] = 0;  // the one feature no column feeds
     }
-    // features no column feeds
-    if (lane == 0) row[kFInMap + 1] = 0;
-    for (int k = kFRbf + kFeatClasses * kMapRbf + static_cast<int>(lane); k < kPrescreenFeatures; k += kWave)
-      row[k] = 0;
   }
 }
 
 // Map values at listed grid points (test hook): the energy planes from the
-// cell records (the point as a corner of an adjacent cell) and the radial
-// densities, as f32.
-__global__ void k_map_gather(const uint16_t* __restrict__ erec, const uint16_t* __restrict__ rbf,
-                             int32_t n0, int32_t n1, int32_t n2, unsigned long long ncells,
+// cell records (the point as a corner of an adjacent cell), as f32, and the
+// radial densities where the point is on the coarse grid (NaN elsewhere).
+__global__ void k_map_gather(const uint16_t* __restrict__ erec, const uint16_t* __restrict__ esul,
+                             const uint16_t* __restrict__ rbf,
+                             int32_t n0, int32_t n1, int32_t n2, int32_t nc0, int32_t nc1,
                              const int32_t* __restrict__ ijk, uint32_t n, float* __restrict__ out) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
@@ -440,9 +461,14 @@ __global__ void k_map_gather(const uint16_t* __restrict__ erec, const uint16_t* 
   const int corner = (i - ci) + 2 * (j - cj) + 4 * (k - ck);
   const size_t cell = (static_cast<size_t>(ck) * cy + cj) * cx + ci;
   float* o = out + static_cast<size_t>(q) * (kMapPlanes + kMapRbf);
-  for (int p = 0; p < kMapPlanes; ++p) o[p] = half_bits_to_f32(erec[(p * ncells + cell) * 8 + corner]);
-  const size_t pt = (static_cast<size_t>(k) * n1 + j) * n0 + i;
-  for (int t = 0; t < kMapRbf; ++t) o[kMapPlanes + t] = half_bits_to_f32(rbf[pt * kMapRbf + t]);
+  for (int p = 0; p < kMapPlanes; ++p) {
+    const int sl = map_slot(p);
+    o[p] = half_bits_to_f32(sl < 0 ? esul[cell * 8 + corner] : erec[cell * (kMapRecSlots * 8) + sl * 8 + corner]);
+  }
+  const bool coarse = ((i | j | k) & 1) == 0;
+  const size_t pt = (static_cast<size_t>(k >> 1) * nc1 + (j >> 1)) * nc0 + (i >> 1);
+  for (int t = 0; t < kMapRbf; ++t)
+    o[kMapPlanes + t] = coarse ? half_bits_to_f32(rbf[pt * kMapRbf + t]) : __builtin_nanf("");
 }
 
 static inline uint32_t nblk(unsigned long long n, uint32_t t) {
@@ -456,8 +482,9 @@ int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing) {
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "maps: spacing must be >= 0.05 A");
   MapDev& M = ag->map;
   M.erec.release();
+  M.esul.release();
   M.rbf.release();
-  M.npts = M.ncells = 0;
+  M.npts = M.ncells = M.ncpts = 0;
   M.h = spacing;
   M.inv_h = 1.0f / spacing;
   const float pad = ag->cutoff + 2.0f * spacing;
@@ -470,14 +497,20 @@ int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing) {
     npts *= static_cast<size_t>(M.n[d]);
   }
   const size_t ncells = static_cast<size_t>(M.n[0] - 1) * (M.n[1] - 1) * (M.n[2] - 1);
-  const size_t bytes = ncells * kMapPlanes * 16 + npts * kMapRbf * 2 + npts * kMapPlanes * 2;
+  size_t ncpts = 1;
+  for (int d = 0; d < 3; ++d) {
+    M.nc[d] = (M.n[d] - 1) / 2 + 1;
+    ncpts *= static_cast<size_t>(M.nc[d]);
+  }
+  const size_t bytes = ncells * (kMapRecSlots + 1) * 16 + ncpts * kMapRbf * 2 + npts * kMapPlanes * 2;
   if (bytes > kMapMaxBytes)
     return dnab_set_error(ctx, DNAB_ERR_CAPACITY, "maps: %zu bytes at spacing %.3f A exceed %zu",
                           bytes, spacing, kMapMaxBytes);
   DevBuf<uint16_t> pts;
-  if (pts.alloc(npts * kMapPlanes) != hipSuccess || M.rbf.alloc(npts * kMapRbf) != hipSuccess ||
-      M.erec.alloc(ncells * kMapPlanes * 8) != hipSuccess) {
+  if (pts.alloc(npts * kMapPlanes) != hipSuccess || M.rbf.alloc(ncpts * kMapRbf) != hipSuccess ||
+      M.erec.alloc(ncells * kMapRecSlots * 8) != hipSuccess || M.esul.alloc(ncells * 8) != hipSuccess) {
     M.erec.release();
+    M.esul.release();
     M.rbf.release();
     return dnab_set_error(ctx, DNAB_ERR_ALLOC, "maps: allocation of %zu bytes failed", bytes);
   }
@@ -489,6 +522,7 @@ int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing) {
     a.kmax[d] = empty ? ag->grid.kmin[d] - 1 : ag->grid.kmax[d];
     a.origin[d] = M.origin[d];
     a.n[d] = M.n[d];
+    a.nc[d] = M.nc[d];
   }
   a.inv = ag->grid.inv;
   a.rc = ag->cutoff;
@@ -503,12 +537,13 @@ int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing) {
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_map_expand, dim3(nblk(ncells * kMapPlanes, 256)), dim3(256), 0, ctx->stream,
                      pts.p, static_cast<unsigned long long>(npts), M.n[0], M.n[1], M.n[2],
-                     static_cast<unsigned long long>(ncells), M.erec.p);
+                     static_cast<unsigned long long>(ncells), M.erec.p, M.esul.p);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   // the temporary point planes are freed when this returns: wait for the expansion
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   M.npts = npts;
   M.ncells = ncells;
+  M.ncpts = ncpts;
   return DNAB_OK;
 }
 
@@ -520,6 +555,7 @@ int population_map_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* feat,
     const MapDev& M = ag->map;
     MapView& v = views[a];
     v.erec = M.erec.p;
+    v.esul = M.esul.p;
     v.rbf = M.rbf.p;
     for (int d = 0; d < 3; ++d) {
       v.origin[d] = M.origin[d];
@@ -527,7 +563,8 @@ int population_map_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* feat,
     }
     v.inv_h = M.inv_h;
     v.valid = M.ncells > 0 && ag->grid.n > 0 ? 1u : 0u;
-    v.ncells = M.ncells;
+    for (int d = 0; d < 3; ++d) v.nc[d] = M.nc[d];
+    v.pad = 0;
   }
   DNAB_ALLOC_CHECK(ctx, pop->map_views.ensure(views.size()));
   DNAB_RETURN_IF(upload(ctx, pop->map_views.p, views.data(), views.size() * sizeof(MapView)));
@@ -571,7 +608,7 @@ int dnab_antigen_map_info(const dnab_antigen* ag, float* spacing, float origin[3
     if (origin) origin[d] = M.origin[d];
     if (dims) dims[d] = M.ncells ? M.n[d] : 0;
   }
-  if (bytes) *bytes = M.ncells * kMapPlanes * 16 + M.npts * kMapRbf * 2;
+  if (bytes) *bytes = M.ncells * (kMapRecSlots + 1) * 16 + M.ncpts * kMapRbf * 2;
   return DNAB_OK;
 }
 
@@ -592,8 +629,8 @@ int dnab_antigen_map_points(dnab_ctx* ctx, const dnab_antigen* ag, const int32_t
   DNAB_ALLOC_CHECK(ctx, d_ijk.alloc(3 * n));
   DNAB_ALLOC_CHECK(ctx, d_out.alloc(n * (kMapPlanes + kMapRbf)));
   DNAB_RETURN_IF(upload(ctx, d_ijk.p, ijk, 3 * n * sizeof(int32_t)));
-  hipLaunchKernelGGL(k_map_gather, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, M.erec.p, M.rbf.p,
-                     M.n[0], M.n[1], M.n[2], static_cast<unsigned long long>(M.ncells), d_ijk.p,
+  hipLaunchKernelGGL(k_map_gather, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, M.erec.p, M.esul.p, M.rbf.p,
+                     M.n[0], M.n[1], M.n[2], M.nc[0], M.nc[1], d_ijk.p,
                      static_cast<uint32_t>(n), d_out.p);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   DNAB_RETURN_IF(download(ctx, out, d_out.p, n * (kMapPlanes + kMapRbf) * sizeof(float)));

@@ -1,4 +1,4 @@
-// Learned prescreen (no reference counterpart): a 96 -> 256 -> 256 -> 256 -> 1
+// Learned prescreen (no reference counterpart): a 32 -> 256 -> 256 -> 256 -> 1
 // SiLU MLP over per-candidate features read from antigen maps (maps.hip),
 // with bf16 weights and activations and FP32 accumulation on the MFMA units,
 // trained on the device (Adam) on energies that the exact pair kernel
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
 
 // ---- features (maps.hip) ------------------------------------------------------
 
-constexpr int kFeat = kPrescreenFeatures;  // 96
+constexpr int kFeat = kPrescreenFeatures;  // 32
 
 // ---- small kernels ------------------------------------------------------------------
 

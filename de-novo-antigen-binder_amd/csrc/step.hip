@@ -415,10 +415,9 @@ int select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m, 
                     DevBuf<uint32_t>& scratch) {
   if (m == 0 || m > 2048) return dnab_set_error(ctx, DNAB_ERR_INVALID, "select: m must be 1..2048");
   if (n == 0) return DNAB_OK;
-  constexpr uint32_t kChunk = kSelBlock * 4;       // stage inputs per block
-  constexpr uint32_t kLast = kSelBlock * 32;       // the last stage's inputs (one block)
+  constexpr uint32_t kChunk = kSelBlock * 4;  // stage inputs per block
   const uint32_t nb1 = (n + kChunk - 1) / kChunk;
-  if (n <= kLast && nb1 == 1) {
+  if (nb1 == 1) {
     hipLaunchKernelGGL((k_select_smallest<4, true>), dim3(1), dim3(kSelBlock), 0, ctx->stream, scores,
                        nullptr, nullptr, nullptr, 1u, n, m, nullptr, out_idx, nullptr);
     DNAB_HIP_CHECK(ctx, hipGetLastError());
@@ -433,7 +432,7 @@ int select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m, 
   hipLaunchKernelGGL((k_select_smallest<4, true>), dim3(nb1), dim3(kSelBlock), 0, ctx->stream, scores,
                      nullptr, nullptr, nullptr, 1u, n, m, kbuf[0], ibuf[0], cbuf[0]);
   uint32_t slots = nb1 * m, cur = 0;
-  while (slots > kLast) {  // more chunked stages (each at least halves the slots)
+  while (slots > kChunk) {  // more chunked stages (m <= kChunk / 2: each at least halves)
     const uint32_t nb = (slots + kChunk - 1) / kChunk;
     hipLaunchKernelGGL((k_select_smallest<4, false>), dim3(nb), dim3(kSelBlock), 0, ctx->stream,
                        nullptr, kbuf[cur], ibuf[cur], cbuf[cur], m, slots, m, kbuf[cur ^ 1],
@@ -441,7 +440,7 @@ int select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m, 
     cur ^= 1;
     slots = nb * m;
   }
-  hipLaunchKernelGGL((k_select_smallest<32, false>), dim3(1), dim3(kSelBlock), 0, ctx->stream,
+  hipLaunchKernelGGL((k_select_smallest<4, false>), dim3(1), dim3(kSelBlock), 0, ctx->stream,
                      nullptr, kbuf[cur], ibuf[cur], cbuf[cur], m, slots, m, nullptr, out_idx, nullptr);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   return DNAB_OK;

@@ -144,7 +144,7 @@ class TrainParamsC(C.Structure):
                 ("seed", C.c_uint64)]
 
 
-FEATURE_DIM = 96
+FEATURE_DIM = 32
 MAP_VALUES = 17  # per grid point: 7 LJ classes, potential, hydrophobic, 8 radial densities
 MLP_HIDDEN = 256
 

@@ -307,7 +307,7 @@ class DeviceAntigen:
 
     def build_maps(self, spacing: float = 0.0):
         """(Re)build the prescreen's antigen maps (dnab_antigen_build_maps);
-        spacing <= 0 selects the default 0.5 A."""
+        spacing <= 0 selects the default 0.625 A."""
         self.ctx.check(N.lib.dnab_antigen_build_maps(self.ctx.handle, self._h, spacing))
 
     def map_info(self):
@@ -676,7 +676,7 @@ class DevicePopulation:
 
 
 class Mlp:
-    """The prescreen MLP (96 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights)."""
+    """The prescreen MLP (32 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights)."""
 
     def __init__(self, ctx: GpuContext, seed: int = 0):
         self.ctx = ctx

@@ -672,28 +672,29 @@ int dnab_population_read_scores(dnab_ctx* ctx, const dnab_population* pop, uint3
 
 /* ---- learned prescreen (no reference counterpart; the reference has no
  * learned components, Cargo.toml:4) -------------------------------------------
- * Antigen maps: on a grid of spacing h (default 0.5 A) over the antigen's atom
+ * Antigen maps: on a grid of spacing h (default 0.625 A) over the antigen's atom
  * box widened by cutoff + 2h, the sums the exact scorer forms for a probe atom
  * at a grid point, over antigen atoms with 0.25 <= r^2 <= cutoff^2: LJ
  * eps_ij (s^12 - 2 s^6), s = R_ij / r, for each of the 7 distinct (Rmin/2, eps)
  * classes of the AMBER99SB table; 332 q_i / r; -0.5 per hydrophobic atom with
  * r^2 < 36; and 8 radial densities exp(-(r - mu_k)^2 / (2 * 0.75^2)),
- * mu_k = 2 + k A. Stored in fp16 (saturated at +-60000).
- * Features (96 per candidate, bf16): a candidate atom reads its class's LJ
+ * mu_k = 2 + k A (on every second grid point). Stored in fp16 (saturated at
+ * +-60000).
+ * Features (32 per candidate, bf16): a candidate atom reads its class's LJ
  * plane, the potential and the hydrophobic plane by trilinear interpolation
- * and the radial densities at the nearest grid point; per candidate the sums
- * per atom class (7 LJ classes, hydrophobic carbons their own) of the LJ map
- * energies and of the radial densities (asinh(x/2), log1p), the Coulomb sums
- * over positive and negative atoms and the hydrophobic sum (asinh(x/2)), the
- * total map energy (asinh(x/10)) and log1p counts of clashing (LJ > 1),
- * contacting (LJ < -0.05) and in-map atoms.
- * Model: 96 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights and activations,
+ * and the radial densities at the nearest grid point; per candidate the LJ
+ * map energy summed per atom class (7 LJ classes, hydrophobic atoms their
+ * own), the Coulomb sums over positive and negative atoms and the
+ * hydrophobic sum (asinh(x/2)), the total map energy (asinh(x/10)), log1p
+ * counts of clashing (LJ > 1), contacting (LJ < -0.05) and in-map atoms, and
+ * the radial densities summed over N/O atoms and over the others (log1p).
+ * Model: 32 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights and activations,
  * FP32 accumulation (MFMA), fp32 master weights, Adam. Target: asinh(E / s)
  * with E the exact LJ + Coulomb + hydrophobic energy, s = energy_scale. */
-#define DNAB_FEATURE_DIM 96
+#define DNAB_FEATURE_DIM 32
 #define DNAB_MLP_HIDDEN 256
 int dnab_prescreen_feature_count(size_t* n);
-/* Builds (or rebuilds) the antigen's maps; spacing <= 0 selects 0.5 A.
+/* Builds (or rebuilds) the antigen's maps; spacing <= 0 selects 0.625 A.
  * Features build missing maps with the default spacing. */
 int dnab_antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing);
 /* spacing 0 and dims 0 when no maps are built; bytes of device memory held. */
@@ -709,13 +710,13 @@ int dnab_population_map_energy(dnab_ctx* ctx, dnab_population* pop, float* out);
 typedef struct dnab_mlp dnab_mlp;
 typedef struct dnab_dataset dnab_dataset;
 
-/* Features of every candidate; copied to host_bf16 (n_cand * 96) when not
+/* Features of every candidate; copied to host_bf16 (n_cand * 32) when not
  * NULL. */
 int dnab_population_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* host_bf16);
 int dnab_mlp_create(dnab_ctx* ctx, uint64_t seed, dnab_mlp** out);
 void dnab_mlp_destroy(dnab_mlp* mlp);
-/* Flat parameter vector: W1[96][256], b1[256], W2[256][256], b2, W3[256][256],
- * b3, w4[256], b4[1] (156,673 floats, fp32 master copy). */
+/* Flat parameter vector: W1[32][256], b1[256], W2[256][256], b2, W3[256][256],
+ * b3, w4[256], b4[1] (140,289 floats, fp32 master copy). */
 int dnab_mlp_param_count(size_t* n);
 int dnab_mlp_get_params(dnab_ctx* ctx, const dnab_mlp* mlp, float* out, size_t n);
 int dnab_mlp_set_params(dnab_ctx* ctx, dnab_mlp* mlp, const float* in, size_t n);

@@ -15,7 +15,8 @@ antigen atoms i passing the float32 pair predicate 0.25 <= r^2 <= rc^2 (r^2 =
 (dx*dx + dy*dy) + dz*dz in float32) of, per LJ class c, sqrt(eps_c eps_i)
 (s^12 - 2 s^6) with s = (Rmin/2_c + Rmin/2_i) / r; 332 q_i / r; -0.5 for
 hydrophobic i with r^2 < 36; exp(-(r - mu_k)^2 / (2 sigma^2)). Stored as
-fp16 after clamping to +-60000.
+fp16 after clamping to +-60000; the radial densities are kept on every
+second grid point only (an atom reads the nearest of those).
 
 Model: x -> SiLU(x W1 + b1) -> SiLU(. W2 + b2) -> SiLU(. W3 + b3) -> . w4 + b4.
 With bf16=True the oracle rounds at the same points as the device path:
@@ -26,13 +27,15 @@ from __future__ import annotations
 
 import numpy as np
 
-D, H = 96, 256
-MAP_H = 0.5
+D, H = 32, 256
+MAP_H = 0.625
 RBF_MU0, RBF_DMU, RBF_SIGMA, N_RBF = 2.0, 1.0, 0.75, 8
 MAP_CLAMP = 60000.0
 CLASH_E, CONTACT_E = 1.0, -0.05
 N_LJ = 7
-F_COUL_POS, F_COUL_NEG, F_HYD, F_TOTAL, F_CLASH, F_CONTACT, F_INMAP, F_RBF = 8, 9, 10, 11, 12, 13, 14, 16
+F_COUL_POS, F_COUL_NEG, F_HYD, F_TOTAL, F_CLASH, F_CONTACT, F_INMAP = 8, 9, 10, 11, 12, 13, 14
+F_RBF_POLAR, F_RBF_APOLAR = 16, 24
+POLAR_CLASSES = (2, 3, 4, 5)   # N*, O/O2, OH, OS
 
 
 def bf16_round(a) -> np.ndarray:
@@ -122,7 +125,8 @@ def atom_class(rmh, eps, classes):
 
 def atom_map_lookup(x, y, z, origin, h, dims):
     """Cell (i, j, k), fractions (float32 arithmetic as the kernel) and the
-    nearest grid point of one atom; None outside the map."""
+    nearest point of the radial-density grid (every second point, given as a
+    point of the full grid) of one atom; None outside the map."""
     inv = np.float32(1.0) / np.float32(h)
     u = ((np.array([x, y, z], np.float32) - origin) * inv).astype(np.float32)
     f = np.floor(u)
@@ -130,8 +134,11 @@ def atom_map_lookup(x, y, z, origin, h, dims):
         return None
     frac = (u - f).astype(np.float32)
     i = f.astype(np.int64)
-    near = i + (frac >= np.float32(0.5))
-    return i, frac.astype(np.float64), near
+    uc = (u * np.float32(0.5)).astype(np.float32)
+    fc = np.floor(uc)
+    nc = (np.asarray(dims) - 1) // 2 + 1
+    near = np.minimum(fc.astype(np.int64) + ((uc - fc) >= np.float32(0.5)), nc - 1)
+    return i, frac.astype(np.float64), 2 * near
 
 
 def trilinear(corners, frac):
@@ -154,7 +161,7 @@ def features(atoms, origin, h, dims, lookup_values, classes):
     candidate's x, y, z, charge, r_min_half, epsilon, hydrophobic; the map
     grid (origin, spacing, points per axis); lookup_values(ijk [m, 3]) ->
     [m, 17] map values at grid points. An atom outside the map adds nothing.
-    Returns (row[96] transformed, total map energy)."""
+    Returns (row[32] transformed, total map energy)."""
     raw = np.zeros(D)
     n = len(atoms["x"])
     for a in range(n):
@@ -183,12 +190,15 @@ def features(atoms, origin, h, dims, lookup_values, classes):
         raw[F_CLASH] += elj > CLASH_E
         raw[F_CONTACT] += elj < CONTACT_E
         raw[F_INMAP] += 1
-        raw[F_RBF + fc * N_RBF:F_RBF + (fc + 1) * N_RBF] += rb
+        if fc in POLAR_CLASSES:
+            raw[F_RBF_POLAR:F_RBF_POLAR + N_RBF] += rb
+        else:
+            raw[F_RBF_APOLAR:F_RBF_APOLAR + N_RBF] += rb
     row = np.zeros(D)
     row[:F_HYD + 1] = np.arcsinh(raw[:F_HYD + 1] * 0.5)
     row[F_TOTAL] = np.arcsinh(raw[F_TOTAL] * 0.1)
     row[F_CLASH:F_INMAP + 1] = np.log1p(raw[F_CLASH:F_INMAP + 1])
-    row[F_RBF:F_RBF + (N_LJ + 1) * N_RBF] = np.log1p(raw[F_RBF:F_RBF + (N_LJ + 1) * N_RBF])
+    row[F_RBF_POLAR:] = np.log1p(raw[F_RBF_POLAR:])
     return row, raw[F_TOTAL]
 
 

@@ -55,8 +55,12 @@ def test_map_points_vs_oracle(ctx):
     near = np.stack([a.x[pick], a.y[pick], a.z[pick]], 1) + rng.uniform(-3, 3, (150, 3))
     ijk_near = np.clip(np.round((near - origin) / M.MAP_H), 0, dims - 1).astype(np.int32)
     ijk = np.concatenate([rng.integers(0, dims, (150, 3)).astype(np.int32), ijk_near])
+    ijk[::2] -= ijk[::2] % 2            # half the points on the radial-density grid
     got = dag.map_points(ijk).astype(np.float64)
     ref = M.map_values(_ag_dict(a), ijk, origin, M.MAP_H, w.cutoff, _cls(), a.hydrophobic)
+    coarse = (ijk % 2 == 0).all(1)
+    assert np.isnan(got[~coarse, M.N_LJ + 2:]).all() and not np.isnan(got[coarse]).any()
+    got[~coarse, M.N_LJ + 2:] = ref[~coarse, M.N_LJ + 2:]
     bad = np.abs(got - ref) > 2 ** -10 * np.abs(ref) + 0.02
     assert not bad.any(), (np.argwhere(bad)[:5], got[bad][:5], ref[bad][:5])
     assert (np.abs(ref[:, :M.N_LJ]) == M.MAP_CLAMP).any()   # some points sit in a wall
@@ -107,12 +111,13 @@ def test_map_energy_tracks_exact_energy(ctx):
     E, _ = pop.energies()
     neg = E < -1.0
     assert neg.sum() > 100
-    # the exact best 32 are among the map's best eighth (the oracle's maps
-    # give 30 on this pool: interpolation errors of a few kcal/mol reorder the
-    # weak binders, rank correlation 0.56 over E < -1, not the strong ones)
+    # the exact best 32 are among the map's best eighth (27 on this pool at
+    # the 0.625 A default, 30 at 0.5 A: interpolation errors of a few kcal/mol
+    # reorder the weak binders, rank correlation 0.56 over E < -1, less so
+    # the strong ones)
     top = set(np.argsort(E, kind="stable")[:32].tolist())
     kept = set(np.argsort(S, kind="stable")[:len(S) // 8].tolist())
-    assert len(top & kept) >= 29
+    assert len(top & kept) >= 26
 
 
 def test_mlp_loss_grad_vs_oracle(ctx):
