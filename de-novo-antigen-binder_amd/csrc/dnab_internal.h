@@ -209,11 +209,14 @@ struct MapDev {
   DevBuf<uint16_t> erec;       // [cell][kMapRecSlots][8] fp16
   DevBuf<uint16_t> esul;       // [cell][8] fp16: the sulfur class's LJ
   DevBuf<uint16_t> rbf;        // [coarse point][8] fp16
+  DevBuf<uint32_t> hot;        // one bit per cell: some corner has an antigen atom within the
+                               // cutoff (else every value the cell's atoms read is zero)
 };
 struct MapView {
   const uint16_t* erec;
   const uint16_t* esul;
   const uint16_t* rbf;
+  const uint32_t* hot;
   float origin[3];
   float inv_h;
   int32_t n[3];

@@ -25,7 +25,8 @@
 // summed per atom class (the 7 LJ parameter classes, hydrophobic atoms a class
 // of their own), the Coulomb sums over positive and negative atoms, the
 // hydrophobic sum, the total map energy, counts of clashing, contacting and
-// in-map atoms, and the radial densities summed over polar (N, O) and over
+// near atoms (in a cell with an antigen atom within the cutoff of a corner),
+// and the radial densities summed over polar (N, O) and over
 // the other atoms: atom-class x RBF distance histograms against the antigen's
 // heavy atoms.
 #include <hip/hip_runtime.h>
@@ -54,7 +55,7 @@ constexpr float kContactE = -0.05f; // ... below this, a contact
 // Feature layout
 constexpr int kFLj = 0;            // [kFeatClasses] LJ map energy per atom class
 constexpr int kFCoulPos = 8, kFCoulNeg = 9, kFHyd = 10, kFTotal = 11;
-constexpr int kFClash = 12, kFContact = 13, kFInMap = 14;
+constexpr int kFClash = 12, kFContact = 13, kFNear = 14;
 constexpr int kFRbfPolar = 16;     // [kMapRbf] radial densities at N and O atoms
 constexpr int kFRbfApolar = 24;    // [kMapRbf] ... at the other atoms
 static_assert(kFRbfApolar + kMapRbf == kPrescreenFeatures, "feature layout");
@@ -105,6 +106,7 @@ struct MapBuildArgs {
   int32_t n[3];
   MapClasses cls;
   uint16_t* pts;  // [plane][point] fp16
+  uint8_t* phot;  // [point]: an antigen atom passes the pair predicate
   uint16_t* rbf;  // [coarse point][kMapRbf] fp16
   int32_t nc[3];  // coarse points per axis (every second point)
   unsigned long long npts;
@@ -149,6 +151,7 @@ __global__ void __launch_bounds__(256) k_map_points(MapBuildArgs a) {
 #pragma unroll
   for (int k = 0; k < kMapRbf; ++k) rb[k] = 0.0f;
   float phi = 0.0f, hyd = 0.0f;
+  bool any = false;
   const float inv2s2 = 1.0f / (2.0f * kRbfSigma * kRbfSigma);
 
   for (int32_t cb = 0; cb < ncell; cb += 256) {
@@ -195,6 +198,7 @@ __global__ void __launch_bounds__(256) k_map_points(MapBuildArgs a) {
         const float dx = px - B.x, dy = py - B.y, dz = pz - B.z;
         const float r2 = (dx * dx + dy * dy) + dz * dz;
         if (valid && r2 <= a.rc2 && r2 >= kMinRSq) {
+          any = true;
           const float rinv = 1.0f / sqrtf(r2);
           const float rinv2 = rinv * rinv;
 #pragma unroll
@@ -223,6 +227,7 @@ __global__ void __launch_bounds__(256) k_map_points(MapBuildArgs a) {
   for (int k = 0; k < kMapLjClasses; ++k) a.pts[k * a.npts + pt] = to_half_bits(lj[k]);
   a.pts[kMapLjClasses * a.npts + pt] = to_half_bits(phi);
   a.pts[(kMapLjClasses + 1) * a.npts + pt] = to_half_bits(hyd);
+  a.phot[pt] = any ? 1 : 0;
   if ((ix | iy | iz) & 1) return;  // radial densities on every second point
   uint16_t r16[kMapRbf];
 #pragma unroll
@@ -257,6 +262,30 @@ __global__ void k_map_expand(const uint16_t* __restrict__ pts, unsigned long lon
   *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(v);
 }
 
+// One bit per cell (32 cells a word, in cell order): some corner of the
+// cell has an antigen atom within the cutoff. An atom in a cold cell reads
+// only zeros (every corner, and its nearest radial-density point, which is
+// a corner).
+__global__ void k_map_hot(const uint8_t* __restrict__ phot, int32_t n0, int32_t n1,
+                          unsigned long long ncells, uint32_t* __restrict__ hot) {
+  const unsigned long long w = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x;
+  if (w * 32 >= ncells) return;
+  const int32_t cx = n0 - 1, cy = n1 - 1;
+  uint32_t bits = 0;
+  for (int b = 0; b < 32; ++b) {
+    const unsigned long long cell = w * 32 + b;
+    if (cell >= ncells) break;
+    const int32_t i = static_cast<int32_t>(cell % cx);
+    const int32_t j = static_cast<int32_t>((cell / cx) % cy);
+    const int32_t k = static_cast<int32_t>(cell / (static_cast<unsigned long long>(cx) * cy));
+    bool h = false;
+    for (int c = 0; c < 8; ++c)
+      h = h || phot[(static_cast<size_t>(k + (c >> 2)) * n1 + (j + ((c >> 1) & 1))) * n0 + (i + (c & 1))];
+    bits |= h ? (1u << b) : 0u;
+  }
+  hot[w] = bits;
+}
+
 struct FeatArgs {
   const MapView* views;  // per antigen
   const uint32_t* cand_ag;
@@ -289,7 +318,7 @@ __device__ inline float trilerp(uint4 w, float fx, float fy, float fz) {
 // Per-lane sums, transposed through LDS into column sums: the LJ map energy
 // per class, the radial densities at polar and at other atoms, then the
 // Coulomb sums over positive and negative atoms, the hydrophobic sum, the
-// total map energy and the clash, contact and in-map counts.
+// total map energy and the clash, contact and near-atom counts.
 constexpr int kCols = kFeatClasses + 2 * kMapRbf + 7;  // 31
 constexpr int kTrStride = 33;                          // odd: conflict-free rows and columns
 constexpr int kAtomsPerLane = 3;                       // atoms whose loads a lane has in flight
@@ -363,9 +392,14 @@ __global__ void __launch_bounds__(256) k_map_features(FeatArgs g) {
       fy[q] = uy - ffy;
       fz[q] = uz - ffz;
       rl[q] = rp[q] = rh[q] = rr[q] = make_uint4(0u, 0u, 0u, 0u);
+      size_t cell = 0;
       if (in[q]) {
         const int32_t i = static_cast<int32_t>(ffx), j = static_cast<int32_t>(ffy), k = static_cast<int32_t>(ffz);
-        const size_t cell = (static_cast<size_t>(k) * cy + j) * cx + i;
+        cell = (static_cast<size_t>(k) * cy + j) * cx + i;
+        in[q] = (V.hot[cell >> 5] >> (cell & 31)) & 1u;  // a cold cell reads zeros
+      }
+      if (in[q]) {
+        const int32_t i = static_cast<int32_t>(ffx), j = static_cast<int32_t>(ffy), k = static_cast<int32_t>(ffz);
         const uint16_t* rec = V.erec + cell * (kMapRecSlots * 8);
         const int sl = map_slot(lc[q]);
         rl[q] = *reinterpret_cast<const uint4*>(sl < 0 ? V.esul + cell * 8 : rec + sl * 8);
@@ -441,7 +475,7 @@ __global__ void __launch_bounds__(256) k_map_features(FeatArgs g) {
       else t = log1pf(sum);  // counts and radial densities
       row[k] = f32_to_bf16(t);
     } else if (lane == static_cast<uint32_t>(kCols)) {
-      row[kFInMap + 1] = 0;  // the one feature no column feeds
+      row[kFNear + 1] = 0;  // the one feature no column feeds
     }
   }
 }
@@ -450,7 +484,7 @@ __global__ void __launch_bounds__(256) k_map_features(FeatArgs g) {
 // cell records (the point as a corner of an adjacent cell), as f32, and the
 // radial densities where the point is on the coarse grid (NaN elsewhere).
 __global__ void k_map_gather(const uint16_t* __restrict__ erec, const uint16_t* __restrict__ esul,
-                             const uint16_t* __restrict__ rbf,
+                             const uint16_t* __restrict__ rbf, const uint32_t* __restrict__ hot,
                              int32_t n0, int32_t n1, int32_t n2, int32_t nc0, int32_t nc1,
                              const int32_t* __restrict__ ijk, uint32_t n, float* __restrict__ out) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -460,7 +494,7 @@ __global__ void k_map_gather(const uint16_t* __restrict__ erec, const uint16_t* 
   const int32_t ci = min(i, cx - 1), cj = min(j, cy - 1), ck = min(k, n2 - 2);
   const int corner = (i - ci) + 2 * (j - cj) + 4 * (k - ck);
   const size_t cell = (static_cast<size_t>(ck) * cy + cj) * cx + ci;
-  float* o = out + static_cast<size_t>(q) * (kMapPlanes + kMapRbf);
+  float* o = out + static_cast<size_t>(q) * (kMapPlanes + kMapRbf + 1);
   for (int p = 0; p < kMapPlanes; ++p) {
     const int sl = map_slot(p);
     o[p] = half_bits_to_f32(sl < 0 ? esul[cell * 8 + corner] : erec[cell * (kMapRecSlots * 8) + sl * 8 + corner]);
@@ -469,6 +503,10 @@ __global__ void k_map_gather(const uint16_t* __restrict__ erec, const uint16_t* 
   const size_t pt = (static_cast<size_t>(k >> 1) * nc1 + (j >> 1)) * nc0 + (i >> 1);
   for (int t = 0; t < kMapRbf; ++t)
     o[kMapPlanes + t] = coarse ? half_bits_to_f32(rbf[pt * kMapRbf + t]) : __builtin_nanf("");
+  // the occupancy bit of the cell whose lower corner is the point
+  const bool lower = i < cx && j < cy && k < n2 - 1;
+  const size_t own = (static_cast<size_t>(k) * cy + j) * cx + i;
+  o[kMapPlanes + kMapRbf] = lower ? static_cast<float>((hot[own >> 5] >> (own & 31)) & 1u) : __builtin_nanf("");
 }
 
 static inline uint32_t nblk(unsigned long long n, uint32_t t) {
@@ -484,6 +522,7 @@ int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing) {
   M.erec.release();
   M.esul.release();
   M.rbf.release();
+  M.hot.release();
   M.npts = M.ncells = M.ncpts = 0;
   M.h = spacing;
   M.inv_h = 1.0f / spacing;
@@ -502,16 +541,20 @@ int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing) {
     M.nc[d] = (M.n[d] - 1) / 2 + 1;
     ncpts *= static_cast<size_t>(M.nc[d]);
   }
-  const size_t bytes = ncells * (kMapRecSlots + 1) * 16 + ncpts * kMapRbf * 2 + npts * kMapPlanes * 2;
+  const size_t bytes = ncells * (kMapRecSlots + 1) * 16 + ncells / 8 + ncpts * kMapRbf * 2 +
+                       npts * (kMapPlanes * 2 + 1);
   if (bytes > kMapMaxBytes)
     return dnab_set_error(ctx, DNAB_ERR_CAPACITY, "maps: %zu bytes at spacing %.3f A exceed %zu",
                           bytes, spacing, kMapMaxBytes);
   DevBuf<uint16_t> pts;
-  if (pts.alloc(npts * kMapPlanes) != hipSuccess || M.rbf.alloc(ncpts * kMapRbf) != hipSuccess ||
+  DevBuf<uint8_t> phot;
+  if (pts.alloc(npts * kMapPlanes) != hipSuccess || phot.alloc(npts) != hipSuccess ||
+      M.hot.alloc((ncells + 31) / 32) != hipSuccess || M.rbf.alloc(ncpts * kMapRbf) != hipSuccess ||
       M.erec.alloc(ncells * kMapRecSlots * 8) != hipSuccess || M.esul.alloc(ncells * 8) != hipSuccess) {
     M.erec.release();
     M.esul.release();
     M.rbf.release();
+    M.hot.release();
     return dnab_set_error(ctx, DNAB_ERR_ALLOC, "maps: allocation of %zu bytes failed", bytes);
   }
   MapBuildArgs a{};
@@ -530,6 +573,7 @@ int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing) {
   a.h = spacing;
   a.cls = map_classes();
   a.pts = pts.p;
+  a.phot = phot.p;
   a.rbf = M.rbf.p;
   a.npts = npts;
   const dim3 grid(nblk(M.n[0], 8), nblk(M.n[1], 8), nblk(M.n[2], 4));
@@ -538,6 +582,9 @@ int antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing) {
   hipLaunchKernelGGL(k_map_expand, dim3(nblk(ncells * kMapPlanes, 256)), dim3(256), 0, ctx->stream,
                      pts.p, static_cast<unsigned long long>(npts), M.n[0], M.n[1], M.n[2],
                      static_cast<unsigned long long>(ncells), M.erec.p, M.esul.p);
+  DNAB_HIP_CHECK(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_map_hot, dim3(nblk((ncells + 31) / 32, 256)), dim3(256), 0, ctx->stream, phot.p,
+                     M.n[0], M.n[1], static_cast<unsigned long long>(ncells), M.hot.p);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   // the temporary point planes are freed when this returns: wait for the expansion
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -557,6 +604,7 @@ int population_map_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* feat,
     v.erec = M.erec.p;
     v.esul = M.esul.p;
     v.rbf = M.rbf.p;
+    v.hot = M.hot.p;
     for (int d = 0; d < 3; ++d) {
       v.origin[d] = M.origin[d];
       v.n[d] = M.n[d];
@@ -608,7 +656,7 @@ int dnab_antigen_map_info(const dnab_antigen* ag, float* spacing, float origin[3
     if (origin) origin[d] = M.origin[d];
     if (dims) dims[d] = M.ncells ? M.n[d] : 0;
   }
-  if (bytes) *bytes = M.ncells * (kMapRecSlots + 1) * 16 + M.ncpts * kMapRbf * 2;
+  if (bytes) *bytes = M.ncells * (kMapRecSlots + 1) * 16 + (M.ncells + 31) / 32 * 4 + M.ncpts * kMapRbf * 2;
   return DNAB_OK;
 }
 
@@ -627,13 +675,13 @@ int dnab_antigen_map_points(dnab_ctx* ctx, const dnab_antigen* ag, const int32_t
   DevBuf<int32_t> d_ijk;
   DevBuf<float> d_out;
   DNAB_ALLOC_CHECK(ctx, d_ijk.alloc(3 * n));
-  DNAB_ALLOC_CHECK(ctx, d_out.alloc(n * (kMapPlanes + kMapRbf)));
+  DNAB_ALLOC_CHECK(ctx, d_out.alloc(n * (kMapPlanes + kMapRbf + 1)));
   DNAB_RETURN_IF(upload(ctx, d_ijk.p, ijk, 3 * n * sizeof(int32_t)));
-  hipLaunchKernelGGL(k_map_gather, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, M.erec.p, M.esul.p, M.rbf.p,
+  hipLaunchKernelGGL(k_map_gather, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, M.erec.p, M.esul.p, M.rbf.p, M.hot.p,
                      M.n[0], M.n[1], M.n[2], M.nc[0], M.nc[1], d_ijk.p,
                      static_cast<uint32_t>(n), d_out.p);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
-  DNAB_RETURN_IF(download(ctx, out, d_out.p, n * (kMapPlanes + kMapRbf) * sizeof(float)));
+  DNAB_RETURN_IF(download(ctx, out, d_out.p, n * (kMapPlanes + kMapRbf + 1) * sizeof(float)));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DNAB_OK;
 }

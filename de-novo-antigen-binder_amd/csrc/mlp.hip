@@ -265,7 +265,7 @@ __global__ void k_transpose_bf16(const uint16_t* __restrict__ W, uint32_t K, uin
 // W given transposed (WT[256][K], k contiguous, read from global memory in
 // the MFMA B layout). Wave w computes columns 64w..64w+63 as 4 x 4 blocks.
 template <int K, int IN_STRIDE>
-__device__ inline void infer_layer(const uint16_t* __restrict__ in, const uint16_t* __restrict__ WT,
+__device__ __attribute__((always_inline)) inline void infer_layer(const uint16_t* __restrict__ in, const uint16_t* __restrict__ WT,
                                    const float* __restrict__ bias, uint16_t* __restrict__ out,
                                    uint32_t lane, uint32_t wave) {
   const int n0 = static_cast<int>(wave) * 64;
@@ -340,13 +340,18 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(InferArgs g) {
   __syncthreads();
   infer_layer<kHid, kHStride>(sB, g.wt3, g.b3, sA, lane, wave);
   __syncthreads();
+  // w4 (lane-strided) and b4 held in registers across the rows
+  float w4[kHid / kWave];
+#pragma unroll
+  for (int q = 0; q < kHid / kWave; ++q) w4[q] = bf16_to_f32(g.w4[lane + q * kWave]);
+  const float b4 = g.b4[0];
   for (uint32_t q = 0; q < kInferRows / 4; ++q) {
     const uint32_t m = wave * (kInferRows / 4) + q;
     float s = 0.0f;
-    for (uint32_t n = lane; n < static_cast<uint32_t>(kHid); n += kWave)
-      s += bf16_to_f32(sA[m * kHStride + n]) * bf16_to_f32(g.w4[n]);
+#pragma unroll
+    for (int e = 0; e < kHid / kWave; ++e) s += bf16_to_f32(sA[m * kHStride + lane + e * kWave]) * w4[e];
     s = wave_reduce_f32<OpAdd>(s);
-    if (lane == 0 && r0 + m < g.M) g.y[r0 + m] = s + g.b4[0];
+    if (lane == 0 && r0 + m < g.M) g.y[r0 + m] = s + b4;
   }
 }
 

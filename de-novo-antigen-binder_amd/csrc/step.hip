@@ -284,13 +284,14 @@ void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* fina
 // ---- selection of the m smallest scores (unordered, in index order) -----------
 //
 // A block of 1024 threads holds up to 1024 * KPT entries (thread t the
-// consecutive slots t * KPT ...). It finds the m-th smallest key by a
-// bitwise binary search with block-wide counts (T grows bit by bit while
-// fewer than m keys lie below it), then writes, in slot order, every entry
-// with a key below T and the first ties with T until m are taken. Keys are
-// the scores' order-preserving integer images (NaN last). Stage inputs are
-// scores (slot = index) or the previous stage's per-block lists (block b's
-// entries at b * m_in, count cnt_in[b]).
+// consecutive slots t * KPT ...). It finds the m-th smallest key digit by
+// digit (eight 4-bit digits from the top: a histogram of the digit among the
+// keys that match the digits found so far, per wave in LDS, then the digit
+// where the running count reaches the rank still needed), then writes, in
+// slot order, every entry with a key below it and the first ties with it
+// until m are taken. Keys are the scores' order-preserving integer images
+// (NaN last). Stage inputs are scores (slot = index) or the previous stage's
+// per-block lists (block b's entries at b * m_in, count cnt_in[b]).
 constexpr uint32_t kSelBlock = 1024;
 constexpr uint32_t kSelNone = 0xFFFFFFFFu;
 
@@ -364,24 +365,46 @@ k_select_smallest(const float* __restrict__ scores, const uint32_t* __restrict__
   }
   const uint32_t mm = min(m, sel_block_sum(nv, s_w));
   if (mm > 0) {
-    uint32_t T = 0;
-    for (int b = 31; b >= 0; --b) {
-      const uint32_t cand = T | (1u << b);
-      uint32_t c = 0;
+    __shared__ uint32_t s_hist[kSelBlock / kWave][16];
+    __shared__ uint32_t s_sel[2];
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    uint32_t T = 0, need = mm;  // the digits found so far; the rank still needed among their keys
+    for (int sh = 28; sh >= 0; sh -= 4) {
+      const uint32_t hi = sh == 28 ? 0u : (0xFFFFFFFFu << (sh + 4));
+      if (lane < 16) s_hist[wave][lane] = 0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-      for (int q = 0; q < KPT; ++q) c += (idx[q] != kSelNone && key[q] < cand) ? 1u : 0u;
-      if (sel_block_sum(c, s_w) < mm) T = cand;
+      for (int q = 0; q < KPT; ++q)
+        if (idx[q] != kSelNone && (key[q] & hi) == (T & hi)) atomicAdd(&s_hist[wave][(key[q] >> sh) & 15u], 1u);
+      __syncthreads();
+      if (wave == 0) {
+        uint32_t tot = 0;
+        if (lane < 16)
+          for (uint32_t w = 0; w < kSelBlock / kWave; ++w) tot += s_hist[w][lane];
+        uint32_t incl = tot;
+        for (int o = 1; o < 16; o <<= 1) {
+          const uint32_t u = __shfl_up(incl, o, 64);
+          if (lane >= static_cast<uint32_t>(o)) incl += u;
+        }
+        // the first digit whose running count reaches the rank needed
+        const uint64_t reach = ballot64(lane < 16 && incl >= need);
+        const int d = __builtin_ctzll(reach);  // exists: the keys matching T number >= need
+        if (lane == static_cast<uint32_t>(d)) {
+          s_sel[0] = T | (static_cast<uint32_t>(d) << sh);
+          s_sel[1] = need - (incl - tot);
+        }
+      }
+      __syncthreads();
+      T = s_sel[0];
+      need = s_sel[1];
     }
-    // T is the mm-th smallest key: all below it, then ties in slot order
-    uint32_t nlt = 0, neq = 0;
+    // T is the mm-th smallest key and `need` its rank among the keys equal to
+    // it: all keys below T, then the first `need` ties in slot order
+    uint32_t neq = 0;
 #pragma unroll
-    for (int q = 0; q < KPT; ++q) {
-      nlt += (idx[q] != kSelNone && key[q] < T) ? 1u : 0u;
-      neq += (idx[q] != kSelNone && key[q] == T) ? 1u : 0u;
-    }
-    uint32_t lt_all;
-    (void)sel_block_scan(nlt, s_w, &lt_all);
-    const uint32_t need = mm - lt_all;
+    for (int q = 0; q < KPT; ++q) neq += (idx[q] != kSelNone && key[q] == T) ? 1u : 0u;
     uint32_t eq_all;
     uint32_t eq_rank = sel_block_scan(neq, s_w, &eq_all);
     uint32_t ntake = 0;
@@ -415,10 +438,10 @@ int select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m, 
                     DevBuf<uint32_t>& scratch) {
   if (m == 0 || m > 2048) return dnab_set_error(ctx, DNAB_ERR_INVALID, "select: m must be 1..2048");
   if (n == 0) return DNAB_OK;
-  constexpr uint32_t kChunk = kSelBlock * 4;  // stage inputs per block
+  constexpr uint32_t kChunk = kSelBlock * 8;  // stage inputs per block
   const uint32_t nb1 = (n + kChunk - 1) / kChunk;
   if (nb1 == 1) {
-    hipLaunchKernelGGL((k_select_smallest<4, true>), dim3(1), dim3(kSelBlock), 0, ctx->stream, scores,
+    hipLaunchKernelGGL((k_select_smallest<8, true>), dim3(1), dim3(kSelBlock), 0, ctx->stream, scores,
                        nullptr, nullptr, nullptr, 1u, n, m, nullptr, out_idx, nullptr);
     DNAB_HIP_CHECK(ctx, hipGetLastError());
     return DNAB_OK;
@@ -429,18 +452,18 @@ int select_smallest(dnab_ctx* ctx, const float* scores, uint32_t n, uint32_t m, 
   uint32_t* kbuf[2] = {scratch.p, scratch.p + cap};
   uint32_t* ibuf[2] = {scratch.p + 2 * cap, scratch.p + 3 * cap};
   uint32_t* cbuf[2] = {scratch.p + 4 * cap, scratch.p + 4 * cap + nb1 + 1};
-  hipLaunchKernelGGL((k_select_smallest<4, true>), dim3(nb1), dim3(kSelBlock), 0, ctx->stream, scores,
+  hipLaunchKernelGGL((k_select_smallest<8, true>), dim3(nb1), dim3(kSelBlock), 0, ctx->stream, scores,
                      nullptr, nullptr, nullptr, 1u, n, m, kbuf[0], ibuf[0], cbuf[0]);
   uint32_t slots = nb1 * m, cur = 0;
-  while (slots > kChunk) {  // more chunked stages (m <= kChunk / 2: each at least halves)
+  while (slots > kChunk) {  // more chunked stages (m <= kChunk / 4: each at least quarters)
     const uint32_t nb = (slots + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL((k_select_smallest<4, false>), dim3(nb), dim3(kSelBlock), 0, ctx->stream,
+    hipLaunchKernelGGL((k_select_smallest<8, false>), dim3(nb), dim3(kSelBlock), 0, ctx->stream,
                        nullptr, kbuf[cur], ibuf[cur], cbuf[cur], m, slots, m, kbuf[cur ^ 1],
                        ibuf[cur ^ 1], cbuf[cur ^ 1]);
     cur ^= 1;
     slots = nb * m;
   }
-  hipLaunchKernelGGL((k_select_smallest<4, false>), dim3(1), dim3(kSelBlock), 0, ctx->stream,
+  hipLaunchKernelGGL((k_select_smallest<8, false>), dim3(1), dim3(kSelBlock), 0, ctx->stream,
                      nullptr, kbuf[cur], ibuf[cur], cbuf[cur], m, slots, m, nullptr, out_idx, nullptr);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   return DNAB_OK;

@@ -145,7 +145,7 @@ class TrainParamsC(C.Structure):
 
 
 FEATURE_DIM = 32
-MAP_VALUES = 17  # per grid point: 7 LJ classes, potential, hydrophobic, 8 radial densities
+MAP_VALUES = 18  # per grid point: 7 LJ classes, potential, hydrophobic, 8 radial densities, cell occupancy
 MLP_HIDDEN = 256
 
 assert C.sizeof(GpuAtom) == 32

@@ -320,8 +320,9 @@ class DeviceAntigen:
         return {"spacing": h.value, "origin": o, "dims": d, "bytes": b.value}
 
     def map_points(self, ijk) -> np.ndarray:
-        """Map values at grid points ijk [n, 3]: [n, 17] (7 LJ classes,
-        potential, hydrophobic, 8 radial densities)."""
+        """Map values at grid points ijk [n, 3]: [n, 18] (7 LJ classes,
+        potential, hydrophobic, 8 radial densities, the occupancy bit of the
+        cell whose lower corner the point is)."""
         ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
         out = np.zeros((max(len(ijk), 1), N.MAP_VALUES), np.float32)
         self.ctx.check(N.lib.dnab_antigen_map_points(self.ctx.handle, self._h,

@@ -686,7 +686,9 @@ int dnab_population_read_scores(dnab_ctx* ctx, const dnab_population* pop, uint3
  * map energy summed per atom class (7 LJ classes, hydrophobic atoms their
  * own), the Coulomb sums over positive and negative atoms and the
  * hydrophobic sum (asinh(x/2)), the total map energy (asinh(x/10)), log1p
- * counts of clashing (LJ > 1), contacting (LJ < -0.05) and in-map atoms, and
+ * counts of clashing (LJ > 1), contacting (LJ < -0.05) and near atoms (in a
+ * cell with an antigen atom within the cutoff of a corner; the others read
+ * only zeros and are skipped), and
  * the radial densities summed over N/O atoms and over the others (log1p).
  * Model: 32 -> 256 -> 256 -> 256 -> 1, SiLU, bf16 weights and activations,
  * FP32 accumulation (MFMA), fp32 master weights, Adam. Target: asinh(E / s)
@@ -700,8 +702,10 @@ int dnab_antigen_build_maps(dnab_ctx* ctx, dnab_antigen* ag, float spacing);
 /* spacing 0 and dims 0 when no maps are built; bytes of device memory held. */
 int dnab_antigen_map_info(const dnab_antigen* ag, float* spacing, float origin[3], int32_t dims[3],
                           size_t* bytes);
-/* Map values at n grid points ijk[3 n] (test hook): out[n][17] = 7 LJ
- * classes, potential, hydrophobic, 8 radial densities. */
+/* Map values at n grid points ijk[3 n] (test hook): out[n][18] = 7 LJ
+ * classes, potential, hydrophobic, 8 radial densities (NaN off the grid of
+ * every second point), and the occupancy bit of the cell whose lower corner
+ * the point is (NaN on the upper faces). */
 int dnab_antigen_map_points(dnab_ctx* ctx, const dnab_antigen* ag, const int32_t* ijk, size_t n,
                             float* out);
 /* Per candidate the map energy (the sum the features' total carries): the

@@ -33,7 +33,7 @@ RBF_MU0, RBF_DMU, RBF_SIGMA, N_RBF = 2.0, 1.0, 0.75, 8
 MAP_CLAMP = 60000.0
 CLASH_E, CONTACT_E = 1.0, -0.05
 N_LJ = 7
-F_COUL_POS, F_COUL_NEG, F_HYD, F_TOTAL, F_CLASH, F_CONTACT, F_INMAP = 8, 9, 10, 11, 12, 13, 14
+F_COUL_POS, F_COUL_NEG, F_HYD, F_TOTAL, F_CLASH, F_CONTACT, F_NEAR = 8, 9, 10, 11, 12, 13, 14
 F_RBF_POLAR, F_RBF_APOLAR = 16, 24
 POLAR_CLASSES = (2, 3, 4, 5)   # N*, O/O2, OH, OS
 
@@ -90,7 +90,7 @@ def map_values(ag, ijk, origin, h, cutoff, classes, hydrophobic_flag, round16=Tr
     rmh = ag["r_min_half"].astype(np.float64)
     eps = ag["epsilon"].astype(np.float64)
     hyd = np.asarray(hydrophobic_flag, bool)
-    out = np.zeros((len(P), 2 + len(classes) + N_RBF))
+    out = np.zeros((len(P), 3 + len(classes) + N_RBF))
     mu = RBF_MU0 + RBF_DMU * np.arange(N_RBF)
     for q in range(len(P)):
         d = P[q][None, :] - A
@@ -104,9 +104,12 @@ def map_values(ag, ijk, origin, h, cutoff, classes, hydrophobic_flag, round16=Tr
             out[q, c] = (np.sqrt(float(ec) * eps[sel]) * (s6 * s6 - 2.0 * s6)).sum()
         out[q, len(classes)] = (q332[sel] / r).sum()
         out[q, len(classes) + 1] = -0.5 * (hyd[sel] & (r2[sel] < np.float32(36.0))).sum()
-        out[q, len(classes) + 2:] = np.exp(-(r[:, None] - mu[None, :]) ** 2
-                                            / (2 * RBF_SIGMA ** 2)).sum(0)
-    return fp16_round(out) if round16 else out
+        out[q, len(classes) + 2:len(classes) + 2 + N_RBF] = np.exp(
+            -(r[:, None] - mu[None, :]) ** 2 / (2 * RBF_SIGMA ** 2)).sum(0)
+        out[q, -1] = 1.0 if sel.any() else 0.0   # an antigen atom passes the predicate
+    if round16:
+        out[:, :-1] = fp16_round(out[:, :-1])
+    return out
 
 
 def fp16_round(v):
@@ -160,7 +163,9 @@ def features(atoms, origin, h, dims, lookup_values, classes):
     """Features of one candidate (float64 before bf16). atoms: dict of the
     candidate's x, y, z, charge, r_min_half, epsilon, hydrophobic; the map
     grid (origin, spacing, points per axis); lookup_values(ijk [m, 3]) ->
-    [m, 17] map values at grid points. An atom outside the map adds nothing.
+    [m, 18] map values at grid points (the last, at the cell's lower corner,
+    the cell's occupancy: an atom in a cold cell adds nothing, nor does one
+    outside the map).
     Returns (row[32] transformed, total map energy)."""
     raw = np.zeros(D)
     n = len(atoms["x"])
@@ -169,6 +174,8 @@ def features(atoms, origin, h, dims, lookup_values, classes):
         if lk is None:
             continue
         i, frac, near = lk
+        if lookup_values(i[None, :])[0, -1] == 0:
+            continue                                # a cold cell: zeros only
         lc = atom_class(atoms["r_min_half"][a], atoms["epsilon"][a], classes)
         hydp = atoms["hydrophobic"][a] != 0
         fc = N_LJ if hydp else lc
@@ -176,7 +183,7 @@ def features(atoms, origin, h, dims, lookup_values, classes):
         elj = trilinear(vals[:, lc], frac)
         phi = trilinear(vals[:, N_LJ], frac)
         hh = trilinear(vals[:, N_LJ + 1], frac)
-        rb = lookup_values(near[None, :])[0, N_LJ + 2:]
+        rb = lookup_values(near[None, :])[0, N_LJ + 2:N_LJ + 2 + N_RBF]
         q = float(atoms["charge"][a])
         ec = q * phi
         eh = hh if hydp else 0.0
@@ -189,7 +196,7 @@ def features(atoms, origin, h, dims, lookup_values, classes):
         raw[F_TOTAL] += elj + ec + eh
         raw[F_CLASH] += elj > CLASH_E
         raw[F_CONTACT] += elj < CONTACT_E
-        raw[F_INMAP] += 1
+        raw[F_NEAR] += 1
         if fc in POLAR_CLASSES:
             raw[F_RBF_POLAR:F_RBF_POLAR + N_RBF] += rb
         else:
@@ -197,7 +204,7 @@ def features(atoms, origin, h, dims, lookup_values, classes):
     row = np.zeros(D)
     row[:F_HYD + 1] = np.arcsinh(raw[:F_HYD + 1] * 0.5)
     row[F_TOTAL] = np.arcsinh(raw[F_TOTAL] * 0.1)
-    row[F_CLASH:F_INMAP + 1] = np.log1p(raw[F_CLASH:F_INMAP + 1])
+    row[F_CLASH:F_NEAR + 1] = np.log1p(raw[F_CLASH:F_NEAR + 1])
     row[F_RBF_POLAR:] = np.log1p(raw[F_RBF_POLAR:])
     return row, raw[F_TOTAL]
 

@@ -58,12 +58,22 @@ def test_map_points_vs_oracle(ctx):
     ijk[::2] -= ijk[::2] % 2            # half the points on the radial-density grid
     got = dag.map_points(ijk).astype(np.float64)
     ref = M.map_values(_ag_dict(a), ijk, origin, M.MAP_H, w.cutoff, _cls(), a.hydrophobic)
+    nv = M.N_LJ + 2 + M.N_RBF
     coarse = (ijk % 2 == 0).all(1)
-    assert np.isnan(got[~coarse, M.N_LJ + 2:]).all() and not np.isnan(got[coarse]).any()
-    got[~coarse, M.N_LJ + 2:] = ref[~coarse, M.N_LJ + 2:]
-    bad = np.abs(got - ref) > 2 ** -10 * np.abs(ref) + 0.02
-    assert not bad.any(), (np.argwhere(bad)[:5], got[bad][:5], ref[bad][:5])
+    assert np.isnan(got[~coarse, M.N_LJ + 2:nv]).all() and not np.isnan(got[coarse, :nv]).any()
+    got[~coarse, M.N_LJ + 2:nv] = ref[~coarse, M.N_LJ + 2:nv]
+    bad = np.abs(got[:, :nv] - ref[:, :nv]) > 2 ** -10 * np.abs(ref[:, :nv]) + 0.02
+    assert not bad.any(), (np.argwhere(bad)[:5], got[:, :nv][bad][:5], ref[:, :nv][bad][:5])
     assert (np.abs(ref[:, :M.N_LJ]) == M.MAP_CLAMP).any()   # some points sit in a wall
+    # a cell's occupancy bit: some corner has an antigen atom within the cutoff
+    lower = (ijk < dims - 1).all(1)
+    assert not np.isnan(got[lower, -1]).any() and np.isnan(got[~lower, -1]).all()
+    cells = ijk[lower][:60]
+    corners = (cells[:, None, :] + M.CORNERS[None]).reshape(-1, 3)
+    flags = M.map_values(_ag_dict(a), corners, origin, M.MAP_H, w.cutoff, _cls(),
+                         a.hydrophobic)[:, -1].reshape(-1, 8)
+    assert np.array_equal(got[lower][:60, -1], flags.max(1))
+    assert 0 < flags.max(1).sum() < len(cells)              # both kinds sampled
 
 
 def test_features_vs_oracle(ctx):
@@ -97,8 +107,8 @@ def test_features_vs_oracle(ctx):
         ref = M.bf16_round(row.astype(np.float32))
         assert (np.abs(X[c] - ref) <= 2 ** -7 * np.abs(ref) + 0.01).all(), (c, np.abs(X[c] - ref).max())
         assert S[c] == pytest.approx(total, rel=1e-4, abs=1e-3)
-        n_in += row[M.F_INMAP] > 0
-    assert n_in == w.pop.n_cand
+        n_in += row[M.F_NEAR] > 0
+    assert n_in >= w.pop.n_cand // 2
 
 
 def test_map_energy_tracks_exact_energy(ctx):
