@@ -146,6 +146,7 @@ struct AgView {
   const AgAtom* atoms;         // grid-sorted
   const float2* raw;           // (q, epsilon), grid-sorted, for the FP64 kernel
   const uint32_t* cell_start;  // Morton cell ranges
+  const float4* cell_box;      // per Morton code: its atoms' box (lo, hi)
   int32_t kmin[3];
   int32_t kmax[3];
   int bits;
@@ -272,6 +273,9 @@ struct dnab_antigen {
   float bbox_hi[3] = {0, 0, 0};
   dnab::DevBuf<dnab::AgAtom> atoms;
   dnab::DevBuf<float2> raw;
+  // per Morton code of the grid: the box of the cell's atoms (lo, hi; an
+  // empty cell +inf, -inf)
+  dnab::DevBuf<float4> cell_box;
   size_t n_ca = 0;
   dnab::DevBuf<float> ca_x, ca_y, ca_z, ca_q, ca_eps, ca_sig;
   dnab::DevBuf<uint8_t> ca_hyd, ca_aa;

@@ -116,6 +116,27 @@ __global__ void k_ag_gather(const uint32_t* __restrict__ sorted, uint32_t n,
   raw[s] = make_float2(q[i], eps[i]);
 }
 
+// The box of each grid cell's atoms (thread per Morton code): lo, hi, from
+// the grid-sorted records; +inf, -inf for an empty cell.
+__global__ void k_cell_box(const AgAtom* __restrict__ atoms, const uint32_t* __restrict__ cell_start,
+                           uint64_t ncodes, float4* __restrict__ box) {
+  const uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= ncodes) return;
+  const float inf = __builtin_inff();
+  float4 lo = make_float4(inf, inf, inf, 0.0f), hi = make_float4(-inf, -inf, -inf, 0.0f);
+  for (uint32_t i = cell_start[c]; i < cell_start[c + 1]; ++i) {
+    const AgAtom& a = atoms[i];
+    lo.x = fminf(lo.x, a.x);
+    lo.y = fminf(lo.y, a.y);
+    lo.z = fminf(lo.z, a.z);
+    hi.x = fmaxf(hi.x, a.x);
+    hi.y = fmaxf(hi.y, a.y);
+    hi.z = fmaxf(hi.z, a.z);
+  }
+  box[2 * c] = lo;
+  box[2 * c + 1] = hi;
+}
+
 // ---- candidate-atom spatial sort ---------------------------------------------------
 // count (one wave per candidate, or the step kernel for the next step) ->
 // scan (one workgroup: exclusive starts per antigen segment, each segment
@@ -1416,6 +1437,15 @@ int dnab_antigen_create(dnab_ctx* ctx, const dnab_antigen_desc* d, dnab_antigen*
   }
   if (hipGetLastError() != hipSuccess)
     return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "antigen: gather launch failed"));
+  {
+    const uint64_t nc = g.ncodes;  // (an empty grid has one empty cell)
+    if (ag->cell_box.alloc(2 * nc) != hipSuccess)
+      return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "antigen: allocation failed"));
+    hipLaunchKernelGGL(k_cell_box, dim3(static_cast<uint32_t>((nc + 255) / 256)), dim3(256), 0,
+                       ctx->stream, ag->atoms.p, g.cell_start.p, nc, ag->cell_box.p);
+    if (hipGetLastError() != hipSuccess)
+      return fail(dnab_set_error(ctx, DNAB_ERR_HIP, "antigen: cell box launch failed"));
+  }
   if (d->ca_amino_acid)
     for (size_t i = 0; i < d->ca.n; ++i)
       if (d->ca_amino_acid[i] >= 20)
@@ -1553,6 +1583,7 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
     v.atoms = ag->atoms.p;
     v.raw = ag->raw.p;
     v.cell_start = ag->grid.cell_start.p;
+    v.cell_box = ag->cell_box.p;
     for (int k = 0; k < 3; ++k) {
       v.kmin[k] = ag->grid.kmin[k];
       v.kmax[k] = ag->grid.kmax[k];

@@ -45,12 +45,16 @@ constexpr uint32_t kTrajMaxCpw = 16;      // candidates per workgroup
 constexpr uint16_t kTrajPad = 0xFFFFu;    // residue slot of a padding atom
 constexpr uint32_t kTileChunks = 192;     // chunk list entries per tile
 constexpr uint32_t kTileOverflow = 0xFFFFFFFFu;
+// The fused loop lists a tile's chunks for its box widened by this much (A),
+// and lists again when the box leaves the widened one.
+constexpr float kListMargin = 1.0f;
 constexpr uint32_t kTrajMaxAgAtoms = 1u << 26;  // a chunk entry holds (first atom << 6 | len - 1)
 constexpr size_t kTrajLdsLimit = 160 * 1024;
 constexpr float kTrajErrPerAbs = 32.0f / 16777216.0f;  // 32 ulp of f32 per unit of sum |e|
 
 struct TrajTile {
   float lo[3], hi[3];
+  float llo[3], lhi[3];  // the box the chunk list was made for
   int32_t k0[3];
   int32_t nx, ny, nz, nxy, ncell;
   uint32_t nchunk;  // listed chunks, or kTileOverflow
@@ -73,6 +77,7 @@ struct TrajCand {
 struct TrajAg {
   const AgAtom* atoms;
   const uint32_t* cell_start;
+  const float4* cell_box;
   float excl_lo[3], excl_hi[3];
   float rc, rc2, inv;
   int32_t kmin[3], kmax[3];
@@ -416,6 +421,7 @@ k_trajectory(TrajArgs g) {
       TrajAg& G = s_ag[j];
       G.atoms = V.atoms;
       G.cell_start = V.cell_start;
+      G.cell_box = V.cell_box;
       for (int d = 0; d < 3; ++d) {
         G.excl_lo[d] = V.excl_lo[d];
         G.excl_hi[d] = V.excl_hi[d];
@@ -433,7 +439,11 @@ k_trajectory(TrajArgs g) {
     }
     const uint32_t slots = (a_off + kWave - 1) & ~(kWave - 1);
     for (uint32_t t = 0; t < slots / kWave; ++t) {
-      s_tile[t].nx = INT32_MIN;  // no chunk list yet (no cell range has this width)
+      // no chunk list yet: no box lies inside an empty one
+      for (int d = 0; d < 3; ++d) {
+        s_tile[t].llo[d] = __builtin_inff();
+        s_tile[t].lhi[d] = -__builtin_inff();
+      }
       s_tile[t].nchunk = 0;
     }
     s_ncand = nc;
@@ -551,74 +561,96 @@ k_trajectory(TrajArgs g) {
       const float hiy = wave_reduce_f32<OpMax>(valid ? P.y : -inf);
       const float hiz = wave_reduce_f32<OpMax>(valid ? P.z : -inf);
       const bool any = ballot64(valid) != 0;
-      // cells that can hold an antigen atom within rc of the box (cell_key
-      // is monotone, so a slightly widened range is conservative)
-      const float margin = V.rc * 1.001f + 1e-3f;
-      const int32_t kx0 = max(cell_key(lox - margin, V.inv), V.kmin[0]);
-      const int32_t ky0 = max(cell_key(loy - margin, V.inv), V.kmin[1]);
-      const int32_t kz0 = max(cell_key(loz - margin, V.inv), V.kmin[2]);
-      const int32_t kx1 = min(cell_key(hix + margin, V.inv), V.kmax[0]);
-      const int32_t ky1 = min(cell_key(hiy + margin, V.inv), V.kmax[1]);
-      const int32_t kz1 = min(cell_key(hiz + margin, V.inv), V.kmax[2]);
-      // (a tile without a valid atom has an empty box and no cells)
-      const int32_t nx = any ? kx1 - kx0 + 1 : 0;
-      const int32_t ny = any ? ky1 - ky0 + 1 : 0;
-      const int32_t nz = any ? kz1 - kz0 + 1 : 0;
-      const int32_t nxy = nx * ny;
-      const int32_t ncell = (nx > 0 && ny > 0 && nz > 0) ? nxy * nz : 0;
+      // The chunk list is made for the tile box widened by kListMargin, and
+      // holds the chunks of the cells whose atom box lies within rc of that
+      // box. It stands while the tile box stays inside: an antigen atom
+      // within rc of the tile box is then within rc of the listing box, and
+      // so is its cell's box (the same rounded differences and sums, each
+      // monotone in its operands), so the list holds every chunk with an atom
+      // the pair phase stages. (The antigen does not move.)
+      const TrajTile& tc = s_tile[t];
+      const bool same = any && lox >= tc.llo[0] && loy >= tc.llo[1] && loz >= tc.llo[2] &&
+                        hix <= tc.lhi[0] && hiy <= tc.lhi[1] && hiz <= tc.lhi[2];
       uint32_t* desc = s_desc + t * kTileChunks;
       uint32_t nt = 0;
       bool over = false;
-      // the antigen does not move: the list stands while the cell range does
-      const TrajTile& tc = s_tile[t];
-      const bool same = tc.nx == nx && tc.ny == ny && tc.nz == nz && tc.k0[0] == kx0 &&
-                        tc.k0[1] == ky0 && tc.k0[2] == kz0;
       if (same) {
         nt = tc.nchunk;
         over = nt == kTileOverflow;
-      }
-      for (int32_t cb0 = 0; !same && cb0 < ncell; cb0 += kWave) {
-        // lane = a cell, in (kz, ky, kx) order
-        const int32_t cl = cb0 + static_cast<int32_t>(lane);
-        uint32_t cb = 0, ce = 0;
-        if (cl < ncell) {
-          const int32_t iz = cl / nxy, rem = cl - iz * nxy;
-          const int32_t iy = rem / nx, ix = rem - iy * nx;
-          const uint32_t code = morton3(static_cast<uint32_t>(kx0 + ix - V.kmin[0]),
-                                        static_cast<uint32_t>(ky0 + iy - V.kmin[1]),
-                                        static_cast<uint32_t>(kz0 + iz - V.kmin[2]));
-          cb = V.cell_start[code];
-          ce = V.cell_start[code + 1];
-        }
-        const uint32_t nch = (ce - cb + kWave - 1) / kWave;
-        uint32_t incl = nch;
-        for (int o = 1; o < 64; o <<= 1) {
-          const uint32_t u = __shfl_up(incl, o, 64);
-          if (lane >= static_cast<uint32_t>(o)) incl += u;
-        }
-        const uint32_t tot = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
-        if (!over && nt + tot <= kTileChunks) {
-          const uint32_t off = nt + incl - nch;
-          for (uint32_t m = 0; m < nch; ++m) {
-            const uint32_t f = cb + kWave * m;
-            desc[off + m] = (f << 6) | (min(static_cast<uint32_t>(kWave), ce - f) - 1u);
+      } else {
+        const float inf = __builtin_inff();
+        const float llx = any ? lox - kListMargin : inf, lly = any ? loy - kListMargin : inf,
+                    llz = any ? loz - kListMargin : inf;
+        const float lhx = any ? hix + kListMargin : -inf, lhy = any ? hiy + kListMargin : -inf,
+                    lhz = any ? hiz + kListMargin : -inf;
+        // cells that can hold an antigen atom within rc of the listing box
+        // (cell_key is monotone, so a slightly widened range is conservative)
+        const float margin = V.rc * 1.001f + 1e-3f;
+        const int32_t kx0 = max(cell_key(llx - margin, V.inv), V.kmin[0]);
+        const int32_t ky0 = max(cell_key(lly - margin, V.inv), V.kmin[1]);
+        const int32_t kz0 = max(cell_key(llz - margin, V.inv), V.kmin[2]);
+        const int32_t kx1 = min(cell_key(lhx + margin, V.inv), V.kmax[0]);
+        const int32_t ky1 = min(cell_key(lhy + margin, V.inv), V.kmax[1]);
+        const int32_t kz1 = min(cell_key(lhz + margin, V.inv), V.kmax[2]);
+        // (a tile without a valid atom has an empty box and no cells)
+        const int32_t nx = any ? kx1 - kx0 + 1 : 0;
+        const int32_t ny = any ? ky1 - ky0 + 1 : 0;
+        const int32_t nz = any ? kz1 - kz0 + 1 : 0;
+        const int32_t nxy = nx * ny;
+        const int32_t ncell = (nx > 0 && ny > 0 && nz > 0) ? nxy * nz : 0;
+        for (int32_t cb0 = 0; cb0 < ncell; cb0 += kWave) {
+          // lane = a cell, in (kz, ky, kx) order
+          const int32_t cl = cb0 + static_cast<int32_t>(lane);
+          uint32_t cb = 0, ce = 0;
+          if (cl < ncell) {
+            const int32_t iz = cl / nxy, rem = cl - iz * nxy;
+            const int32_t iy = rem / nx, ix = rem - iy * nx;
+            const uint32_t code = morton3(static_cast<uint32_t>(kx0 + ix - V.kmin[0]),
+                                          static_cast<uint32_t>(ky0 + iy - V.kmin[1]),
+                                          static_cast<uint32_t>(kz0 + iz - V.kmin[2]));
+            cb = V.cell_start[code];
+            ce = V.cell_start[code + 1];
+            const float4 clo = V.cell_box[2 * code], chi = V.cell_box[2 * code + 1];
+            const float dx = fmaxf(fmaxf(llx - chi.x, clo.x - lhx), 0.0f);
+            const float dy = fmaxf(fmaxf(lly - chi.y, clo.y - lhy), 0.0f);
+            const float dz = fmaxf(fmaxf(llz - chi.z, clo.z - lhz), 0.0f);
+            if (!(((dx * dx + dy * dy) + dz * dz) <= V.rc2)) ce = cb;  // (an empty cell fails too)
           }
-        } else {
-          over = true;  // the tile's chunks go through cell items
+          const uint32_t nch = (ce - cb + kWave - 1) / kWave;
+          uint32_t incl = nch;
+          for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if (lane >= static_cast<uint32_t>(o)) incl += u;
+          }
+          const uint32_t tot = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+          if (!over && nt + tot <= kTileChunks) {
+            const uint32_t off = nt + incl - nch;
+            for (uint32_t m = 0; m < nch; ++m) {
+              const uint32_t f = cb + kWave * m;
+              desc[off + m] = (f << 6) | (min(static_cast<uint32_t>(kWave), ce - f) - 1u);
+            }
+          } else {
+            over = true;  // the tile's chunks go through cell items
+          }
+          nt += tot;
         }
-        nt += tot;
+        if (lane == 0) {
+          TrajTile& tb = s_tile[t];
+          tb.llo[0] = llx; tb.llo[1] = lly; tb.llo[2] = llz;
+          tb.lhi[0] = lhx; tb.lhi[1] = lhy; tb.lhi[2] = lhz;
+          tb.k0[0] = kx0; tb.k0[1] = ky0; tb.k0[2] = kz0;
+          tb.nx = nx;
+          tb.ny = ny;
+          tb.nz = nz;
+          tb.nxy = nxy;
+          tb.ncell = ncell;
+          tb.nchunk = over ? kTileOverflow : nt;
+        }
       }
       if (lane == 0) {
         TrajTile& tb = s_tile[t];
         tb.lo[0] = lox; tb.lo[1] = loy; tb.lo[2] = loz;
         tb.hi[0] = hix; tb.hi[1] = hiy; tb.hi[2] = hiz;
-        tb.k0[0] = kx0; tb.k0[1] = ky0; tb.k0[2] = kz0;
-        tb.nx = nx;
-        tb.ny = ny;
-        tb.nz = nz;
-        tb.nxy = nxy;
-        tb.ncell = ncell;
-        tb.nchunk = over ? kTileOverflow : nt;
       }
       for (int q = 0; q < 5; ++q) s_acc[q * AC + k] = 0;
       s_st[k] = 0;
