@@ -178,40 +178,67 @@ __global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict
   sq[m] = r * r;
 }
 
-// Last-layer backward: gw[n] = sum_m A[m][n] dy[m], gb = sum_m dy[m] (thread
-// per column, fixed order); dZ[m][n] = bf16(dy[m] w[n] silu'(Z[m][n])).
-__global__ void k_last_backward(const uint16_t* __restrict__ A, const float* __restrict__ Z,
-                                const uint16_t* __restrict__ w, const float* __restrict__ dy,
-                                uint32_t M, uint32_t H, float* __restrict__ gw,
-                                float* __restrict__ gb, uint16_t* __restrict__ dZ) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+// Column sums over rows for 64 columns per workgroup: wave w of the 16 adds
+// rows w, w + 16, ... of its lane's column, then the 16 partial sums are
+// added in wave order (a fixed order).
+constexpr int kColWaves = 16;
+__device__ inline float colsum_finish(float v, float (*part)[64], uint32_t wave, uint32_t lane) {
+  part[wave][lane] = v;
+  __syncthreads();
+  float s = 0.0f;
+  if (wave == 0)
+    for (int w = 0; w < kColWaves; ++w) s += part[w][lane];
+  return s;
+}
+
+// Last-layer backward: gw[n] = sum_m A[m][n] dy[m], gb = sum_m dy[m];
+// dZ[m][n] = bf16(dy[m] w[n] silu'(Z[m][n])). Workgroup = 64 columns.
+__global__ void __launch_bounds__(kColWaves * 64)
+k_last_backward(const uint16_t* __restrict__ A, const float* __restrict__ Z,
+                const uint16_t* __restrict__ w, const float* __restrict__ dy, uint32_t M,
+                uint32_t H, float* __restrict__ gw, float* __restrict__ gb,
+                uint16_t* __restrict__ dZ) {
+  __shared__ float part[kColWaves][64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t n = blockIdx.x * 64 + lane;
+  float s = 0.0f;
   if (n < H) {
-    float s = 0.0f;
     const float wn = bf16_to_f32(w[n]);
-    for (uint32_t m = 0; m < M; ++m) {
+    for (uint32_t m = wave; m < M; m += kColWaves) {
       const size_t o = static_cast<size_t>(m) * H + n;
-      s += bf16_to_f32(A[o]) * dy[m];
+      const float d = dy[m];
+      s += bf16_to_f32(A[o]) * d;
       const float z = Z[o];
       const float sg = sigmoidf_(z);
-      dZ[o] = f32_to_bf16(dy[m] * wn * (sg * (1.0f + z * (1.0f - sg))));
+      dZ[o] = f32_to_bf16(d * wn * (sg * (1.0f + z * (1.0f - sg))));
     }
-    gw[n] = s;
   }
-  if (n == 0) {
-    float s = 0.0f;
-    for (uint32_t m = 0; m < M; ++m) s += dy[m];
-    gb[0] = s;
+  s = colsum_finish(s, part, wave, lane);
+  if (wave == 0 && n < H) gw[n] = s;
+  if (blockIdx.x == 0) {
+    __syncthreads();
+    float b = 0.0f;
+    for (uint32_t m = threadIdx.x; m < M; m += kColWaves * 64) b += dy[m];
+    b = colsum_finish(b, part, wave, lane);
+    if (wave == 0) {
+      b = wave_reduce_f32<OpAdd>(b);
+      if (lane == 0) gb[0] = b;
+    }
   }
 }
 
 // Column sums of a bf16 [M x N] matrix (bias gradients), fixed order.
-__global__ void k_colsum_bf16(const uint16_t* __restrict__ D, uint32_t M, uint32_t N,
-                              float* __restrict__ out) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// Workgroup = 64 columns.
+__global__ void __launch_bounds__(kColWaves * 64)
+k_colsum_bf16(const uint16_t* __restrict__ D, uint32_t M, uint32_t N, float* __restrict__ out) {
+  __shared__ float part[kColWaves][64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t n = blockIdx.x * 64 + lane;
   float s = 0.0f;
-  for (uint32_t m = 0; m < M; ++m) s += bf16_to_f32(D[static_cast<size_t>(m) * N + n]);
-  out[n] = s;
+  if (n < N)
+    for (uint32_t m = wave; m < M; m += kColWaves) s += bf16_to_f32(D[static_cast<size_t>(m) * N + n]);
+  s = colsum_finish(s, part, wave, lane);
+  if (wave == 0 && n < N) out[n] = s;
 }
 
 // Adam (Kingma & Ba 2015) on the fp32 master weights; refresh bf16 copies.
@@ -445,27 +472,27 @@ void backward(dnab_ctx* ctx, dnab_mlp* n, const uint16_t* X, uint32_t M) {
   float* g = n->g.p;
   hipLaunchKernelGGL(k_loss_grad, dim3(nb(M, 256)), dim3(256), 0, ctx->stream, n->y.p, n->tb.p,
                      M, n->dy.p, n->sq.p);
-  hipLaunchKernelGGL(k_last_backward, dim3(nb(kH, 256)), dim3(256), 0, ctx->stream, n->A3.p,
+  hipLaunchKernelGGL(k_last_backward, dim3(nb(kH, 64)), dim3(kColWaves * 64), 0, ctx->stream, n->A3.p,
                      n->Z3.p, w + ow4, n->dy.p, M, static_cast<uint32_t>(kH), g + ow4, g + ob4,
                      n->dZ3.p);
   // layer 3
   gemm<true, false, kEpiStoreF32>(ctx, n->A2.p, kH, n->dZ3.p, kH, kH, kH, M, g + oW3, nullptr,
                                   kH, nullptr);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(256), 0, ctx->stream, n->dZ3.p, M,
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(nb(kH, 64)), dim3(kColWaves * 64), 0, ctx->stream, n->dZ3.p, M,
                      static_cast<uint32_t>(kH), g + ob3);
   gemm<false, true, kEpiSiluGrad>(ctx, n->dZ3.p, kH, w + oW3, kH, M, kH, kH, n->Z2.p, n->dZ2.p,
                                   kH, nullptr);
   // layer 2
   gemm<true, false, kEpiStoreF32>(ctx, n->A1.p, kH, n->dZ2.p, kH, kH, kH, M, g + oW2, nullptr,
                                   kH, nullptr);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(256), 0, ctx->stream, n->dZ2.p, M,
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(nb(kH, 64)), dim3(kColWaves * 64), 0, ctx->stream, n->dZ2.p, M,
                      static_cast<uint32_t>(kH), g + ob2);
   gemm<false, true, kEpiSiluGrad>(ctx, n->dZ2.p, kH, w + oW2, kH, M, kH, kH, n->Z1.p, n->dZ1.p,
                                   kH, nullptr);
   // layer 1 (no input gradient)
   gemm<true, false, kEpiStoreF32>(ctx, X, kD, n->dZ1.p, kH, kD, kH, M, g + oW1, nullptr, kH,
                                   nullptr);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(256), 0, ctx->stream, n->dZ1.p, M,
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(nb(kH, 64)), dim3(kColWaves * 64), 0, ctx->stream, n->dZ1.p, M,
                      static_cast<uint32_t>(kH), g + ob1);
 }
 
