@@ -223,3 +223,42 @@ def test_multi_antigen_ragged_run(ctx):
     p = langevin_params(n_steps=10, score_every=5, top_k=8)
     idx, ek = pop.run(p)
     assert len(idx) == 8 and (np.diff(ek) >= 0).all()
+
+
+def test_fused_loop_relists_as_tiles_move(ctx):
+    # large clamped steps move the tile boxes out of their listing boxes
+    # (the box widened by 1 A) most steps, so the chunk lists are made again
+    # and again; the loop still ends in the per-step path's bits
+    w = W.config2(n_cand=64)
+    kw = dict(n_steps=12, score_every=4, top_k=8, with_noise=1, seed=21, step=0.5, max_disp=3.0)
+    fused = _run_state(ctx, w, step_kernels=0, **kw)
+    steps = _run_state(ctx, w, step_kernels=1, use_graph=0, **kw)
+    _assert_same(fused, steps)
+    start = (w.pop.atoms.x, w.pop.atoms.y, w.pop.atoms.z)
+    moved = [np.abs(a - b).max() for a, b in zip(fused["coords"], start)]
+    assert max(moved) > 2.0
+
+
+def test_fused_loop_with_an_empty_antigen(ctx):
+    # an antigen without atoms: no cell lists a chunk and every pair sum is
+    # zero; the binders still move under the restraint, both engines agree
+    from dnab.structure import AtomCloud
+    from dnab import DeviceAntigen, DevicePopulation
+    w = W.config2(n_cand=8)
+    z = np.zeros(0, np.float32)
+    empty = AtomCloud(z, z, z, z, z, z, np.zeros(0, np.uint8))
+    ctr = w.antigens[0].ca_center_of_mass()
+    out = []
+    for step_kernels in (0, 1):
+        ag = DeviceAntigen(ctx, empty, cutoff=10.0, center=ctr)
+        pop = DevicePopulation(ctx, [ag], w.pop)
+        p = langevin_params(n_steps=6, score_every=3, top_k=4, restraint_radius=5.0,
+                            step_kernels=step_kernels, use_graph=0)
+        idx, e = pop.run(p)
+        assert (e == 0).all()
+        assert (pop.energies()[0] == 0).all()
+        out.append(pop.coords())
+    for a, b in zip(*out):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    start = (w.pop.atoms.x, w.pop.atoms.y, w.pop.atoms.z)
+    assert any(not np.array_equal(a, b) for a, b in zip(out[0], start))
