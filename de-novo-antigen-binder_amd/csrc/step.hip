@@ -710,23 +710,27 @@ int dnab_population_autotune_trajectory(dnab_ctx* ctx, dnab_population* pop,
   double best_s = INFINITY;
   DNAB_RETURN_IF(trial(best, &best_s));
   // coordinate descent over the shape, one dimension at a time, until a
-  // pass changes nothing (at most three passes)
+  // pass changes nothing (at most three passes). Candidates per group and
+  // the grouping mode form one dimension of 12 pairs: they interact (larger
+  // groups pay off when grouped by position), and one at a time the descent
+  // stops short of such pairs.
   static const uint32_t waves[] = {4, 8, 16}, cpw[] = {1, 2, 4, 8}, chunks[] = {1, 2, 4},
                         cells[] = {2, 8}, bal[] = {0, 1, 2};
-  struct Dim { const uint32_t* v; int n; uint32_t dnab_traj_tuning::*f; };
-  const Dim dims[] = {{waves, 3, &dnab_traj_tuning::waves},
-                      {cpw, 4, &dnab_traj_tuning::cand_per_group},
-                      {chunks, 3, &dnab_traj_tuning::chunks_per_item},
-                      {cells, 2, &dnab_traj_tuning::cells_per_item},
-                      {bal, 3, &dnab_traj_tuning::balance}};
+  struct Dim { const uint32_t* v; int n; uint32_t dnab_traj_tuning::*f; const uint32_t* v2;
+               int n2; uint32_t dnab_traj_tuning::*f2; };
+  const Dim dims[] = {{waves, 3, &dnab_traj_tuning::waves, nullptr, 1, nullptr},
+                      {cpw, 4, &dnab_traj_tuning::cand_per_group, bal, 3, &dnab_traj_tuning::balance},
+                      {chunks, 3, &dnab_traj_tuning::chunks_per_item, nullptr, 1, nullptr},
+                      {cells, 2, &dnab_traj_tuning::cells_per_item, nullptr, 1, nullptr}};
   int rc = DNAB_OK;
   for (int pass = 0; pass < 3 && rc == DNAB_OK; ++pass) {
     bool changed = false;
     for (const Dim& d : dims) {
-      for (int i = 0; i < d.n && rc == DNAB_OK; ++i) {
-        if (best.*d.f == d.v[i]) continue;
+      for (int i = 0; i < d.n * d.n2 && rc == DNAB_OK; ++i) {
         dnab_traj_tuning t = best;
-        t.*d.f = d.v[i];
+        t.*d.f = d.v[i % d.n];
+        if (d.f2) t.*d.f2 = d.v2[i / d.n];
+        if (t.*d.f == best.*d.f && (!d.f2 || t.*d.f2 == best.*d.f2)) continue;
         double s;
         rc = trial(t, &s);
         if (rc == DNAB_OK && s < best_s) {

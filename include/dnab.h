@@ -653,8 +653,8 @@ int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
  * parameters (no reference counterpart): times trial_steps (1..10000) steps
  * of the fused loop, wall clock with the host's regrouping, from the
  * population's coordinates for each shape of a coordinate descent over waves
- * {4, 8, 16}, cand_per_group {1, 2, 4, 8}, chunks_per_item {1, 2, 4},
- * cells_per_item {2, 8} and balance {0, 1, 2} (rebalance_steps kept), and keeps
+ * {4, 8, 16}, the pair (cand_per_group {1, 2, 4, 8}, balance {0, 1, 2}),
+ * chunks_per_item {1, 2, 4} and cells_per_item {2, 8} (rebalance_steps kept), and keeps
  * the fastest in the population (also written to `chosen`, nullable). The
  * coordinates are put back afterwards and no evaluation is current. The
  * shape changes only the speed: every shape gives the same results. */
