@@ -59,6 +59,9 @@ constexpr int kFClash = 12, kFContact = 13, kFNear = 14;
 constexpr int kFRbfPolar = 16;     // [kMapRbf] radial densities at N and O atoms
 constexpr int kFRbfApolar = 24;    // [kMapRbf] ... at the other atoms
 static_assert(kFRbfApolar + kMapRbf == kPrescreenFeatures, "feature layout");
+static_assert(kFCoulNeg == kFCoulPos + 1 && kFHyd == kFCoulPos + 2 && kFTotal == kFCoulPos + 3 &&
+                  kFClash == kFCoulPos + 4 && kFContact == kFCoulPos + 5 && kFNear == kFCoulPos + 6,
+              "the physics features follow the columns in order");
 
 // N* (class 2), O, O2 (3), OH (4), OS (5) in the class order of map_classes
 __device__ inline bool polar_class(int fc) { return fc >= 2 && fc <= 5; }
@@ -399,7 +402,6 @@ __global__ void __launch_bounds__(256) k_map_features(FeatArgs g) {
         in[q] = (V.hot[cell >> 5] >> (cell & 31)) & 1u;  // a cold cell reads zeros
       }
       if (in[q]) {
-        const int32_t i = static_cast<int32_t>(ffx), j = static_cast<int32_t>(ffy), k = static_cast<int32_t>(ffz);
         const uint16_t* rec = V.erec + cell * (kMapRecSlots * 8);
         const int sl = map_slot(lc[q]);
         rl[q] = *reinterpret_cast<const uint4*>(sl < 0 ? V.esul + cell * 8 : rec + sl * 8);
