@@ -305,6 +305,7 @@ struct dnab_population {
   uint32_t traj_waves = 16, traj_cpw = 4, traj_cells = 2, traj_chunks = 2;
   uint32_t traj_balance = 1;
   uint32_t traj_rebalance = 0;   // steps per launch between regroupings (0: one launch)
+  uint32_t traj_engine = 0;      // the engine of step_kernels = 2 (0 fused, 1 per-step kernels)
   dnab::DevBuf<uint32_t> traj_work, traj_group_off, traj_group_cand;
   dnab::DevBuf<unsigned long long> traj_key;  // per candidate, for grouping by position
   std::vector<uint32_t> h_group_off, h_group_cand;  // sources of the group-table copies

@@ -489,6 +489,8 @@ static int validate_params(dnab_ctx* ctx, const dnab_langevin_params* p) {
   if (!std::isfinite(p->step) || !std::isfinite(p->max_disp) || !std::isfinite(p->restraint_k) ||
       !std::isfinite(p->restraint_radius) || !(p->t_hot > 0) || !(p->t_cold > 0))
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: non-finite parameter");
+  if (p->step_kernels < 0 || p->step_kernels > 2)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "langevin: step_kernels must be 0, 1 or 2");
   return DNAB_OK;
 }
 
@@ -638,7 +640,8 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
   pop->masked = false;
   hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, ctx->stream, pop->step_counter.p, 0u);
   const StepParams sp = make_step_params(p);
-  if (!p->step_kernels && trajectory_supported(pop, p)) {
+  const bool fused = p->step_kernels == 0 || (p->step_kernels == 2 && pop->traj_engine == 0);
+  if (fused && trajectory_supported(pop, p)) {
     // every step in one launch, which leaves each scoring's candidate
     // energies in a row of score_e; then the top k of every row at once
     DNAB_ALLOC_CHECK(ctx, pop->score_e.ensure(static_cast<size_t>(rows) * nc));
@@ -741,6 +744,27 @@ int dnab_population_autotune_trajectory(dnab_ctx* ctx, dnab_population* pop,
       }
     }
     if (!changed) break;
+  }
+  // the per-step kernels on the same loop: the engine of step_kernels = 2 is
+  // the faster of the two (the per-step run, best of two, leaves its score
+  // history, which is cleared below)
+  if (rc == DNAB_OK) {
+    dnab_langevin_params sk = lp;
+    sk.step_kernels = 1;
+    sk.count_pairs = 0;
+    double step_s = INFINITY;
+    for (int rep = 0; rep < 2 && rc == DNAB_OK; ++rep) {
+      DNAB_HIP_CHECK(ctx, hipMemcpyAsync(pop->pos.p, saved.p, pos_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      pop->sorted = false;
+      pop->evaluated = false;
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = dnab_langevin_run(ctx, pop, &sk, nullptr, nullptr, nullptr);
+      DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      step_s = std::min(step_s, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    best.engine = step_s < best_s ? 1u : 0u;
+    pop->hist_rows = 0;
   }
   // the winner stays in the population; the coordinates are the caller's
   // again and no evaluation is current

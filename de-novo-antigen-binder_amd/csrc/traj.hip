@@ -1429,6 +1429,7 @@ int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_
   t->cells_per_item = pop->traj_cells;
   t->chunks_per_item = pop->traj_chunks;
   t->rebalance_steps = pop->traj_rebalance;
+  t->engine = pop->traj_engine;
   return DNAB_OK;
 }
 
@@ -1447,12 +1448,15 @@ int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
   if (t->chunks_per_item < 1 || t->chunks_per_item > static_cast<uint32_t>(kLoadAhead))
     return dnab_set_error(ctx, DNAB_ERR_INVALID, "trajectory tuning: chunks_per_item must be 1..%d",
                           kLoadAhead);
+  if (t->engine > 1)
+    return dnab_set_error(ctx, DNAB_ERR_INVALID, "trajectory tuning: engine must be 0 or 1");
   pop->traj_chunks = t->chunks_per_item;
   pop->traj_rebalance = t->rebalance_steps;
   pop->traj_waves = t->waves;
   pop->traj_cells = t->cells_per_item;
   pop->traj_cpw = t->cand_per_group;
   pop->traj_balance = t->balance;
+  pop->traj_engine = t->engine;
   return DNAB_OK;
 }
 

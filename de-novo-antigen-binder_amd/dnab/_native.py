@@ -75,7 +75,7 @@ class PairTuningC(C.Structure):
 class TrajTuningC(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("cand_per_group", C.c_uint32), ("balance", C.c_uint32),
                 ("cells_per_item", C.c_uint32), ("chunks_per_item", C.c_uint32),
-                ("rebalance_steps", C.c_uint32)]
+                ("rebalance_steps", C.c_uint32), ("engine", C.c_uint32)]
 
 
 class ProteinOutC(C.Structure):

@@ -593,13 +593,13 @@ class DevicePopulation:
         N.check(N.lib.dnab_population_get_trajectory_tuning(self._h, C.byref(t)))
         return {"waves": t.waves, "cand_per_group": t.cand_per_group, "balance": t.balance,
                 "cells_per_item": t.cells_per_item, "chunks_per_item": t.chunks_per_item,
-                "rebalance_steps": t.rebalance_steps}
+                "rebalance_steps": t.rebalance_steps, "engine": t.engine}
 
     def set_trajectory_tuning(self, waves: int, cand_per_group: int, balance: int = 1,
                               cells_per_item: int = 2, chunks_per_item: int = 2,
-                              rebalance_steps: int = 0):
+                              rebalance_steps: int = 0, engine: int = 0):
         t = N.TrajTuningC(waves, cand_per_group, balance, cells_per_item, chunks_per_item,
-                          rebalance_steps)
+                          rebalance_steps, engine)
         self.ctx.check(N.lib.dnab_population_set_trajectory_tuning(self.ctx.handle, self._h,
                                                                    C.byref(t)))
 

@@ -521,9 +521,13 @@ typedef struct dnab_langevin_params {
                                step in LDS) when a candidate fits one
                                workgroup (up to ~1,500 atoms);
                                1: one set of population kernels per step
-                               (hipGraph replay with use_graph). Results are
-                               bit-identical; pair statistics agree on pairs
-                               in the cutoff, not on pairs tested. */
+                               (hipGraph replay with use_graph);
+                               2: the engine the population's trajectory
+                               tuning names (dnab_traj_tuning.engine, chosen
+                               by dnab_population_autotune_trajectory).
+                               Results are bit-identical; pair statistics
+                               agree on pairs in the cutoff, not on pairs
+                               tested. */
 } dnab_langevin_params;
 int dnab_langevin_default_params(dnab_langevin_params* p);
 
@@ -645,6 +649,8 @@ typedef struct dnab_traj_tuning {
                                of this many steps, regrouping between them by
                                the work each candidate had in the last, or by
                                the candidates' new positions (0: one launch) */
+  uint32_t engine;          /* the engine a loop with step_kernels = 2 runs:
+                               0 the fused kernel, 1 the per-step kernels */
 } dnab_traj_tuning;
 int dnab_population_get_trajectory_tuning(const dnab_population* pop, dnab_traj_tuning* t);
 int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
@@ -655,9 +661,12 @@ int dnab_population_set_trajectory_tuning(dnab_ctx* ctx, dnab_population* pop,
  * population's coordinates for each shape of a coordinate descent over waves
  * {4, 8, 16}, the pair (cand_per_group {1, 2, 4, 8}, balance {0, 1, 2}),
  * chunks_per_item {1, 2, 4} and cells_per_item {2, 8} (rebalance_steps kept), and keeps
- * the fastest in the population (also written to `chosen`, nullable). The
- * coordinates are put back afterwards and no evaluation is current. The
- * shape changes only the speed: every shape gives the same results. */
+ * the fastest in the population (also written to `chosen`, nullable). It then
+ * times the per-step kernels on the same loop (graph replay when the
+ * parameters ask for it) and sets `engine` to the faster of the two. The
+ * coordinates are put back afterwards, no evaluation is current and the score
+ * history is empty. The shape and the engine change only the speed: every
+ * choice gives the same results. */
 int dnab_population_autotune_trajectory(dnab_ctx* ctx, dnab_population* pop,
                                         const dnab_langevin_params* p, uint32_t trial_steps,
                                         dnab_traj_tuning* chosen);

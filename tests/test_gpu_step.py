@@ -137,7 +137,7 @@ def test_trajectory_tuning_changes_speed_not_results(ctx):
         pop.set_trajectory_tuning(waves, cpw, bal, cells, chunks, reb)
         assert pop.trajectory_tuning() == {"waves": waves, "cand_per_group": cpw, "balance": bal,
                                            "cells_per_item": cells, "chunks_per_item": chunks,
-                                           "rebalance_steps": reb}
+                                           "rebalance_steps": reb, "engine": 0}
         p = langevin_params(restraint_radius=w.restraint_radius[0], **kw)
         idx, e = pop.run(p)
         hi, he = pop.scores()
@@ -151,6 +151,8 @@ def test_trajectory_tuning_changes_speed_not_results(ctx):
         pop.set_trajectory_tuning(12, 4)
     with pytest.raises(RuntimeError):
         pop.set_trajectory_tuning(16, 4, 3)
+    with pytest.raises(RuntimeError):
+        pop.set_trajectory_tuning(16, 4, 1, 2, 2, 0, 2)
 
 
 def test_trajectory_autotune_keeps_state_and_results(ctx):
@@ -175,6 +177,7 @@ def test_trajectory_autotune_keeps_state_and_results(ctx):
     assert t["waves"] in (4, 8, 16) and t["cand_per_group"] in (1, 2, 4, 8)
     assert t["chunks_per_item"] in (1, 2, 4) and t["cells_per_item"] in (2, 8)
     assert t["balance"] in (0, 1, 2) and t["rebalance_steps"] == 0
+    assert t["engine"] in (0, 1)
     for a, b in zip(x0, pop.coords()):
         assert np.array_equal(a, b)
     with pytest.raises(RuntimeError):
@@ -262,3 +265,23 @@ def test_fused_loop_with_an_empty_antigen(ctx):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     start = (w.pop.atoms.x, w.pop.atoms.y, w.pop.atoms.z)
     assert any(not np.array_equal(a, b) for a, b in zip(out[0], start))
+
+
+def test_engine_choice_changes_speed_not_results(ctx):
+    # step_kernels = 2 runs the engine the trajectory tuning names; either
+    # engine ends in the same bits
+    w = W.config2(n_cand=64)
+    kw = dict(n_steps=10, score_every=5, top_k=8, with_noise=1, seed=13)
+    ref = _run_state(ctx, w, step_kernels=0, **kw)
+    for engine in (0, 1):
+        ags, pop = W.device_objects(ctx, w)
+        pop.set_trajectory_tuning(16, 4, 1, 2, 2, 0, engine)
+        assert pop.trajectory_tuning()["engine"] == engine
+        p = langevin_params(restraint_radius=w.restraint_radius[0], step_kernels=2, use_graph=1, **kw)
+        idx, e = pop.run(p)
+        hi, he = pop.scores()
+        st = dict(idx=idx, e=e, coords=pop.coords(), forces=pop.forces(), energy=pop.energies()[0],
+                  err=pop.energies()[1], hist_idx=hi, hist_e=he)
+        _assert_same(ref, st)
+    with pytest.raises(RuntimeError):
+        pop.run(langevin_params(restraint_radius=w.restraint_radius[0], step_kernels=3, **kw))
