@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -28,14 +29,16 @@ template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  bool owned = true;  // false: a view into another buffer (DevSlab)
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = true;
   }
   hipError_t alloc(size_t count) {
     release();
@@ -46,6 +49,39 @@ struct DevBuf {
     return e;
   }
   hipError_t ensure(size_t count) { return count <= n ? hipSuccess : alloc(count); }
+  void view(T* q, size_t count) {
+    release();
+    p = q;
+    n = count;
+    owned = false;
+  }
+};
+
+// Many buffers in one device allocation: carve() plans each (256-byte
+// aligned), alloc() makes the allocation and points every planned DevBuf
+// into it. A free of a large allocation waits for the device and unmaps it
+// (~0.1-0.2 ms each on MI355X), so one slab instead of dozens of buffers
+// saves milliseconds per population. The DevBufs must not outlive the slab;
+// one that later needs more room allocates its own memory.
+struct DevSlab {
+  DevBuf<unsigned char> mem;
+  size_t bytes = 0;
+  std::vector<std::function<void(unsigned char*)>> bind;
+  template <class T>
+  void carve(DevBuf<T>& b, size_t count) {
+    if (count == 0) count = 1;
+    const size_t off = (bytes + 255) & ~static_cast<size_t>(255);
+    bytes = off + count * sizeof(T);
+    DevBuf<T>* pb = &b;
+    bind.push_back([pb, off, count](unsigned char* base) { pb->view(reinterpret_cast<T*>(base + off), count); });
+  }
+  hipError_t alloc() {
+    const hipError_t e = mem.alloc(bytes);
+    if (e != hipSuccess) return e;
+    for (auto& f : bind) f(mem.p);
+    bind.clear();
+    return hipSuccess;
+  }
 };
 
 // Scratch of a multi-pass top-k (step.hip): chunk lengths per pass and two
@@ -286,6 +322,8 @@ struct dnab_antigen {
 
 struct dnab_population {
   dnab_ctx* ctx = nullptr;
+  dnab::DevSlab slab;  // the per-atom and per-tile arrays below, in one allocation
+  dnab::DevBuf<float> pack_soa;  // the caller's SoA, staged for k_pack_atoms
   size_t n_cand = 0, n_res = 0, n_atoms = 0, n_ag = 0;
   std::vector<dnab_antigen*> antigens;
   std::vector<dnab::AgView> views_host;

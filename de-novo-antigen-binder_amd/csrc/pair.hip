@@ -1651,15 +1651,49 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
   if (nr && ((rc = up(ctx, pop->res_atom_off, d->res_atom_off, nr + 1)) ||
              (rc = up(ctx, pop->res_ca, d->res_ca_atom, nr))))
     return fail(rc);
+  // the per-atom and per-tile arrays in one allocation
+  const size_t nf = (na + 63) / 64 * 64;
+  {
+    DevSlab& sl = pop->slab;
+    sl.carve(pop->pack_soa, 6 * nf + (2 * nf + 3) / 4);
+    sl.carve(pop->pos, na);
+    sl.carve(pop->par, na);
+    sl.carve(pop->atom_ag, na);
+    sl.carve(pop->atom_cand, na);
+    sl.carve(pop->atom_res, na);
+    sl.carve(pop->sort_count, static_cast<size_t>(pop->sort_codes) + 4);
+    sl.carve(pop->sort_start, static_cast<size_t>(pop->sort_codes) + 4);
+    sl.carve(pop->sort_code, na);
+    sl.carve(pop->sort_rank, na);
+    sl.carve(pop->perm, pop->tile_cap * kWave);
+    sl.carve(pop->spos, pop->tile_cap * kWave);
+    sl.carve(pop->spar, pop->tile_cap * kWave);
+    sl.carve(pop->ho_rest, pop->tile_cap);
+    sl.carve(pop->ho_list, pop->tile_cap + 4);
+    sl.carve(pop->ho_chunks, pop->tile_cap * kHandoverCap);
+    sl.carve(pop->ho_part, 5 * pop->tile_cap * kWave);
+    sl.carve(pop->ho_status, pop->tile_cap * kWave);
+    sl.carve(pop->ag_tile_start, n_antigens + 1);
+    sl.carve(pop->force, na);
+    sl.carve(pop->e_atom, na);
+    sl.carve(pop->bd_atom, kBdTerms * na);
+    sl.carve(pop->f64_atom, 3 * na);
+    sl.carve(pop->energy, nc);
+    sl.carve(pop->err_bound, nc);
+    sl.carve(pop->breakdown, kBdTerms * nc);
+    sl.carve(pop->disulfide, nc);
+    sl.carve(pop->res_delta, nr);
+    sl.carve(pop->step_counter, 4);
+    sl.carve(pop->pair_stats, 2);
+    sl.carve(pop->cand_mask, nc);
+    sl.carve(pop->cand_status, nc);
+    if (sl.alloc() != hipSuccess)
+      return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
+  }
   {
     // the caller's per-atom SoA, 26 bytes per atom, packed into pos / par
     // and the per-atom owner indices on the device
-    const size_t nf = (na + 63) / 64 * 64;
-    DevBuf<float> soa;
-    if (soa.alloc(6 * nf + (2 * nf + 3) / 4) != hipSuccess || pop->pos.alloc(na) != hipSuccess ||
-        pop->par.alloc(na) != hipSuccess || pop->atom_ag.alloc(na) != hipSuccess ||
-        pop->atom_cand.alloc(na) != hipSuccess || pop->atom_res.alloc(na) != hipSuccess)
-      return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
+    DevBuf<float>& soa = pop->pack_soa;
     const float* src[6] = {d->atoms.x, d->atoms.y, d->atoms.z, d->atoms.charge,
                            d->atoms.r_min_half, d->atoms.epsilon};
     for (int k = 0; k < 6; ++k)
@@ -1676,33 +1710,11 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
                          pop->atom_cand.p, pop->atom_res.p);
       DNAB_HIP_CHECK(ctx, hipGetLastError());
     }
-    // the scratch is freed at the end of this block: order it behind the kernel
-    DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   }
   if (nr && d->res_amino_acid) {  // types checked above (< 20)
     if ((rc = up(ctx, pop->res_aa, d->res_amino_acid, nr))) return fail(rc);
     pop->has_res_aa = true;
   }
-  if (pop->sort_count.alloc(static_cast<size_t>(pop->sort_codes) + 4) != hipSuccess ||
-      pop->sort_start.alloc(static_cast<size_t>(pop->sort_codes) + 4) != hipSuccess ||
-      pop->sort_code.alloc(na) != hipSuccess || pop->sort_rank.alloc(na) != hipSuccess ||
-      pop->perm.alloc(pop->tile_cap * kWave) != hipSuccess ||
-      pop->spos.alloc(pop->tile_cap * kWave) != hipSuccess ||
-      pop->spar.alloc(pop->tile_cap * kWave) != hipSuccess ||
-      pop->ho_rest.alloc(pop->tile_cap) != hipSuccess ||
-      pop->ho_list.alloc(pop->tile_cap + 4) != hipSuccess ||
-      pop->ho_chunks.alloc(pop->tile_cap * kHandoverCap) != hipSuccess ||
-      pop->ho_part.alloc(5 * pop->tile_cap * kWave) != hipSuccess ||
-      pop->ho_status.alloc(pop->tile_cap * kWave) != hipSuccess ||
-      pop->ag_tile_start.alloc(n_antigens + 1) != hipSuccess || pop->force.alloc(na) != hipSuccess ||
-      pop->e_atom.alloc(na) != hipSuccess || pop->bd_atom.alloc(kBdTerms * na) != hipSuccess ||
-      pop->f64_atom.alloc(3 * na) != hipSuccess || pop->energy.alloc(nc) != hipSuccess ||
-      pop->err_bound.alloc(nc) != hipSuccess || pop->breakdown.alloc(kBdTerms * nc) != hipSuccess ||
-      pop->disulfide.alloc(nc) != hipSuccess ||
-      pop->res_delta.alloc(nr) != hipSuccess || pop->step_counter.alloc(4) != hipSuccess ||
-      pop->pair_stats.alloc(2) != hipSuccess || pop->cand_mask.alloc(nc) != hipSuccess ||
-      pop->cand_status.alloc(nc) != hipSuccess)
-    return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
   {
     std::vector<uint8_t> ones(nc ? nc : 1, 1);
     if ((rc = upload(ctx, pop->cand_mask.p, ones.data(), nc))) return fail(rc);
