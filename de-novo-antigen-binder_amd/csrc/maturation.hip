@@ -337,12 +337,21 @@ int dnab_affinity_maturation(dnab_ctx* ctx, dnab_antigen* antigen,
   if (!out) return DNAB_OK;
   if (!out->x || !out->y || !out->z) return DNAB_ERR_INVALID;
   if (n_out > out->atom_capacity) return DNAB_ERR_CAPACITY;
-  size_t o = 0;
-  if (out->res_start) out->res_start[0] = 0;
-  for (size_t a = 0; a < n_ab; ++a) {
+  // each residue's first output atom and its antibody, then the residues'
+  // atoms copied in parallel
+  std::vector<size_t> r_out(n_res + 1, 0);
+  std::vector<uint32_t> r_ab(n_res);
+  for (size_t a = 0; a < n_ab; ++a)
     for (uint32_t g = ab->cand_res_off[a]; g < ab->cand_res_off[a + 1]; ++g) {
+      r_ab[g] = static_cast<uint32_t>(a);
+      r_out[g + 1] = r_out[g] + (c_atom_off[state[g] + 1] - c_atom_off[state[g]]);
+    }
+  if (out->res_start) out->res_start[0] = 0;
+  parallel_for(n_res, [&](size_t g0, size_t g1) {
+    for (size_t g = g0; g < g1; ++g) {
       const uint32_t c = state[g], b = c_atom_off[c], e = c_atom_off[c + 1];
-      const uint32_t local = g - ab->cand_res_off[a];
+      const uint32_t local = static_cast<uint32_t>(g) - ab->cand_res_off[r_ab[g]];
+      size_t o = r_out[g];
       if (out->ca_atom_idx) out->ca_atom_idx[g] = static_cast<uint32_t>(o + (ca[c] - b));
       if (out->amino_acid) out->amino_acid[g] = c_aa[c];
       for (uint32_t i = b; i < e; ++i, ++o) {
@@ -359,7 +368,7 @@ int dnab_affinity_maturation(dnab_ctx* ctx, dnab_antigen* antigen,
       }
       if (out->res_start) out->res_start[g + 1] = static_cast<uint32_t>(o);
     }
-  }
+  });
   timer_host(ctx, "maturation_host_assemble", ms_since(t0));
   return DNAB_OK;
 }

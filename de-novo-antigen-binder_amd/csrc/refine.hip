@@ -1156,19 +1156,22 @@ int dnab_refine(dnab_ctx* ctx, dnab_antigen* ag, const dnab_population_desc* ab,
     temp[s] = lp->t_hot * std::pow(lp->t_cold / lp->t_hot, it / static_cast<float>(lp->schedule_total));
     sigma[s] = lp->with_noise ? lp->noise_base * std::sqrt(temp[s]) : 0.0f;
   }
+  // the call's buffers in one allocation (one free at the end)
   DevBuf<RefTables> d_tab;
   DevBuf<float4> d_slots, d_agpar;
   DevBuf<uint8_t> d_aa, d_rot;
   DevBuf<uint32_t> d_off;
   DevBuf<float> d_temp, d_sigma;
-  DNAB_ALLOC_CHECK(ctx, d_tab.alloc(1));
-  DNAB_ALLOC_CHECK(ctx, d_slots.alloc(slots.size()));
-  DNAB_ALLOC_CHECK(ctx, d_agpar.alloc(n_ag));
-  DNAB_ALLOC_CHECK(ctx, d_aa.alloc(raa.size()));
-  DNAB_ALLOC_CHECK(ctx, d_rot.alloc(rrot.size()));
-  DNAB_ALLOC_CHECK(ctx, d_off.alloc(nc + 1));
-  DNAB_ALLOC_CHECK(ctx, d_temp.alloc(temp.size()));
-  DNAB_ALLOC_CHECK(ctx, d_sigma.alloc(sigma.size()));
+  DevSlab slab;
+  slab.carve(d_tab, 1);
+  slab.carve(d_slots, slots.size());
+  slab.carve(d_agpar, n_ag);
+  slab.carve(d_aa, raa.size());
+  slab.carve(d_rot, rrot.size());
+  slab.carve(d_off, nc + 1);
+  slab.carve(d_temp, temp.size());
+  slab.carve(d_sigma, sigma.size());
+  DNAB_ALLOC_CHECK(ctx, slab.alloc());
   DNAB_RETURN_IF(upload(ctx, d_tab.p, &tab, sizeof(tab)));
   DNAB_RETURN_IF(upload(ctx, d_slots.p, slots.data(), slots.size() * sizeof(float4)));
   DNAB_RETURN_IF(upload(ctx, d_agpar.p, ag_par.data(), n_ag * sizeof(float4)));
