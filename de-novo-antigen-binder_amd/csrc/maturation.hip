@@ -225,8 +225,11 @@ int dnab_affinity_maturation(dnab_ctx* ctx, dnab_antigen* antigen,
   for (size_t c = n_res; c < nc; ++c)
     c_atom_off[c + 1] = c_atom_off[c] + kResidueTopology[c_aa[c]].n_atoms;
   const size_t na = c_atom_off[nc];
-  std::vector<float> x(na), y(na), z(na), q(na), rmh(na), eps(na);
-  std::vector<uint8_t> hyd(na), typ(na);
+  // every element is written below (the input atoms copied, each built
+  // residue's atoms by its build): no zero fill of ~30 B per atom first
+  HostVec<float> x(na), y(na), z(na), q(na), rmh(na), eps(na);
+  HostVec<uint8_t> hyd(na);
+  std::vector<uint8_t> typ(na);  // zero where the input has no types
   std::vector<uint32_t> ca(nc), cand_res_off(nc + 1), cand_ag(nc, 0);
   const bool typed = ab->atoms.atom_type != nullptr;
   std::memcpy(x.data(), ab->atoms.x, n_in * sizeof(float));
