@@ -1687,7 +1687,7 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
     sl.carve(pop->pair_stats, 2);
     sl.carve(pop->cand_mask, nc);
     sl.carve(pop->cand_status, nc);
-    if (sl.alloc() != hipSuccess)
+    if (sl.alloc(&ctx->spare_slab) != hipSuccess)
       return fail(dnab_set_error(ctx, DNAB_ERR_ALLOC, "population: device allocation failed"));
   }
   {
@@ -1740,7 +1740,19 @@ int dnab_population_create(dnab_ctx* ctx, dnab_antigen* const* antigens, size_t 
 void dnab_population_destroy(dnab_population* pop) {
   if (!pop) return;
   dnab_ctx* ctx = pop->ctx;
-  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    // its slab, idle now, kept for the context's next population (the
+    // larger of it and the one kept so far, up to kSpareSlabMax)
+    DevBuf<unsigned char>& m = pop->slab.mem;
+    if (m.p && m.owned && m.n <= kSpareSlabMax && m.n > ctx->spare_slab.n) {
+      ctx->spare_slab.release();
+      ctx->spare_slab.p = m.p;
+      ctx->spare_slab.n = m.n;
+      m.p = nullptr;
+      m.n = 0;
+    }
+  }
   delete pop;
   ctx_release(ctx);
 }
