@@ -241,12 +241,16 @@ __device__ inline void hot_chunk(const float4 (*st)[2], uint32_t n_s, float xb, 
 // an FP32 term (the loop's steps: a clash force moves its residue by the
 // clamped step whatever its rounding). A chunk with no pair in range in the
 // whole wave leaves `acc` as it is (its sums are exact zeros), without the
-// integer conversions.
-template <bool FORCES, bool COUNT, bool CLASH>
+// integer conversions. Without ENERGY (forces only: the loop's steps whose
+// energies nothing reads) the energy, sum |e| and hydrophobic count are not
+// formed and acc.e, acc.a stay as they are; the forces are formed by the
+// same operations either way.
+template <bool FORCES, bool COUNT, bool CLASH, bool ENERGY = true>
 __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb, float yb,
                                    float zb, float qb, float rmhb, float sqeb, float sqe12b,
                                    float epsb, bool hydb, float rc2_lane, bool valid, FixAcc& acc,
                                    uint32_t& n_in, uint32_t& n_tested) {
+  static_assert(ENERGY || !CLASH, "clash pairs are evaluated only where energies are reported");
   float fx = 0.f, fy = 0.f, fz = 0.f, es = 0.f, ec = 0.f, eabs = 0.f;
   uint32_t nh = 0;
   uint64_t any = 0;     // wave-uniform
@@ -264,17 +268,19 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
     float ep, f;
     uint32_t h;
     pair_value<FORCES>(B0, B1, qb, rmhb, sqeb, sqe12b, hydb, r2, in, hot, ep, f, h);
-    nh += h;
     if (FORCES) {
       fx = __builtin_fmaf(f, dx, fx);
       fy = __builtin_fmaf(f, dy, fy);
       fz = __builtin_fmaf(f, dz, fz);
     }
-    float sum, err;
-    two_sum(es, ep, sum, err);
-    es = sum;
-    ec += err;
-    eabs += fabsf(ep);
+    if (ENERGY) {
+      nh += h;
+      float sum, err;
+      two_sum(es, ep, sum, err);
+      es = sum;
+      ec += err;
+      eabs += fabsf(ep);
+    }
   };
   uint32_t q = 0;
   for (; q + 2 <= n_s; q += 2) {
@@ -306,16 +312,18 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
     if (b1) add(A0, st[q][1], dx, dy, dz, r2, in);
   }
   if (!any) return;
-  // hydrophobic: -0.5 per pair, exactly -2^19 in fixed point
-  const long long eh = -static_cast<long long>(nh) << 19;
-  const double e_chunk = static_cast<double>(es) + static_cast<double>(ec);
-  acc.e = fix_add(acc.e, fix_round(e_chunk, acc.st, kStE) + eh, acc.st, kStE);
-  // sum |e| feeds the error estimate only; an atom's energy rounding can err
-  // only in a chunk where it has a pair (a chunk is fixed by the antigen, so
-  // this count does not depend on the tiles)
-  uint32_t ignore = 0;
-  acc.a = fix_add(acc.a, fix_round(static_cast<double>(mine ? eabs + kRoundAbs : eabs), ignore, 0u),
-                  ignore, 0u);
+  if (ENERGY) {
+    // hydrophobic: -0.5 per pair, exactly -2^19 in fixed point
+    const long long eh = -static_cast<long long>(nh) << 19;
+    const double e_chunk = static_cast<double>(es) + static_cast<double>(ec);
+    acc.e = fix_add(acc.e, fix_round(e_chunk, acc.st, kStE) + eh, acc.st, kStE);
+    // sum |e| feeds the error estimate only; an atom's energy rounding can err
+    // only in a chunk where it has a pair (a chunk is fixed by the antigen, so
+    // this count does not depend on the tiles)
+    uint32_t ignore = 0;
+    acc.a = fix_add(acc.a, fix_round(static_cast<double>(mine ? eabs + kRoundAbs : eabs), ignore, 0u),
+                    ignore, 0u);
+  }
   if (FORCES) {
     acc.x = fix_add(acc.x, fix_round(static_cast<double>(fx), acc.st, kStX), acc.st, kStX);
     acc.y = fix_add(acc.y, fix_round(static_cast<double>(fy), acc.st, kStY), acc.st, kStY);

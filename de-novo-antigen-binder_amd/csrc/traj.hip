@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <type_traits>
 #include <vector>
 
 #include "device_common.h"
@@ -546,6 +547,8 @@ k_trajectory(TrajArgs g) {
     const uint32_t it = g.s0 + s;                // the loop's step index
     // this step's noise amplitude, loaded early (used in phase 4)
     const float sigma = (g.sp.with_noise && s < g.n_steps) ? g.sigma_table[it] : 0.0f;
+    // this step's energies are read: a scoring step, or the evaluated final state
+    const bool need_e = s == g.n_steps || (g.score_every && it % g.score_every == 0);
     // 1. tile boxes, the chunks of the grid cells within the cutoff of
     //    each box (listed in LDS), and cleared accumulators
     for (uint32_t t = wave; t < T; t += WAVES) {
@@ -680,141 +683,155 @@ k_trajectory(TrajArgs g) {
         return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(i)));
       };
 
-      FixAcc acc;
-      int32_t cur = -1;
-      float xb = 0.f, yb = 0.f, zb = 0.f, qb = 0.f, rmhb = 0.f, sqeb = 0.f, sqe12b = 0.f, epsb = 0.f;
-      bool hydb = false, valid = false;
-      float rc2_lane = -1.0f;
-      auto flush = [&]() {
-        const uint32_t k = static_cast<uint32_t>(cur) * kWave + lane;
-        // one flush per (wave, tile) and step: at most WAVES sums of <= 2^59
-        if (acc.e | acc.a | acc.x | acc.y | acc.z) {
-          atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[0 * AC + k]), static_cast<unsigned long long>(acc.e));
-          atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[1 * AC + k]), static_cast<unsigned long long>(acc.a));
-          atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[2 * AC + k]), static_cast<unsigned long long>(acc.x));
-          atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[3 * AC + k]), static_cast<unsigned long long>(acc.y));
-          atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[4 * AC + k]), static_cast<unsigned long long>(acc.z));
-        }
-        if (acc.st) atomicOr(&s_st[k], acc.st);
-        acc = FixAcc();
-      };
-
-      for (uint32_t i = grab(); i < total; i = grab()) {
-        // the item's tile: the last tile whose first item is <= i
-        const uint64_t m = ballot64(lane < T && first <= i);
-        const int32_t t = 63 - static_cast<int32_t>(__builtin_clzll(m));
-        const uint32_t b = i - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(first), t));
-        const TrajTile& tb = s_tile[t];
-        const TrajAg& V = s_ag[tb.cand];
-        if (t != cur) {
-          if (cur >= 0) flush();
-          cur = t;
-          const uint32_t k = static_cast<uint32_t>(t) * kWave + lane;
-          const float4 P = s_pos[k];
-          const float4 Q = s_par[k];
-          valid = s_res[k] != kTrajPad && traj_can_pair(V, P);
-          xb = P.x; yb = P.y; zb = P.z; qb = P.w;
-          rmhb = Q.x;
-          sqeb = sqrtf(Q.y);
-          sqe12b = 12.0f * sqeb;
-          epsb = Q.y;
-          hydb = Q.z != 0.0f;
-          rc2_lane = valid ? V.rc2 : -1.0f;  // an atom without pairs takes none
-        }
-        const AgAtom* __restrict__ atoms = V.atoms;
-        const float rc2 = V.rc2;
-        const float lox = tb.lo[0], loy = tb.lo[1], loz = tb.lo[2];
-        const float hix = tb.hi[0], hiy = tb.hi[1], hiz = tb.hi[2];
-        // up to kLoadAhead chunks [cbv, cev): loads in flight together, then
-        // the atoms near the tile box are staged and paired in chunk order
-        auto run_chunks = [&](const uint32_t (&cbv)[kLoadAhead], const uint32_t (&cev)[kLoadAhead]) {
-          float4 L0[kLoadAhead], L1[kLoadAhead];
-          #pragma unroll
-          for (int u = 0; u < kLoadAhead; ++u) {
-            L0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            L1[u] = L0[u];
-            if (cbv[u] + lane < cev[u]) {
-              L0[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].x);
-              L1[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].rmh);
+      // a step whose energies nothing reads (no scoring, not the evaluated
+      // final state) forms forces only: the same forces, fewer operations
+      auto run_items = [&](auto energy) {
+        FixAcc acc;
+        int32_t cur = -1;
+        float xb = 0.f, yb = 0.f, zb = 0.f, qb = 0.f, rmhb = 0.f, sqeb = 0.f, sqe12b = 0.f, epsb = 0.f;
+        bool hydb = false, valid = false;
+        float rc2_lane = -1.0f;
+        auto flush = [&]() {
+          const uint32_t k = static_cast<uint32_t>(cur) * kWave + lane;
+          // one flush per (wave, tile) and step: at most WAVES sums of <= 2^59
+          if (acc.e | acc.a | acc.x | acc.y | acc.z) {
+            if (decltype(energy)::value) {  // (forces only: acc.e and acc.a stay zero)
+              atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[0 * AC + k]), static_cast<unsigned long long>(acc.e));
+              atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[1 * AC + k]), static_cast<unsigned long long>(acc.a));
             }
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[2 * AC + k]), static_cast<unsigned long long>(acc.x));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[3 * AC + k]), static_cast<unsigned long long>(acc.y));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s_acc[4 * AC + k]), static_cast<unsigned long long>(acc.z));
           }
-          #pragma unroll
-          for (int u = 0; u < kLoadAhead; ++u) {
-            if (cev[u] <= cbv[u]) continue;
-            const float bx = fmaxf(fmaxf(lox - L0[u].x, L0[u].x - hix), 0.0f);
-            const float by = fmaxf(fmaxf(loy - L0[u].y, L0[u].y - hiy), 0.0f);
-            const float bz = fmaxf(fmaxf(loz - L0[u].z, L0[u].z - hiz), 0.0f);
-            const bool near = cbv[u] + lane < cev[u] && ((bx * bx + by * by) + bz * bz) <= rc2;
-            const uint64_t mask = ballot64(near);
-            const uint32_t n_s = static_cast<uint32_t>(__builtin_popcountll(mask));
-            if (n_s == 0) continue;
-            if (lane == 0 && g.work) atomicAdd(&s_work[tb.cand], n_s);
-            stage_chunk(s_stage[wave], mask, lane, L0[u], L1[u]);
-            chunk_pairs<true, COUNT, CLASH>(s_stage[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
-                                     hydb, rc2_lane, valid, acc, n_in, n_tested);
-          }
+          if (acc.st) atomicOr(&s_st[k], acc.st);
+          acc = FixAcc();
         };
-        uint32_t cbv[kLoadAhead], cev[kLoadAhead];
-        if (tb.nchunk != kTileOverflow) {
-          // listed chunks b * g.chunks ...
-          const uint32_t base = b * g.chunks;
-          const uint32_t* desc = s_desc + t * kTileChunks;
-          #pragma unroll
-          for (int u = 0; u < kLoadAhead; ++u) {
-            cbv[u] = 0;
-            cev[u] = 0;
-            if (static_cast<uint32_t>(u) < g.chunks && base + u < tb.nchunk) {
-              const uint32_t d = desc[base + u];
-              cbv[u] = d >> 6;
-              cev[u] = cbv[u] + (d & 63u) + 1u;
-            }
+
+        for (uint32_t i = grab(); i < total; i = grab()) {
+          // the item's tile: the last tile whose first item is <= i
+          const uint64_t m = ballot64(lane < T && first <= i);
+          const int32_t t = 63 - static_cast<int32_t>(__builtin_clzll(m));
+          const uint32_t b = i - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(first), t));
+          const TrajTile& tb = s_tile[t];
+          const TrajAg& V = s_ag[tb.cand];
+          if (t != cur) {
+            if (cur >= 0) flush();
+            cur = t;
+            const uint32_t k = static_cast<uint32_t>(t) * kWave + lane;
+            const float4 P = s_pos[k];
+            const float4 Q = s_par[k];
+            valid = s_res[k] != kTrajPad && traj_can_pair(V, P);
+            xb = P.x; yb = P.y; zb = P.z; qb = P.w;
+            rmhb = Q.x;
+            sqeb = sqrtf(Q.y);
+            sqe12b = 12.0f * sqeb;
+            epsb = Q.y;
+            hydb = Q.z != 0.0f;
+            rc2_lane = valid ? V.rc2 : -1.0f;  // an atom without pairs takes none
           }
-          run_chunks(cbv, cev);
-        } else {
-          // cells b * g.cells ...: their chunks, kLoadAhead at a time
-          const uint32_t* __restrict__ cstart = V.cell_start;
-          const int32_t nx = tb.nx, nxy = tb.nxy;
-          const int32_t cbeg = static_cast<int32_t>(b) * g.cells;
-          const int32_t cend = min(cbeg + g.cells, tb.ncell);
-          const int32_t cl = cbeg + static_cast<int32_t>(lane);
-          uint32_t cb = 0, ce = 0;
-          if (cl < cend) {
-            const int32_t iz = cl / nxy, rem = cl - iz * nxy;
-            const int32_t iy = rem / nx, ix = rem - iy * nx;
-            const uint32_t code = morton3(static_cast<uint32_t>(tb.k0[0] + ix - V.kmin[0]),
-                                          static_cast<uint32_t>(tb.k0[1] + iy - V.kmin[1]),
-                                          static_cast<uint32_t>(tb.k0[2] + iz - V.kmin[2]));
-            cb = cstart[code];
-            ce = cstart[code + 1];
-          }
-          const uint32_t nch = (ce - cb + kWave - 1) / kWave;
-          uint32_t cincl = nch;
-          for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = __shfl_up(cincl, o, 64);
-            if (lane >= static_cast<uint32_t>(o)) cincl += u;
-          }
-          const uint32_t off = cincl - nch;
-          const uint32_t nchunk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cincl), 63));
-          for (uint32_t kc = 0; kc < nchunk; kc += kLoadAhead) {
+          const AgAtom* __restrict__ atoms = V.atoms;
+          const float rc2 = V.rc2;
+          const float lox = tb.lo[0], loy = tb.lo[1], loz = tb.lo[2];
+          const float hix = tb.hi[0], hiy = tb.hi[1], hiz = tb.hi[2];
+          // up to kLoadAhead chunks [cbv, cev): loads in flight together, then
+          // the atoms near the tile box are staged and paired in chunk order
+          auto run_chunks = [&](const uint32_t (&cbv)[kLoadAhead], const uint32_t (&cev)[kLoadAhead]) {
+            float4 L0[kLoadAhead], L1[kLoadAhead];
             #pragma unroll
             for (int u = 0; u < kLoadAhead; ++u) {
-              const uint32_t kk = kc + u;
+              L0[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+              L1[u] = L0[u];
+              if (cbv[u] + lane < cev[u]) {
+                L0[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].x);
+                L1[u] = *reinterpret_cast<const float4*>(&atoms[cbv[u] + lane].rmh);
+              }
+            }
+            #pragma unroll
+            for (int u = 0; u < kLoadAhead; ++u) {
+              if (cev[u] <= cbv[u]) continue;
+              const float bx = fmaxf(fmaxf(lox - L0[u].x, L0[u].x - hix), 0.0f);
+              const float by = fmaxf(fmaxf(loy - L0[u].y, L0[u].y - hiy), 0.0f);
+              const float bz = fmaxf(fmaxf(loz - L0[u].z, L0[u].z - hiz), 0.0f);
+              const bool near = cbv[u] + lane < cev[u] && ((bx * bx + by * by) + bz * bz) <= rc2;
+              const uint64_t mask = ballot64(near);
+              const uint32_t n_s = static_cast<uint32_t>(__builtin_popcountll(mask));
+              if (n_s == 0) continue;
+              if (lane == 0 && g.work) atomicAdd(&s_work[tb.cand], n_s);
+              stage_chunk(s_stage[wave], mask, lane, L0[u], L1[u]);
+              chunk_pairs<true, COUNT, CLASH, decltype(energy)::value>(s_stage[wave], n_s, xb, yb, zb, qb, rmhb, sqeb, sqe12b, epsb,
+                                       hydb, rc2_lane, valid, acc, n_in, n_tested);
+            }
+          };
+          uint32_t cbv[kLoadAhead], cev[kLoadAhead];
+          if (tb.nchunk != kTileOverflow) {
+            // listed chunks b * g.chunks ...
+            const uint32_t base = b * g.chunks;
+            const uint32_t* desc = s_desc + t * kTileChunks;
+            #pragma unroll
+            for (int u = 0; u < kLoadAhead; ++u) {
               cbv[u] = 0;
               cev[u] = 0;
-              if (kk < nchunk) {
-                // the cell holding chunk kk, and the chunk's atom range
-                const uint64_t mc = ballot64(nch > 0 && off <= kk);
-                const int ln = 63 - static_cast<int>(__builtin_clzll(mc));
-                cbv[u] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cb), ln)) +
-                         kWave * (kk - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off), ln)));
-                cev[u] = min(cbv[u] + kWave, static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ce), ln)));
+              if (static_cast<uint32_t>(u) < g.chunks && base + u < tb.nchunk) {
+                const uint32_t d = desc[base + u];
+                cbv[u] = d >> 6;
+                cev[u] = cbv[u] + (d & 63u) + 1u;
               }
             }
             run_chunks(cbv, cev);
+          } else {
+            // cells b * g.cells ...: their chunks, kLoadAhead at a time
+            const uint32_t* __restrict__ cstart = V.cell_start;
+            const int32_t nx = tb.nx, nxy = tb.nxy;
+            const int32_t cbeg = static_cast<int32_t>(b) * g.cells;
+            const int32_t cend = min(cbeg + g.cells, tb.ncell);
+            const int32_t cl = cbeg + static_cast<int32_t>(lane);
+            uint32_t cb = 0, ce = 0;
+            if (cl < cend) {
+              const int32_t iz = cl / nxy, rem = cl - iz * nxy;
+              const int32_t iy = rem / nx, ix = rem - iy * nx;
+              const uint32_t code = morton3(static_cast<uint32_t>(tb.k0[0] + ix - V.kmin[0]),
+                                            static_cast<uint32_t>(tb.k0[1] + iy - V.kmin[1]),
+                                            static_cast<uint32_t>(tb.k0[2] + iz - V.kmin[2]));
+              cb = cstart[code];
+              ce = cstart[code + 1];
+            }
+            const uint32_t nch = (ce - cb + kWave - 1) / kWave;
+            uint32_t cincl = nch;
+            for (int o = 1; o < 64; o <<= 1) {
+              const uint32_t u = __shfl_up(cincl, o, 64);
+              if (lane >= static_cast<uint32_t>(o)) cincl += u;
+            }
+            const uint32_t off = cincl - nch;
+            const uint32_t nchunk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cincl), 63));
+            for (uint32_t kc = 0; kc < nchunk; kc += kLoadAhead) {
+              #pragma unroll
+              for (int u = 0; u < kLoadAhead; ++u) {
+                const uint32_t kk = kc + u;
+                cbv[u] = 0;
+                cev[u] = 0;
+                if (kk < nchunk) {
+                  // the cell holding chunk kk, and the chunk's atom range
+                  const uint64_t mc = ballot64(nch > 0 && off <= kk);
+                  const int ln = 63 - static_cast<int>(__builtin_clzll(mc));
+                  cbv[u] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cb), ln)) +
+                           kWave * (kk - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off), ln)));
+                  cev[u] = min(cbv[u] + kWave, static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ce), ln)));
+                }
+              }
+              run_chunks(cbv, cev);
+            }
           }
         }
+        if (cur >= 0) flush();
+      };
+      if constexpr (CLASH) {  // (a launch evaluating the final state: no steps)
+        run_items(std::true_type{});
+      } else {
+        if (need_e)
+          run_items(std::true_type{});
+        else
+          run_items(std::false_type{});
       }
-      if (cur >= 0) flush();
     }
     __syncthreads();
 
