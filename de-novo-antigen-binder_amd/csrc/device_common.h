@@ -119,7 +119,10 @@ __device__ inline uint32_t wave_sum_u32(uint32_t v) {
   return (r0 + r1) + (r2 + r3);
 }
 
-__device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
+// (the compare mask itself, with no round trip through a 0/1 register)
+__device__ inline uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+// This lane's bit of a wave mask (the inverse of ballot64).
+__device__ inline bool lane_in(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
 
 // TwoSum (Knuth, TAOCP vol. 2, 4.2.2): s + e = a + b exactly, no branch and
 // no magnitude precondition. Requires IEEE evaluation without contraction or

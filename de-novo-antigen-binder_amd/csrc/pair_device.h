@@ -98,6 +98,13 @@ __device__ inline void pair_pred(float4 B0, float xb, float yb, float zb, float 
   in = !(r2 > rc2_lane || r2 < kMinRSq);
 }
 
+// The lanes whose pair passes the predicate of pair_pred, as a wave mask:
+// one mask per comparison (each the comparison's own result), then their
+// intersection.
+__device__ inline uint64_t pred_mask(float r2, float rc2_lane) {
+  return ballot64(!(r2 > rc2_lane)) & ballot64(!(r2 < kMinRSq));
+}
+
 // One pair term (the reference's formulas, src/energy.rs:261-314) after the
 // predicate: the f32 energy ep and, with FORCES, the scalar f with
 // F = f (dx, dy, dz); both exactly zero when the pair is out of range or
@@ -289,27 +296,29 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
     bool ina, inb;
     pair_pred(A0, xb, yb, zb, rc2_lane, dxa, dya, dza, r2a, ina);
     pair_pred(B0, xb, yb, zb, rc2_lane, dxb, dyb, dzb, r2b, inb);
+    const uint64_t ba = pred_mask(r2a, rc2_lane), bb = pred_mask(r2b, rc2_lane);
     if (COUNT) {
       n_tested += valid ? 2u : 0u;
-      n_in += (ina ? 1u : 0u) + (inb ? 1u : 0u);
+      n_in += (lane_in(ba) ? 1u : 0u) + (lane_in(bb) ? 1u : 0u);
     }
-    const uint64_t ba = ballot64(ina), bb = ballot64(inb);
     any |= ba | bb;
-    if (ba) add(A0, st[q][1], dxa, dya, dza, r2a, ina);
-    if (bb) add(B0, st[q + 1][1], dxb, dyb, dzb, r2b, inb);
+    // (the lane's flag taken back from the mask: no per-atom copy of the
+    // flag is kept in a vector register while the mask decides the branch)
+    if (ba) add(A0, st[q][1], dxa, dya, dza, r2a, lane_in(ba));
+    if (bb) add(B0, st[q + 1][1], dxb, dyb, dzb, r2b, lane_in(bb));
   }
   if (q < n_s) {
     const float4 A0 = st[q][0];
     float dx, dy, dz, r2;
     bool in;
     pair_pred(A0, xb, yb, zb, rc2_lane, dx, dy, dz, r2, in);
+    const uint64_t b1 = pred_mask(r2, rc2_lane);
     if (COUNT) {
       n_tested += valid ? 1u : 0u;
-      n_in += in ? 1u : 0u;
+      n_in += lane_in(b1) ? 1u : 0u;
     }
-    const uint64_t b1 = ballot64(in);
     any |= b1;
-    if (b1) add(A0, st[q][1], dx, dy, dz, r2, in);
+    if (b1) add(A0, st[q][1], dx, dy, dz, r2, lane_in(b1));
   }
   if (!any) return;
   if (ENERGY) {
