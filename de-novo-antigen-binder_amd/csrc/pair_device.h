@@ -98,12 +98,13 @@ __device__ inline void pair_pred(float4 B0, float xb, float yb, float zb, float 
   in = !(r2 > rc2_lane || r2 < kMinRSq);
 }
 
-// The lanes whose pair passes the predicate of pair_pred, as a wave mask:
-// one mask per comparison (each the comparison's own result), then their
-// intersection.
-__device__ inline uint64_t pred_mask(float r2, float rc2_lane) {
-  return ballot64(!(r2 > rc2_lane)) & ballot64(!(r2 < kMinRSq));
-}
+// The lanes whose pair passes the cutoff half of the predicate of pair_pred
+// (a NaN distance passes), as a wave mask: the comparison's own result. The
+// pair loop skips a staged atom outside it for every lane; the other half,
+// r^2 >= kMinRSq, is applied where the pair is evaluated (pair_in).
+__device__ inline uint64_t pred_mask(float r2, float rc2_lane) { return ballot64(!(r2 > rc2_lane)); }
+// The lane's pair passes the whole predicate, given its bit of pred_mask.
+__device__ inline bool pair_in(uint64_t mask, float r2) { return lane_in(mask) && !(r2 < kMinRSq); }
 
 // One pair term (the reference's formulas, src/energy.rs:261-314) after the
 // predicate: the f32 energy ep and, with FORCES, the scalar f with
@@ -240,15 +241,15 @@ __device__ inline void hot_chunk(const float4 (*st)[2], uint32_t n_s, float xb, 
 
 // Pairs of the lane's atom with the `n_s` antigen atoms staged in `st` (in
 // chunk order); folds the chunk's sums into `acc`. The predicate is formed
-// for two atoms at a time; an atom in range of no lane of the wave is
-// skipped (each of its terms would add an exact zero), the others are
+// for two atoms at a time; an atom within the cutoff of no lane of the wave
+// is skipped (each of its terms would add an exact zero), the others are
 // added in chunk order, which keeps the sums those of a plain loop. With
 // CLASH a clash pair adds exact zeros to these FP32 sums and the chunk's
 // clash pairs are then summed in FP64 by hot_chunk; without it every pair is
 // an FP32 term (the loop's steps: a clash force moves its residue by the
-// clamped step whatever its rounding). A chunk with no pair in range in the
-// whole wave leaves `acc` as it is (its sums are exact zeros), without the
-// integer conversions. Without ENERGY (forces only: the loop's steps whose
+// clamped step whatever its rounding). A chunk with no atom within the
+// cutoff of any lane leaves `acc` as it is (its sums are exact zeros),
+// without the integer conversions. Without ENERGY (forces only: the loop's steps whose
 // energies nothing reads) the energy, sum |e| and hydrophobic count are not
 // formed and acc.e, acc.a stay as they are; the forces are formed by the
 // same operations either way.
@@ -299,13 +300,13 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
     const uint64_t ba = pred_mask(r2a, rc2_lane), bb = pred_mask(r2b, rc2_lane);
     if (COUNT) {
       n_tested += valid ? 2u : 0u;
-      n_in += (lane_in(ba) ? 1u : 0u) + (lane_in(bb) ? 1u : 0u);
+      n_in += (pair_in(ba, r2a) ? 1u : 0u) + (pair_in(bb, r2b) ? 1u : 0u);
     }
     any |= ba | bb;
     // (the lane's flag taken back from the mask: no per-atom copy of the
     // flag is kept in a vector register while the mask decides the branch)
-    if (ba) add(A0, st[q][1], dxa, dya, dza, r2a, lane_in(ba));
-    if (bb) add(B0, st[q + 1][1], dxb, dyb, dzb, r2b, lane_in(bb));
+    if (ba) add(A0, st[q][1], dxa, dya, dza, r2a, pair_in(ba, r2a));
+    if (bb) add(B0, st[q + 1][1], dxb, dyb, dzb, r2b, pair_in(bb, r2b));
   }
   if (q < n_s) {
     const float4 A0 = st[q][0];
@@ -315,10 +316,10 @@ __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb
     const uint64_t b1 = pred_mask(r2, rc2_lane);
     if (COUNT) {
       n_tested += valid ? 1u : 0u;
-      n_in += lane_in(b1) ? 1u : 0u;
+      n_in += pair_in(b1, r2) ? 1u : 0u;
     }
     any |= b1;
-    if (b1) add(A0, st[q][1], dx, dy, dz, r2, lane_in(b1));
+    if (b1) add(A0, st[q][1], dx, dy, dz, r2, pair_in(b1, r2));
   }
   if (!any) return;
   if (ENERGY) {
