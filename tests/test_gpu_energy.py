@@ -50,6 +50,31 @@ def test_two_atom_closed_forms(ctx):
     assert ctx.interaction_energy_atoms(ag, ab_in) == 0.0
 
 
+def test_close_pairs_in_the_pair_loop(ctx):
+    # The pair loop skips a staged antigen atom by the cutoff comparison alone
+    # and drops pairs with r^2 < 0.25 where it evaluates them. Binder atoms:
+    # 0.49 A from antigen atom 0 (no pair) beside atoms that pair with it, one
+    # 0.5 A away (r^2 = 0.25: a pair), and one whose only antigen atom within
+    # the cutoff is 0.49 A away (antigen atom 1 is evaluated for it alone).
+    ag = cloud([[0, 0, 0], [30.0, 0, 0]], [0.8, -0.3], 1.908, 0.1094, [1, 0])
+    ab = cloud([[0.49, 0, 0], [0.5, 0, 0], [3.0, 0, 0], [30.49, 0, 0]], -0.5, 1.824, 0.17, 1)
+    fx, fy, fz, e = ctx.atom_forces(ag, ab)
+    (lj, co, hy), f64, npairs = O.energy_f64(ag, ab)
+    assert npairs == 2
+    assert fx[0] == 0.0 and fy[0] == 0.0 and fz[0] == 0.0
+    assert fx[3] == 0.0 and fy[3] == 0.0 and fz[3] == 0.0
+    assert fx[1] != 0.0 and fx[2] != 0.0
+    assert tol(e, lj + co + hy)
+    for got, ref in zip((fx, fy, fz), f64):
+        assert tol(got, ref).all(), (got, ref)
+    # the close atoms add exact zeros: the same bits without them
+    ab2 = cloud([[0.5, 0, 0], [3.0, 0, 0]], -0.5, 1.824, 0.17, 1)
+    gx, gy, gz, e2 = ctx.atom_forces(ag, ab2)
+    assert np.array_equal(np.array([fx[1], fx[2]]).view(np.uint32), gx.view(np.uint32))
+    assert np.array_equal(np.array([fy[1], fy[2]]).view(np.uint32), gy.view(np.uint32))
+    assert np.float64(e).view(np.uint64) == np.float64(e2).view(np.uint64)
+
+
 def test_atom_breakdown_sums_to_total(ctx):
     # src/energy.rs:496-507
     ag = protein_from_ca_trace([0.0], [0.0], [0.0], aa_from_sequence("D"))

@@ -104,6 +104,25 @@ def _assert_same(a, b):
         assert np.array_equal(np.asarray(u).view(np.uint32), np.asarray(v).view(np.uint32))
 
 
+def test_force_only_steps_keep_the_trajectory(ctx):
+    # the fused loop's steps between scorings form forces only; scoring every
+    # step (every step forms energies) gives the same trajectory bit for bit,
+    # and the scorings both runs make give the same rows
+    w = W.config2(n_cand=96)
+    kw = dict(n_steps=20, top_k=8, with_noise=1, seed=5)
+    a = _run_state(ctx, w, score_every=10, **kw)
+    b = _run_state(ctx, w, score_every=1, **kw)
+    for u, v in zip(a["coords"], b["coords"]):
+        assert np.array_equal(u.view(np.uint32), v.view(np.uint32))
+    for u, v in zip(a["forces"], b["forces"]):
+        assert np.array_equal(np.asarray(u).view(np.uint32), np.asarray(v).view(np.uint32))
+    assert np.array_equal(a["energy"].view(np.uint64), b["energy"].view(np.uint64))
+    assert a["hist_e"].shape[0] == 3 and b["hist_e"].shape[0] == 21
+    for ra, rb in ((0, 0), (1, 10), (2, 20)):
+        assert np.array_equal(a["hist_idx"][ra], b["hist_idx"][rb])
+        assert np.array_equal(a["hist_e"][ra].view(np.uint64), b["hist_e"][rb].view(np.uint64))
+
+
 def test_graph_replay_equals_eager(ctx):
     # per-step kernels eager, per-step kernels replayed from a graph, and the
     # fused trajectory kernel: the same trajectory bit for bit
