@@ -38,6 +38,7 @@
 #include "dnab_internal.h"
 #include "pair_device.h"
 #include "step_device.h"
+#include "host_parallel.h"
 
 namespace dnab {
 
@@ -1304,8 +1305,12 @@ int trajectory_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_para
     e.score_every = 0;
     e.work = pop->traj_work.p;
     DNAB_HIP_CHECK(ctx, launch_plan(e, pl, false, false, ctx->stream));
+    auto t0 = std::chrono::steady_clock::now();
     DNAB_RETURN_IF(download(ctx, work.data(), pop->traj_work.p, nc * sizeof(uint32_t)));
+    timer_host(ctx, "trajectory_host_work", ms_since(t0));
+    t0 = std::chrono::steady_clock::now();
     DNAB_RETURN_IF(regroup());
+    timer_host(ctx, "trajectory_host_group", ms_since(t0));
   } else if (mode == 2) {
     DNAB_RETURN_IF(regroup_space());
   }
