@@ -285,8 +285,8 @@ void topk_launch(dnab_ctx* ctx, TopkPlan& pl, const double* energy, double* fina
 //
 // A block of 1024 threads holds up to 1024 * KPT entries (thread t the
 // consecutive slots t * KPT ...). It finds the m-th smallest key digit by
-// digit (eight 4-bit digits from the top: a histogram of the digit among the
-// keys that match the digits found so far, per wave in LDS, then the digit
+// digit (four 8-bit digits from the top: a block histogram of the digit in
+// LDS among the keys that match the digits found so far, then the digit
 // where the running count reaches the rank still needed), then writes, in
 // slot order, every entry with a key below it and the first ties with it
 // until m are taken. Keys are the scores' order-preserving integer images
@@ -365,35 +365,48 @@ k_select_smallest(const float* __restrict__ scores, const uint32_t* __restrict__
   }
   const uint32_t mm = min(m, sel_block_sum(nv, s_w));
   if (mm > 0) {
-    __shared__ uint32_t s_hist[kSelBlock / kWave][16];
+    // the key of rank mm, 8 bits at a time from the top: a block histogram
+    // of the digit over the keys that match the digits found so far
+    __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_sel[2];
     const uint32_t lane = t & 63u, wave = t >> 6;
     uint32_t T = 0, need = mm;  // the digits found so far; the rank still needed among their keys
-    for (int sh = 28; sh >= 0; sh -= 4) {
-      const uint32_t hi = sh == 28 ? 0u : (0xFFFFFFFFu << (sh + 4));
-      if (lane < 16) s_hist[wave][lane] = 0;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int sh = 24; sh >= 0; sh -= 8) {
+      const uint32_t hi = sh == 24 ? 0u : (0xFFFFFFFFu << (sh + 8));
+      if (t < 256) s_hist[t] = 0;
+      __syncthreads();
 #pragma unroll
       for (int q = 0; q < KPT; ++q)
-        if (idx[q] != kSelNone && (key[q] & hi) == (T & hi)) atomicAdd(&s_hist[wave][(key[q] >> sh) & 15u], 1u);
+        if (idx[q] != kSelNone && (key[q] & hi) == (T & hi)) atomicAdd(&s_hist[(key[q] >> sh) & 255u], 1u);
       __syncthreads();
       if (wave == 0) {
-        uint32_t tot = 0;
-        if (lane < 16)
-          for (uint32_t w = 0; w < kSelBlock / kWave; ++w) tot += s_hist[w][lane];
-        uint32_t incl = tot;
-        for (int o = 1; o < 16; o <<= 1) {
+        // lane: digits 4 lane .. 4 lane + 3, running counts within the lane,
+        // then across the lanes
+        const uint4 h = *reinterpret_cast<const uint4*>(&s_hist[4 * lane]);
+        const uint32_t c0 = h.x, c1 = c0 + h.y, c2 = c1 + h.z, c3 = c2 + h.w;
+        uint32_t incl = c3;
+        for (int o = 1; o < 64; o <<= 1) {
           const uint32_t u = __shfl_up(incl, o, 64);
           if (lane >= static_cast<uint32_t>(o)) incl += u;
         }
+        const uint32_t before = incl - c3;  // keys in digits below 4 lane
         // the first digit whose running count reaches the rank needed
-        const uint64_t reach = ballot64(lane < 16 && incl >= need);
-        const int d = __builtin_ctzll(reach);  // exists: the keys matching T number >= need
-        if (lane == static_cast<uint32_t>(d)) {
-          s_sel[0] = T | (static_cast<uint32_t>(d) << sh);
-          s_sel[1] = need - (incl - tot);
+        const uint64_t reach = ballot64(incl >= need);
+        const int L = __builtin_ctzll(reach);  // exists: the keys matching T number >= need
+        if (lane == static_cast<uint32_t>(L)) {
+          uint32_t d = 3, below = before + c2;
+          if (before + c0 >= need) {
+            d = 0;
+            below = before;
+          } else if (before + c1 >= need) {
+            d = 1;
+            below = before + c0;
+          } else if (before + c2 >= need) {
+            d = 2;
+            below = before + c1;
+          }
+          s_sel[0] = T | ((4u * lane + d) << sh);
+          s_sel[1] = need - below;
         }
       }
       __syncthreads();
