@@ -18,7 +18,9 @@
 //     chunk) sum is rounded once to 2^-20 and chunk sums add exactly, the
 //     per-atom results are bit-identical to the population kernels' on the
 //     loop's steps (whose pair terms are all FP32: a clash force moves its
-//     residue by the clamped step whatever its rounding);
+//     residue by the clamped step whatever its rounding). A step whose
+//     energies nothing reads (not a scoring step, not the evaluated final
+//     state) forms forces only, by the same operations;
 //  3. per-atom energies and forces from the integer sums;
 //  4. the residue update (residue_delta, step_device.h), one thread per residue;
 //  5. the rigid residue moves, one thread per atom.
