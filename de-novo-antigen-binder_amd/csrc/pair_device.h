@@ -249,10 +249,10 @@ __device__ inline void hot_chunk(const float4 (*st)[2], uint32_t n_s, float xb, 
 // an FP32 term (the loop's steps: a clash force moves its residue by the
 // clamped step whatever its rounding). A chunk with no atom within the
 // cutoff of any lane leaves `acc` as it is (its sums are exact zeros),
-// without the integer conversions. Without ENERGY (forces only: the loop's steps whose
-// energies nothing reads) the energy, sum |e| and hydrophobic count are not
-// formed and acc.e, acc.a stay as they are; the forces are formed by the
-// same operations either way.
+// without the integer conversions. Without ENERGY (forces only: the loop's
+// steps whose energies nothing reads) the energy, sum |e| and hydrophobic
+// count are not formed and acc.e, acc.a stay as they are; the forces are
+// formed by the same operations either way.
 template <bool FORCES, bool COUNT, bool CLASH, bool ENERGY = true>
 __device__ inline void chunk_pairs(const float4 (*st)[2], uint32_t n_s, float xb, float yb,
                                    float zb, float qb, float rmhb, float sqeb, float sqe12b,
