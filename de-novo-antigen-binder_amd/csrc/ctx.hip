@@ -98,6 +98,26 @@ int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return DNAB_OK;
 }
 
+// Two device-to-host copies (either may be empty: a null destination) with
+// one wait for the stream.
+int download2(dnab_ctx* ctx, void* dst1, const void* src1, size_t b1, void* dst2, const void* src2,
+              size_t b2) {
+  if (!dst1) b1 = 0;
+  if (!dst2) b2 = 0;
+  const size_t off2 = (b1 + 15) & ~size_t(15);
+  if (b1 + b2 == 0) return DNAB_OK;
+  if (off2 + b2 > kStageMax || stream_capturing(ctx) || !ensure_pinned(ctx->dl_stage, ctx->dl_cap, off2 + b2)) {
+    DNAB_RETURN_IF(download(ctx, dst1, src1, b1));
+    return download(ctx, dst2, src2, b2);
+  }
+  if (b1) DNAB_HIP_CHECK(ctx, hipMemcpyAsync(ctx->dl_stage, src1, b1, hipMemcpyDeviceToHost, ctx->stream));
+  if (b2) DNAB_HIP_CHECK(ctx, hipMemcpyAsync(ctx->dl_stage + off2, src2, b2, hipMemcpyDeviceToHost, ctx->stream));
+  DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (b1) std::memcpy(dst1, ctx->dl_stage, b1);
+  if (b2) std::memcpy(dst2, ctx->dl_stage + off2, b2);
+  return DNAB_OK;
+}
+
 static hipEvent_t take_event(dnab_ctx* ctx) {
   if (!ctx->event_pool.empty()) {
     hipEvent_t e = ctx->event_pool.back();

@@ -419,7 +419,8 @@ struct dnab_population {
   dnab::DevBuf<uint8_t> cand_status;  // per candidate: DNAB_STATUS_* of the last evaluation
   // prescreen
   dnab::DevBuf<uint16_t> feat;   // n_cand x kPrescreenFeatures bf16
-  dnab::DevBuf<dnab::MapView> map_views;  // per antigen, refreshed per feature pass
+  dnab::DevBuf<dnab::MapView> map_views;  // per antigen, uploaded when they change
+  std::vector<dnab::MapView> map_views_host;  // what map_views holds
   dnab::DevBuf<uint32_t> sel_scratch;     // select_smallest's stage lists
   dnab::DevBuf<float> scores;    // n_cand
   dnab::DevBuf<double> scores64;
@@ -471,6 +472,8 @@ void timer_end(dnab_ctx* ctx, const char* name, hipEvent_t start, hipEvent_t end
 void timer_host(dnab_ctx* ctx, const char* name, double ms);  // host-clock phase, when timing
 int upload(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
 int download(dnab_ctx* ctx, void* dst, const void* src, size_t bytes);
+int download2(dnab_ctx* ctx, void* dst1, const void* src1, size_t b1, void* dst2, const void* src2,
+              size_t b2);
 // step.hip: multi-pass top-k (ascending, NaN last, ties by index)
 int topk_plan(dnab_ctx* ctx, TopkPlan& pl, uint32_t n, uint32_t k, uint32_t rows = 1);
 // first_idx (nullable): the entries' indices (else their positions), for

@@ -616,8 +616,13 @@ int population_map_features(dnab_ctx* ctx, dnab_population* pop, uint16_t* feat,
     for (int d = 0; d < 3; ++d) v.nc[d] = M.nc[d];
     v.pad = 0;
   }
-  DNAB_ALLOC_CHECK(ctx, pop->map_views.ensure(views.size()));
-  DNAB_RETURN_IF(upload(ctx, pop->map_views.p, views.data(), views.size() * sizeof(MapView)));
+  const bool same = pop->map_views_host.size() == views.size() &&
+                    std::memcmp(pop->map_views_host.data(), views.data(), views.size() * sizeof(MapView)) == 0;
+  if (!same) {
+    DNAB_ALLOC_CHECK(ctx, pop->map_views.ensure(views.size()));
+    DNAB_RETURN_IF(upload(ctx, pop->map_views.p, views.data(), views.size() * sizeof(MapView)));
+    pop->map_views_host = views;
+  }
   if (pop->n_cand == 0) return DNAB_OK;
   FeatArgs g{};
   g.views = pop->map_views.p;

@@ -820,8 +820,7 @@ int dnab_prescreen(dnab_ctx* ctx, dnab_mlp* n, dnab_population* pop, uint32_t n_
   // the other candidates' outputs were not refreshed
   pop->evaluated = false;
   DNAB_HIP_CHECK(ctx, hipGetLastError());
-  if (idx) DNAB_RETURN_IF(download(ctx, idx, pop->topk_idx.p, k * 4));
-  if (energy) DNAB_RETURN_IF(download(ctx, energy, pop->topk_e.p, k * 8));
+  DNAB_RETURN_IF(download2(ctx, idx, pop->topk_idx.p, k * 4, energy, pop->topk_e.p, k * 8));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DNAB_OK;
 }
@@ -844,8 +843,7 @@ int dnab_exact_topk(dnab_ctx* ctx, dnab_population* pop, uint32_t top_k, uint32_
   DNAB_RETURN_IF(population_evaluate(ctx, pop, 0));
   topk_launch(ctx, pe, pop->energy.p, pop->topk_e.p, pop->topk_idx.p);
   DNAB_HIP_CHECK(ctx, hipGetLastError());
-  if (idx) DNAB_RETURN_IF(download(ctx, idx, pop->topk_idx.p, k * 4));
-  if (energy) DNAB_RETURN_IF(download(ctx, energy, pop->topk_e.p, k * 8));
+  DNAB_RETURN_IF(download2(ctx, idx, pop->topk_idx.p, k * 4, energy, pop->topk_e.p, k * 8));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return DNAB_OK;
 }

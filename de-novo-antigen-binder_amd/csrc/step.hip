@@ -671,8 +671,8 @@ int dnab_langevin_run(dnab_ctx* ctx, dnab_population* pop, const dnab_langevin_p
   }
   DNAB_HIP_CHECK(ctx, hipGetLastError());
   const size_t last = static_cast<size_t>(n_mid) * k;
-  if (topk_idx) DNAB_RETURN_IF(download(ctx, topk_idx, pop->hist_idx.p + last, k * sizeof(uint32_t)));
-  if (topk_energy) DNAB_RETURN_IF(download(ctx, topk_energy, pop->hist_e.p + last, k * sizeof(double)));
+  DNAB_RETURN_IF(download2(ctx, topk_idx, pop->hist_idx.p + last, k * sizeof(uint32_t), topk_energy,
+                           pop->hist_e.p + last, k * sizeof(double)));
   DNAB_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   pop->hist_rows = rows;
   return DNAB_OK;
@@ -800,8 +800,8 @@ int dnab_population_read_scores(dnab_ctx* ctx, const dnab_population* pop, uint3
   if (k) *k = pop->hist_k;
   const size_t n = static_cast<size_t>(pop->hist_rows) * pop->hist_k;
   if (n == 0) return DNAB_OK;
-  if (topk_idx) DNAB_RETURN_IF(download(ctx, topk_idx, pop->hist_idx.p, n * sizeof(uint32_t)));
-  if (topk_energy) DNAB_RETURN_IF(download(ctx, topk_energy, pop->hist_e.p, n * sizeof(double)));
+  DNAB_RETURN_IF(download2(ctx, topk_idx, pop->hist_idx.p, n * sizeof(uint32_t), topk_energy, pop->hist_e.p,
+                           n * sizeof(double)));
   return DNAB_OK;
 }
 
